@@ -1,0 +1,255 @@
+"""Synthetic stair-step problems.
+
+A :class:`Problem` holds one frame's inputs in the solver interface's Fortran
+layouts (see ``flowfpmpp_amd.stairstep``). Builders:
+
+* :func:`reference_benchmark` -- the reference's public benchmark workload: a
+  sphere on a sinusoidally stretched grid with analytic flow fields
+  (tests/benchmark_stairstep_3d.f90:51-77).
+* :func:`stretched_frame` -- the tanh-stretched workloads on [-4,4]x[-3,3]x[-2,2]
+  with seeded blocked bodies and seeded 16-mode divergence-free Fourier
+  velocity/acceleration fields: ``frame_128x96x64`` (one sphere r=0.5 at the
+  origin plus one box touching the +x face) and ``frame_512x384x256`` (two
+  spheres and two boxes drawn from seed 2, one box touching the boundary).
+
+Per force direction i the influence problem is body_grad = e_i, rhs = 0 and
+zero outer-face data before the compatibility correction.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .grids import local_axis, stretched_axis, tanh_axis
+
+PROC_NULL = -1
+
+
+@dataclasses.dataclass
+class Problem:
+    nx: int
+    ny: int
+    nz: int
+    solid: np.ndarray          # int32 (nx+2, ny+2, nz), halo zero
+    xc: np.ndarray             # (nx+2) centres with mirrored halo centres
+    yc: np.ndarray             # (ny+2)
+    zc: np.ndarray             # (nz)
+    dx: np.ndarray
+    dy: np.ndarray
+    dz: np.ndarray
+    fields: dict               # u v w ox oy oz ax ay az, each (nx, ny, nz) F-order
+    bounds: np.ndarray         # (6)
+    label: str = ""
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def zeros(self):
+        return np.zeros((self.nx, self.ny, self.nz), order="F")
+
+    def phi0(self):
+        return np.zeros((self.nx + 2, self.ny + 2, self.nz), order="F")
+
+    def faces(self, value=(0.0, 0.0, 0.0)):
+        """Outer-face normal derivatives xlo, xhi, ylo, yhi, zlo, zhi set to the
+        given per-axis constants."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        gx, gy, gz = value
+        return (np.full((ny, nz), gx, order="F"), np.full((ny, nz), gx, order="F"),
+                np.full((nx, nz), gy, order="F"), np.full((nx, nz), gy, order="F"),
+                np.full((nx, ny), gz, order="F"), np.full((nx, ny), gz, order="F"))
+
+
+def _grid_from_axes(xcg, dxg, ycg, dyg, zc, dz):
+    xc, dx = local_axis(xcg, dxg)
+    yc, dy = local_axis(ycg, dyg)
+    return xc, yc, np.array(zc, dtype=np.float64), dx, dy, np.array(dz, dtype=np.float64)
+
+
+def reference_benchmark(gnx: int = 256, gny: int = 256, nz: int = 64) -> Problem:
+    """tests/benchmark_stairstep_3d.f90:51-77: sphere of radius 0.4*lz at the
+    box centre; amplitudes 0.2, -0.2, 0.1 for the x, y, z stretching."""
+    lz = nz / gnx
+    _, gxc, gdx = stretched_axis(gnx, 0.0, 1.0, 0.2)
+    _, gyc, gdy = stretched_axis(gny, 0.0, gny / gnx, -0.2)
+    _, zc, dz = stretched_axis(nz, 0.0, lz, 0.1)
+    xc, yc, zc, dx, dy, dz = _grid_from_axes(gxc, gdx, gyc, gdy, zc, dz)
+    center = (0.5, 0.5 * gny / gnx, 0.5 * lz)
+    radius = 0.4 * lz
+    X = xc[1:gnx + 1][:, None, None]
+    Y = yc[1:gny + 1][None, :, None]
+    Z = zc[None, None, :]
+    r2 = (X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2
+    solid = np.zeros((gnx + 2, gny + 2, nz), dtype=np.int32, order="F")
+    solid[1:gnx + 1, 1:gny + 1, :] = (r2 < radius ** 2)
+    shape = (gnx, gny, nz)
+    f = {}
+    f["u"] = np.asfortranarray(np.broadcast_to(1 + 0.1 * np.sin(6 * X) * np.cos(4 * Y), shape))
+    f["v"] = np.asfortranarray(np.broadcast_to(0.2 * np.cos(3 * X + 2 * Z), shape))
+    f["w"] = np.asfortranarray(np.broadcast_to(0.1 * np.sin(5 * Y), shape))
+    f["ox"] = np.asfortranarray(np.broadcast_to(np.cos(4 * Z + Y), shape))
+    f["oy"] = np.asfortranarray(np.broadcast_to(np.sin(3 * X), shape))
+    f["oz"] = np.asfortranarray(np.broadcast_to(np.cos(2 * X) * np.sin(5 * Y), shape))
+    f["ax"] = np.asfortranarray(0.1 * f["u"])
+    f["ay"] = np.asfortranarray(-0.2 * f["v"])
+    f["az"] = np.asfortranarray(0.3 * f["w"])
+    bounds = np.array([-1.0, 2.0, -1.0, 2.0, -1.0, 2.0])
+    return Problem(gnx, gny, nz, solid, xc, yc, zc, dx, dy, dz, f, bounds,
+                   label=f"reference-benchmark-{gnx}x{gny}x{nz}")
+
+
+# --- bodies ------------------------------------------------------------------
+
+@dataclasses.dataclass
+class Sphere:
+    center: tuple
+    radius: float
+
+    def inside(self, X, Y, Z):
+        c = self.center
+        return (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 < self.radius ** 2
+
+
+@dataclasses.dataclass
+class Box:
+    lo: tuple
+    hi: tuple
+
+    def inside(self, X, Y, Z):
+        return ((X > self.lo[0]) & (X < self.hi[0]) & (Y > self.lo[1]) & (Y < self.hi[1])
+                & (Z > self.lo[2]) & (Z < self.hi[2]))
+
+
+def solid_from_bodies(xc, yc, zc, bodies) -> np.ndarray:
+    """Blocked-cell flags: a cell is solid when its centre is strictly inside a
+    body. Fluid regions not connected (6-neighbour) to the largest fluid
+    region are filled, so the fluid has no isolated cells and no enclosed
+    pockets (each would add a null-space direction the single compatibility
+    correction cannot make consistent)."""
+    nx, ny, nz = len(xc) - 2, len(yc) - 2, len(zc)
+    X = xc[1:nx + 1][:, None, None]
+    Y = yc[1:ny + 1][None, :, None]
+    Z = np.asarray(zc)[None, None, :]
+    inner = np.zeros((nx, ny, nz), dtype=bool)
+    for b in bodies:
+        inner |= np.broadcast_to(b.inside(X, Y, Z), inner.shape)
+    from scipy import ndimage
+    labels, count = ndimage.label(~inner)
+    if count > 1:
+        sizes = np.bincount(labels.ravel())
+        sizes[0] = 0
+        keep = int(np.argmax(sizes))
+        inner |= (labels != keep)
+    solid = np.zeros((nx + 2, ny + 2, nz), dtype=np.int32, order="F")
+    solid[1:nx + 1, 1:ny + 1, :] = inner
+    return solid
+
+
+# --- divergence-free Fourier fields -------------------------------------------
+
+def _draw_modes(rng, nmodes, box, kmin=1, kmax=4):
+    modes = []
+    while len(modes) < nmodes:
+        n = rng.integers(-kmax, kmax + 1, size=3)
+        nn = float(np.sqrt((n * n).sum()))
+        if nn < kmin or nn > kmax:
+            continue
+        k = 2.0 * math.pi * n / np.asarray(box, dtype=np.float64)
+        khat = k / np.linalg.norm(k)
+        a = rng.standard_normal(3)
+        amp = (a - a.dot(khat) * khat) / nn ** 2
+        theta = rng.uniform(0.0, 2.0 * math.pi)
+        modes.append((k, amp, theta))
+    return modes
+
+
+def _eval_modes(modes, X, Y, Z, shape, with_curl):
+    """Sum of A cos(k.x + theta) (and its curl -(k x A) sin(k.x + theta)) on
+    the tensor grid, one z-block at a time via separable complex phases."""
+    nx, ny, nz = shape
+    out = [np.zeros(shape, order="F") for _ in range(6 if with_curl else 3)]
+    zb = max(1, min(nz, 4_000_000 // max(1, nx * ny)))
+    for k, amp, theta in modes:
+        ex = np.exp(1j * k[0] * X)[:, None, None]
+        ey = np.exp(1j * k[1] * Y)[None, :, None]
+        curl = np.cross(k, amp)
+        for z0 in range(0, nz, zb):
+            z1 = min(nz, z0 + zb)
+            ez = np.exp(1j * (k[2] * Z[z0:z1] + theta))[None, None, :]
+            ph = ex * ey * ez
+            c, s = ph.real, ph.imag
+            for m in range(3):
+                out[m][:, :, z0:z1] += amp[m] * c
+                if with_curl:
+                    out[3 + m][:, :, z0:z1] -= curl[m] * s
+    return out
+
+
+def fourier_fields(xc, yc, zc, seed: int, box=(8.0, 6.0, 4.0), nmodes: int = 16):
+    """Velocity u (16 random divergence-free modes, wavenumbers 1-4, amplitudes
+    N(0,1)/|n|^2), its analytic vorticity, and an acceleration field from an
+    independent draw; FP64 on the host."""
+    nx, ny, nz = len(xc) - 2, len(yc) - 2, len(zc)
+    X, Y, Z = xc[1:nx + 1], yc[1:ny + 1], np.asarray(zc)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vel = _draw_modes(rng, nmodes, box)
+    acc = _draw_modes(rng, nmodes, box)
+    u, v, w, ox, oy, oz = _eval_modes(vel, X, Y, Z, (nx, ny, nz), True)
+    ax, ay, az = _eval_modes(acc, X, Y, Z, (nx, ny, nz), False)
+    return dict(u=u, v=v, w=w, ox=ox, oy=oy, oz=oz, ax=ax, ay=ay, az=az)
+
+
+def stretched_frame(nx: int, ny: int, nz: int, bodies, seed: int, label: str = "",
+                    ratio: float = 4.0, with_fields: bool = True) -> Problem:
+    _, gxc, gdx = tanh_axis(nx, -4.0, 4.0, ratio)
+    _, gyc, gdy = tanh_axis(ny, -3.0, 3.0, ratio)
+    _, zc, dz = tanh_axis(nz, -2.0, 2.0, ratio)
+    xc, yc, zc, dx, dy, dz = _grid_from_axes(gxc, gdx, gyc, gdy, zc, dz)
+    solid = solid_from_bodies(xc, yc, zc, bodies)
+    fields = fourier_fields(xc, yc, zc, seed) if with_fields else {}
+    bounds = np.array([-4.0, 4.0, -3.0, 3.0, -2.0, 2.0]) * (1 + 1e-12)
+    return Problem(nx, ny, nz, solid, xc, yc, zc, dx, dy, dz, fields, bounds, label=label)
+
+
+def seeded_bodies(seed: int, nspheres: int = 2, nboxes: int = 2):
+    """Spheres r ~ U[0.3,0.8] and axis-aligned boxes with sides ~ U[0.3,1.0];
+    the last box is moved to touch the +x outer face."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    bodies = []
+    for _ in range(nspheres):
+        r = rng.uniform(0.3, 0.8)
+        c = (rng.uniform(-3.0, 3.0), rng.uniform(-2.0, 2.0), rng.uniform(-1.2, 1.2))
+        bodies.append(Sphere(c, r))
+    for b in range(nboxes):
+        s = rng.uniform(0.3, 1.0, size=3)
+        c = np.array([rng.uniform(-3.0, 3.0), rng.uniform(-2.0, 2.0), rng.uniform(-1.2, 1.2)])
+        if b == nboxes - 1:
+            c[0] = 4.0 - 0.5 * s[0]
+            lo, hi = c - 0.5 * s, c + 0.5 * s
+            hi[0] = 4.5  # reaches past the outer face: blocks the boundary cells
+        else:
+            lo, hi = c - 0.5 * s, c + 0.5 * s
+        bodies.append(Box(tuple(lo), tuple(hi)))
+    return bodies
+
+
+def frame_128x96x64(with_fields: bool = True) -> Problem:
+    bodies = [Sphere((0.0, 0.0, 0.0), 0.5), Box((3.25, -1.0, -0.75), (4.5, 1.0, 0.75))]
+    return stretched_frame(128, 96, 64, bodies, seed=1, label="128x96x64",
+                           with_fields=with_fields)
+
+
+def frame_512x384x256(with_fields: bool = True) -> Problem:
+    return stretched_frame(512, 384, 256, seeded_bodies(2), seed=2, label="512x384x256",
+                           with_fields=with_fields)
+
+
+def influence_inputs(p: Problem, direction: int):
+    """rhs, body_grad and zero outer-face data for the influence field of
+    force direction 0, 1 or 2 (before the compatibility correction)."""
+    bg = np.zeros(3)
+    bg[direction] = 1.0
+    return p.zeros(), bg, p.faces()
