@@ -1,0 +1,207 @@
+// Shared device/host helpers of the stair-step solver library (gfx950).
+//
+// Internal field layout ("dense"): cell (i,j,k), 0-based, lives at
+// i + nx*(j + ny*k) -- the Fortran layout of an (nx,ny,nz) array, so rhs and
+// the flow fields are used in place. phi and solid keep the caller's
+// one-layer x-y halo layout (0:nx+1, 0:ny+1, nz) only at the interface; the
+// iterations run on dense fields and substitute the zero physical halo.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace ffpm {
+
+// ---------------------------------------------------------------- errors --
+void set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define FFPM_HIP(call)                                                   \
+  do {                                                                   \
+    hipError_t _e = (call);                                              \
+    if (_e != hipSuccess) return ::ffpm::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define FFPM_LAUNCHED(what)                                              \
+  do {                                                                   \
+    hipError_t _e = hipGetLastError();                                   \
+    if (_e != hipSuccess) return ::ffpm::hip_fail(_e, what, __FILE__, __LINE__); \
+  } while (0)
+
+enum Status : int {
+  OK = 0,
+  ERR_ARGUMENT = 1,
+  ERR_ISOLATED = 2,
+  ERR_NO_OUTER_FACE = 3,
+  ERR_METHOD = 4,
+  ERR_SHAPE = 5,
+  ERR_DECOMPOSITION = 6,
+  ERR_DEVICE = 7,
+};
+
+// ------------------------------------------------------------ launch shape --
+constexpr int kTileX = 64;   // one wavefront along x
+constexpr int kTileY = 4;    // four wavefronts per workgroup
+constexpr int kBlock = kTileX * kTileY;
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kReduceSlots = 4;
+
+// z-streaming tiles: (tiles_x, tiles_y, z chunks) flattened x-fastest, then
+// dealt to XCDs in contiguous ranges so that y-neighbour tiles (whose halo
+// rows a tile reads) share an L2.
+struct Tiling {
+  int tiles_x, tiles_y, chunks_z, zchunk;
+  int total;      // tiles
+  int per_xcd;    // ceil(total / 8)
+  int blocks;     // launched workgroups = 8 * per_xcd
+};
+
+Tiling make_tiling(int nx, int ny, int nz, int zchunk);
+
+struct TileCoord {
+  int i, j, k0, k1;
+  bool valid;  // workgroup owns a tile
+};
+
+__device__ inline TileCoord tile_coord(const Tiling &t, int nx, int ny, int nz) {
+  TileCoord c;
+  const int L = blockIdx.x;
+  const int tile = (L % kXcds) * t.per_xcd + L / kXcds;
+  c.valid = tile < t.total;
+  const int tx = tile % t.tiles_x;
+  const int rest = tile / t.tiles_x;
+  const int ty = rest % t.tiles_y;
+  const int tz = rest / t.tiles_y;
+  c.i = tx * kTileX + threadIdx.x;
+  c.j = ty * kTileY + threadIdx.y;
+  c.k0 = tz * t.zchunk;
+  c.k1 = min(nz, c.k0 + t.zchunk);
+  return c;
+}
+
+// ---------------------------------------------------------------- grids ----
+// Geometry of one frame on the device: 1-D stencil coefficients and widths.
+struct Geom {
+  int nx, ny, nz;
+  const double *cw, *ce, *cs, *cn, *cb, *ct;  // src/fpm_stairstep_mpi_3d.f90:499-510,138-142
+  const double *dx, *dy, *dz;
+  const double *xc, *yc, *zc;  // xc(0:nx+1), yc(0:ny+1), zc(nz)
+};
+
+// ------------------------------------------------------------ reductions ---
+// Block-level sum of up to kReduceSlots doubles (wave shuffles, then LDS), and
+// a grid-level finish in the last workgroup to arrive. The final sum runs over
+// the per-workgroup partials in a fixed order, so a reduction is reproducible
+// run to run for a fixed launch shape.
+template <int N>
+__device__ inline void block_sum(double (&v)[N], double *lds /* [N * 16] */) {
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < N; ++s) {
+    double x = v[s];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[s] = x;
+  }
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nwaves = (blockDim.x * blockDim.y + 63) >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) lds[s * 16 + wave] = v[s];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      double x = 0.0;
+      for (int w = 0; w < nwaves; ++w) x += lds[s * 16 + w];
+      v[s] = x;
+    }
+  }
+}
+
+__device__ inline void block_max(double &v, double *lds) {
+  __syncthreads();
+  double x = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int nwaves = (blockDim.x * blockDim.y + 63) >> 6;
+  if (lane == 0) lds[wave] = x;
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int w = 0; w < nwaves; ++w) m = fmax(m, lds[w]);
+    v = m;
+  }
+}
+
+// Last-workgroup reduction finish. Each workgroup reduces its values, thread 0
+// stores them as this workgroup's partials, and arrive_last() returns true in
+// every thread of the one workgroup that arrives last; that workgroup then sums
+// the partials in a fixed order (grid_finish_*), so results are reproducible
+// run to run for a fixed launch shape.
+__device__ inline void store_partials(double *partials, int nslots, const double *vals,
+                                      int nblocks) {
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    for (int s = 0; s < nslots; ++s) partials[(size_t)s * nblocks + blockIdx.x] = vals[s];
+}
+
+__device__ inline bool arrive_last(unsigned int *counter, int nblocks) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    __threadfence();
+    const unsigned int prev = atomicAdd(counter, 1u);
+    s_last = (prev == (unsigned int)(nblocks - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (s_last) __threadfence();
+  return s_last != 0;
+}
+
+// Called by every thread of the last workgroup; thread 0 gets the sums.
+template <int N>
+__device__ inline void grid_finish_sum(const double *partials, int nblocks, double (&v)[N],
+                                       double *lds) {
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int nthreads = blockDim.x * blockDim.y;
+  const volatile double *p = partials;
+#pragma unroll
+  for (int s = 0; s < N; ++s) {
+    double x = 0.0;
+    for (int b = tid; b < nblocks; b += nthreads) x += p[(size_t)s * nblocks + b];
+    v[s] = x;
+  }
+  block_sum<N>(v, lds);
+}
+
+__device__ inline double grid_finish_max(const double *partials, int slot, int nblocks,
+                                         double *lds) {
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int nthreads = blockDim.x * blockDim.y;
+  const volatile double *p = partials + (size_t)slot * nblocks;
+  double x = 0.0;
+  for (int b = tid; b < nblocks; b += nthreads) x = fmax(x, p[b]);
+  block_max(x, lds);
+  return x;
+}
+
+__device__ inline void reset_counter(unsigned int *counter) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) *counter = 0u;
+}
+
+// Elementwise launch shape: grid-stride over n cells.
+inline int elementwise_blocks(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+}  // namespace ffpm

@@ -1,0 +1,79 @@
+// Kernel declarations and the device-resident solver state.
+#pragma once
+
+#include "ffpm_common.hpp"
+
+namespace ffpm {
+
+constexpr int kBreakdown = 1;  // pq <= 0 or rz <= 0 (or NaN): restart from the true residual
+constexpr int kNegative = 2;   // rz_new < 0 or NaN: leave the iteration
+
+// Scalars the iteration keeps on the device, so that a chunk of iterations
+// runs without a host round trip; the host reads them at convergence checks.
+struct Scalars {
+  double rz, pq, rz_new, alpha, beta;
+  double res;         // max-norm residual of the last residual kernel
+  double red1, red2;  // sum of the true residual and the fluid-cell count
+  double sums[4];     // generic reduction results
+  int status;         // 0, kBreakdown or kNegative
+  int iters;          // iterations completed since the last chunk start
+  unsigned int counter;  // last-workgroup arrival counter (0 between kernels)
+  int pad;
+};
+
+struct Bounds {
+  double b[6];
+};
+
+__global__ void k_axis_coefficients(int nx, int ny, int nz, const double *xc, const double *yc,
+                                    const double *zc, const double *dx, const double *dy,
+                                    const double *dz, double *cw, double *ce, double *cs,
+                                    double *cn, double *cb, double *ct);
+__global__ void k_build_operator(Geom g, const int32_t *solid, const double *rhs,
+                                 const double *xlo, const double *xhi, const double *ylo,
+                                 const double *yhi, const double *zlo, const double *zhi,
+                                 double bg0, double bg1, double bg2, int for_pcg, double *src,
+                                 double *diag, double *rdiag, int *isolated);
+__global__ void k_pack_phi(int nx, int ny, int nz, const int32_t *solid, const double *phi_ref,
+                           double *phi);
+__global__ void k_unpack_phi(int nx, int ny, int nz, const double *phi, double *phi_ref);
+__global__ void k_jacobi(Tiling t, Geom g, const double *a, double *b, const double *src,
+                         const double *rdiag);
+__global__ void k_residual_max(Tiling t, Geom g, const double *phi, const double *src,
+                               const double *diag, double *partials, Scalars *sc);
+__global__ void k_true_residual_1(Tiling t, Geom g, const double *phi, const double *src,
+                                  const double *diag, double *r, double *partials, Scalars *sc);
+__global__ void k_true_residual_2(size_t n, double *r, const double *rdiag, const double *diag,
+                                  double *p, int reset, double *partials, Scalars *sc);
+__global__ void k_spmv_dot(Tiling t, Geom g, const double *p, double *q, const double *diag,
+                           double *partials, Scalars *sc);
+__global__ void k_update_dot(size_t n, double *phi, const double *p, double *r, const double *q,
+                             const double *rdiag, double *partials, Scalars *sc);
+__global__ void k_direction(size_t n, double *p, const double *r, const double *rdiag,
+                            const Scalars *sc);
+__global__ void k_chunk_begin(Scalars *sc);
+__global__ void k_clear_status(Scalars *sc);
+__global__ void k_gauge_sums(Geom g, const int32_t *solid, const double *phi, double *partials,
+                             Scalars *sc);
+__global__ void k_gauge_apply(int nx, int ny, int nz, const int32_t *solid, double *phi,
+                              double *phi_ref, const Scalars *sc);
+__global__ void k_compat_sums(Geom g, const int32_t *solid, const double *rhs, const double *xlo,
+                              const double *xhi, const double *ylo, const double *yhi,
+                              const double *zlo, const double *zhi, double bg0, double bg1,
+                              double bg2, double *partials, Scalars *sc);
+__global__ void k_shift_faces(int nx, int ny, int nz, double *xlo, double *xhi, double *ylo,
+                              double *yhi, double *zlo, double *zhi, double c);
+__global__ void k_face_residual(Geom g, const double *phi, const double *rhs,
+                                const int32_t *solid, const double *xlo, const double *xhi,
+                                const double *ylo, const double *yhi, const double *zlo,
+                                const double *zhi, double bg0, double bg1, double bg2,
+                                double *partials, Scalars *sc);
+__global__ void k_terms(Geom g, const double *phi, const int32_t *solid, const double *u,
+                        const double *v, const double *w, const double *ox, const double *oy,
+                        const double *oz, const double *ax, const double *ay, const double *az,
+                        const double *xlo, const double *xhi, const double *ylo,
+                        const double *yhi, const double *zlo, const double *zhi, double bg0,
+                        double bg1, double bg2, Bounds bd, double *gx, double *gy, double *gz,
+                        double *vortex, double *advection, double *partials, Scalars *sc);
+
+}  // namespace ffpm

@@ -1,0 +1,186 @@
+// Error state, stream, workspaces and kernel timing of the library.
+#include "ffpm_runtime.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/ffpm.h"
+
+namespace ffpm {
+
+namespace {
+thread_local std::string g_error;
+hipStream_t g_stream = nullptr;
+}  // namespace
+
+void set_error(int code, const std::string &msg) {
+  (void)code;
+  g_error = msg;
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e,
+                hipGetErrorString(e), what, file, line);
+  set_error(ERR_DEVICE, buf);
+  return ERR_DEVICE;
+}
+
+hipStream_t stream() { return g_stream; }
+
+Tiling make_tiling(int nx, int ny, int nz, int zchunk) {
+  Tiling t;
+  t.tiles_x = (nx + kTileX - 1) / kTileX;
+  t.tiles_y = (ny + kTileY - 1) / kTileY;
+  if (zchunk <= 0) {
+    // Enough workgroups to fill 256 CUs several times over, and z chunks of
+    // at least 4 planes so the z halo re-read stays small.
+    const long want = 4096;
+    const long txy = (long)t.tiles_x * t.tiles_y;
+    long chunks = (want + txy - 1) / txy;
+    if (chunks < 1) chunks = 1;
+    zchunk = (int)((nz + chunks - 1) / chunks);
+    if (zchunk < 4) zchunk = 4;
+  }
+  if (zchunk > nz) zchunk = nz;
+  t.zchunk = zchunk;
+  t.chunks_z = (nz + zchunk - 1) / zchunk;
+  t.total = t.tiles_x * t.tiles_y * t.chunks_z;
+  t.per_xcd = (t.total + kXcds - 1) / kXcds;
+  t.blocks = t.per_xcd * kXcds;
+  return t;
+}
+
+void *Workspace::get(int slot, size_t bytes, int *status) {
+  if (bytes == 0) bytes = 256;
+  Buf &b = bufs_[slot];
+  if (b.bytes >= bytes) return b.ptr;
+  if (b.ptr) {
+    hipError_t e = hipFree(b.ptr);
+    if (e != hipSuccess) {
+      *status = hip_fail(e, "hipFree", __FILE__, __LINE__);
+      b.ptr = nullptr;
+      b.bytes = 0;
+      return nullptr;
+    }
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    *status = hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    return nullptr;
+  }
+  b.ptr = p;
+  b.bytes = bytes;
+  return p;
+}
+
+void Workspace::release_all() {
+  for (auto &kv : bufs_)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  bufs_.clear();
+}
+
+Workspace::~Workspace() {
+  // Process teardown: the runtime may already be gone, so leave the memory
+  // to the driver.
+}
+
+Workspace &workspace() {
+  static Workspace w;
+  return w;
+}
+
+KernelTimer &timer() {
+  static KernelTimer t;
+  return t;
+}
+
+hipEvent_t KernelTimer::take() {
+  if (!free_events.empty()) {
+    hipEvent_t e = free_events.back();
+    free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void KernelTimer::start(const char *name, hipEvent_t *a) {
+  (void)name;
+  *a = take();
+  if (*a) (void)hipEventRecord(*a, stream());
+}
+
+void KernelTimer::stop(const char *name, hipEvent_t a) {
+  if (!a) return;
+  hipEvent_t b = take();
+  if (!b) return;
+  (void)hipEventRecord(b, stream());
+  pending.push_back({name, a, b});
+  if (pending.size() > 4096) (void)collect();
+}
+
+int KernelTimer::collect() {
+  if (pending.empty()) return 0;
+  FFPM_HIP(hipEventSynchronize(pending.back().b));
+  for (auto &r : pending) {
+    float ms = 0.f;
+    FFPM_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    auto &t = totals[r.name];
+    t.first += 1;
+    t.second += ms;
+    free_events.push_back(r.a);
+    free_events.push_back(r.b);
+  }
+  pending.clear();
+  return 0;
+}
+
+}  // namespace ffpm
+
+extern "C" {
+
+const char *ffpm_last_error(void) { return ffpm::g_error.c_str(); }
+
+int ffpm_set_stream(void *s) {
+  ffpm::g_stream = (hipStream_t)s;
+  return FFPM_OK;
+}
+
+int ffpm_release(void) {
+  FFPM_HIP(hipDeviceSynchronize());
+  ffpm::workspace().release_all();
+  return FFPM_OK;
+}
+
+int ffpm_timing_enable(int on) {
+  ffpm::timer().enabled = on != 0;
+  return FFPM_OK;
+}
+
+int ffpm_timing_reset(void) {
+  int st = ffpm::timer().collect();
+  ffpm::timer().totals.clear();
+  return st;
+}
+
+int ffpm_timing_query(const char *kernel, long *count, double *total_ms) {
+  int st = ffpm::timer().collect();
+  if (st) return st;
+  auto it = ffpm::timer().totals.find(kernel ? kernel : "");
+  if (it == ffpm::timer().totals.end()) {
+    *count = 0;
+    *total_ms = 0.0;
+  } else {
+    *count = it->second.first;
+    *total_ms = it->second.second;
+  }
+  return FFPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Host-side runtime of the library: error state, the stream the library
+// launches on, grow-only device workspaces, and optional HIP-event timing of
+// the hot kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ffpm_kernels.hpp"
+
+namespace ffpm {
+
+hipStream_t stream();
+
+// Grow-only device buffers addressed by slot id; reallocated only when a
+// larger size is requested. One workspace per process (the library is not
+// re-entrant on one device; separate threads must serialise their calls).
+class Workspace {
+ public:
+  void *get(int slot, size_t bytes, int *status);
+  void release_all();
+  ~Workspace();
+
+ private:
+  struct Buf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<int, Buf> bufs_;
+};
+
+Workspace &workspace();
+
+enum Slot : int {
+  S_COEF = 1, S_PHI, S_NEXT, S_SRC, S_DIAG, S_RDIAG, S_R, S_Q, S_PARTIALS, S_SCALARS,
+  S_ISOLATED,
+  // host-pointer entry points stage their arguments here
+  S_U_PHI = 40, S_U_RHS, S_U_SOLID, S_U_XLO, S_U_XHI, S_U_YLO, S_U_YHI, S_U_ZLO, S_U_ZHI,
+  S_U_XC, S_U_YC, S_U_ZC, S_U_DX, S_U_DY, S_U_DZ, S_U_F0, S_U_F1, S_U_F2, S_U_F3, S_U_F4,
+  S_U_F5, S_U_F6, S_U_F7, S_U_F8, S_U_O0, S_U_O1, S_U_O2, S_U_O3, S_U_O4,
+  // no-body Neumann solver
+  S_L_NEXT = 80, S_L_COEF,
+};
+
+// ---- kernel timing with HIP events on the library stream --------------------
+struct KernelTimer {
+  bool enabled = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<long, double>> totals;  // count, ms
+  hipEvent_t take();
+  void start(const char *name, hipEvent_t *a);
+  void stop(const char *name, hipEvent_t a);
+  int collect();
+};
+
+KernelTimer &timer();
+
+// Brackets one launch with events when timing is on.
+struct TimedLaunch {
+  const char *name;
+  hipEvent_t a{};
+  explicit TimedLaunch(const char *n) : name(n) {
+    if (timer().enabled) timer().start(name, &a);
+  }
+  ~TimedLaunch() {
+    if (timer().enabled) timer().stop(name, a);
+  }
+};
+
+}  // namespace ffpm

@@ -1,0 +1,667 @@
+// Host orchestration of the stair-step solve (src/fpm_stairstep_mpi_3d.f90)
+// and its C ABI. The iteration logic (convergence checks, breakdown restart,
+// stall exit) follows the reference; the per-iteration scalars stay on the
+// device, and the host reads them once per convergence check.
+#include <cfloat>
+#include <cstring>
+#include <string>
+
+#include "../../include/ffpm.h"
+#include "ffpm_runtime.hpp"
+
+namespace ffpm {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+bool any_neighbour(int west, int east, int south, int north) {
+  return west >= 0 || east >= 0 || south >= 0 || north >= 0;
+}
+
+int decomposition_error() {
+  set_error(ERR_DECOMPOSITION,
+            "stair-step solver holds the whole domain on one GPU: every neighbour rank must be "
+            "MPI_PROC_NULL");
+  return ERR_DECOMPOSITION;
+}
+
+// Device buffers of one solve; fields are dense (nx,ny,nz).
+struct Frame {
+  int nx = 0, ny = 0, nz = 0;
+  size_t n = 0;
+  Geom g{};
+  Tiling tiling{};
+  double *phi = nullptr, *next = nullptr, *src = nullptr, *diag = nullptr, *rdiag = nullptr;
+  double *r = nullptr, *q = nullptr, *partials = nullptr;
+  Scalars *sc = nullptr;
+  int *isolated = nullptr;
+};
+
+int max_blocks_for(const Tiling &t, size_t n) {
+  int b = elementwise_blocks(n);
+  return t.blocks > b ? t.blocks : b;
+}
+
+// Allocates the frame's work fields and 1-D coefficients and fills the
+// coefficient arrays from the axes.
+int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double *yc,
+                const double *zc, const double *dx, const double *dy, const double *dz,
+                bool fields, bool pcg_fields) {
+  int st = 0;
+  Workspace &ws = workspace();
+  f.nx = nx;
+  f.ny = ny;
+  f.nz = nz;
+  f.n = (size_t)nx * ny * nz;
+  f.tiling = make_tiling(nx, ny, nz, 0);
+  const size_t nb = (size_t)max_blocks_for(f.tiling, f.n);
+  double *coef = (double *)ws.get(S_COEF, sizeof(double) * 2 * (nx + ny + nz), &st);
+  if (!coef) return st;
+  f.partials = (double *)ws.get(S_PARTIALS, sizeof(double) * kReduceSlots * nb, &st);
+  if (!f.partials) return st;
+  f.sc = (Scalars *)ws.get(S_SCALARS, sizeof(Scalars), &st);
+  if (!f.sc) return st;
+  f.isolated = (int *)ws.get(S_ISOLATED, sizeof(int) * 4, &st);
+  if (!f.isolated) return st;
+  f.g.nx = nx;
+  f.g.ny = ny;
+  f.g.nz = nz;
+  double *cw = coef, *ce = cw + nx, *cs = ce + nx, *cn = cs + ny, *cb = cn + ny, *ct = cb + nz;
+  f.g.cw = cw; f.g.ce = ce; f.g.cs = cs; f.g.cn = cn; f.g.cb = cb; f.g.ct = ct;
+  f.g.dx = dx; f.g.dy = dy; f.g.dz = dz;
+  f.g.xc = xc; f.g.yc = yc; f.g.zc = zc;
+  FFPM_HIP(hipMemsetAsync(f.sc, 0, sizeof(Scalars), stream()));
+  const int tot = nx + ny + nz;
+  k_axis_coefficients<<<(tot + 255) / 256, 256, 0, stream()>>>(nx, ny, nz, xc, yc, zc, dx, dy,
+                                                               dz, cw, ce, cs, cn, cb, ct);
+  FFPM_LAUNCHED("k_axis_coefficients");
+  if (fields) {
+    const size_t bytes = sizeof(double) * f.n;
+    if (!(f.phi = (double *)ws.get(S_PHI, bytes, &st))) return st;
+    if (!(f.next = (double *)ws.get(S_NEXT, bytes, &st))) return st;
+    if (!(f.src = (double *)ws.get(S_SRC, bytes, &st))) return st;
+    if (!(f.diag = (double *)ws.get(S_DIAG, bytes, &st))) return st;
+    if (!(f.rdiag = (double *)ws.get(S_RDIAG, bytes, &st))) return st;
+    if (pcg_fields) {
+      if (!(f.r = (double *)ws.get(S_R, bytes, &st))) return st;
+      if (!(f.q = (double *)ws.get(S_Q, bytes, &st))) return st;
+    }
+  }
+  return OK;
+}
+
+int read_scalars(const Frame &f, Scalars *h) {
+  FFPM_HIP(hipMemcpyAsync(h, f.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream()));
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  return OK;
+}
+
+// true_residual (src/fpm_stairstep_mpi_3d.f90:297-329); res read back.
+int true_residual(Frame &f, bool reset, double *res) {
+  const int eb = elementwise_blocks(f.n);
+  {
+    TimedLaunch tl("true_residual");
+    k_true_residual_1<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
+        f.tiling, f.g, f.phi, f.src, f.diag, f.r, f.partials, f.sc);
+    FFPM_LAUNCHED("k_true_residual_1");
+    k_true_residual_2<<<eb, kThreads, 0, stream()>>>(f.n, f.r, f.rdiag, f.diag, f.next,
+                                                     reset ? 1 : 0, f.partials, f.sc);
+    FFPM_LAUNCHED("k_true_residual_2");
+  }
+  Scalars h;
+  int st = read_scalars(f, &h);
+  if (st) return st;
+  *res = h.res;
+  return OK;
+}
+
+int residual_max(Frame &f, const double *phi, double *res) {
+  {
+    TimedLaunch tl("residual_max");
+    k_residual_max<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
+        f.tiling, f.g, phi, f.src, f.diag, f.partials, f.sc);
+    FFPM_LAUNCHED("k_residual_max");
+  }
+  Scalars h;
+  int st = read_scalars(f, &h);
+  if (st) return st;
+  *res = h.res;
+  return OK;
+}
+
+// pcg_iterate (src/fpm_stairstep_mpi_3d.f90:235-293). Iterations between two
+// convergence checks run as one chunk of launches without a host sync; a
+// breakdown or a negative rz_new inside a chunk turns the rest of the chunk
+// into no-ops, and the host resumes exactly where the reference would.
+int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
+                double *residual) {
+  const int ci = check_interval > 1 ? check_interval : 1;
+  const int eb = elementwise_blocks(f.n);
+  double res;
+  int st = true_residual(f, true, &res);
+  if (st) return st;
+  int it = 0, restarts = 0, stalled = 0;
+  double best = res;
+  while (*sweeps < max_sweeps && res > tolerance) {
+    int n = ci - it % ci;
+    if (max_sweeps - *sweeps < n) n = max_sweeps - *sweeps;
+    k_chunk_begin<<<1, 64, 0, stream()>>>(f.sc);
+    FFPM_LAUNCHED("k_chunk_begin");
+    for (int m = 0; m < n; ++m) {
+      {
+        TimedLaunch tl("spmv_dot");
+        k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
+            f.tiling, f.g, f.next, f.q, f.diag, f.partials, f.sc);
+      }
+      {
+        TimedLaunch tl("update_dot");
+        k_update_dot<<<eb, kThreads, 0, stream()>>>(f.n, f.phi, f.next, f.r, f.q, f.rdiag,
+                                                    f.partials, f.sc);
+      }
+      {
+        TimedLaunch tl("direction");
+        k_direction<<<eb, kThreads, 0, stream()>>>(f.n, f.next, f.r, f.rdiag, f.sc);
+      }
+    }
+    FFPM_LAUNCHED("pcg chunk");
+    Scalars h;
+    if ((st = read_scalars(f, &h))) return st;
+    *sweeps += h.iters;
+    it += h.iters;
+    if (h.iters > 0) restarts = 0;
+    if (h.status == kNegative) break;
+    if (h.status == kBreakdown) {
+      // Breakdown (converged to roundoff or lost conjugacy): restart once.
+      restarts = restarts + 1;
+      if (restarts > 1) break;
+      k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
+      FFPM_LAUNCHED("k_clear_status");
+      if ((st = true_residual(f, true, &res))) return st;
+      continue;
+    }
+    if (it % ci == 0 || *sweeps == max_sweeps) {
+      if ((st = true_residual(f, false, &res))) return st;
+      // Stop at the roundoff floor: no 1% improvement over 50 checks.
+      if (res < 0.99 * best) {
+        best = res;
+        stalled = 0;
+      } else {
+        stalled = stalled + 1;
+      }
+      if (stalled >= 50 || !(res == res)) break;
+    }
+  }
+  *residual = res;
+  return OK;
+}
+
+// jacobi_iterate (src/fpm_stairstep_mpi_3d.f90:200-225) with ping-pong
+// buffers: the current iterate is tracked instead of copied back.
+int jacobi_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
+                   double *residual, double **current) {
+  double *cur = f.phi, *oth = f.next;
+  while (*sweeps < max_sweeps && *residual > tolerance) {
+    int nsteps = check_interval > 1 ? check_interval : 1;
+    if (max_sweeps - *sweeps < nsteps) nsteps = max_sweeps - *sweeps;
+    for (int s = 0; s < nsteps; ++s) {
+      {
+        TimedLaunch tl("jacobi");
+        k_jacobi<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(f.tiling, f.g, cur, oth,
+                                                                         f.src, f.rdiag);
+      }
+      double *t = cur;
+      cur = oth;
+      oth = t;
+      *sweeps += 1;
+    }
+    FFPM_LAUNCHED("k_jacobi");
+    int st = residual_max(f, cur, residual);
+    if (st) return st;
+  }
+  *current = cur;
+  return OK;
+}
+
+}  // namespace
+
+// The whole solve on device pointers (src/fpm_stairstep_mpi_3d.f90:106-330).
+int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid,
+                        const double *bg, const double *xlo, const double *xhi,
+                        const double *ylo, const double *yhi, const double *zlo,
+                        const double *zhi, const double *xc, const double *yc, const double *zc,
+                        const double *dx, const double *dy, const double *dz, int nx, int ny,
+                        int nz, int west, int east, int south, int north, double tolerance,
+                        int max_sweeps, int check_interval, int method, int *sweeps,
+                        double *residual) {
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI) {
+    set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
+    return ERR_METHOD;
+  }
+  if (nx < 1 || ny < 1 || nz < 2) {
+    set_error(ERR_SHAPE, "stair-step solve needs nz>=2 and nonempty ranks");
+    return ERR_SHAPE;
+  }
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  if (!sweeps || !residual || !bg) {
+    set_error(ERR_ARGUMENT, "null output or body_grad pointer");
+    return ERR_ARGUMENT;
+  }
+  const bool pcg = method == FFPM_METHOD_PCG;
+  Frame f;
+  int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, pcg);
+  if (st) return st;
+  *sweeps = 0;
+  *residual = DBL_MAX;
+  const int eb = elementwise_blocks(f.n);
+  FFPM_HIP(hipMemsetAsync(f.isolated, 0, sizeof(int), stream()));
+  k_build_operator<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
+                                                  bg[0], bg[1], bg[2], pcg ? 1 : 0, f.src, f.diag,
+                                                  f.rdiag, f.isolated);
+  FFPM_LAUNCHED("k_build_operator");
+  int isolated = 0;
+  FFPM_HIP(hipMemcpyAsync(&isolated, f.isolated, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  if (isolated > 0) {
+    set_error(ERR_ISOLATED, "stair-step fluid cell has no fluid neighbour");
+    return ERR_ISOLATED;
+  }
+  k_pack_phi<<<eb, kThreads, 0, stream()>>>(nx, ny, nz, solid, phi_ref, f.phi);
+  FFPM_LAUNCHED("k_pack_phi");
+  double *cur = f.phi;
+  if (pcg) {
+    st = pcg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual);
+  } else {
+    st = jacobi_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual, &cur);
+  }
+  if (st) return st;
+  // gauge (src/fpm_stairstep_mpi_3d.f90:170-187)
+  k_gauge_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, cur, f.partials, f.sc);
+  FFPM_LAUNCHED("k_gauge_sums");
+  const int pb = elementwise_blocks((size_t)(nx + 2) * (ny + 2) * nz);
+  k_gauge_apply<<<pb, kThreads, 0, stream()>>>(nx, ny, nz, solid, cur, phi_ref, f.sc);
+  FFPM_LAUNCHED("k_gauge_apply");
+  return residual_max(f, cur, residual);
+}
+
+int compat_dev(const double *rhs, const int32_t *solid, const double *bg, double *xlo,
+               double *xhi, double *ylo, double *yhi, double *zlo, double *zhi,
+               const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+               int west, int east, int south, int north, double *correction,
+               double *imbalance) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error(ERR_SHAPE, "empty grid");
+    return ERR_SHAPE;
+  }
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  Frame f;
+  int st = frame_setup(f, nx, ny, nz, nullptr, nullptr, nullptr, dx, dy, dz, false, false);
+  if (st) return st;
+  const int eb = elementwise_blocks(f.n);
+  k_compat_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
+                                               bg[0], bg[1], bg[2], f.partials, f.sc);
+  FFPM_LAUNCHED("k_compat_sums");
+  Scalars h;
+  if ((st = read_scalars(f, &h))) return st;
+  const double source = h.sums[0], flux = h.sums[1], area = h.sums[2];
+  if (area <= 0) {
+    set_error(ERR_NO_OUTER_FACE, "stair-step compatibility needs at least one fluid outer face");
+    return ERR_NO_OUTER_FACE;
+  }
+  const double c = -(flux - source) / area;
+  const size_t nf = (size_t)ny * nz + (size_t)nx * nz + (size_t)nx * ny;
+  k_shift_faces<<<elementwise_blocks(nf), kThreads, 0, stream()>>>(nx, ny, nz, xlo, xhi, ylo,
+                                                                   yhi, zlo, zhi, c);
+  FFPM_LAUNCHED("k_shift_faces");
+  k_compat_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
+                                               bg[0], bg[1], bg[2], f.partials, f.sc);
+  FFPM_LAUNCHED("k_compat_sums");
+  if ((st = read_scalars(f, &h))) return st;
+  *correction = c;
+  // the reference keeps the first source integral and recomputes only the flux
+  *imbalance = h.sums[1] - source;
+  return OK;
+}
+
+int residual_face_dev(const double *phi, const double *rhs, const int32_t *solid,
+                      const double *bg, const double *xlo, const double *xhi, const double *ylo,
+                      const double *yhi, const double *zlo, const double *zhi,
+                      const double *xc, const double *yc, const double *zc, const double *dx,
+                      const double *dy, const double *dz, int nx, int ny, int nz, int west,
+                      int east, int south, int north, double *value) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error(ERR_SHAPE, "empty grid");
+    return ERR_SHAPE;
+  }
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  Frame f;
+  int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
+  if (st) return st;
+  k_face_residual<<<elementwise_blocks(f.n), kThreads, 0, stream()>>>(
+      f.g, phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, bg[0], bg[1], bg[2], f.partials, f.sc);
+  FFPM_LAUNCHED("k_face_residual");
+  Scalars h;
+  if ((st = read_scalars(f, &h))) return st;
+  *value = h.res;
+  return OK;
+}
+
+int terms_dev(const double *phi, const double *u, const double *v, const double *w,
+              const double *ox, const double *oy, const double *oz, const double *ax,
+              const double *ay, const double *az, const int32_t *solid, const double *bg,
+              const double *xc, const double *yc, const double *zc, const double *dx,
+              const double *dy, const double *dz, const double *xlo, const double *xhi,
+              const double *ylo, const double *yhi, const double *zlo, const double *zhi,
+              const double *bounds, double *gx, double *gy, double *gz, double *vortex,
+              double *advection, double *totals, int nx, int ny, int nz, int west, int east,
+              int south, int north) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error(ERR_SHAPE, "empty grid");
+    return ERR_SHAPE;
+  }
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  Frame f;
+  int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
+  if (st) return st;
+  Bounds bd;
+  for (int i = 0; i < 6; ++i) bd.b[i] = bounds[i];
+  {
+    TimedLaunch tl("terms");
+    k_terms<<<elementwise_blocks(f.n), kThreads, 0, stream()>>>(
+        f.g, phi, solid, u, v, w, ox, oy, oz, ax, ay, az, xlo, xhi, ylo, yhi, zlo, zhi, bg[0],
+        bg[1], bg[2], bd, gx, gy, gz, vortex, advection, f.partials, f.sc);
+  }
+  FFPM_LAUNCHED("k_terms");
+  Scalars h;
+  if ((st = read_scalars(f, &h))) return st;
+  totals[0] = h.sums[0];
+  totals[1] = h.sums[1];
+  return OK;
+}
+
+// ---- host-pointer staging ---------------------------------------------------
+namespace {
+
+template <class T>
+int stage_in(int slot, const T *host, size_t count, T **dev) {
+  int st = 0;
+  *dev = (T *)workspace().get(slot, sizeof(T) * count, &st);
+  if (!*dev) return st;
+  FFPM_HIP(hipMemcpyAsync(*dev, host, sizeof(T) * count, hipMemcpyHostToDevice, stream()));
+  return OK;
+}
+
+template <class T>
+int stage_out(T *host, const T *dev, size_t count) {
+  FFPM_HIP(hipMemcpyAsync(host, dev, sizeof(T) * count, hipMemcpyDeviceToHost, stream()));
+  return OK;
+}
+
+int alloc_out(int slot, size_t count, double **dev) {
+  int st = 0;
+  *dev = (double *)workspace().get(slot, sizeof(double) * count, &st);
+  return *dev ? OK : st;
+}
+
+struct Staged {
+  double *xlo, *xhi, *ylo, *yhi, *zlo, *zhi;
+  double *xc, *yc, *zc, *dx, *dy, *dz;
+};
+
+int stage_faces(int nx, int ny, int nz, const double *xlo, const double *xhi, const double *ylo,
+                const double *yhi, const double *zlo, const double *zhi, Staged &s) {
+  int st;
+  const size_t nyz = (size_t)ny * nz, nxz = (size_t)nx * nz, nxy = (size_t)nx * ny;
+  if ((st = stage_in(S_U_XLO, xlo, nyz, &s.xlo))) return st;
+  if ((st = stage_in(S_U_XHI, xhi, nyz, &s.xhi))) return st;
+  if ((st = stage_in(S_U_YLO, ylo, nxz, &s.ylo))) return st;
+  if ((st = stage_in(S_U_YHI, yhi, nxz, &s.yhi))) return st;
+  if ((st = stage_in(S_U_ZLO, zlo, nxy, &s.zlo))) return st;
+  if ((st = stage_in(S_U_ZHI, zhi, nxy, &s.zhi))) return st;
+  return OK;
+}
+
+int stage_axes(int nx, int ny, int nz, const double *xc, const double *yc, const double *zc,
+               const double *dx, const double *dy, const double *dz, Staged &s) {
+  int st;
+  if (xc && (st = stage_in(S_U_XC, xc, (size_t)nx + 2, &s.xc))) return st;
+  if (yc && (st = stage_in(S_U_YC, yc, (size_t)ny + 2, &s.yc))) return st;
+  if (zc && (st = stage_in(S_U_ZC, zc, (size_t)nz, &s.zc))) return st;
+  if ((st = stage_in(S_U_DX, dx, (size_t)nx, &s.dx))) return st;
+  if ((st = stage_in(S_U_DY, dy, (size_t)ny, &s.dy))) return st;
+  if ((st = stage_in(S_U_DZ, dz, (size_t)nz, &s.dz))) return st;
+  return OK;
+}
+
+int sync() {
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  return OK;
+}
+
+bool bad_dims(int nx, int ny, int nz) { return nx < 1 || ny < 1 || nz < 1; }
+
+int dims_error() {
+  set_error(ERR_SHAPE, "empty grid");
+  return ERR_SHAPE;
+}
+
+// Physical-boundary halo of the solid flags (src/fpm_stairstep_mpi_3d.f90:29-32).
+__global__ void k_zero_solid_halo(int nx, int ny, int nz, int32_t *solid) {
+  const size_t sx = (size_t)nx + 2, sy = (size_t)ny + 2;
+  const size_t n = sx * sy * nz;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const size_t I = p % sx, J = (p / sx) % sy;
+    if (I == 0 || I == sx - 1 || J == 0 || J == sy - 1) solid[p] = 0;
+  }
+}
+
+}  // namespace
+}  // namespace ffpm
+
+using namespace ffpm;
+
+extern "C" {
+
+int ffpm_exchange_solid_halo_3d_dev(int32_t *solid, int nx, int ny, int nz, int west, int east,
+                                    int south, int north);
+
+int ffpm_exchange_solid_halo_3d(int32_t *solid, int nx, int ny, int nz, int west, int east,
+                                int south, int north) {
+  // src/fpm_stairstep_mpi_3d.f90:29-32: zero the halo at physical boundaries.
+  if (bad_dims(nx, ny, nz)) return dims_error();
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  const size_t sx = (size_t)nx + 2, sy = (size_t)ny + 2;
+  for (int k = 0; k < nz; ++k) {
+    int32_t *plane = solid + sx * sy * k;
+    for (size_t j = 0; j < sy; ++j) {
+      plane[sx * j] = 0;
+      plane[sx * j + nx + 1] = 0;
+    }
+    for (size_t i = 0; i < sx; ++i) {
+      plane[i] = 0;
+      plane[sx * (ny + 1) + i] = 0;
+    }
+  }
+  return FFPM_OK;
+}
+
+int ffpm_exchange_solid_halo_3d_dev(int32_t *solid, int nx, int ny, int nz, int west, int east,
+                                    int south, int north) {
+  if (bad_dims(nx, ny, nz)) return dims_error();
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  const size_t n = ((size_t)nx + 2) * ((size_t)ny + 2) * nz;
+  k_zero_solid_halo<<<elementwise_blocks(n), 256, 0, stream()>>>(nx, ny, nz, solid);
+  FFPM_LAUNCHED("k_zero_solid_halo");
+  return sync();
+}
+
+int ffpm_enforce_stairstep_compatibility_3d_dev(
+    const double *rhs, const int32_t *solid, const double *body_grad, double *xlo, double *xhi,
+    double *ylo, double *yhi, double *zlo, double *zhi, const double *dx, const double *dy,
+    const double *dz, int nx, int ny, int nz, int west, int east, int south, int north,
+    double *correction, double *imbalance) {
+  return compat_dev(rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, dx, dy, dz, nx, ny, nz,
+                    west, east, south, north, correction, imbalance);
+}
+
+int ffpm_enforce_stairstep_compatibility_3d(
+    const double *rhs, const int32_t *solid, const double *body_grad, double *xlo, double *xhi,
+    double *ylo, double *yhi, double *zlo, double *zhi, const double *dx, const double *dy,
+    const double *dz, int nx, int ny, int nz, int west, int east, int south, int north,
+    double *correction, double *imbalance) {
+  if (bad_dims(nx, ny, nz)) return dims_error();
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  int st;
+  const size_t n = (size_t)nx * ny * nz, np = ((size_t)nx + 2) * ((size_t)ny + 2) * nz;
+  double *d_rhs;
+  int32_t *d_solid;
+  Staged s{};
+  if ((st = stage_in(S_U_RHS, rhs, n, &d_rhs))) return st;
+  if ((st = stage_in(S_U_SOLID, solid, np, &d_solid))) return st;
+  if ((st = stage_faces(nx, ny, nz, xlo, xhi, ylo, yhi, zlo, zhi, s))) return st;
+  if ((st = stage_axes(nx, ny, nz, nullptr, nullptr, nullptr, dx, dy, dz, s))) return st;
+  st = compat_dev(d_rhs, d_solid, body_grad, s.xlo, s.xhi, s.ylo, s.yhi, s.zlo, s.zhi, s.dx,
+                  s.dy, s.dz, nx, ny, nz, west, east, south, north, correction, imbalance);
+  if (st) return st;
+  const size_t nyz = (size_t)ny * nz, nxz = (size_t)nx * nz, nxy = (size_t)nx * ny;
+  if ((st = stage_out(xlo, s.xlo, nyz)) || (st = stage_out(xhi, s.xhi, nyz)) ||
+      (st = stage_out(ylo, s.ylo, nxz)) || (st = stage_out(yhi, s.yhi, nxz)) ||
+      (st = stage_out(zlo, s.zlo, nxy)) || (st = stage_out(zhi, s.zhi, nxy)))
+    return st;
+  return sync();
+}
+
+int ffpm_solve_stairstep_poisson_3d_dev(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
+    int method, int *sweeps, double *residual) {
+  int st = solve_stairstep_dev(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc,
+                               zc, dx, dy, dz, nx, ny, nz, west, east, south, north, tolerance,
+                               max_sweeps, check_interval, method, sweeps, residual);
+  if (st) return st;
+  return sync();
+}
+
+int ffpm_solve_stairstep_poisson_3d(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
+    int method, int *sweeps, double *residual) {
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI) {
+    set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
+    return ERR_METHOD;
+  }
+  if (nx < 1 || ny < 1 || nz < 2) {
+    set_error(ERR_SHAPE, "stair-step solve needs nz>=2 and nonempty ranks");
+    return ERR_SHAPE;
+  }
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  int st;
+  const size_t n = (size_t)nx * ny * nz, np = ((size_t)nx + 2) * ((size_t)ny + 2) * nz;
+  double *d_phi, *d_rhs;
+  int32_t *d_solid;
+  Staged s{};
+  if ((st = stage_in(S_U_PHI, phi, np, &d_phi))) return st;
+  if ((st = stage_in(S_U_RHS, rhs, n, &d_rhs))) return st;
+  if ((st = stage_in(S_U_SOLID, solid, np, &d_solid))) return st;
+  if ((st = stage_faces(nx, ny, nz, xlo, xhi, ylo, yhi, zlo, zhi, s))) return st;
+  if ((st = stage_axes(nx, ny, nz, xc, yc, zc, dx, dy, dz, s))) return st;
+  st = solve_stairstep_dev(d_phi, d_rhs, d_solid, body_grad, s.xlo, s.xhi, s.ylo, s.yhi, s.zlo,
+                           s.zhi, s.xc, s.yc, s.zc, s.dx, s.dy, s.dz, nx, ny, nz, west, east,
+                           south, north, tolerance, max_sweeps, check_interval, method, sweeps,
+                           residual);
+  if (st) return st;
+  if ((st = stage_out(phi, d_phi, np))) return st;
+  return sync();
+}
+
+int ffpm_stairstep_residual_3d_dev(
+    const double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double *value) {
+  return residual_face_dev(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc,
+                           dx, dy, dz, nx, ny, nz, west, east, south, north, value);
+}
+
+int ffpm_stairstep_residual_3d(
+    const double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double *value) {
+  if (bad_dims(nx, ny, nz)) return dims_error();
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  int st;
+  const size_t n = (size_t)nx * ny * nz, np = ((size_t)nx + 2) * ((size_t)ny + 2) * nz;
+  double *d_phi, *d_rhs;
+  int32_t *d_solid;
+  Staged s{};
+  if ((st = stage_in(S_U_PHI, phi, np, &d_phi))) return st;
+  if ((st = stage_in(S_U_RHS, rhs, n, &d_rhs))) return st;
+  if ((st = stage_in(S_U_SOLID, solid, np, &d_solid))) return st;
+  if ((st = stage_faces(nx, ny, nz, xlo, xhi, ylo, yhi, zlo, zhi, s))) return st;
+  if ((st = stage_axes(nx, ny, nz, xc, yc, zc, dx, dy, dz, s))) return st;
+  return residual_face_dev(d_phi, d_rhs, d_solid, body_grad, s.xlo, s.xhi, s.ylo, s.yhi, s.zlo,
+                           s.zhi, s.xc, s.yc, s.zc, s.dx, s.dy, s.dz, nx, ny, nz, west, east,
+                           south, north, value);
+}
+
+int ffpm_compute_stairstep_fpm_terms_3d_dev(
+    const double *phi, const double *u, const double *v, const double *w, const double *ox,
+    const double *oy, const double *oz, const double *ax, const double *ay, const double *az,
+    const int32_t *solid, const double *body_grad, const double *xc, const double *yc,
+    const double *zc, const double *dx, const double *dy, const double *dz, const double *xlo,
+    const double *xhi, const double *ylo, const double *yhi, const double *zlo,
+    const double *zhi, const double *bounds, double *gx, double *gy, double *gz, double *vortex,
+    double *advection, double *totals, int nx, int ny, int nz, int west, int east, int south,
+    int north) {
+  return terms_dev(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, body_grad, xc, yc, zc, dx, dy, dz,
+                   xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex, advection, totals,
+                   nx, ny, nz, west, east, south, north);
+}
+
+int ffpm_compute_stairstep_fpm_terms_3d(
+    const double *phi, const double *u, const double *v, const double *w, const double *ox,
+    const double *oy, const double *oz, const double *ax, const double *ay, const double *az,
+    const int32_t *solid, const double *body_grad, const double *xc, const double *yc,
+    const double *zc, const double *dx, const double *dy, const double *dz, const double *xlo,
+    const double *xhi, const double *ylo, const double *yhi, const double *zlo,
+    const double *zhi, const double *bounds, double *gx, double *gy, double *gz, double *vortex,
+    double *advection, double *totals, int nx, int ny, int nz, int west, int east, int south,
+    int north) {
+  if (bad_dims(nx, ny, nz)) return dims_error();
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  int st;
+  const size_t n = (size_t)nx * ny * nz, np = ((size_t)nx + 2) * ((size_t)ny + 2) * nz;
+  double *d_phi, *f[9], *o[5];
+  int32_t *d_solid;
+  Staged s{};
+  const double *fin[9] = {u, v, w, ox, oy, oz, ax, ay, az};
+  if ((st = stage_in(S_U_PHI, phi, np, &d_phi))) return st;
+  if ((st = stage_in(S_U_SOLID, solid, np, &d_solid))) return st;
+  for (int m = 0; m < 9; ++m)
+    if ((st = stage_in(S_U_F0 + m, fin[m], n, &f[m]))) return st;
+  for (int m = 0; m < 5; ++m)
+    if ((st = alloc_out(S_U_O0 + m, n, &o[m]))) return st;
+  if ((st = stage_faces(nx, ny, nz, xlo, xhi, ylo, yhi, zlo, zhi, s))) return st;
+  if ((st = stage_axes(nx, ny, nz, xc, yc, zc, dx, dy, dz, s))) return st;
+  st = terms_dev(d_phi, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], d_solid, body_grad,
+                 s.xc, s.yc, s.zc, s.dx, s.dy, s.dz, s.xlo, s.xhi, s.ylo, s.yhi, s.zlo, s.zhi,
+                 bounds, o[0], o[1], o[2], o[3], o[4], totals, nx, ny, nz, west, east, south,
+                 north);
+  if (st) return st;
+  double *hout[5] = {gx, gy, gz, vortex, advection};
+  for (int m = 0; m < 5; ++m)
+    if ((st = stage_out(hout[m], o[m], n))) return st;
+  return sync();
+}
+
+}  // extern "C"

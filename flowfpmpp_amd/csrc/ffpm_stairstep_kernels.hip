@@ -1,0 +1,687 @@
+// gfx950 kernels of the stair-step influence-potential solve and the FPM
+// force densities (src/fpm_stairstep_mpi_3d.f90). FP64 throughout; built with
+// -ffp-contract=off so every expression keeps the reference's rounding.
+//
+// Stencil kernels stream along z: a workgroup owns a 64x4 x-y tile and a
+// chunk of z-planes, keeps the (k-1, k, k+1) values of its own column in
+// registers and reads the four x-y neighbours of plane k through the cache
+// (each row is read from HBM once; the halo rows come from the neighbour
+// tile, which the XCD-aware tile order keeps in the same L2). The operator is
+// mask-free: solid cells hold zero in every iterate and zero src/rdiag/diag,
+// and the zero physical halo is substituted instead of stored.
+#include "ffpm_kernels.hpp"
+
+namespace ffpm {
+
+// (0:nx+1, 0:ny+1, 1:nz) halo layout, 0-based z
+__device__ __forceinline__ size_t ph(int i, int j, int k, int nx, int ny) {
+  return (size_t)i + (size_t)(nx + 2) * ((size_t)j + (size_t)(ny + 2) * (size_t)k);
+}
+
+// axis_coefficients (src/fpm_stairstep_mpi_3d.f90:499-510) and the z
+// coefficients (src/fpm_stairstep_mpi_3d.f90:138-142), one thread per entry.
+__global__ void k_axis_coefficients(int nx, int ny, int nz, const double *xc, const double *yc,
+                                    const double *zc, const double *dx, const double *dy,
+                                    const double *dz, double *cw, double *ce, double *cs,
+                                    double *cn, double *cb, double *ct) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nx) {
+    const int m = t + 1;
+    cw[t] = (m == 1) ? 0.0 : 1 / (dx[t] * (xc[m] - xc[m - 1]));
+    ce[t] = (m == nx) ? 0.0 : 1 / (dx[t] * (xc[m + 1] - xc[m]));
+  } else if (t < nx + ny) {
+    const int u = t - nx, m = u + 1;
+    cs[u] = (m == 1) ? 0.0 : 1 / (dy[u] * (yc[m] - yc[m - 1]));
+    cn[u] = (m == ny) ? 0.0 : 1 / (dy[u] * (yc[m + 1] - yc[m]));
+  } else if (t < nx + ny + nz) {
+    const int k = t - nx - ny;  // 0-based; zc is (1:nz)
+    cb[k] = (k > 0) ? 1 / (dz[k] * (zc[k] - zc[k - 1])) : 0.0;
+    ct[k] = (k < nz - 1) ? 1 / (dz[k] * (zc[k + 1] - zc[k])) : 0.0;
+  }
+}
+
+// build_operator (src/fpm_stairstep_mpi_3d.f90:336-379): folds the outer and
+// body face gradients into src, sums fluid-neighbour coefficients into diag,
+// and sets rdiag = 1/(V diag) (PCG) or 0.8/diag (Jacobi); 0 in solids.
+__global__ __launch_bounds__(256) void k_build_operator(
+    Geom g, const int32_t *__restrict__ solid, const double *__restrict__ rhs,
+    const double *__restrict__ xlo, const double *__restrict__ xhi,
+    const double *__restrict__ ylo, const double *__restrict__ yhi,
+    const double *__restrict__ zlo, const double *__restrict__ zhi, double bg0, double bg1,
+    double bg2, int for_pcg, double *__restrict__ src, double *__restrict__ diag,
+    double *__restrict__ rdiag, int *__restrict__ isolated) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  int local_isolated = 0;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx);
+    const int j = (int)((c / nx) % ny);
+    const int k = (int)(c / ((size_t)nx * ny));
+    const int I = i + 1, J = j + 1;
+    double s = 0, d = 0;
+    if (solid[ph(I, J, k, nx, ny)] == 0) {
+      const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
+      s = -rhs[c];
+      if (I == 1) s = s - xlo[j + (size_t)ny * k] / dxi;
+      else if (solid[ph(I - 1, J, k, nx, ny)] != 0) s = s - bg0 / dxi;
+      else d = d + g.cw[i];
+      if (I == nx) s = s + xhi[j + (size_t)ny * k] / dxi;
+      else if (solid[ph(I + 1, J, k, nx, ny)] != 0) s = s + bg0 / dxi;
+      else d = d + g.ce[i];
+      if (J == 1) s = s - ylo[i + (size_t)nx * k] / dyj;
+      else if (solid[ph(I, J - 1, k, nx, ny)] != 0) s = s - bg1 / dyj;
+      else d = d + g.cs[j];
+      if (J == ny) s = s + yhi[i + (size_t)nx * k] / dyj;
+      else if (solid[ph(I, J + 1, k, nx, ny)] != 0) s = s + bg1 / dyj;
+      else d = d + g.cn[j];
+      if (k == 0) s = s - zlo[i + (size_t)nx * j] / dzk;
+      else if (solid[ph(I, J, k - 1, nx, ny)] != 0) s = s - bg2 / dzk;
+      else d = d + g.cb[k];
+      if (k == nz - 1) s = s + zhi[i + (size_t)nx * j] / dzk;
+      else if (solid[ph(I, J, k + 1, nx, ny)] != 0) s = s + bg2 / dzk;
+      else d = d + g.ct[k];
+      if (d <= 0) local_isolated += 1;
+    }
+    src[c] = s;
+    diag[c] = d;
+    double rd;
+    if (d <= 0) rd = 0;
+    else if (for_pcg) rd = 1 / (g.dx[i] * g.dy[j] * g.dz[k] * d);
+    else rd = 0.8 / d;
+    rdiag[c] = rd;
+  }
+  if (local_isolated) atomicAdd(isolated, local_isolated);
+}
+
+// Init loop of src/fpm_stairstep_mpi_3d.f90:154-160 on the dense copy: phi in
+// solid cells becomes 0 (the halo is substituted, not stored).
+__global__ void k_pack_phi(int nx, int ny, int nz, const int32_t *__restrict__ solid,
+                           const double *__restrict__ phi_ref, double *__restrict__ phi) {
+  const size_t n = (size_t)nx * ny * nz;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    const size_t p = ph(i + 1, j + 1, k, nx, ny);
+    phi[c] = solid[p] != 0 ? 0.0 : phi_ref[p];
+  }
+}
+
+// Writes the dense field back to the caller's halo layout with the physical
+// halo at zero (what the reference leaves there after its init loop).
+__global__ void k_unpack_phi(int nx, int ny, int nz, const double *__restrict__ phi,
+                             double *__restrict__ phi_ref) {
+  const size_t n = (size_t)(nx + 2) * (ny + 2) * nz;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const int I = (int)(p % (nx + 2)), J = (int)((p / (nx + 2)) % (ny + 2));
+    const int k = (int)(p / ((size_t)(nx + 2) * (ny + 2)));
+    double v = 0.0;
+    if (I >= 1 && I <= nx && J >= 1 && J <= ny)
+      v = phi[(size_t)(I - 1) + (size_t)nx * ((size_t)(J - 1) + (size_t)ny * k)];
+    phi_ref[p] = v;
+  }
+}
+
+// jacobi_sweep (src/fpm_stairstep_mpi_3d.f90:384-399), z-streaming.
+__global__ __launch_bounds__(256) void k_jacobi(Tiling t, Geom g, const double *__restrict__ a,
+                                                double *__restrict__ b,
+                                                const double *__restrict__ src,
+                                                const double *__restrict__ rdiag) {
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (!tc.valid || tc.i >= nx || tc.j >= ny) return;
+  const int i = tc.i, j = tc.j;
+  const size_t nxy = (size_t)nx * ny;
+  size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+  const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+  const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+  double a_cur = a[c];
+  double a_prev = tc.k0 > 0 ? a[c - nxy] : a_cur;
+  for (int k = tc.k0; k < tc.k1; ++k) {
+    const double a_next = (k + 1 < nz) ? a[c + nxy] : a_cur;
+    const double aw = hw ? a[c - 1] : 0.0;
+    const double ae = he ? a[c + 1] : 0.0;
+    const double as = hs ? a[c - nx] : 0.0;
+    const double an = hn ? a[c + nx] : 0.0;
+    b[c] = 0.2 * a_cur + rdiag[c] * (src[c] + cw * aw + ce * ae + cs * as + cn * an +
+                                     g.cb[k] * a_prev + g.ct[k] * a_next);
+    a_prev = a_cur;
+    a_cur = a_next;
+    c += nxy;
+  }
+}
+
+// operator_residual (src/fpm_stairstep_mpi_3d.f90:514-530): max-norm of
+// src + sum(c phi_nb) - diag phi over diag > 0. Result -> sc->res.
+__global__ __launch_bounds__(256) void k_residual_max(Tiling t, Geom g,
+                                                      const double *__restrict__ phi,
+                                                      const double *__restrict__ src,
+                                                      const double *__restrict__ diag,
+                                                      double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double m = 0.0;
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    double p_cur = phi[c];
+    double p_prev = tc.k0 > 0 ? phi[c - nxy] : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const double p_next = (k + 1 < nz) ? phi[c + nxy] : p_cur;
+      const double d = diag[c];
+      if (d > 0) {
+        const double pw = hw ? phi[c - 1] : 0.0;
+        const double pe = he ? phi[c + 1] : 0.0;
+        const double ps = hs ? phi[c - nx] : 0.0;
+        const double pn = hn ? phi[c + nx] : 0.0;
+        const double v = fabs(src[c] + cw * pw + ce * pe + cs * ps + cn * pn + g.cb[k] * p_prev +
+                              g.ct[k] * p_next - d * p_cur);
+        m = fmax(m, v);
+      }
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  block_max(m, lds);
+  store_partials(partials, 1, &m, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    const double r = grid_finish_max(partials, 0, gridDim.x, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->res = r;
+      sc->counter = 0u;
+    }
+  }
+}
+
+// true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319): the
+// max-norm residual and r = V (src + sum c phi_nb - diag phi) in fluid cells
+// share one evaluation of the bracket; sums of r and of the fluid count.
+__global__ __launch_bounds__(256) void k_true_residual_1(Tiling t, Geom g,
+                                                         const double *__restrict__ phi,
+                                                         const double *__restrict__ src,
+                                                         const double *__restrict__ diag,
+                                                         double *__restrict__ r,
+                                                         double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double m = 0.0;
+  double s[2] = {0.0, 0.0};
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const double dxdy = g.dx[i] * g.dy[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    double p_cur = phi[c];
+    double p_prev = tc.k0 > 0 ? phi[c - nxy] : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const double p_next = (k + 1 < nz) ? phi[c + nxy] : p_cur;
+      const double d = diag[c];
+      double rv = 0.0;
+      if (d > 0) {
+        const double pw = hw ? phi[c - 1] : 0.0;
+        const double pe = he ? phi[c + 1] : 0.0;
+        const double ps = hs ? phi[c - nx] : 0.0;
+        const double pn = hn ? phi[c + nx] : 0.0;
+        const double v = src[c] + cw * pw + ce * pe + cs * ps + cn * pn + g.cb[k] * p_prev +
+                         g.ct[k] * p_next - d * p_cur;
+        m = fmax(m, fabs(v));
+        rv = dxdy * g.dz[k] * v;
+        s[0] += rv;
+        s[1] += 1.0;
+      }
+      r[c] = rv;
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  block_max(m, lds);
+  block_sum<2>(s, lds + 16);
+  const double vals[3] = {s[0], s[1], m};
+  store_partials(partials, 3, vals, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    const double mx = grid_finish_max(partials, 2, gridDim.x, lds);
+    double v2[2];
+    {
+      const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+      const int nthreads = blockDim.x * blockDim.y;
+      const volatile double *p = partials;
+      v2[0] = 0.0;
+      v2[1] = 0.0;
+      for (int b = tid; b < (int)gridDim.x; b += nthreads) {
+        v2[0] += p[b];
+        v2[1] += p[(size_t)gridDim.x + b];
+      }
+    }
+    block_sum<2>(v2, lds + 16);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->res = mx;
+      sc->red1 = v2[0];
+      sc->red2 = v2[1];
+      sc->counter = 0u;
+    }
+  }
+}
+
+// true_residual, second pass (src/fpm_stairstep_mpi_3d.f90:319-328): project
+// r off the constants, optionally reset p = M^-1 r, and rz = r.M^-1 r.
+__global__ __launch_bounds__(256) void k_true_residual_2(size_t n, double *__restrict__ r,
+                                                         const double *__restrict__ rdiag,
+                                                         const double *__restrict__ diag,
+                                                         double *__restrict__ p, int reset,
+                                                         double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const double shift = sc->red1 / sc->red2;
+  double s[1] = {0.0};
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    double rv = r[c];
+    if (diag[c] > 0) {
+      rv = rv - shift;
+      r[c] = rv;
+    }
+    const double rd = rdiag[c];
+    if (reset) p[c] = rd * rv;
+    s[0] += rd * rv * rv;
+  }
+  block_sum<1>(s, lds);
+  store_partials(partials, 1, s, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    double v[1];
+    grid_finish_sum<1>(partials, gridDim.x, v, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->rz = v[0];
+      sc->counter = 0u;
+    }
+  }
+}
+
+// PCG SpMV + dot (src/fpm_stairstep_mpi_3d.f90:246-265): q = A p with
+// A = V (diag - offdiag), 0 in solids; pq = p.q; then the breakdown test and
+// alpha = rz / pq on the device.
+__global__ __launch_bounds__(256) void k_spmv_dot(Tiling t, Geom g, const double *__restrict__ p,
+                                                  double *__restrict__ q,
+                                                  const double *__restrict__ diag,
+                                                  double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double s[1] = {0.0};
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const double dxdy = g.dx[i] * g.dy[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    double p_cur = p[c];
+    double p_prev = tc.k0 > 0 ? p[c - nxy] : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const double p_next = (k + 1 < nz) ? p[c + nxy] : p_cur;
+      const double d = diag[c];
+      double qv = 0.0;
+      if (d > 0) {
+        const double pw = hw ? p[c - 1] : 0.0;
+        const double pe = he ? p[c + 1] : 0.0;
+        const double ps = hs ? p[c - nx] : 0.0;
+        const double pn = hn ? p[c + nx] : 0.0;
+        qv = dxdy * g.dz[k] *
+             (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
+              g.ct[k] * p_next);
+      }
+      q[c] = qv;
+      s[0] += p_cur * qv;
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  block_sum<1>(s, lds);
+  store_partials(partials, 1, s, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    double v[1];
+    grid_finish_sum<1>(partials, gridDim.x, v, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const double pq = v[0], rz = sc->rz;
+      sc->pq = pq;
+      // The negated tests are also true for NaN.
+      if (!(pq > 0) || !(rz > 0)) sc->status = kBreakdown;
+      else sc->alpha = rz / pq;
+      sc->counter = 0u;
+    }
+  }
+}
+
+// PCG update (src/fpm_stairstep_mpi_3d.f90:267-278): phi += alpha p,
+// r -= alpha q, rz_new = r.M^-1 r; beta on the device; counts the iteration.
+__global__ __launch_bounds__(256) void k_update_dot(size_t n, double *__restrict__ phi,
+                                                    const double *__restrict__ p,
+                                                    double *__restrict__ r,
+                                                    const double *__restrict__ q,
+                                                    const double *__restrict__ rdiag,
+                                                    double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double alpha = sc->alpha;
+  double s[1] = {0.0};
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    phi[c] = phi[c] + alpha * p[c];
+    const double rv = r[c] - alpha * q[c];
+    r[c] = rv;
+    s[0] += rdiag[c] * rv * rv;
+  }
+  block_sum<1>(s, lds);
+  store_partials(partials, 1, s, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    double v[1];
+    grid_finish_sum<1>(partials, gridDim.x, v, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const double rz_new = v[0];
+      sc->rz_new = rz_new;
+      if (!(rz_new >= 0)) {
+        sc->status = kNegative;
+      } else {
+        sc->beta = rz_new / sc->rz;
+        sc->rz = rz_new;
+        sc->iters += 1;
+      }
+      sc->counter = 0u;
+    }
+  }
+}
+
+// PCG direction (src/fpm_stairstep_mpi_3d.f90:279-283): p = M^-1 r + beta p.
+__global__ __launch_bounds__(256) void k_direction(size_t n, double *__restrict__ p,
+                                                   const double *__restrict__ r,
+                                                   const double *__restrict__ rdiag,
+                                                   const Scalars *sc) {
+  if (sc->status != 0) return;
+  const double beta = sc->beta;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    p[c] = rdiag[c] * r[c] + beta * p[c];
+}
+
+__global__ void k_chunk_begin(Scalars *sc) {
+  if (threadIdx.x == 0) sc->iters = 0;
+}
+
+__global__ void k_clear_status(Scalars *sc) {
+  if (threadIdx.x == 0) {
+    sc->status = 0;
+    sc->iters = 0;
+  }
+}
+
+// Gauge sums (src/fpm_stairstep_mpi_3d.f90:170-179): fluid-volume integral
+// of phi and the fluid volume -> sc->sums[0..1].
+__global__ __launch_bounds__(256) void k_gauge_sums(Geom g, const int32_t *__restrict__ solid,
+                                                   const double *__restrict__ phi,
+                                                   double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  double s[2] = {0.0, 0.0};
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    if (solid[ph(i + 1, j + 1, k, nx, ny)] == 0) {
+      s[0] += phi[c] * g.dx[i] * g.dy[j] * g.dz[k];
+      s[1] += g.dx[i] * g.dy[j] * g.dz[k];
+    }
+  }
+  block_sum<2>(s, lds);
+  store_partials(partials, 2, s, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    double v[2];
+    grid_finish_sum<2>(partials, gridDim.x, v, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->sums[0] = v[0];
+      sc->sums[1] = v[1];
+      sc->counter = 0u;
+    }
+  }
+}
+
+// Gauge shift (src/fpm_stairstep_mpi_3d.f90:180-185) on the dense field, with
+// the caller's halo-layout copy written in the same pass.
+__global__ void k_gauge_apply(int nx, int ny, int nz, const int32_t *__restrict__ solid,
+                              double *__restrict__ phi, double *__restrict__ phi_ref,
+                              const Scalars *sc) {
+  const double mean_value = sc->sums[0] / sc->sums[1];
+  const size_t n = (size_t)(nx + 2) * (ny + 2) * nz;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const int I = (int)(p % (nx + 2)), J = (int)((p / (nx + 2)) % (ny + 2));
+    const int k = (int)(p / ((size_t)(nx + 2) * (ny + 2)));
+    double v = 0.0;
+    if (I >= 1 && I <= nx && J >= 1 && J <= ny) {
+      const size_t c = (size_t)(I - 1) + (size_t)nx * ((size_t)(J - 1) + (size_t)ny * k);
+      v = phi[c];
+      if (solid[p] == 0) {
+        v = v - mean_value;
+        phi[c] = v;
+      }
+    }
+    phi_ref[p] = v;
+  }
+}
+
+// Flux balance of src/fpm_stairstep_mpi_3d.f90:49-52,69-91: optional fluid
+// source integral, net outward flux through fluid outer faces and body faces,
+// and fluid outer-face area -> sc->sums[0..2] (source, flux, area).
+__global__ __launch_bounds__(256) void k_compat_sums(
+    Geom g, const int32_t *__restrict__ solid, const double *__restrict__ rhs,
+    const double *__restrict__ xlo, const double *__restrict__ xhi,
+    const double *__restrict__ ylo, const double *__restrict__ yhi,
+    const double *__restrict__ zlo, const double *__restrict__ zhi, double bg0, double bg1,
+    double bg2, double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    const int I = i + 1, J = j + 1;
+    if (solid[ph(I, J, k, nx, ny)] != 0) continue;
+    const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
+    s[0] += rhs[c] * dxi * dyj * dzk;
+    double f = 0.0, ar = 0.0, a;
+    a = dyj * dzk;
+    if (I == 1) { f = f - xlo[j + (size_t)ny * k] * a; ar = ar + a; }
+    else if (solid[ph(I - 1, J, k, nx, ny)] != 0) { f = f - bg0 * a; }
+    if (I == nx) { f = f + xhi[j + (size_t)ny * k] * a; ar = ar + a; }
+    else if (solid[ph(I + 1, J, k, nx, ny)] != 0) { f = f + bg0 * a; }
+    a = dxi * dzk;
+    if (J == 1) { f = f - ylo[i + (size_t)nx * k] * a; ar = ar + a; }
+    else if (solid[ph(I, J - 1, k, nx, ny)] != 0) { f = f - bg1 * a; }
+    if (J == ny) { f = f + yhi[i + (size_t)nx * k] * a; ar = ar + a; }
+    else if (solid[ph(I, J + 1, k, nx, ny)] != 0) { f = f + bg1 * a; }
+    a = dxi * dyj;
+    if (k == 0) { f = f - zlo[i + (size_t)nx * j] * a; ar = ar + a; }
+    else if (solid[ph(I, J, k - 1, nx, ny)] != 0) { f = f - bg2 * a; }
+    if (k == nz - 1) { f = f + zhi[i + (size_t)nx * j] * a; ar = ar + a; }
+    else if (solid[ph(I, J, k + 1, nx, ny)] != 0) { f = f + bg2 * a; }
+    s[1] += f;
+    s[2] += ar;
+  }
+  block_sum<3>(s, lds);
+  store_partials(partials, 3, s, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    double v[3];
+    grid_finish_sum<3>(partials, gridDim.x, v, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->sums[0] = v[0];
+      sc->sums[1] = v[1];
+      sc->sums[2] = v[2];
+      sc->counter = 0u;
+    }
+  }
+}
+
+// Uniform correction of the outer-face data (src/fpm_stairstep_mpi_3d.f90:58-62).
+__global__ void k_shift_faces(int nx, int ny, int nz, double *xlo, double *xhi, double *ylo,
+                              double *yhi, double *zlo, double *zhi, double c) {
+  const size_t nyz = (size_t)ny * nz, nxz = (size_t)nx * nz, nxy = (size_t)nx * ny;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nyz + nxz + nxy;
+       t += (size_t)gridDim.x * blockDim.x) {
+    if (t < nyz) {
+      xlo[t] = xlo[t] - c;
+      xhi[t] = xhi[t] + c;
+    } else if (t < nyz + nxz) {
+      const size_t u = t - nyz;
+      ylo[u] = ylo[u] - c;
+      yhi[u] = yhi[u] + c;
+    } else {
+      const size_t u = t - nyz - nxz;
+      zlo[u] = zlo[u] - c;
+      zhi[u] = zhi[u] + c;
+    }
+  }
+}
+
+// face_gradients (src/fpm_stairstep_mpi_3d.f90:472-497) of cell (I,J,k),
+// I,J 1-based, k 0-based; phi in the halo layout.
+__device__ __forceinline__ void face_gradients(const Geom &g, const double *__restrict__ phi,
+                                               const int32_t *__restrict__ solid,
+                                               const double *__restrict__ xlo,
+                                               const double *__restrict__ xhi,
+                                               const double *__restrict__ ylo,
+                                               const double *__restrict__ yhi,
+                                               const double *__restrict__ zlo,
+                                               const double *__restrict__ zhi, double bg0,
+                                               double bg1, double bg2, int I, int J, int k,
+                                               double gr[6]) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int i = I - 1, j = J - 1;
+  const size_t pc = ph(I, J, k, nx, ny);
+  const double p0 = phi[pc];
+  if (I == 1) gr[0] = xlo[j + (size_t)ny * k];
+  else if (solid[pc - 1] != 0) gr[0] = bg0;
+  else gr[0] = (p0 - phi[pc - 1]) / (g.xc[I] - g.xc[I - 1]);
+  if (I == nx) gr[1] = xhi[j + (size_t)ny * k];
+  else if (solid[pc + 1] != 0) gr[1] = bg0;
+  else gr[1] = (phi[pc + 1] - p0) / (g.xc[I + 1] - g.xc[I]);
+  const size_t sy = (size_t)(nx + 2);
+  if (J == 1) gr[2] = ylo[i + (size_t)nx * k];
+  else if (solid[pc - sy] != 0) gr[2] = bg1;
+  else gr[2] = (p0 - phi[pc - sy]) / (g.yc[J] - g.yc[J - 1]);
+  if (J == ny) gr[3] = yhi[i + (size_t)nx * k];
+  else if (solid[pc + sy] != 0) gr[3] = bg1;
+  else gr[3] = (phi[pc + sy] - p0) / (g.yc[J + 1] - g.yc[J]);
+  const size_t sz = (size_t)(nx + 2) * (ny + 2);
+  if (k == 0) gr[4] = zlo[i + (size_t)nx * j];
+  else if (solid[pc - sz] != 0) gr[4] = bg2;
+  else gr[4] = (p0 - phi[pc - sz]) / (g.zc[k] - g.zc[k - 1]);
+  if (k == nz - 1) gr[5] = zhi[i + (size_t)nx * j];
+  else if (solid[pc + sz] != 0) gr[5] = bg2;
+  else gr[5] = (phi[pc + sz] - p0) / (g.zc[k + 1] - g.zc[k]);
+}
+
+// stairstep_residual_3d (src/fpm_stairstep_mpi_3d.f90:403-419) from face
+// gradients -> sc->res.
+__global__ __launch_bounds__(256) void k_face_residual(
+    Geom g, const double *__restrict__ phi, const double *__restrict__ rhs,
+    const int32_t *__restrict__ solid, const double *__restrict__ xlo,
+    const double *__restrict__ xhi, const double *__restrict__ ylo,
+    const double *__restrict__ yhi, const double *__restrict__ zlo,
+    const double *__restrict__ zhi, double bg0, double bg1, double bg2, double *partials,
+    Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  double m = 0.0;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    if (solid[ph(i + 1, j + 1, k, nx, ny)] != 0) continue;
+    double gr[6];
+    face_gradients(g, phi, solid, xlo, xhi, ylo, yhi, zlo, zhi, bg0, bg1, bg2, i + 1, j + 1, k,
+                   gr);
+    m = fmax(m, fabs((gr[1] - gr[0]) / g.dx[i] + (gr[3] - gr[2]) / g.dy[j] +
+                     (gr[5] - gr[4]) / g.dz[k] - rhs[c]));
+  }
+  block_max(m, lds);
+  store_partials(partials, 1, &m, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    const double r = grid_finish_max(partials, 0, gridDim.x, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->res = r;
+      sc->counter = 0u;
+    }
+  }
+}
+
+// compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469):
+// face gradients fused with the densities and their bounded integrals
+// -> sc->sums[0..1].
+__global__ __launch_bounds__(256) void k_terms(
+    Geom g, const double *__restrict__ phi, const int32_t *__restrict__ solid,
+    const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
+    const double *__restrict__ ox, const double *__restrict__ oy,
+    const double *__restrict__ oz, const double *__restrict__ ax,
+    const double *__restrict__ ay, const double *__restrict__ az,
+    const double *__restrict__ xlo, const double *__restrict__ xhi,
+    const double *__restrict__ ylo, const double *__restrict__ yhi,
+    const double *__restrict__ zlo, const double *__restrict__ zhi, double bg0, double bg1,
+    double bg2, Bounds bd, double *__restrict__ gx, double *__restrict__ gy,
+    double *__restrict__ gz, double *__restrict__ vortex, double *__restrict__ advection,
+    double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  double s[2] = {0.0, 0.0};
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    double gr[6];
+    face_gradients(g, phi, solid, xlo, xhi, ylo, yhi, zlo, zhi, bg0, bg1, bg2, i + 1, j + 1, k,
+                   gr);
+    const double mask = solid[ph(i + 1, j + 1, k, nx, ny)] == 0 ? 1.0 : 0.0;
+    const double gxv = mask * 0.5 * (gr[0] + gr[1]);
+    const double gyv = mask * 0.5 * (gr[2] + gr[3]);
+    const double gzv = mask * 0.5 * (gr[4] + gr[5]);
+    gx[c] = gxv;
+    gy[c] = gyv;
+    gz[c] = gzv;
+    const double uu = u[c], vv = v[c], ww = w[c], ox_ = ox[c], oy_ = oy[c], oz_ = oz[c];
+    const double cx = ww * oy_ - vv * oz_;
+    const double cy = uu * oz_ - ww * ox_;
+    const double cz = vv * ox_ - uu * oy_;
+    const double vo = -(cx * gxv + cy * gyv + cz * gzv);
+    const double ad = -(ax[c] * gxv + ay[c] * gyv + az[c] * gzv);
+    vortex[c] = vo;
+    advection[c] = ad;
+    const double X = g.xc[i + 1], Y = g.yc[j + 1], Z = g.zc[k];
+    const bool inside = X > bd.b[0] && X < bd.b[1] && Y > bd.b[2] && Y < bd.b[3] &&
+                        Z > bd.b[4] && Z < bd.b[5];
+    const double volume = inside ? g.dx[i] * g.dy[j] * g.dz[k] : 0.0;
+    s[0] += volume * vo;
+    s[1] += volume * ad;
+  }
+  block_sum<2>(s, lds);
+  store_partials(partials, 2, s, gridDim.x);
+  if (arrive_last(&sc->counter, gridDim.x)) {
+    double vv[2];
+    grid_finish_sum<2>(partials, gridDim.x, vv, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      sc->sums[0] = vv[0];
+      sc->sums[1] = vv[1];
+      sc->counter = 0u;
+    }
+  }
+}
+
+}  // namespace ffpm

@@ -1,0 +1,195 @@
+"""Python mirror of the reference's ``fpm_stairstep_mpi_3d`` module over the
+HIP library's C ABI (include/ffpm.h).
+
+Same routine names, argument order and meaning as the Fortran module
+(src/fpm_stairstep_mpi_3d.f90:21-22); the MPI ``comm`` argument is dropped
+and the four neighbour ranks default to MPI_PROC_NULL (-1). Intent(out)
+scalars are returned instead of passed. Arrays may be
+
+* numpy arrays in Fortran order (float64 / int32) -- staged through host
+  memory by the library's host-pointer entry points, or
+* torch CUDA tensors holding the same Fortran-order data (a tensor ``t`` of
+  Fortran shape (a, b, c) is stored as a contiguous tensor of shape
+  (c, b, a); see :func:`fortran_tensor`) -- passed to the ``_dev`` entry
+  points, which keep everything resident in HBM.
+
+``use_openacc`` is accepted for source compatibility: this library has one
+backend, the GPU, and runs there whatever the flag says.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import FFPMError, check, lib
+
+PROC_NULL = -1
+METHODS = {"pcg": 0, "jacobi": 1}
+
+
+def _is_dev(a) -> bool:
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+def _ptr(a, dtype):
+    if _is_dev(a):
+        import torch
+        want = torch.float64 if dtype == np.float64 else torch.int32
+        if a.dtype != want or not a.is_contiguous():
+            raise TypeError(f"device array must be a contiguous {want} tensor")
+        return C.c_void_p(a.data_ptr())
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.f_contiguous:
+        raise TypeError(f"expected a Fortran-ordered numpy {np.dtype(dtype).name} array")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _d(a):
+    return _ptr(a, np.float64)
+
+
+def _i(a):
+    return _ptr(a, np.int32)
+
+
+def _vec(x, n):
+    v = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n))
+    return v, v.ctypes.data_as(C.c_void_p)
+
+
+def _mode(*arrays) -> str:
+    dev = [_is_dev(a) for a in arrays if a is not None]
+    if all(dev):
+        return "_dev"
+    if not any(dev):
+        return ""
+    raise TypeError("mix of host and device arrays")
+
+
+def _shape(solid):
+    if _is_dev(solid):
+        nz, nyp, nxp = solid.shape
+    else:
+        nxp, nyp, nz = solid.shape
+    return nxp - 2, nyp - 2, nz
+
+
+def fortran_tensor(a, device="cuda"):
+    """A contiguous torch tensor holding the Fortran-order data of numpy
+    array ``a`` (its shape is ``a.shape`` reversed)."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(device)
+
+
+def from_fortran_tensor(t) -> np.ndarray:
+    return np.asfortranarray(t.detach().cpu().numpy().T)
+
+
+def exchange_solid_halo_3d(solid, west=PROC_NULL, east=PROC_NULL, south=PROC_NULL,
+                           north=PROC_NULL):
+    """src/fpm_stairstep_mpi_3d.f90:25-36"""
+    nx, ny, nz = _shape(solid)
+    fn = getattr(lib(), "ffpm_exchange_solid_halo_3d" + _mode(solid))
+    check(fn(_i(solid), nx, ny, nz, west, east, south, north))
+
+
+def enforce_stairstep_compatibility_3d(rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, dx,
+                                       dy, dz, west=PROC_NULL, east=PROC_NULL, south=PROC_NULL,
+                                       north=PROC_NULL):
+    """src/fpm_stairstep_mpi_3d.f90:41-92; updates the face arrays in place and
+    returns (correction, imbalance)."""
+    nx, ny, nz = _shape(solid)
+    mode = _mode(rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, dx, dy, dz)
+    bg, bgp = _vec(body_grad, 3)
+    corr, imb = C.c_double(), C.c_double()
+    fn = getattr(lib(), "ffpm_enforce_stairstep_compatibility_3d" + mode)
+    check(fn(_d(rhs), _i(solid), bgp, _d(xlo), _d(xhi), _d(ylo), _d(yhi), _d(zlo), _d(zhi),
+             _d(dx), _d(dy), _d(dz), nx, ny, nz, west, east, south, north, C.byref(corr),
+             C.byref(imb)))
+    return corr.value, imb.value
+
+
+def solve_stairstep_poisson_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc,
+                               zc, dx, dy, dz, tolerance, max_sweeps, check_interval,
+                               use_openacc=True, method="pcg", west=PROC_NULL, east=PROC_NULL,
+                               south=PROC_NULL, north=PROC_NULL):
+    """src/fpm_stairstep_mpi_3d.f90:106-330; phi is updated in place and
+    (sweeps, residual) is returned."""
+    del use_openacc
+    if method not in METHODS:
+        raise FFPMError(4, "FFPM_ERR_METHOD: stair-step method must be pcg or jacobi")
+    nx, ny, nz = _shape(solid)
+    mode = _mode(phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz)
+    bg, bgp = _vec(body_grad, 3)
+    sweeps, res = C.c_int(), C.c_double()
+    fn = getattr(lib(), "ffpm_solve_stairstep_poisson_3d" + mode)
+    check(fn(_d(phi), _d(rhs), _i(solid), bgp, _d(xlo), _d(xhi), _d(ylo), _d(yhi), _d(zlo),
+             _d(zhi), _d(xc), _d(yc), _d(zc), _d(dx), _d(dy), _d(dz), nx, ny, nz, west, east,
+             south, north, C.c_double(float(tolerance)), int(max_sweeps), int(check_interval),
+             METHODS[method], C.byref(sweeps), C.byref(res)))
+    return sweeps.value, res.value
+
+
+def stairstep_residual_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc,
+                          dx, dy, dz, west=PROC_NULL, east=PROC_NULL, south=PROC_NULL,
+                          north=PROC_NULL):
+    """src/fpm_stairstep_mpi_3d.f90:403-419"""
+    nx, ny, nz = _shape(solid)
+    mode = _mode(phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz)
+    bg, bgp = _vec(body_grad, 3)
+    val = C.c_double()
+    fn = getattr(lib(), "ffpm_stairstep_residual_3d" + mode)
+    check(fn(_d(phi), _d(rhs), _i(solid), bgp, _d(xlo), _d(xhi), _d(ylo), _d(yhi), _d(zlo),
+             _d(zhi), _d(xc), _d(yc), _d(zc), _d(dx), _d(dy), _d(dz), nx, ny, nz, west, east,
+             south, north, C.byref(val)))
+    return val.value
+
+
+def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, body_grad, xc,
+                                   yc, zc, dx, dy, dz, xlo, xhi, ylo, yhi, zlo, zhi, bounds,
+                                   gx=None, gy=None, gz=None, vortex=None, advection=None,
+                                   use_openacc=True, west=PROC_NULL, east=PROC_NULL,
+                                   south=PROC_NULL, north=PROC_NULL):
+    """src/fpm_stairstep_mpi_3d.f90:426-469; returns
+    (gx, gy, gz, vortex, advection, totals). Output arrays are allocated
+    when not given (numpy for host inputs, torch for device inputs)."""
+    del use_openacc
+    nx, ny, nz = _shape(solid)
+    mode = _mode(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, xc, yc, zc, dx, dy, dz, xlo, xhi,
+                 ylo, yhi, zlo, zhi)
+    outs = [gx, gy, gz, vortex, advection]
+    for m in range(5):
+        if outs[m] is None:
+            if mode == "_dev":
+                import torch
+                outs[m] = torch.empty((nz, ny, nx), dtype=torch.float64, device=u.device)
+            else:
+                outs[m] = np.zeros((nx, ny, nz), order="F")
+    bg, bgp = _vec(body_grad, 3)
+    bd, bdp = _vec(bounds, 6)
+    totals = np.zeros(2)
+    fn = getattr(lib(), "ffpm_compute_stairstep_fpm_terms_3d" + mode)
+    check(fn(_d(phi), _d(u), _d(v), _d(w), _d(ox), _d(oy), _d(oz), _d(ax), _d(ay), _d(az),
+             _i(solid), bgp, _d(xc), _d(yc), _d(zc), _d(dx), _d(dy), _d(dz), _d(xlo), _d(xhi),
+             _d(ylo), _d(yhi), _d(zlo), _d(zhi), bdp, *[_d(o) for o in outs],
+             totals.ctypes.data_as(C.c_void_p), nx, ny, nz, west, east, south, north))
+    return (*outs, totals)
+
+
+def set_stream(stream) -> None:
+    """Launch on the given HIP stream (a torch.cuda.Stream, a raw handle, or
+    None for the null stream)."""
+    handle = 0 if stream is None else getattr(stream, "cuda_stream", stream)
+    check(lib().ffpm_set_stream(C.c_void_p(handle)))
+
+
+def timing(enable: bool) -> None:
+    check(lib().ffpm_timing_enable(1 if enable else 0))
+    check(lib().ffpm_timing_reset())
+
+
+def timing_query(kernel: str):
+    """(launches, total milliseconds) of a named hot kernel since the last reset."""
+    cnt, ms = C.c_long(), C.c_double()
+    check(lib().ffpm_timing_query(kernel.encode(), C.byref(cnt), C.byref(ms)))
+    return cnt.value, ms.value

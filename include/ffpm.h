@@ -1,0 +1,152 @@
+/*
+ * ffpm.h -- C ABI of the MI355X stair-step influence-potential solver and the
+ * FPM force densities (library: flowfpmpp_amd/lib/libffpm.so).
+ *
+ * Every entry point replaces one Fortran module procedure of the reference
+ * (cited per declaration) with the same argument order and meaning. Arrays
+ * are plain pointers to column-major (Fortran-order) data, x fastest:
+ *
+ *   phi, solid : (0:nx+1, 0:ny+1, nz)  one x-y halo layer
+ *   rhs, fields, outputs : (nx, ny, nz)
+ *   xlo, xhi : (ny, nz)   ylo, yhi : (nx, nz)   zlo, zhi : (nx, ny)
+ *   xc : (0:nx+1)  yc : (0:ny+1)  zc : (nz)  dx : (nx)  dy : (ny)  dz : (nz)
+ *   body_grad : (3)  bounds : (6)  totals : (2)
+ *
+ * Entry points without a suffix take HOST pointers (the drop-in for callers
+ * holding host arrays): they stage the arrays in device memory, run on the
+ * GPU and copy the results back. Entry points ending in _dev take DEVICE
+ * pointers for every array argument (body_grad and bounds stay on the host;
+ * scalar outputs are host pointers); their inputs stay resident in HBM.
+ *
+ * Neighbour ranks follow MPI: any negative value (MPI_PROC_NULL in MPICH,
+ * Open MPI and Intel MPI) marks a physical boundary. One GPU holds the whole
+ * domain, so every neighbour must be a physical boundary; otherwise the call
+ * returns FFPM_ERR_DECOMPOSITION. The comm argument of the reference is not
+ * needed and is not passed.
+ *
+ * Every function returns FFPM_OK (0) or an error status; ffpm_last_error()
+ * returns the message of the last failure on the calling thread. Messages of
+ * the reference's own checks are reproduced verbatim.
+ *
+ * Calls are blocking and run on one HIP stream (ffpm_set_stream). The
+ * library keeps grow-only device workspaces between calls; calls that
+ * overlap on one device must be serialised by the caller.
+ */
+#ifndef FFPM_H
+#define FFPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FFPM_OK = 0,
+  FFPM_ERR_ARGUMENT = 1,
+  FFPM_ERR_ISOLATED = 2,      /* 'stair-step fluid cell has no fluid neighbour' */
+  FFPM_ERR_NO_OUTER_FACE = 3, /* 'stair-step compatibility needs at least one fluid outer face' */
+  FFPM_ERR_METHOD = 4,        /* 'stair-step method must be pcg or jacobi' */
+  FFPM_ERR_SHAPE = 5,         /* 'stair-step solve needs nz>=2 and nonempty ranks' */
+  FFPM_ERR_DECOMPOSITION = 6, /* a neighbour rank is present */
+  FFPM_ERR_DEVICE = 7         /* HIP failure: no device, launch or memory error */
+};
+
+/* Iteration selector (src/fpm_stairstep_mpi_3d.f90:125-129). */
+enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1 };
+
+const char *ffpm_last_error(void);
+/* Sets the HIP stream (hipStream_t) every later call launches on; 0 = null stream. */
+int ffpm_set_stream(void *stream);
+/* Frees the library's device workspaces. */
+int ffpm_release(void);
+
+/* ------------------------------------------------------------------------ *
+ * Stair-step solve, src/fpm_stairstep_mpi_3d.f90
+ * ------------------------------------------------------------------------ */
+
+/* exchange_solid_halo_3d, src/fpm_stairstep_mpi_3d.f90:25-36 */
+int ffpm_exchange_solid_halo_3d(int32_t *solid, int nx, int ny, int nz, int west, int east,
+                                int south, int north);
+int ffpm_exchange_solid_halo_3d_dev(int32_t *solid, int nx, int ny, int nz, int west, int east,
+                                    int south, int north);
+
+/* enforce_stairstep_compatibility_3d, src/fpm_stairstep_mpi_3d.f90:41-92 */
+int ffpm_enforce_stairstep_compatibility_3d(
+    const double *rhs, const int32_t *solid, const double *body_grad, double *xlo, double *xhi,
+    double *ylo, double *yhi, double *zlo, double *zhi, const double *dx, const double *dy,
+    const double *dz, int nx, int ny, int nz, int west, int east, int south, int north,
+    double *correction, double *imbalance);
+int ffpm_enforce_stairstep_compatibility_3d_dev(
+    const double *rhs, const int32_t *solid, const double *body_grad, double *xlo, double *xhi,
+    double *ylo, double *yhi, double *zlo, double *zhi, const double *dx, const double *dy,
+    const double *dz, int nx, int ny, int nz, int west, int east, int south, int north,
+    double *correction, double *imbalance);
+
+/* solve_stairstep_poisson_3d, src/fpm_stairstep_mpi_3d.f90:106-330.
+ * method: FFPM_METHOD_PCG or FFPM_METHOD_JACOBI; tolerance < 0 runs exactly
+ * max_sweeps iterations. phi is the initial guess on entry; on return it
+ * holds the gauged solution, with 0 in solid cells and in the halo. */
+int ffpm_solve_stairstep_poisson_3d(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
+    int method, int *sweeps, double *residual);
+int ffpm_solve_stairstep_poisson_3d_dev(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
+    int method, int *sweeps, double *residual);
+
+/* stairstep_residual_3d, src/fpm_stairstep_mpi_3d.f90:403-419 (value out) */
+int ffpm_stairstep_residual_3d(
+    const double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double *value);
+int ffpm_stairstep_residual_3d_dev(
+    const double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
+    int east, int south, int north, double *value);
+
+/* compute_stairstep_fpm_terms_3d, src/fpm_stairstep_mpi_3d.f90:426-469 */
+int ffpm_compute_stairstep_fpm_terms_3d(
+    const double *phi, const double *u, const double *v, const double *w, const double *ox,
+    const double *oy, const double *oz, const double *ax, const double *ay, const double *az,
+    const int32_t *solid, const double *body_grad, const double *xc, const double *yc,
+    const double *zc, const double *dx, const double *dy, const double *dz, const double *xlo,
+    const double *xhi, const double *ylo, const double *yhi, const double *zlo,
+    const double *zhi, const double *bounds, double *gx, double *gy, double *gz, double *vortex,
+    double *advection, double *totals, int nx, int ny, int nz, int west, int east, int south,
+    int north);
+int ffpm_compute_stairstep_fpm_terms_3d_dev(
+    const double *phi, const double *u, const double *v, const double *w, const double *ox,
+    const double *oy, const double *oz, const double *ax, const double *ay, const double *az,
+    const int32_t *solid, const double *body_grad, const double *xc, const double *yc,
+    const double *zc, const double *dx, const double *dy, const double *dz, const double *xlo,
+    const double *xhi, const double *ylo, const double *yhi, const double *zlo,
+    const double *zhi, const double *bounds, double *gx, double *gy, double *gz, double *vortex,
+    double *advection, double *totals, int nx, int ny, int nz, int west, int east, int south,
+    int north);
+
+/* ------------------------------------------------------------------------ *
+ * Kernel timing (HIP events on the library stream around each launch of the
+ * named hot kernels; no counterpart in the reference).
+ * ------------------------------------------------------------------------ */
+int ffpm_timing_enable(int on);
+int ffpm_timing_reset(void);
+/* Kernel names: "jacobi", "spmv_dot", "update_dot", "direction",
+ * "true_residual", "residual_max", "terms". count launches, total ms. */
+int ffpm_timing_query(const char *kernel, long *count, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFPM_H */
