@@ -1,0 +1,212 @@
+"""Headline benchmark: time to solve phi_x, phi_y, phi_z to a max-norm
+residual of 1e-10 (plus their FPM force densities) on one MI355X.
+
+Workload: the 512x384x256 tanh-stretched frame (4:1 width ratio) on
+[-4,4]x[-3,3]x[-2,2] with four seeded blocked bodies (seed 2) and seeded
+16-mode divergence-free velocity/acceleration fields. One step = for each of
+the three force directions: compatibility correction, FP64 diagonal PCG
+(check interval 100) to 1e-10, gauge, and the FPM densities. Every input is
+resident in HBM before the timed region; phi starts from zero each step.
+
+Prints ONE JSON line. ``value`` is milliseconds per step (lower is better).
+``roofline`` covers the kernel with the largest share of device time, timed
+live with HIP events on the library's stream; ``cpu_baseline`` times the CPU
+oracle on this machine's host on a bounded sample of the same frame.
+
+    python bench.py [--steps K] [--warmup W] [--method pcg|jacobi]
+                    [--grid 512 384 256] [--tol 1e-10] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("time-to-solve 3 phi to 1e-10 (ms), HBM GB/s; learned-FCG vs plain FP64 PCG")
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per cell and launch (FP64 fields; 1-D coefficient
+# arrays are negligible).
+BYTES_PER_CELL = {
+    "spmv_dot": 24,      # read p, diag; write q
+    "update_dot": 56,    # read phi, p, r, q, rdiag; write phi, r
+    "direction": 32,     # read rdiag, r, p; write p
+    "jacobi": 32,        # read a, src, rdiag; write b
+    "true_residual": 72,  # pass 1: phi, src, diag, r(w); pass 2: r(rw), rdiag, diag, p(w)
+    "residual_max": 24,  # phi, src, diag
+    "terms": 124,        # phi, solid(4), 9 fields; write 5 outputs
+}
+
+
+def frame(grid):
+    from flowfpmpp_amd import workloads as W
+    nx, ny, nz = grid
+    if (nx, ny, nz) == (512, 384, 256):
+        return W.frame_512x384x256()
+    if (nx, ny, nz) == (128, 96, 64):
+        return W.frame_128x96x64()
+    return W.stretched_frame(nx, ny, nz, W.seeded_bodies(2), seed=2,
+                             label=f"{nx}x{ny}x{nz}")
+
+
+def cpu_baseline(p, gpu_iters, tol, method, budget_s=25.0):
+    """Times the CPU oracle (single thread) on this host: the marginal cost of
+    one iteration from two fixed-work runs of the x-direction solve, plus the
+    fixed setup/gauge cost, scaled to the GPU's iteration counts."""
+    from oracle import oracle as orc
+    from flowfpmpp_amd import workloads as W
+    rhs, bg, faces = W.influence_inputs(p, 0)
+    orc.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
+
+    def run(k):
+        phi = p.phi0()
+        t0 = time.perf_counter()
+        orc.solve_stairstep_poisson_3d(phi, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc, p.dx,
+                                       p.dy, p.dz, -1.0, k, k, method)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    per_guess = max(t1 / 6.0, 1e-6)
+    k2 = int(max(3, min(40, budget_s / per_guess)))
+    t2 = run(k2)
+    per_iter = (t2 - t1) / (k2 - 1)
+    setup = max(t1 - per_iter, 0.0)
+    est_s = sum(setup + per_iter * it for it in gpu_iters)
+    return {
+        "value": est_s * 1e3,
+        "unit": "ms",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"CPU oracle (gcc -O2, 1 thread), {method} on this frame, direction x: "
+                   f"1 and {k2} fixed iterations -> {per_iter * 1e3:.1f} ms/iteration + "
+                   f"{setup * 1e3:.0f} ms setup/gauge, scaled to the GPU's iteration counts "
+                   f"{list(gpu_iters)} over the three directions"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--method", default="pcg", choices=["pcg", "jacobi"])
+    ap.add_argument("--grid", type=int, nargs=3, default=[512, 384, 256])
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--check", type=int, default=100)
+    ap.add_argument("--max-iters", type=int, default=200000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the whole frame lives on one MI355X and this build "
+              "has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+
+    t_gen = time.perf_counter()
+    p = frame(tuple(args.grid))
+    T = DeviceArray.from_host
+    dev = {k: T(v) for k, v in p.fields.items()}
+    d_solid = T(p.solid)
+    axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+    d_rhs = T(p.zeros())
+    d_phi = [T(p.phi0()) for _ in range(3)]
+    faces0 = p.faces()
+    d_faces = [[T(f) for f in faces0] for _ in range(3)]
+    outs = [[DeviceArray((p.nx, p.ny, p.nz)) for _ in range(5)] for _ in range(3)]
+    bgs = [np.eye(3)[d] for d in range(3)]
+    synchronize()
+    gen_s = time.perf_counter() - t_gen
+
+    def step():
+        iters, res, totals = [], [], []
+        for d in range(3):
+            d_phi[d].zero_()
+            for f in d_faces[d]:
+                f.zero_()
+            S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, bgs[d], *d_faces[d], *axes[3:])
+            sw, r = S.solve_stairstep_poisson_3d(d_phi[d], d_rhs, d_solid, bgs[d], *d_faces[d],
+                                                 *axes, args.tol, args.max_iters, args.check,
+                                                 method=args.method)
+            o = S.compute_stairstep_fpm_terms_3d(d_phi[d], *[dev[k] for k in
+                                                             ("u", "v", "w", "ox", "oy", "oz",
+                                                              "ax", "ay", "az")],
+                                                 d_solid, bgs[d], *axes, *d_faces[d], p.bounds,
+                                                 *outs[d])
+            iters.append(sw)
+            res.append(r)
+            totals.append([float(x) for x in o[5]])
+        return iters, res, totals
+
+    for _ in range(args.warmup):
+        step()
+    synchronize()
+    S.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iters, res, totals = step()
+    synchronize()
+    elapsed = time.perf_counter() - t0
+    S.timing(False)
+
+    ms_per_step = elapsed * 1e3 / max(1, args.steps)
+    cells = p.cells
+    kern = {}
+    for name, bpc in BYTES_PER_CELL.items():
+        cnt, ms = S.timing_query(name)
+        if cnt:
+            avg_ms = ms / cnt
+            kern[name] = {"launches": cnt, "avg_us": avg_ms * 1e3, "total_ms": ms,
+                          "gbps": bpc * cells / (avg_ms * 1e-3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
+    roofline = None
+    if dom:
+        ach = kern[dom]["gbps"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": None,
+                    "bytes_per_cell": BYTES_PER_CELL[dom], "cells_per_launch": cells,
+                    "avg_launch_us": round(kern[dom]["avg_us"], 2)}
+    out = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{p.label} tanh-stretched 4:1 frame, seeded bodies and Fourier fields; "
+                         f"3 directions x (compatibility + {args.method} to {args.tol:g} "
+                         f"max-norm, check {args.check} + gauge + FPM densities)"),
+            "grid": list(args.grid), "cells": cells, "method": args.method,
+            "tolerance": args.tol, "check_interval": args.check,
+        },
+        "iterations": iters,
+        "residuals": res,
+        "converged": all(r <= args.tol for r in res) if args.tol > 0 else None,
+        "solves_per_s": round(3 * 1e3 / ms_per_step, 4),
+        "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kern.items()},
+        "roofline": roofline,
+        "input_setup_s": round(gen_s, 2),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, iters, args.tol, args.method)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

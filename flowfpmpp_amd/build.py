@@ -16,6 +16,7 @@ SOURCES = [
     "ffpm_runtime.hip",
     "ffpm_stairstep_kernels.hip",
     "ffpm_stairstep_host.hip",
+    "ffpm_laplace.hip",
 ]
 
 # -ffp-contract=off keeps every expression's rounding identical to the

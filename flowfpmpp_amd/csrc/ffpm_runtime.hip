@@ -158,6 +158,55 @@ int ffpm_release(void) {
   return FFPM_OK;
 }
 
+int ffpm_malloc(void **ptr, size_t bytes) {
+  if (!ptr) {
+    ffpm::set_error(FFPM_ERR_ARGUMENT, "null output pointer");
+    return FFPM_ERR_ARGUMENT;
+  }
+  *ptr = nullptr;
+  FFPM_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return FFPM_OK;
+}
+
+int ffpm_free(void *ptr) {
+  if (ptr) FFPM_HIP(hipFree(ptr));
+  return FFPM_OK;
+}
+
+int ffpm_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  FFPM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ffpm::stream()));
+  FFPM_HIP(hipStreamSynchronize(ffpm::stream()));
+  return FFPM_OK;
+}
+
+int ffpm_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  FFPM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ffpm::stream()));
+  FFPM_HIP(hipStreamSynchronize(ffpm::stream()));
+  return FFPM_OK;
+}
+
+int ffpm_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  FFPM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ffpm::stream()));
+  return FFPM_OK;
+}
+
+int ffpm_memset_zero(void *ptr, size_t bytes) {
+  FFPM_HIP(hipMemsetAsync(ptr, 0, bytes, ffpm::stream()));
+  return FFPM_OK;
+}
+
+int ffpm_synchronize(void) {
+  FFPM_HIP(hipStreamSynchronize(ffpm::stream()));
+  return FFPM_OK;
+}
+
+int ffpm_device_count(int *count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = e == hipSuccess ? n : 0;
+  return FFPM_OK;
+}
+
 int ffpm_timing_enable(int on) {
   ffpm::timer().enabled = on != 0;
   return FFPM_OK;

@@ -3,6 +3,7 @@
 // stall exit) follows the reference; the per-iteration scalars stay on the
 // device, and the host reads them once per convergence check.
 #include <cfloat>
+#include <initializer_list>
 #include <cstring>
 #include <string>
 
@@ -17,6 +18,18 @@ constexpr int kThreads = 256;
 
 bool any_neighbour(int west, int east, int south, int north) {
   return west >= 0 || east >= 0 || south >= 0 || north >= 0;
+}
+
+// Every array argument of a device entry point must be present.
+bool all_present(std::initializer_list<const void *> ptrs) {
+  for (const void *p : ptrs)
+    if (!p) return false;
+  return true;
+}
+
+int null_error() {
+  set_error(ERR_ARGUMENT, "null array argument");
+  return ERR_ARGUMENT;
 }
 
 int decomposition_error() {
@@ -72,10 +85,21 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   f.g.dx = dx; f.g.dy = dy; f.g.dz = dz;
   f.g.xc = xc; f.g.yc = yc; f.g.zc = zc;
   FFPM_HIP(hipMemsetAsync(f.sc, 0, sizeof(Scalars), stream()));
-  const int tot = nx + ny + nz;
-  k_axis_coefficients<<<(tot + 255) / 256, 256, 0, stream()>>>(nx, ny, nz, xc, yc, zc, dx, dy,
-                                                               dz, cw, ce, cs, cn, cb, ct);
-  FFPM_LAUNCHED("k_axis_coefficients");
+  if (!dx || !dy || !dz) {
+    set_error(ERR_ARGUMENT, "null cell-width array");
+    return ERR_ARGUMENT;
+  }
+  // The stencil coefficients need the centres; callers without centres
+  // (the compatibility correction) use only the widths.
+  if (xc && yc && zc) {
+    const int tot = nx + ny + nz;
+    k_axis_coefficients<<<(tot + 255) / 256, 256, 0, stream()>>>(nx, ny, nz, xc, yc, zc, dx, dy,
+                                                                 dz, cw, ce, cs, cn, cb, ct);
+    FFPM_LAUNCHED("k_axis_coefficients");
+  } else if (fields) {
+    set_error(ERR_ARGUMENT, "null cell-centre array");
+    return ERR_ARGUMENT;
+  }
   if (fields) {
     const size_t bytes = sizeof(double) * f.n;
     if (!(f.phi = (double *)ws.get(S_PHI, bytes, &st))) return st;
@@ -247,6 +271,8 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
     set_error(ERR_ARGUMENT, "null output or body_grad pointer");
     return ERR_ARGUMENT;
   }
+  if (!all_present({phi_ref, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz}))
+    return null_error();
   const bool pcg = method == FFPM_METHOD_PCG;
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, pcg);
@@ -294,6 +320,9 @@ int compat_dev(const double *rhs, const int32_t *solid, const double *bg, double
     return ERR_SHAPE;
   }
   if (any_neighbour(west, east, south, north)) return decomposition_error();
+  if (!all_present({rhs, solid, bg, xlo, xhi, ylo, yhi, zlo, zhi, dx, dy, dz, correction,
+                    imbalance}))
+    return null_error();
   Frame f;
   int st = frame_setup(f, nx, ny, nz, nullptr, nullptr, nullptr, dx, dy, dz, false, false);
   if (st) return st;
@@ -334,6 +363,9 @@ int residual_face_dev(const double *phi, const double *rhs, const int32_t *solid
     return ERR_SHAPE;
   }
   if (any_neighbour(west, east, south, north)) return decomposition_error();
+  if (!all_present({phi, rhs, solid, bg, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz,
+                    value}))
+    return null_error();
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
   if (st) return st;
@@ -360,6 +392,9 @@ int terms_dev(const double *phi, const double *u, const double *v, const double 
     return ERR_SHAPE;
   }
   if (any_neighbour(west, east, south, north)) return decomposition_error();
+  if (!all_present({phi, u, v, w, ox, oy, oz, ax, ay, az, solid, bg, xc, yc, zc, dx, dy, dz, xlo,
+                    xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex, advection, totals}))
+    return null_error();
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
   if (st) return st;

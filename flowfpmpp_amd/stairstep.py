@@ -8,10 +8,9 @@ scalars are returned instead of passed. Arrays may be
 
 * numpy arrays in Fortran order (float64 / int32) -- staged through host
   memory by the library's host-pointer entry points, or
-* torch CUDA tensors holding the same Fortran-order data (a tensor ``t`` of
-  Fortran shape (a, b, c) is stored as a contiguous tensor of shape
-  (c, b, a); see :func:`fortran_tensor`) -- passed to the ``_dev`` entry
-  points, which keep everything resident in HBM.
+* :class:`flowfpmpp_amd.device.DeviceArray` objects (Fortran-order data in
+  HBM) -- passed to the ``_dev`` entry points, which keep everything
+  resident on the device.
 
 ``use_openacc`` is accepted for source compatibility: this library has one
 backend, the GPU, and runs there whatever the flag says.
@@ -23,22 +22,21 @@ import ctypes as C
 import numpy as np
 
 from ._lib import FFPMError, check, lib
+from .device import DeviceArray
 
 PROC_NULL = -1
 METHODS = {"pcg": 0, "jacobi": 1}
 
 
 def _is_dev(a) -> bool:
-    return hasattr(a, "is_cuda") and a.is_cuda
+    return isinstance(a, DeviceArray)
 
 
 def _ptr(a, dtype):
     if _is_dev(a):
-        import torch
-        want = torch.float64 if dtype == np.float64 else torch.int32
-        if a.dtype != want or not a.is_contiguous():
-            raise TypeError(f"device array must be a contiguous {want} tensor")
-        return C.c_void_p(a.data_ptr())
+        if a.dtype != np.dtype(dtype):
+            raise TypeError(f"device array must hold {np.dtype(dtype).name}")
+        return a.ptr
     if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.f_contiguous:
         raise TypeError(f"expected a Fortran-ordered numpy {np.dtype(dtype).name} array")
     return a.ctypes.data_as(C.c_void_p)
@@ -67,22 +65,8 @@ def _mode(*arrays) -> str:
 
 
 def _shape(solid):
-    if _is_dev(solid):
-        nz, nyp, nxp = solid.shape
-    else:
-        nxp, nyp, nz = solid.shape
+    nxp, nyp, nz = solid.shape
     return nxp - 2, nyp - 2, nz
-
-
-def fortran_tensor(a, device="cuda"):
-    """A contiguous torch tensor holding the Fortran-order data of numpy
-    array ``a`` (its shape is ``a.shape`` reversed)."""
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(device)
-
-
-def from_fortran_tensor(t) -> np.ndarray:
-    return np.asfortranarray(t.detach().cpu().numpy().T)
 
 
 def exchange_solid_halo_3d(solid, west=PROC_NULL, east=PROC_NULL, south=PROC_NULL,
@@ -152,7 +136,7 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
                                    south=PROC_NULL, north=PROC_NULL):
     """src/fpm_stairstep_mpi_3d.f90:426-469; returns
     (gx, gy, gz, vortex, advection, totals). Output arrays are allocated
-    when not given (numpy for host inputs, torch for device inputs)."""
+    when not given (numpy for host inputs, DeviceArray for device inputs)."""
     del use_openacc
     nx, ny, nz = _shape(solid)
     mode = _mode(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, xc, yc, zc, dx, dy, dz, xlo, xhi,
@@ -160,11 +144,8 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
     outs = [gx, gy, gz, vortex, advection]
     for m in range(5):
         if outs[m] is None:
-            if mode == "_dev":
-                import torch
-                outs[m] = torch.empty((nz, ny, nx), dtype=torch.float64, device=u.device)
-            else:
-                outs[m] = np.zeros((nx, ny, nz), order="F")
+            outs[m] = (DeviceArray((nx, ny, nz)) if mode == "_dev"
+                       else np.zeros((nx, ny, nz), order="F"))
     bg, bgp = _vec(body_grad, 3)
     bd, bdp = _vec(bounds, 6)
     totals = np.zeros(2)
@@ -176,16 +157,17 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
     return (*outs, totals)
 
 
-def set_stream(stream) -> None:
-    """Launch on the given HIP stream (a torch.cuda.Stream, a raw handle, or
-    None for the null stream)."""
-    handle = 0 if stream is None else getattr(stream, "cuda_stream", stream)
-    check(lib().ffpm_set_stream(C.c_void_p(handle)))
+def set_stream(handle) -> None:
+    """Launch on the given raw hipStream_t handle (None: the null stream)."""
+    check(lib().ffpm_set_stream(C.c_void_p(0 if handle is None else int(handle))))
 
 
-def timing(enable: bool) -> None:
+def timing(enable: bool, reset: bool = True) -> None:
+    """Turns HIP-event timing of the hot kernels on or off; turning it on
+    clears the totals unless reset is False."""
     check(lib().ffpm_timing_enable(1 if enable else 0))
-    check(lib().ffpm_timing_reset())
+    if enable and reset:
+        check(lib().ffpm_timing_reset())
 
 
 def timing_query(kernel: str):

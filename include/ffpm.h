@@ -35,6 +35,7 @@
 #ifndef FFPM_H
 #define FFPM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -60,6 +61,19 @@ const char *ffpm_last_error(void);
 int ffpm_set_stream(void *stream);
 /* Frees the library's device workspaces. */
 int ffpm_release(void);
+
+/* Device memory for callers that hold their arrays in HBM (no counterpart in
+ * the reference, whose OpenACC data regions played this role). Copies are
+ * synchronous except device-to-device and the zero fill, which are ordered
+ * on the library stream. */
+int ffpm_device_count(int *count);
+int ffpm_malloc(void **ptr, size_t bytes);
+int ffpm_free(void *ptr);
+int ffpm_memcpy_htod(void *dst, const void *src, size_t bytes);
+int ffpm_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int ffpm_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int ffpm_memset_zero(void *ptr, size_t bytes);
+int ffpm_synchronize(void);
 
 /* ------------------------------------------------------------------------ *
  * Stair-step solve, src/fpm_stairstep_mpi_3d.f90
@@ -137,13 +151,104 @@ int ffpm_compute_stairstep_fpm_terms_3d_dev(
     int north);
 
 /* ------------------------------------------------------------------------ *
+ * No-body pure-Neumann solvers, src/fpm_laplace_3d.f90
+ * ------------------------------------------------------------------------ */
+
+/* enforce_neumann_compatibility_3d, src/fpm_laplace_3d.f90:9-53 */
+int ffpm_enforce_neumann_compatibility_3d(const double *rhs, double *xlo, double *xhi,
+                                          double *ylo, double *yhi, double *zlo, double *zhi,
+                                          int nx, int ny, int nz, double dx, double dy,
+                                          double dz, double *correction, double *imbalance);
+int ffpm_enforce_neumann_compatibility_3d_dev(const double *rhs, double *xlo, double *xhi,
+                                              double *ylo, double *yhi, double *zlo,
+                                              double *zhi, int nx, int ny, int nz, double dx,
+                                              double dy, double dz, double *correction,
+                                              double *imbalance);
+
+/* solve_neumann_poisson_3d, src/fpm_laplace_3d.f90:55-107; phi is
+ * (0:nx+1, 0:ny+1, 0:nz+1) with a z halo as well. */
+int ffpm_solve_neumann_poisson_3d(double *phi, const double *rhs, const double *xlo,
+                                  const double *xhi, const double *ylo, const double *yhi,
+                                  const double *zlo, const double *zhi, int nx, int ny, int nz,
+                                  double dx, double dy, double dz, double tolerance,
+                                  int max_iterations, int check_interval, int *iterations,
+                                  double *final_residual);
+int ffpm_solve_neumann_poisson_3d_dev(double *phi, const double *rhs, const double *xlo,
+                                      const double *xhi, const double *ylo, const double *yhi,
+                                      const double *zlo, const double *zhi, int nx, int ny,
+                                      int nz, double dx, double dy, double dz, double tolerance,
+                                      int max_iterations, int check_interval, int *iterations,
+                                      double *final_residual);
+
+/* enforce_neumann_compatibility_nonuniform_3d, src/fpm_laplace_3d.f90:150-217
+ * (note the reference's argument order: dx, dy, dz before nx, ny, nz). */
+int ffpm_enforce_neumann_compatibility_nonuniform_3d(
+    const double *rhs, double *xlo, double *xhi, double *ylo, double *yhi, double *zlo,
+    double *zhi, const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+    double *correction, double *imbalance);
+int ffpm_enforce_neumann_compatibility_nonuniform_3d_dev(
+    const double *rhs, double *xlo, double *xhi, double *ylo, double *yhi, double *zlo,
+    double *zhi, const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+    double *correction, double *imbalance);
+
+/* solve_neumann_poisson_nonuniform_3d, src/fpm_laplace_3d.f90:219-293;
+ * phi is (nx, ny, nz); xc (nx), yc (ny), zc (nz). */
+int ffpm_solve_neumann_poisson_nonuniform_3d(
+    double *phi, const double *rhs, const double *xlo, const double *xhi, const double *ylo,
+    const double *yhi, const double *zlo, const double *zhi, const double *xc, const double *yc,
+    const double *zc, const double *dx, const double *dy, const double *dz, int nx, int ny,
+    int nz, double tolerance, int max_iterations, int check_interval, int *iterations,
+    double *final_residual);
+int ffpm_solve_neumann_poisson_nonuniform_3d_dev(
+    double *phi, const double *rhs, const double *xlo, const double *xhi, const double *ylo,
+    const double *yhi, const double *zlo, const double *zhi, const double *xc, const double *yc,
+    const double *zc, const double *dx, const double *dy, const double *dz, int nx, int ny,
+    int nz, double tolerance, int max_iterations, int check_interval, int *iterations,
+    double *final_residual);
+
+/* compute_fpm_terms_uniform_3d, src/fpm_terms_3d.f90:8-37; phi is
+ * (0:nx+1, 0:ny+1, 0:nz+1), fluid (nx, ny, nz) int32. */
+int ffpm_compute_fpm_terms_uniform_3d(const double *phi, const double *u, const double *v,
+                                      const double *w, const double *ox, const double *oy,
+                                      const double *oz, const double *ax, const double *ay,
+                                      const double *az, const int32_t *fluid, double *gx,
+                                      double *gy, double *gz, double *vortex, double *advection,
+                                      double *totals, int nx, int ny, int nz, double dx,
+                                      double dy, double dz);
+int ffpm_compute_fpm_terms_uniform_3d_dev(const double *phi, const double *u, const double *v,
+                                          const double *w, const double *ox, const double *oy,
+                                          const double *oz, const double *ax, const double *ay,
+                                          const double *az, const int32_t *fluid, double *gx,
+                                          double *gy, double *gz, double *vortex,
+                                          double *advection, double *totals, int nx, int ny,
+                                          int nz, double dx, double dy, double dz);
+
+/* compute_fpm_terms_nonuniform_3d, src/fpm_terms_3d.f90:39-87; phi (nx, ny, nz). */
+int ffpm_compute_fpm_terms_nonuniform_3d(
+    const double *phi, const double *u, const double *v, const double *w, const double *ox,
+    const double *oy, const double *oz, const double *ax, const double *ay, const double *az,
+    const int32_t *fluid, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, const double *xlo, const double *xhi,
+    const double *ylo, const double *yhi, const double *zlo, const double *zhi,
+    const double *bounds, double *gx, double *gy, double *gz, double *vortex, double *advection,
+    double *totals, int nx, int ny, int nz);
+int ffpm_compute_fpm_terms_nonuniform_3d_dev(
+    const double *phi, const double *u, const double *v, const double *w, const double *ox,
+    const double *oy, const double *oz, const double *ax, const double *ay, const double *az,
+    const int32_t *fluid, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, const double *xlo, const double *xhi,
+    const double *ylo, const double *yhi, const double *zlo, const double *zhi,
+    const double *bounds, double *gx, double *gy, double *gz, double *vortex, double *advection,
+    double *totals, int nx, int ny, int nz);
+
+/* ------------------------------------------------------------------------ *
  * Kernel timing (HIP events on the library stream around each launch of the
  * named hot kernels; no counterpart in the reference).
  * ------------------------------------------------------------------------ */
 int ffpm_timing_enable(int on);
 int ffpm_timing_reset(void);
 /* Kernel names: "jacobi", "spmv_dot", "update_dot", "direction",
- * "true_residual", "residual_max", "terms". count launches, total ms. */
+ * "true_residual", "residual_max", "terms", "lap_sweep". count launches, total ms. */
 int ffpm_timing_query(const char *kernel, long *count, double *total_ms);
 
 #ifdef __cplusplus

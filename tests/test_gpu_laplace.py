@@ -1,0 +1,107 @@
+"""GPU parity of the no-body Neumann solvers (src/fpm_laplace_3d.f90) and the
+no-body FPM densities (src/fpm_terms_3d.f90) against the CPU oracle.
+
+The sweeps carry no reductions and the max-norm residual is order-free, so
+iteration counts and pre-gauge iterates are bit-identical; after the mean
+removal the fields differ by at most the difference of the mean sums (a
+constant). Per-cell densities are bit-identical for identical phi.
+"""
+import numpy as np
+import pytest
+
+import cases
+from flowfpmpp_amd import laplace as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _faces(c):
+    return tuple(np.array(f, order="F", copy=True) for f in c.faces)
+
+
+def test_uniform_solver_matches_oracle_and_criteria(orc):
+    print("case: uniform manufactured quadratic", flush=True)
+    c = cases.uniform_case()
+    fg, fo = _faces(c), _faces(c)
+    cg, ig = L.enforce_neumann_compatibility_3d(c.rhs, *fg, c.nx, c.ny, c.nz, c.dx, c.dy, c.dz)
+    co, io = orc.enforce_neumann_compatibility_3d(c.rhs, *fo, c.dx, c.dy, c.dz)
+    assert abs(cg - 1.0 / 6) < 1e-12 and abs(ig) < 1e-12 and abs(cg - co) < 1e-13
+    pg = np.array(c.phi, order="F", copy=True)
+    ig_, rg = L.solve_neumann_poisson_3d(pg, c.rhs, *fo, c.nx, c.ny, c.nz, c.dx, c.dy, c.dz,
+                                         1e-10, 50000, 25)
+    po = np.array(c.phi, order="F", copy=True)
+    io_, ro = orc.solve_neumann_poisson_3d(po, c.rhs, *fo, c.dx, c.dy, c.dz, 1e-10, 50000, 25)
+    assert ig_ == io_ and rg <= 1e-10
+    d = (pg - po)[1:-1, 1:-1, 1:-1]
+    assert np.abs(d).max() <= 4e-16 * np.abs(po).max() and d.max() - d.min() <= 4e-16
+    # ghosts from set_ghosts; edges and corners untouched (zero here)
+    assert np.abs(pg - po).max() <= 1e-15
+    f = c.fields
+    out = L.compute_fpm_terms_uniform_3d(pg, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"],
+                                         f["ax"], f["ay"], f["az"], c.fluid, c.nx, c.ny, c.nz,
+                                         c.dx, c.dy, c.dz)
+    ref = orc.compute_fpm_terms_uniform_3d(pg, f["u"], f["v"], f["w"], f["ox"], f["oy"],
+                                           f["oz"], f["ax"], f["ay"], f["az"], c.fluid, c.dx,
+                                           c.dy, c.dz)
+    for a, b in zip(out[:5], ref[:5]):
+        assert np.array_equal(a, b)
+    assert np.abs(out[5]).max() < 1e-12 and np.abs(out[5] - ref[5]).max() < 1e-14
+
+
+def test_nonuniform_solver_matches_oracle_and_criteria(orc):
+    print("case: nonuniform manufactured quadratic", flush=True)
+    c = cases.nonuniform_case()
+    tg, to = _faces(c), _faces(c)
+    r37 = np.asfortranarray(c.rhs + 0.37)
+    cg, ig = L.enforce_neumann_compatibility_nonuniform_3d(r37, *tg, c.dx, c.dy, c.dz, c.nx,
+                                                           c.ny, c.nz)
+    co, io = orc.enforce_neumann_compatibility_nonuniform_3d(r37, *to, c.dx, c.dy, c.dz)
+    assert abs(cg - 0.37 / 6.0) < 1e-12 and abs(ig) < 1e-12 and abs(cg - co) < 1e-13
+    pg = np.zeros((c.nx, c.ny, c.nz), order="F")
+    ig_, rg = L.solve_neumann_poisson_nonuniform_3d(pg, c.rhs, *c.faces, c.xc, c.yc, c.zc, c.dx,
+                                                    c.dy, c.dz, c.nx, c.ny, c.nz, 1e-10, 100000,
+                                                    25)
+    po = np.zeros((c.nx, c.ny, c.nz), order="F")
+    io_, ro = orc.solve_neumann_poisson_nonuniform_3d(po, c.rhs, *c.faces, c.xc, c.yc, c.zc,
+                                                      c.dx, c.dy, c.dz, 1e-10, 100000, 25)
+    assert ig_ == io_ and rg <= 1e-10
+    d = pg - po
+    assert np.abs(d).max() <= 4e-16 * np.abs(po).max() and d.max() - d.min() <= 4e-16
+    assert np.abs(pg - c.exact).max() < 1e-8
+    f = c.fields
+    args = (pg, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"], f["az"],
+            c.fluid, c.xc, c.yc, c.zc, c.dx, c.dy, c.dz, *c.faces, c.bounds)
+    out = L.compute_fpm_terms_nonuniform_3d(*args, c.nx, c.ny, c.nz)
+    ref = orc.compute_fpm_terms_nonuniform_3d(*args)
+    for a, b in zip(out[:5], ref[:5]):
+        assert np.array_equal(a, b)
+    assert np.abs(out[5] - ref[5]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("check", [10, 1])
+def test_fixed_iterations_bitwise(orc, check):
+    """Fixed work on random data pins every iterate (pre-gauge bitwise)."""
+    print(f"case: laplace fixed iterations check={check}", flush=True)
+    rng = np.random.default_rng(4)
+    c = cases.nonuniform_case()
+    rhs = np.asfortranarray(rng.standard_normal((c.nx, c.ny, c.nz)))
+    faces = tuple(np.asfortranarray(rng.standard_normal(f.shape)) for f in c.faces)
+    pg = np.zeros((c.nx, c.ny, c.nz), order="F")
+    po = np.zeros((c.nx, c.ny, c.nz), order="F")
+    a = L.solve_neumann_poisson_nonuniform_3d(pg, rhs, *faces, c.xc, c.yc, c.zc, c.dx, c.dy,
+                                              c.dz, c.nx, c.ny, c.nz, -1.0, 137, check)
+    b = orc.solve_neumann_poisson_nonuniform_3d(po, rhs, *faces, c.xc, c.yc, c.zc, c.dx, c.dy,
+                                                c.dz, -1.0, 137, check)
+    assert a[0] == b[0] == 137 and abs(a[1] - b[1]) <= 1e-13 * b[1]
+    d = pg - po
+    assert d.max() - d.min() <= 1e-15 * np.abs(po).max()
+    u = cases.uniform_case()
+    r = np.asfortranarray(rng.standard_normal((u.nx, u.ny, u.nz)))
+    pg = np.zeros((u.nx + 2, u.ny + 2, u.nz + 2), order="F")
+    po = np.zeros((u.nx + 2, u.ny + 2, u.nz + 2), order="F")
+    a = L.solve_neumann_poisson_3d(pg, r, *u.faces, u.nx, u.ny, u.nz, u.dx, u.dy, u.dz, -1.0, 77,
+                                   check)
+    b = orc.solve_neumann_poisson_3d(po, r, *u.faces, u.dx, u.dy, u.dz, -1.0, 77, check)
+    assert a[0] == b[0] == 77 and abs(a[1] - b[1]) <= 1e-13 * b[1]
+    d = (pg - po)[1:-1, 1:-1, 1:-1]
+    assert d.max() - d.min() <= 1e-15 * np.abs(po).max()

@@ -18,6 +18,7 @@ import cases
 from flowfpmpp_amd import stairstep as S
 from flowfpmpp_amd import workloads as W
 from flowfpmpp_amd._lib import FFPMError
+from flowfpmpp_amd.device import DeviceArray
 
 pytestmark = pytest.mark.gpu
 
@@ -177,7 +178,6 @@ def test_frame_128x96x64_fixed_work_and_device_entry(orc):
     """128x96x64 tanh-stretched frame, all three directions: 300 fixed PCG
     iterations and 400 fixed Jacobi sweeps agree with the oracle, and the
     device-pointer entry points give the host-pointer results."""
-    import torch
     p = W.frame_128x96x64(with_fields=True)
     for d in range(3):
         print(f"case: 128x96x64 direction {d}", flush=True)
@@ -199,8 +199,8 @@ def test_frame_128x96x64_fixed_work_and_device_entry(orc):
             else:
                 assert np.abs(pg - po).max() <= 1e-10 * np.abs(po).max()
                 assert abs(rg - ro) <= 1e-8 * ro
-        # device entry points on torch tensors
-        T = S.fortran_tensor
+        # device entry points on arrays resident in HBM
+        T = DeviceArray.from_host
         dphi = T(p.phi0())
         sd, rd = S.solve_stairstep_poisson_3d(dphi, T(rhs), T(p.solid), bg, *[T(f) for f in faces],
                                               T(p.xc), T(p.yc), T(p.zc), T(p.dx), T(p.dy),
@@ -209,8 +209,7 @@ def test_frame_128x96x64_fixed_work_and_device_entry(orc):
         sh, rh = S.solve_stairstep_poisson_3d(ph, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc, p.dx,
                                               p.dy, p.dz, -1.0, 300, 100, method="pcg")
         assert sd == sh and rd == rh
-        assert np.array_equal(S.from_fortran_tensor(dphi), ph)
-        torch.cuda.synchronize()
+        assert np.array_equal(dphi.download(), ph)
 
 
 def test_error_paths_on_gpu():
