@@ -45,6 +45,10 @@ BYTES_PER_CELL = {
 }
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def frame(grid):
     from flowfpmpp_amd import workloads as W
     nx, ny, nz = grid
@@ -125,6 +129,7 @@ def main():
     bgs = [np.eye(3)[d] for d in range(3)]
     synchronize()
     gen_s = time.perf_counter() - t_gen
+    log(f"inputs resident on the device: {p.label}, {p.cells} cells, {gen_s:.1f} s")
 
     def step():
         iters, res, totals = [], [], []
@@ -144,6 +149,7 @@ def main():
             iters.append(sw)
             res.append(r)
             totals.append([float(x) for x in o[5]])
+            log(f"direction {d}: {sw} iterations, residual {r:.3e}")
         return iters, res, totals
 
     for _ in range(args.warmup):
@@ -204,6 +210,7 @@ def main():
         "input_setup_s": round(gen_s, 2),
     }
     if not args.no_cpu:
+        log("timing the CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(p, iters, args.tol, args.method)
     print(json.dumps(out), flush=True)
 

@@ -168,23 +168,34 @@ def _draw_modes(rng, nmodes, box, kmin=1, kmax=4):
 
 def _eval_modes(modes, X, Y, Z, shape, with_curl):
     """Sum of A cos(k.x + theta) (and its curl -(k x A) sin(k.x + theta)) on
-    the tensor grid, one z-block at a time via separable complex phases."""
+    the tensor grid, one z-block at a time via separable complex phases,
+    accumulated in place through the C-contiguous (z, y, x) view of each
+    F-ordered output; work buffers are allocated once."""
     nx, ny, nz = shape
     out = [np.zeros(shape, order="F") for _ in range(6 if with_curl else 3)]
-    zb = max(1, min(nz, 4_000_000 // max(1, nx * ny)))
+    views = [o.T for o in out]  # (nz, ny, nx), C-contiguous
+    zb = max(1, min(nz, 2_000_000 // max(1, nx * ny)))
+    ph = np.empty((zb, ny, nx), dtype=np.complex128)
+    c = np.empty((zb, ny, nx))
+    s = np.empty((zb, ny, nx))
+    t = np.empty((zb, ny, nx))
     for k, amp, theta in modes:
-        ex = np.exp(1j * k[0] * X)[:, None, None]
-        ey = np.exp(1j * k[1] * Y)[None, :, None]
+        exy = np.exp(1j * k[1] * Y)[:, None] * np.exp(1j * k[0] * X)[None, :]
         curl = np.cross(k, amp)
         for z0 in range(0, nz, zb):
             z1 = min(nz, z0 + zb)
-            ez = np.exp(1j * (k[2] * Z[z0:z1] + theta))[None, None, :]
-            ph = ex * ey * ez
-            c, s = ph.real, ph.imag
+            m_ = z1 - z0
+            ez = np.exp(1j * (k[2] * Z[z0:z1] + theta))
+            np.multiply(ez[:, None, None], exy[None, :, :], out=ph[:m_])
+            np.copyto(c[:m_], ph[:m_].real)
+            if with_curl:
+                np.copyto(s[:m_], ph[:m_].imag)
             for m in range(3):
-                out[m][:, :, z0:z1] += amp[m] * c
+                np.multiply(c[:m_], amp[m], out=t[:m_])
+                np.add(views[m][z0:z1], t[:m_], out=views[m][z0:z1])
                 if with_curl:
-                    out[3 + m][:, :, z0:z1] -= curl[m] * s
+                    np.multiply(s[:m_], curl[m], out=t[:m_])
+                    np.subtract(views[3 + m][z0:z1], t[:m_], out=views[3 + m][z0:z1])
     return out
 
 
