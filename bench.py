@@ -35,6 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell and launch (FP64 fields; 1-D coefficient
 # arrays are negligible).
 BYTES_PER_CELL = {
+    "pcg_dir_spmv": 40,  # read r, rdiag, p_old, diag; write p
+    "pcg_update": 56,    # read p, diag, phi, r, rdiag; write phi, r
     "spmv_dot": 24,      # read p, diag; write q
     "update_dot": 56,    # read phi, p, r, q, rdiag; write phi, r
     "direction": 32,     # read rdiag, r, p; write p
@@ -209,6 +211,13 @@ def main():
         "roofline": roofline,
         "input_setup_s": round(gen_s, 2),
     }
+    import ctypes
+    from flowfpmpp_amd._lib import lib
+    gb, us = ctypes.c_double(), ctypes.c_double()
+    lib().ffpm_copy_bandwidth(ctypes.c_size_t(cells * 8), 20, ctypes.byref(gb), ctypes.byref(us))
+    out["copy_kernel_gbps"] = round(gb.value, 1)
+    if roofline:
+        roofline["frac_of_copy"] = round(roofline["achieved"] / gb.value, 4)
     if not args.no_cpu:
         log("timing the CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(p, iters, args.tol, args.method)

@@ -141,41 +141,27 @@ __device__ inline void block_max(double &v, double *lds) {
   }
 }
 
-// Last-workgroup reduction finish. Each workgroup reduces its values, thread 0
-// stores them as this workgroup's partials, and arrive_last() returns true in
-// every thread of the one workgroup that arrives last; that workgroup then sums
-// the partials in a fixed order (grid_finish_*), so results are reproducible
-// run to run for a fixed launch shape.
+// Grid reductions: each workgroup reduces its values and thread 0 stores them
+// as this workgroup's partials (plain stores); a one-workgroup finish kernel
+// launched next on the same stream sums the partials in a fixed order, so a
+// reduction is reproducible run to run for a fixed launch shape. The kernel
+// boundary makes the partials visible without a per-workgroup device fence.
 __device__ inline void store_partials(double *partials, int nslots, const double *vals,
                                       int nblocks) {
   if (threadIdx.x == 0 && threadIdx.y == 0)
     for (int s = 0; s < nslots; ++s) partials[(size_t)s * nblocks + blockIdx.x] = vals[s];
 }
 
-__device__ inline bool arrive_last(unsigned int *counter, int nblocks) {
-  __shared__ int s_last;
-  __syncthreads();
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    __threadfence();
-    const unsigned int prev = atomicAdd(counter, 1u);
-    s_last = (prev == (unsigned int)(nblocks - 1)) ? 1 : 0;
-  }
-  __syncthreads();
-  if (s_last) __threadfence();
-  return s_last != 0;
-}
-
-// Called by every thread of the last workgroup; thread 0 gets the sums.
+// Called by every thread of the finishing workgroup; thread 0 gets the sums.
 template <int N>
 __device__ inline void grid_finish_sum(const double *partials, int nblocks, double (&v)[N],
                                        double *lds) {
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int nthreads = blockDim.x * blockDim.y;
-  const volatile double *p = partials;
 #pragma unroll
   for (int s = 0; s < N; ++s) {
     double x = 0.0;
-    for (int b = tid; b < nblocks; b += nthreads) x += p[(size_t)s * nblocks + b];
+    for (int b = tid; b < nblocks; b += nthreads) x += partials[(size_t)s * nblocks + b];
     v[s] = x;
   }
   block_sum<N>(v, lds);
@@ -185,15 +171,11 @@ __device__ inline double grid_finish_max(const double *partials, int slot, int n
                                          double *lds) {
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int nthreads = blockDim.x * blockDim.y;
-  const volatile double *p = partials + (size_t)slot * nblocks;
+  const double *p = partials + (size_t)slot * nblocks;
   double x = 0.0;
   for (int b = tid; b < nblocks; b += nthreads) x = fmax(x, p[b]);
   block_max(x, lds);
   return x;
-}
-
-__device__ inline void reset_counter(unsigned int *counter) {
-  if (threadIdx.x == 0 && threadIdx.y == 0) *counter = 0u;
 }
 
 // Elementwise launch shape: grid-stride over n cells.

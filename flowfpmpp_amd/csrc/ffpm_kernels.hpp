@@ -8,6 +8,11 @@ namespace ffpm {
 constexpr int kBreakdown = 1;  // pq <= 0 or rz <= 0 (or NaN): restart from the true residual
 constexpr int kNegative = 2;   // rz_new < 0 or NaN: leave the iteration
 
+// direction modes of the fused PCG SpMV
+constexpr int kDirUpdate = 0;   // p = M^-1 r + beta p_old
+constexpr int kDirReset = 1;    // p = M^-1 r
+constexpr int kDirCurrent = 2;  // p_old already holds p
+
 // Scalars the iteration keeps on the device, so that a chunk of iterations
 // runs without a host round trip; the host reads them at convergence checks.
 struct Scalars {
@@ -17,8 +22,17 @@ struct Scalars {
   double sums[4];     // generic reduction results
   int status;         // 0, kBreakdown or kNegative
   int iters;          // iterations completed since the last chunk start
-  unsigned int counter;  // last-workgroup arrival counter (0 between kernels)
-  int pad;
+};
+
+// Finish operations of k_finish (what the partials of the preceding
+// reduction kernel mean and where their total goes).
+enum FinishOp : int {
+  FIN_MAX = 0,    // res = max(slot 0)
+  FIN_TR1 = 1,    // red1 = sum(slot 0), red2 = sum(slot 1), res = max(slot 2)
+  FIN_RZ = 2,     // rz = sum(slot 0)
+  FIN_PQ = 3,     // pq = sum(slot 0); breakdown test; alpha = rz / pq
+  FIN_RZNEW = 4,  // rz_new = sum(slot 0); rz_new < 0 test; beta; rz; iters += 1
+  FIN_SUMS = 5,   // sums[s] = sum(slot s), s < nslots
 };
 
 struct Bounds {
@@ -51,6 +65,12 @@ __global__ void k_update_dot(size_t n, double *phi, const double *p, double *r, 
                              const double *rdiag, double *partials, Scalars *sc);
 __global__ void k_direction(size_t n, double *p, const double *r, const double *rdiag,
                             const Scalars *sc);
+__global__ void k_pcg_dir_spmv(Tiling t, Geom g, const double *r, const double *rdiag,
+                               const double *pin, double *pout, const double *diag, int mode,
+                               double *partials, Scalars *sc);
+__global__ void k_pcg_update(Tiling t, Geom g, const double *p, const double *diag, double *phi,
+                             double *r, const double *rdiag, double *partials, Scalars *sc);
+__global__ void k_finish(int op, int nslots, const double *partials, int nblocks, Scalars *sc);
 __global__ void k_chunk_begin(Scalars *sc);
 __global__ void k_clear_status(Scalars *sc);
 __global__ void k_gauge_sums(Geom g, const int32_t *solid, const double *phi, double *partials,

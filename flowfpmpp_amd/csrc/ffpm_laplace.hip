@@ -95,13 +95,6 @@ __global__ __launch_bounds__(256) void k_lap_uniform_residual(Uniform u, Faces f
   }
   block_max(m, lds);
   store_partials(partials, 1, &m, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    const double r = grid_finish_max(partials, 0, gridDim.x, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->res = r;
-      sc->counter = 0u;
-    }
-  }
 }
 
 // Sum of phi (uniform) or of phi*V and V (weights given) -> sc->sums[0..1].
@@ -127,15 +120,6 @@ __global__ __launch_bounds__(256) void k_lap_mean_sums(int nx, int ny, int nz,
   }
   block_sum<2>(s, lds);
   store_partials(partials, 2, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[2];
-    grid_finish_sum<2>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->sums[0] = v[0];
-      sc->sums[1] = v[1];
-      sc->counter = 0u;
-    }
-  }
 }
 
 // mean_value = sums[0] / divisor (divisor < 0: sums[1]); a -= mean_value.
@@ -274,13 +258,6 @@ __global__ __launch_bounds__(256) void k_lap_nonuniform_residual(NonUniform g, F
   }
   block_max(m, lds);
   store_partials(partials, 1, &m, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    const double r = grid_finish_max(partials, 0, gridDim.x, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->res = r;
-      sc->counter = 0u;
-    }
-  }
 }
 
 // Compatibility sums of src/fpm_laplace_3d.f90:18-30 and :160-182:
@@ -330,16 +307,6 @@ __global__ __launch_bounds__(256) void k_lap_compat_sums(int nx, int ny, int nz,
   }
   block_sum<3>(s, lds);
   store_partials(partials, 3, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[3];
-    grid_finish_sum<3>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->sums[0] = v[0];
-      sc->sums[1] = v[1];
-      sc->sums[2] = v[2];
-      sc->counter = 0u;
-    }
-  }
 }
 
 // compute_fpm_terms_uniform_3d (src/fpm_terms_3d.f90:8-37): central
@@ -398,15 +365,6 @@ __global__ __launch_bounds__(256) void k_terms_nobody(
   }
   block_sum<2>(s, lds);
   store_partials(partials, 2, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double vv[2];
-    grid_finish_sum<2>(partials, gridDim.x, vv, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->sums[0] = vv[0];
-      sc->sums[1] = vv[1];
-      sc->counter = 0u;
-    }
-  }
 }
 
 struct Ctx {
@@ -423,6 +381,12 @@ int ctx_setup(Ctx &x, size_t n) {
   x.sc = (Scalars *)ws.get(S_SCALARS, sizeof(Scalars), &st);
   if (!x.sc) return st;
   FFPM_HIP(hipMemsetAsync(x.sc, 0, sizeof(Scalars), stream()));
+  return OK;
+}
+
+int finish(const Ctx &x, int op, int nslots, int nblocks) {
+  k_finish<<<1, 256, 0, stream()>>>(op, nslots, x.partials, nblocks, x.sc);
+  FFPM_LAUNCHED("k_finish");
   return OK;
 }
 
@@ -458,6 +422,7 @@ int lap_compat_dev(const double *rhs, double *xlo, double *xhi, double *ylo, dou
   k_lap_compat_sums<<<eb, kThreads, 0, stream()>>>(nx, ny, nz, rhs, f, dxv, dyv, dzv, dx, dy, dz,
                                                    1, x.partials, x.sc);
   FFPM_LAUNCHED("k_lap_compat_sums");
+  finish(x, FIN_SUMS, 3, eb);
   Scalars h;
   if ((st = read_sc(x, &h))) return st;
   const double source = h.sums[0], flux = h.sums[1];
@@ -473,6 +438,7 @@ int lap_compat_dev(const double *rhs, double *xlo, double *xhi, double *ylo, dou
   k_lap_compat_sums<<<elementwise_blocks(nf), kThreads, 0, stream()>>>(
       nx, ny, nz, rhs, f, dxv, dyv, dzv, dx, dy, dz, 0, x.partials, x.sc);
   FFPM_LAUNCHED("k_lap_compat_sums");
+  finish(x, FIN_SUMS, 3, elementwise_blocks(nf));
   if ((st = read_sc(x, &h))) return st;
   *correction = c;
   *imbalance = h.sums[1] - source;
@@ -527,12 +493,14 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
     FFPM_LAUNCHED("k_lap_uniform_sweep");
     k_lap_uniform_residual<<<eb, kThreads, 0, stream()>>>(u, f, a, rhs, x.partials, x.sc);
     FFPM_LAUNCHED("k_lap_uniform_residual");
+    finish(x, FIN_MAX, 1, eb);
     if ((st = read_sc(x, &h))) return st;
     res = h.res;
   }
   k_lap_mean_sums<<<eb, kThreads, 0, stream()>>>(nx, ny, nz, a, nullptr, nullptr, nullptr,
                                                  x.partials, x.sc);
   FFPM_LAUNCHED("k_lap_mean_sums");
+  finish(x, FIN_SUMS, 2, eb);
   k_lap_subtract_mean<<<eb, kThreads, 0, stream()>>>(n, a, (double)(nx * ny * nz), x.sc);
   FFPM_LAUNCHED("k_lap_subtract_mean");
   const size_t np = ((size_t)nx + 2) * ((size_t)ny + 2) * ((size_t)nz + 2);
@@ -541,6 +509,7 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
   FFPM_LAUNCHED("k_lap_unpack_ghosts");
   k_lap_uniform_residual<<<eb, kThreads, 0, stream()>>>(u, f, a, rhs, x.partials, x.sc);
   FFPM_LAUNCHED("k_lap_uniform_residual");
+  finish(x, FIN_MAX, 1, eb);
   if ((st = read_sc(x, &h))) return st;
   *final_residual = h.res;
   return OK;
@@ -601,6 +570,7 @@ int lap_nonuniform_solve_dev(double *phi, const double *rhs, const double *xlo,
     FFPM_LAUNCHED("k_lap_nonuniform_sweep");
     k_lap_nonuniform_residual<<<eb, kThreads, 0, stream()>>>(g, f, a, rhs, x.partials, x.sc);
     FFPM_LAUNCHED("k_lap_nonuniform_residual");
+    finish(x, FIN_MAX, 1, eb);
     if ((st = read_sc(x, &h))) return st;
     res = h.res;
   }
@@ -610,10 +580,12 @@ int lap_nonuniform_solve_dev(double *phi, const double *rhs, const double *xlo,
   }
   k_lap_mean_sums<<<eb, kThreads, 0, stream()>>>(nx, ny, nz, a, dx, dy, dz, x.partials, x.sc);
   FFPM_LAUNCHED("k_lap_mean_sums");
+  finish(x, FIN_SUMS, 2, eb);
   k_lap_subtract_mean<<<eb, kThreads, 0, stream()>>>(n, a, -1.0, x.sc);
   FFPM_LAUNCHED("k_lap_subtract_mean");
   k_lap_nonuniform_residual<<<eb, kThreads, 0, stream()>>>(g, f, a, rhs, x.partials, x.sc);
   FFPM_LAUNCHED("k_lap_nonuniform_residual");
+  finish(x, FIN_MAX, 1, eb);
   if ((st = read_sc(x, &h))) return st;
   *final_residual = h.res;
   return OK;
@@ -648,6 +620,7 @@ int terms_nobody_dev(int uniform, const double *phi, const double *u, const doub
       uniform, g, f, phi, dx, dy, dz, u, v, w, ox, oy, oz, ax, ay, az, fluid, bd, gx, gy, gz,
       vortex, advection, x.partials, x.sc);
   FFPM_LAUNCHED("k_terms_nobody");
+  finish(x, FIN_SUMS, 2, elementwise_blocks(n));
   Scalars h;
   if ((st = read_sc(x, &h))) return st;
   totals[0] = h.sums[0];

@@ -141,9 +141,47 @@ int KernelTimer::collect() {
   return 0;
 }
 
+// Plain copy of n doubles, 16-byte vector loads and stores: the achievable
+// streaming bandwidth the stencil kernels are compared with.
+__global__ __launch_bounds__(256) void k_copy(size_t n2, const double2 *__restrict__ a,
+                                              double2 *__restrict__ b) {
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n2;
+       c += (size_t)gridDim.x * blockDim.x)
+    b[c] = a[c];
+}
+
 }  // namespace ffpm
 
 extern "C" {
+
+int ffpm_copy_bandwidth(size_t bytes, int reps, double *gbps, double *avg_us) {
+  void *a = nullptr, *b = nullptr;
+  const size_t n2 = bytes / 16;
+  FFPM_HIP(hipMalloc(&a, n2 * 16));
+  FFPM_HIP(hipMalloc(&b, n2 * 16));
+  FFPM_HIP(hipMemsetAsync(a, 0, n2 * 16, ffpm::stream()));
+  hipEvent_t e0, e1;
+  FFPM_HIP(hipEventCreate(&e0));
+  FFPM_HIP(hipEventCreate(&e1));
+  size_t blocks = (n2 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  ffpm::k_copy<<<(int)blocks, 256, 0, ffpm::stream()>>>(n2, (const double2 *)a, (double2 *)b);
+  FFPM_HIP(hipEventRecord(e0, ffpm::stream()));
+  for (int r = 0; r < reps; ++r)
+    ffpm::k_copy<<<(int)blocks, 256, 0, ffpm::stream()>>>(n2, (const double2 *)a, (double2 *)b);
+  FFPM_HIP(hipEventRecord(e1, ffpm::stream()));
+  FFPM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FFPM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *avg_us = 1e3 * ms / reps;
+  *gbps = 2.0 * (double)(n2 * 16) / (*avg_us * 1e-6) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  FFPM_HIP(hipFree(a));
+  FFPM_HIP(hipFree(b));
+  return FFPM_OK;
+}
+
 
 const char *ffpm_last_error(void) { return ffpm::g_error.c_str(); }
 

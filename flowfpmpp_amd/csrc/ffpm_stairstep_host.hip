@@ -3,6 +3,7 @@
 // stall exit) follows the reference; the per-iteration scalars stay on the
 // device, and the host reads them once per convergence check.
 #include <cfloat>
+#include <cstdlib>
 #include <initializer_list>
 #include <cstring>
 #include <string>
@@ -115,13 +116,22 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   return OK;
 }
 
+// Totals the partials of the reduction kernel just launched (k_finish).
+int finish(const Frame &f, int op, int nslots, int nblocks) {
+  k_finish<<<1, 256, 0, stream()>>>(op, nslots, f.partials, nblocks, f.sc);
+  FFPM_LAUNCHED("k_finish");
+  return OK;
+}
+
 int read_scalars(const Frame &f, Scalars *h) {
   FFPM_HIP(hipMemcpyAsync(h, f.sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream()));
   FFPM_HIP(hipStreamSynchronize(stream()));
   return OK;
 }
 
-// true_residual (src/fpm_stairstep_mpi_3d.f90:297-329); res read back.
+// true_residual (src/fpm_stairstep_mpi_3d.f90:297-329); res read back. With
+// reset the direction p = M^-1 r is written to f.next (the three-kernel
+// iteration); the fused iteration applies the reset in its next SpMV instead.
 int true_residual(Frame &f, bool reset, double *res) {
   const int eb = elementwise_blocks(f.n);
   {
@@ -129,10 +139,15 @@ int true_residual(Frame &f, bool reset, double *res) {
     k_true_residual_1<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
         f.tiling, f.g, f.phi, f.src, f.diag, f.r, f.partials, f.sc);
     FFPM_LAUNCHED("k_true_residual_1");
+  }
+  finish(f, FIN_TR1, 3, f.tiling.blocks);
+  {
+    TimedLaunch tl("true_residual");
     k_true_residual_2<<<eb, kThreads, 0, stream()>>>(f.n, f.r, f.rdiag, f.diag, f.next,
                                                      reset ? 1 : 0, f.partials, f.sc);
     FFPM_LAUNCHED("k_true_residual_2");
   }
+  finish(f, FIN_RZ, 1, eb);
   Scalars h;
   int st = read_scalars(f, &h);
   if (st) return st;
@@ -147,6 +162,7 @@ int residual_max(Frame &f, const double *phi, double *res) {
         f.tiling, f.g, phi, f.src, f.diag, f.partials, f.sc);
     FFPM_LAUNCHED("k_residual_max");
   }
+  finish(f, FIN_MAX, 1, f.tiling.blocks);
   Scalars h;
   int st = read_scalars(f, &h);
   if (st) return st;
@@ -178,11 +194,13 @@ int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, 
         k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
             f.tiling, f.g, f.next, f.q, f.diag, f.partials, f.sc);
       }
+      finish(f, FIN_PQ, 1, f.tiling.blocks);
       {
         TimedLaunch tl("update_dot");
         k_update_dot<<<eb, kThreads, 0, stream()>>>(f.n, f.phi, f.next, f.r, f.q, f.rdiag,
                                                     f.partials, f.sc);
       }
+      finish(f, FIN_RZNEW, 1, eb);
       {
         TimedLaunch tl("direction");
         k_direction<<<eb, kThreads, 0, stream()>>>(f.n, f.next, f.r, f.rdiag, f.sc);
@@ -218,6 +236,90 @@ int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, 
   }
   *residual = res;
   return OK;
+}
+
+// pcg_iterate with the fused two-kernel iteration (k_pcg_dir_spmv +
+// k_pcg_update): the same formulas and the same convergence logic; the
+// direction update p = M^-1 r + beta p of iteration k runs inside the SpMV of
+// iteration k+1, except before a convergence check, where it runs on its own
+// with the recursively updated r exactly as the reference orders it
+// (src/fpm_stairstep_mpi_3d.f90:279-286), and after a restart, where the next
+// SpMV starts from p = M^-1 r.
+int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_interval,
+                      int *sweeps, double *residual) {
+  const int ci = check_interval > 1 ? check_interval : 1;
+  const int eb = elementwise_blocks(f.n);
+  double *pbuf[2] = {f.next, f.q};
+  int cur = 0;
+  double res;
+  int st = true_residual(f, false, &res);
+  if (st) return st;
+  int mode = kDirReset;
+  int it = 0, restarts = 0, stalled = 0;
+  double best = res;
+  const dim3 blk(kTileX, kTileY);
+  while (*sweeps < max_sweeps && res > tolerance) {
+    int n = ci - it % ci;
+    if (max_sweeps - *sweeps < n) n = max_sweeps - *sweeps;
+    k_chunk_begin<<<1, 64, 0, stream()>>>(f.sc);
+    FFPM_LAUNCHED("k_chunk_begin");
+    for (int m = 0; m < n; ++m) {
+      const int dst = cur ^ 1;
+      {
+        TimedLaunch tl("pcg_dir_spmv");
+        k_pcg_dir_spmv<<<f.tiling.blocks, blk, 0, stream()>>>(
+            f.tiling, f.g, f.r, f.rdiag, pbuf[cur], pbuf[dst], f.diag, mode, f.partials, f.sc);
+      }
+      finish(f, FIN_PQ, 1, f.tiling.blocks);
+      if (mode != kDirCurrent) cur = dst;
+      {
+        TimedLaunch tl("pcg_update");
+        k_pcg_update<<<f.tiling.blocks, blk, 0, stream()>>>(f.tiling, f.g, pbuf[cur], f.diag,
+                                                            f.phi, f.r, f.rdiag, f.partials, f.sc);
+      }
+      finish(f, FIN_RZNEW, 1, f.tiling.blocks);
+      mode = kDirUpdate;
+    }
+    FFPM_LAUNCHED("fused pcg chunk");
+    Scalars h;
+    if ((st = read_scalars(f, &h))) return st;
+    *sweeps += h.iters;
+    it += h.iters;
+    if (h.iters > 0) restarts = 0;
+    if (h.status == kNegative) break;
+    if (h.status == kBreakdown) {
+      restarts = restarts + 1;
+      if (restarts > 1) break;
+      k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
+      FFPM_LAUNCHED("k_clear_status");
+      if ((st = true_residual(f, false, &res))) return st;
+      mode = kDirReset;
+      continue;
+    }
+    if (it % ci == 0 || *sweeps == max_sweeps) {
+      {
+        TimedLaunch tl("direction");
+        k_direction<<<eb, kThreads, 0, stream()>>>(f.n, pbuf[cur], f.r, f.rdiag, f.sc);
+      }
+      FFPM_LAUNCHED("k_direction");
+      mode = kDirCurrent;
+      if ((st = true_residual(f, false, &res))) return st;
+      if (res < 0.99 * best) {
+        best = res;
+        stalled = 0;
+      } else {
+        stalled = stalled + 1;
+      }
+      if (stalled >= 50 || !(res == res)) break;
+    }
+  }
+  *residual = res;
+  return OK;
+}
+
+bool classic_pcg() {
+  const char *e = std::getenv("FFPM_PCG");
+  return e && std::strcmp(e, "classic") == 0;
 }
 
 // jacobi_iterate (src/fpm_stairstep_mpi_3d.f90:200-225) with ping-pong
@@ -296,7 +398,9 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   FFPM_LAUNCHED("k_pack_phi");
   double *cur = f.phi;
   if (pcg) {
-    st = pcg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual);
+    st = classic_pcg()
+             ? pcg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual)
+             : pcg_iterate_fused(f, tolerance, max_sweeps, check_interval, sweeps, residual);
   } else {
     st = jacobi_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual, &cur);
   }
@@ -304,6 +408,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   // gauge (src/fpm_stairstep_mpi_3d.f90:170-187)
   k_gauge_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, cur, f.partials, f.sc);
   FFPM_LAUNCHED("k_gauge_sums");
+  finish(f, FIN_SUMS, 2, eb);
   const int pb = elementwise_blocks((size_t)(nx + 2) * (ny + 2) * nz);
   k_gauge_apply<<<pb, kThreads, 0, stream()>>>(nx, ny, nz, solid, cur, phi_ref, f.sc);
   FFPM_LAUNCHED("k_gauge_apply");
@@ -330,6 +435,7 @@ int compat_dev(const double *rhs, const int32_t *solid, const double *bg, double
   k_compat_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
                                                bg[0], bg[1], bg[2], f.partials, f.sc);
   FFPM_LAUNCHED("k_compat_sums");
+  finish(f, FIN_SUMS, 3, eb);
   Scalars h;
   if ((st = read_scalars(f, &h))) return st;
   const double source = h.sums[0], flux = h.sums[1], area = h.sums[2];
@@ -345,6 +451,7 @@ int compat_dev(const double *rhs, const int32_t *solid, const double *bg, double
   k_compat_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
                                                bg[0], bg[1], bg[2], f.partials, f.sc);
   FFPM_LAUNCHED("k_compat_sums");
+  finish(f, FIN_SUMS, 3, eb);
   if ((st = read_scalars(f, &h))) return st;
   *correction = c;
   // the reference keeps the first source integral and recomputes only the flux
@@ -372,6 +479,7 @@ int residual_face_dev(const double *phi, const double *rhs, const int32_t *solid
   k_face_residual<<<elementwise_blocks(f.n), kThreads, 0, stream()>>>(
       f.g, phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, bg[0], bg[1], bg[2], f.partials, f.sc);
   FFPM_LAUNCHED("k_face_residual");
+  finish(f, FIN_MAX, 1, elementwise_blocks(f.n));
   Scalars h;
   if ((st = read_scalars(f, &h))) return st;
   *value = h.res;
@@ -407,6 +515,7 @@ int terms_dev(const double *phi, const double *u, const double *v, const double 
         bg[1], bg[2], bd, gx, gy, gz, vortex, advection, f.partials, f.sc);
   }
   FFPM_LAUNCHED("k_terms");
+  finish(f, FIN_SUMS, 2, elementwise_blocks(f.n));
   Scalars h;
   if ((st = read_scalars(f, &h))) return st;
   totals[0] = h.sums[0];

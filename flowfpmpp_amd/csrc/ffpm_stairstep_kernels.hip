@@ -190,13 +190,6 @@ __global__ __launch_bounds__(256) void k_residual_max(Tiling t, Geom g,
   }
   block_max(m, lds);
   store_partials(partials, 1, &m, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    const double r = grid_finish_max(partials, 0, gridDim.x, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->res = r;
-      sc->counter = 0u;
-    }
-  }
 }
 
 // true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319): the
@@ -248,28 +241,6 @@ __global__ __launch_bounds__(256) void k_true_residual_1(Tiling t, Geom g,
   block_sum<2>(s, lds + 16);
   const double vals[3] = {s[0], s[1], m};
   store_partials(partials, 3, vals, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    const double mx = grid_finish_max(partials, 2, gridDim.x, lds);
-    double v2[2];
-    {
-      const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-      const int nthreads = blockDim.x * blockDim.y;
-      const volatile double *p = partials;
-      v2[0] = 0.0;
-      v2[1] = 0.0;
-      for (int b = tid; b < (int)gridDim.x; b += nthreads) {
-        v2[0] += p[b];
-        v2[1] += p[(size_t)gridDim.x + b];
-      }
-    }
-    block_sum<2>(v2, lds + 16);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->res = mx;
-      sc->red1 = v2[0];
-      sc->red2 = v2[1];
-      sc->counter = 0u;
-    }
-  }
 }
 
 // true_residual, second pass (src/fpm_stairstep_mpi_3d.f90:319-328): project
@@ -295,14 +266,6 @@ __global__ __launch_bounds__(256) void k_true_residual_2(size_t n, double *__res
   }
   block_sum<1>(s, lds);
   store_partials(partials, 1, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[1];
-    grid_finish_sum<1>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->rz = v[0];
-      sc->counter = 0u;
-    }
-  }
 }
 
 // PCG SpMV + dot (src/fpm_stairstep_mpi_3d.f90:246-265): q = A p with
@@ -348,18 +311,6 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Tiling t, Geom g, const double
   }
   block_sum<1>(s, lds);
   store_partials(partials, 1, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[1];
-    grid_finish_sum<1>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const double pq = v[0], rz = sc->rz;
-      sc->pq = pq;
-      // The negated tests are also true for NaN.
-      if (!(pq > 0) || !(rz > 0)) sc->status = kBreakdown;
-      else sc->alpha = rz / pq;
-      sc->counter = 0u;
-    }
-  }
 }
 
 // PCG update (src/fpm_stairstep_mpi_3d.f90:267-278): phi += alpha p,
@@ -383,22 +334,6 @@ __global__ __launch_bounds__(256) void k_update_dot(size_t n, double *__restrict
   }
   block_sum<1>(s, lds);
   store_partials(partials, 1, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[1];
-    grid_finish_sum<1>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const double rz_new = v[0];
-      sc->rz_new = rz_new;
-      if (!(rz_new >= 0)) {
-        sc->status = kNegative;
-      } else {
-        sc->beta = rz_new / sc->rz;
-        sc->rz = rz_new;
-        sc->iters += 1;
-      }
-      sc->counter = 0u;
-    }
-  }
 }
 
 // PCG direction (src/fpm_stairstep_mpi_3d.f90:279-283): p = M^-1 r + beta p.
@@ -411,6 +346,57 @@ __global__ __launch_bounds__(256) void k_direction(size_t n, double *__restrict_
   for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
        c += (size_t)gridDim.x * blockDim.x)
     p[c] = rdiag[c] * r[c] + beta * p[c];
+}
+
+// One workgroup of 256 threads: totals the partials of the reduction kernel
+// launched just before on the same stream and applies the PCG scalar logic
+// (src/fpm_stairstep_mpi_3d.f90:257-278) on the device.
+__global__ __launch_bounds__(256) void k_finish(int op, int nslots, const double *partials,
+                                                int nblocks, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if ((op == FIN_PQ || op == FIN_RZNEW) && sc->status != 0) return;
+  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  if (op == FIN_MAX) {
+    const double m = grid_finish_max(partials, 0, nblocks, lds);
+    if (lead) sc->res = m;
+  } else if (op == FIN_TR1) {
+    const double m = grid_finish_max(partials, 2, nblocks, lds);
+    double v[2];
+    grid_finish_sum<2>(partials, nblocks, v, lds + 16);
+    if (lead) {
+      sc->res = m;
+      sc->red1 = v[0];
+      sc->red2 = v[1];
+    }
+  } else if (op == FIN_SUMS) {
+    double v[kReduceSlots];
+    grid_finish_sum<kReduceSlots>(partials, nblocks, v, lds);
+    if (lead)
+      for (int s = 0; s < nslots && s < kReduceSlots; ++s) sc->sums[s] = v[s];
+  } else {
+    double v[1];
+    grid_finish_sum<1>(partials, nblocks, v, lds);
+    if (!lead) return;
+    if (op == FIN_RZ) {
+      sc->rz = v[0];
+    } else if (op == FIN_PQ) {
+      const double pq = v[0], rz = sc->rz;
+      sc->pq = pq;
+      // The negated tests are also true for NaN.
+      if (!(pq > 0) || !(rz > 0)) sc->status = kBreakdown;
+      else sc->alpha = rz / pq;
+    } else if (op == FIN_RZNEW) {
+      const double rz_new = v[0];
+      sc->rz_new = rz_new;
+      if (!(rz_new >= 0)) {
+        sc->status = kNegative;
+      } else {
+        sc->beta = rz_new / sc->rz;
+        sc->rz = rz_new;
+        sc->iters += 1;
+      }
+    }
+  }
 }
 
 __global__ void k_chunk_begin(Scalars *sc) {
@@ -443,15 +429,6 @@ __global__ __launch_bounds__(256) void k_gauge_sums(Geom g, const int32_t *__res
   }
   block_sum<2>(s, lds);
   store_partials(partials, 2, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[2];
-    grid_finish_sum<2>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->sums[0] = v[0];
-      sc->sums[1] = v[1];
-      sc->counter = 0u;
-    }
-  }
 }
 
 // Gauge shift (src/fpm_stairstep_mpi_3d.f90:180-185) on the dense field, with
@@ -519,16 +496,6 @@ __global__ __launch_bounds__(256) void k_compat_sums(
   }
   block_sum<3>(s, lds);
   store_partials(partials, 3, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double v[3];
-    grid_finish_sum<3>(partials, gridDim.x, v, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->sums[0] = v[0];
-      sc->sums[1] = v[1];
-      sc->sums[2] = v[2];
-      sc->counter = 0u;
-    }
-  }
 }
 
 // Uniform correction of the outer-face data (src/fpm_stairstep_mpi_3d.f90:58-62).
@@ -615,13 +582,6 @@ __global__ __launch_bounds__(256) void k_face_residual(
   }
   block_max(m, lds);
   store_partials(partials, 1, &m, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    const double r = grid_finish_max(partials, 0, gridDim.x, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->res = r;
-      sc->counter = 0u;
-    }
-  }
 }
 
 // compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469):
@@ -673,15 +633,164 @@ __global__ __launch_bounds__(256) void k_terms(
   }
   block_sum<2>(s, lds);
   store_partials(partials, 2, s, gridDim.x);
-  if (arrive_last(&sc->counter, gridDim.x)) {
-    double vv[2];
-    grid_finish_sum<2>(partials, gridDim.x, vv, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      sc->sums[0] = vv[0];
-      sc->sums[1] = vv[1];
-      sc->counter = 0u;
+}
+
+}  // namespace ffpm
+
+namespace ffpm {
+
+// ---------------------------------------------------------------------------
+// Fused PCG iteration: two kernels and 96 B/cell instead of three kernels and
+// 112 B/cell, with the reference's formulas unchanged
+// (src/fpm_stairstep_mpi_3d.f90:246-283):
+//   k_pcg_dir_spmv: p = M^-1 r + beta p (the previous iteration's direction
+//     update, deferred), q = A p and pq = p.q, breakdown test, alpha.
+//     The x-y neighbours of p are exchanged through an LDS tile; the ring of
+//     the tile recomputes p from r, rdiag and the old p of the neighbour tile
+//     (read through the cache), so p ping-pongs between two buffers. q is not
+//     stored.
+//   k_pcg_update: recomputes q = A p (bit-identical: same expression, same
+//     inputs), phi += alpha p, r -= alpha q, rz_new = r.M^-1 r, beta.
+// mode of the first kernel: kDirUpdate (p = M^-1 r + beta p_old),
+// kDirReset (p = M^-1 r, after a true-residual reset) or kDirCurrent (p_old is
+// already the direction: it is read, not rewritten).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double dir_value(int mode, double beta, const double *__restrict__ r,
+                                            const double *__restrict__ rdiag,
+                                            const double *__restrict__ pin, size_t c) {
+  if (mode == kDirCurrent) return pin[c];
+  const double z = rdiag[c] * r[c];
+  if (mode == kDirReset) return z;
+  return z + beta * pin[c];
+}
+
+__global__ __launch_bounds__(256) void k_pcg_dir_spmv(Tiling t, Geom g,
+                                                      const double *__restrict__ r,
+                                                      const double *__restrict__ rdiag,
+                                                      const double *__restrict__ pin,
+                                                      double *__restrict__ pout,
+                                                      const double *__restrict__ diag, int mode,
+                                                      double *partials, Scalars *sc) {
+  __shared__ double tile[kTileY + 2][kTileX + 2];
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double beta = sc->beta;
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int lin = tx + kTileX * ty;
+  const int i = tc.i, j = tc.j;
+  const int i0 = i - tx, j0 = j - ty;
+  const bool active = tc.valid && i < nx && j < ny;
+  const size_t nxy = (size_t)nx * ny;
+  // halo cell owned by this thread (lin < 136), in tile coordinates
+  int hi = -1, hj = -1, hx = 0, hy = 0;
+  if (lin < kTileX) {
+    hx = i0 + lin; hy = j0 - 1; hi = lin + 1; hj = 0;
+  } else if (lin < 2 * kTileX) {
+    hx = i0 + lin - kTileX; hy = j0 + kTileY; hi = lin - kTileX + 1; hj = kTileY + 1;
+  } else if (lin < 2 * kTileX + kTileY) {
+    hx = i0 - 1; hy = j0 + lin - 2 * kTileX; hi = 0; hj = lin - 2 * kTileX + 1;
+  } else if (lin < 2 * kTileX + 2 * kTileY) {
+    hx = i0 + kTileX; hy = j0 + lin - 2 * kTileX - kTileY; hi = kTileX + 1;
+    hj = lin - 2 * kTileX - kTileY + 1;
+  }
+  const bool halo_in = hi >= 0 && hx >= 0 && hx < nx && hy >= 0 && hy < ny;
+  double cw = 0, ce = 0, cs = 0, cn = 0, dxdy = 0;
+  size_t c = 0;
+  double p_prev = 0, p_cur = 0;
+  if (active) {
+    cw = g.cw[i]; ce = g.ce[i]; cs = g.cs[j]; cn = g.cn[j];
+    dxdy = g.dx[i] * g.dy[j];
+    c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    if (tc.k0 < nz) {
+      p_cur = dir_value(mode, beta, r, rdiag, pin, c);
+      p_prev = tc.k0 > 0 ? dir_value(mode, beta, r, rdiag, pin, c - nxy) : p_cur;
     }
   }
+  size_t hc = halo_in ? (size_t)hx + (size_t)nx * ((size_t)hy + (size_t)ny * tc.k0) : 0;
+  double s = 0.0;
+  for (int k = tc.k0; k < tc.k1; ++k) {
+    double p_next = p_cur;
+    if (active && k + 1 < nz) p_next = dir_value(mode, beta, r, rdiag, pin, c + nxy);
+    tile[ty + 1][tx + 1] = active ? p_cur : 0.0;
+    if (hi >= 0) tile[hj][hi] = halo_in ? dir_value(mode, beta, r, rdiag, pin, hc) : 0.0;
+    __syncthreads();
+    if (active) {
+      const double d = diag[c];
+      double qv = 0.0;
+      if (d > 0) {
+        qv = dxdy * g.dz[k] *
+             (d * p_cur - cw * tile[ty + 1][tx] - ce * tile[ty + 1][tx + 2] -
+              cs * tile[ty][tx + 1] - cn * tile[ty + 2][tx + 1] - g.cb[k] * p_prev -
+              g.ct[k] * p_next);
+      }
+      s += p_cur * qv;
+      if (mode != kDirCurrent) pout[c] = p_cur;
+      c += nxy;
+    }
+    hc += nxy;
+    p_prev = p_cur;
+    p_cur = p_next;
+    __syncthreads();
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, gridDim.x);
 }
+
+__global__ __launch_bounds__(256) void k_pcg_update(Tiling t, Geom g, const double *__restrict__ p,
+                                                    const double *__restrict__ diag,
+                                                    double *__restrict__ phi,
+                                                    double *__restrict__ r,
+                                                    const double *__restrict__ rdiag,
+                                                    double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double alpha = sc->alpha;
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double s = 0.0;
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const double dxdy = g.dx[i] * g.dy[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    double p_cur = p[c];
+    double p_prev = tc.k0 > 0 ? p[c - nxy] : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const double p_next = (k + 1 < nz) ? p[c + nxy] : p_cur;
+      const double d = diag[c];
+      double qv = 0.0;
+      if (d > 0) {
+        const double pw = hw ? p[c - 1] : 0.0;
+        const double pe = he ? p[c + 1] : 0.0;
+        const double ps = hs ? p[c - nx] : 0.0;
+        const double pn = hn ? p[c + nx] : 0.0;
+        qv = dxdy * g.dz[k] *
+             (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
+              g.ct[k] * p_next);
+      }
+      phi[c] = phi[c] + alpha * p_cur;
+      const double rv = r[c] - alpha * qv;
+      r[c] = rv;
+      s += rdiag[c] * rv * rv;
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, gridDim.x);
+}
+
+// Residual-only pass of true_residual without the direction reset
+// (src/fpm_stairstep_mpi_3d.f90:319-328 with reset false) is k_true_residual_2
+// with reset = 0; the deferred direction update before a convergence check is
+// k_direction.
 
 }  // namespace ffpm

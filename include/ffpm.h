@@ -247,9 +247,12 @@ int ffpm_compute_fpm_terms_nonuniform_3d_dev(
  * ------------------------------------------------------------------------ */
 int ffpm_timing_enable(int on);
 int ffpm_timing_reset(void);
-/* Kernel names: "jacobi", "spmv_dot", "update_dot", "direction",
+/* Kernel names: "pcg_dir_spmv", "pcg_update", "jacobi", "spmv_dot", "update_dot", "direction",
  * "true_residual", "residual_max", "terms", "lap_sweep". count launches, total ms. */
 int ffpm_timing_query(const char *kernel, long *count, double *total_ms);
+/* Bandwidth of a plain device copy (read + write) of the given size, averaged
+ * over reps launches; the achievable streaming figure on this device. */
+int ffpm_copy_bandwidth(size_t bytes, int reps, double *gbps, double *avg_us);
 
 #ifdef __cplusplus
 }

@@ -276,3 +276,35 @@ def test_max_sweeps_zero_and_check_interval_zero(orc):
         po, so, ro = _orc_solve(orc, c, faces, -1.0, 13, 0, method)
         assert sg == so == 13
         assert np.abs(pg - po).max() <= 1e-12 * np.abs(po).max()
+
+
+def test_fused_and_three_kernel_pcg_agree(orc, monkeypatch):
+    """The fused two-kernel iteration and the three-kernel iteration run the
+    same formulas; at fixed work they agree to summation-order roundoff, and
+    both match the oracle's converged answer and iteration count."""
+    p = W.frame_128x96x64(with_fields=False)
+    rhs, bg, faces = W.influence_inputs(p, 1)
+    orc.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
+    out = {}
+    for variant in ("classic", "fused"):
+        print(f"case: 128x96x64 direction y, {variant} PCG", flush=True)
+        monkeypatch.setenv("FFPM_PCG", variant)
+        ph = p.phi0()
+        sw, res = S.solve_stairstep_poisson_3d(ph, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc,
+                                               p.dx, p.dy, p.dz, -1.0, 250, 50, method="pcg")
+        out[variant] = (ph, sw, res)
+    a, b = out["classic"], out["fused"]
+    assert a[1] == b[1] == 250
+    assert np.abs(a[0] - b[0]).max() <= 1e-12 * np.abs(a[0]).max()
+    assert abs(a[2] - b[2]) <= 1e-9 * a[2]
+    po = p.phi0()
+    so, ro = orc.solve_stairstep_poisson_3d(po, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc,
+                                            p.dx, p.dy, p.dz, 1e-10, 100000, 100, "pcg")
+    for variant in ("classic", "fused"):
+        monkeypatch.setenv("FFPM_PCG", variant)
+        ph = p.phi0()
+        sw, res = S.solve_stairstep_poisson_3d(ph, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc,
+                                               p.dx, p.dy, p.dz, 1e-10, 100000, 100,
+                                               method="pcg")
+        assert sw == so and res <= 1e-10
+        assert np.abs(ph - po).max() <= 1e-10 * np.abs(po).max()
