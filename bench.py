@@ -35,13 +35,14 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell and launch (FP64 fields; 1-D coefficient
 # arrays are negligible).
 BYTES_PER_CELL = {
-    "pcg_dir_spmv": 40,  # read r, rdiag, p_old, diag; write p
-    "pcg_update": 56,    # read p, diag, phi, r, rdiag; write phi, r
+    "pcg_dir_spmv": 25,  # read z, p_old, neighbour code (1 B); write p
+    "pcg_update": 49,    # read p, code (1 B), phi, r; write phi, r, z
     "spmv_dot": 24,      # read p, diag; write q
     "update_dot": 56,    # read phi, p, r, q, rdiag; write phi, r
     "direction": 32,     # read rdiag, r, p; write p
     "jacobi": 32,        # read a, src, rdiag; write b
-    "true_residual": 72,  # pass 1: phi, src, diag, r(w); pass 2: r(rw), rdiag, diag, p(w)
+    "true_residual_1": 32,  # phi, src, diag; write r
+    "true_residual_2": 32,  # r (read, write), rdiag, diag
     "residual_max": 24,  # phi, src, diag
     "terms": 124,        # phi, solid(4), 9 fields; write 5 outputs
 }

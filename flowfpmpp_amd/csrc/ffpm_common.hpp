@@ -49,17 +49,21 @@ constexpr int kBlock = kTileX * kTileY;
 constexpr int kXcds = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kReduceSlots = 4;
 
-// z-streaming tiles: (tiles_x, tiles_y, z chunks) flattened x-fastest, then
-// dealt to XCDs in contiguous ranges so that y-neighbour tiles (whose halo
-// rows a tile reads) share an L2.
+// z-streaming tiles: (tiles_x, tiles_y, z chunks) flattened x-fastest. A
+// workgroup is 64 x ty threads (one wavefront per row of the tile). With
+// xcd_map the tiles are dealt to the XCDs in contiguous ranges, so that
+// y-neighbour tiles (whose halo rows a tile reads) share an L2.
 struct Tiling {
-  int tiles_x, tiles_y, chunks_z, zchunk;
+  int tiles_x, tiles_y, chunks_z, zchunk, ty;
   int total;      // tiles
   int per_xcd;    // ceil(total / 8)
-  int blocks;     // launched workgroups = 8 * per_xcd
+  int blocks;     // launched workgroups
+  int xcd_map;
 };
 
-Tiling make_tiling(int nx, int ny, int nz, int zchunk);
+// cells_x: cells one wavefront covers along x (64, or 128 for the paired-cell
+// kernels).
+Tiling make_tiling(int nx, int ny, int nz, int zchunk, int ty = 0, int cells_x = kTileX);
 
 struct TileCoord {
   int i, j, k0, k1;
@@ -69,14 +73,14 @@ struct TileCoord {
 __device__ inline TileCoord tile_coord(const Tiling &t, int nx, int ny, int nz) {
   TileCoord c;
   const int L = blockIdx.x;
-  const int tile = (L % kXcds) * t.per_xcd + L / kXcds;
+  const int tile = t.xcd_map ? (L % kXcds) * t.per_xcd + L / kXcds : L;
   c.valid = tile < t.total;
   const int tx = tile % t.tiles_x;
   const int rest = tile / t.tiles_x;
   const int ty = rest % t.tiles_y;
   const int tz = rest / t.tiles_y;
   c.i = tx * kTileX + threadIdx.x;
-  c.j = ty * kTileY + threadIdx.y;
+  c.j = ty * t.ty + threadIdx.y;
   c.k0 = tz * t.zchunk;
   c.k1 = min(nz, c.k0 + t.zchunk);
   return c;

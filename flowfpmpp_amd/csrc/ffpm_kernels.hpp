@@ -8,6 +8,12 @@ namespace ffpm {
 constexpr int kBreakdown = 1;  // pq <= 0 or rz <= 0 (or NaN): restart from the true residual
 constexpr int kNegative = 2;   // rz_new < 0 or NaN: leave the iteration
 
+// Neighbour code of a cell (one byte): which of its six faces lead to a fluid
+// neighbour (and so contribute a coefficient to diag), and whether the cell
+// itself is fluid.
+constexpr unsigned kCodeW = 1u, kCodeE = 2u, kCodeS = 4u, kCodeN = 8u, kCodeB = 16u,
+                   kCodeT = 32u, kCodeFluid = 64u;
+
 // direction modes of the fused PCG SpMV
 constexpr int kDirUpdate = 0;   // p = M^-1 r + beta p_old
 constexpr int kDirReset = 1;    // p = M^-1 r
@@ -47,7 +53,7 @@ __global__ void k_build_operator(Geom g, const int32_t *solid, const double *rhs
                                  const double *xlo, const double *xhi, const double *ylo,
                                  const double *yhi, const double *zlo, const double *zhi,
                                  double bg0, double bg1, double bg2, int for_pcg, double *src,
-                                 double *diag, double *rdiag, int *isolated);
+                                 double *diag, double *rdiag, uint8_t *code, int *isolated);
 __global__ void k_pack_phi(int nx, int ny, int nz, const int32_t *solid, const double *phi_ref,
                            double *phi);
 __global__ void k_unpack_phi(int nx, int ny, int nz, const double *phi, double *phi_ref);
@@ -58,18 +64,31 @@ __global__ void k_residual_max(Tiling t, Geom g, const double *phi, const double
 __global__ void k_true_residual_1(Tiling t, Geom g, const double *phi, const double *src,
                                   const double *diag, double *r, double *partials, Scalars *sc);
 __global__ void k_true_residual_2(size_t n, double *r, const double *rdiag, const double *diag,
-                                  double *p, int reset, double *partials, Scalars *sc);
+                                  double *p, int reset, double *z, double *partials, Scalars *sc);
 __global__ void k_spmv_dot(Tiling t, Geom g, const double *p, double *q, const double *diag,
                            double *partials, Scalars *sc);
 __global__ void k_update_dot(size_t n, double *phi, const double *p, double *r, const double *q,
                              const double *rdiag, double *partials, Scalars *sc);
 __global__ void k_direction(size_t n, double *p, const double *r, const double *rdiag,
                             const Scalars *sc);
-__global__ void k_pcg_dir_spmv(Tiling t, Geom g, const double *r, const double *rdiag,
-                               const double *pin, double *pout, const double *diag, int mode,
-                               double *partials, Scalars *sc);
-__global__ void k_pcg_update(Tiling t, Geom g, const double *p, const double *diag, double *phi,
-                             double *r, const double *rdiag, double *partials, Scalars *sc);
+__global__ void k_pcg_dir_spmv(Tiling t, Geom g, const double *z, const uint8_t *code,
+                               const double *pin, double *pout, int mode, double *partials,
+                               Scalars *sc);
+__global__ void k_pcg_update(Tiling t, Geom g, const double *p, const uint8_t *code, double *phi,
+                             double *r, double *z, double *partials, Scalars *sc);
+__global__ void k_direction_z(size_t n, double *p, const double *z, const Scalars *sc);
+__global__ void k_pcg_dir_spmv_x2(Tiling t, Geom g, const double *z, const uint8_t *code,
+                                  const double *pin, double *pout, int mode, double *partials,
+                                  Scalars *sc);
+__global__ void k_pcg_update_x2(Tiling t, Geom g, const double *p, const uint8_t *code,
+                                double *phi, double *r, double *z, double *partials, Scalars *sc);
+template <int U>
+__global__ void k_pcg_dir_spmv_u(Tiling t, Geom g, const double *z, const uint8_t *code,
+                                 const double *pin, double *pout, int mode, double *partials,
+                                 Scalars *sc);
+template <int U>
+__global__ void k_pcg_update_u(Tiling t, Geom g, const double *p, const uint8_t *code,
+                               double *phi, double *r, double *z, double *partials, Scalars *sc);
 __global__ void k_finish(int op, int nslots, const double *partials, int nblocks, Scalars *sc);
 __global__ void k_chunk_begin(Scalars *sc);
 __global__ void k_clear_status(Scalars *sc);

@@ -2,6 +2,7 @@
 #include "ffpm_runtime.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -29,10 +30,25 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 
 hipStream_t stream() { return g_stream; }
 
-Tiling make_tiling(int nx, int ny, int nz, int zchunk) {
+namespace {
+int env_int(const char *name, int dflt) {
+  const char *e = std::getenv(name);
+  return e && *e ? std::atoi(e) : dflt;
+}
+}  // namespace
+
+Tiling make_tiling(int nx, int ny, int nz, int zchunk, int ty, int cells_x) {
   Tiling t;
-  t.tiles_x = (nx + kTileX - 1) / kTileX;
-  t.tiles_y = (ny + kTileY - 1) / kTileY;
+  if (ty <= 0) ty = env_int("FFPM_TILE_Y", 4);
+  if (ty != 1 && ty != 2 && ty != 4 && ty != 8) ty = 4;
+  t.ty = ty;
+  t.tiles_x = (nx + cells_x - 1) / cells_x;
+  t.tiles_y = (ny + ty - 1) / ty;
+  if (zchunk <= 0) zchunk = env_int("FFPM_ZCHUNK", 0);
+  // 16-plane chunks measured best for the paired-cell PCG kernels on a
+  // 512x384x256 frame (tile heights 1-8, chunks 8-64 and both tile orders
+  // within 10%); smaller grids fall back to the occupancy rule below.
+  if (zchunk <= 0 && (long)t.tiles_x * t.tiles_y * ((nz + 15) / 16) >= 2048) zchunk = 16;
   if (zchunk <= 0) {
     // Enough workgroups to fill 256 CUs several times over, and z chunks of
     // at least 4 planes so the z halo re-read stays small.
@@ -47,8 +63,9 @@ Tiling make_tiling(int nx, int ny, int nz, int zchunk) {
   t.zchunk = zchunk;
   t.chunks_z = (nz + zchunk - 1) / zchunk;
   t.total = t.tiles_x * t.tiles_y * t.chunks_z;
+  t.xcd_map = env_int("FFPM_XCD_MAP", 0) != 0;
   t.per_xcd = (t.total + kXcds - 1) / kXcds;
-  t.blocks = t.per_xcd * kXcds;
+  t.blocks = t.xcd_map ? t.per_xcd * kXcds : t.total;
   return t;
 }
 

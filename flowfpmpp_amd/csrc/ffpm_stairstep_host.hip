@@ -46,8 +46,11 @@ struct Frame {
   size_t n = 0;
   Geom g{};
   Tiling tiling{};
+  Tiling tiling2{};  // paired-cell kernels (even nx)
+  bool paired = false;
   double *phi = nullptr, *next = nullptr, *src = nullptr, *diag = nullptr, *rdiag = nullptr;
-  double *r = nullptr, *q = nullptr, *partials = nullptr;
+  double *r = nullptr, *q = nullptr, *z = nullptr, *partials = nullptr;
+  uint8_t *code = nullptr;
   Scalars *sc = nullptr;
   int *isolated = nullptr;
 };
@@ -69,7 +72,13 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   f.nz = nz;
   f.n = (size_t)nx * ny * nz;
   f.tiling = make_tiling(nx, ny, nz, 0);
-  const size_t nb = (size_t)max_blocks_for(f.tiling, f.n);
+  f.tiling2 = make_tiling(nx, ny, nz, 0, 0, 2 * kTileX);
+  {
+    const char *e = std::getenv("FFPM_X2");
+    f.paired = nx % 2 == 0 && !(e && std::strcmp(e, "0") == 0);
+  }
+  size_t nb = (size_t)max_blocks_for(f.tiling, f.n);
+  if ((size_t)f.tiling2.blocks > nb) nb = (size_t)f.tiling2.blocks;
   double *coef = (double *)ws.get(S_COEF, sizeof(double) * 2 * (nx + ny + nz), &st);
   if (!coef) return st;
   f.partials = (double *)ws.get(S_PARTIALS, sizeof(double) * kReduceSlots * nb, &st);
@@ -108,9 +117,11 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
     if (!(f.src = (double *)ws.get(S_SRC, bytes, &st))) return st;
     if (!(f.diag = (double *)ws.get(S_DIAG, bytes, &st))) return st;
     if (!(f.rdiag = (double *)ws.get(S_RDIAG, bytes, &st))) return st;
+    if (!(f.code = (uint8_t *)ws.get(S_CODE, f.n, &st))) return st;
     if (pcg_fields) {
       if (!(f.r = (double *)ws.get(S_R, bytes, &st))) return st;
       if (!(f.q = (double *)ws.get(S_Q, bytes, &st))) return st;
+      if (!(f.z = (double *)ws.get(S_Z, bytes, &st))) return st;
     }
   }
   return OK;
@@ -135,16 +146,16 @@ int read_scalars(const Frame &f, Scalars *h) {
 int true_residual(Frame &f, bool reset, double *res) {
   const int eb = elementwise_blocks(f.n);
   {
-    TimedLaunch tl("true_residual");
-    k_true_residual_1<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
+    TimedLaunch tl("true_residual_1");
+    k_true_residual_1<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
         f.tiling, f.g, f.phi, f.src, f.diag, f.r, f.partials, f.sc);
     FFPM_LAUNCHED("k_true_residual_1");
   }
   finish(f, FIN_TR1, 3, f.tiling.blocks);
   {
-    TimedLaunch tl("true_residual");
+    TimedLaunch tl("true_residual_2");
     k_true_residual_2<<<eb, kThreads, 0, stream()>>>(f.n, f.r, f.rdiag, f.diag, f.next,
-                                                     reset ? 1 : 0, f.partials, f.sc);
+                                                     reset ? 1 : 0, f.z, f.partials, f.sc);
     FFPM_LAUNCHED("k_true_residual_2");
   }
   finish(f, FIN_RZ, 1, eb);
@@ -158,7 +169,7 @@ int true_residual(Frame &f, bool reset, double *res) {
 int residual_max(Frame &f, const double *phi, double *res) {
   {
     TimedLaunch tl("residual_max");
-    k_residual_max<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
+    k_residual_max<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
         f.tiling, f.g, phi, f.src, f.diag, f.partials, f.sc);
     FFPM_LAUNCHED("k_residual_max");
   }
@@ -191,7 +202,7 @@ int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, 
     for (int m = 0; m < n; ++m) {
       {
         TimedLaunch tl("spmv_dot");
-        k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(
+        k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
             f.tiling, f.g, f.next, f.q, f.diag, f.partials, f.sc);
       }
       finish(f, FIN_PQ, 1, f.tiling.blocks);
@@ -257,7 +268,9 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
   int mode = kDirReset;
   int it = 0, restarts = 0, stalled = 0;
   double best = res;
-  const dim3 blk(kTileX, kTileY);
+  const dim3 blk(kTileX, f.tiling.ty);
+  const char *kv = std::getenv("FFPM_UNROLL");
+  const int unroll = kv ? std::atoi(kv) : 1;
   while (*sweeps < max_sweeps && res > tolerance) {
     int n = ci - it % ci;
     if (max_sweeps - *sweeps < n) n = max_sweeps - *sweeps;
@@ -267,17 +280,37 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
       const int dst = cur ^ 1;
       {
         TimedLaunch tl("pcg_dir_spmv");
-        k_pcg_dir_spmv<<<f.tiling.blocks, blk, 0, stream()>>>(
-            f.tiling, f.g, f.r, f.rdiag, pbuf[cur], pbuf[dst], f.diag, mode, f.partials, f.sc);
+        if (f.paired)
+          k_pcg_dir_spmv_x2<<<f.tiling2.blocks, blk, 0, stream()>>>(
+              f.tiling2, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
+        else if (unroll == 4)
+          k_pcg_dir_spmv_u<4><<<f.tiling.blocks, blk, 0, stream()>>>(
+              f.tiling, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
+        else if (unroll == 2)
+          k_pcg_dir_spmv_u<2><<<f.tiling.blocks, blk, 0, stream()>>>(
+              f.tiling, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
+        else
+          k_pcg_dir_spmv<<<f.tiling.blocks, blk, 0, stream()>>>(
+              f.tiling, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
       }
-      finish(f, FIN_PQ, 1, f.tiling.blocks);
+      finish(f, FIN_PQ, 1, f.paired ? f.tiling2.blocks : f.tiling.blocks);
       if (mode != kDirCurrent) cur = dst;
       {
         TimedLaunch tl("pcg_update");
-        k_pcg_update<<<f.tiling.blocks, blk, 0, stream()>>>(f.tiling, f.g, pbuf[cur], f.diag,
-                                                            f.phi, f.r, f.rdiag, f.partials, f.sc);
+        if (f.paired)
+          k_pcg_update_x2<<<f.tiling2.blocks, blk, 0, stream()>>>(
+              f.tiling2, f.g, pbuf[cur], f.code, f.phi, f.r, f.z, f.partials, f.sc);
+        else if (unroll == 4)
+          k_pcg_update_u<4><<<f.tiling.blocks, blk, 0, stream()>>>(
+              f.tiling, f.g, pbuf[cur], f.code, f.phi, f.r, f.z, f.partials, f.sc);
+        else if (unroll == 2)
+          k_pcg_update_u<2><<<f.tiling.blocks, blk, 0, stream()>>>(
+              f.tiling, f.g, pbuf[cur], f.code, f.phi, f.r, f.z, f.partials, f.sc);
+        else
+          k_pcg_update<<<f.tiling.blocks, blk, 0, stream()>>>(f.tiling, f.g, pbuf[cur], f.code,
+                                                              f.phi, f.r, f.z, f.partials, f.sc);
       }
-      finish(f, FIN_RZNEW, 1, f.tiling.blocks);
+      finish(f, FIN_RZNEW, 1, f.paired ? f.tiling2.blocks : f.tiling.blocks);
       mode = kDirUpdate;
     }
     FFPM_LAUNCHED("fused pcg chunk");
@@ -299,9 +332,9 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
     if (it % ci == 0 || *sweeps == max_sweeps) {
       {
         TimedLaunch tl("direction");
-        k_direction<<<eb, kThreads, 0, stream()>>>(f.n, pbuf[cur], f.r, f.rdiag, f.sc);
+        k_direction_z<<<eb, kThreads, 0, stream()>>>(f.n, pbuf[cur], f.z, f.sc);
       }
-      FFPM_LAUNCHED("k_direction");
+      FFPM_LAUNCHED("k_direction_z");
       mode = kDirCurrent;
       if ((st = true_residual(f, false, &res))) return st;
       if (res < 0.99 * best) {
@@ -333,7 +366,7 @@ int jacobi_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     for (int s = 0; s < nsteps; ++s) {
       {
         TimedLaunch tl("jacobi");
-        k_jacobi<<<f.tiling.blocks, dim3(kTileX, kTileY), 0, stream()>>>(f.tiling, f.g, cur, oth,
+        k_jacobi<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(f.tiling, f.g, cur, oth,
                                                                          f.src, f.rdiag);
       }
       double *t = cur;
@@ -385,7 +418,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   FFPM_HIP(hipMemsetAsync(f.isolated, 0, sizeof(int), stream()));
   k_build_operator<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
                                                   bg[0], bg[1], bg[2], pcg ? 1 : 0, f.src, f.diag,
-                                                  f.rdiag, f.isolated);
+                                                  f.rdiag, f.code, f.isolated);
   FFPM_LAUNCHED("k_build_operator");
   int isolated = 0;
   FFPM_HIP(hipMemcpyAsync(&isolated, f.isolated, sizeof(int), hipMemcpyDeviceToHost, stream()));

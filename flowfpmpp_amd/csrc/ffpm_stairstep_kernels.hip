@@ -43,13 +43,13 @@ __global__ void k_axis_coefficients(int nx, int ny, int nz, const double *xc, co
 // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379): folds the outer and
 // body face gradients into src, sums fluid-neighbour coefficients into diag,
 // and sets rdiag = 1/(V diag) (PCG) or 0.8/diag (Jacobi); 0 in solids.
-__global__ __launch_bounds__(256) void k_build_operator(
+__global__ __launch_bounds__(512) void k_build_operator(
     Geom g, const int32_t *__restrict__ solid, const double *__restrict__ rhs,
     const double *__restrict__ xlo, const double *__restrict__ xhi,
     const double *__restrict__ ylo, const double *__restrict__ yhi,
     const double *__restrict__ zlo, const double *__restrict__ zhi, double bg0, double bg1,
     double bg2, int for_pcg, double *__restrict__ src, double *__restrict__ diag,
-    double *__restrict__ rdiag, int *__restrict__ isolated) {
+    double *__restrict__ rdiag, uint8_t *__restrict__ code, int *__restrict__ isolated) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)nx * ny * nz;
   int local_isolated = 0;
@@ -60,31 +60,34 @@ __global__ __launch_bounds__(256) void k_build_operator(
     const int k = (int)(c / ((size_t)nx * ny));
     const int I = i + 1, J = j + 1;
     double s = 0, d = 0;
+    unsigned cb = 0;
     if (solid[ph(I, J, k, nx, ny)] == 0) {
       const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
+      cb = kCodeFluid;
       s = -rhs[c];
       if (I == 1) s = s - xlo[j + (size_t)ny * k] / dxi;
       else if (solid[ph(I - 1, J, k, nx, ny)] != 0) s = s - bg0 / dxi;
-      else d = d + g.cw[i];
+      else { d = d + g.cw[i]; cb |= kCodeW; }
       if (I == nx) s = s + xhi[j + (size_t)ny * k] / dxi;
       else if (solid[ph(I + 1, J, k, nx, ny)] != 0) s = s + bg0 / dxi;
-      else d = d + g.ce[i];
+      else { d = d + g.ce[i]; cb |= kCodeE; }
       if (J == 1) s = s - ylo[i + (size_t)nx * k] / dyj;
       else if (solid[ph(I, J - 1, k, nx, ny)] != 0) s = s - bg1 / dyj;
-      else d = d + g.cs[j];
+      else { d = d + g.cs[j]; cb |= kCodeS; }
       if (J == ny) s = s + yhi[i + (size_t)nx * k] / dyj;
       else if (solid[ph(I, J + 1, k, nx, ny)] != 0) s = s + bg1 / dyj;
-      else d = d + g.cn[j];
+      else { d = d + g.cn[j]; cb |= kCodeN; }
       if (k == 0) s = s - zlo[i + (size_t)nx * j] / dzk;
       else if (solid[ph(I, J, k - 1, nx, ny)] != 0) s = s - bg2 / dzk;
-      else d = d + g.cb[k];
+      else { d = d + g.cb[k]; cb |= kCodeB; }
       if (k == nz - 1) s = s + zhi[i + (size_t)nx * j] / dzk;
       else if (solid[ph(I, J, k + 1, nx, ny)] != 0) s = s + bg2 / dzk;
-      else d = d + g.ct[k];
+      else { d = d + g.ct[k]; cb |= kCodeT; }
       if (d <= 0) local_isolated += 1;
     }
     src[c] = s;
     diag[c] = d;
+    code[c] = (uint8_t)cb;
     double rd;
     if (d <= 0) rd = 0;
     else if (for_pcg) rd = 1 / (g.dx[i] * g.dy[j] * g.dz[k] * d);
@@ -124,7 +127,7 @@ __global__ void k_unpack_phi(int nx, int ny, int nz, const double *__restrict__ 
 }
 
 // jacobi_sweep (src/fpm_stairstep_mpi_3d.f90:384-399), z-streaming.
-__global__ __launch_bounds__(256) void k_jacobi(Tiling t, Geom g, const double *__restrict__ a,
+__global__ __launch_bounds__(512) void k_jacobi(Tiling t, Geom g, const double *__restrict__ a,
                                                 double *__restrict__ b,
                                                 const double *__restrict__ src,
                                                 const double *__restrict__ rdiag) {
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(256) void k_jacobi(Tiling t, Geom g, const double *
 
 // operator_residual (src/fpm_stairstep_mpi_3d.f90:514-530): max-norm of
 // src + sum(c phi_nb) - diag phi over diag > 0. Result -> sc->res.
-__global__ __launch_bounds__(256) void k_residual_max(Tiling t, Geom g,
+__global__ __launch_bounds__(512) void k_residual_max(Tiling t, Geom g,
                                                       const double *__restrict__ phi,
                                                       const double *__restrict__ src,
                                                       const double *__restrict__ diag,
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void k_residual_max(Tiling t, Geom g,
 // true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319): the
 // max-norm residual and r = V (src + sum c phi_nb - diag phi) in fluid cells
 // share one evaluation of the bracket; sums of r and of the fluid count.
-__global__ __launch_bounds__(256) void k_true_residual_1(Tiling t, Geom g,
+__global__ __launch_bounds__(512) void k_true_residual_1(Tiling t, Geom g,
                                                          const double *__restrict__ phi,
                                                          const double *__restrict__ src,
                                                          const double *__restrict__ diag,
@@ -245,10 +248,11 @@ __global__ __launch_bounds__(256) void k_true_residual_1(Tiling t, Geom g,
 
 // true_residual, second pass (src/fpm_stairstep_mpi_3d.f90:319-328): project
 // r off the constants, optionally reset p = M^-1 r, and rz = r.M^-1 r.
-__global__ __launch_bounds__(256) void k_true_residual_2(size_t n, double *__restrict__ r,
+__global__ __launch_bounds__(512) void k_true_residual_2(size_t n, double *__restrict__ r,
                                                          const double *__restrict__ rdiag,
                                                          const double *__restrict__ diag,
                                                          double *__restrict__ p, int reset,
+                                                         double *__restrict__ z,
                                                          double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
   const double shift = sc->red1 / sc->red2;
@@ -260,9 +264,10 @@ __global__ __launch_bounds__(256) void k_true_residual_2(size_t n, double *__res
       rv = rv - shift;
       r[c] = rv;
     }
-    const double rd = rdiag[c];
-    if (reset) p[c] = rd * rv;
-    s[0] += rd * rv * rv;
+    const double zv = rdiag[c] * rv;
+    if (reset) p[c] = zv;
+    if (z) z[c] = zv;
+    s[0] += zv * rv;
   }
   block_sum<1>(s, lds);
   store_partials(partials, 1, s, gridDim.x);
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(256) void k_true_residual_2(size_t n, double *__res
 // PCG SpMV + dot (src/fpm_stairstep_mpi_3d.f90:246-265): q = A p with
 // A = V (diag - offdiag), 0 in solids; pq = p.q; then the breakdown test and
 // alpha = rz / pq on the device.
-__global__ __launch_bounds__(256) void k_spmv_dot(Tiling t, Geom g, const double *__restrict__ p,
+__global__ __launch_bounds__(512) void k_spmv_dot(Tiling t, Geom g, const double *__restrict__ p,
                                                   double *__restrict__ q,
                                                   const double *__restrict__ diag,
                                                   double *partials, Scalars *sc) {
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Tiling t, Geom g, const double
 
 // PCG update (src/fpm_stairstep_mpi_3d.f90:267-278): phi += alpha p,
 // r -= alpha q, rz_new = r.M^-1 r; beta on the device; counts the iteration.
-__global__ __launch_bounds__(256) void k_update_dot(size_t n, double *__restrict__ phi,
+__global__ __launch_bounds__(512) void k_update_dot(size_t n, double *__restrict__ phi,
                                                     const double *__restrict__ p,
                                                     double *__restrict__ r,
                                                     const double *__restrict__ q,
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(256) void k_update_dot(size_t n, double *__restrict
 }
 
 // PCG direction (src/fpm_stairstep_mpi_3d.f90:279-283): p = M^-1 r + beta p.
-__global__ __launch_bounds__(256) void k_direction(size_t n, double *__restrict__ p,
+__global__ __launch_bounds__(512) void k_direction(size_t n, double *__restrict__ p,
                                                    const double *__restrict__ r,
                                                    const double *__restrict__ rdiag,
                                                    const Scalars *sc) {
@@ -351,7 +356,7 @@ __global__ __launch_bounds__(256) void k_direction(size_t n, double *__restrict_
 // One workgroup of 256 threads: totals the partials of the reduction kernel
 // launched just before on the same stream and applies the PCG scalar logic
 // (src/fpm_stairstep_mpi_3d.f90:257-278) on the device.
-__global__ __launch_bounds__(256) void k_finish(int op, int nslots, const double *partials,
+__global__ __launch_bounds__(512) void k_finish(int op, int nslots, const double *partials,
                                                 int nblocks, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
   if ((op == FIN_PQ || op == FIN_RZNEW) && sc->status != 0) return;
@@ -412,7 +417,7 @@ __global__ void k_clear_status(Scalars *sc) {
 
 // Gauge sums (src/fpm_stairstep_mpi_3d.f90:170-179): fluid-volume integral
 // of phi and the fluid volume -> sc->sums[0..1].
-__global__ __launch_bounds__(256) void k_gauge_sums(Geom g, const int32_t *__restrict__ solid,
+__global__ __launch_bounds__(512) void k_gauge_sums(Geom g, const int32_t *__restrict__ solid,
                                                    const double *__restrict__ phi,
                                                    double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
@@ -458,7 +463,7 @@ __global__ void k_gauge_apply(int nx, int ny, int nz, const int32_t *__restrict_
 // Flux balance of src/fpm_stairstep_mpi_3d.f90:49-52,69-91: optional fluid
 // source integral, net outward flux through fluid outer faces and body faces,
 // and fluid outer-face area -> sc->sums[0..2] (source, flux, area).
-__global__ __launch_bounds__(256) void k_compat_sums(
+__global__ __launch_bounds__(512) void k_compat_sums(
     Geom g, const int32_t *__restrict__ solid, const double *__restrict__ rhs,
     const double *__restrict__ xlo, const double *__restrict__ xhi,
     const double *__restrict__ ylo, const double *__restrict__ yhi,
@@ -559,7 +564,7 @@ __device__ __forceinline__ void face_gradients(const Geom &g, const double *__re
 
 // stairstep_residual_3d (src/fpm_stairstep_mpi_3d.f90:403-419) from face
 // gradients -> sc->res.
-__global__ __launch_bounds__(256) void k_face_residual(
+__global__ __launch_bounds__(512) void k_face_residual(
     Geom g, const double *__restrict__ phi, const double *__restrict__ rhs,
     const int32_t *__restrict__ solid, const double *__restrict__ xlo,
     const double *__restrict__ xhi, const double *__restrict__ ylo,
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(256) void k_face_residual(
 // compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469):
 // face gradients fused with the densities and their bounded integrals
 // -> sc->sums[0..1].
-__global__ __launch_bounds__(256) void k_terms(
+__global__ __launch_bounds__(512) void k_terms(
     Geom g, const double *__restrict__ phi, const int32_t *__restrict__ solid,
     const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ w,
     const double *__restrict__ ox, const double *__restrict__ oy,
@@ -656,96 +661,94 @@ namespace ffpm {
 // already the direction: it is read, not rewritten).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double dir_value(int mode, double beta, const double *__restrict__ r,
-                                            const double *__restrict__ rdiag,
-                                            const double *__restrict__ pin, size_t c) {
-  if (mode == kDirCurrent) return pin[c];
-  const double z = rdiag[c] * r[c];
-  if (mode == kDirReset) return z;
-  return z + beta * pin[c];
+// diag of cell (i,j,k) from its neighbour code: the fluid-neighbour
+// coefficients summed in build_operator's order, so the value is
+// bit-identical to the stored diag.
+__device__ __forceinline__ double diag_of(unsigned code, const Geom &g, int i, int j, int k) {
+  double d = 0;
+  if (code & kCodeW) d = d + g.cw[i];
+  if (code & kCodeE) d = d + g.ce[i];
+  if (code & kCodeS) d = d + g.cs[j];
+  if (code & kCodeN) d = d + g.cn[j];
+  if (code & kCodeB) d = d + g.cb[k];
+  if (code & kCodeT) d = d + g.ct[k];
+  return d;
 }
 
-__global__ __launch_bounds__(256) void k_pcg_dir_spmv(Tiling t, Geom g,
-                                                      const double *__restrict__ r,
-                                                      const double *__restrict__ rdiag,
+// rdiag = 1/(V diag) of the PCG operator, 0 where diag <= 0
+// (src/fpm_stairstep_mpi_3d.f90:374-376), from the code.
+__device__ __forceinline__ double rdiag_of(double d, double dxdy, double dzk) {
+  return d > 0 ? 1 / (dxdy * dzk * d) : 0.0;
+}
+
+// Direction value p at cell c: z + beta p_old (kDirUpdate), z (kDirReset) or
+// p_old (kDirCurrent), where z = M^-1 r was written by the previous update.
+// Every thread evaluating it for a cell gets the same bits.
+__device__ __forceinline__ double dir_value(int mode, double beta, const double *__restrict__ z,
+                                            const double *__restrict__ pin, size_t c) {
+  if (mode == kDirCurrent) return pin[c];
+  if (mode == kDirReset) return z[c];
+  return z[c] + beta * pin[c];
+}
+
+// Direction update fused with the SpMV and p.q: each thread rebuilds the new
+// direction of its x-y neighbours from z and p_old (through the cache) and of
+// its z-neighbours in registers, so there is no LDS exchange and no barrier.
+__global__ __launch_bounds__(512) void k_pcg_dir_spmv(Tiling t, Geom g,
+                                                      const double *__restrict__ z,
+                                                      const uint8_t *__restrict__ code,
                                                       const double *__restrict__ pin,
-                                                      double *__restrict__ pout,
-                                                      const double *__restrict__ diag, int mode,
+                                                      double *__restrict__ pout, int mode,
                                                       double *partials, Scalars *sc) {
-  __shared__ double tile[kTileY + 2][kTileX + 2];
   __shared__ double lds[16 * kReduceSlots];
   if (sc->status != 0) return;
   const double beta = sc->beta;
   const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int lin = tx + kTileX * ty;
-  const int i = tc.i, j = tc.j;
-  const int i0 = i - tx, j0 = j - ty;
-  const bool active = tc.valid && i < nx && j < ny;
-  const size_t nxy = (size_t)nx * ny;
-  // halo cell owned by this thread (lin < 136), in tile coordinates
-  int hi = -1, hj = -1, hx = 0, hy = 0;
-  if (lin < kTileX) {
-    hx = i0 + lin; hy = j0 - 1; hi = lin + 1; hj = 0;
-  } else if (lin < 2 * kTileX) {
-    hx = i0 + lin - kTileX; hy = j0 + kTileY; hi = lin - kTileX + 1; hj = kTileY + 1;
-  } else if (lin < 2 * kTileX + kTileY) {
-    hx = i0 - 1; hy = j0 + lin - 2 * kTileX; hi = 0; hj = lin - 2 * kTileX + 1;
-  } else if (lin < 2 * kTileX + 2 * kTileY) {
-    hx = i0 + kTileX; hy = j0 + lin - 2 * kTileX - kTileY; hi = kTileX + 1;
-    hj = lin - 2 * kTileX - kTileY + 1;
-  }
-  const bool halo_in = hi >= 0 && hx >= 0 && hx < nx && hy >= 0 && hy < ny;
-  double cw = 0, ce = 0, cs = 0, cn = 0, dxdy = 0;
-  size_t c = 0;
-  double p_prev = 0, p_cur = 0;
-  if (active) {
-    cw = g.cw[i]; ce = g.ce[i]; cs = g.cs[j]; cn = g.cn[j];
-    dxdy = g.dx[i] * g.dy[j];
-    c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    if (tc.k0 < nz) {
-      p_cur = dir_value(mode, beta, r, rdiag, pin, c);
-      p_prev = tc.k0 > 0 ? dir_value(mode, beta, r, rdiag, pin, c - nxy) : p_cur;
-    }
-  }
-  size_t hc = halo_in ? (size_t)hx + (size_t)nx * ((size_t)hy + (size_t)ny * tc.k0) : 0;
   double s = 0.0;
-  for (int k = tc.k0; k < tc.k1; ++k) {
-    double p_next = p_cur;
-    if (active && k + 1 < nz) p_next = dir_value(mode, beta, r, rdiag, pin, c + nxy);
-    tile[ty + 1][tx + 1] = active ? p_cur : 0.0;
-    if (hi >= 0) tile[hj][hi] = halo_in ? dir_value(mode, beta, r, rdiag, pin, hc) : 0.0;
-    __syncthreads();
-    if (active) {
-      const double d = diag[c];
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const double dxdy = g.dx[i] * g.dy[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    double p_cur = dir_value(mode, beta, z, pin, c);
+    double p_prev = tc.k0 > 0 ? dir_value(mode, beta, z, pin, c - nxy) : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const double p_next = (k + 1 < nz) ? dir_value(mode, beta, z, pin, c + nxy) : p_cur;
+      const double d = diag_of(code[c], g, i, j, k);
       double qv = 0.0;
       if (d > 0) {
+        const double pw = hw ? dir_value(mode, beta, z, pin, c - 1) : 0.0;
+        const double pe = he ? dir_value(mode, beta, z, pin, c + 1) : 0.0;
+        const double ps = hs ? dir_value(mode, beta, z, pin, c - nx) : 0.0;
+        const double pn = hn ? dir_value(mode, beta, z, pin, c + nx) : 0.0;
         qv = dxdy * g.dz[k] *
-             (d * p_cur - cw * tile[ty + 1][tx] - ce * tile[ty + 1][tx + 2] -
-              cs * tile[ty][tx + 1] - cn * tile[ty + 2][tx + 1] - g.cb[k] * p_prev -
+             (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
               g.ct[k] * p_next);
       }
       s += p_cur * qv;
       if (mode != kDirCurrent) pout[c] = p_cur;
+      p_prev = p_cur;
+      p_cur = p_next;
       c += nxy;
     }
-    hc += nxy;
-    p_prev = p_cur;
-    p_cur = p_next;
-    __syncthreads();
   }
   double v[1] = {s};
   block_sum<1>(v, lds);
   store_partials(partials, 1, v, gridDim.x);
 }
 
-__global__ __launch_bounds__(256) void k_pcg_update(Tiling t, Geom g, const double *__restrict__ p,
-                                                    const double *__restrict__ diag,
+// Update (src/fpm_stairstep_mpi_3d.f90:267-276) with q = A p recomputed
+// bit-identically, writing z = M^-1 r for the next direction update; the
+// rz_new terms (rdiag r) r are z r.
+__global__ __launch_bounds__(512) void k_pcg_update(Tiling t, Geom g, const double *__restrict__ p,
+                                                    const uint8_t *__restrict__ code,
                                                     double *__restrict__ phi,
                                                     double *__restrict__ r,
-                                                    const double *__restrict__ rdiag,
-                                                    double *partials, Scalars *sc) {
+                                                    double *__restrict__ z, double *partials,
+                                                    Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
   if (sc->status != 0) return;
   const double alpha = sc->alpha;
@@ -763,21 +766,24 @@ __global__ __launch_bounds__(256) void k_pcg_update(Tiling t, Geom g, const doub
     double p_prev = tc.k0 > 0 ? p[c - nxy] : p_cur;
     for (int k = tc.k0; k < tc.k1; ++k) {
       const double p_next = (k + 1 < nz) ? p[c + nxy] : p_cur;
-      const double d = diag[c];
+      const double d = diag_of(code[c], g, i, j, k);
+      const double dzk = g.dz[k];
       double qv = 0.0;
       if (d > 0) {
         const double pw = hw ? p[c - 1] : 0.0;
         const double pe = he ? p[c + 1] : 0.0;
         const double ps = hs ? p[c - nx] : 0.0;
         const double pn = hn ? p[c + nx] : 0.0;
-        qv = dxdy * g.dz[k] *
+        qv = dxdy * dzk *
              (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
               g.ct[k] * p_next);
       }
       phi[c] = phi[c] + alpha * p_cur;
       const double rv = r[c] - alpha * qv;
       r[c] = rv;
-      s += rdiag[c] * rv * rv;
+      const double zv = rdiag_of(d, dxdy, dzk) * rv;
+      z[c] = zv;
+      s += zv * rv;
       p_prev = p_cur;
       p_cur = p_next;
       c += nxy;
@@ -788,9 +794,299 @@ __global__ __launch_bounds__(256) void k_pcg_update(Tiling t, Geom g, const doub
   store_partials(partials, 1, v, gridDim.x);
 }
 
+// Deferred direction update before a convergence check, p = z + beta p, with
+// z = M^-1 r of the recursively updated r (src/fpm_stairstep_mpi_3d.f90:279-283).
+__global__ __launch_bounds__(512) void k_direction_z(size_t n, double *__restrict__ p,
+                                                     const double *__restrict__ z,
+                                                     const Scalars *sc) {
+  if (sc->status != 0) return;
+  const double beta = sc->beta;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    p[c] = z[c] + beta * p[c];
+}
+
 // Residual-only pass of true_residual without the direction reset
 // (src/fpm_stairstep_mpi_3d.f90:319-328 with reset false) is k_true_residual_2
 // with reset = 0; the deferred direction update before a convergence check is
 // k_direction.
+
+}  // namespace ffpm
+
+namespace ffpm {
+
+// ---------------------------------------------------------------------------
+// Latency-tolerant variants of the fused PCG kernels: branch-free bodies
+// (neighbour indices are clamped into the array and the zero halo is applied
+// with a select, so every load is issued unconditionally) and a z-loop
+// unrolled kUnroll planes deep, so each wavefront keeps the loads of several
+// planes in flight. Same arithmetic, same bits.
+// ---------------------------------------------------------------------------
+template <int U>
+__global__ __launch_bounds__(512) void k_pcg_dir_spmv_u(Tiling t, Geom g,
+                                                        const double *__restrict__ z,
+                                                        const uint8_t *__restrict__ code,
+                                                        const double *__restrict__ pin,
+                                                        double *__restrict__ pout, int mode,
+                                                        double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double beta = sc->beta;
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double s = 0.0;
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const double dxdy = g.dx[i] * g.dy[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 1 : 0, os = hs ? -(ptrdiff_t)nx : 0,
+                    on = hn ? (ptrdiff_t)nx : 0;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    double p_cur = dir_value(mode, beta, z, pin, c);
+    double p_prev = tc.k0 > 0 ? dir_value(mode, beta, z, pin, c - nxy) : p_cur;
+#pragma unroll U
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
+      const double p_next = dir_value(mode, beta, z, pin, cn_);
+      const double vw = dir_value(mode, beta, z, pin, c + ow);
+      const double ve = dir_value(mode, beta, z, pin, c + oe);
+      const double vs = dir_value(mode, beta, z, pin, c + os);
+      const double vn = dir_value(mode, beta, z, pin, c + on);
+      const double pw = hw ? vw : 0.0, pe = he ? ve : 0.0, ps = hs ? vs : 0.0,
+                   pn = hn ? vn : 0.0;
+      const double d = diag_of(code[c], g, i, j, k);
+      const double qraw = dxdy * g.dz[k] *
+                          (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
+                           g.ct[k] * p_next);
+      const double qv = d > 0 ? qraw : 0.0;
+      s += p_cur * qv;
+      if (mode != kDirCurrent) pout[c] = p_cur;
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, gridDim.x);
+}
+
+template <int U>
+__global__ __launch_bounds__(512) void k_pcg_update_u(Tiling t, Geom g,
+                                                      const double *__restrict__ p,
+                                                      const uint8_t *__restrict__ code,
+                                                      double *__restrict__ phi,
+                                                      double *__restrict__ r,
+                                                      double *__restrict__ z, double *partials,
+                                                      Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double alpha = sc->alpha;
+  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double s = 0.0;
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
+    const double dxdy = g.dx[i] * g.dy[j];
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 1 : 0, os = hs ? -(ptrdiff_t)nx : 0,
+                    on = hn ? (ptrdiff_t)nx : 0;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    double p_cur = p[c];
+    double p_prev = tc.k0 > 0 ? p[c - nxy] : p_cur;
+#pragma unroll U
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
+      const double p_next = p[cn_];
+      const double vw = p[c + ow], ve = p[c + oe], vs = p[c + os], vn = p[c + on];
+      const double pw = hw ? vw : 0.0, pe = he ? ve : 0.0, ps = hs ? vs : 0.0,
+                   pn = hn ? vn : 0.0;
+      const double d = diag_of(code[c], g, i, j, k);
+      const double dzk = g.dz[k];
+      const double qraw = dxdy * dzk *
+                          (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
+                           g.ct[k] * p_next);
+      const double qv = d > 0 ? qraw : 0.0;
+      const double ph_ = phi[c];
+      const double rr = r[c];
+      phi[c] = ph_ + alpha * p_cur;
+      const double rv = rr - alpha * qv;
+      r[c] = rv;
+      const double zv = rdiag_of(d, dxdy, dzk) * rv;
+      z[c] = zv;
+      s += zv * rv;
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, gridDim.x);
+}
+
+template __global__ void k_pcg_dir_spmv_u<2>(Tiling, Geom, const double *, const uint8_t *,
+                                              const double *, double *, int, double *, Scalars *);
+template __global__ void k_pcg_dir_spmv_u<4>(Tiling, Geom, const double *, const uint8_t *,
+                                              const double *, double *, int, double *, Scalars *);
+template __global__ void k_pcg_update_u<2>(Tiling, Geom, const double *, const uint8_t *, double *,
+                                            double *, double *, double *, Scalars *);
+template __global__ void k_pcg_update_u<4>(Tiling, Geom, const double *, const uint8_t *, double *,
+                                            double *, double *, double *, Scalars *);
+
+// ---------------------------------------------------------------------------
+// Paired-cell variants: each lane owns two x-adjacent cells (i, i+1), i even,
+// and moves them with 16-byte loads and stores, halving the memory
+// instructions per cell (the kernels are bound by memory-instruction issue,
+// not by bytes). A wavefront covers 128 cells of a row. Requires even nx.
+// The pair's inner neighbours come from registers; the outer ones (i-1,
+// i+2) with 8-byte loads. Same arithmetic, same bits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ TileCoord tile_coord2(const Tiling &t, int nx, int ny, int nz) {
+  TileCoord c = tile_coord(t, nx, ny, nz);
+  c.i = (c.i - threadIdx.x) * 2 + 2 * threadIdx.x;  // tiles of 128 cells along x
+  return c;
+}
+
+__device__ __forceinline__ double2 ld2(const double *__restrict__ a, size_t c) {
+  return *reinterpret_cast<const double2 *>(a + c);
+}
+
+__device__ __forceinline__ void st2(double *__restrict__ a, size_t c, double x, double y) {
+  *reinterpret_cast<double2 *>(a + c) = make_double2(x, y);
+}
+
+__device__ __forceinline__ double2 dir2(int mode, double beta, const double *__restrict__ z,
+                                        const double *__restrict__ pin, size_t c) {
+  if (mode == kDirCurrent) return ld2(pin, c);
+  const double2 zz = ld2(z, c);
+  if (mode == kDirReset) return zz;
+  const double2 pp = ld2(pin, c);
+  return make_double2(zz.x + beta * pp.x, zz.y + beta * pp.y);
+}
+
+__global__ __launch_bounds__(512) void k_pcg_dir_spmv_x2(Tiling t, Geom g,
+                                                         const double *__restrict__ z,
+                                                         const uint8_t *__restrict__ code,
+                                                         const double *__restrict__ pin,
+                                                         double *__restrict__ pout, int mode,
+                                                         double *partials, Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double beta = sc->beta;
+  const TileCoord tc = tile_coord2(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double s = 0.0;
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    const double cw0 = g.cw[i], ce0 = g.ce[i], cw1 = g.cw[i + 1], ce1 = g.ce[i + 1];
+    const double cs = g.cs[j], cn = g.cn[j];
+    const double dxdy0 = g.dx[i] * g.dy[j], dxdy1 = g.dx[i + 1] * g.dy[j];
+    const bool hw = i > 0, he = i + 2 < nx, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 2 : 1;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    double2 p_cur = dir2(mode, beta, z, pin, c);
+    double2 p_prev = tc.k0 > 0 ? dir2(mode, beta, z, pin, c - nxy) : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
+      const double2 p_next = dir2(mode, beta, z, pin, cn_);
+      const double2 vs = dir2(mode, beta, z, pin, c + os);
+      const double2 vn = dir2(mode, beta, z, pin, c + on);
+      const double vw = dir_value(mode, beta, z, pin, c + ow);
+      const double ve = dir_value(mode, beta, z, pin, c + oe);
+      const unsigned cc = *reinterpret_cast<const uint16_t *>(code + c);
+      const double pw0 = hw ? vw : 0.0, pe1 = he ? ve : 0.0;
+      const double ps0 = hs ? vs.x : 0.0, ps1 = hs ? vs.y : 0.0;
+      const double pn0 = hn ? vn.x : 0.0, pn1 = hn ? vn.y : 0.0;
+      const double d0 = diag_of(cc & 0xffu, g, i, j, k);
+      const double d1 = diag_of(cc >> 8, g, i + 1, j, k);
+      const double dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
+      const double q0 = dxdy0 * dzk *
+                        (d0 * p_cur.x - cw0 * pw0 - ce0 * p_cur.y - cs * ps0 - cn * pn0 -
+                         cb * p_prev.x - ct * p_next.x);
+      const double q1 = dxdy1 * dzk *
+                        (d1 * p_cur.y - cw1 * p_cur.x - ce1 * pe1 - cs * ps1 - cn * pn1 -
+                         cb * p_prev.y - ct * p_next.y);
+      s += p_cur.x * (d0 > 0 ? q0 : 0.0);
+      s += p_cur.y * (d1 > 0 ? q1 : 0.0);
+      if (mode != kDirCurrent) st2(pout, c, p_cur.x, p_cur.y);
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, gridDim.x);
+}
+
+__global__ __launch_bounds__(512) void k_pcg_update_x2(Tiling t, Geom g,
+                                                       const double *__restrict__ p,
+                                                       const uint8_t *__restrict__ code,
+                                                       double *__restrict__ phi,
+                                                       double *__restrict__ r,
+                                                       double *__restrict__ z, double *partials,
+                                                       Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  if (sc->status != 0) return;
+  const double alpha = sc->alpha;
+  const TileCoord tc = tile_coord2(t, g.nx, g.ny, g.nz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  double s = 0.0;
+  if (tc.valid && tc.i < nx && tc.j < ny) {
+    const int i = tc.i, j = tc.j;
+    const size_t nxy = (size_t)nx * ny;
+    const double cw0 = g.cw[i], ce0 = g.ce[i], cw1 = g.cw[i + 1], ce1 = g.ce[i + 1];
+    const double cs = g.cs[j], cn = g.cn[j];
+    const double dxdy0 = g.dx[i] * g.dy[j], dxdy1 = g.dx[i + 1] * g.dy[j];
+    const bool hw = i > 0, he = i + 2 < nx, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 2 : 1;
+    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
+    double2 p_cur = ld2(p, c);
+    double2 p_prev = tc.k0 > 0 ? ld2(p, c - nxy) : p_cur;
+    for (int k = tc.k0; k < tc.k1; ++k) {
+      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
+      const double2 p_next = ld2(p, cn_);
+      const double2 vs = ld2(p, c + os), vn = ld2(p, c + on);
+      const double vw = p[c + ow], ve = p[c + oe];
+      const unsigned cc = *reinterpret_cast<const uint16_t *>(code + c);
+      const double2 ph2 = ld2(phi, c), r2 = ld2(r, c);
+      const double pw0 = hw ? vw : 0.0, pe1 = he ? ve : 0.0;
+      const double ps0 = hs ? vs.x : 0.0, ps1 = hs ? vs.y : 0.0;
+      const double pn0 = hn ? vn.x : 0.0, pn1 = hn ? vn.y : 0.0;
+      const double d0 = diag_of(cc & 0xffu, g, i, j, k);
+      const double d1 = diag_of(cc >> 8, g, i + 1, j, k);
+      const double dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
+      const double q0r = dxdy0 * dzk *
+                         (d0 * p_cur.x - cw0 * pw0 - ce0 * p_cur.y - cs * ps0 - cn * pn0 -
+                          cb * p_prev.x - ct * p_next.x);
+      const double q1r = dxdy1 * dzk *
+                         (d1 * p_cur.y - cw1 * p_cur.x - ce1 * pe1 - cs * ps1 - cn * pn1 -
+                          cb * p_prev.y - ct * p_next.y);
+      const double q0 = d0 > 0 ? q0r : 0.0, q1 = d1 > 0 ? q1r : 0.0;
+      st2(phi, c, ph2.x + alpha * p_cur.x, ph2.y + alpha * p_cur.y);
+      const double rv0 = r2.x - alpha * q0, rv1 = r2.y - alpha * q1;
+      st2(r, c, rv0, rv1);
+      const double zv0 = rdiag_of(d0, dxdy0, dzk) * rv0;
+      const double zv1 = rdiag_of(d1, dxdy1, dzk) * rv1;
+      st2(z, c, zv0, zv1);
+      s += zv0 * rv0;
+      s += zv1 * rv1;
+      p_prev = p_cur;
+      p_cur = p_next;
+      c += nxy;
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, gridDim.x);
+}
 
 }  // namespace ffpm
