@@ -37,6 +37,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {
     "pcg_dir_spmv": 25,  # read z, p_old, neighbour code (1 B); write p
     "pcg_update": 49,    # read p, code (1 B), phi, r; write phi, r, z
+    "pcg32_dir_spmv": 13,  # FP32 inner: read z, p_old (4 B each), code; write p
+    "pcg32_update": 25,    # FP32 inner: read p, code, e, r; write e, r, z
     "spmv_dot": 24,      # read p, diag; write q
     "update_dot": 56,    # read phi, p, r, q, rdiag; write phi, r
     "direction": 32,     # read rdiag, r, p; write p
@@ -103,7 +105,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="pcg", choices=["pcg", "jacobi"])
+    ap.add_argument("--method", default="pcg", choices=["pcg", "jacobi", "pcg-mixed"])
     ap.add_argument("--grid", type=int, nargs=3, default=[512, 384, 256])
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check", type=int, default=100)
@@ -134,8 +136,11 @@ def main():
     gen_s = time.perf_counter() - t_gen
     log(f"inputs resident on the device: {p.label}, {p.cells} cells, {gen_s:.1f} s")
 
+    mixed_outer = []
+
     def step():
         iters, res, totals = [], [], []
+        mixed_outer.clear()
         for d in range(3):
             d_phi[d].zero_()
             for f in d_faces[d]:
@@ -151,6 +156,8 @@ def main():
                                                  *outs[d])
             iters.append(sw)
             res.append(r)
+            if args.method == "pcg-mixed":
+                mixed_outer.append(S.last_mixed_solve()[0])
             totals.append([float(x) for x in o[5]])
             log(f"direction {d}: {sw} iterations, residual {r:.3e}")
         return iters, res, totals
@@ -205,6 +212,7 @@ def main():
             "tolerance": args.tol, "check_interval": args.check,
         },
         "iterations": iters,
+        "mixed_outer_corrections": mixed_outer if args.method == "pcg-mixed" else None,
         "residuals": res,
         "converged": all(r <= args.tol for r in res) if args.tol > 0 else None,
         "solves_per_s": round(3 * 1e3 / ms_per_step, 4),

@@ -15,6 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [
     "ffpm_runtime.hip",
     "ffpm_stairstep_kernels.hip",
+    "ffpm_pcg_kernels.hip",
     "ffpm_stairstep_host.hip",
     "ffpm_laplace.hip",
 ]

@@ -71,24 +71,6 @@ __global__ void k_update_dot(size_t n, double *phi, const double *p, double *r, 
                              const double *rdiag, double *partials, Scalars *sc);
 __global__ void k_direction(size_t n, double *p, const double *r, const double *rdiag,
                             const Scalars *sc);
-__global__ void k_pcg_dir_spmv(Tiling t, Geom g, const double *z, const uint8_t *code,
-                               const double *pin, double *pout, int mode, double *partials,
-                               Scalars *sc);
-__global__ void k_pcg_update(Tiling t, Geom g, const double *p, const uint8_t *code, double *phi,
-                             double *r, double *z, double *partials, Scalars *sc);
-__global__ void k_direction_z(size_t n, double *p, const double *z, const Scalars *sc);
-__global__ void k_pcg_dir_spmv_x2(Tiling t, Geom g, const double *z, const uint8_t *code,
-                                  const double *pin, double *pout, int mode, double *partials,
-                                  Scalars *sc);
-__global__ void k_pcg_update_x2(Tiling t, Geom g, const double *p, const uint8_t *code,
-                                double *phi, double *r, double *z, double *partials, Scalars *sc);
-template <int U>
-__global__ void k_pcg_dir_spmv_u(Tiling t, Geom g, const double *z, const uint8_t *code,
-                                 const double *pin, double *pout, int mode, double *partials,
-                                 Scalars *sc);
-template <int U>
-__global__ void k_pcg_update_u(Tiling t, Geom g, const double *p, const uint8_t *code,
-                               double *phi, double *r, double *z, double *partials, Scalars *sc);
 __global__ void k_finish(int op, int nslots, const double *partials, int nblocks, Scalars *sc);
 __global__ void k_chunk_begin(Scalars *sc);
 __global__ void k_clear_status(Scalars *sc);

@@ -37,7 +37,7 @@ Workspace &workspace();
 
 enum Slot : int {
   S_COEF = 1, S_PHI, S_NEXT, S_SRC, S_DIAG, S_RDIAG, S_R, S_Q, S_PARTIALS, S_SCALARS,
-  S_ISOLATED, S_CODE, S_Z,
+  S_ISOLATED, S_CODE, S_Z, S_COEF32, S_E32, S_R32, S_Z32, S_P32A, S_P32B,
   // host-pointer entry points stage their arguments here
   S_U_PHI = 40, S_U_RHS, S_U_SOLID, S_U_XLO, S_U_XHI, S_U_YLO, S_U_YHI, S_U_ZLO, S_U_ZHI,
   S_U_XC, S_U_YC, S_U_ZC, S_U_DX, S_U_DY, S_U_DZ, S_U_F0, S_U_F1, S_U_F2, S_U_F3, S_U_F4,

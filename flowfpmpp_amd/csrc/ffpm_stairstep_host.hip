@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/ffpm.h"
+#include "ffpm_pcg.hpp"
 #include "ffpm_runtime.hpp"
 
 namespace ffpm {
@@ -46,8 +47,7 @@ struct Frame {
   size_t n = 0;
   Geom g{};
   Tiling tiling{};
-  Tiling tiling2{};  // paired-cell kernels (even nx)
-  bool paired = false;
+  Tiling tl[5]{};  // launch shapes of the CL-cells-per-lane kernels, CL = 1, 2, 4
   double *phi = nullptr, *next = nullptr, *src = nullptr, *diag = nullptr, *rdiag = nullptr;
   double *r = nullptr, *q = nullptr, *z = nullptr, *partials = nullptr;
   uint8_t *code = nullptr;
@@ -72,13 +72,11 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   f.nz = nz;
   f.n = (size_t)nx * ny * nz;
   f.tiling = make_tiling(nx, ny, nz, 0);
-  f.tiling2 = make_tiling(nx, ny, nz, 0, 0, 2 * kTileX);
-  {
-    const char *e = std::getenv("FFPM_X2");
-    f.paired = nx % 2 == 0 && !(e && std::strcmp(e, "0") == 0);
-  }
   size_t nb = (size_t)max_blocks_for(f.tiling, f.n);
-  if ((size_t)f.tiling2.blocks > nb) nb = (size_t)f.tiling2.blocks;
+  for (int cl : {1, 2, 4}) {
+    f.tl[cl] = make_tiling(nx, ny, nz, 0, 0, kTileX * cl);
+    if ((size_t)f.tl[cl].blocks > nb) nb = (size_t)f.tl[cl].blocks;
+  }
   double *coef = (double *)ws.get(S_COEF, sizeof(double) * 2 * (nx + ny + nz), &st);
   if (!coef) return st;
   f.partials = (double *)ws.get(S_PARTIALS, sizeof(double) * kReduceSlots * nb, &st);
@@ -249,8 +247,81 @@ int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, 
   return OK;
 }
 
-// pcg_iterate with the fused two-kernel iteration (k_pcg_dir_spmv +
-// k_pcg_update): the same formulas and the same convergence logic; the
+// LDS-tiled direction kernel (FFPM_DIR_LDS=1) or the cache-only one.
+bool dir_lds() {
+  const char *e = std::getenv("FFPM_DIR_LDS");
+  return e && std::strcmp(e, "1") == 0;
+}
+
+// Cells per lane of the fused kernels: 16-byte vectors where nx allows.
+template <typename T>
+int cells_per_lane(int nx) {
+  const char *e = std::getenv("FFPM_X2");
+  if (e && std::strcmp(e, "0") == 0) return 1;
+  const int w = (int)(16 / sizeof(T));
+  if (w >= 4 && nx % 4 == 0) return 4;
+  if (nx % 2 == 0) return 2;
+  return 1;
+}
+
+template <typename T>
+int launch_dir(const Frame &f, int cl, const GeomT<T> &g, const T *z, const T *pin, T *pout,
+               int mode) {
+  const Tiling &t = f.tl[cl];
+  const dim3 blk(kTileX, t.ty);
+  {
+    TimedLaunch tl(sizeof(T) == 8 ? "pcg_dir_spmv" : "pcg32_dir_spmv");
+    if (dir_lds() && (t.ty == 4 || t.ty == 8)) {
+      // The workgroup shape is fixed by the template; the tiling's rows per
+      // workgroup must match it.
+      auto go = [&](auto kern) {
+        kern<<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, mode, f.partials, f.sc);
+      };
+      if (cl == 4) {
+        if constexpr (sizeof(T) == 4) {
+          if (t.ty == 8) go(k_pcg_dir_lds_t<T, 4, 8>); else go(k_pcg_dir_lds_t<T, 4, 4>);
+        }
+      } else if (cl == 2) {
+        if (t.ty == 8) go(k_pcg_dir_lds_t<T, 2, 8>); else go(k_pcg_dir_lds_t<T, 2, 4>);
+      } else {
+        if (t.ty == 8) go(k_pcg_dir_lds_t<T, 1, 8>); else go(k_pcg_dir_lds_t<T, 1, 4>);
+      }
+    } else if (cl == 4)
+      k_pcg_dir_t<T, (sizeof(T) == 4 ? 4 : 2)><<<t.blocks, blk, 0, stream()>>>(
+          t, g, z, f.code, pin, pout, mode, f.partials, f.sc);
+    else if (cl == 2)
+      k_pcg_dir_t<T, 2><<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, mode,
+                                                        f.partials, f.sc);
+    else
+      k_pcg_dir_t<T, 1><<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, mode,
+                                                        f.partials, f.sc);
+  }
+  FFPM_LAUNCHED("k_pcg_dir_t");
+  return finish(f, FIN_PQ, 1, t.blocks);
+}
+
+template <typename T>
+int launch_upd(const Frame &f, int cl, const GeomT<T> &g, const T *p, T *x, T *r, T *z) {
+  const Tiling &t = f.tl[cl];
+  const dim3 blk(kTileX, t.ty);
+  {
+    TimedLaunch tl(sizeof(T) == 8 ? "pcg_update" : "pcg32_update");
+    if (cl == 4)
+      k_pcg_upd_t<T, (sizeof(T) == 4 ? 4 : 2)><<<t.blocks, blk, 0, stream()>>>(
+          t, g, p, f.code, x, r, z, f.partials, f.sc);
+    else if (cl == 2)
+      k_pcg_upd_t<T, 2><<<t.blocks, blk, 0, stream()>>>(t, g, p, f.code, x, r, z, f.partials,
+                                                        f.sc);
+    else
+      k_pcg_upd_t<T, 1><<<t.blocks, blk, 0, stream()>>>(t, g, p, f.code, x, r, z, f.partials,
+                                                        f.sc);
+  }
+  FFPM_LAUNCHED("k_pcg_upd_t");
+  return finish(f, FIN_RZNEW, 1, t.blocks);
+}
+
+// pcg_iterate with the fused two-kernel iteration (k_pcg_dir_t +
+// k_pcg_upd_t): the same formulas and the same convergence logic; the
 // direction update p = M^-1 r + beta p of iteration k runs inside the SpMV of
 // iteration k+1, except before a convergence check, where it runs on its own
 // with the recursively updated r exactly as the reference orders it
@@ -260,6 +331,8 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
                       int *sweeps, double *residual) {
   const int ci = check_interval > 1 ? check_interval : 1;
   const int eb = elementwise_blocks(f.n);
+  const int cl = cells_per_lane<double>(f.nx);
+  const GeomT<double> g = geom_t(f.g);
   double *pbuf[2] = {f.next, f.q};
   int cur = 0;
   double res;
@@ -268,9 +341,6 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
   int mode = kDirReset;
   int it = 0, restarts = 0, stalled = 0;
   double best = res;
-  const dim3 blk(kTileX, f.tiling.ty);
-  const char *kv = std::getenv("FFPM_UNROLL");
-  const int unroll = kv ? std::atoi(kv) : 1;
   while (*sweeps < max_sweeps && res > tolerance) {
     int n = ci - it % ci;
     if (max_sweeps - *sweeps < n) n = max_sweeps - *sweeps;
@@ -278,42 +348,11 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
     FFPM_LAUNCHED("k_chunk_begin");
     for (int m = 0; m < n; ++m) {
       const int dst = cur ^ 1;
-      {
-        TimedLaunch tl("pcg_dir_spmv");
-        if (f.paired)
-          k_pcg_dir_spmv_x2<<<f.tiling2.blocks, blk, 0, stream()>>>(
-              f.tiling2, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
-        else if (unroll == 4)
-          k_pcg_dir_spmv_u<4><<<f.tiling.blocks, blk, 0, stream()>>>(
-              f.tiling, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
-        else if (unroll == 2)
-          k_pcg_dir_spmv_u<2><<<f.tiling.blocks, blk, 0, stream()>>>(
-              f.tiling, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
-        else
-          k_pcg_dir_spmv<<<f.tiling.blocks, blk, 0, stream()>>>(
-              f.tiling, f.g, f.z, f.code, pbuf[cur], pbuf[dst], mode, f.partials, f.sc);
-      }
-      finish(f, FIN_PQ, 1, f.paired ? f.tiling2.blocks : f.tiling.blocks);
+      if ((st = launch_dir<double>(f, cl, g, f.z, pbuf[cur], pbuf[dst], mode))) return st;
       if (mode != kDirCurrent) cur = dst;
-      {
-        TimedLaunch tl("pcg_update");
-        if (f.paired)
-          k_pcg_update_x2<<<f.tiling2.blocks, blk, 0, stream()>>>(
-              f.tiling2, f.g, pbuf[cur], f.code, f.phi, f.r, f.z, f.partials, f.sc);
-        else if (unroll == 4)
-          k_pcg_update_u<4><<<f.tiling.blocks, blk, 0, stream()>>>(
-              f.tiling, f.g, pbuf[cur], f.code, f.phi, f.r, f.z, f.partials, f.sc);
-        else if (unroll == 2)
-          k_pcg_update_u<2><<<f.tiling.blocks, blk, 0, stream()>>>(
-              f.tiling, f.g, pbuf[cur], f.code, f.phi, f.r, f.z, f.partials, f.sc);
-        else
-          k_pcg_update<<<f.tiling.blocks, blk, 0, stream()>>>(f.tiling, f.g, pbuf[cur], f.code,
-                                                              f.phi, f.r, f.z, f.partials, f.sc);
-      }
-      finish(f, FIN_RZNEW, 1, f.paired ? f.tiling2.blocks : f.tiling.blocks);
+      if ((st = launch_upd<double>(f, cl, g, pbuf[cur], f.phi, f.r, f.z))) return st;
       mode = kDirUpdate;
     }
-    FFPM_LAUNCHED("fused pcg chunk");
     Scalars h;
     if ((st = read_scalars(f, &h))) return st;
     *sweeps += h.iters;
@@ -332,9 +371,9 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
     if (it % ci == 0 || *sweeps == max_sweeps) {
       {
         TimedLaunch tl("direction");
-        k_direction_z<<<eb, kThreads, 0, stream()>>>(f.n, pbuf[cur], f.z, f.sc);
+        k_dir_t<double><<<eb, kThreads, 0, stream()>>>(f.n, pbuf[cur], f.z, f.sc);
       }
-      FFPM_LAUNCHED("k_direction_z");
+      FFPM_LAUNCHED("k_dir_t");
       mode = kDirCurrent;
       if ((st = true_residual(f, false, &res))) return st;
       if (res < 0.99 * best) {
@@ -348,6 +387,102 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
   }
   *residual = res;
   return OK;
+}
+
+// Mixed precision: FP64 defect correction around an FP32 PCG. Each outer step
+// forms the projected FP64 true residual r (src/fpm_stairstep_mpi_3d.f90:
+// 297-329), solves A e = r approximately with the fused PCG in FP32 (FP32
+// stencil, coefficients and preconditioner; dot products summed in FP64)
+// until the preconditioned residual norm has dropped by inner_rtol, and adds
+// e to phi in FP64. The outer loop stops on the reference's FP64 max-norm
+// residual test, so the result is judged exactly like the FP64 solve.
+struct MixedStats {
+  int outer = 0, inner = 0;
+};
+thread_local MixedStats g_mixed;
+
+int pcg_mixed(Frame &f, double tolerance, int max_inner, int check_interval, double inner_rtol,
+              int *sweeps, double *residual) {
+  int st = 0;
+  Workspace &ws = workspace();
+  const int nx = f.nx, ny = f.ny, nz = f.nz;
+  const size_t n = f.n;
+  const int eb = elementwise_blocks(n);
+  // FP32 geometry
+  float *c32 = (float *)ws.get(S_COEF32, sizeof(float) * 3 * (nx + ny + nz), &st);
+  if (!c32) return st;
+  const int ncoef = 2 * (nx + ny + nz);
+  k_to_float<<<(ncoef + 255) / 256, 256, 0, stream()>>>(ncoef, f.g.cw, c32);
+  k_to_float<<<(nx + 255) / 256, 256, 0, stream()>>>(nx, f.g.dx, c32 + ncoef);
+  k_to_float<<<(ny + 255) / 256, 256, 0, stream()>>>(ny, f.g.dy, c32 + ncoef + nx);
+  k_to_float<<<(nz + 255) / 256, 256, 0, stream()>>>(nz, f.g.dz, c32 + ncoef + nx + ny);
+  FFPM_LAUNCHED("k_to_float");
+  GeomT<float> g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.cw = c32; g.ce = c32 + nx; g.cs = c32 + 2 * nx; g.cn = c32 + 2 * nx + ny;
+  g.cb = c32 + 2 * nx + 2 * ny; g.ct = c32 + 2 * nx + 2 * ny + nz;
+  g.dx = c32 + ncoef; g.dy = c32 + ncoef + nx; g.dz = c32 + ncoef + nx + ny;
+  float *e = (float *)ws.get(S_E32, sizeof(float) * n, &st);
+  float *r32 = e ? (float *)ws.get(S_R32, sizeof(float) * n, &st) : nullptr;
+  float *z32 = r32 ? (float *)ws.get(S_Z32, sizeof(float) * n, &st) : nullptr;
+  float *p0 = z32 ? (float *)ws.get(S_P32A, sizeof(float) * n, &st) : nullptr;
+  float *p1 = p0 ? (float *)ws.get(S_P32B, sizeof(float) * n, &st) : nullptr;
+  if (!p1) return st;
+  float *pbuf[2] = {p0, p1};
+  const int cl = cells_per_lane<float>(nx);
+  const int ci = check_interval > 1 ? check_interval : 1;
+  double res;
+  if ((st = true_residual(f, false, &res))) return st;
+  double best = res;
+  int stalled = 0, outer = 0, inner = 0;
+  while (res > tolerance && inner < max_inner && outer < 100) {
+    k_mixed_begin<<<eb, kThreads, 0, stream()>>>(g, f.r, f.code, r32, z32, e, f.partials);
+    FFPM_LAUNCHED("k_mixed_begin");
+    finish(f, FIN_RZ, 1, eb);
+    k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
+    Scalars h;
+    if ((st = read_scalars(f, &h))) return st;
+    const double rz0 = h.rz;
+    if (!(rz0 > 0)) break;
+    const double target = rz0 * inner_rtol * inner_rtol;
+    int mode = kDirReset, cur = 0;
+    while (inner < max_inner) {
+      int m_n = ci;
+      if (max_inner - inner < m_n) m_n = max_inner - inner;
+      k_chunk_begin<<<1, 64, 0, stream()>>>(f.sc);
+      for (int m = 0; m < m_n; ++m) {
+        const int dst = cur ^ 1;
+        if ((st = launch_dir<float>(f, cl, g, z32, pbuf[cur], pbuf[dst], mode))) return st;
+        cur = dst;
+        if ((st = launch_upd<float>(f, cl, g, pbuf[cur], e, r32, z32))) return st;
+        mode = kDirUpdate;
+      }
+      if ((st = read_scalars(f, &h))) return st;
+      inner += h.iters;
+      if (h.status != 0 || h.rz <= target || h.iters == 0) break;
+    }
+    k_mixed_accumulate<<<eb, kThreads, 0, stream()>>>(n, f.phi, e);
+    FFPM_LAUNCHED("k_mixed_accumulate");
+    k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
+    ++outer;
+    if ((st = true_residual(f, false, &res))) return st;
+    if (res < 0.99 * best) {
+      best = res;
+      stalled = 0;
+    } else if (++stalled >= 3) {
+      break;
+    }
+  }
+  g_mixed.outer = outer;
+  g_mixed.inner = inner;
+  *sweeps = inner;
+  *residual = res;
+  return OK;
+}
+
+double mixed_inner_rtol() {
+  const char *e = std::getenv("FFPM_MIXED_RTOL");
+  return e && *e ? std::atof(e) : 1e-4;
 }
 
 bool classic_pcg() {
@@ -393,7 +528,8 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
                         int nz, int west, int east, int south, int north, double tolerance,
                         int max_sweeps, int check_interval, int method, int *sweeps,
                         double *residual) {
-  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI) {
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI &&
+      method != FFPM_METHOD_PCG_MIXED) {
     set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
     return ERR_METHOD;
   }
@@ -408,7 +544,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   }
   if (!all_present({phi_ref, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz}))
     return null_error();
-  const bool pcg = method == FFPM_METHOD_PCG;
+  const bool pcg = method == FFPM_METHOD_PCG || method == FFPM_METHOD_PCG_MIXED;
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, pcg);
   if (st) return st;
@@ -430,7 +566,11 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   k_pack_phi<<<eb, kThreads, 0, stream()>>>(nx, ny, nz, solid, phi_ref, f.phi);
   FFPM_LAUNCHED("k_pack_phi");
   double *cur = f.phi;
-  if (pcg) {
+  g_mixed = MixedStats();
+  if (method == FFPM_METHOD_PCG_MIXED) {
+    st = pcg_mixed(f, tolerance, max_sweeps, check_interval, mixed_inner_rtol(), sweeps,
+                   residual);
+  } else if (pcg) {
     st = classic_pcg()
              ? pcg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual)
              : pcg_iterate_fused(f, tolerance, max_sweeps, check_interval, sweeps, residual);
@@ -643,6 +783,12 @@ extern "C" {
 int ffpm_exchange_solid_halo_3d_dev(int32_t *solid, int nx, int ny, int nz, int west, int east,
                                     int south, int north);
 
+int ffpm_last_mixed_solve(int *outer, int *inner) {
+  *outer = g_mixed.outer;
+  *inner = g_mixed.inner;
+  return FFPM_OK;
+}
+
 int ffpm_exchange_solid_halo_3d(int32_t *solid, int nx, int ny, int nz, int west, int east,
                                 int south, int north) {
   // src/fpm_stairstep_mpi_3d.f90:29-32: zero the halo at physical boundaries.
@@ -730,7 +876,8 @@ int ffpm_solve_stairstep_poisson_3d(
     const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
     int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
     int method, int *sweeps, double *residual) {
-  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI) {
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI &&
+      method != FFPM_METHOD_PCG_MIXED) {
     set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
     return ERR_METHOD;
   }

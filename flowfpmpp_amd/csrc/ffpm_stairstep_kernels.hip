@@ -641,452 +641,3 @@ __global__ __launch_bounds__(512) void k_terms(
 }
 
 }  // namespace ffpm
-
-namespace ffpm {
-
-// ---------------------------------------------------------------------------
-// Fused PCG iteration: two kernels and 96 B/cell instead of three kernels and
-// 112 B/cell, with the reference's formulas unchanged
-// (src/fpm_stairstep_mpi_3d.f90:246-283):
-//   k_pcg_dir_spmv: p = M^-1 r + beta p (the previous iteration's direction
-//     update, deferred), q = A p and pq = p.q, breakdown test, alpha.
-//     The x-y neighbours of p are exchanged through an LDS tile; the ring of
-//     the tile recomputes p from r, rdiag and the old p of the neighbour tile
-//     (read through the cache), so p ping-pongs between two buffers. q is not
-//     stored.
-//   k_pcg_update: recomputes q = A p (bit-identical: same expression, same
-//     inputs), phi += alpha p, r -= alpha q, rz_new = r.M^-1 r, beta.
-// mode of the first kernel: kDirUpdate (p = M^-1 r + beta p_old),
-// kDirReset (p = M^-1 r, after a true-residual reset) or kDirCurrent (p_old is
-// already the direction: it is read, not rewritten).
-// ---------------------------------------------------------------------------
-
-// diag of cell (i,j,k) from its neighbour code: the fluid-neighbour
-// coefficients summed in build_operator's order, so the value is
-// bit-identical to the stored diag.
-__device__ __forceinline__ double diag_of(unsigned code, const Geom &g, int i, int j, int k) {
-  double d = 0;
-  if (code & kCodeW) d = d + g.cw[i];
-  if (code & kCodeE) d = d + g.ce[i];
-  if (code & kCodeS) d = d + g.cs[j];
-  if (code & kCodeN) d = d + g.cn[j];
-  if (code & kCodeB) d = d + g.cb[k];
-  if (code & kCodeT) d = d + g.ct[k];
-  return d;
-}
-
-// rdiag = 1/(V diag) of the PCG operator, 0 where diag <= 0
-// (src/fpm_stairstep_mpi_3d.f90:374-376), from the code.
-__device__ __forceinline__ double rdiag_of(double d, double dxdy, double dzk) {
-  return d > 0 ? 1 / (dxdy * dzk * d) : 0.0;
-}
-
-// Direction value p at cell c: z + beta p_old (kDirUpdate), z (kDirReset) or
-// p_old (kDirCurrent), where z = M^-1 r was written by the previous update.
-// Every thread evaluating it for a cell gets the same bits.
-__device__ __forceinline__ double dir_value(int mode, double beta, const double *__restrict__ z,
-                                            const double *__restrict__ pin, size_t c) {
-  if (mode == kDirCurrent) return pin[c];
-  if (mode == kDirReset) return z[c];
-  return z[c] + beta * pin[c];
-}
-
-// Direction update fused with the SpMV and p.q: each thread rebuilds the new
-// direction of its x-y neighbours from z and p_old (through the cache) and of
-// its z-neighbours in registers, so there is no LDS exchange and no barrier.
-__global__ __launch_bounds__(512) void k_pcg_dir_spmv(Tiling t, Geom g,
-                                                      const double *__restrict__ z,
-                                                      const uint8_t *__restrict__ code,
-                                                      const double *__restrict__ pin,
-                                                      double *__restrict__ pout, int mode,
-                                                      double *partials, Scalars *sc) {
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const double beta = sc->beta;
-  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
-    const double dxdy = g.dx[i] * g.dy[j];
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    double p_cur = dir_value(mode, beta, z, pin, c);
-    double p_prev = tc.k0 > 0 ? dir_value(mode, beta, z, pin, c - nxy) : p_cur;
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const double p_next = (k + 1 < nz) ? dir_value(mode, beta, z, pin, c + nxy) : p_cur;
-      const double d = diag_of(code[c], g, i, j, k);
-      double qv = 0.0;
-      if (d > 0) {
-        const double pw = hw ? dir_value(mode, beta, z, pin, c - 1) : 0.0;
-        const double pe = he ? dir_value(mode, beta, z, pin, c + 1) : 0.0;
-        const double ps = hs ? dir_value(mode, beta, z, pin, c - nx) : 0.0;
-        const double pn = hn ? dir_value(mode, beta, z, pin, c + nx) : 0.0;
-        qv = dxdy * g.dz[k] *
-             (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
-              g.ct[k] * p_next);
-      }
-      s += p_cur * qv;
-      if (mode != kDirCurrent) pout[c] = p_cur;
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-// Update (src/fpm_stairstep_mpi_3d.f90:267-276) with q = A p recomputed
-// bit-identically, writing z = M^-1 r for the next direction update; the
-// rz_new terms (rdiag r) r are z r.
-__global__ __launch_bounds__(512) void k_pcg_update(Tiling t, Geom g, const double *__restrict__ p,
-                                                    const uint8_t *__restrict__ code,
-                                                    double *__restrict__ phi,
-                                                    double *__restrict__ r,
-                                                    double *__restrict__ z, double *partials,
-                                                    Scalars *sc) {
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const double alpha = sc->alpha;
-  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
-    const double dxdy = g.dx[i] * g.dy[j];
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    double p_cur = p[c];
-    double p_prev = tc.k0 > 0 ? p[c - nxy] : p_cur;
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const double p_next = (k + 1 < nz) ? p[c + nxy] : p_cur;
-      const double d = diag_of(code[c], g, i, j, k);
-      const double dzk = g.dz[k];
-      double qv = 0.0;
-      if (d > 0) {
-        const double pw = hw ? p[c - 1] : 0.0;
-        const double pe = he ? p[c + 1] : 0.0;
-        const double ps = hs ? p[c - nx] : 0.0;
-        const double pn = hn ? p[c + nx] : 0.0;
-        qv = dxdy * dzk *
-             (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
-              g.ct[k] * p_next);
-      }
-      phi[c] = phi[c] + alpha * p_cur;
-      const double rv = r[c] - alpha * qv;
-      r[c] = rv;
-      const double zv = rdiag_of(d, dxdy, dzk) * rv;
-      z[c] = zv;
-      s += zv * rv;
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-// Deferred direction update before a convergence check, p = z + beta p, with
-// z = M^-1 r of the recursively updated r (src/fpm_stairstep_mpi_3d.f90:279-283).
-__global__ __launch_bounds__(512) void k_direction_z(size_t n, double *__restrict__ p,
-                                                     const double *__restrict__ z,
-                                                     const Scalars *sc) {
-  if (sc->status != 0) return;
-  const double beta = sc->beta;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x)
-    p[c] = z[c] + beta * p[c];
-}
-
-// Residual-only pass of true_residual without the direction reset
-// (src/fpm_stairstep_mpi_3d.f90:319-328 with reset false) is k_true_residual_2
-// with reset = 0; the deferred direction update before a convergence check is
-// k_direction.
-
-}  // namespace ffpm
-
-namespace ffpm {
-
-// ---------------------------------------------------------------------------
-// Latency-tolerant variants of the fused PCG kernels: branch-free bodies
-// (neighbour indices are clamped into the array and the zero halo is applied
-// with a select, so every load is issued unconditionally) and a z-loop
-// unrolled kUnroll planes deep, so each wavefront keeps the loads of several
-// planes in flight. Same arithmetic, same bits.
-// ---------------------------------------------------------------------------
-template <int U>
-__global__ __launch_bounds__(512) void k_pcg_dir_spmv_u(Tiling t, Geom g,
-                                                        const double *__restrict__ z,
-                                                        const uint8_t *__restrict__ code,
-                                                        const double *__restrict__ pin,
-                                                        double *__restrict__ pout, int mode,
-                                                        double *partials, Scalars *sc) {
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const double beta = sc->beta;
-  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
-    const double dxdy = g.dx[i] * g.dy[j];
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 1 : 0, os = hs ? -(ptrdiff_t)nx : 0,
-                    on = hn ? (ptrdiff_t)nx : 0;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    double p_cur = dir_value(mode, beta, z, pin, c);
-    double p_prev = tc.k0 > 0 ? dir_value(mode, beta, z, pin, c - nxy) : p_cur;
-#pragma unroll U
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
-      const double p_next = dir_value(mode, beta, z, pin, cn_);
-      const double vw = dir_value(mode, beta, z, pin, c + ow);
-      const double ve = dir_value(mode, beta, z, pin, c + oe);
-      const double vs = dir_value(mode, beta, z, pin, c + os);
-      const double vn = dir_value(mode, beta, z, pin, c + on);
-      const double pw = hw ? vw : 0.0, pe = he ? ve : 0.0, ps = hs ? vs : 0.0,
-                   pn = hn ? vn : 0.0;
-      const double d = diag_of(code[c], g, i, j, k);
-      const double qraw = dxdy * g.dz[k] *
-                          (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
-                           g.ct[k] * p_next);
-      const double qv = d > 0 ? qraw : 0.0;
-      s += p_cur * qv;
-      if (mode != kDirCurrent) pout[c] = p_cur;
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-template <int U>
-__global__ __launch_bounds__(512) void k_pcg_update_u(Tiling t, Geom g,
-                                                      const double *__restrict__ p,
-                                                      const uint8_t *__restrict__ code,
-                                                      double *__restrict__ phi,
-                                                      double *__restrict__ r,
-                                                      double *__restrict__ z, double *partials,
-                                                      Scalars *sc) {
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const double alpha = sc->alpha;
-  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
-    const double dxdy = g.dx[i] * g.dy[j];
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 1 : 0, os = hs ? -(ptrdiff_t)nx : 0,
-                    on = hn ? (ptrdiff_t)nx : 0;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    double p_cur = p[c];
-    double p_prev = tc.k0 > 0 ? p[c - nxy] : p_cur;
-#pragma unroll U
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
-      const double p_next = p[cn_];
-      const double vw = p[c + ow], ve = p[c + oe], vs = p[c + os], vn = p[c + on];
-      const double pw = hw ? vw : 0.0, pe = he ? ve : 0.0, ps = hs ? vs : 0.0,
-                   pn = hn ? vn : 0.0;
-      const double d = diag_of(code[c], g, i, j, k);
-      const double dzk = g.dz[k];
-      const double qraw = dxdy * dzk *
-                          (d * p_cur - cw * pw - ce * pe - cs * ps - cn * pn - g.cb[k] * p_prev -
-                           g.ct[k] * p_next);
-      const double qv = d > 0 ? qraw : 0.0;
-      const double ph_ = phi[c];
-      const double rr = r[c];
-      phi[c] = ph_ + alpha * p_cur;
-      const double rv = rr - alpha * qv;
-      r[c] = rv;
-      const double zv = rdiag_of(d, dxdy, dzk) * rv;
-      z[c] = zv;
-      s += zv * rv;
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-template __global__ void k_pcg_dir_spmv_u<2>(Tiling, Geom, const double *, const uint8_t *,
-                                              const double *, double *, int, double *, Scalars *);
-template __global__ void k_pcg_dir_spmv_u<4>(Tiling, Geom, const double *, const uint8_t *,
-                                              const double *, double *, int, double *, Scalars *);
-template __global__ void k_pcg_update_u<2>(Tiling, Geom, const double *, const uint8_t *, double *,
-                                            double *, double *, double *, Scalars *);
-template __global__ void k_pcg_update_u<4>(Tiling, Geom, const double *, const uint8_t *, double *,
-                                            double *, double *, double *, Scalars *);
-
-// ---------------------------------------------------------------------------
-// Paired-cell variants: each lane owns two x-adjacent cells (i, i+1), i even,
-// and moves them with 16-byte loads and stores, halving the memory
-// instructions per cell (the kernels are bound by memory-instruction issue,
-// not by bytes). A wavefront covers 128 cells of a row. Requires even nx.
-// The pair's inner neighbours come from registers; the outer ones (i-1,
-// i+2) with 8-byte loads. Same arithmetic, same bits.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ TileCoord tile_coord2(const Tiling &t, int nx, int ny, int nz) {
-  TileCoord c = tile_coord(t, nx, ny, nz);
-  c.i = (c.i - threadIdx.x) * 2 + 2 * threadIdx.x;  // tiles of 128 cells along x
-  return c;
-}
-
-__device__ __forceinline__ double2 ld2(const double *__restrict__ a, size_t c) {
-  return *reinterpret_cast<const double2 *>(a + c);
-}
-
-__device__ __forceinline__ void st2(double *__restrict__ a, size_t c, double x, double y) {
-  *reinterpret_cast<double2 *>(a + c) = make_double2(x, y);
-}
-
-__device__ __forceinline__ double2 dir2(int mode, double beta, const double *__restrict__ z,
-                                        const double *__restrict__ pin, size_t c) {
-  if (mode == kDirCurrent) return ld2(pin, c);
-  const double2 zz = ld2(z, c);
-  if (mode == kDirReset) return zz;
-  const double2 pp = ld2(pin, c);
-  return make_double2(zz.x + beta * pp.x, zz.y + beta * pp.y);
-}
-
-__global__ __launch_bounds__(512) void k_pcg_dir_spmv_x2(Tiling t, Geom g,
-                                                         const double *__restrict__ z,
-                                                         const uint8_t *__restrict__ code,
-                                                         const double *__restrict__ pin,
-                                                         double *__restrict__ pout, int mode,
-                                                         double *partials, Scalars *sc) {
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const double beta = sc->beta;
-  const TileCoord tc = tile_coord2(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    const double cw0 = g.cw[i], ce0 = g.ce[i], cw1 = g.cw[i + 1], ce1 = g.ce[i + 1];
-    const double cs = g.cs[j], cn = g.cn[j];
-    const double dxdy0 = g.dx[i] * g.dy[j], dxdy1 = g.dx[i + 1] * g.dy[j];
-    const bool hw = i > 0, he = i + 2 < nx, hs = j > 0, hn = j < ny - 1;
-    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
-    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 2 : 1;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    double2 p_cur = dir2(mode, beta, z, pin, c);
-    double2 p_prev = tc.k0 > 0 ? dir2(mode, beta, z, pin, c - nxy) : p_cur;
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
-      const double2 p_next = dir2(mode, beta, z, pin, cn_);
-      const double2 vs = dir2(mode, beta, z, pin, c + os);
-      const double2 vn = dir2(mode, beta, z, pin, c + on);
-      const double vw = dir_value(mode, beta, z, pin, c + ow);
-      const double ve = dir_value(mode, beta, z, pin, c + oe);
-      const unsigned cc = *reinterpret_cast<const uint16_t *>(code + c);
-      const double pw0 = hw ? vw : 0.0, pe1 = he ? ve : 0.0;
-      const double ps0 = hs ? vs.x : 0.0, ps1 = hs ? vs.y : 0.0;
-      const double pn0 = hn ? vn.x : 0.0, pn1 = hn ? vn.y : 0.0;
-      const double d0 = diag_of(cc & 0xffu, g, i, j, k);
-      const double d1 = diag_of(cc >> 8, g, i + 1, j, k);
-      const double dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
-      const double q0 = dxdy0 * dzk *
-                        (d0 * p_cur.x - cw0 * pw0 - ce0 * p_cur.y - cs * ps0 - cn * pn0 -
-                         cb * p_prev.x - ct * p_next.x);
-      const double q1 = dxdy1 * dzk *
-                        (d1 * p_cur.y - cw1 * p_cur.x - ce1 * pe1 - cs * ps1 - cn * pn1 -
-                         cb * p_prev.y - ct * p_next.y);
-      s += p_cur.x * (d0 > 0 ? q0 : 0.0);
-      s += p_cur.y * (d1 > 0 ? q1 : 0.0);
-      if (mode != kDirCurrent) st2(pout, c, p_cur.x, p_cur.y);
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-__global__ __launch_bounds__(512) void k_pcg_update_x2(Tiling t, Geom g,
-                                                       const double *__restrict__ p,
-                                                       const uint8_t *__restrict__ code,
-                                                       double *__restrict__ phi,
-                                                       double *__restrict__ r,
-                                                       double *__restrict__ z, double *partials,
-                                                       Scalars *sc) {
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const double alpha = sc->alpha;
-  const TileCoord tc = tile_coord2(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    const double cw0 = g.cw[i], ce0 = g.ce[i], cw1 = g.cw[i + 1], ce1 = g.ce[i + 1];
-    const double cs = g.cs[j], cn = g.cn[j];
-    const double dxdy0 = g.dx[i] * g.dy[j], dxdy1 = g.dx[i + 1] * g.dy[j];
-    const bool hw = i > 0, he = i + 2 < nx, hs = j > 0, hn = j < ny - 1;
-    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
-    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? 2 : 1;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    double2 p_cur = ld2(p, c);
-    double2 p_prev = tc.k0 > 0 ? ld2(p, c - nxy) : p_cur;
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
-      const double2 p_next = ld2(p, cn_);
-      const double2 vs = ld2(p, c + os), vn = ld2(p, c + on);
-      const double vw = p[c + ow], ve = p[c + oe];
-      const unsigned cc = *reinterpret_cast<const uint16_t *>(code + c);
-      const double2 ph2 = ld2(phi, c), r2 = ld2(r, c);
-      const double pw0 = hw ? vw : 0.0, pe1 = he ? ve : 0.0;
-      const double ps0 = hs ? vs.x : 0.0, ps1 = hs ? vs.y : 0.0;
-      const double pn0 = hn ? vn.x : 0.0, pn1 = hn ? vn.y : 0.0;
-      const double d0 = diag_of(cc & 0xffu, g, i, j, k);
-      const double d1 = diag_of(cc >> 8, g, i + 1, j, k);
-      const double dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
-      const double q0r = dxdy0 * dzk *
-                         (d0 * p_cur.x - cw0 * pw0 - ce0 * p_cur.y - cs * ps0 - cn * pn0 -
-                          cb * p_prev.x - ct * p_next.x);
-      const double q1r = dxdy1 * dzk *
-                         (d1 * p_cur.y - cw1 * p_cur.x - ce1 * pe1 - cs * ps1 - cn * pn1 -
-                          cb * p_prev.y - ct * p_next.y);
-      const double q0 = d0 > 0 ? q0r : 0.0, q1 = d1 > 0 ? q1r : 0.0;
-      st2(phi, c, ph2.x + alpha * p_cur.x, ph2.y + alpha * p_cur.y);
-      const double rv0 = r2.x - alpha * q0, rv1 = r2.y - alpha * q1;
-      st2(r, c, rv0, rv1);
-      const double zv0 = rdiag_of(d0, dxdy0, dzk) * rv0;
-      const double zv1 = rdiag_of(d1, dxdy1, dzk) * rv1;
-      st2(z, c, zv0, zv1);
-      s += zv0 * rv0;
-      s += zv1 * rv1;
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-}  // namespace ffpm

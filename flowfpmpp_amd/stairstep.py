@@ -25,7 +25,7 @@ from ._lib import FFPMError, check, lib
 from .device import DeviceArray
 
 PROC_NULL = -1
-METHODS = {"pcg": 0, "jacobi": 1}
+METHODS = {"pcg": 0, "jacobi": 1, "pcg-mixed": 2}
 
 
 def _is_dev(a) -> bool:
@@ -98,7 +98,9 @@ def solve_stairstep_poisson_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, z
                                use_openacc=True, method="pcg", west=PROC_NULL, east=PROC_NULL,
                                south=PROC_NULL, north=PROC_NULL):
     """src/fpm_stairstep_mpi_3d.f90:106-330; phi is updated in place and
-    (sweeps, residual) is returned."""
+    (sweeps, residual) is returned. method "pcg-mixed" (no reference
+    counterpart) runs FP64 defect correction around an FP32 PCG and stops on
+    the same FP64 max-norm residual; sweeps then counts inner iterations."""
     del use_openacc
     if method not in METHODS:
         raise FFPMError(4, "FFPM_ERR_METHOD: stair-step method must be pcg or jacobi")
@@ -160,6 +162,14 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
 def set_stream(handle) -> None:
     """Launch on the given raw hipStream_t handle (None: the null stream)."""
     check(lib().ffpm_set_stream(C.c_void_p(0 if handle is None else int(handle))))
+
+
+def last_mixed_solve():
+    """(outer defect corrections, inner FP32 iterations) of the last
+    method="pcg-mixed" solve."""
+    o, i = C.c_int(), C.c_int()
+    check(lib().ffpm_last_mixed_solve(C.byref(o), C.byref(i)))
+    return o.value, i.value
 
 
 def timing(enable: bool, reset: bool = True) -> None:

@@ -53,8 +53,11 @@ enum {
   FFPM_ERR_DEVICE = 7         /* HIP failure: no device, launch or memory error */
 };
 
-/* Iteration selector (src/fpm_stairstep_mpi_3d.f90:125-129). */
-enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1 };
+/* Iteration selector (src/fpm_stairstep_mpi_3d.f90:125-129). PCG_MIXED has
+ * no counterpart in the reference: FP64 defect correction around an FP32 PCG
+ * (FP32 stencil and preconditioner, FP64 dots), stopping on the same FP64
+ * max-norm residual; sweeps then counts inner iterations. */
+enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1, FFPM_METHOD_PCG_MIXED = 2 };
 
 const char *ffpm_last_error(void);
 /* Sets the HIP stream (hipStream_t) every later call launches on; 0 = null stream. */
@@ -115,6 +118,10 @@ int ffpm_solve_stairstep_poisson_3d_dev(
     const double *dx, const double *dy, const double *dz, int nx, int ny, int nz, int west,
     int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
     int method, int *sweeps, double *residual);
+
+/* Outer defect corrections and inner FP32 iterations of the last
+ * FFPM_METHOD_PCG_MIXED solve on the calling thread (no reference counterpart). */
+int ffpm_last_mixed_solve(int *outer, int *inner);
 
 /* stairstep_residual_3d, src/fpm_stairstep_mpi_3d.f90:403-419 (value out) */
 int ffpm_stairstep_residual_3d(
