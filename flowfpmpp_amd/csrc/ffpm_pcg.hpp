@@ -18,15 +18,13 @@ inline GeomT<double> geom_t(const Geom &g) {
   return GeomT<double>{g.nx, g.ny, g.nz, g.cw, g.ce, g.cs, g.cn, g.cb, g.ct, g.dx, g.dy, g.dz};
 }
 
-template <typename T, int CL>
+// MODE is kDirUpdate, kDirReset or kDirCurrent.
+template <typename T, int CL, int MODE>
 __global__ void k_pcg_dir_t(Tiling t, GeomT<T> g, const T *z, const uint8_t *code, const T *pin,
-                            T *pout, int mode, double *partials, Scalars *sc);
+                            T *pout, double *partials, Scalars *sc);
 template <typename T, int CL>
 __global__ void k_pcg_upd_t(Tiling t, GeomT<T> g, const T *p, const uint8_t *code, T *x, T *r,
                             T *z, double *partials, Scalars *sc);
-template <typename T, int CL, int TY>
-__global__ void k_pcg_dir_lds_t(Tiling t, GeomT<T> g, const T *z, const uint8_t *code,
-                                const T *pin, T *pout, int mode, double *partials, Scalars *sc);
 template <typename T>
 __global__ void k_dir_t(size_t n, T *p, const T *z, const Scalars *sc);
 __global__ void k_mixed_begin(GeomT<float> g, const double *r64, const uint8_t *code,

@@ -40,40 +40,94 @@ __device__ __forceinline__ void stv(T *__restrict__ a, size_t c, const Vec<T, CL
   *reinterpret_cast<Vec<T, CL> *>(a + c) = x;
 }
 
-// Direction value (see kDir* modes): z + beta p_old, z, or p_old.
-template <typename T>
-__device__ __forceinline__ T dir1(int mode, T beta, const T *__restrict__ z,
-                                  const T *__restrict__ pin, size_t c) {
-  if (mode == kDirCurrent) return pin[c];
-  if (mode == kDirReset) return z[c];
-  return z[c] + beta * pin[c];
-}
-
-template <typename T, int CL>
-__device__ __forceinline__ Vec<T, CL> dirv(int mode, T beta, const T *__restrict__ z,
-                                           const T *__restrict__ pin, size_t c) {
-  if (mode == kDirCurrent) return ldv<T, CL>(pin, c);
-  Vec<T, CL> zz = ldv<T, CL>(z, c);
-  if (mode == kDirReset) return zz;
-  const Vec<T, CL> pp = ldv<T, CL>(pin, c);
+// Raw loads behind one direction vector (z and p_old for kDirUpdate, z for
+// kDirReset, p_old for kDirCurrent), combined only when used, so the loads of
+// a whole plane can be issued one plane ahead. The mode is a template
+// argument: with a run-time mode the compiler puts every load pair in its own
+// branch and waits for it there, which serialises the loads of a plane.
+template <int MODE, typename T, int CL>
+struct RawV {
+  Vec<T, CL> a, b;
+  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+                                       size_t c) {
+    if constexpr (MODE == kDirCurrent) {
+      a = ldv<T, CL>(pin, c);
+    } else if constexpr (MODE == kDirReset) {
+      a = ldv<T, CL>(z, c);
+    } else {
+      a = ldv<T, CL>(z, c);
+      b = ldv<T, CL>(pin, c);
+    }
+  }
+  __device__ __forceinline__ Vec<T, CL> value(T beta) const {
+    if constexpr (MODE != kDirUpdate) {
+      return a;
+    } else {
+      Vec<T, CL> v;
 #pragma unroll
-  for (int m = 0; m < CL; ++m) zz.v[m] = zz.v[m] + beta * pp.v[m];
-  return zz;
-}
+      for (int m = 0; m < CL; ++m) v.v[m] = a.v[m] + beta * b.v[m];
+      return v;
+    }
+  }
+};
+
+template <int MODE, typename T>
+struct Raw1 {
+  T a, b;
+  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+                                       size_t c) {
+    if constexpr (MODE == kDirCurrent) {
+      a = pin[c];
+    } else if constexpr (MODE == kDirReset) {
+      a = z[c];
+    } else {
+      a = z[c];
+      b = pin[c];
+    }
+  }
+  __device__ __forceinline__ T value(T beta) const {
+    if constexpr (MODE != kDirUpdate) return a;
+    else return a + beta * b;
+  }
+};
+
+// The in-plane neighbours of a lane's cells (south and north rows, the west
+// and east cells) and their neighbour codes, for one plane.
+template <int MODE, typename T, int CL>
+struct Plane {
+  RawV<MODE, T, CL> s, n;
+  Raw1<MODE, T> w, e;
+  Codes<CL> code;
+  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+                                       const uint8_t *__restrict__ cd, size_t c, ptrdiff_t os,
+                                       ptrdiff_t on, ptrdiff_t ow, ptrdiff_t oe) {
+    s.load(z, pin, c + os);
+    n.load(z, pin, c + on);
+    w.load(z, pin, c + ow);
+    e.load(z, pin, c + oe);
+    code = *reinterpret_cast<const Codes<CL> *>(cd + c);
+  }
+};
 
 // diag from the neighbour code: the fluid-neighbour coefficients summed in
 // build_operator's order (src/fpm_stairstep_mpi_3d.f90:353-370), so for
 // T = double the value is bit-identical to the stored diag.
 template <typename T>
-__device__ __forceinline__ T diag_code(unsigned code, const GeomT<T> &g, int i, int j, int k) {
+__device__ __forceinline__ T diag_code(unsigned code, T cw, T ce, T cs, T cn, T cb, T ct) {
   T d = 0;
-  if (code & kCodeW) d = d + g.cw[i];
-  if (code & kCodeE) d = d + g.ce[i];
-  if (code & kCodeS) d = d + g.cs[j];
-  if (code & kCodeN) d = d + g.cn[j];
-  if (code & kCodeB) d = d + g.cb[k];
-  if (code & kCodeT) d = d + g.ct[k];
+  if (code & kCodeW) d = d + cw;
+  if (code & kCodeE) d = d + ce;
+  if (code & kCodeS) d = d + cs;
+  if (code & kCodeN) d = d + cn;
+  if (code & kCodeB) d = d + cb;
+  if (code & kCodeT) d = d + ct;
   return d;
+}
+
+// The same from the coefficient arrays (for kernels that hold no copies).
+template <typename T>
+__device__ __forceinline__ T diag_code(unsigned code, const GeomT<T> &g, int i, int j, int k) {
+  return diag_code<T>(code, g.cw[i], g.ce[i], g.cs[j], g.cn[j], g.cb[k], g.ct[k]);
 }
 
 template <typename T>
@@ -100,11 +154,11 @@ __device__ __forceinline__ Lane lane_of(const Tiling &t, int nx, int ny, int nz)
 
 }  // namespace
 
-template <typename T, int CL>
+template <typename T, int CL, int MODE>
 __global__ __launch_bounds__(512) void k_pcg_dir_t(Tiling t, GeomT<T> g, const T *__restrict__ z,
                                                    const uint8_t *__restrict__ code,
                                                    const T *__restrict__ pin, T *__restrict__ pout,
-                                                   int mode, double *partials, Scalars *sc) {
+                                                   double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
   if (sc->status != 0) return;
   const T beta = (T)sc->beta;
@@ -124,34 +178,65 @@ __global__ __launch_bounds__(512) void k_pcg_dir_t(Tiling t, GeomT<T> g, const T
     const T cs = g.cs[j], cn = g.cn[j];
     const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
     const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
-    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? CL : CL - 1;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * l.k0);
-    Vec<T, CL> p_cur = dirv<T, CL>(mode, beta, z, pin, c);
-    Vec<T, CL> p_prev = l.k0 > 0 ? dirv<T, CL>(mode, beta, z, pin, c - nxy) : p_cur;
-    for (int k = l.k0; k < l.k1; ++k) {
-      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
-      const Vec<T, CL> p_next = dirv<T, CL>(mode, beta, z, pin, cn_);
-      const Vec<T, CL> vs = dirv<T, CL>(mode, beta, z, pin, c + os);
-      const Vec<T, CL> vn = dirv<T, CL>(mode, beta, z, pin, c + on);
-      const T vw = dir1<T>(mode, beta, z, pin, c + ow);
-      const T ve = dir1<T>(mode, beta, z, pin, c + oe);
-      const Codes<CL> cc = *reinterpret_cast<const Codes<CL> *>(code + c);
+    // West and east neighbours come from the adjacent lanes' registers by a
+    // wave shuffle; only lanes 0 and 63 load theirs (from the adjacent tile).
+    // The other lanes point their edge loads at the row start, so those load
+    // instructions touch one cache line.
+    const int lane = (int)threadIdx.x;
+    const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
+    const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : row0;
+    const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : row0;
+    // Software pipeline: the loads of plane k+1's neighbours and of plane
+    // k+2's own row are issued before plane k is computed. Two register sets
+    // alternate (the loop steps two planes), so no loaded value is copied
+    // before it arrives. Planes past the chunk are clamped to planes already
+    // loaded (their values are unused); past the top, p_next is p_cur's own
+    // plane as in the unpipelined form.
+    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+    auto at = [&](int k) { return base + nxy * (size_t)k; };
+    RawV<MODE, T, CL> own_a, own_b;
+    Plane<MODE, T, CL> pl_a, pl_b;
+    own_a.load(z, pin, at(k0));
+    Vec<T, CL> p_cur = own_a.value(beta);
+    Vec<T, CL> p_prev = p_cur;
+    if (k0 > 0) {
+      own_a.load(z, pin, at(k0 - 1));
+      p_prev = own_a.value(beta);
+    }
+    own_a.load(z, pin, at(min(k0 + 1, last)));
+    pl_a.load(z, pin, code, at(k0), os, on, ow, oe);
+    auto stage = [&](int k, const RawV<MODE, T, CL> &own_use, const Plane<MODE, T, CL> &pl,
+                     RawV<MODE, T, CL> &own_load, Plane<MODE, T, CL> &pl_load) {
+      own_load.load(z, pin, at(min(k + 2, last)));
+      pl_load.load(z, pin, code, at(min(k + 1, k1 - 1)), os, on, ow, oe);
+      const Vec<T, CL> p_next = own_use.value(beta);
+      const Vec<T, CL> vs = pl.s.value(beta), vn = pl.n.value(beta);
+      // The shuffles run on every active lane (a source lane that is masked
+      // off reads as 0), then the edge lanes take their loaded values.
+      const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
+      const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
+      const T vw = lane == 0 ? pl.w.value(beta) : from_w;
+      const T ve = lane == kTileX - 1 ? pl.e.value(beta) : from_e;
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
         const T west = m == 0 ? (hw ? vw : T(0)) : p_cur.v[m - 1];
         const T east = m == CL - 1 ? (he ? ve : T(0)) : p_cur.v[m + 1];
         const T south = hs ? vs.v[m] : T(0), north = hn ? vn.v[m] : T(0);
-        const T d = diag_code<T>(cc.b[m], g, i + m, j, k);
+        const T d = diag_code<T>(pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
         const T q = dxdy[m] * dzk *
                     (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
                      cb * p_prev.v[m] - ct * p_next.v[m]);
         s += (double)p_cur.v[m] * (double)(d > 0 ? q : T(0));
       }
-      if (mode != kDirCurrent) stv<T, CL>(pout, c, p_cur);
+      if constexpr (MODE != kDirCurrent) stv<T, CL>(pout, at(k), p_cur);
       p_prev = p_cur;
       p_cur = p_next;
-      c += nxy;
+    };
+    for (int k = k0; k < k1; k += 2) {
+      stage(k, own_a, pl_a, own_b, pl_b);
+      if (k + 1 < k1) stage(k + 1, own_b, pl_b, own_a, pl_a);
     }
   }
   double v[1] = {s};
@@ -184,157 +269,67 @@ __global__ __launch_bounds__(512) void k_pcg_upd_t(Tiling t, GeomT<T> g, const T
     const T cs = g.cs[j], cn = g.cn[j];
     const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
     const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
-    const ptrdiff_t ow = hw ? -1 : 0, oe = he ? CL : CL - 1;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * l.k0);
-    Vec<T, CL> p_cur = ldv<T, CL>(p, c);
-    Vec<T, CL> p_prev = l.k0 > 0 ? ldv<T, CL>(p, c - nxy) : p_cur;
-    for (int k = l.k0; k < l.k1; ++k) {
-      const size_t cn_ = (k + 1 < nz) ? c + nxy : c;
-      const Vec<T, CL> p_next = ldv<T, CL>(p, cn_);
-      const Vec<T, CL> vs = ldv<T, CL>(p, c + os), vn = ldv<T, CL>(p, c + on);
-      const T vw = p[c + ow], ve = p[c + oe];
-      const Codes<CL> cc = *reinterpret_cast<const Codes<CL> *>(code + c);
-      Vec<T, CL> xv = ldv<T, CL>(x, c), rv = ldv<T, CL>(r, c), zv;
+    // West and east neighbours come from the adjacent lanes' registers by a
+    // wave shuffle; only lanes 0 and 63 load theirs (from the adjacent tile).
+    // The other lanes point their edge loads at the row start, so those load
+    // instructions touch one cache line.
+    const int lane = (int)threadIdx.x;
+    const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
+    const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : row0;
+    const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : row0;
+    // Software pipeline as in k_pcg_dir_t: plane k+1's neighbours, phi and r
+    // and plane k+2's own direction row load while plane k is computed.
+    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+    auto at = [&](int k) { return base + nxy * (size_t)k; };
+    struct Set {
+      Vec<T, CL> p_next, x, r;
+      Plane<kDirCurrent, T, CL> pl;
+    };
+    Set sa, sb;
+    Vec<T, CL> p_cur = ldv<T, CL>(p, at(k0));
+    Vec<T, CL> p_prev = k0 > 0 ? ldv<T, CL>(p, at(k0 - 1)) : p_cur;
+    sa.p_next = ldv<T, CL>(p, at(min(k0 + 1, last)));
+    sa.pl.load(p, p, code, at(k0), os, on, ow, oe);
+    sa.x = ldv<T, CL>(x, at(k0));
+    sa.r = ldv<T, CL>(r, at(k0));
+    auto stage = [&](int k, const Set &u, Set &ld) {
+      const int kn = min(k + 1, k1 - 1);
+      ld.p_next = ldv<T, CL>(p, at(min(k + 2, last)));
+      ld.pl.load(p, p, code, at(kn), os, on, ow, oe);
+      ld.x = ldv<T, CL>(x, at(kn));
+      ld.r = ldv<T, CL>(r, at(kn));
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
+      Vec<T, CL> xv = u.x, rv = u.r, zv;
+      const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
+      const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
+      const T vw = lane == 0 ? u.pl.w.a : from_w;
+      const T ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
         const T west = m == 0 ? (hw ? vw : T(0)) : p_cur.v[m - 1];
         const T east = m == CL - 1 ? (he ? ve : T(0)) : p_cur.v[m + 1];
-        const T south = hs ? vs.v[m] : T(0), north = hn ? vn.v[m] : T(0);
-        const T d = diag_code<T>(cc.b[m], g, i + m, j, k);
+        const T south = hs ? u.pl.s.a.v[m] : T(0), north = hn ? u.pl.n.a.v[m] : T(0);
+        const T d = diag_code<T>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
         const T qr = dxdy[m] * dzk *
                      (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
-                      cb * p_prev.v[m] - ct * p_next.v[m]);
+                      cb * p_prev.v[m] - ct * u.p_next.v[m]);
         const T q = d > 0 ? qr : T(0);
         xv.v[m] = xv.v[m] + alpha * p_cur.v[m];
         rv.v[m] = rv.v[m] - alpha * q;
         zv.v[m] = rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m];
         s += (double)zv.v[m] * (double)rv.v[m];
       }
-      stv<T, CL>(x, c, xv);
-      stv<T, CL>(r, c, rv);
-      stv<T, CL>(z, c, zv);
+      stv<T, CL>(x, at(k), xv);
+      stv<T, CL>(r, at(k), rv);
+      stv<T, CL>(z, at(k), zv);
       p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
+      p_cur = u.p_next;
+    };
+    for (int k = k0; k < k1; k += 2) {
+      stage(k, sa, sb);
+      if (k + 1 < k1) stage(k + 1, sb, sa);
     }
-  }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, gridDim.x);
-}
-
-// LDS-tiled direction + SpMV: the same results as k_pcg_dir_t with fewer
-// global memory instructions. A workgroup of ty wavefronts owns ty rows of
-// 64*CL cells; for every plane the new direction of its rows, the two halo
-// rows and the two edge columns is built once, exchanged through LDS, and each
-// lane reads its x-y neighbours from there (z-neighbours stay in registers).
-// Two LDS planes alternate, so one barrier per plane suffices: plane k+1 is
-// built while plane k is consumed.
-template <typename T, int CL, int TY>
-__global__ __launch_bounds__(64 * TY) void k_pcg_dir_lds_t(
-    Tiling t, GeomT<T> g, const T *__restrict__ z, const uint8_t *__restrict__ code,
-    const T *__restrict__ pin, T *__restrict__ pout, int mode, double *partials, Scalars *sc) {
-  constexpr int W = 64 * CL + 2;  // LDS row: west edge, 64*CL cells, east edge
-  __shared__ T tile[2][TY + 2][W];
-  __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  const T beta = (T)sc->beta;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const TileCoord tc = tile_coord(t, nx, ny, nz);
-  const int lane = threadIdx.x, w = threadIdx.y;
-  const int i0 = (tc.i - lane) * CL;  // first cell of the tile row
-  const int j0 = tc.j - w;
-  const int i = i0 + lane * CL, j = tc.j;
-  const bool active = tc.valid && i < nx && j < ny;
-  const size_t nxy = (size_t)nx * ny;
-  // The halo rows are built by the first and last wavefront, the edge
-  // columns by lanes of the first wavefront.
-  const int hrow = (w == 0) ? j0 - 1 : (w == TY - 1 ? j0 + TY : -2);
-  const int hslot = (w == 0) ? 0 : TY + 1;
-  const bool hrow_in = tc.valid && hrow >= 0 && hrow < ny && i < nx;
-  // Only the owned rows need edge columns (south/north reads stay inside).
-  int erow = -1, ecol = -1, eslot_r = 0, eslot_c = 0;
-  if (w == 0 && lane < TY) {
-    erow = j0 + lane; ecol = i0 - 1; eslot_r = lane + 1; eslot_c = 0;
-  } else if (w == 0 && lane >= 32 && lane < 32 + TY) {
-    erow = j0 + lane - 32; ecol = i0 + 64 * CL; eslot_r = lane - 31; eslot_c = W - 1;
-  }
-  const bool edge_in = tc.valid && erow >= 0 && erow < ny && ecol >= 0 && ecol < nx;
-  const bool edge_lane = erow != -1;
-  auto plane_base = [&](int jj, int kk) -> size_t {
-    return (size_t)i + (size_t)nx * ((size_t)jj + (size_t)ny * kk);
-  };
-  auto put_row = [&](int b, int slot, const Vec<T, CL> &v) {
-#pragma unroll
-    for (int m = 0; m < CL; ++m) tile[b][slot][1 + lane * CL + m] = v.v[m];
-  };
-  Vec<T, CL> zero;
-#pragma unroll
-  for (int m = 0; m < CL; ++m) zero.v[m] = T(0);
-  // Builds plane kk of buffer b from this thread's share of the loads.
-  auto build = [&](int b, int kk, const Vec<T, CL> &own) {
-    put_row(b, w + 1, active ? own : zero);
-    if (hrow != -2)
-      put_row(b, hslot, hrow_in ? dirv<T, CL>(mode, beta, z, pin, plane_base(hrow, kk)) : zero);
-    if (edge_lane)
-      tile[b][eslot_r][eslot_c] =
-          edge_in ? dir1<T>(mode, beta, z, pin,
-                            (size_t)ecol + (size_t)nx * ((size_t)erow + (size_t)ny * kk))
-                  : T(0);
-  };
-  double s = 0.0;
-  const int k0 = tc.k0, k1 = tc.valid ? tc.k1 : tc.k0;
-  T cw[CL], ce[CL], dxdy[CL];
-  T cs = 0, cn = 0;
-  if (active) {
-#pragma unroll
-    for (int m = 0; m < CL; ++m) {
-      cw[m] = g.cw[i + m];
-      ce[m] = g.ce[i + m];
-      dxdy[m] = g.dx[i + m] * g.dy[j];
-    }
-    cs = g.cs[j];
-    cn = g.cn[j];
-  }
-  const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
-  Vec<T, CL> p_cur = zero, p_prev = zero;
-  if (k0 < k1) {
-    if (active) {
-      const size_t c = plane_base(j, k0);
-      p_cur = dirv<T, CL>(mode, beta, z, pin, c);
-      p_prev = k0 > 0 ? dirv<T, CL>(mode, beta, z, pin, c - nxy) : p_cur;
-    }
-    build(0, k0, p_cur);
-  }
-  __syncthreads();
-  for (int k = k0; k < k1; ++k) {
-    const int b = (k - k0) & 1;
-    Vec<T, CL> p_next = p_cur;
-    if (active && k + 1 < nz) p_next = dirv<T, CL>(mode, beta, z, pin, plane_base(j, k + 1));
-    if (k + 1 < k1) build(b ^ 1, k + 1, p_next);
-    if (active) {
-      const size_t c = plane_base(j, k);
-      const Codes<CL> cc = *reinterpret_cast<const Codes<CL> *>(code + c);
-      const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
-      const int col = 1 + lane * CL;
-#pragma unroll
-      for (int m = 0; m < CL; ++m) {
-        const T west = m > 0 ? p_cur.v[m - 1] : (hw ? tile[b][w + 1][col - 1] : T(0));
-        const T east = m < CL - 1 ? p_cur.v[m + 1] : (he ? tile[b][w + 1][col + CL] : T(0));
-        const T south = hs ? tile[b][w][col + m] : T(0);
-        const T north = hn ? tile[b][w + 2][col + m] : T(0);
-        const T d = diag_code<T>(cc.b[m], g, i + m, j, k);
-        const T q = dxdy[m] * dzk *
-                    (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
-                     cb * p_prev.v[m] - ct * p_next.v[m]);
-        s += (double)p_cur.v[m] * (double)(d > 0 ? q : T(0));
-      }
-      if (mode != kDirCurrent) stv<T, CL>(pout, c, p_cur);
-    }
-    p_prev = p_cur;
-    p_cur = p_next;
-    __syncthreads();
   }
   double v[1] = {s};
   block_sum<1>(v, lds);
@@ -392,9 +387,14 @@ __global__ void k_to_float(int n, const double *__restrict__ a, float *__restric
   if (t < n) b[t] = (float)a[t];
 }
 
+#define FFPM_INSTANTIATE_DIR(T, CL, MODE)                                                       \
+  template __global__ void k_pcg_dir_t<T, CL, MODE>(Tiling, GeomT<T>, const T *,               \
+                                                    const uint8_t *, const T *, T *, double *,  \
+                                                    Scalars *);
 #define FFPM_INSTANTIATE(T, CL)                                                                 \
-  template __global__ void k_pcg_dir_t<T, CL>(Tiling, GeomT<T>, const T *, const uint8_t *,    \
-                                              const T *, T *, int, double *, Scalars *);       \
+  FFPM_INSTANTIATE_DIR(T, CL, kDirUpdate)                                                       \
+  FFPM_INSTANTIATE_DIR(T, CL, kDirReset)                                                        \
+  FFPM_INSTANTIATE_DIR(T, CL, kDirCurrent)                                                      \
   template __global__ void k_pcg_upd_t<T, CL>(Tiling, GeomT<T>, const T *, const uint8_t *,    \
                                               T *, T *, T *, double *, Scalars *);
 FFPM_INSTANTIATE(double, 1)
@@ -403,21 +403,7 @@ FFPM_INSTANTIATE(float, 1)
 FFPM_INSTANTIATE(float, 2)
 FFPM_INSTANTIATE(float, 4)
 #undef FFPM_INSTANTIATE
-#define FFPM_INSTANTIATE_LDS(T, CL, TY)                                                        \
-  template __global__ void k_pcg_dir_lds_t<T, CL, TY>(Tiling, GeomT<T>, const T *,               \
-                                                      const uint8_t *, const T *, T *, int,      \
-                                                      double *, Scalars *);
-FFPM_INSTANTIATE_LDS(double, 1, 4)
-FFPM_INSTANTIATE_LDS(double, 1, 8)
-FFPM_INSTANTIATE_LDS(double, 2, 4)
-FFPM_INSTANTIATE_LDS(double, 2, 8)
-FFPM_INSTANTIATE_LDS(float, 1, 4)
-FFPM_INSTANTIATE_LDS(float, 1, 8)
-FFPM_INSTANTIATE_LDS(float, 2, 4)
-FFPM_INSTANTIATE_LDS(float, 2, 8)
-FFPM_INSTANTIATE_LDS(float, 4, 4)
-FFPM_INSTANTIATE_LDS(float, 4, 8)
-#undef FFPM_INSTANTIATE_LDS
+#undef FFPM_INSTANTIATE_DIR
 template __global__ void k_dir_t<double>(size_t, double *, const double *, const Scalars *);
 template __global__ void k_dir_t<float>(size_t, float *, const float *, const Scalars *);
 

@@ -73,8 +73,11 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   f.n = (size_t)nx * ny * nz;
   f.tiling = make_tiling(nx, ny, nz, 0);
   size_t nb = (size_t)max_blocks_for(f.tiling, f.n);
+  // Eight-row workgroups measured best for the pipelined PCG kernels on a
+  // 512x384x256 frame (2, 4 and 8 rows within 10%); FFPM_TILE_Y overrides.
+  const int pcg_rows = std::getenv("FFPM_TILE_Y") ? 0 : 8;
   for (int cl : {1, 2, 4}) {
-    f.tl[cl] = make_tiling(nx, ny, nz, 0, 0, kTileX * cl);
+    f.tl[cl] = make_tiling(nx, ny, nz, 0, pcg_rows, kTileX * cl);
     if ((size_t)f.tl[cl].blocks > nb) nb = (size_t)f.tl[cl].blocks;
   }
   double *coef = (double *)ws.get(S_COEF, sizeof(double) * 2 * (nx + ny + nz), &st);
@@ -247,12 +250,6 @@ int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, 
   return OK;
 }
 
-// LDS-tiled direction kernel (FFPM_DIR_LDS=1) or the cache-only one.
-bool dir_lds() {
-  const char *e = std::getenv("FFPM_DIR_LDS");
-  return e && std::strcmp(e, "1") == 0;
-}
-
 // Cells per lane of the fused kernels: 16-byte vectors where nx allows.
 template <typename T>
 int cells_per_lane(int nx) {
@@ -271,30 +268,24 @@ int launch_dir(const Frame &f, int cl, const GeomT<T> &g, const T *z, const T *p
   const dim3 blk(kTileX, t.ty);
   {
     TimedLaunch tl(sizeof(T) == 8 ? "pcg_dir_spmv" : "pcg32_dir_spmv");
-    if (dir_lds() && (t.ty == 4 || t.ty == 8)) {
-      // The workgroup shape is fixed by the template; the tiling's rows per
-      // workgroup must match it.
-      auto go = [&](auto kern) {
-        kern<<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, mode, f.partials, f.sc);
-      };
-      if (cl == 4) {
-        if constexpr (sizeof(T) == 4) {
-          if (t.ty == 8) go(k_pcg_dir_lds_t<T, 4, 8>); else go(k_pcg_dir_lds_t<T, 4, 4>);
-        }
-      } else if (cl == 2) {
-        if (t.ty == 8) go(k_pcg_dir_lds_t<T, 2, 8>); else go(k_pcg_dir_lds_t<T, 2, 4>);
-      } else {
-        if (t.ty == 8) go(k_pcg_dir_lds_t<T, 1, 8>); else go(k_pcg_dir_lds_t<T, 1, 4>);
-      }
-    } else if (cl == 4)
-      k_pcg_dir_t<T, (sizeof(T) == 4 ? 4 : 2)><<<t.blocks, blk, 0, stream()>>>(
-          t, g, z, f.code, pin, pout, mode, f.partials, f.sc);
+    auto go = [&](auto kern) {
+      kern<<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, f.partials, f.sc);
+    };
+    auto by_mode = [&](auto k_update, auto k_reset, auto k_current) {
+      if (mode == kDirUpdate) go(k_update);
+      else if (mode == kDirReset) go(k_reset);
+      else go(k_current);
+    };
+    constexpr int CLV = sizeof(T) == 4 ? 4 : 2;
+    if (cl == 4)
+      by_mode(k_pcg_dir_t<T, CLV, kDirUpdate>, k_pcg_dir_t<T, CLV, kDirReset>,
+              k_pcg_dir_t<T, CLV, kDirCurrent>);
     else if (cl == 2)
-      k_pcg_dir_t<T, 2><<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, mode,
-                                                        f.partials, f.sc);
+      by_mode(k_pcg_dir_t<T, 2, kDirUpdate>, k_pcg_dir_t<T, 2, kDirReset>,
+              k_pcg_dir_t<T, 2, kDirCurrent>);
     else
-      k_pcg_dir_t<T, 1><<<t.blocks, blk, 0, stream()>>>(t, g, z, f.code, pin, pout, mode,
-                                                        f.partials, f.sc);
+      by_mode(k_pcg_dir_t<T, 1, kDirUpdate>, k_pcg_dir_t<T, 1, kDirReset>,
+              k_pcg_dir_t<T, 1, kDirCurrent>);
   }
   FFPM_LAUNCHED("k_pcg_dir_t");
   return finish(f, FIN_PQ, 1, t.blocks);

@@ -344,33 +344,3 @@ def test_mixed_precision_reaches_fp64_tolerance(orc, nx_mult):
     for a, b in zip(gm[3:5], go[3:5]):
         assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
 
-
-@pytest.mark.parametrize("grid", [(200, 37, 19), (150, 36, 21), (45, 35, 23)])
-@pytest.mark.parametrize("rows", ["4", "8"])
-def test_lds_direction_kernel_is_bit_identical(monkeypatch, grid, rows):
-    """The LDS-tiled direction kernel computes every value with the same
-    operands and order as the cache-only kernel, so whole solves (FP64 PCG
-    for a fixed iteration count and mixed precision to convergence) are
-    bit-identical. Grids cover partial tiles in x and y and each cells-per-lane
-    width (nx a multiple of 4, of 2 only, odd)."""
-    nx, ny, nz = grid
-    print(f"case: LDS direction kernel {nx}x{ny}x{nz}, {rows} rows", flush=True)
-    p = W.stretched_frame(nx, ny, nz, [W.Sphere((0.0, 0.0, 0.0), 0.9)], seed=3,
-                          with_fields=False)
-    rhs, bg, faces = W.influence_inputs(p, 2)
-    S.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
-    monkeypatch.setenv("FFPM_TILE_Y", rows)
-    out = {}
-    for lds in ("0", "1"):
-        monkeypatch.setenv("FFPM_DIR_LDS", lds)
-        a = p.phi0()
-        ra = S.solve_stairstep_poisson_3d(a, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc, p.dx,
-                                          p.dy, p.dz, -1.0, 120, 40, method="pcg")
-        b = p.phi0()
-        rb = S.solve_stairstep_poisson_3d(b, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc, p.dx,
-                                          p.dy, p.dz, 1e-10, 100000, 25, method="pcg-mixed")
-        out[lds] = (a, ra, b, rb, S.last_mixed_solve())
-    x, y = out["0"], out["1"]
-    assert x[1] == y[1] and x[3] == y[3] and x[4] == y[4]
-    assert np.array_equal(x[0], y[0]) and np.array_equal(x[2], y[2])
-    assert x[3][1] <= 1e-10
