@@ -100,6 +100,87 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=25.0):
     }
 
 
+def bench_batch(args):
+    """64 uneven frames (grids from 64^3 to 160x128x96, 1-3 bodies each) x 3
+    force directions = 192 FP64 PCG solves to the tolerance in one batched
+    solve (shared operator per frame, per-solve masks, HIP-graph chunks,
+    autotuned launch shape). A step solves all 192 from phi = 0."""
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd import workloads as W
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    T = DeviceArray.from_host
+    t_gen = time.perf_counter()
+    frames = []
+    for i in range(64):
+        p = W.batch_instance(i)
+        solid = T(p.solid)
+        axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+        dirs = []
+        for d in range(3):
+            rhs, bg, faces = W.influence_inputs(p, d)
+            d_rhs, d_faces = T(rhs), [T(f) for f in faces]
+            S.enforce_stairstep_compatibility_3d(d_rhs, solid, bg, *d_faces, *axes[3:])
+            dirs.append(S.BatchProblem(T(p.phi0()), d_rhs, solid, bg, *d_faces, *axes))
+        frames.append((p, dirs))
+    probs = [q for _, dirs in frames for q in dirs]
+    cells = sum(p.cells for p, _ in frames) * 3
+    synchronize()
+    log(f"64 frames x 3 directions resident: {cells} cells, {time.perf_counter() - t_gen:.1f} s")
+    rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20)
+    log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
+
+    def step():
+        for q in probs:
+            q.phi.zero_()
+        S.solve_stairstep_batch(probs, args.tol, args.max_iters, args.check, rows=rows,
+                                zchunk=zchunk)
+
+    for _ in range(args.warmup):
+        step()
+    synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        synchronize()
+        times.append(time.perf_counter() - t0)
+    ms_per_step = 1e3 * sum(times) / len(times)
+    solve_ms = np.array([q.solve_ms for q in probs])
+    sweeps = np.array([q.sweeps for q in probs])
+    res = np.array([q.residual for q in probs])
+    log(f"batch: {ms_per_step:.1f} ms for 192 solves; iterations {sweeps.min()}-{sweeps.max()}")
+    out = {
+        "metric": "batched solves/s: 64 uneven frames x 3 directions, FP64 PCG to 1e-10",
+        "value": round(192 * 1e3 / ms_per_step, 3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "64 frames (seeds 4000-4063), grids 64^3/96x80x64/128x96x64/"
+                               "160x128x96, 1-3 bodies, 3 directions each; one batched "
+                               "FP64 PCG solve, check 100",
+                   "cells": cells, "tolerance": args.tol, "rows": rows, "zchunk": zchunk},
+        "solve_ms_p50": round(float(np.percentile(solve_ms, 50)), 3),
+        "solve_ms_p99": round(float(np.percentile(solve_ms, 99)), 3),
+        "iterations_min_max": [int(sweeps.min()), int(sweeps.max())],
+        "max_residual": float(res.max()),
+        "converged": bool((res <= args.tol).all()),
+        "autotune_ms_per_iteration": round(ms_it, 4),
+    }
+    if args.sequential:
+        t0 = time.perf_counter()
+        for (p, dirs) in frames:
+            for q in dirs:
+                q.phi.zero_()
+                S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
+                                             *q.axes, args.tol, args.max_iters, args.check,
+                                             method="pcg")
+        synchronize()
+        seq = time.perf_counter() - t0
+        out["sequential_ms"] = round(seq * 1e3, 3)
+        out["batch_speedup_vs_sequential"] = round(seq * 1e3 / ms_per_step, 3)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +192,11 @@ def main():
     ap.add_argument("--check", type=int, default=100)
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="frame", choices=["frame", "batch64"],
+                    help="frame: the 3-direction solve of one frame (the headline); batch64: "
+                         "64 uneven frames x 3 directions in one batched solve")
+    ap.add_argument("--sequential", action="store_true",
+                    help="batch64: also time the 192 solves one after another")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole frame lives on one MI355X and this build "
@@ -119,6 +205,9 @@ def main():
 
     from flowfpmpp_amd import stairstep as S
     from flowfpmpp_amd.device import DeviceArray, synchronize
+
+    if args.workload == "batch64":
+        return bench_batch(args)
 
     t_gen = time.perf_counter()
     p = frame(tuple(args.grid))

@@ -18,6 +18,7 @@ SOURCES = [
     "ffpm_pcg_kernels.hip",
     "ffpm_stairstep_host.hip",
     "ffpm_laplace.hip",
+    "ffpm_batch.hip",
 ]
 
 # -ffp-contract=off keeps every expression's rounding identical to the

@@ -1,0 +1,698 @@
+// Batched stair-step PCG: many independent solves advance together, one grid
+// per kernel for the whole batch. A batch holds uneven frames, and the three
+// force-direction right-hand sides of one frame share that frame's operator
+// (neighbour codes, diagonals and 1-D coefficients are built once).
+//
+// Every workgroup of a batched kernel looks up its solve and its index inside
+// that solve's own launch shape and runs the same device body as the
+// standalone kernel (ffpm_bodies.hpp, ffpm_pcg_body.hpp), so each solve
+// computes bit-for-bit what its standalone solve computes. The convergence
+// logic of pcg_iterate (src/fpm_stairstep_mpi_3d.f90:235-293, as in the
+// standalone fused iteration) runs per solve on the device, with a per-solve
+// active mask: a converged, stalled or failed solve turns every later launch
+// of its workgroups into a no-op. One chunk of check_interval iterations plus
+// the convergence check is captured once as a HIP graph and replayed until no
+// solve is active; the host reads one counter per replay.
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ffpm.h"
+#include "ffpm_bodies.hpp"
+#include "ffpm_pcg_body.hpp"
+#include "ffpm_runtime.hpp"
+
+namespace ffpm {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// Device descriptor of one solve.
+struct BSolve {
+  GeomT<double> g;  // 1-D coefficients and widths (shared by the frame's solves)
+  Geom geom;        // the same, for the true-residual body
+  Tiling tl;        // launch shape of the PCG kernels (CL cells per lane)
+  Tiling tt;        // launch shape of the true-residual kernel
+  size_t n;         // cells
+  int eb;           // workgroups of the elementwise kernels
+  const uint8_t *code;
+  const double *src, *diag, *rdiag;
+  double *x, *r, *z, *p[2];
+  double *partials;
+  Scalars *sc;
+};
+
+// Per-solve state of the convergence logic (the host loop of pcg_iterate).
+struct BState {
+  int active, it, sweeps, restarts, stalled, mode, cur, n_chunk;
+  int need_tr;   // 0 none, 1 after a breakdown restart, 2 convergence check, 3 initial
+  int need_dir;  // deferred direction update before the check
+  int finished_at;
+  double res, best;
+};
+
+// Workgroup -> (solve, index within the solve's launch) tables of one grid.
+struct Grid {
+  int *solve = nullptr, *local = nullptr;
+  int blocks = 0;
+};
+
+// ------------------------------------------------------------ kernels ------
+template <int CL>
+__global__ __launch_bounds__(512) void k_b_dir(const BSolve *solves, Grid grid,
+                                               const BState *state, int slot) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!st.active || slot >= st.n_chunk) return;
+  const BSolve &d = solves[s];
+  const int lb = grid.local[blockIdx.x];
+  const double *pin = d.p[st.cur];
+  double *pout = d.p[st.cur ^ 1];
+  if (st.mode == kDirUpdate)
+    pcg_dir_body<double, CL, kDirUpdate>(d.tl, lb, d.tl.blocks, d.g, d.z, d.code, pin, pout,
+                                         d.partials, d.sc, lds);
+  else if (st.mode == kDirReset)
+    pcg_dir_body<double, CL, kDirReset>(d.tl, lb, d.tl.blocks, d.g, d.z, d.code, pin, pout,
+                                        d.partials, d.sc, lds);
+  else
+    pcg_dir_body<double, CL, kDirCurrent>(d.tl, lb, d.tl.blocks, d.g, d.z, d.code, pin, pout,
+                                          d.partials, d.sc, lds);
+}
+
+template <int CL>
+__global__ __launch_bounds__(512) void k_b_upd(const BSolve *solves, Grid grid,
+                                               const BState *state, int slot) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!st.active || slot >= st.n_chunk) return;
+  const BSolve &d = solves[s];
+  pcg_upd_body<double, CL>(d.tl, grid.local[blockIdx.x], d.tl.blocks, d.g, d.p[st.cur], d.code,
+                           d.x, d.r, d.z, d.partials, d.sc, lds);
+}
+
+// One workgroup per solve: the finish of the preceding reduction, plus the
+// bookkeeping the standalone host loop does between launches (which direction
+// buffer is current; the mode of the next direction kernel).
+__global__ __launch_bounds__(256) void k_b_finish(const BSolve *solves, BState *state, int op,
+                                                  int slot) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = blockIdx.x;
+  const BState st = state[s];
+  const BSolve &d = solves[s];
+  int nblocks, nslots = 1;
+  if (op == FIN_PQ || op == FIN_RZNEW) {
+    if (!st.active || slot >= st.n_chunk) return;
+    nblocks = d.tl.blocks;
+  } else {
+    if (!st.need_tr) return;
+    nblocks = op == FIN_TR1 ? d.tt.blocks : d.eb;
+    if (op == FIN_TR1) nslots = 3;
+  }
+  finish_body(op, nslots, d.partials, nblocks, d.sc, lds);
+  if (threadIdx.x == 0) {
+    if (op == FIN_PQ && st.mode != kDirCurrent) state[s].cur = st.cur ^ 1;
+    if (op == FIN_RZNEW) state[s].mode = kDirUpdate;
+  }
+}
+
+// After a chunk: the host logic of the standalone loop that follows a chunk
+// (iteration count, breakdown restart, negative rz exit, check points).
+__global__ void k_b_check_begin(const BSolve *solves, BState *state, int count, int ci,
+                                int max_sweeps) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  st.need_tr = 0;
+  st.need_dir = 0;
+  if (st.active) {
+    Scalars *sc = solves[s].sc;
+    const int iters = sc->iters;
+    st.sweeps += iters;
+    st.it += iters;
+    if (iters > 0) st.restarts = 0;
+    if (sc->status == kNegative) {
+      st.active = 0;
+    } else if (sc->status == kBreakdown) {
+      st.restarts += 1;
+      if (st.restarts > 1) {
+        st.active = 0;
+      } else {
+        sc->status = 0;
+        sc->iters = 0;
+        st.need_tr = 1;
+        st.mode = kDirReset;
+      }
+    } else if (st.it % ci == 0 || st.sweeps == max_sweeps) {
+      st.need_dir = 1;
+      st.need_tr = 2;
+      st.mode = kDirCurrent;
+    }
+  }
+  state[s] = st;
+}
+
+__global__ __launch_bounds__(256) void k_b_dir_final(const BSolve *solves, Grid grid,
+                                                     const BState *state) {
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!st.need_dir) return;
+  const BSolve &d = solves[s];
+  dir_update_body<double>(d.n, grid.local[blockIdx.x], d.eb, d.p[st.cur], d.z, d.sc);
+}
+
+__global__ __launch_bounds__(512) void k_b_tr1(const BSolve *solves, Grid grid,
+                                               const BState *state) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  if (!state[s].need_tr) return;
+  const BSolve &d = solves[s];
+  true_residual_1_body(d.tt, grid.local[blockIdx.x], d.tt.blocks, d.geom, d.x, d.src, d.diag,
+                       d.r, d.partials, lds);
+}
+
+__global__ __launch_bounds__(256) void k_b_tr2(const BSolve *solves, Grid grid,
+                                               const BState *state) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  if (!state[s].need_tr) return;
+  const BSolve &d = solves[s];
+  true_residual_2_body(d.n, grid.local[blockIdx.x], d.eb, d.r, d.rdiag, d.diag, nullptr, 0, d.z,
+                       d.partials, d.sc, lds);
+}
+
+// After the true residual: stall test, loop condition, next chunk length.
+__global__ void k_b_check_end(const BSolve *solves, BState *state, int count, int ci,
+                              int max_sweeps, double tolerance, const int *replay) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  const int was_active = st.active;
+  Scalars *sc = solves[s].sc;
+  if (st.need_tr) {
+    st.res = sc->res;
+    if (st.need_tr == 3) {
+      st.best = st.res;
+    } else if (st.need_tr == 2) {
+      if (st.res < 0.99 * st.best) {
+        st.best = st.res;
+        st.stalled = 0;
+      } else {
+        st.stalled += 1;
+      }
+      if (st.stalled >= 50 || !(st.res == st.res)) st.active = 0;
+    }
+  }
+  if (st.active && !(st.sweeps < max_sweeps && st.res > tolerance)) st.active = 0;
+  if (st.active) {
+    int n = ci - st.it % ci;
+    if (max_sweeps - st.sweeps < n) n = max_sweeps - st.sweeps;
+    st.n_chunk = n;
+    sc->iters = 0;
+  }
+  if (was_active && !st.active) st.finished_at = *replay;
+  st.need_tr = 0;
+  state[s] = st;
+}
+
+// Active solves after a replay; advances the replay counter.
+__global__ __launch_bounds__(256) void k_b_count(const BState *state, int count, int *replay,
+                                                 int *n_active) {
+  __shared__ int part[256];
+  int a = 0;
+  for (int s = threadIdx.x; s < count; s += blockDim.x) a += state[s].active;
+  part[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < (int)blockDim.x; ++w) t += part[w];
+    *n_active = t;
+    *replay = *replay + 1;
+  }
+}
+
+// Initial state of every solve (before the first true residual).
+__global__ void k_b_init(BState *state, int count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st{};
+  st.active = 1;
+  st.mode = kDirReset;
+  st.need_tr = 3;
+  st.finished_at = 0;
+  state[s] = st;
+}
+
+__global__ void k_b_zero_int(int *p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0;
+}
+
+__global__ void k_b_zero_scalars(Scalars *sc) {
+  if (threadIdx.x == 0) *sc = Scalars{};
+}
+
+// ------------------------------------------------------------- host --------
+int cells_per_lane_f64(int nx) {
+  const char *e = std::getenv("FFPM_X2");
+  if (e && std::strcmp(e, "0") == 0) return 1;
+  return nx % 2 == 0 ? 2 : 1;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bump allocation inside one device arena.
+struct Arena {
+  char *base = nullptr;
+  size_t used = 0;
+  template <typename P>
+  P *take(size_t bytes) {
+    P *p = reinterpret_cast<P *>(base + used);
+    used += align256(bytes);
+    return p;
+  }
+};
+
+bool same_operator(const ffpm_stairstep_problem &a, const ffpm_stairstep_problem &b) {
+  return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.solid == b.solid && a.xc == b.xc &&
+         a.yc == b.yc && a.zc == b.zc && a.dx == b.dx && a.dy == b.dy && a.dz == b.dz;
+}
+
+// Launch shape of the PCG kernels for one solve: the standalone solve's
+// defaults unless the caller (or the autotuner) fixes rows and z chunk.
+Tiling pcg_tiling(int nx, int ny, int nz, int cl, int rows, int zchunk) {
+  const int r = rows > 0 ? rows : (std::getenv("FFPM_TILE_Y") ? 0 : 8);
+  return make_tiling(nx, ny, nz, zchunk > 0 ? zchunk : 0, r, kTileX * cl);
+}
+
+struct Batch {
+  int count = 0;
+  std::vector<BSolve> h;    // host copies of the descriptors
+  std::vector<int> cl;      // cells per lane of each solve
+  std::vector<int> owner;   // operator owner (first solve of the frame)
+  BSolve *d_solves = nullptr;
+  BState *d_state = nullptr;
+  int *d_counters = nullptr;  // [0] replay, [1] active count
+  Grid g_pcg[3], g_tr, g_ew;  // g_pcg[CL]
+  int *d_isolated = nullptr;
+};
+
+int build_grid(Arena &a, Grid &g, const std::vector<int> &blocks, const std::vector<int> &use,
+               std::vector<int> &h_solve, std::vector<int> &h_local) {
+  h_solve.clear();
+  h_local.clear();
+  for (size_t s = 0; s < blocks.size(); ++s) {
+    if (!use[s]) continue;
+    for (int b = 0; b < blocks[s]; ++b) {
+      h_solve.push_back((int)s);
+      h_local.push_back(b);
+    }
+  }
+  g.blocks = (int)h_solve.size();
+  if (g.blocks == 0) return OK;
+  g.solve = a.take<int>(sizeof(int) * g.blocks);
+  g.local = a.take<int>(sizeof(int) * g.blocks);
+  FFPM_HIP(hipMemcpyAsync(g.solve, h_solve.data(), sizeof(int) * g.blocks,
+                          hipMemcpyHostToDevice, stream()));
+  FFPM_HIP(hipMemcpyAsync(g.local, h_local.data(), sizeof(int) * g.blocks,
+                          hipMemcpyHostToDevice, stream()));
+  return OK;
+}
+
+// Builds every solve's operator and state; arrays live in one arena kept in
+// the workspace (grow-only).
+int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zchunk) {
+  B.count = count;
+  B.h.assign(count, BSolve{});
+  B.cl.assign(count, 1);
+  B.owner.assign(count, -1);
+  for (int s = 0; s < count; ++s) {
+    const ffpm_stairstep_problem &p = P[s];
+    if (p.nx < 1 || p.ny < 1 || p.nz < 2) {
+      set_error(ERR_SHAPE, "stair-step solve needs nz>=2 and nonempty ranks");
+      return ERR_SHAPE;
+    }
+    if (!p.phi || !p.rhs || !p.solid || !p.xlo || !p.xhi || !p.ylo || !p.yhi || !p.zlo ||
+        !p.zhi || !p.xc || !p.yc || !p.zc || !p.dx || !p.dy || !p.dz) {
+      set_error(ERR_ARGUMENT, "null array argument");
+      return ERR_ARGUMENT;
+    }
+    B.owner[s] = s;
+    for (int o = 0; o < s; ++o)
+      if (B.owner[o] == o && same_operator(P[o], p)) {
+        B.owner[s] = o;
+        break;
+      }
+  }
+  // sizes
+  size_t total = align256(sizeof(BSolve) * count) + align256(sizeof(BState) * count) +
+                 align256(sizeof(int) * 4) + align256(sizeof(int));
+  std::vector<Tiling> tl(count), tt(count);
+  std::vector<int> eb(count);
+  size_t grid_blocks = 0;
+  for (int s = 0; s < count; ++s) {
+    const ffpm_stairstep_problem &p = P[s];
+    const size_t n = (size_t)p.nx * p.ny * p.nz;
+    B.cl[s] = cells_per_lane_f64(p.nx);
+    tl[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl[s], rows, zchunk);
+    tt[s] = make_tiling(p.nx, p.ny, p.nz, 0);
+    eb[s] = elementwise_blocks(n);
+    const size_t nb = (size_t)std::max(std::max(tl[s].blocks, tt[s].blocks), eb[s]);
+    grid_blocks += (size_t)tl[s].blocks + tt[s].blocks + eb[s];
+    total += 6 * align256(sizeof(double) * n) + align256(sizeof(double) * kReduceSlots * nb) +
+             align256(sizeof(Scalars));
+    if (B.owner[s] == s)
+      total += 2 * align256(sizeof(double) * n) + align256(n) +
+               align256(sizeof(double) * 2 * (p.nx + p.ny + p.nz));
+  }
+  // workgroup tables: two int arrays per grid (four grids), each 256-aligned
+  total += 2 * align256(sizeof(int) * grid_blocks) + 8 * 256;
+  int st = 0;
+  Arena a;
+  a.base = (char *)workspace().get(S_BATCH, total, &st);
+  if (!a.base) return st;
+  B.d_solves = a.take<BSolve>(sizeof(BSolve) * count);
+  B.d_state = a.take<BState>(sizeof(BState) * count);
+  B.d_counters = a.take<int>(sizeof(int) * 4);
+  B.d_isolated = a.take<int>(sizeof(int));
+  k_b_zero_int<<<1, 64, 0, stream()>>>(B.d_isolated, 1);
+  FFPM_LAUNCHED("k_b_zero_int");
+  for (int s = 0; s < count; ++s) {
+    const ffpm_stairstep_problem &p = P[s];
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const size_t n = (size_t)nx * ny * nz;
+    BSolve &d = B.h[s];
+    d.n = n;
+    d.tl = tl[s];
+    d.tt = tt[s];
+    d.eb = eb[s];
+    const int o = B.owner[s];
+    if (o == s) {
+      double *coef = a.take<double>(sizeof(double) * 2 * (nx + ny + nz));
+      Geom g{};
+      g.nx = nx; g.ny = ny; g.nz = nz;
+      g.cw = coef; g.ce = coef + nx; g.cs = coef + 2 * nx; g.cn = coef + 2 * nx + ny;
+      g.cb = coef + 2 * nx + 2 * ny; g.ct = coef + 2 * nx + 2 * ny + nz;
+      g.dx = p.dx; g.dy = p.dy; g.dz = p.dz;
+      g.xc = p.xc; g.yc = p.yc; g.zc = p.zc;
+      const int tot = nx + ny + nz;
+      k_axis_coefficients<<<(tot + 255) / 256, 256, 0, stream()>>>(
+          nx, ny, nz, p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, (double *)g.cw, (double *)g.ce,
+          (double *)g.cs, (double *)g.cn, (double *)g.cb, (double *)g.ct);
+      FFPM_LAUNCHED("k_axis_coefficients");
+      d.geom = g;
+      d.g = GeomT<double>{nx, ny, nz, g.cw, g.ce, g.cs, g.cn, g.cb, g.ct, g.dx, g.dy, g.dz};
+      d.diag = a.take<double>(sizeof(double) * n);
+      d.rdiag = a.take<double>(sizeof(double) * n);
+      d.code = a.take<uint8_t>(n);
+    } else {
+      const BSolve &od = B.h[o];
+      d.geom = od.geom;
+      d.g = od.g;
+      d.diag = od.diag;
+      d.rdiag = od.rdiag;
+      d.code = od.code;
+    }
+    d.x = a.take<double>(sizeof(double) * n);
+    d.r = a.take<double>(sizeof(double) * n);
+    d.z = a.take<double>(sizeof(double) * n);
+    d.p[0] = a.take<double>(sizeof(double) * n);
+    d.p[1] = a.take<double>(sizeof(double) * n);
+    double *src = a.take<double>(sizeof(double) * n);
+    d.src = src;
+    const size_t nb = (size_t)std::max(std::max(tl[s].blocks, tt[s].blocks), eb[s]);
+    d.partials = a.take<double>(sizeof(double) * kReduceSlots * nb);
+    d.sc = a.take<Scalars>(sizeof(Scalars));
+    // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379); a frame's later
+    // right-hand sides rewrite the identical diag, rdiag and code.
+    k_build_operator<<<d.eb, kThreads, 0, stream()>>>(
+        d.geom, p.solid, p.rhs, p.xlo, p.xhi, p.ylo, p.yhi, p.zlo, p.zhi, p.body_grad[0],
+        p.body_grad[1], p.body_grad[2], 1, src, (double *)d.diag, (double *)d.rdiag,
+        (uint8_t *)d.code, B.d_isolated);
+    FFPM_LAUNCHED("k_build_operator");
+    k_pack_phi<<<d.eb, kThreads, 0, stream()>>>(nx, ny, nz, p.solid, p.phi, d.x);
+    FFPM_LAUNCHED("k_pack_phi");
+    k_b_zero_scalars<<<1, 64, 0, stream()>>>(d.sc);
+    FFPM_LAUNCHED("k_b_zero_scalars");
+  }
+  int isolated = 0;
+  FFPM_HIP(hipMemcpyAsync(&isolated, B.d_isolated, sizeof(int), hipMemcpyDeviceToHost,
+                          stream()));
+  FFPM_HIP(hipMemcpyAsync(B.d_solves, B.h.data(), sizeof(BSolve) * count,
+                          hipMemcpyHostToDevice, stream()));
+  std::vector<int> blocks(count), use(count), hs, hl;
+  for (int c = 1; c <= 2; ++c) {
+    for (int s = 0; s < count; ++s) {
+      blocks[s] = tl[s].blocks;
+      use[s] = B.cl[s] == c;
+    }
+    if ((st = build_grid(a, B.g_pcg[c], blocks, use, hs, hl))) return st;
+  }
+  for (int s = 0; s < count; ++s) {
+    blocks[s] = tt[s].blocks;
+    use[s] = 1;
+  }
+  if ((st = build_grid(a, B.g_tr, blocks, use, hs, hl))) return st;
+  for (int s = 0; s < count; ++s) blocks[s] = eb[s];
+  if ((st = build_grid(a, B.g_ew, blocks, use, hs, hl))) return st;
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  if (isolated > 0) {
+    set_error(ERR_ISOLATED, "stair-step fluid cell has no fluid neighbour");
+    return ERR_ISOLATED;
+  }
+  return OK;
+}
+
+int launch_check(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol) {
+  const int cb = (B.count + 63) / 64;
+  if (B.g_ew.blocks) {
+    k_b_dir_final<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
+    FFPM_LAUNCHED("k_b_dir_final");
+  }
+  k_b_tr1<<<B.g_tr.blocks, dim3(kTileX, B.h[0].tt.ty), 0, s>>>(B.d_solves, B.g_tr, B.d_state);
+  FFPM_LAUNCHED("k_b_tr1");
+  k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_TR1, 0);
+  FFPM_LAUNCHED("k_b_finish");
+  k_b_tr2<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
+  FFPM_LAUNCHED("k_b_tr2");
+  k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZ, 0);
+  FFPM_LAUNCHED("k_b_finish");
+  k_b_check_end<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps, tol,
+                                  B.d_counters);
+  FFPM_LAUNCHED("k_b_check_end");
+  return OK;
+}
+
+// One chunk of ci iteration slots and the convergence check.
+int launch_chunk(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol, int rows) {
+  const dim3 blk(kTileX, rows);
+  for (int m = 0; m < ci; ++m) {
+    for (int c = 1; c <= 2; ++c) {
+      if (!B.g_pcg[c].blocks) continue;
+      if (c == 2)
+        k_b_dir<2><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
+      else
+        k_b_dir<1><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
+      FFPM_LAUNCHED("k_b_dir");
+    }
+    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_PQ, m);
+    FFPM_LAUNCHED("k_b_finish");
+    for (int c = 1; c <= 2; ++c) {
+      if (!B.g_pcg[c].blocks) continue;
+      if (c == 2)
+        k_b_upd<2><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
+      else
+        k_b_upd<1><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
+      FFPM_LAUNCHED("k_b_upd");
+    }
+    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZNEW, m);
+    FFPM_LAUNCHED("k_b_finish");
+  }
+  k_b_check_begin<<<(B.count + 63) / 64, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci,
+                                                      max_sweeps);
+  FFPM_LAUNCHED("k_b_check_begin");
+  int st = launch_check(B, s, ci, max_sweeps, tol);
+  if (st) return st;
+  k_b_count<<<1, 256, 0, s>>>(B.d_state, B.count, B.d_counters, B.d_counters + 1);
+  FFPM_LAUNCHED("k_b_count");
+  return OK;
+}
+
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~StreamGuard() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+// Runs the batch. Returns per-solve states in `out` (host), the host time of
+// each replay's end in `replay_ms` (entry 0: the initial residuals) and the
+// time the first replay was issued (after the graph is built) in loop_ms.
+int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, int use_graph,
+              std::vector<BState> &out, std::vector<double> &replay_ms, double *loop_ms) {
+  const int ci = check_interval > 1 ? check_interval : 1;
+  const int count = B.count;
+  int st;
+  // All solves must share the PCG workgroup height (one launch per grid).
+  const int ty = B.h[0].tl.ty;
+  for (int s = 1; s < count; ++s)
+    if (B.h[s].tl.ty != ty || B.h[s].tt.ty != B.h[0].tt.ty) {
+      set_error(ERR_ARGUMENT, "batched solves need one workgroup height");
+      return ERR_ARGUMENT;
+    }
+  StreamGuard g;
+  FFPM_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  const auto t0 = std::chrono::steady_clock::now();
+  k_b_init<<<(count + 63) / 64, 64, 0, g.s>>>(B.d_state, count);
+  FFPM_LAUNCHED("k_b_init");
+  k_b_zero_int<<<1, 64, 0, g.s>>>(B.d_counters, 4);
+  FFPM_LAUNCHED("k_b_zero_int");
+  // the initial true residual of every solve
+  if ((st = launch_check(B, g.s, ci, max_sweeps, tolerance))) return st;
+  k_b_count<<<1, 256, 0, g.s>>>(B.d_state, count, B.d_counters, B.d_counters + 1);
+  FFPM_LAUNCHED("k_b_count");
+  int active = 0;
+  FFPM_HIP(hipMemcpyAsync(&active, B.d_counters + 1, sizeof(int), hipMemcpyDeviceToHost, g.s));
+  FFPM_HIP(hipStreamSynchronize(g.s));
+  replay_ms.assign(1, std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0).count());
+  if (use_graph && active > 0) {
+    FFPM_HIP(hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal));
+    st = launch_chunk(B, g.s, ci, max_sweeps, tolerance, ty);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(g.s, &graph);
+    g.graph = graph;
+    if (st) return st;
+    FFPM_HIP(e);
+    FFPM_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  }
+  *loop_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                 .count();
+  while (active > 0) {
+    if (use_graph) {
+      FFPM_HIP(hipGraphLaunch(g.exec, g.s));
+    } else if ((st = launch_chunk(B, g.s, ci, max_sweeps, tolerance, ty))) {
+      return st;
+    }
+    FFPM_HIP(hipMemcpyAsync(&active, B.d_counters + 1, sizeof(int), hipMemcpyDeviceToHost,
+                            g.s));
+    FFPM_HIP(hipStreamSynchronize(g.s));
+    replay_ms.push_back(std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0).count());
+  }
+  out.resize(count);
+  FFPM_HIP(hipMemcpyAsync(out.data(), B.d_state, sizeof(BState) * count, hipMemcpyDeviceToHost,
+                          g.s));
+  FFPM_HIP(hipStreamSynchronize(g.s));
+  return OK;
+}
+
+// Gauge, unpack and the final residual of one solve (as the standalone solve
+// ends, src/fpm_stairstep_mpi_3d.f90:170-190).
+int batch_finalize(const Batch &B, ffpm_stairstep_problem *P, int s, double *residual) {
+  const BSolve &d = B.h[s];
+  const ffpm_stairstep_problem &p = P[s];
+  k_gauge_sums<<<d.eb, kThreads, 0, stream()>>>(d.geom, p.solid, d.x, d.partials, d.sc);
+  FFPM_LAUNCHED("k_gauge_sums");
+  k_finish<<<1, 256, 0, stream()>>>(FIN_SUMS, 2, d.partials, d.eb, d.sc);
+  FFPM_LAUNCHED("k_finish");
+  const int pb = elementwise_blocks((size_t)(p.nx + 2) * (p.ny + 2) * p.nz);
+  k_gauge_apply<<<pb, kThreads, 0, stream()>>>(p.nx, p.ny, p.nz, p.solid, d.x, p.phi, d.sc);
+  FFPM_LAUNCHED("k_gauge_apply");
+  k_residual_max<<<d.tt.blocks, dim3(kTileX, d.tt.ty), 0, stream()>>>(d.tt, d.geom, d.x, d.src,
+                                                                        d.diag, d.partials, d.sc);
+  FFPM_LAUNCHED("k_residual_max");
+  k_finish<<<1, 256, 0, stream()>>>(FIN_MAX, 1, d.partials, d.tt.blocks, d.sc);
+  FFPM_LAUNCHED("k_finish");
+  FFPM_HIP(hipMemcpyAsync(residual, &d.sc->res, sizeof(double), hipMemcpyDeviceToHost,
+                          stream()));
+  return OK;
+}
+
+}  // namespace
+
+}  // namespace ffpm
+
+using namespace ffpm;
+
+extern "C" {
+
+int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
+                                   double tolerance, int max_sweeps, int check_interval,
+                                   int rows, int zchunk, int use_graph) {
+  if (!problems || count < 1) {
+    set_error(ERR_ARGUMENT, "empty batch");
+    return ERR_ARGUMENT;
+  }
+  if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8) {
+    set_error(ERR_ARGUMENT, "rows must be 0, 1, 2, 4 or 8");
+    return ERR_ARGUMENT;
+  }
+  Batch B;
+  int st = batch_setup(B, problems, count, rows, zchunk);
+  if (st) return st;
+  std::vector<BState> out;
+  std::vector<double> replay_ms;
+  double loop_ms = 0;
+  if ((st = batch_run(B, tolerance, max_sweeps, check_interval, use_graph, out, replay_ms,
+                      &loop_ms)))
+    return st;
+  std::vector<double> res(count);
+  for (int s = 0; s < count; ++s)
+    if ((st = batch_finalize(B, problems, s, &res[s]))) return st;
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  for (int s = 0; s < count; ++s) {
+    problems[s].sweeps = out[s].sweeps;
+    problems[s].residual = res[s];
+    const int r = std::min(std::max(out[s].finished_at, 0), (int)replay_ms.size() - 1);
+    problems[s].solve_ms = replay_ms[r];
+  }
+  return FFPM_OK;
+}
+
+int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
+                                      int iterations, int *rows, int *zchunk,
+                                      double *ms_per_iteration) {
+  if (!problems || count < 1 || !rows || !zchunk || !ms_per_iteration || iterations < 1) {
+    set_error(ERR_ARGUMENT, "autotune needs a batch, an iteration count and outputs");
+    return ERR_ARGUMENT;
+  }
+  static const int kRows[] = {4, 8};
+  static const int kChunks[] = {8, 16, 32};
+  double best = 1e300;
+  int br = 8, bz = 16;
+  for (int r : kRows) {
+    for (int zc : kChunks) {
+      Batch B;
+      int st = batch_setup(B, problems, count, r, zc);
+      if (st) return st;
+      std::vector<BState> out;
+      std::vector<double> replay_ms;
+      // Fixed work: every solve runs `iterations` iterations in one chunk.
+      double loop_ms = 0;
+      if ((st = batch_run(B, -1.0, iterations, iterations, 1, out, replay_ms, &loop_ms)))
+        return st;
+      const double ms = replay_ms.back() - loop_ms;
+      if (ms < best) {
+        best = ms;
+        br = r;
+        bz = zc;
+      }
+    }
+  }
+  *rows = br;
+  *zchunk = bz;
+  *ms_per_iteration = best / iterations;
+  return FFPM_OK;
+}
+
+}  // extern "C"

@@ -70,9 +70,11 @@ struct TileCoord {
   bool valid;  // workgroup owns a tile
 };
 
-__device__ inline TileCoord tile_coord(const Tiling &t, int nx, int ny, int nz) {
+// block: the workgroup's index within its own solve's launch (blockIdx.x,
+// except in the batched kernels, which run many solves in one grid).
+__device__ inline TileCoord tile_coord(const Tiling &t, int nx, int ny, int nz, int block) {
   TileCoord c;
-  const int L = blockIdx.x;
+  const int L = block;
   const int tile = t.xcd_map ? (L % kXcds) * t.per_xcd + L / kXcds : L;
   c.valid = tile < t.total;
   const int tx = tile % t.tiles_x;
@@ -84,6 +86,10 @@ __device__ inline TileCoord tile_coord(const Tiling &t, int nx, int ny, int nz) 
   c.k0 = tz * t.zchunk;
   c.k1 = min(nz, c.k0 + t.zchunk);
   return c;
+}
+
+__device__ inline TileCoord tile_coord(const Tiling &t, int nx, int ny, int nz) {
+  return tile_coord(t, nx, ny, nz, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- grids ----
@@ -151,9 +157,14 @@ __device__ inline void block_max(double &v, double *lds) {
 // reduction is reproducible run to run for a fixed launch shape. The kernel
 // boundary makes the partials visible without a per-workgroup device fence.
 __device__ inline void store_partials(double *partials, int nslots, const double *vals,
-                                      int nblocks) {
+                                      int nblocks, int block) {
   if (threadIdx.x == 0 && threadIdx.y == 0)
-    for (int s = 0; s < nslots; ++s) partials[(size_t)s * nblocks + blockIdx.x] = vals[s];
+    for (int s = 0; s < nslots; ++s) partials[(size_t)s * nblocks + block] = vals[s];
+}
+
+__device__ inline void store_partials(double *partials, int nslots, const double *vals,
+                                      int nblocks) {
+  store_partials(partials, nslots, vals, nblocks, blockIdx.x);
 }
 
 // Called by every thread of the finishing workgroup; thread 0 gets the sums.

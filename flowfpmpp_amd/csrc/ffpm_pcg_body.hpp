@@ -1,0 +1,342 @@
+// Device bodies of the fused PCG iteration, templated on the value type T
+// (double for the FP64 solve, float for the inner solve of mixed precision)
+// and on CL, the x-adjacent cells one lane owns (16-byte loads: CL = 2 for
+// double, 4 for float; CL = 1 when nx is not a multiple of the vector width).
+// Used by the single-solve kernels and by the batched kernels.
+//
+// Iteration (src/fpm_stairstep_mpi_3d.f90:246-283, formulas unchanged):
+//   pcg_dir_body : p = z + beta p_old (the previous iteration's direction
+//                  update, deferred), q = A p, p.q (FP64 sum)
+//   pcg_upd_body : q = A p recomputed bit-identically, x += alpha p,
+//                  r -= alpha q, z = M^-1 r, z.r (FP64 sum)
+// A = V (diag - offdiag) with diag and rdiag = 1/(V diag) rebuilt in
+// registers from the one-byte neighbour code and the 1-D coefficients, so the
+// kernels read no diagonal arrays. Each workgroup streams its tile along z;
+// the loads of the next plane are issued before the current plane is
+// computed, and every value moves in 16-byte vectors where the layout allows.
+#pragma once
+
+#include "ffpm_pcg.hpp"
+
+namespace ffpm {
+
+
+template <typename T, int CL>
+struct alignas(CL * sizeof(T)) Vec {
+  T v[CL];
+};
+
+template <int CL>
+struct alignas(CL) Codes {
+  uint8_t b[CL];
+};
+
+template <typename T, int CL>
+__device__ __forceinline__ Vec<T, CL> ldv(const T *__restrict__ a, size_t c) {
+  return *reinterpret_cast<const Vec<T, CL> *>(a + c);
+}
+
+template <typename T, int CL>
+__device__ __forceinline__ void stv(T *__restrict__ a, size_t c, const Vec<T, CL> &x) {
+  *reinterpret_cast<Vec<T, CL> *>(a + c) = x;
+}
+
+// Raw loads behind one direction vector (z and p_old for kDirUpdate, z for
+// kDirReset, p_old for kDirCurrent), combined only when used, so the loads of
+// a whole plane can be issued one plane ahead. The mode is a template
+// argument: with a run-time mode the compiler puts every load pair in its own
+// branch and waits for it there, which serialises the loads of a plane.
+template <int MODE, typename T, int CL>
+struct RawV {
+  Vec<T, CL> a, b;
+  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+                                       size_t c) {
+    if constexpr (MODE == kDirCurrent) {
+      a = ldv<T, CL>(pin, c);
+    } else if constexpr (MODE == kDirReset) {
+      a = ldv<T, CL>(z, c);
+    } else {
+      a = ldv<T, CL>(z, c);
+      b = ldv<T, CL>(pin, c);
+    }
+  }
+  __device__ __forceinline__ Vec<T, CL> value(T beta) const {
+    if constexpr (MODE != kDirUpdate) {
+      return a;
+    } else {
+      Vec<T, CL> v;
+#pragma unroll
+      for (int m = 0; m < CL; ++m) v.v[m] = a.v[m] + beta * b.v[m];
+      return v;
+    }
+  }
+};
+
+template <int MODE, typename T>
+struct Raw1 {
+  T a, b;
+  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+                                       size_t c) {
+    if constexpr (MODE == kDirCurrent) {
+      a = pin[c];
+    } else if constexpr (MODE == kDirReset) {
+      a = z[c];
+    } else {
+      a = z[c];
+      b = pin[c];
+    }
+  }
+  __device__ __forceinline__ T value(T beta) const {
+    if constexpr (MODE != kDirUpdate) return a;
+    else return a + beta * b;
+  }
+};
+
+// The in-plane neighbours of a lane's cells (south and north rows, the west
+// and east cells) and their neighbour codes, for one plane.
+template <int MODE, typename T, int CL>
+struct Plane {
+  RawV<MODE, T, CL> s, n;
+  Raw1<MODE, T> w, e;
+  Codes<CL> code;
+  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+                                       const uint8_t *__restrict__ cd, size_t c, ptrdiff_t os,
+                                       ptrdiff_t on, ptrdiff_t ow, ptrdiff_t oe) {
+    s.load(z, pin, c + os);
+    n.load(z, pin, c + on);
+    w.load(z, pin, c + ow);
+    e.load(z, pin, c + oe);
+    code = *reinterpret_cast<const Codes<CL> *>(cd + c);
+  }
+};
+
+// diag from the neighbour code: the fluid-neighbour coefficients summed in
+// build_operator's order (src/fpm_stairstep_mpi_3d.f90:353-370), so for
+// T = double the value is bit-identical to the stored diag.
+template <typename T>
+__device__ __forceinline__ T diag_code(unsigned code, T cw, T ce, T cs, T cn, T cb, T ct) {
+  T d = 0;
+  if (code & kCodeW) d = d + cw;
+  if (code & kCodeE) d = d + ce;
+  if (code & kCodeS) d = d + cs;
+  if (code & kCodeN) d = d + cn;
+  if (code & kCodeB) d = d + cb;
+  if (code & kCodeT) d = d + ct;
+  return d;
+}
+
+// The same from the coefficient arrays (for kernels that hold no copies).
+template <typename T>
+__device__ __forceinline__ T diag_code(unsigned code, const GeomT<T> &g, int i, int j, int k) {
+  return diag_code<T>(code, g.cw[i], g.ce[i], g.cs[j], g.cn[j], g.cb[k], g.ct[k]);
+}
+
+template <typename T>
+__device__ __forceinline__ T rdiag_code(T d, T dxdy, T dzk) {
+  return d > 0 ? T(1) / (dxdy * dzk * d) : T(0);
+}
+
+struct Lane {
+  int i, j, k0, k1;
+  bool active;
+};
+
+template <int CL>
+__device__ __forceinline__ Lane lane_of(const Tiling &t, int nx, int ny, int nz, int block) {
+  const TileCoord tc = tile_coord(t, nx, ny, nz, block);
+  Lane l;
+  l.i = (tc.i - (int)threadIdx.x) * CL + CL * (int)threadIdx.x;
+  l.j = tc.j;
+  l.k0 = tc.k0;
+  l.k1 = tc.k1;
+  l.active = tc.valid && l.i < nx && l.j < ny;
+  return l;
+}
+
+
+// p = z + beta p_old (or z, or p_old, by MODE), q = A p and p.q for the
+// workgroup `block` of a launch of nblocks workgroups.
+template <typename T, int CL, int MODE>
+__device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nblocks,
+                                             const GeomT<T> &g, const T *__restrict__ z,
+                                             const uint8_t *__restrict__ code,
+                                             const T *__restrict__ pin, T *__restrict__ pout,
+                                             double *partials, const Scalars *sc, double *lds) {
+  if (sc->status != 0) return;
+  const T beta = (T)sc->beta;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Lane l = lane_of<CL>(t, nx, ny, nz, block);
+  double s = 0.0;
+  if (l.active) {
+    const int i = l.i, j = l.j;
+    const size_t nxy = (size_t)nx * ny;
+    T cw[CL], ce[CL], dxdy[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+      cw[m] = g.cw[i + m];
+      ce[m] = g.ce[i + m];
+      dxdy[m] = g.dx[i + m] * g.dy[j];
+    }
+    const T cs = g.cs[j], cn = g.cn[j];
+    const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+    // West and east neighbours come from the adjacent lanes' registers by a
+    // wave shuffle; only lanes 0 and 63 load theirs (from the adjacent tile).
+    // The other lanes point their edge loads at the row start, so those load
+    // instructions touch one cache line.
+    const int lane = (int)threadIdx.x;
+    const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
+    const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : row0;
+    const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : row0;
+    // Software pipeline: the loads of plane k+1's neighbours and of plane
+    // k+2's own row are issued before plane k is computed. Two register sets
+    // alternate (the loop steps two planes), so no loaded value is copied
+    // before it arrives. Planes past the chunk are clamped to planes already
+    // loaded (their values are unused); past the top, p_next is p_cur's own
+    // plane as in the unpipelined form.
+    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+    auto at = [&](int k) { return base + nxy * (size_t)k; };
+    RawV<MODE, T, CL> own_a, own_b;
+    Plane<MODE, T, CL> pl_a, pl_b;
+    own_a.load(z, pin, at(k0));
+    Vec<T, CL> p_cur = own_a.value(beta);
+    Vec<T, CL> p_prev = p_cur;
+    if (k0 > 0) {
+      own_a.load(z, pin, at(k0 - 1));
+      p_prev = own_a.value(beta);
+    }
+    own_a.load(z, pin, at(min(k0 + 1, last)));
+    pl_a.load(z, pin, code, at(k0), os, on, ow, oe);
+    auto stage = [&](int k, const RawV<MODE, T, CL> &own_use, const Plane<MODE, T, CL> &pl,
+                     RawV<MODE, T, CL> &own_load, Plane<MODE, T, CL> &pl_load) {
+      own_load.load(z, pin, at(min(k + 2, last)));
+      pl_load.load(z, pin, code, at(min(k + 1, k1 - 1)), os, on, ow, oe);
+      const Vec<T, CL> p_next = own_use.value(beta);
+      const Vec<T, CL> vs = pl.s.value(beta), vn = pl.n.value(beta);
+      // The shuffles run on every active lane (a source lane that is masked
+      // off reads as 0), then the edge lanes take their loaded values.
+      const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
+      const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
+      const T vw = lane == 0 ? pl.w.value(beta) : from_w;
+      const T ve = lane == kTileX - 1 ? pl.e.value(beta) : from_e;
+      const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
+#pragma unroll
+      for (int m = 0; m < CL; ++m) {
+        const T west = m == 0 ? (hw ? vw : T(0)) : p_cur.v[m - 1];
+        const T east = m == CL - 1 ? (he ? ve : T(0)) : p_cur.v[m + 1];
+        const T south = hs ? vs.v[m] : T(0), north = hn ? vn.v[m] : T(0);
+        const T d = diag_code<T>(pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
+        const T q = dxdy[m] * dzk *
+                    (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
+                     cb * p_prev.v[m] - ct * p_next.v[m]);
+        s += (double)p_cur.v[m] * (double)(d > 0 ? q : T(0));
+      }
+      if constexpr (MODE != kDirCurrent) stv<T, CL>(pout, at(k), p_cur);
+      p_prev = p_cur;
+      p_cur = p_next;
+    };
+    for (int k = k0; k < k1; k += 2) {
+      stage(k, own_a, pl_a, own_b, pl_b);
+      if (k + 1 < k1) stage(k + 1, own_b, pl_b, own_a, pl_a);
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, nblocks, block);
+}
+
+// q = A p (recomputed), x += alpha p, r -= alpha q, z = M^-1 r and z.r.
+template <typename T, int CL>
+__device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
+                                             const GeomT<T> &g, const T *__restrict__ p,
+                                             const uint8_t *__restrict__ code,
+                                             T *__restrict__ x, T *__restrict__ r,
+                                             T *__restrict__ z, double *partials,
+                                             const Scalars *sc, double *lds) {
+  if (sc->status != 0) return;
+  const T alpha = (T)sc->alpha;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const Lane l = lane_of<CL>(t, nx, ny, nz, block);
+  double s = 0.0;
+  if (l.active) {
+    const int i = l.i, j = l.j;
+    const size_t nxy = (size_t)nx * ny;
+    T cw[CL], ce[CL], dxdy[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+      cw[m] = g.cw[i + m];
+      ce[m] = g.ce[i + m];
+      dxdy[m] = g.dx[i + m] * g.dy[j];
+    }
+    const T cs = g.cs[j], cn = g.cn[j];
+    const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+    // West and east neighbours come from the adjacent lanes' registers by a
+    // wave shuffle; only lanes 0 and 63 load theirs (from the adjacent tile).
+    // The other lanes point their edge loads at the row start, so those load
+    // instructions touch one cache line.
+    const int lane = (int)threadIdx.x;
+    const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
+    const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : row0;
+    const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : row0;
+    // Software pipeline as in k_pcg_dir_t: plane k+1's neighbours, phi and r
+    // and plane k+2's own direction row load while plane k is computed.
+    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+    auto at = [&](int k) { return base + nxy * (size_t)k; };
+    struct Set {
+      Vec<T, CL> p_next, x, r;
+      Plane<kDirCurrent, T, CL> pl;
+    };
+    Set sa, sb;
+    Vec<T, CL> p_cur = ldv<T, CL>(p, at(k0));
+    Vec<T, CL> p_prev = k0 > 0 ? ldv<T, CL>(p, at(k0 - 1)) : p_cur;
+    sa.p_next = ldv<T, CL>(p, at(min(k0 + 1, last)));
+    sa.pl.load(p, p, code, at(k0), os, on, ow, oe);
+    sa.x = ldv<T, CL>(x, at(k0));
+    sa.r = ldv<T, CL>(r, at(k0));
+    auto stage = [&](int k, const Set &u, Set &ld) {
+      const int kn = min(k + 1, k1 - 1);
+      ld.p_next = ldv<T, CL>(p, at(min(k + 2, last)));
+      ld.pl.load(p, p, code, at(kn), os, on, ow, oe);
+      ld.x = ldv<T, CL>(x, at(kn));
+      ld.r = ldv<T, CL>(r, at(kn));
+      const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
+      Vec<T, CL> xv = u.x, rv = u.r, zv;
+      const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
+      const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
+      const T vw = lane == 0 ? u.pl.w.a : from_w;
+      const T ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
+#pragma unroll
+      for (int m = 0; m < CL; ++m) {
+        const T west = m == 0 ? (hw ? vw : T(0)) : p_cur.v[m - 1];
+        const T east = m == CL - 1 ? (he ? ve : T(0)) : p_cur.v[m + 1];
+        const T south = hs ? u.pl.s.a.v[m] : T(0), north = hn ? u.pl.n.a.v[m] : T(0);
+        const T d = diag_code<T>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
+        const T qr = dxdy[m] * dzk *
+                     (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
+                      cb * p_prev.v[m] - ct * u.p_next.v[m]);
+        const T q = d > 0 ? qr : T(0);
+        xv.v[m] = xv.v[m] + alpha * p_cur.v[m];
+        rv.v[m] = rv.v[m] - alpha * q;
+        zv.v[m] = rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m];
+        s += (double)zv.v[m] * (double)rv.v[m];
+      }
+      stv<T, CL>(x, at(k), xv);
+      stv<T, CL>(r, at(k), rv);
+      stv<T, CL>(z, at(k), zv);
+      p_prev = p_cur;
+      p_cur = u.p_next;
+    };
+    for (int k = k0; k < k1; k += 2) {
+      stage(k, sa, sb);
+      if (k + 1 < k1) stage(k + 1, sb, sa);
+    }
+  }
+  double v[1] = {s};
+  block_sum<1>(v, lds);
+  store_partials(partials, 1, v, nblocks, block);
+}
+
+}  // namespace ffpm

@@ -44,6 +44,8 @@ enum Slot : int {
   S_U_F5, S_U_F6, S_U_F7, S_U_F8, S_U_O0, S_U_O1, S_U_O2, S_U_O3, S_U_O4,
   // no-body Neumann solver
   S_L_NEXT = 80, S_L_COEF,
+  // batched solves (one arena)
+  S_BATCH = 90,
 };
 
 // ---- kernel timing with HIP events on the library stream --------------------

@@ -9,7 +9,7 @@
 // tile, which the XCD-aware tile order keeps in the same L2). The operator is
 // mask-free: solid cells hold zero in every iterate and zero src/rdiag/diag,
 // and the zero physical halo is substituted instead of stored.
-#include "ffpm_kernels.hpp"
+#include "ffpm_bodies.hpp"
 
 namespace ffpm {
 
@@ -195,9 +195,8 @@ __global__ __launch_bounds__(512) void k_residual_max(Tiling t, Geom g,
   store_partials(partials, 1, &m, gridDim.x);
 }
 
-// true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319): the
-// max-norm residual and r = V (src + sum c phi_nb - diag phi) in fluid cells
-// share one evaluation of the bracket; sums of r and of the fluid count.
+// true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319); body in
+// ffpm_bodies.hpp.
 __global__ __launch_bounds__(512) void k_true_residual_1(Tiling t, Geom g,
                                                          const double *__restrict__ phi,
                                                          const double *__restrict__ src,
@@ -205,49 +204,10 @@ __global__ __launch_bounds__(512) void k_true_residual_1(Tiling t, Geom g,
                                                          double *__restrict__ r,
                                                          double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double m = 0.0;
-  double s[2] = {0.0, 0.0};
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
-    const double dxdy = g.dx[i] * g.dy[j];
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    double p_cur = phi[c];
-    double p_prev = tc.k0 > 0 ? phi[c - nxy] : p_cur;
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const double p_next = (k + 1 < nz) ? phi[c + nxy] : p_cur;
-      const double d = diag[c];
-      double rv = 0.0;
-      if (d > 0) {
-        const double pw = hw ? phi[c - 1] : 0.0;
-        const double pe = he ? phi[c + 1] : 0.0;
-        const double ps = hs ? phi[c - nx] : 0.0;
-        const double pn = hn ? phi[c + nx] : 0.0;
-        const double v = src[c] + cw * pw + ce * pe + cs * ps + cn * pn + g.cb[k] * p_prev +
-                         g.ct[k] * p_next - d * p_cur;
-        m = fmax(m, fabs(v));
-        rv = dxdy * g.dz[k] * v;
-        s[0] += rv;
-        s[1] += 1.0;
-      }
-      r[c] = rv;
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  block_max(m, lds);
-  block_sum<2>(s, lds + 16);
-  const double vals[3] = {s[0], s[1], m};
-  store_partials(partials, 3, vals, gridDim.x);
+  true_residual_1_body(t, blockIdx.x, gridDim.x, g, phi, src, diag, r, partials, lds);
 }
 
-// true_residual, second pass (src/fpm_stairstep_mpi_3d.f90:319-328): project
-// r off the constants, optionally reset p = M^-1 r, and rz = r.M^-1 r.
+// true_residual, second pass (src/fpm_stairstep_mpi_3d.f90:319-328).
 __global__ __launch_bounds__(512) void k_true_residual_2(size_t n, double *__restrict__ r,
                                                          const double *__restrict__ rdiag,
                                                          const double *__restrict__ diag,
@@ -255,22 +215,7 @@ __global__ __launch_bounds__(512) void k_true_residual_2(size_t n, double *__res
                                                          double *__restrict__ z,
                                                          double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  const double shift = sc->red1 / sc->red2;
-  double s[1] = {0.0};
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    double rv = r[c];
-    if (diag[c] > 0) {
-      rv = rv - shift;
-      r[c] = rv;
-    }
-    const double zv = rdiag[c] * rv;
-    if (reset) p[c] = zv;
-    if (z) z[c] = zv;
-    s[0] += zv * rv;
-  }
-  block_sum<1>(s, lds);
-  store_partials(partials, 1, s, gridDim.x);
+  true_residual_2_body(n, blockIdx.x, gridDim.x, r, rdiag, diag, p, reset, z, partials, sc, lds);
 }
 
 // PCG SpMV + dot (src/fpm_stairstep_mpi_3d.f90:246-265): q = A p with
@@ -353,55 +298,12 @@ __global__ __launch_bounds__(512) void k_direction(size_t n, double *__restrict_
     p[c] = rdiag[c] * r[c] + beta * p[c];
 }
 
-// One workgroup of 256 threads: totals the partials of the reduction kernel
-// launched just before on the same stream and applies the PCG scalar logic
-// (src/fpm_stairstep_mpi_3d.f90:257-278) on the device.
+// One workgroup: totals the partials of the reduction kernel launched just
+// before on the same stream and applies the PCG scalar logic on the device.
 __global__ __launch_bounds__(512) void k_finish(int op, int nslots, const double *partials,
                                                 int nblocks, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  if ((op == FIN_PQ || op == FIN_RZNEW) && sc->status != 0) return;
-  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
-  if (op == FIN_MAX) {
-    const double m = grid_finish_max(partials, 0, nblocks, lds);
-    if (lead) sc->res = m;
-  } else if (op == FIN_TR1) {
-    const double m = grid_finish_max(partials, 2, nblocks, lds);
-    double v[2];
-    grid_finish_sum<2>(partials, nblocks, v, lds + 16);
-    if (lead) {
-      sc->res = m;
-      sc->red1 = v[0];
-      sc->red2 = v[1];
-    }
-  } else if (op == FIN_SUMS) {
-    double v[kReduceSlots];
-    grid_finish_sum<kReduceSlots>(partials, nblocks, v, lds);
-    if (lead)
-      for (int s = 0; s < nslots && s < kReduceSlots; ++s) sc->sums[s] = v[s];
-  } else {
-    double v[1];
-    grid_finish_sum<1>(partials, nblocks, v, lds);
-    if (!lead) return;
-    if (op == FIN_RZ) {
-      sc->rz = v[0];
-    } else if (op == FIN_PQ) {
-      const double pq = v[0], rz = sc->rz;
-      sc->pq = pq;
-      // The negated tests are also true for NaN.
-      if (!(pq > 0) || !(rz > 0)) sc->status = kBreakdown;
-      else sc->alpha = rz / pq;
-    } else if (op == FIN_RZNEW) {
-      const double rz_new = v[0];
-      sc->rz_new = rz_new;
-      if (!(rz_new >= 0)) {
-        sc->status = kNegative;
-      } else {
-        sc->beta = rz_new / sc->rz;
-        sc->rz = rz_new;
-        sc->iters += 1;
-      }
-    }
-  }
+  finish_body(op, nslots, partials, nblocks, sc, lds);
 }
 
 __global__ void k_chunk_begin(Scalars *sc) {

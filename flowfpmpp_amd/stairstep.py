@@ -185,3 +185,74 @@ def timing_query(kernel: str):
     cnt, ms = C.c_long(), C.c_double()
     check(lib().ffpm_timing_query(kernel.encode(), C.byref(cnt), C.byref(ms)))
     return cnt.value, ms.value
+
+
+# ---------------------------------------------------------------- batches --
+class _Problem(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("phi", C.c_void_p), ("rhs", C.c_void_p), ("solid", C.c_void_p),
+                ("body_grad", C.c_double * 3),
+                ("xlo", C.c_void_p), ("xhi", C.c_void_p), ("ylo", C.c_void_p),
+                ("yhi", C.c_void_p), ("zlo", C.c_void_p), ("zhi", C.c_void_p),
+                ("xc", C.c_void_p), ("yc", C.c_void_p), ("zc", C.c_void_p),
+                ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p),
+                ("sweeps", C.c_int), ("residual", C.c_double), ("solve_ms", C.c_double)]
+
+
+class BatchProblem:
+    """One solve of a batch: the device arrays of one solve_stairstep_poisson_3d
+    call. Problems that pass the same ``solid`` and axis arrays (the force
+    directions of one frame) share one operator in the batch."""
+
+    def __init__(self, phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc,
+                 dx, dy, dz):
+        arrays = (phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz)
+        if not all(_is_dev(a) for a in arrays):
+            raise TypeError("batched solves take device arrays")
+        self.phi, self.rhs, self.solid = phi, rhs, solid
+        self.body_grad = np.asarray(body_grad, dtype=np.float64).reshape(3)
+        self.faces = (xlo, xhi, ylo, yhi, zlo, zhi)
+        self.axes = (xc, yc, zc, dx, dy, dz)
+        self.sweeps, self.residual, self.solve_ms = 0, float("nan"), float("nan")
+
+    def _struct(self):
+        nx, ny, nz = _shape(self.solid)
+        s = _Problem()
+        s.nx, s.ny, s.nz = nx, ny, nz
+        s.phi, s.rhs, s.solid = _d(self.phi).value, _d(self.rhs).value, _i(self.solid).value
+        s.body_grad = (C.c_double * 3)(*self.body_grad)
+        (s.xlo, s.xhi, s.ylo, s.yhi, s.zlo, s.zhi) = [_d(f).value for f in self.faces]
+        (s.xc, s.yc, s.zc, s.dx, s.dy, s.dz) = [_d(a).value for a in self.axes]
+        return s
+
+
+def _structs(problems):
+    arr = (_Problem * len(problems))()
+    for k, p in enumerate(problems):
+        arr[k] = p._struct()
+    return arr
+
+
+def solve_stairstep_batch(problems, tolerance, max_sweeps, check_interval, rows=0, zchunk=0,
+                          use_graph=True):
+    """FP64 PCG on many problems in one batch (per-problem convergence masks,
+    HIP-graph chunks). Each problem ends as its own
+    solve_stairstep_poisson_3d(method='pcg') would; sweeps, residual and the
+    time to its convergence (ms from the batch start) are stored on it."""
+    arr = _structs(problems)
+    check(lib().ffpm_solve_stairstep_batch_dev(arr, len(problems), C.c_double(tolerance),
+                                                int(max_sweeps), int(check_interval), int(rows),
+                                                int(zchunk), 1 if use_graph else 0))
+    for k, p in enumerate(problems):
+        p.sweeps, p.residual, p.solve_ms = arr[k].sweeps, arr[k].residual, arr[k].solve_ms
+    return [(p.sweeps, p.residual) for p in problems]
+
+
+def autotune_stairstep_batch(problems, iterations=20):
+    """Times the candidate launch shapes on the batch; returns (rows, zchunk,
+    ms per batched iteration) of the fastest."""
+    arr = _structs(problems)
+    rows, zchunk, ms = C.c_int(), C.c_int(), C.c_double()
+    check(lib().ffpm_autotune_stairstep_batch_dev(arr, len(problems), int(iterations),
+                                                   C.byref(rows), C.byref(zchunk), C.byref(ms)))
+    return rows.value, zchunk.value, ms.value

@@ -264,3 +264,25 @@ def influence_inputs(p: Problem, direction: int):
     bg = np.zeros(3)
     bg[direction] = 1.0
     return p.zeros(), bg, p.faces()
+
+
+# --- batched uneven frames ------------------------------------------------------
+BATCH_GRIDS = ((64, 64, 64), (96, 80, 64), (128, 96, 64), (160, 128, 96))
+
+
+def batch_instance(i: int) -> Problem:
+    """Instance i of the batched workload (seed 4000 + i): a grid drawn from
+    BATCH_GRIDS and 1-3 spheres or boxes (radius / sides ~ U[0.3, 1.0]) on the
+    same stretched domain."""
+    rng = np.random.Generator(np.random.PCG64(4000 + i))
+    nx, ny, nz = BATCH_GRIDS[int(rng.integers(len(BATCH_GRIDS)))]
+    bodies = []
+    for _ in range(int(rng.integers(1, 4))):
+        c = (rng.uniform(-3.0, 3.0), rng.uniform(-2.0, 2.0), rng.uniform(-1.2, 1.2))
+        if rng.uniform() < 0.5:
+            bodies.append(Sphere(c, rng.uniform(0.3, 1.0)))
+        else:
+            s = rng.uniform(0.3, 1.0, size=3)
+            bodies.append(Box(tuple(np.array(c) - 0.5 * s), tuple(np.array(c) + 0.5 * s)))
+    return stretched_frame(nx, ny, nz, bodies, seed=4000 + i, label=f"batch{i}",
+                           with_fields=False)

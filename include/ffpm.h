@@ -123,6 +123,38 @@ int ffpm_solve_stairstep_poisson_3d_dev(
  * FFPM_METHOD_PCG_MIXED solve on the calling thread (no reference counterpart). */
 int ffpm_last_mixed_solve(int *outer, int *inner);
 
+/* Batched solves (no counterpart in the reference, which solves one frame per
+ * call of solve_stairstep_poisson_3d, src/fpm_stairstep_mpi_3d.f90:106-330).
+ * Each problem is one call's worth of device arrays in the same layouts; the
+ * problems of one batch advance together (one grid per kernel, per-problem
+ * convergence masks, one HIP graph per check interval) and each ends exactly
+ * as its own FP64 PCG solve would: same iterations, same phi. Problems with
+ * the same solid and axis pointers (e.g. the three force directions of a
+ * frame) share one operator. rows/zchunk fix the PCG workgroup height (0, 4
+ * or 8) and z chunk (0 = the standalone defaults); solve_ms is the time from
+ * the batch start until the problem converged. */
+typedef struct ffpm_stairstep_problem {
+  int nx, ny, nz;
+  double *phi;             /* (0:nx+1,0:ny+1,nz), in/out */
+  const double *rhs;       /* (nx,ny,nz) */
+  const int32_t *solid;    /* (0:nx+1,0:ny+1,nz) */
+  double body_grad[3];
+  const double *xlo, *xhi, *ylo, *yhi, *zlo, *zhi;
+  const double *xc, *yc, *zc, *dx, *dy, *dz;
+  int sweeps;              /* out */
+  double residual;         /* out */
+  double solve_ms;         /* out */
+} ffpm_stairstep_problem;
+int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
+                                   double tolerance, int max_sweeps, int check_interval,
+                                   int rows, int zchunk, int use_graph);
+/* Times `iterations` batched iterations for each candidate launch shape
+ * (rows 4, 8 x z chunks 8, 16, 32) and returns the fastest; phi is not
+ * written. */
+int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
+                                      int iterations, int *rows, int *zchunk,
+                                      double *ms_per_iteration);
+
 /* stairstep_residual_3d, src/fpm_stairstep_mpi_3d.f90:403-419 (value out) */
 int ffpm_stairstep_residual_3d(
     const double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
