@@ -19,6 +19,7 @@ SOURCES = [
     "ffpm_stairstep_host.hip",
     "ffpm_laplace.hip",
     "ffpm_batch.hip",
+    "ffpm_unet.hip",
 ]
 
 # -ffp-contract=off keeps every expression's rounding identical to the

@@ -46,6 +46,8 @@ enum Slot : int {
   S_L_NEXT = 80, S_L_COEF,
   // batched solves (one arena)
   S_BATCH = 90,
+  // flexible CG: the network's source field
+  S_FCG_SRC = 91,
 };
 
 // ---- kernel timing with HIP events on the library stream --------------------

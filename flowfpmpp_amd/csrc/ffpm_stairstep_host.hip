@@ -11,6 +11,7 @@
 #include "../../include/ffpm.h"
 #include "ffpm_pcg.hpp"
 #include "ffpm_runtime.hpp"
+#include "ffpm_unet.hpp"
 
 namespace ffpm {
 
@@ -511,6 +512,175 @@ int jacobi_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
 }  // namespace
 
 // The whole solve on device pointers (src/fpm_stairstep_mpi_3d.f90:106-330).
+// ------------------------------------------------ learned flexible CG ------
+// Flexible CG with one-vector truncation (Notay's FCG(1)) preconditioned by
+// the U-Net, in FP64 except inside the network: z = sigma * net(features,
+// (r / V) / sigma), sigma = max |r / V| over fluid cells, z = 0 in solids;
+// p = z - (z.q_old / p_old.q_old) p_old; q = A p; alpha = p.r / p.q. With the
+// initial phi the first step x += alpha z is the network's initial guess.
+// Every check_interval iterations the FP64 true residual (the reference's
+// projected residual and max-norm) replaces r and the direction restarts;
+// three checks without a 1% improvement switch to diagonal PCG from the
+// current phi. The answer is always judged by the FP64 max-norm residual.
+__global__ __launch_bounds__(256) void k_fcg_absmax(Geom g, const double *__restrict__ r,
+                                                    const double *__restrict__ diag,
+                                                    double *partials) {
+  __shared__ double lds[16 * kReduceSlots];
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
+  double m = 0.0;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    if (!(diag[c] > 0)) continue;
+    const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((size_t)g.nx * g.ny));
+    m = fmax(m, fabs(r[c] / (g.dx[i] * g.dy[j] * g.dz[k])));
+  }
+  block_max(m, lds);
+  store_partials(partials, 1, &m, gridDim.x);
+}
+
+__global__ void k_fcg_source(Geom g, const double *__restrict__ r, const double *__restrict__ diag,
+                             const Scalars *sc, float *__restrict__ src) {
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
+  const double inv = sc->res > 0 ? 1.0 / sc->res : 0.0;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % g.nx), j = (int)((c / g.nx) % g.ny), k = (int)(c / ((size_t)g.nx * g.ny));
+    src[c] = diag[c] > 0 ? (float)(r[c] / (g.dx[i] * g.dy[j] * g.dz[k]) * inv) : 0.f;
+  }
+}
+
+__global__ void k_fcg_correction(size_t n, const float *__restrict__ out,
+                                 const double *__restrict__ diag, const Scalars *sc,
+                                 double *__restrict__ z) {
+  const double sigma = sc->res;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    z[c] = diag[c] > 0 ? sigma * (double)out[c] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_dot2(size_t n, const double *__restrict__ a,
+                                              const double *__restrict__ b,
+                                              const double *__restrict__ c,
+                                              const double *__restrict__ d, double *partials) {
+  __shared__ double lds[16 * kReduceSlots];
+  double s[2] = {0.0, 0.0};
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    s[0] += a[e] * b[e];
+    s[1] += c[e] * d[e];
+  }
+  block_sum<2>(s, lds);
+  store_partials(partials, 2, s, gridDim.x);
+}
+
+__global__ void k_fcg_dir(size_t n, const double *__restrict__ z, double beta,
+                          double *__restrict__ p) {
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    p[c] = z[c] - beta * p[c];
+}
+
+__global__ void k_fcg_update(size_t n, double alpha, const double *__restrict__ p,
+                             const double *__restrict__ q, double *__restrict__ x,
+                             double *__restrict__ r) {
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x) {
+    x[c] = x[c] + alpha * p[c];
+    r[c] = r[c] - alpha * q[c];
+  }
+}
+
+struct FcgArgs {
+  UNet *net;
+  const float *features;
+  int *fallback_at;  // FCG iterations before the switch to PCG, or -1
+};
+
+int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
+                int check_interval, int *sweeps, double *residual) {
+  const int ci = check_interval > 1 ? check_interval : 1;
+  const int eb = elementwise_blocks(f.n);
+  int st = 0;
+  Workspace &ws = workspace();
+  float *src = (float *)ws.get(S_FCG_SRC, sizeof(float) * f.n, &st);
+  if (!src) return st;
+  double *p = f.next, *q = f.q, *z = f.z;  // r in f.r, x in f.phi
+  double res;
+  if ((st = true_residual(f, false, &res))) return st;
+  double best = res, pq_old = 0.0;
+  int stalled = 0, k = 0;
+  *a.fallback_at = -1;
+  auto sums = [&](int nslots, int nblocks, double *out) -> int {
+    finish(f, FIN_SUMS, nslots, nblocks);
+    Scalars h;
+    int e = read_scalars(f, &h);
+    for (int s2 = 0; s2 < nslots; ++s2) out[s2] = h.sums[s2];
+    return e;
+  };
+  while (*sweeps < max_sweeps && res > tolerance) {
+    // z = M(r)
+    k_fcg_absmax<<<eb, kThreads, 0, stream()>>>(f.g, f.r, f.diag, f.partials);
+    FFPM_LAUNCHED("k_fcg_absmax");
+    finish(f, FIN_MAX, 1, eb);  // sigma -> sc->res
+    k_fcg_source<<<eb, kThreads, 0, stream()>>>(f.g, f.r, f.diag, f.sc, src);
+    FFPM_LAUNCHED("k_fcg_source");
+    const float *out = nullptr;
+    if ((st = unet_apply(*a.net, f.nx, f.ny, f.nz, a.features, src, &out))) return st;
+    k_fcg_correction<<<eb, kThreads, 0, stream()>>>(f.n, out, f.diag, f.sc, z);
+    FFPM_LAUNCHED("k_fcg_correction");
+    // p = z - beta p_old
+    double beta = 0.0;
+    if (k % ci != 0) {
+      double v[2];
+      k_dot2<<<eb, kThreads, 0, stream()>>>(f.n, z, q, z, z, f.partials);
+      FFPM_LAUNCHED("k_dot2");
+      if ((st = sums(2, eb, v))) return st;
+      beta = pq_old != 0.0 ? v[0] / pq_old : 0.0;
+    }
+    k_fcg_dir<<<eb, kThreads, 0, stream()>>>(f.n, z, beta, p);
+    FFPM_LAUNCHED("k_fcg_dir");
+    // q = A p, p.q, p.r
+    k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
+        f.tiling, f.g, p, q, f.diag, f.partials, f.sc);
+    FFPM_LAUNCHED("k_spmv_dot");
+    double pqv;
+    if ((st = sums(1, f.tiling.blocks, &pqv))) return st;
+    double pr[2];
+    k_dot2<<<eb, kThreads, 0, stream()>>>(f.n, p, f.r, p, p, f.partials);
+    FFPM_LAUNCHED("k_dot2");
+    if ((st = sums(2, eb, pr))) return st;
+    if (!(pqv > 0)) break;  // breakdown: leave to the final check
+    const double alpha = pr[0] / pqv;
+    pq_old = pqv;
+    k_fcg_update<<<eb, kThreads, 0, stream()>>>(f.n, alpha, p, q, f.phi, f.r);
+    FFPM_LAUNCHED("k_fcg_update");
+    *sweeps += 1;
+    k += 1;
+    if (k % ci == 0 || *sweeps == max_sweeps) {
+      if ((st = true_residual(f, false, &res))) return st;
+      if (res < 0.99 * best) {
+        best = res;
+        stalled = 0;
+      } else if (++stalled >= 3) {
+        break;
+      }
+      if (!(res == res)) break;
+    }
+  }
+  if ((st = true_residual(f, false, &res))) return st;
+  if (*sweeps < max_sweeps && res > tolerance) {
+    // fall back to diagonal PCG from the current phi
+    *a.fallback_at = k;
+    int more = 0;
+    if ((st = pcg_iterate_fused(f, tolerance, max_sweeps - *sweeps, check_interval, &more,
+                                &res)))
+      return st;
+    *sweeps += more;
+  }
+  *residual = res;
+  return OK;
+}
+
 int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid,
                         const double *bg, const double *xlo, const double *xhi,
                         const double *ylo, const double *yhi, const double *zlo,
@@ -518,9 +688,9 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
                         const double *dx, const double *dy, const double *dz, int nx, int ny,
                         int nz, int west, int east, int south, int north, double tolerance,
                         int max_sweeps, int check_interval, int method, int *sweeps,
-                        double *residual) {
+                        double *residual, const FcgArgs *fcg = nullptr) {
   if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI &&
-      method != FFPM_METHOD_PCG_MIXED) {
+      method != FFPM_METHOD_PCG_MIXED && !(method == FFPM_METHOD_FCG && fcg)) {
     set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
     return ERR_METHOD;
   }
@@ -535,7 +705,8 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   }
   if (!all_present({phi_ref, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz}))
     return null_error();
-  const bool pcg = method == FFPM_METHOD_PCG || method == FFPM_METHOD_PCG_MIXED;
+  const bool pcg = method == FFPM_METHOD_PCG || method == FFPM_METHOD_PCG_MIXED ||
+                   method == FFPM_METHOD_FCG;
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, pcg);
   if (st) return st;
@@ -558,7 +729,9 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   FFPM_LAUNCHED("k_pack_phi");
   double *cur = f.phi;
   g_mixed = MixedStats();
-  if (method == FFPM_METHOD_PCG_MIXED) {
+  if (method == FFPM_METHOD_FCG) {
+    st = fcg_iterate(f, *fcg, tolerance, max_sweeps, check_interval, sweeps, residual);
+  } else if (method == FFPM_METHOD_PCG_MIXED) {
     st = pcg_mixed(f, tolerance, max_sweeps, check_interval, mixed_inner_rtol(), sweeps,
                    residual);
   } else if (pcg) {
@@ -856,6 +1029,55 @@ int ffpm_solve_stairstep_poisson_3d_dev(
   int st = solve_stairstep_dev(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc,
                                zc, dx, dy, dz, nx, ny, nz, west, east, south, north, tolerance,
                                max_sweeps, check_interval, method, sweeps, residual);
+  if (st) return st;
+  return sync();
+}
+
+int ffpm_stairstep_apply_operator_dev(const int32_t *solid, const double *xc, const double *yc,
+                                      const double *zc, const double *dx, const double *dy,
+                                      const double *dz, int nx, int ny, int nz, const double *x,
+                                      double *y) {
+  if (!all_present({solid, xc, yc, zc, dx, dy, dz, x, y})) return null_error();
+  if (nx < 1 || ny < 1 || nz < 2) {
+    set_error(ERR_SHAPE, "stair-step solve needs nz>=2 and nonempty ranks");
+    return ERR_SHAPE;
+  }
+  Frame f;
+  int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, false);
+  if (st) return st;
+  // the operator only: zero right-hand side and face data
+  Workspace &ws = workspace();
+  const size_t nf = (size_t)nx * ny * nz;
+  double *zero = (double *)ws.get(S_U_F8, sizeof(double) * nf, &st);
+  if (!zero) return st;
+  FFPM_HIP(hipMemsetAsync(zero, 0, sizeof(double) * nf, stream()));
+  const int eb = elementwise_blocks(f.n);
+  FFPM_HIP(hipMemsetAsync(f.isolated, 0, sizeof(int), stream()));
+  k_build_operator<<<eb, kThreads, 0, stream()>>>(f.g, solid, zero, zero, zero, zero, zero, zero,
+                                                  zero, 0.0, 0.0, 0.0, 1, f.src, f.diag, f.rdiag,
+                                                  f.code, f.isolated);
+  FFPM_LAUNCHED("k_build_operator");
+  k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
+      f.tiling, f.g, x, y, f.diag, f.partials, f.sc);
+  FFPM_LAUNCHED("k_spmv_dot");
+  return sync();
+}
+
+int ffpm_solve_stairstep_fcg_dev(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+    double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, const float *features,
+    int *sweeps, double *residual, int *fallback_at) {
+  if (!net || !features || !fallback_at) {
+    set_error(ERR_ARGUMENT, "flexible CG needs a network, its features and an output");
+    return ERR_ARGUMENT;
+  }
+  FcgArgs a{reinterpret_cast<UNet *>(net), features, fallback_at};
+  int st = solve_stairstep_dev(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc,
+                               zc, dx, dy, dz, nx, ny, nz, -1, -1, -1, -1, tolerance, max_sweeps,
+                               check_interval, FFPM_METHOD_FCG, sweeps, residual, &a);
   if (st) return st;
   return sync();
 }

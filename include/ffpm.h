@@ -57,7 +57,8 @@ enum {
  * no counterpart in the reference: FP64 defect correction around an FP32 PCG
  * (FP32 stencil and preconditioner, FP64 dots), stopping on the same FP64
  * max-norm residual; sweeps then counts inner iterations. */
-enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1, FFPM_METHOD_PCG_MIXED = 2 };
+enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1, FFPM_METHOD_PCG_MIXED = 2,
+       FFPM_METHOD_FCG = 3 /* only through ffpm_solve_stairstep_fcg_dev */ };
 
 const char *ffpm_last_error(void);
 /* Sets the HIP stream (hipStream_t) every later call launches on; 0 = null stream. */
@@ -148,6 +149,54 @@ typedef struct ffpm_stairstep_problem {
 int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
                                    double tolerance, int max_sweeps, int check_interval,
                                    int rows, int zchunk, int use_graph);
+/* Learned preconditioner (no counterpart in the reference): a 3-D U-Net
+ * (3 levels, 16/32/64 channels, 3x3x3 convolutions, ~0.36M parameters), BF16
+ * weights on MFMA with FP32 accumulation, trained by its own forward, backward
+ * and Adam kernels. Device arrays: features (nx,ny,nz,8) FP32 channels-last
+ * (fluid mask, signed distance, dx, dy, dz, direction one-hot), source and
+ * out (nx,ny,nz) FP32; every extent a multiple of 4 and nx >= 16. */
+typedef struct ffpm_unet ffpm_unet;
+int ffpm_unet_create(ffpm_unet **net, uint64_t seed);
+int ffpm_unet_destroy(ffpm_unet *net);
+int ffpm_unet_param_count(ffpm_unet *net, size_t *count);
+/* FP32 master parameters (setting them also resets the Adam moments). */
+int ffpm_unet_set_params(ffpm_unet *net, const float *host, size_t count);
+int ffpm_unet_get_params(ffpm_unet *net, float *host, size_t count);
+/* Gradients of the last training step (before its Adam update). */
+int ffpm_unet_get_grads(ffpm_unet *net, float *host, size_t count);
+int ffpm_unet_forward_dev(ffpm_unet *net, int nx, int ny, int nz, const float *features,
+                          const float *source, float *out);
+/* One step on the masked mean-square loss against target: forward, backward,
+ * and an Adam update with learning rate lr (lr <= 0: gradients only). */
+int ffpm_unet_train_step_dev(ffpm_unet *net, int nx, int ny, int nz, const float *features,
+                             const float *source, const float *target, const float *mask,
+                             float lr, float *loss);
+
+/* y = A x for the PCG operator of the frame (A = V (diag - offdiag), 0 in
+ * solid cells; src/fpm_stairstep_mpi_3d.f90:246-256), x and y dense
+ * (nx,ny,nz) device arrays. No counterpart in the reference; used to form
+ * training pairs for the network. */
+int ffpm_stairstep_apply_operator_dev(const int32_t *solid, const double *xc, const double *yc,
+                                      const double *zc, const double *dx, const double *dy,
+                                      const double *dz, int nx, int ny, int nz, const double *x,
+                                      double *y);
+
+/* Flexible CG preconditioned by the network (no counterpart in the
+ * reference): arguments as ffpm_solve_stairstep_poisson_3d_dev (one rank),
+ * plus the network and its features for this frame and direction. The first
+ * preconditioned step from the given phi is the network's initial guess;
+ * three convergence checks without a 1% gain switch to diagonal PCG from the
+ * current phi (fallback_at = FCG iterations before the switch, -1 if none).
+ * sweeps counts FCG plus PCG iterations; residual is the FP64 max-norm
+ * residual of src/fpm_stairstep_mpi_3d.f90:514-530. */
+int ffpm_solve_stairstep_fcg_dev(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+    double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, const float *features,
+    int *sweeps, double *residual, int *fallback_at);
+
 /* Times `iterations` batched iterations for each candidate launch shape
  * (rows 4, 8 x z chunks 8, 16, 32) and returns the fastest; phi is not
  * written. */

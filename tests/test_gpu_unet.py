@@ -1,0 +1,165 @@
+"""GPU tests of the U-Net kernels (BF16 MFMA, FP32 accumulation) against the
+CPU FP64 restatement (tests/unet_ref.py), of training, and of the
+network-preconditioned flexible CG.
+
+Tolerances (BF16): the reference rounds weights, input and stored
+activations to BF16 exactly where the kernels do, so the forward agrees to
+FP32-vs-FP64 accumulation roundoff (relative L2 <= 1e-2); the GPU also stores
+gradient tensors in BF16, so each parameter tensor's gradient agrees to
+relative L2 <= 5e-2 (cosine >= 0.998). Adam agrees with a numpy Adam step on
+the GPU's own gradients to FP32 roundoff.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import unet_ref as R
+from flowfpmpp_amd import learned as L
+from flowfpmpp_amd import stairstep as S
+from flowfpmpp_amd import workloads as W
+from flowfpmpp_amd.device import DeviceArray
+
+pytestmark = pytest.mark.gpu
+T = DeviceArray.from_host
+
+
+def _sample(nx=32, ny=24, nz=16, seed=0):
+    rng = np.random.default_rng(seed)
+    p = W.stretched_frame(nx, ny, nz, [W.Sphere((0.0, 0.0, 0.0), 1.0)], seed=seed,
+                          with_fields=False)
+    feats = L.features(p, 1)
+    source = rng.standard_normal((nx, ny, nz)).astype(np.float32)
+    target = rng.standard_normal((nx, ny, nz)).astype(np.float32)
+    mask = feats[0].copy()
+    return feats, source, target, mask
+
+
+def test_param_count_and_roundtrip():
+    print("case: unet parameters", flush=True)
+    net = L.UNet(seed=3)
+    assert net.nparams == R.TOTAL
+    a = net.get_params()
+    assert np.isfinite(a).all() and np.abs(a).max() > 0
+    b = np.random.default_rng(0).standard_normal(R.TOTAL).astype(np.float32) * 0.1
+    net.set_params(b)
+    assert np.array_equal(net.get_params(), b)
+
+
+def test_forward_matches_cpu_fp64():
+    print("case: unet forward 32x24x16 vs CPU", flush=True)
+    feats, source, _, _ = _sample()
+    net = L.UNet(seed=5)
+    flat = net.get_params()
+    out = DeviceArray(source.shape, np.float32)
+    net.forward(T(feats), T(source), out)
+    g = out.download().astype(np.float64)
+    ps, hw, hb = R.unpack(flat, True)
+    ref = R.forward(ps, hw, hb, feats, source, emulate=True).detach().numpy()
+    rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    print(f"forward relative L2 error {rel:.2e}")
+    assert rel <= 1e-2
+
+
+def test_gradients_match_cpu_fp64():
+    print("case: unet gradients 32x24x16 vs CPU", flush=True)
+    feats, source, target, mask = _sample(seed=1)
+    net = L.UNet(seed=7)
+    flat = net.get_params()
+    loss = net.train_step(T(feats), T(source), T(target), T(mask), 0.0)  # gradients only
+    g = net.get_grads().astype(np.float64)
+    rloss, _, rg = R.loss_and_grads(flat, feats, source, target, mask, emulate=True)
+    assert abs(loss - rloss) <= 1e-2 * rloss
+    worst = 0.0
+    for l, (cin, cout) in enumerate(R.SPEC):
+        for a, b in ((R.W_OFF[l], R.W_OFF[l] + 27 * cin * cout), (R.B_OFF[l], R.B_OFF[l] + cout)):
+            x, y = g[a:b], rg[a:b]
+            rel = np.linalg.norm(x - y) / np.linalg.norm(y)
+            cos = x @ y / (np.linalg.norm(x) * np.linalg.norm(y))
+            worst = max(worst, rel)
+            assert rel <= 5e-2 and cos >= 0.998, (l, a, rel, cos)
+    hx, hy = g[R.HEAD_W:R.TOTAL], rg[R.HEAD_W:R.TOTAL]
+    assert np.linalg.norm(hx - hy) <= 1e-2 * np.linalg.norm(hy)
+    print(f"worst per-tensor gradient relative L2 error {worst:.2e}")
+
+
+def test_adam_step_matches_numpy():
+    print("case: unet Adam", flush=True)
+    feats, source, target, mask = _sample(seed=2)
+    net = L.UNet(seed=9)
+    p0 = net.get_params().astype(np.float64)
+    lr = 1e-3
+    net.train_step(T(feats), T(source), T(target), T(mask), lr)
+    g = net.get_grads().astype(np.float64)
+    m = 0.1 * g
+    v = 0.001 * g * g
+    ref = p0 - lr * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    p1 = net.get_params().astype(np.float64)
+    assert np.abs(p1 - ref).max() <= 1e-6 + 1e-5 * np.abs(ref).max()
+
+
+def test_training_reduces_loss():
+    """A short run on solver-produced pairs (two 32^3 frames, three
+    directions, iterates after 5 and 20 iterations) lowers the loss."""
+    print("case: unet training 32^3", flush=True)
+    frames = L.training_frames(2, (32, 32, 32), seed0=7000)
+    pairs = L.make_pairs(frames, snapshots=(5, 20))
+    assert len(pairs) == 2 * 3 * 3
+    net = L.UNet(seed=11)
+    hist = L.train(net, pairs, 300, lr0=2e-3, lr1=2e-4)
+    first, last = np.mean(hist[:30]), np.mean(hist[-30:])
+    print(f"loss {first:.3e} -> {last:.3e}")
+    assert np.isfinite(hist).all() and last < 0.5 * first
+
+
+def _frame_problem(nx, ny, nz, seed, direction=0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = W.stretched_frame(nx, ny, nz, L.random_bodies(rng), seed=seed, with_fields=False)
+    rhs, bg, faces = W.influence_inputs(p, direction)
+    solid = T(p.solid)
+    axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+    d_rhs, d_faces = T(rhs), [T(f) for f in faces]
+    S.enforce_stairstep_compatibility_3d(d_rhs, solid, bg, *d_faces, *axes[3:])
+    return p, solid, axes, d_rhs, bg, d_faces
+
+
+def test_fcg_untrained_network_reaches_tolerance():
+    """Whatever the network, FCG ends at the FP64 tolerance (switching to
+    diagonal PCG when it stalls) and agrees with plain PCG's phi."""
+    print("case: FCG with an untrained network, 48x32x32", flush=True)
+    p, solid, axes, rhs, bg, faces = _frame_problem(48, 32, 32, seed=8100)
+    net = L.UNet(seed=13)
+    feats = T(L.features(p, 0))
+    phi = T(p.phi0())
+    sw, res, fb = L.solve_fcg(phi, rhs, solid, bg, *faces, *axes, 1e-10, 100000, 25, net, feats)
+    ref = T(p.phi0())
+    sw0, res0 = S.solve_stairstep_poisson_3d(ref, rhs, solid, bg, *faces, *axes, 1e-10, 100000,
+                                             25, method="pcg")
+    print(f"FCG {sw} iterations (fallback at {fb}), PCG {sw0}")
+    assert res <= 1e-10 and res0 <= 1e-10
+    a, b = phi.download(), ref.download()
+    assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+
+
+@pytest.mark.skipif(not os.path.exists(L.WEIGHTS), reason="no trained weights in the tree")
+def test_trained_network_ablation():
+    """What the trained network gains: FCG iterations to 1e-10 on an unseen
+    frame with the trained weights, against the same FCG with the untrained
+    initial weights and against diagonal PCG; all three reach the FP64
+    tolerance."""
+    print("case: FCG ablation, unseen 64x64x64 frame", flush=True)
+    p, solid, axes, rhs, bg, faces = _frame_problem(64, 64, 64, seed=9100, direction=2)
+    feats = T(L.features(p, 2))
+    out = {}
+    for name, net in (("trained", L.UNet.load()), ("untrained", L.UNet(seed=1))):
+        phi = T(p.phi0())
+        out[name] = L.solve_fcg(phi, rhs, solid, bg, *faces, *axes, 1e-10, 100000, 25, net,
+                                feats)
+    ref = T(p.phi0())
+    sw0, res0 = S.solve_stairstep_poisson_3d(ref, rhs, solid, bg, *faces, *axes, 1e-10, 100000,
+                                             25, method="pcg")
+    print(f"trained FCG {out['trained']}, untrained FCG {out['untrained']}, PCG {sw0}")
+    assert all(v[1] <= 1e-10 for v in out.values()) and res0 <= 1e-10
+    assert out["trained"][0] < out["untrained"][0]
+    assert out["trained"][2] == -1  # no fallback with the trained network
+    assert out["trained"][0] < sw0
