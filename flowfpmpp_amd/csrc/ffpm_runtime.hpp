@@ -46,8 +46,8 @@ enum Slot : int {
   S_L_NEXT = 80, S_L_COEF,
   // batched solves (one arena)
   S_BATCH = 90,
-  // flexible CG: the network's source field
-  S_FCG_SRC = 91,
+  // flexible CG: the network's source field and the two-term preconditioner
+  S_FCG_SRC = 91, S_FCG_U, S_FCG_AU, S_FCG_AN, S_FCG_N, S_FCG_RING,
 };
 
 // ---- kernel timing with HIP events on the library stream --------------------

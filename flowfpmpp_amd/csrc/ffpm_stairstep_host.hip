@@ -511,17 +511,20 @@ int jacobi_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
 
 }  // namespace
 
-// The whole solve on device pointers (src/fpm_stairstep_mpi_3d.f90:106-330).
 // ------------------------------------------------ learned flexible CG ------
-// Flexible CG with one-vector truncation (Notay's FCG(1)) preconditioned by
-// the U-Net, in FP64 except inside the network: z = sigma * net(features,
-// (r / V) / sigma), sigma = max |r / V| over fluid cells, z = 0 in solids;
-// p = z - (z.q_old / p_old.q_old) p_old; q = A p; alpha = p.r / p.q. With the
-// initial phi the first step x += alpha z is the network's initial guess.
+// Flexible CG with truncated orthogonalisation (Notay's FCG(m), m = 8), FP64
+// except inside the network, with a two-term preconditioner: z = a u + g n,
+// u = M_D^-1 r (the diagonal preconditioner of the plain PCG) and n = sigma *
+// net(features, (r / V) / sigma) (sigma = max |r / V| over fluid cells, 0 in
+// solids), where (a, g) minimise the A-norm error of z as a correction (a 2x2
+// Galerkin system from u.Au, u.An, n.An, u.r, n.r). Then p = z minus its
+// A-projections on the last m directions, q = A p formed from Au, An and the
+// stored q's, alpha = p.r / p.q.
+// With the initial phi the first step is the network-assisted initial guess.
 // Every check_interval iterations the FP64 true residual (the reference's
-// projected residual and max-norm) replaces r and the direction restarts;
-// three checks without a 1% improvement switch to diagonal PCG from the
-// current phi. The answer is always judged by the FP64 max-norm residual.
+// projected residual and max-norm) replaces r; three checks without a 1%
+// improvement switch to diagonal PCG from the current phi. The answer is
+// always judged by the FP64 max-norm residual.
 __global__ __launch_bounds__(256) void k_fcg_absmax(Geom g, const double *__restrict__ r,
                                                     const double *__restrict__ diag,
                                                     double *partials) {
@@ -573,12 +576,6 @@ __global__ __launch_bounds__(256) void k_dot2(size_t n, const double *__restrict
   store_partials(partials, 2, s, gridDim.x);
 }
 
-__global__ void k_fcg_dir(size_t n, const double *__restrict__ z, double beta,
-                          double *__restrict__ p) {
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x)
-    p[c] = z[c] - beta * p[c];
-}
 
 __global__ void k_fcg_update(size_t n, double alpha, const double *__restrict__ p,
                              const double *__restrict__ q, double *__restrict__ x,
@@ -596,18 +593,86 @@ struct FcgArgs {
   int *fallback_at;  // FCG iterations before the switch to PCG, or -1
 };
 
+// Truncated orthogonalisation of flexible CG: p = z - sum_j c_j P_j and
+// q = a Au + g An - sum_j c_j Q_j over the stored directions.
+constexpr int kFcgMax = 8;
+struct Combo {
+  const double *v[kFcgMax];
+  double c[kFcgMax];
+  int m;
+};
+
+__global__ void k_fcg_orth(size_t n, const double *__restrict__ z, Combo P,
+                           double *__restrict__ p) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    double v = z[e];
+    for (int j = 0; j < P.m; ++j) v -= P.c[j] * P.v[j][e];
+    p[e] = v;
+  }
+}
+
+__global__ void k_fcg_orth_q(size_t n, double a, const double *__restrict__ au, double g,
+                             const double *__restrict__ an, Combo Q, double *__restrict__ q) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    double v = a * au[e] + g * an[e];
+    for (int j = 0; j < Q.m; ++j) v -= Q.c[j] * Q.v[j][e];
+    q[e] = v;
+  }
+}
+
+__global__ void k_fcg_combine(size_t n, double a, const double *__restrict__ u, double g,
+                              const double *__restrict__ nn, double *__restrict__ z) {
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    z[c] = a * u[c] + g * nn[c];
+}
+
+__global__ void k_fcg_jacobi(size_t n, const double *__restrict__ rdiag,
+                             const double *__restrict__ r, double *__restrict__ u) {
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    u[c] = rdiag[c] * r[c];
+}
+
+// Directions kept by the truncated orthogonalisation (FFPM_FCG_M, 1..8).
+int fcg_depth() {
+  const char *e = std::getenv("FFPM_FCG_M");
+  int m = e ? std::atoi(e) : kFcgMax;
+  return m < 1 ? 1 : (m > kFcgMax ? kFcgMax : m);
+}
+
 int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
                 int check_interval, int *sweeps, double *residual) {
   const int ci = check_interval > 1 ? check_interval : 1;
   const int eb = elementwise_blocks(f.n);
+  const int m = fcg_depth(), ring = m + 1;
+  const size_t bytes = sizeof(double) * f.n;
   int st = 0;
   Workspace &ws = workspace();
   float *src = (float *)ws.get(S_FCG_SRC, sizeof(float) * f.n, &st);
   if (!src) return st;
-  double *p = f.next, *q = f.q, *z = f.z;  // r in f.r, x in f.phi
+  double *u = (double *)ws.get(S_FCG_U, bytes, &st);
+  if (!u) return st;
+  double *au = (double *)ws.get(S_FCG_AU, bytes, &st);
+  if (!au) return st;
+  double *an = (double *)ws.get(S_FCG_AN, bytes, &st);
+  if (!an) return st;
+  double *nn = (double *)ws.get(S_FCG_N, bytes, &st);
+  if (!nn) return st;
+  double *pr_ring = (double *)ws.get(S_FCG_RING, 2 * ring * bytes, &st);
+  if (!pr_ring) return st;
+  std::vector<double *> P(ring), Q(ring);
+  for (int j = 0; j < ring; ++j) {
+    P[j] = pr_ring + (size_t)j * f.n;
+    Q[j] = pr_ring + (size_t)(ring + j) * f.n;
+  }
+  std::vector<double> pq(ring, 0.0);
+  double *z = f.z;  // r in f.r, x in f.phi
   double res;
   if ((st = true_residual(f, false, &res))) return st;
-  double best = res, pq_old = 0.0;
+  double best = res;
   int stalled = 0, k = 0;
   *a.fallback_at = -1;
   auto sums = [&](int nslots, int nblocks, double *out) -> int {
@@ -617,8 +682,21 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
     for (int s2 = 0; s2 < nslots; ++s2) out[s2] = h.sums[s2];
     return e;
   };
+  auto spmv = [&](const double *x, double *y, double *xy) -> int {
+    k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
+        f.tiling, f.g, x, y, f.diag, f.partials, f.sc);
+    FFPM_LAUNCHED("k_spmv_dot");
+    return sums(1, f.tiling.blocks, xy);
+  };
+  auto dot2 = [&](const double *x1, const double *y1, const double *x2, const double *y2,
+                  double *out) -> int {
+    k_dot2<<<eb, kThreads, 0, stream()>>>(f.n, x1, y1, x2, y2, f.partials);
+    FFPM_LAUNCHED("k_dot2");
+    return sums(2, eb, out);
+  };
   while (*sweeps < max_sweeps && res > tolerance) {
-    // z = M(r)
+    // u = M_D^-1 r and n = sigma net(r / (V sigma))
+    k_fcg_jacobi<<<eb, kThreads, 0, stream()>>>(f.n, f.rdiag, f.r, u);
     k_fcg_absmax<<<eb, kThreads, 0, stream()>>>(f.g, f.r, f.diag, f.partials);
     FFPM_LAUNCHED("k_fcg_absmax");
     finish(f, FIN_MAX, 1, eb);  // sigma -> sc->res
@@ -626,33 +704,50 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
     FFPM_LAUNCHED("k_fcg_source");
     const float *out = nullptr;
     if ((st = unet_apply(*a.net, f.nx, f.ny, f.nz, a.features, src, &out))) return st;
-    k_fcg_correction<<<eb, kThreads, 0, stream()>>>(f.n, out, f.diag, f.sc, z);
+    k_fcg_correction<<<eb, kThreads, 0, stream()>>>(f.n, out, f.diag, f.sc, nn);
     FFPM_LAUNCHED("k_fcg_correction");
-    // p = z - beta p_old
-    double beta = 0.0;
-    if (k % ci != 0) {
-      double v[2];
-      k_dot2<<<eb, kThreads, 0, stream()>>>(f.n, z, q, z, z, f.partials);
-      FFPM_LAUNCHED("k_dot2");
-      if ((st = sums(2, eb, v))) return st;
-      beta = pq_old != 0.0 ? v[0] / pq_old : 0.0;
+    // Galerkin weights of the two terms
+    double uau, nan_, v[2], w[2];
+    if ((st = spmv(u, au, &uau))) return st;
+    if ((st = spmv(nn, an, &nan_))) return st;
+    if ((st = dot2(u, an, u, f.r, v))) return st;   // u.An, u.r
+    if ((st = dot2(nn, f.r, nn, nn, w))) return st;  // n.r
+    const double uan = v[0], ur = v[1], nr = w[0];
+    double ca = uau > 0 ? ur / uau : 0.0, cg = 0.0;
+    const double det = uau * nan_ - uan * uan;
+    if (nan_ > 0 && det > 1e-12 * uau * nan_) {
+      ca = (nan_ * ur - uan * nr) / det;
+      cg = (uau * nr - uan * ur) / det;
     }
-    k_fcg_dir<<<eb, kThreads, 0, stream()>>>(f.n, z, beta, p);
-    FFPM_LAUNCHED("k_fcg_dir");
-    // q = A p, p.q, p.r
-    k_spmv_dot<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
-        f.tiling, f.g, p, q, f.diag, f.partials, f.sc);
-    FFPM_LAUNCHED("k_spmv_dot");
-    double pqv;
-    if ((st = sums(1, f.tiling.blocks, &pqv))) return st;
-    double pr[2];
-    k_dot2<<<eb, kThreads, 0, stream()>>>(f.n, p, f.r, p, p, f.partials);
-    FFPM_LAUNCHED("k_dot2");
-    if ((st = sums(2, eb, pr))) return st;
-    if (!(pqv > 0)) break;  // breakdown: leave to the final check
-    const double alpha = pr[0] / pqv;
-    pq_old = pqv;
-    k_fcg_update<<<eb, kThreads, 0, stream()>>>(f.n, alpha, p, q, f.phi, f.r);
+    k_fcg_combine<<<eb, kThreads, 0, stream()>>>(f.n, ca, u, cg, nn, z);
+    FFPM_LAUNCHED("k_fcg_combine");
+    // orthogonalise against the last min(k, m) directions (A-inner product)
+    Combo cp{}, cq{};
+    const int used = k < m ? k : m;
+    for (int t = 0; t < used; t += 2) {
+      const int j0 = (k - 1 - t) % ring, j1 = (k - 2 - t + ring) % ring;
+      double zq[2] = {0.0, 0.0};
+      if ((st = dot2(z, Q[j0], z, t + 1 < used ? Q[j1] : Q[j0], zq))) return st;
+      for (int h = 0; h < 2 && t + h < used; ++h) {
+        const int j = h ? j1 : j0;
+        const double c = pq[j] > 0 ? zq[h] / pq[j] : 0.0;
+        cp.v[cp.m] = P[j];
+        cp.c[cp.m++] = c;
+        cq.v[cq.m] = Q[j];
+        cq.c[cq.m++] = c;
+      }
+    }
+    const int slot = k % ring;
+    k_fcg_orth<<<eb, kThreads, 0, stream()>>>(f.n, z, cp, P[slot]);
+    k_fcg_orth_q<<<eb, kThreads, 0, stream()>>>(f.n, ca, au, cg, an, cq, Q[slot]);
+    FFPM_LAUNCHED("k_fcg_orth");
+    double prq[2];
+    if ((st = dot2(P[slot], f.r, P[slot], Q[slot], prq))) return st;
+    const double pqv = prq[1];
+    if (!(pqv > 0)) break;  // breakdown: the final check decides
+    const double alpha = prq[0] / pqv;
+    pq[slot] = pqv;
+    k_fcg_update<<<eb, kThreads, 0, stream()>>>(f.n, alpha, P[slot], Q[slot], f.phi, f.r);
     FFPM_LAUNCHED("k_fcg_update");
     *sweeps += 1;
     k += 1;
@@ -681,6 +776,7 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
   return OK;
 }
 
+// The whole solve on device pointers (src/fpm_stairstep_mpi_3d.f90:106-330).
 int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid,
                         const double *bg, const double *xlo, const double *xhi,
                         const double *ylo, const double *yhi, const double *zlo,

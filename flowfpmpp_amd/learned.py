@@ -5,7 +5,8 @@ the plain solver produces, and the network-preconditioned flexible CG solve.
 The network maps a residual r to an approximation of A^-1 r. It sees the
 frame (fluid mask, signed distance, the three cell widths, the direction
 one-hot) and the source density r / V scaled by sigma = max |r / V|; its
-output times sigma is the correction. Training pairs come from the plain PCG
+output (scaled by sigma) is the correction, whose length flexible CG's
+optimal step sets. Training pairs come from the plain PCG
 solver on random stair-step bodies: the converged phi* of each direction and
 the errors e_k = phi* - phi_k of its iterates after k iterations, with the
 exact residual r_k = A e_k (so the network learns A^-1 on the residuals FCG
@@ -28,12 +29,18 @@ WEIGHTS = os.path.join(HERE, "data", "unet_weights.npz")
 
 
 # ------------------------------------------------------------- features ------
-def features(p, direction: int) -> np.ndarray:
+def features(p, direction: int, base: np.ndarray = None) -> np.ndarray:
     """FP32 input features of frame p for force direction 0/1/2, shape
     (8, nx, ny, nz) in Fortran order (channels-last in memory): fluid mask,
     signed distance to the solid/fluid interface (positive in fluid, scaled by
     0.5 domain units and clipped to [-1, 2]), dx, dy, dz relative to their
-    axis means, and the direction one-hot."""
+    axis means, and the direction one-hot. ``base`` (the features of another
+    direction of the same frame) skips the geometry work."""
+    if base is not None:
+        f = np.array(base, order="F", copy=True)
+        f[5:8] = 0.0
+        f[5 + direction] = 1.0
+        return f
     from scipy import ndimage
     nx, ny, nz = p.nx, p.ny, p.nz
     fluid = p.solid[1:nx + 1, 1:ny + 1, :] == 0
@@ -184,11 +191,12 @@ class Pair:
         self.feats, self.source, self.target, self.mask = feats, source, target, mask
 
 
-def make_pairs(frames, snapshots=(10, 50, 200), log=None):
+def make_pairs(frames, snapshots=(10, 50, 200, 800), log=None):
     """Training pairs from the plain solver on the given frames (Problem
     objects): for every frame and direction the converged phi* (FP64 PCG to
     1e-10, batched) and the iterates after each snapshot count; pair =
-    (source (A e)/(V sigma), target e / sigma) with e = phi* - phi_k."""
+    (source (A e)/(V sigma), target e / rms(e)) with e = phi* - phi_k and
+    sigma = max |A e / V|."""
     from . import stairstep as S
     from . import workloads as W
     T = DeviceArray.from_host
@@ -214,11 +222,14 @@ def make_pairs(frames, snapshots=(10, 50, 200), log=None):
         S.solve_stairstep_batch(probs, -1.0, k, k)
         iterates[k] = [q.phi.download() for q in probs]
     pairs = []
+    base = {}
     for idx, (p, d, solid, axes) in enumerate(meta):
         nx, ny, nz = p.nx, p.ny, p.nz
         fluid = (p.solid[1:nx + 1, 1:ny + 1, :] == 0)
         vol = (p.dx[:, None, None] * p.dy[None, :, None] * p.dz[None, None, :])
-        feats = T(features(p, d))
+        f = features(p, d, base.get(id(p)))
+        base[id(p)] = f
+        feats = T(f)
         mask = T(fluid.astype(np.float32))
         for k in (0,) + tuple(snapshots):
             x = iterates[k][idx] if k else np.zeros_like(star[idx])
@@ -228,10 +239,13 @@ def make_pairs(frames, snapshots=(10, 50, 200), log=None):
             apply_operator(solid, *axes, de, r)
             rv = r.download() / vol
             sigma = np.abs(rv[fluid]).max()
-            if not sigma > 0:
+            rms = np.sqrt(np.mean(e[fluid] ** 2))
+            if not (sigma > 0 and rms > 0):
                 continue
+            # Only the direction of the correction matters to flexible CG (its
+            # step length is optimal), so the target is e at unit RMS.
             pairs.append(Pair(feats, T((rv / sigma).astype(np.float32)),
-                              T((e / sigma).astype(np.float32)), mask))
+                              T((e / rms).astype(np.float32)), mask))
     synchronize()
     return pairs
 
