@@ -27,6 +27,9 @@ __global__ void k_pcg_upd_t(Tiling t, GeomT<T> g, const T *p, const uint8_t *cod
                             T *z, double *partials, Scalars *sc);
 template <typename T>
 __global__ void k_dir_t(size_t n, T *p, const T *z, const Scalars *sc);
+template <int CL>
+__global__ void k_jacobi_t(Tiling t, GeomT<double> g, const double *a, double *b,
+                           const double *src, const uint8_t *code);
 __global__ void k_mixed_begin(GeomT<float> g, const double *r64, const uint8_t *code,
                               float *r32, float *z32, float *e32, double *partials);
 __global__ void k_mixed_accumulate(size_t n, double *phi, const float *e32);

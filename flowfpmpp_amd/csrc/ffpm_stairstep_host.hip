@@ -484,17 +484,34 @@ bool classic_pcg() {
 
 // jacobi_iterate (src/fpm_stairstep_mpi_3d.f90:200-225) with ping-pong
 // buffers: the current iterate is tracked instead of copied back.
+// The one-cell Jacobi kernel with the stored rdiag (FFPM_JACOBI=classic), for
+// comparison.
+bool classic_jacobi() {
+  const char *e = std::getenv("FFPM_JACOBI");
+  return e && std::strcmp(e, "classic") == 0;
+}
+
 int jacobi_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
                    double *residual, double **current) {
   double *cur = f.phi, *oth = f.next;
   while (*sweeps < max_sweeps && *residual > tolerance) {
     int nsteps = check_interval > 1 ? check_interval : 1;
     if (max_sweeps - *sweeps < nsteps) nsteps = max_sweeps - *sweeps;
+    const int cl = cells_per_lane<double>(f.nx);
+    const Tiling &t = f.tl[cl];
+    const GeomT<double> g = geom_t(f.g);
     for (int s = 0; s < nsteps; ++s) {
       {
         TimedLaunch tl("jacobi");
-        k_jacobi<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(f.tiling, f.g, cur, oth,
-                                                                         f.src, f.rdiag);
+        if (classic_jacobi())
+          k_jacobi<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
+              f.tiling, f.g, cur, oth, f.src, f.rdiag);
+        else if (cl == 2)
+          k_jacobi_t<2><<<t.blocks, dim3(kTileX, t.ty), 0, stream()>>>(t, g, cur, oth, f.src,
+                                                                        f.code);
+        else
+          k_jacobi_t<1><<<t.blocks, dim3(kTileX, t.ty), 0, stream()>>>(t, g, cur, oth, f.src,
+                                                                        f.code);
       }
       double *t = cur;
       cur = oth;
