@@ -42,12 +42,27 @@ BYTES_PER_CELL = {
     "spmv_dot": 24,      # read p, diag; write q
     "update_dot": 56,    # read phi, p, r, q, rdiag; write phi, r
     "direction": 32,     # read rdiag, r, p; write p
-    "jacobi": 32,        # read a, src, rdiag; write b
+    "jacobi": 32,        # read a, src, rdiag; write b (the reference's sweep; this one
+                         # reads the 1-byte neighbour code instead of rdiag)
     "true_residual_1": 32,  # phi, src, diag; write r
     "true_residual_2": 32,  # r (read, write), rdiag, diag
     "residual_max": 24,  # phi, src, diag
     "terms": 124,        # phi, solid(4), 9 fields; write 5 outputs
 }
+
+
+def measured_traffic(kernel, grid):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
+    / WRITE_SIZE measurement of this workload (profiles/hbm_traffic.json), or
+    None when the kernel or grid was not measured."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return k["traffic_bytes"] if k and t.get("grid") == grid else None
 
 
 def log(msg):
@@ -275,11 +290,14 @@ def main():
     roofline = None
     if dom:
         ach = kern[dom]["gbps"]
+        traffic = measured_traffic(dom, list(args.grid))
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": None,
+                    "traffic": traffic,
                     "bytes_per_cell": BYTES_PER_CELL[dom], "cells_per_launch": cells,
                     "avg_launch_us": round(kern[dom]["avg_us"], 2)}
+        if traffic:
+            roofline["traffic_gbps"] = round(traffic / (kern[dom]["avg_us"] * 1e-6) / 1e9, 1)
     out = {
         "metric": METRIC,
         "value": round(ms_per_step, 3),
