@@ -368,3 +368,38 @@ def test_streaming_jacobi_matches_one_cell_kernel(monkeypatch, grid):
     a, b = out["classic"], out["streaming"]
     assert a[1] == b[1] == 301 and a[2] == b[2]
     assert np.array_equal(a[0], b[0])
+
+
+def test_mixed_precision_on_the_128x96x64_frame():
+    """The mixed-precision workload: the 128x96x64 frame (sphere r=0.5 and a
+    box touching the outer boundary), all three directions, FP32-inner /
+    FP64-outer PCG to the FP64 residual 1e-10, against the FP64 PCG solve of
+    the same frame (which matches the CPU oracle to 1e-10, see the tests
+    above). Reported: max |phi - phi_FP64| relative to max |phi_FP64| and the
+    vortex/advection density errors; bounds 1e-8 and 1e-6 relative."""
+    print("case: mixed precision, 128x96x64 frame, 3 directions", flush=True)
+    p = W.frame_128x96x64()
+    f = p.fields
+    for d in range(3):
+        rhs, bg, faces = W.influence_inputs(p, d)
+        S.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
+        out = {}
+        for method in ("pcg", "pcg-mixed"):
+            ph = p.phi0()
+            sw, res = S.solve_stairstep_poisson_3d(ph, rhs, p.solid, bg, *faces, p.xc, p.yc,
+                                                   p.zc, p.dx, p.dy, p.dz, 1e-10, 100000, 100,
+                                                   method=method)
+            dens = S.compute_stairstep_fpm_terms_3d(
+                ph, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"],
+                f["az"], p.solid, bg, p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, *faces, p.bounds)
+            out[method] = (ph, sw, res, dens)
+        a, b = out["pcg-mixed"], out["pcg"]
+        assert a[2] <= 1e-10 and b[2] <= 1e-10
+        e_phi = np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()
+        e_vort = np.abs(a[3][3] - b[3][3]).max() / np.abs(b[3][3]).max()
+        e_adv = np.abs(a[3][4] - b[3][4]).max() / np.abs(b[3][4]).max()
+        outer, inner = S.last_mixed_solve()
+        print(f"direction {d}: FP64 PCG {b[1]} iterations; mixed {outer} corrections, {inner} "
+              f"FP32 iterations; max rel |phi - phi_FP64| {e_phi:.2e}, vortex density "
+              f"{e_vort:.2e}, advection density {e_adv:.2e}", flush=True)
+        assert e_phi <= 1e-8 and e_vort <= 1e-6 and e_adv <= 1e-6
