@@ -6,7 +6,7 @@
 namespace ffpm {
 
 template <typename T, int CL, int MODE>
-__global__ __launch_bounds__(512) void k_pcg_dir_t(Tiling t, GeomT<T> g, const T *__restrict__ z,
+__global__ __launch_bounds__(1024) void k_pcg_dir_t(Tiling t, GeomT<T> g, const T *__restrict__ z,
                                                    const uint8_t *__restrict__ code,
                                                    const T *__restrict__ pin, T *__restrict__ pout,
                                                    double *partials, Scalars *sc) {
@@ -15,7 +15,7 @@ __global__ __launch_bounds__(512) void k_pcg_dir_t(Tiling t, GeomT<T> g, const T
 }
 
 template <typename T, int CL>
-__global__ __launch_bounds__(512) void k_pcg_upd_t(Tiling t, GeomT<T> g, const T *__restrict__ p,
+__global__ __launch_bounds__(1024) void k_pcg_upd_t(Tiling t, GeomT<T> g, const T *__restrict__ p,
                                                    const uint8_t *__restrict__ code,
                                                    T *__restrict__ x, T *__restrict__ r,
                                                    T *__restrict__ z, double *partials,
@@ -32,7 +32,7 @@ __global__ __launch_bounds__(512) void k_pcg_upd_t(Tiling t, GeomT<T> g, const T
 // plane k+2 (own row) are issued before plane k is computed; west/east
 // neighbours by wave shuffle.
 template <int CL>
-__global__ __launch_bounds__(512) void k_jacobi_t(Tiling t, GeomT<double> g,
+__global__ __launch_bounds__(1024) void k_jacobi_t(Tiling t, GeomT<double> g,
                                                   const double *__restrict__ a,
                                                   double *__restrict__ b,
                                                   const double *__restrict__ src,

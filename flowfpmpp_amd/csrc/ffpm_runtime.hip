@@ -39,8 +39,13 @@ int env_int(const char *name, int dflt) {
 
 Tiling make_tiling(int nx, int ny, int nz, int zchunk, int ty, int cells_x) {
   Tiling t;
-  if (ty <= 0) ty = env_int("FFPM_TILE_Y", 4);
-  if (ty != 1 && ty != 2 && ty != 4 && ty != 8) ty = 4;
+  // 16 rows (1024 threads) only on request: most kernels are built for
+  // workgroups of at most 512 threads.
+  if (ty <= 0) {
+    ty = env_int("FFPM_TILE_Y", 4);
+    if (ty > 8) ty = 4;
+  }
+  if (ty != 1 && ty != 2 && ty != 4 && ty != 8 && ty != 16) ty = 4;
   t.ty = ty;
   t.tiles_x = (nx + cells_x - 1) / cells_x;
   t.tiles_y = (ny + ty - 1) / ty;

@@ -74,9 +74,11 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   f.n = (size_t)nx * ny * nz;
   f.tiling = make_tiling(nx, ny, nz, 0);
   size_t nb = (size_t)max_blocks_for(f.tiling, f.n);
-  // Eight-row workgroups measured best for the pipelined PCG kernels on a
-  // 512x384x256 frame (2, 4 and 8 rows within 10%); FFPM_TILE_Y overrides.
-  const int pcg_rows = std::getenv("FFPM_TILE_Y") ? 0 : 8;
+  // Rows per workgroup of the pipelined PCG and Jacobi kernels (built for up
+  // to 1024 threads): FFPM_PCG_ROWS (1, 2, 4, 8, 16), else FFPM_TILE_Y, else
+  // 8, measured best on a 512x384x256 frame.
+  const char *pr = std::getenv("FFPM_PCG_ROWS");
+  const int pcg_rows = pr ? std::atoi(pr) : (std::getenv("FFPM_TILE_Y") ? 0 : 8);
   for (int cl : {1, 2, 4}) {
     f.tl[cl] = make_tiling(nx, ny, nz, 0, pcg_rows, kTileX * cl);
     if ((size_t)f.tl[cl].blocks > nb) nb = (size_t)f.tl[cl].blocks;
