@@ -196,6 +196,87 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+# The reference's published A100 times (seconds) for the same frames:
+# benchmarks/pcg_time_to_solution.csv:5-9 and benchmarks/stairstep_benchmark.csv:9-12.
+A100_PCG_1E8 = {(128, 128, 32): (1000, 0.1759), (256, 256, 64): (1900, 1.249),
+                (512, 512, 64): (3600, 7.112)}
+A100_JACOBI_FRAME = {(64, 64, 16): (0.00702, 19.9), (128, 128, 32): (0.0279, 26.1),
+                     (256, 256, 64): (0.1862, 115.2), (512, 512, 64): (0.7039, 426.0)}
+A100_JACOBI_1E8 = {(128, 128, 32): (226000, 5.329)}
+
+
+def bench_reference(args):
+    """The reference's benchmark (tests/benchmark_stairstep_3d.f90:51-77: a
+    stair-step sphere on a sinusoidally stretched grid, x influence field,
+    then the FPM terms) timed as the reference times it
+    (tests/benchmark_stairstep_3d.f90:142-177): compatibility + solve +
+    gauge + FPM terms including every host-device transfer, median of 3
+    after a warm-up; host numpy arrays through the host-pointer entry
+    points. PCG to 1e-8 (check 100), the 250-sweep Jacobi frame, the
+    marginal Jacobi sweep (solve at 2500 minus 250 sweeps, / 2250) and Jacobi
+    to 1e-8 at 128x128x32."""
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd import workloads as W
+
+    def frame_time(p, method, tol, cap, check):
+        rhs, bg, faces = W.influence_inputs(p, 0)
+        f = p.fields
+
+        def once():
+            fc = [np.array(a, order="F", copy=True) for a in faces]
+            phi = p.phi0()
+            t0 = time.perf_counter()
+            S.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *fc, p.dx, p.dy, p.dz)
+            sw, res = S.solve_stairstep_poisson_3d(phi, rhs, p.solid, bg, *fc, p.xc, p.yc, p.zc,
+                                                   p.dx, p.dy, p.dz, tol, cap, check,
+                                                   method=method)
+            S.compute_stairstep_fpm_terms_3d(phi, f["u"], f["v"], f["w"], f["ox"], f["oy"],
+                                             f["oz"], f["ax"], f["ay"], f["az"], p.solid, bg,
+                                             p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, *fc, p.bounds)
+            return time.perf_counter() - t0, sw, res
+
+        once()
+        runs = [once() for _ in range(3)]
+        t = sorted(r[0] for r in runs)[1]
+        return t, runs[0][1], runs[0][2]
+
+    rows = []
+    for g, (it_ref, a100) in A100_PCG_1E8.items():
+        p = W.reference_benchmark(*g)
+        t, sw, res = frame_time(p, "pcg", 1e-8, 1000000, 100)
+        rows.append({"grid": list(g), "method": "pcg", "tol": 1e-8, "iterations": sw,
+                     "reference_iterations": it_ref, "residual": res, "frame_s": round(t, 4),
+                     "a100_frame_s": a100, "speedup_vs_a100": round(a100 / t, 2)})
+        log(f"PCG {g}: {sw} iterations, {t:.3f} s (A100 {a100} s)")
+    for g, (a100, a100_sweep) in A100_JACOBI_FRAME.items():
+        p = W.reference_benchmark(*g)
+        t, sw, res = frame_time(p, "jacobi", -1.0, 250, 250)
+        t250 = frame_time(p, "jacobi", -1.0, 250, 2500)[0]
+        t2500 = frame_time(p, "jacobi", -1.0, 2500, 2500)[0]
+        sweep_us = (t2500 - t250) / 2250 * 1e6
+        rows.append({"grid": list(g), "method": "jacobi", "sweeps": 250, "frame_s": round(t, 4),
+                     "a100_frame_s": a100, "speedup_vs_a100": round(a100 / t, 2),
+                     "sweep_us": round(sweep_us, 2), "a100_sweep_us": a100_sweep,
+                     "sweep_speedup_vs_a100": round(a100_sweep / sweep_us, 2)})
+        log(f"Jacobi {g}: 250-sweep frame {t:.4f} s (A100 {a100}), sweep {sweep_us:.1f} us "
+            f"(A100 {a100_sweep})")
+    for g, (sw_ref, a100) in A100_JACOBI_1E8.items():
+        p = W.reference_benchmark(*g)
+        t, sw, res = frame_time(p, "jacobi", 1e-8, 10000000, 100)
+        rows.append({"grid": list(g), "method": "jacobi", "tol": 1e-8, "sweeps": sw,
+                     "reference_sweeps": sw_ref, "residual": res, "frame_s": round(t, 4),
+                     "a100_frame_s": a100, "speedup_vs_a100": round(a100 / t, 2)})
+        log(f"Jacobi {g} to 1e-8: {sw} sweeps, {t:.3f} s (A100 {a100} s)")
+    pcg = [r for r in rows if r["method"] == "pcg"]
+    out = {"metric": "reference benchmark frame time (s): compatibility + solve + gauge + FPM "
+                     "terms with host-device transfers", "value": pcg[-1]["frame_s"], "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "tests/benchmark_stairstep_3d.f90 frames (sphere, sinusoidal "
+                                  "stretching), x influence field; value = 512x512x64 PCG to 1e-8"},
+           "vs_baseline": None, "rows": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,9 +288,10 @@ def main():
     ap.add_argument("--check", type=int, default=100)
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="frame", choices=["frame", "batch64"],
+    ap.add_argument("--workload", default="frame", choices=["frame", "batch64", "reference"],
                     help="frame: the 3-direction solve of one frame (the headline); batch64: "
-                         "64 uneven frames x 3 directions in one batched solve")
+                         "64 uneven frames x 3 directions in one batched solve; reference: "
+                         "the reference's own benchmark frames, timed as it times them")
     ap.add_argument("--sequential", action="store_true",
                     help="batch64: also time the 192 solves one after another")
     args = ap.parse_args()
@@ -223,6 +305,8 @@ def main():
 
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "reference":
+        return bench_reference(args)
 
     t_gen = time.perf_counter()
     p = frame(tuple(args.grid))
