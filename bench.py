@@ -141,14 +141,15 @@ def bench_batch(args):
     cells = sum(p.cells for p, _ in frames) * 3
     synchronize()
     log(f"64 frames x 3 directions resident: {cells} cells, {time.perf_counter() - t_gen:.1f} s")
-    rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20)
+    bmethod = args.method if args.method in ("pcg", "pcg-mixed") else "pcg"
+    rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20, method=bmethod)
     log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
 
     def step():
         for q in probs:
             q.phi.zero_()
         S.solve_stairstep_batch(probs, args.tol, args.max_iters, args.check, rows=rows,
-                                zchunk=zchunk)
+                                zchunk=zchunk, method=bmethod)
 
     for _ in range(args.warmup):
         step()
@@ -165,14 +166,15 @@ def bench_batch(args):
     res = np.array([q.residual for q in probs])
     log(f"batch: {ms_per_step:.1f} ms for 192 solves; iterations {sweeps.min()}-{sweeps.max()}")
     out = {
-        "metric": "batched solves/s: 64 uneven frames x 3 directions, FP64 PCG to 1e-10",
+        "metric": f"batched solves/s: 64 uneven frames x 3 directions, {bmethod} to 1e-10",
         "value": round(192 * 1e3 / ms_per_step, 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if bmethod == "pcg" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "64 frames (seeds 4000-4063), grids 64^3/96x80x64/128x96x64/"
                                "160x128x96, 1-3 bodies, 3 directions each; one batched "
-                               "FP64 PCG solve, check 100",
+                               f"{bmethod} solve, check 100",
                    "cells": cells, "tolerance": args.tol, "rows": rows, "zchunk": zchunk},
         "solve_ms_p50": round(float(np.percentile(solve_ms, 50)), 3),
         "solve_ms_p99": round(float(np.percentile(solve_ms, 99)), 3),
@@ -188,7 +190,7 @@ def bench_batch(args):
                 q.phi.zero_()
                 S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
                                              *q.axes, args.tol, args.max_iters, args.check,
-                                             method="pcg")
+                                             method=bmethod)
         synchronize()
         seq = time.perf_counter() - t0
         out["sequential_ms"] = round(seq * 1e3, 3)

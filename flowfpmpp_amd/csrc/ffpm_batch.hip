@@ -12,7 +12,10 @@
 // active mask: a converged, stalled or failed solve turns every later launch
 // of its workgroups into a no-op. One chunk of check_interval iterations plus
 // the convergence check is captured once as a HIP graph and replayed until no
-// solve is active; the host reads one counter per replay.
+// solve is active; the host reads one counter per replay. In mixed precision
+// each solve also runs the FP64 defect-correction loop of the standalone mixed
+// solve (FP32 inner PCG to a relative rz tolerance, FP64 accumulation and true
+// residual) as a per-solve phase on the device.
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -43,6 +46,37 @@ struct BSolve {
   double *x, *r, *z, *p[2];
   double *partials;
   Scalars *sc;
+  // FP32 inner solve of mixed precision
+  GeomT<float> g32;
+  Tiling tl32;
+  float *e32, *r32, *z32, *p32[2];
+};
+
+// Per-solve phase of the loop: the PCG iterations (FP64, or the FP32 inner
+// solve), and for mixed precision the start of a correction, its FP64
+// accumulation and the outer check.
+enum Phase : int { kPhIter = 0, kPhBegin = 1, kPhAcc = 2, kPhCheck = 3 };
+
+// Fields of the value type T's iteration (FP64 solve or FP32 inner solve).
+template <typename T>
+struct Fields;
+template <>
+struct Fields<double> {
+  __device__ static const GeomT<double> &g(const BSolve &d) { return d.g; }
+  __device__ static const Tiling &tl(const BSolve &d) { return d.tl; }
+  __device__ static double *z(const BSolve &d) { return d.z; }
+  __device__ static double *p(const BSolve &d, int i) { return d.p[i]; }
+  __device__ static double *x(const BSolve &d) { return d.x; }
+  __device__ static double *r(const BSolve &d) { return d.r; }
+};
+template <>
+struct Fields<float> {
+  __device__ static const GeomT<float> &g(const BSolve &d) { return d.g32; }
+  __device__ static const Tiling &tl(const BSolve &d) { return d.tl32; }
+  __device__ static float *z(const BSolve &d) { return d.z32; }
+  __device__ static float *p(const BSolve &d, int i) { return d.p32[i]; }
+  __device__ static float *x(const BSolve &d) { return d.e32; }
+  __device__ static float *r(const BSolve &d) { return d.r32; }
 };
 
 // Per-solve state of the convergence logic (the host loop of pcg_iterate).
@@ -51,8 +85,15 @@ struct BState {
   int need_tr;   // 0 none, 1 after a breakdown restart, 2 convergence check, 3 initial
   int need_dir;  // deferred direction update before the check
   int finished_at;
+  int phase;         // Phase
+  int outer, inner;  // mixed precision: corrections and FP32 iterations
   double res, best;
+  double target;     // mixed precision: inner stop rz <= target
 };
+
+__device__ inline bool iterating(const BState &st, int slot) {
+  return st.active && st.phase == kPhIter && slot < st.n_chunk;
+}
 
 // Workgroup -> (solve, index within the solve's launch) tables of one grid.
 struct Grid {
@@ -61,53 +102,63 @@ struct Grid {
 };
 
 // ------------------------------------------------------------ kernels ------
-template <int CL>
+template <typename T, int CL>
 __global__ __launch_bounds__(512) void k_b_dir(const BSolve *solves, Grid grid,
                                                const BState *state, int slot) {
   __shared__ double lds[16 * kReduceSlots];
   const int s = grid.solve[blockIdx.x];
   const BState &st = state[s];
-  if (!st.active || slot >= st.n_chunk) return;
+  if (!iterating(st, slot)) return;
   const BSolve &d = solves[s];
+  using F = Fields<T>;
   const int lb = grid.local[blockIdx.x];
-  const double *pin = d.p[st.cur];
-  double *pout = d.p[st.cur ^ 1];
+  const T *pin = F::p(d, st.cur);
+  T *pout = F::p(d, st.cur ^ 1);
+  const Tiling &t = F::tl(d);
   if (st.mode == kDirUpdate)
-    pcg_dir_body<double, CL, kDirUpdate>(d.tl, lb, d.tl.blocks, d.g, d.z, d.code, pin, pout,
-                                         d.partials, d.sc, lds);
+    pcg_dir_body<T, CL, kDirUpdate>(t, lb, t.blocks, F::g(d), F::z(d), d.code, pin, pout,
+                                    d.partials, d.sc, lds);
   else if (st.mode == kDirReset)
-    pcg_dir_body<double, CL, kDirReset>(d.tl, lb, d.tl.blocks, d.g, d.z, d.code, pin, pout,
-                                        d.partials, d.sc, lds);
+    pcg_dir_body<T, CL, kDirReset>(t, lb, t.blocks, F::g(d), F::z(d), d.code, pin, pout,
+                                   d.partials, d.sc, lds);
   else
-    pcg_dir_body<double, CL, kDirCurrent>(d.tl, lb, d.tl.blocks, d.g, d.z, d.code, pin, pout,
-                                          d.partials, d.sc, lds);
+    pcg_dir_body<T, CL, kDirCurrent>(t, lb, t.blocks, F::g(d), F::z(d), d.code, pin, pout,
+                                     d.partials, d.sc, lds);
 }
 
-template <int CL>
+template <typename T, int CL>
 __global__ __launch_bounds__(512) void k_b_upd(const BSolve *solves, Grid grid,
                                                const BState *state, int slot) {
   __shared__ double lds[16 * kReduceSlots];
   const int s = grid.solve[blockIdx.x];
   const BState &st = state[s];
-  if (!st.active || slot >= st.n_chunk) return;
+  if (!iterating(st, slot)) return;
   const BSolve &d = solves[s];
-  pcg_upd_body<double, CL>(d.tl, grid.local[blockIdx.x], d.tl.blocks, d.g, d.p[st.cur], d.code,
-                           d.x, d.r, d.z, d.partials, d.sc, lds);
+  using F = Fields<T>;
+  const Tiling &t = F::tl(d);
+  pcg_upd_body<T, CL>(t, grid.local[blockIdx.x], t.blocks, F::g(d), F::p(d, st.cur), d.code,
+                      F::x(d), F::r(d), F::z(d), d.partials, d.sc, lds);
 }
 
 // One workgroup per solve: the finish of the preceding reduction, plus the
 // bookkeeping the standalone host loop does between launches (which direction
-// buffer is current; the mode of the next direction kernel).
+// buffer is current; the mode of the next direction kernel). `prec` 1: the
+// iteration ran on the FP32 fields. Which solves take part follows the op:
+// iteration ops the iterating solves, FIN_TR1 and FIN_RZ the solves that need
+// a true residual, FIN_RZ with `begin` the solves starting a correction.
 __global__ __launch_bounds__(256) void k_b_finish(const BSolve *solves, BState *state, int op,
-                                                  int slot) {
+                                                  int slot, int prec, int begin) {
   __shared__ double lds[16 * kReduceSlots];
   const int s = blockIdx.x;
   const BState st = state[s];
   const BSolve &d = solves[s];
   int nblocks, nslots = 1;
   if (op == FIN_PQ || op == FIN_RZNEW) {
-    if (!st.active || slot >= st.n_chunk) return;
-    nblocks = d.tl.blocks;
+    if (!iterating(st, slot)) return;
+    nblocks = prec ? d.tl32.blocks : d.tl.blocks;
+  } else if (begin) {
+    if (!st.active || st.phase != kPhBegin) return;
+    nblocks = d.eb;
   } else {
     if (!st.need_tr) return;
     nblocks = op == FIN_TR1 ? d.tt.blocks : d.eb;
@@ -219,6 +270,109 @@ __global__ void k_b_check_end(const BSolve *solves, BState *state, int count, in
   state[s] = st;
 }
 
+// ---- mixed precision: the standalone mixed solve's outer loop per solve ----
+// Start of a correction: r32 = r, z32 = M32^-1 r32, e32 = 0, rz partials.
+__global__ __launch_bounds__(256) void k_bm_begin(const BSolve *solves, Grid grid,
+                                                  const BState *state) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!st.active || st.phase != kPhBegin) return;
+  const BSolve &d = solves[s];
+  mixed_begin_body(d.g32, d.r, d.code, d.r32, d.z32, d.e32, d.partials, grid.local[blockIdx.x],
+                   d.eb, lds);
+}
+
+// After the begin reduction: rz0, the inner target, the first inner chunk.
+__global__ void k_bm_after_begin(const BSolve *solves, BState *state, int count, int ci,
+                                 int max_inner, double rtol) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  if (!st.active || st.phase != kPhBegin) return;
+  Scalars *sc = solves[s].sc;
+  sc->status = 0;
+  sc->iters = 0;
+  const double rz0 = sc->rz;
+  if (!(rz0 > 0)) {
+    st.active = 0;
+  } else {
+    st.target = rz0 * rtol * rtol;
+    st.mode = kDirReset;
+    st.cur = 0;
+    st.phase = kPhIter;
+    st.n_chunk = ci < max_inner - st.inner ? ci : max_inner - st.inner;
+  }
+  state[s] = st;
+}
+
+// After an inner chunk: count, stop test of the inner solve, next chunk.
+__global__ void k_bm_after_inner(const BSolve *solves, BState *state, int count, int ci,
+                                 int max_inner) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  if (!st.active || st.phase != kPhIter) return;
+  Scalars *sc = solves[s].sc;
+  st.inner += sc->iters;
+  if (sc->status != 0 || sc->rz <= st.target || sc->iters == 0 || st.inner >= max_inner) {
+    st.phase = kPhAcc;
+  } else {
+    st.n_chunk = ci < max_inner - st.inner ? ci : max_inner - st.inner;
+    sc->iters = 0;
+  }
+  state[s] = st;
+}
+
+__global__ __launch_bounds__(256) void k_bm_acc(const BSolve *solves, Grid grid,
+                                                const BState *state) {
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!st.active || st.phase != kPhAcc) return;
+  const BSolve &d = solves[s];
+  mixed_accumulate_body(d.n, d.x, d.e32, grid.local[blockIdx.x], d.eb);
+}
+
+__global__ void k_bm_after_acc(const BSolve *solves, BState *state, int count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  if (!st.active || st.phase != kPhAcc) return;
+  Scalars *sc = solves[s].sc;
+  sc->status = 0;
+  sc->iters = 0;
+  st.outer += 1;
+  st.need_tr = 2;
+  st.phase = kPhCheck;
+  state[s] = st;
+}
+
+// After the FP64 true residual: the outer stall test and loop condition.
+__global__ void k_bm_check_end(const BSolve *solves, BState *state, int count, int max_inner,
+                               double tolerance, const int *replay) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  const int was_active = st.active;
+  if (st.need_tr) {
+    st.res = solves[s].sc->res;
+    if (st.need_tr == 3) {
+      st.best = st.res;
+    } else if (st.res < 0.99 * st.best) {
+      st.best = st.res;
+      st.stalled = 0;
+    } else if (++st.stalled >= 3) {
+      st.active = 0;
+    }
+    if (st.active && !(st.res > tolerance && st.inner < max_inner && st.outer < 100))
+      st.active = 0;
+    if (st.active) st.phase = kPhBegin;
+  }
+  if (was_active && !st.active) st.finished_at = *replay;
+  st.need_tr = 0;
+  state[s] = st;
+}
+
 // Active solves after a replay; advances the replay counter.
 __global__ __launch_bounds__(256) void k_b_count(const BState *state, int count, int *replay,
                                                  int *n_active) {
@@ -244,6 +398,7 @@ __global__ void k_b_init(BState *state, int count) {
   st.mode = kDirReset;
   st.need_tr = 3;
   st.finished_at = 0;
+  st.phase = kPhIter;
   state[s] = st;
 }
 
@@ -261,6 +416,18 @@ int cells_per_lane_f64(int nx) {
   const char *e = std::getenv("FFPM_X2");
   if (e && std::strcmp(e, "0") == 0) return 1;
   return nx % 2 == 0 ? 2 : 1;
+}
+
+// FP32 kernels: 16-byte vectors of four cells where nx allows.
+int cells_per_lane_f32(int nx) {
+  const char *e = std::getenv("FFPM_X2");
+  if (e && std::strcmp(e, "0") == 0) return 1;
+  return nx % 4 == 0 ? 4 : (nx % 2 == 0 ? 2 : 1);
+}
+
+double inner_rtol() {
+  const char *e = std::getenv("FFPM_MIXED_RTOL");
+  return e && *e ? std::atof(e) : 1e-4;
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -291,13 +458,15 @@ Tiling pcg_tiling(int nx, int ny, int nz, int cl, int rows, int zchunk) {
 
 struct Batch {
   int count = 0;
+  bool mixed = false;
   std::vector<BSolve> h;    // host copies of the descriptors
-  std::vector<int> cl;      // cells per lane of each solve
+  std::vector<int> cl;      // cells per lane of each solve (FP64 kernels)
+  std::vector<int> cl32;    // the same for the FP32 kernels
   std::vector<int> owner;   // operator owner (first solve of the frame)
   BSolve *d_solves = nullptr;
   BState *d_state = nullptr;
   int *d_counters = nullptr;  // [0] replay, [1] active count
-  Grid g_pcg[3], g_tr, g_ew;  // g_pcg[CL]
+  Grid g_pcg[3], g_pcg32[5], g_tr, g_ew;  // g_pcg[CL], g_pcg32[CL]
   int *d_isolated = nullptr;
 };
 
@@ -325,10 +494,13 @@ int build_grid(Arena &a, Grid &g, const std::vector<int> &blocks, const std::vec
 
 // Builds every solve's operator and state; arrays live in one arena kept in
 // the workspace (grow-only).
-int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zchunk) {
+int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zchunk,
+                bool mixed) {
   B.count = count;
+  B.mixed = mixed;
   B.h.assign(count, BSolve{});
   B.cl.assign(count, 1);
+  B.cl32.assign(count, 1);
   B.owner.assign(count, -1);
   for (int s = 0; s < count; ++s) {
     const ffpm_stairstep_problem &p = P[s];
@@ -351,26 +523,32 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
   // sizes
   size_t total = align256(sizeof(BSolve) * count) + align256(sizeof(BState) * count) +
                  align256(sizeof(int) * 4) + align256(sizeof(int));
-  std::vector<Tiling> tl(count), tt(count);
+  std::vector<Tiling> tl(count), tl32(count), tt(count);
   std::vector<int> eb(count);
   size_t grid_blocks = 0;
   for (int s = 0; s < count; ++s) {
     const ffpm_stairstep_problem &p = P[s];
     const size_t n = (size_t)p.nx * p.ny * p.nz;
     B.cl[s] = cells_per_lane_f64(p.nx);
+    B.cl32[s] = cells_per_lane_f32(p.nx);
     tl[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl[s], rows, zchunk);
+    tl32[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl32[s], rows, zchunk);
     tt[s] = make_tiling(p.nx, p.ny, p.nz, 0);
     eb[s] = elementwise_blocks(n);
-    const size_t nb = (size_t)std::max(std::max(tl[s].blocks, tt[s].blocks), eb[s]);
-    grid_blocks += (size_t)tl[s].blocks + tt[s].blocks + eb[s];
+    const size_t nb = (size_t)std::max(std::max(std::max(tl[s].blocks, tl32[s].blocks),
+                                                tt[s].blocks), eb[s]);
+    grid_blocks += (size_t)tl[s].blocks + tl32[s].blocks + tt[s].blocks + eb[s];
     total += 6 * align256(sizeof(double) * n) + align256(sizeof(double) * kReduceSlots * nb) +
              align256(sizeof(Scalars));
+    if (mixed) total += 5 * align256(sizeof(float) * n);
     if (B.owner[s] == s)
       total += 2 * align256(sizeof(double) * n) + align256(n) +
-               align256(sizeof(double) * 2 * (p.nx + p.ny + p.nz));
+               align256(sizeof(double) * 2 * (p.nx + p.ny + p.nz)) +
+               align256(sizeof(float) * 3 * (p.nx + p.ny + p.nz));
   }
-  // workgroup tables: two int arrays per grid (four grids), each 256-aligned
-  total += 2 * align256(sizeof(int) * grid_blocks) + 8 * 256;
+  // workgroup tables: two int arrays per grid (at most eight grids), each
+  // 256-aligned
+  total += 2 * align256(sizeof(int) * grid_blocks) + 16 * 256;
   int st = 0;
   Arena a;
   a.base = (char *)workspace().get(S_BATCH, total, &st);
@@ -388,6 +566,7 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     BSolve &d = B.h[s];
     d.n = n;
     d.tl = tl[s];
+    d.tl32 = tl32[s];
     d.tt = tt[s];
     d.eb = eb[s];
     const int o = B.owner[s];
@@ -406,6 +585,18 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
       FFPM_LAUNCHED("k_axis_coefficients");
       d.geom = g;
       d.g = GeomT<double>{nx, ny, nz, g.cw, g.ce, g.cs, g.cn, g.cb, g.ct, g.dx, g.dy, g.dz};
+      // FP32 copies for the mixed inner solve (the standalone mixed solve's
+      // conversions): coefficients, then dx, dy, dz
+      float *c32 = a.take<float>(sizeof(float) * 3 * tot);
+      const int ncoef = 2 * tot;
+      k_to_float<<<(ncoef + 255) / 256, 256, 0, stream()>>>(ncoef, coef, c32);
+      k_to_float<<<(nx + 255) / 256, 256, 0, stream()>>>(nx, p.dx, c32 + ncoef);
+      k_to_float<<<(ny + 255) / 256, 256, 0, stream()>>>(ny, p.dy, c32 + ncoef + nx);
+      k_to_float<<<(nz + 255) / 256, 256, 0, stream()>>>(nz, p.dz, c32 + ncoef + nx + ny);
+      FFPM_LAUNCHED("k_to_float");
+      d.g32 = GeomT<float>{nx, ny, nz, c32, c32 + nx, c32 + 2 * nx, c32 + 2 * nx + ny,
+                           c32 + 2 * nx + 2 * ny, c32 + 2 * nx + 2 * ny + nz, c32 + ncoef,
+                           c32 + ncoef + nx, c32 + ncoef + nx + ny};
       d.diag = a.take<double>(sizeof(double) * n);
       d.rdiag = a.take<double>(sizeof(double) * n);
       d.code = a.take<uint8_t>(n);
@@ -413,6 +604,7 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
       const BSolve &od = B.h[o];
       d.geom = od.geom;
       d.g = od.g;
+      d.g32 = od.g32;
       d.diag = od.diag;
       d.rdiag = od.rdiag;
       d.code = od.code;
@@ -424,7 +616,15 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     d.p[1] = a.take<double>(sizeof(double) * n);
     double *src = a.take<double>(sizeof(double) * n);
     d.src = src;
-    const size_t nb = (size_t)std::max(std::max(tl[s].blocks, tt[s].blocks), eb[s]);
+    if (mixed) {
+      d.e32 = a.take<float>(sizeof(float) * n);
+      d.r32 = a.take<float>(sizeof(float) * n);
+      d.z32 = a.take<float>(sizeof(float) * n);
+      d.p32[0] = a.take<float>(sizeof(float) * n);
+      d.p32[1] = a.take<float>(sizeof(float) * n);
+    }
+    const size_t nb = (size_t)std::max(std::max(std::max(tl[s].blocks, tl32[s].blocks),
+                                                tt[s].blocks), eb[s]);
     d.partials = a.take<double>(sizeof(double) * kReduceSlots * nb);
     d.sc = a.take<Scalars>(sizeof(Scalars));
     // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379); a frame's later
@@ -452,6 +652,15 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     }
     if ((st = build_grid(a, B.g_pcg[c], blocks, use, hs, hl))) return st;
   }
+  if (mixed) {
+    for (int c : {1, 2, 4}) {
+      for (int s = 0; s < count; ++s) {
+        blocks[s] = tl32[s].blocks;
+        use[s] = B.cl32[s] == c;
+      }
+      if ((st = build_grid(a, B.g_pcg32[c], blocks, use, hs, hl))) return st;
+    }
+  }
   for (int s = 0; s < count; ++s) {
     blocks[s] = tt[s].blocks;
     use[s] = 1;
@@ -475,48 +684,89 @@ int launch_check(const Batch &B, hipStream_t s, int ci, int max_sweeps, double t
   }
   k_b_tr1<<<B.g_tr.blocks, dim3(kTileX, B.h[0].tt.ty), 0, s>>>(B.d_solves, B.g_tr, B.d_state);
   FFPM_LAUNCHED("k_b_tr1");
-  k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_TR1, 0);
+  k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_TR1, 0, 0, 0);
   FFPM_LAUNCHED("k_b_finish");
   k_b_tr2<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
   FFPM_LAUNCHED("k_b_tr2");
-  k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZ, 0);
+  k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZ, 0, 0, 0);
   FFPM_LAUNCHED("k_b_finish");
-  k_b_check_end<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps, tol,
-                                  B.d_counters);
+  if (B.mixed)
+    k_bm_check_end<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, max_sweeps, tol,
+                                     B.d_counters);
+  else
+    k_b_check_end<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps, tol,
+                                    B.d_counters);
   FFPM_LAUNCHED("k_b_check_end");
   return OK;
 }
 
-// One chunk of ci iteration slots and the convergence check.
-int launch_chunk(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol, int rows) {
+// The iteration slots of one chunk, in value type T: per slot the direction
+// kernel, its finish, the update kernel and its finish.
+template <typename T>
+int launch_slots(const Batch &B, hipStream_t s, int ci, int rows) {
   const dim3 blk(kTileX, rows);
+  const Grid *grids = sizeof(T) == 8 ? B.g_pcg : B.g_pcg32;
+  const int prec = sizeof(T) == 8 ? 0 : 1;
+  const int widths[3] = {1, 2, 4};
   for (int m = 0; m < ci; ++m) {
-    for (int c = 1; c <= 2; ++c) {
-      if (!B.g_pcg[c].blocks) continue;
-      if (c == 2)
-        k_b_dir<2><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
-      else
-        k_b_dir<1><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
+    for (int c : widths) {
+      if ((sizeof(T) == 8 && c == 4) || !grids[c].blocks) continue;
+      if (c == 4) {
+        if constexpr (sizeof(T) == 4)
+          k_b_dir<T, 4><<<grids[c].blocks, blk, 0, s>>>(B.d_solves, grids[c], B.d_state, m);
+      } else if (c == 2) {
+        k_b_dir<T, 2><<<grids[c].blocks, blk, 0, s>>>(B.d_solves, grids[c], B.d_state, m);
+      } else {
+        k_b_dir<T, 1><<<grids[c].blocks, blk, 0, s>>>(B.d_solves, grids[c], B.d_state, m);
+      }
       FFPM_LAUNCHED("k_b_dir");
     }
-    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_PQ, m);
+    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_PQ, m, prec, 0);
     FFPM_LAUNCHED("k_b_finish");
-    for (int c = 1; c <= 2; ++c) {
-      if (!B.g_pcg[c].blocks) continue;
-      if (c == 2)
-        k_b_upd<2><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
-      else
-        k_b_upd<1><<<B.g_pcg[c].blocks, blk, 0, s>>>(B.d_solves, B.g_pcg[c], B.d_state, m);
+    for (int c : widths) {
+      if ((sizeof(T) == 8 && c == 4) || !grids[c].blocks) continue;
+      if (c == 4) {
+        if constexpr (sizeof(T) == 4)
+          k_b_upd<T, 4><<<grids[c].blocks, blk, 0, s>>>(B.d_solves, grids[c], B.d_state, m);
+      } else if (c == 2) {
+        k_b_upd<T, 2><<<grids[c].blocks, blk, 0, s>>>(B.d_solves, grids[c], B.d_state, m);
+      } else {
+        k_b_upd<T, 1><<<grids[c].blocks, blk, 0, s>>>(B.d_solves, grids[c], B.d_state, m);
+      }
       FFPM_LAUNCHED("k_b_upd");
     }
-    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZNEW, m);
+    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZNEW, m, prec, 0);
     FFPM_LAUNCHED("k_b_finish");
   }
-  k_b_check_begin<<<(B.count + 63) / 64, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci,
-                                                      max_sweeps);
-  FFPM_LAUNCHED("k_b_check_begin");
-  int st = launch_check(B, s, ci, max_sweeps, tol);
-  if (st) return st;
+  return OK;
+}
+
+// One chunk: FP64 PCG slots and the convergence check; in mixed precision the
+// start of corrections, FP32 inner slots, accumulation and the outer check.
+int launch_chunk(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol, int rows) {
+  const int cb = (B.count + 63) / 64;
+  int st;
+  if (!B.mixed) {
+    if ((st = launch_slots<double>(B, s, ci, rows))) return st;
+    k_b_check_begin<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps);
+    FFPM_LAUNCHED("k_b_check_begin");
+  } else {
+    k_bm_begin<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
+    FFPM_LAUNCHED("k_bm_begin");
+    k_b_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZ, 0, 0, 1);
+    FFPM_LAUNCHED("k_b_finish");
+    k_bm_after_begin<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps,
+                                       inner_rtol());
+    FFPM_LAUNCHED("k_bm_after_begin");
+    if ((st = launch_slots<float>(B, s, ci, rows))) return st;
+    k_bm_after_inner<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps);
+    FFPM_LAUNCHED("k_bm_after_inner");
+    k_bm_acc<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
+    FFPM_LAUNCHED("k_bm_acc");
+    k_bm_after_acc<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count);
+    FFPM_LAUNCHED("k_bm_after_acc");
+  }
+  if ((st = launch_check(B, s, ci, max_sweeps, tol))) return st;
   k_b_count<<<1, 256, 0, s>>>(B.d_state, B.count, B.d_counters, B.d_counters + 1);
   FFPM_LAUNCHED("k_b_count");
   return OK;
@@ -544,7 +794,7 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
   // All solves must share the PCG workgroup height (one launch per grid).
   const int ty = B.h[0].tl.ty;
   for (int s = 1; s < count; ++s)
-    if (B.h[s].tl.ty != ty || B.h[s].tt.ty != B.h[0].tt.ty) {
+    if (B.h[s].tl.ty != ty || B.h[s].tl32.ty != ty || B.h[s].tt.ty != B.h[0].tt.ty) {
       set_error(ERR_ARGUMENT, "batched solves need one workgroup height");
       return ERR_ARGUMENT;
     }
@@ -628,17 +878,21 @@ extern "C" {
 
 int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
                                    double tolerance, int max_sweeps, int check_interval,
-                                   int rows, int zchunk, int use_graph) {
+                                   int method, int rows, int zchunk, int use_graph) {
   if (!problems || count < 1) {
     set_error(ERR_ARGUMENT, "empty batch");
     return ERR_ARGUMENT;
+  }
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_PCG_MIXED) {
+    set_error(ERR_METHOD, "batched solves support pcg and pcg-mixed");
+    return ERR_METHOD;
   }
   if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8) {
     set_error(ERR_ARGUMENT, "rows must be 0, 1, 2, 4 or 8");
     return ERR_ARGUMENT;
   }
   Batch B;
-  int st = batch_setup(B, problems, count, rows, zchunk);
+  int st = batch_setup(B, problems, count, rows, zchunk, method == FFPM_METHOD_PCG_MIXED);
   if (st) return st;
   std::vector<BState> out;
   std::vector<double> replay_ms;
@@ -651,7 +905,7 @@ int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
     if ((st = batch_finalize(B, problems, s, &res[s]))) return st;
   FFPM_HIP(hipStreamSynchronize(stream()));
   for (int s = 0; s < count; ++s) {
-    problems[s].sweeps = out[s].sweeps;
+    problems[s].sweeps = B.mixed ? out[s].inner : out[s].sweeps;
     problems[s].residual = res[s];
     const int r = std::min(std::max(out[s].finished_at, 0), (int)replay_ms.size() - 1);
     problems[s].solve_ms = replay_ms[r];
@@ -660,11 +914,15 @@ int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
 }
 
 int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
-                                      int iterations, int *rows, int *zchunk,
+                                      int method, int iterations, int *rows, int *zchunk,
                                       double *ms_per_iteration) {
   if (!problems || count < 1 || !rows || !zchunk || !ms_per_iteration || iterations < 1) {
     set_error(ERR_ARGUMENT, "autotune needs a batch, an iteration count and outputs");
     return ERR_ARGUMENT;
+  }
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_PCG_MIXED) {
+    set_error(ERR_METHOD, "batched solves support pcg and pcg-mixed");
+    return ERR_METHOD;
   }
   static const int kRows[] = {4, 8};
   static const int kChunks[] = {8, 16, 32};
@@ -673,7 +931,7 @@ int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int coun
   for (int r : kRows) {
     for (int zc : kChunks) {
       Batch B;
-      int st = batch_setup(B, problems, count, r, zc);
+      int st = batch_setup(B, problems, count, r, zc, method == FFPM_METHOD_PCG_MIXED);
       if (st) return st;
       std::vector<BState> out;
       std::vector<double> replay_ms;

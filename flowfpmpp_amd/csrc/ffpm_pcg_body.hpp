@@ -339,4 +339,40 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
   store_partials(partials, 1, v, nblocks, block);
 }
 
+// Inner right-hand side of a defect correction: r32 = r64 (the projected FP64
+// true residual), z32 = M32^-1 r32, e32 = 0, and rz = z32.r32 summed in FP64;
+// grid-stride over the cells with nblocks workgroups.
+__device__ inline void mixed_begin_body(const GeomT<float> &g, const double *__restrict__ r64,
+                                        const uint8_t *__restrict__ code,
+                                        float *__restrict__ r32, float *__restrict__ z32,
+                                        float *__restrict__ e32, double *partials, int block,
+                                        int nblocks, double *lds) {
+  const int nx = g.nx, ny = g.ny;
+  const size_t n = (size_t)nx * ny * g.nz;
+  const size_t nt = (size_t)blockDim.x * blockDim.y;
+  const size_t tid = threadIdx.x + (size_t)blockDim.x * threadIdx.y;
+  double s[1] = {0.0};
+  for (size_t c = block * nt + tid; c < n; c += (size_t)nblocks * nt) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    const float d = diag_code<float>(code[c], g, i, j, k);
+    const float rv = (float)r64[c];
+    const float zv = rdiag_code<float>(d, g.dx[i] * g.dy[j], g.dz[k]) * rv;
+    r32[c] = rv;
+    z32[c] = zv;
+    e32[c] = 0.0f;
+    s[0] += (double)zv * (double)rv;
+  }
+  block_sum<1>(s, lds);
+  store_partials(partials, 1, s, nblocks, block);
+}
+
+// phi += e (FP64 accumulation of the FP32 correction).
+__device__ inline void mixed_accumulate_body(size_t n, double *__restrict__ phi,
+                                             const float *__restrict__ e32, int block,
+                                             int nblocks) {
+  const size_t nt = (size_t)blockDim.x * blockDim.y;
+  const size_t tid = threadIdx.x + (size_t)blockDim.x * threadIdx.y;
+  for (size_t c = block * nt + tid; c < n; c += (size_t)nblocks * nt) phi[c] = phi[c] + (double)e32[c];
+}
+
 }  // namespace ffpm

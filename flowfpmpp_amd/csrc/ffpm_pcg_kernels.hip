@@ -112,38 +112,20 @@ __global__ __launch_bounds__(256) void k_dir_t(size_t n, T *__restrict__ p,
   dir_update_body<T>(n, blockIdx.x, gridDim.x, p, z, sc);
 }
 
-// Inner right-hand side of a defect correction: r32 = r64 (the projected FP64
-// true residual), z32 = M32^-1 r32, e32 = 0, and rz = z32.r32 summed in FP64.
+// Inner right-hand side of a defect correction (body in ffpm_pcg_body.hpp).
 __global__ __launch_bounds__(256) void k_mixed_begin(GeomT<float> g, const double *__restrict__ r64,
                                                      const uint8_t *__restrict__ code,
                                                      float *__restrict__ r32,
                                                      float *__restrict__ z32,
                                                      float *__restrict__ e32, double *partials) {
   __shared__ double lds[16 * kReduceSlots];
-  const int nx = g.nx, ny = g.ny;
-  const size_t n = (size_t)nx * ny * g.nz;
-  double s[1] = {0.0};
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
-    const float d = diag_code<float>(code[c], g, i, j, k);
-    const float rv = (float)r64[c];
-    const float zv = rdiag_code<float>(d, g.dx[i] * g.dy[j], g.dz[k]) * rv;
-    r32[c] = rv;
-    z32[c] = zv;
-    e32[c] = 0.0f;
-    s[0] += (double)zv * (double)rv;
-  }
-  block_sum<1>(s, lds);
-  store_partials(partials, 1, s, gridDim.x);
+  mixed_begin_body(g, r64, code, r32, z32, e32, partials, blockIdx.x, gridDim.x, lds);
 }
 
 // phi += e (FP64 accumulation of the FP32 correction).
 __global__ __launch_bounds__(256) void k_mixed_accumulate(size_t n, double *__restrict__ phi,
                                                           const float *__restrict__ e32) {
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x)
-    phi[c] = phi[c] + (double)e32[c];
+  mixed_accumulate_body(n, phi, e32, blockIdx.x, gridDim.x);
 }
 
 // FP32 copies of the 1-D coefficients and widths.

@@ -484,14 +484,15 @@ bool classic_pcg() {
   return e && std::strcmp(e, "classic") == 0;
 }
 
-// jacobi_iterate (src/fpm_stairstep_mpi_3d.f90:200-225) with ping-pong
-// buffers: the current iterate is tracked instead of copied back.
 // The one-cell Jacobi kernel with the stored rdiag (FFPM_JACOBI=classic), for
 // comparison.
 bool classic_jacobi() {
   const char *e = std::getenv("FFPM_JACOBI");
   return e && std::strcmp(e, "classic") == 0;
 }
+
+// jacobi_iterate (src/fpm_stairstep_mpi_3d.f90:200-225) with ping-pong
+// buffers: the current iterate is tracked instead of copied back.
 
 int jacobi_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
                    double *residual, double **current) {

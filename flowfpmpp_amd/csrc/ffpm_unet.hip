@@ -203,18 +203,27 @@ __global__ __launch_bounds__(64 * kW) void k_conv3_wgrad(int nx, int ny, int nz,
   }
 }
 
-// db[c] += sum over cells of dy[cell][c] (FP32), one atomic per thread and
-// channel after a strided partial sum.
+// db[c] += sum over cells of dy[cell][c] (FP32): strided partial sums per
+// thread, a workgroup sum per channel in LDS, one atomic per workgroup and
+// channel.
 __global__ __launch_bounds__(256) void k_bias_grad(size_t cells, int c, const bf16 *__restrict__ dy,
                                                    float *__restrict__ db) {
+  __shared__ float part[256];
   const int ch = threadIdx.x % c;
   const int lanes = 256 / c;
   const int sub = threadIdx.x / c;
-  if (sub >= lanes) return;
   float s = 0.f;
-  for (size_t cell = (size_t)blockIdx.x * lanes + sub; cell < cells; cell += (size_t)gridDim.x * lanes)
-    s += bf2f(dy[cell * c + ch]);
-  atomicAdd(db + ch, s);
+  if (sub < lanes)
+    for (size_t cell = (size_t)blockIdx.x * lanes + sub; cell < cells;
+         cell += (size_t)gridDim.x * lanes)
+      s += bf2f(dy[cell * c + ch]);
+  part[threadIdx.x] = sub < lanes ? s : 0.f;
+  __syncthreads();
+  if (threadIdx.x < c) {
+    float t = 0.f;
+    for (int l = 0; l < lanes; ++l) t += part[l * c + threadIdx.x];
+    atomicAdd(db + threadIdx.x, t);
+  }
 }
 
 // Data-gradient weights: wt[t][ci][co] = w[26 - t][co][ci].
@@ -629,8 +638,8 @@ int conv_backward(UNet &N, int l, int nx, int ny, int nz, const bf16 *x, const b
   if ((st = wgrad(nx, ny, nz, cin, cout, gy, x, N.g + N.L.w_off[l]))) return st;
   const size_t cells = (size_t)nx * ny * nz;
   const int lanes = 256 / cout;
-  size_t nb = (cells + (size_t)lanes * 64 - 1) / ((size_t)lanes * 64);
-  if (nb > 4096) nb = 4096;
+  size_t nb = (cells + (size_t)lanes * 256 - 1) / ((size_t)lanes * 256);
+  if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
   k_bias_grad<<<(int)nb, 256, 0, stream()>>>(cells, cout, gy, N.g + N.L.b_off[l]);
   FFPM_LAUNCHED("k_bias_grad");

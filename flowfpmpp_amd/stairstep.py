@@ -234,25 +234,30 @@ def _structs(problems):
 
 
 def solve_stairstep_batch(problems, tolerance, max_sweeps, check_interval, rows=0, zchunk=0,
-                          use_graph=True):
-    """FP64 PCG on many problems in one batch (per-problem convergence masks,
-    HIP-graph chunks). Each problem ends as its own
-    solve_stairstep_poisson_3d(method='pcg') would; sweeps, residual and the
-    time to its convergence (ms from the batch start) are stored on it."""
+                          use_graph=True, method="pcg"):
+    """PCG ('pcg', FP64) or mixed-precision PCG ('pcg-mixed') on many problems
+    in one batch (per-problem convergence masks, HIP-graph chunks). Each
+    problem ends as its own solve_stairstep_poisson_3d with the same method
+    would; sweeps, residual and the time to its convergence (ms from the batch
+    start) are stored on it."""
+    if method not in ("pcg", "pcg-mixed"):
+        raise ValueError("batched solves support 'pcg' and 'pcg-mixed'")
     arr = _structs(problems)
     check(lib().ffpm_solve_stairstep_batch_dev(arr, len(problems), C.c_double(tolerance),
-                                                int(max_sweeps), int(check_interval), int(rows),
-                                                int(zchunk), 1 if use_graph else 0))
+                                                int(max_sweeps), int(check_interval),
+                                                METHODS[method], int(rows), int(zchunk),
+                                                1 if use_graph else 0))
     for k, p in enumerate(problems):
         p.sweeps, p.residual, p.solve_ms = arr[k].sweeps, arr[k].residual, arr[k].solve_ms
     return [(p.sweeps, p.residual) for p in problems]
 
 
-def autotune_stairstep_batch(problems, iterations=20):
+def autotune_stairstep_batch(problems, iterations=20, method="pcg"):
     """Times the candidate launch shapes on the batch; returns (rows, zchunk,
     ms per batched iteration) of the fastest."""
     arr = _structs(problems)
     rows, zchunk, ms = C.c_int(), C.c_int(), C.c_double()
-    check(lib().ffpm_autotune_stairstep_batch_dev(arr, len(problems), int(iterations),
-                                                   C.byref(rows), C.byref(zchunk), C.byref(ms)))
+    check(lib().ffpm_autotune_stairstep_batch_dev(arr, len(problems), METHODS[method],
+                                                   int(iterations), C.byref(rows),
+                                                   C.byref(zchunk), C.byref(ms)))
     return rows.value, zchunk.value, ms.value

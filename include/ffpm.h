@@ -131,9 +131,11 @@ int ffpm_last_mixed_solve(int *outer, int *inner);
  * convergence masks, one HIP graph per check interval) and each ends exactly
  * as its own FP64 PCG solve would: same iterations, same phi. Problems with
  * the same solid and axis pointers (e.g. the three force directions of a
- * frame) share one operator. rows/zchunk fix the PCG workgroup height (0, 4
- * or 8) and z chunk (0 = the standalone defaults); solve_ms is the time from
- * the batch start until the problem converged. */
+ * frame) share one operator. method is FFPM_METHOD_PCG or _PCG_MIXED (each
+ * problem then ends as its standalone mixed solve, sweeps = FP32 inner
+ * iterations). rows/zchunk fix the PCG workgroup height (0, 4 or 8) and z
+ * chunk (0 = the standalone defaults); solve_ms is the time from the batch
+ * start until the problem converged. */
 typedef struct ffpm_stairstep_problem {
   int nx, ny, nz;
   double *phi;             /* (0:nx+1,0:ny+1,nz), in/out */
@@ -148,7 +150,14 @@ typedef struct ffpm_stairstep_problem {
 } ffpm_stairstep_problem;
 int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
                                    double tolerance, int max_sweeps, int check_interval,
-                                   int rows, int zchunk, int use_graph);
+                                   int method, int rows, int zchunk, int use_graph);
+/* Times `iterations` batched iterations (FP64 PCG, or FP32 inner iterations
+ * for PCG_MIXED) for each candidate launch shape (rows 4, 8 x z chunks 8, 16,
+ * 32) and returns the fastest; phi is not written. */
+int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count, int method,
+                                      int iterations, int *rows, int *zchunk,
+                                      double *ms_per_iteration);
+
 /* Learned preconditioner (no counterpart in the reference): a 3-D U-Net
  * (3 levels, 16/32/64 channels, 3x3x3 convolutions, ~0.36M parameters), BF16
  * weights on MFMA with FP32 accumulation, trained by its own forward, backward
@@ -196,13 +205,6 @@ int ffpm_solve_stairstep_fcg_dev(
     const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
     double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, const float *features,
     int *sweeps, double *residual, int *fallback_at);
-
-/* Times `iterations` batched iterations for each candidate launch shape
- * (rows 4, 8 x z chunks 8, 16, 32) and returns the fastest; phi is not
- * written. */
-int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
-                                      int iterations, int *rows, int *zchunk,
-                                      double *ms_per_iteration);
 
 /* stairstep_residual_3d, src/fpm_stairstep_mpi_3d.f90:403-419 (value out) */
 int ffpm_stairstep_residual_3d(

@@ -1,7 +1,8 @@
-"""GPU tests of the batched (multi-right-hand-side, uneven-frame) FP64 PCG.
+"""GPU tests of the batched (multi-right-hand-side, uneven-frame) FP64 and
+mixed-precision PCG.
 
 Bar: every problem of a batch computes bit-for-bit what its standalone
-solve_stairstep_poisson_3d(method='pcg') computes -- the batched kernels run
+solve_stairstep_poisson_3d with the same method computes -- the batched kernels run
 the standalone kernels' bodies with each problem's own launch shape, so the
 iterates, the reductions and the convergence decisions are identical: same
 sweeps, same residual, identical phi.
@@ -44,12 +45,12 @@ class Frame:
                                       *self.axes))
         return out
 
-    def standalone(self, tol, cap, check):
+    def standalone(self, tol, cap, check, method="pcg"):
         res = []
         for d_rhs, bg, d_faces in self.dirs:
             phi = T(self.p.phi0())
             sw, r = S.solve_stairstep_poisson_3d(phi, d_rhs, self.solid, bg, *d_faces,
-                                                 *self.axes, tol, cap, check, method="pcg")
+                                                 *self.axes, tol, cap, check, method=method)
             res.append((phi.download(), sw, r))
         return res
 
@@ -131,6 +132,21 @@ def test_autotuned_shape_converges():
         assert np.abs(q.phi.download() - phi).max() <= 1e-10 * np.abs(phi).max()
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_mixed_batch_equals_standalone_mixed(use_graph):
+    """Mixed precision in a batch: each problem runs the standalone mixed
+    solve's defect-correction loop on the device and ends bit-identical to it
+    (same FP32 inner iteration count, same residual, identical phi). The
+    uneven frames cover nx a multiple of 4, even only, and odd."""
+    print(f"case: uneven mixed batch, graph {use_graph}", flush=True)
+    frames = _uneven_frames()
+    ref = [r for fr in frames for r in fr.standalone(1e-10, 100000, 25, method="pcg-mixed")]
+    probs = [q for fr in frames for q in fr.problems()]
+    S.solve_stairstep_batch(probs, 1e-10, 100000, 25, use_graph=use_graph, method="pcg-mixed")
+    _assert_same(probs, ref)
+    assert all(q.residual <= 1e-10 for q in probs)
+
+
 def test_batch_errors():
     print("case: batch argument errors", flush=True)
     with pytest.raises(FFPMError):
@@ -138,3 +154,5 @@ def test_batch_errors():
     f = Frame(32, 24, 16, [W.Sphere((0.0, 0.0, 0.0), 0.9)], seed=3, dirs=(0,))
     with pytest.raises(FFPMError):
         S.solve_stairstep_batch(f.problems(), 1e-10, 100, 10, rows=3)
+    with pytest.raises(ValueError):
+        S.solve_stairstep_batch(f.problems(), 1e-10, 100, 10, method="jacobi")
