@@ -20,6 +20,7 @@ SOURCES = [
     "ffpm_laplace.hip",
     "ffpm_batch.hip",
     "ffpm_unet.hip",
+    "ffpm_partition.hip",
 ]
 
 # -ffp-contract=off keeps every expression's rounding identical to the

@@ -206,6 +206,26 @@ int ffpm_solve_stairstep_fcg_dev(
     double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, const float *features,
     int *sweeps, double *residual, int *fallback_at);
 
+/* fpm_partition_terms_2d, src/fpm_partition_2d.f90:35-154: force partition of
+ * a 2-D stair-step body with rigid in-plane motion, per unit span and density.
+ * phi, u, v (nx,ny); solid (nx,ny) int32, nonzero = solid (the reference's
+ * logical array); xf(0:nx), yf(0:ny) cell faces; g(2) the body-face gradient
+ * of phi's direction; kin(6) body motion; ring(2nx+2ny,2,2) outer-boundary
+ * velocities at t-dt and t+dt; slip_body as the optional argument (0 =
+ * absent/.false.). Outputs terms(4) = [KINE, VIF, VIS, BOUND] (host),
+ * vif_density and omega (nx,ny). The _dev variant takes device arrays (g,
+ * kin and terms stay host arrays). */
+int ffpm_partition_terms_2d(const double *phi, const double *g, const double *u,
+                            const double *v, const int32_t *solid, const double *xf,
+                            const double *yf, const double *kin, const double *ring, double dt,
+                            double nu, int nx, int ny, int slip_body, double *terms,
+                            double *vif_density, double *omega);
+int ffpm_partition_terms_2d_dev(const double *phi, const double *g, const double *u,
+                                const double *v, const int32_t *solid, const double *xf,
+                                const double *yf, const double *kin, const double *ring,
+                                double dt, double nu, int nx, int ny, int slip_body,
+                                double *terms, double *vif_density, double *omega);
+
 /* stairstep_residual_3d, src/fpm_stairstep_mpi_3d.f90:403-419 (value out) */
 int ffpm_stairstep_residual_3d(
     const double *phi, const double *rhs, const int32_t *solid, const double *body_grad,

@@ -891,3 +891,159 @@ int orc_compute_fpm_terms_nonuniform_3d(const double *phi, const double *u, cons
   totals[1] = at;
   return 0;
 }
+
+
+/* ------------------------------------------------------------------------
+ * fpm_partition_terms_2d, src/fpm_partition_2d.f90:35-154: the force
+ * partition of a 2-D stair-step body with rigid in-plane motion (F_KINE,
+ * F_VIF, F_VIS, F_BOUND), two serial passes over the cells (j outer, i
+ * inner) in the reference's order, every expression left to right.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int ii, jj, axis;
+  double sgn, hw, len, d, fx, fy;
+} P2Face;
+
+/* face(i,j,f), src/fpm_partition_2d.f90:122-140 (1-based i, j; f = 1 east,
+ * 2 west, 3 north, 4 south) */
+static P2Face p2_face(int i, int j, int f, int nx, int ny, const double *xc, const double *yc,
+                      const double *dx, const double *dy) {
+  P2Face o;
+  switch (f) {
+    case 1: o.ii = i + 1; o.jj = j; o.axis = 1; o.sgn = 1; break;
+    case 2: o.ii = i - 1; o.jj = j; o.axis = 1; o.sgn = -1; break;
+    case 3: o.ii = i; o.jj = j + 1; o.axis = 2; o.sgn = 1; break;
+    default: o.ii = i; o.jj = j - 1; o.axis = 2; o.sgn = -1; break;
+  }
+  if (o.axis == 1) {
+    o.hw = 0.5 * dx[i - 1];
+    o.len = dy[j - 1];
+    o.fx = xc[i - 1] + o.sgn * o.hw;
+    o.fy = yc[j - 1];
+    o.d = o.hw;
+    if (o.ii >= 1 && o.ii <= nx) o.d = fabs(xc[o.ii - 1] - xc[i - 1]);
+  } else {
+    o.hw = 0.5 * dy[j - 1];
+    o.len = dx[i - 1];
+    o.fx = xc[i - 1];
+    o.fy = yc[j - 1] + o.sgn * o.hw;
+    o.d = o.hw;
+    if (o.jj >= 1 && o.jj <= ny) o.d = fabs(yc[o.jj - 1] - yc[j - 1]);
+  }
+  return o;
+}
+
+/* ring_index, src/fpm_partition_2d.f90:142-151 (1-based) */
+static int p2_ring_index(int i, int j, int f, int nx, int ny) {
+  switch (f) {
+    case 2: return j;
+    case 1: return ny + j;
+    case 4: return 2 * ny + i;
+    default: return 2 * ny + nx + i;
+  }
+}
+
+int orc_partition_terms_2d(const double *phi, const double g[2], const double *u,
+                           const double *v, const int32_t *solid, const double *xf,
+                           const double *yf, const double kin[6], const double *ring, double dt,
+                           double nu, int nx, int ny, int slip_body, double terms[4],
+                           double *vif_density, double *omega) {
+  if (nx < 1 || ny < 1) return -1;
+  const size_t n = (size_t)nx * ny;
+  const size_t L = 2 * (size_t)nx + 2 * (size_t)ny;
+  double *xc = malloc(sizeof(double) * nx), *yc = malloc(sizeof(double) * ny);
+  double *dx = malloc(sizeof(double) * nx), *dy = malloc(sizeof(double) * ny);
+  double *nxf = calloc(n, sizeof(double)), *nyf = calloc(n, sizeof(double));
+  double *gpx = calloc(n, sizeof(double)), *gpy = calloc(n, sizeof(double));
+  for (int i = 0; i < nx; ++i) {
+    dx[i] = xf[i + 1] - xf[i];
+    xc[i] = 0.5 * (xf[i] + xf[i + 1]);
+  }
+  for (int j = 0; j < ny; ++j) {
+    dy[j] = yf[j + 1] - yf[j];
+    yc[j] = 0.5 * (yf[j] + yf[j + 1]);
+  }
+  for (int t = 0; t < 4; ++t) terms[t] = 0;
+  for (size_t c = 0; c < n; ++c) vif_density[c] = 0, omega[c] = 0;
+#define AT(a, i, j) a[(size_t)((i) - 1) + (size_t)nx * ((j) - 1)]
+  /* pass 1 (src/fpm_partition_2d.f90:60-84) */
+  for (int j = 1; j <= ny; ++j) {
+    for (int i = 1; i <= nx; ++i) {
+      if (AT(solid, i, j)) continue;
+      double fu[4], fv[4], fp[4];
+      for (int f = 1; f <= 4; ++f) {
+        const P2Face o = p2_face(i, j, f, nx, ny, xc, yc, dx, dy);
+        if (o.ii < 1 || o.ii > nx || o.jj < 1 || o.jj > ny) {
+          fu[f - 1] = AT(u, i, j);
+          fv[f - 1] = AT(v, i, j);
+          fp[f - 1] = 0;
+        } else if (AT(solid, o.ii, o.jj)) {
+          if (slip_body) {
+            fu[f - 1] = AT(u, i, j);
+            fv[f - 1] = AT(v, i, j);
+          } else {
+            fu[f - 1] = kin[0] - kin[2] * o.fy;
+            fv[f - 1] = kin[1] + kin[2] * o.fx;
+          }
+          fp[f - 1] = g[o.axis - 1];
+        } else {
+          fu[f - 1] = AT(u, i, j) + (AT(u, o.ii, o.jj) - AT(u, i, j)) * o.hw / o.d;
+          fv[f - 1] = AT(v, i, j) + (AT(v, o.ii, o.jj) - AT(v, i, j)) * o.hw / o.d;
+          fp[f - 1] = o.sgn * (AT(phi, o.ii, o.jj) - AT(phi, i, j)) / o.d;
+        }
+      }
+      AT(nxf, i, j) = AT(u, i, j) * (fu[0] - fu[1]) / dx[i - 1] +
+                      AT(v, i, j) * (fu[2] - fu[3]) / dy[j - 1];
+      AT(nyf, i, j) = AT(u, i, j) * (fv[0] - fv[1]) / dx[i - 1] +
+                      AT(v, i, j) * (fv[2] - fv[3]) / dy[j - 1];
+      AT(omega, i, j) = (fv[0] - fv[1]) / dx[i - 1] - (fu[2] - fu[3]) / dy[j - 1];
+      AT(gpx, i, j) = 0.5 * (fp[0] + fp[1]);
+      AT(gpy, i, j) = 0.5 * (fp[2] + fp[3]);
+    }
+  }
+  /* pass 2 (src/fpm_partition_2d.f90:85-117) */
+  for (int j = 1; j <= ny; ++j) {
+    for (int i = 1; i <= nx; ++i) {
+      if (AT(solid, i, j)) continue;
+      double div = 0, lapu = 0, lapv = 0;
+      for (int f = 1; f <= 4; ++f) {
+        const P2Face o = p2_face(i, j, f, nx, ny, xc, yc, dx, dy);
+        if (o.ii < 1 || o.ii > nx || o.jj < 1 || o.jj > ny) {
+          const double nn = o.axis == 1 ? AT(nxf, i, j) : AT(nyf, i, j);
+          div = div + o.sgn * nn * o.len;
+          const int m = p2_ring_index(i, j, f, nx, ny);
+          const double dudt = (ring[(m - 1) + L * 2] - ring[(m - 1)]) / (2 * dt);
+          const double dvdt = (ring[(m - 1) + L * 3] - ring[(m - 1) + L]) / (2 * dt);
+          const double sel = o.axis == 1 ? dudt + AT(nxf, i, j) : dvdt + AT(nyf, i, j);
+          terms[3] = terms[3] - o.sgn * sel * AT(phi, i, j) * o.len;
+        } else if (AT(solid, o.ii, o.jj)) {
+          const double ubx = kin[0] - kin[2] * o.fy, uby = kin[1] + kin[2] * o.fx;
+          const double ax = kin[3] - kin[5] * o.fy - kin[2] * kin[2] * o.fx;
+          const double ay = kin[4] + kin[5] * o.fx - kin[2] * kin[2] * o.fy;
+          const double phif = AT(phi, i, j) + o.sgn * o.hw * g[o.axis - 1];
+          terms[0] = terms[0] - o.sgn * (o.axis == 1 ? ax : ay) * phif * o.len;
+          if (slip_body) {
+            div = div + o.sgn * (o.axis == 1 ? AT(nxf, i, j) : AT(nyf, i, j)) * o.len;
+          } else {
+            div = div + o.sgn * (o.axis == 1 ? -(kin[2] * uby) : kin[2] * ubx) * o.len;
+            lapu = lapu + (ubx - AT(u, i, j)) / o.hw * o.len;
+            lapv = lapv + (uby - AT(v, i, j)) / o.hw * o.len;
+          }
+        } else {
+          const double sel =
+              o.axis == 1 ? AT(nxf, i, j) + (AT(nxf, o.ii, o.jj) - AT(nxf, i, j)) * o.hw / o.d
+                          : AT(nyf, i, j) + (AT(nyf, o.ii, o.jj) - AT(nyf, i, j)) * o.hw / o.d;
+          div = div + o.sgn * sel * o.len;
+          lapu = lapu + (AT(u, o.ii, o.jj) - AT(u, i, j)) / o.d * o.len;
+          lapv = lapv + (AT(v, o.ii, o.jj) - AT(v, i, j)) / o.d * o.len;
+        }
+      }
+      AT(vif_density, i, j) = AT(phi, i, j) * div / (dx[i - 1] * dy[j - 1]);
+      terms[1] = terms[1] + AT(phi, i, j) * div;
+      terms[2] = terms[2] + nu * (lapu * AT(gpx, i, j) + lapv * AT(gpy, i, j));
+    }
+  }
+#undef AT
+  free(xc); free(yc); free(dx); free(dy); free(nxf); free(nyf); free(gpx); free(gpy);
+  return 0;
+}

@@ -127,6 +127,14 @@ int orc_compute_fpm_terms_nonuniform_3d(const double *phi, const double *u,
     const double *zhi, const double bounds[6], double *gx, double *gy, double *gz,
     double *vortex, double *advection, double totals[2], int nx, int ny, int nz);
 
+/* src/fpm_partition_2d.f90:35-154; phi, u, v, solid (0/1), vif_density, omega
+ * are (nx,ny); xf (0:nx), yf (0:ny); kin(6); ring (2nx+2ny, 2, 2);
+ * terms = [KINE, VIF, VIS, BOUND] */
+int orc_partition_terms_2d(const double *phi, const double g[2], const double *u,
+    const double *v, const int32_t *solid, const double *xf, const double *yf,
+    const double kin[6], const double *ring, double dt, double nu, int nx, int ny,
+    int slip_body, double terms[4], double *vif_density, double *omega);
+
 #ifdef __cplusplus
 }
 #endif

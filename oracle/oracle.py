@@ -20,6 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libffpm_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_laplace.so")
+REF_PART_PATH = os.path.join(HERE, "_ref", "libref_partition.so")
 
 PROC_NULL = -1
 METHODS = {"pcg": 0, "jacobi": 1}
@@ -32,6 +33,7 @@ ERRORS = {
 
 _lib = None
 _ref = None
+_ref_part = None
 
 
 class OracleError(RuntimeError):
@@ -334,3 +336,46 @@ def ref_compute_fpm_terms_nonuniform_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, fl
         _p(ylo), _p(yhi), _p(zlo), _p(zhi), _p(bd), *[_p(o) for o in out], _p(totals), _ri(nx),
         _ri(ny), _ri(nz))
     return (*out, totals)
+
+
+# --- src/fpm_partition_2d.f90 ------------------------------------------------
+
+def partition_terms_2d(phi, g, u, v, solid, xf, yf, kin, ring, dt, nu, slip_body=False):
+    """phi, u, v (nx, ny) float64; solid (nx, ny) int32 0/1 or bool; xf (nx+1),
+    yf (ny+1); kin (6); ring (2nx+2ny, 2, 2). Returns (terms[4], vif_density,
+    omega)."""
+    nx, ny = phi.shape
+    sol = np.asfortranarray(solid, dtype=np.int32)
+    gg = np.asfortranarray(g, dtype=np.float64)
+    kk = np.asfortranarray(kin, dtype=np.float64)
+    terms = np.zeros(4)
+    vif = np.zeros((nx, ny), order="F")
+    om = np.zeros((nx, ny), order="F")
+    _check(lib().orc_partition_terms_2d(
+        _p(phi), _p(gg), _p(u), _p(v), _p(sol, np.int32), _p(xf), _p(yf), _p(kk), _p(ring),
+        _d(dt), _d(nu), nx, ny, 1 if slip_body else 0, _p(terms), _p(vif), _p(om)))
+    return terms, vif, om
+
+
+def ref_partition_available() -> bool:
+    return os.path.exists(REF_PART_PATH)
+
+
+def ref_partition_terms_2d(phi, g, u, v, solid, xf, yf, kin, ring, dt, nu, slip_body=None):
+    """The reference's own fpm_partition_terms_2d (oracle/_ref build)."""
+    global _ref_part
+    if _ref_part is None:
+        _ref_part = C.CDLL(REF_PART_PATH)
+    nx, ny = phi.shape
+    sol = np.asfortranarray(np.asarray(solid) != 0, dtype=np.int32)  # logical(4): 1 / 0
+    gg = np.asfortranarray(g, dtype=np.float64)
+    kk = np.asfortranarray(kin, dtype=np.float64)
+    terms = np.zeros(4)
+    vif = np.zeros((nx, ny), order="F")
+    om = np.zeros((nx, ny), order="F")
+    slip = None if slip_body is None else C.byref(C.c_int32(1 if slip_body else 0))
+    _ref_part._QMfpm_partition_2dPfpm_partition_terms_2d(
+        _p(phi), _p(gg), _p(u), _p(v), _p(sol, np.int32), _p(xf), _p(yf), _p(kk), _p(ring),
+        C.byref(C.c_double(dt)), C.byref(C.c_double(nu)), C.byref(C.c_int(nx)),
+        C.byref(C.c_int(ny)), _p(terms), _p(vif), _p(om), slip)
+    return terms, vif, om
