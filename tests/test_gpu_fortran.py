@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 BUILD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "_build")
 
 
-@pytest.mark.parametrize("program", ["test_stairstep_exact_3d", "test_fpm_3d"])
+@pytest.mark.parametrize("program", ["test_stairstep_exact_3d", "test_fpm_3d",
+                                     "test_fpm_partition_2d"])
 def test_fortran_dropin_program(program):
     exe = os.path.join(BUILD, program)
     if not os.path.exists(exe):
