@@ -48,6 +48,13 @@ BYTES_PER_CELL = {
     "true_residual_2": 32,  # r (read, write), rdiag, diag
     "residual_max": 24,  # phi, src, diag
     "terms": 124,        # phi, solid(4), 9 fields; write 5 outputs
+    # flexible CG with the FP32 multigrid cycle (level-0 kernels; the coarse
+    # levels add 1/8, 1/64, ... of their own small traffic)
+    "fcg_xr": 41,            # read p, code; read/write phi, r
+    "mg_fine_pre": 30,       # x = w D^-1 r (r, code; write x) + one sweep (x, r, code; write)
+    "mg_fine_restrict": 13,  # x, r, code per fine cell (coarse writes 1/8 of 4 B)
+    "mg_fine_post": 26,      # prolongation (x read/write, code) + one sweep
+    "mg_fine_last": 29,      # x, r, code, p_old; write z (FP64)
 }
 
 
@@ -284,7 +291,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--method", default="pcg", choices=["pcg", "jacobi", "pcg-mixed"])
+    ap.add_argument("--method", default="pcg", choices=["pcg", "jacobi", "pcg-mixed", "fcg-mg"])
     ap.add_argument("--grid", type=int, nargs=3, default=[512, 384, 256])
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--check", type=int, default=100)

@@ -21,6 +21,7 @@ SOURCES = [
     "ffpm_batch.hip",
     "ffpm_unet.hip",
     "ffpm_partition.hip",
+    "ffpm_mg.hip",
 ]
 
 # -ffp-contract=off keeps every expression's rounding identical to the

@@ -13,7 +13,7 @@ namespace ffpm {
 // scalar logic (src/fpm_stairstep_mpi_3d.f90:257-278). One workgroup.
 __device__ inline void finish_body(int op, int nslots, const double *partials, int nblocks,
                                    Scalars *sc, double *lds) {
-  if ((op == FIN_PQ || op == FIN_RZNEW) && sc->status != 0) return;
+  if ((op == FIN_PQ || op == FIN_RZNEW || op == FIN_FCG1) && sc->status != 0) return;
   const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
   if (op == FIN_MAX) {
     const double m = grid_finish_max(partials, 0, nblocks, lds);
@@ -26,6 +26,16 @@ __device__ inline void finish_body(int op, int nslots, const double *partials, i
       sc->res = m;
       sc->red1 = v[0];
       sc->red2 = v[1];
+    }
+  } else if (op == FIN_FCG1) {
+    // flexible CG with one stored direction: p = z + beta p_old is made
+    // A-orthogonal to p_old (beta = -z.A p_old / p_old.A p_old), and
+    // alpha = z.r / p.A p follows in FIN_PQ
+    double v[2];
+    grid_finish_sum<2>(partials, nblocks, v, lds);
+    if (lead) {
+      sc->rz = v[0];
+      sc->beta = sc->pq > 0 ? -v[1] / sc->pq : 0.0;
     }
   } else if (op == FIN_SUMS) {
     double v[kReduceSlots];

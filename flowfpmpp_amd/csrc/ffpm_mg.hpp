@@ -1,0 +1,62 @@
+// Geometric multigrid V-cycle used as the preconditioner of flexible CG on
+// the stair-step operator A = V (diag - offdiag) (src/fpm_stairstep_mpi_3d.f90:
+// 336-379, 247-256). There is no counterpart in the reference.
+//
+// Hierarchy: level 0 is the solve's own grid, whose operator is rebuilt from
+// the per-cell neighbour code and FP32 copies of the 1-D coefficients; each
+// coarser level aggregates 2x2x2 cells (ceil for odd extents) and stores, per
+// cell, the conductance to its +x, +y and +z neighbour: half the sum of the
+// finer conductances across the shared face (the Galerkin sum of piecewise-
+// constant aggregation, scaled by 1/2 so that on a smooth grid it equals the
+// rediscretised operator). Rows sum to zero on every level, so the constant
+// null space of the Neumann problem is kept. A cell carries an unknown when
+// its diagonal is positive.
+//
+// Cycle (FP32 arithmetic, FP64 input r and output z): two damped-Jacobi
+// sweeps from zero, restriction of the residual by summation over the
+// children, recursion, trilinear cell-centred prolongation with weights
+// renormalised over the coarse cells that carry unknowns, two damped-Jacobi
+// sweeps. The coarsest level (at most 1024 cells) runs a fixed number of
+// sweeps in one workgroup. Every step is a fixed linear map, so the flexible
+// CG around it sees the same preconditioner at every iteration.
+#pragma once
+
+#include "ffpm_pcg.hpp"
+
+namespace ffpm {
+
+constexpr int kMgMaxLevels = 20;
+constexpr int kMgCoarsestCells = 1024;
+
+struct MgLevel {
+  int nx = 0, ny = 0, nz = 0;
+  size_t n = 0;
+  float *gE = nullptr, *gN = nullptr, *gT = nullptr;  // coarse levels only
+  float *m = nullptr;                                 // coarse levels: 1 where diag > 0
+  float *x = nullptr, *y = nullptr, *b = nullptr;     // b: coarse levels only
+};
+
+struct Multigrid {
+  int levels = 0;
+  MgLevel lv[kMgMaxLevels];
+  GeomT<float> g32{};
+  GeomT<double> g64{};
+  const uint8_t *code = nullptr;
+  float omega = 0.8f;
+  float scale = 0.5f;
+  int coarse_sweeps = 30;
+  int kz = 1;               // z planes per workgroup stride of the last kernel
+  int last_blocks = 0;      // workgroups (partials per slot) of the last kernel
+  double *partials = nullptr;
+};
+
+// Builds the hierarchy for the frame with FP64 geometry g64, its FP32 copy
+// g32 and the neighbour code (workspace slot S_MG).
+int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
+             const uint8_t *code);
+
+// z = B r (one cycle). Leaves in mg.partials two slots of mg.last_blocks
+// partial sums: z.r and z.(A p_old) with A in FP64 (zero when p_old is null).
+int mg_vcycle(const Multigrid &mg, const double *r, const double *p_old, double *z);
+
+}  // namespace ffpm

@@ -50,6 +50,8 @@ enum Slot : int {
   S_FCG_SRC = 91, S_FCG_U, S_FCG_AU, S_FCG_AN, S_FCG_N, S_FCG_RING,
   // 2-D force partition
   S_P2_AXES = 100, S_P2_WORK, S_P2_PART, S_P2_SC, S_P2_HOST, S_P2_SOLID,
+  // multigrid hierarchy of the flexible CG (one arena)
+  S_MG = 110,
 };
 
 // ---- kernel timing with HIP events on the library stream --------------------

@@ -9,7 +9,9 @@
 #include <string>
 
 #include "../../include/ffpm.h"
+#include "ffpm_mg.hpp"
 #include "ffpm_pcg.hpp"
+#include "ffpm_pcg_body.hpp"
 #include "ffpm_runtime.hpp"
 #include "ffpm_unet.hpp"
 
@@ -390,6 +392,26 @@ int pcg_iterate_fused(Frame &f, double tolerance, int max_sweeps, int check_inte
 // until the preconditioned residual norm has dropped by inner_rtol, and adds
 // e to phi in FP64. The outer loop stops on the reference's FP64 max-norm
 // residual test, so the result is judged exactly like the FP64 solve.
+// FP32 copies of the frame's 1-D coefficients and cell widths (slot
+// S_COEF32), for the FP32 inner solves and the multigrid cycle.
+int make_geom32(const Frame &f, GeomT<float> *g) {
+  int st = 0;
+  const int nx = f.nx, ny = f.ny, nz = f.nz;
+  float *c32 = (float *)workspace().get(S_COEF32, sizeof(float) * 3 * (nx + ny + nz), &st);
+  if (!c32) return st;
+  const int ncoef = 2 * (nx + ny + nz);
+  k_to_float<<<(ncoef + 255) / 256, 256, 0, stream()>>>(ncoef, f.g.cw, c32);
+  k_to_float<<<(nx + 255) / 256, 256, 0, stream()>>>(nx, f.g.dx, c32 + ncoef);
+  k_to_float<<<(ny + 255) / 256, 256, 0, stream()>>>(ny, f.g.dy, c32 + ncoef + nx);
+  k_to_float<<<(nz + 255) / 256, 256, 0, stream()>>>(nz, f.g.dz, c32 + ncoef + nx + ny);
+  FFPM_LAUNCHED("k_to_float");
+  g->nx = nx; g->ny = ny; g->nz = nz;
+  g->cw = c32; g->ce = c32 + nx; g->cs = c32 + 2 * nx; g->cn = c32 + 2 * nx + ny;
+  g->cb = c32 + 2 * nx + 2 * ny; g->ct = c32 + 2 * nx + 2 * ny + nz;
+  g->dx = c32 + ncoef; g->dy = c32 + ncoef + nx; g->dz = c32 + ncoef + nx + ny;
+  return OK;
+}
+
 struct MixedStats {
   int outer = 0, inner = 0;
 };
@@ -399,23 +421,10 @@ int pcg_mixed(Frame &f, double tolerance, int max_inner, int check_interval, dou
               int *sweeps, double *residual) {
   int st = 0;
   Workspace &ws = workspace();
-  const int nx = f.nx, ny = f.ny, nz = f.nz;
   const size_t n = f.n;
   const int eb = elementwise_blocks(n);
-  // FP32 geometry
-  float *c32 = (float *)ws.get(S_COEF32, sizeof(float) * 3 * (nx + ny + nz), &st);
-  if (!c32) return st;
-  const int ncoef = 2 * (nx + ny + nz);
-  k_to_float<<<(ncoef + 255) / 256, 256, 0, stream()>>>(ncoef, f.g.cw, c32);
-  k_to_float<<<(nx + 255) / 256, 256, 0, stream()>>>(nx, f.g.dx, c32 + ncoef);
-  k_to_float<<<(ny + 255) / 256, 256, 0, stream()>>>(ny, f.g.dy, c32 + ncoef + nx);
-  k_to_float<<<(nz + 255) / 256, 256, 0, stream()>>>(nz, f.g.dz, c32 + ncoef + nx + ny);
-  FFPM_LAUNCHED("k_to_float");
   GeomT<float> g;
-  g.nx = nx; g.ny = ny; g.nz = nz;
-  g.cw = c32; g.ce = c32 + nx; g.cs = c32 + 2 * nx; g.cn = c32 + 2 * nx + ny;
-  g.cb = c32 + 2 * nx + 2 * ny; g.ct = c32 + 2 * nx + 2 * ny + nz;
-  g.dx = c32 + ncoef; g.dy = c32 + ncoef + nx; g.dz = c32 + ncoef + nx + ny;
+  if ((st = make_geom32(f, &g))) return st;
   float *e = (float *)ws.get(S_E32, sizeof(float) * n, &st);
   float *r32 = e ? (float *)ws.get(S_R32, sizeof(float) * n, &st) : nullptr;
   float *z32 = r32 ? (float *)ws.get(S_Z32, sizeof(float) * n, &st) : nullptr;
@@ -423,7 +432,7 @@ int pcg_mixed(Frame &f, double tolerance, int max_inner, int check_interval, dou
   float *p1 = p0 ? (float *)ws.get(S_P32B, sizeof(float) * n, &st) : nullptr;
   if (!p1) return st;
   float *pbuf[2] = {p0, p1};
-  const int cl = cells_per_lane<float>(nx);
+  const int cl = cells_per_lane<float>(f.nx);
   const int ci = check_interval > 1 ? check_interval : 1;
   double res;
   if ((st = true_residual(f, false, &res))) return st;
@@ -796,6 +805,134 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
   return OK;
 }
 
+// x += alpha p, r -= alpha A p (A p recomputed in FP64 from the neighbour
+// code, as the SpMV of the direction kernel forms it); counts the iteration.
+__global__ __launch_bounds__(256) void k_fcg_xr(GeomT<double> g, const uint8_t *__restrict__ code,
+                                                const double *__restrict__ p,
+                                                double *__restrict__ x, double *__restrict__ r,
+                                                Scalars *sc) {
+  if (sc->status != 0) return;
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
+      threadIdx.y == 0)
+    sc->iters += 1;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (i >= nx || j >= ny) return;
+  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
+  const size_t c = (size_t)i + sx * ((size_t)j + (size_t)ny * k);
+  const unsigned cd = code[c];
+  if (!(cd & kCodeFluid)) return;
+  const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j], cb = g.cb[k], ct = g.ct[k];
+  const double pw = p[i > 0 ? c - 1 : c], pe = p[i < nx - 1 ? c + 1 : c];
+  const double ps = p[j > 0 ? c - sx : c], pn = p[j < ny - 1 ? c + sx : c];
+  const double pb = p[k > 0 ? c - sxy : c], pt = p[k < nz - 1 ? c + sxy : c];
+  double nb = (cd & kCodeW) ? cw * pw : 0.0;
+  nb += (cd & kCodeE) ? ce * pe : 0.0;
+  nb += (cd & kCodeS) ? cs * ps : 0.0;
+  nb += (cd & kCodeN) ? cn * pn : 0.0;
+  nb += (cd & kCodeB) ? cb * pb : 0.0;
+  nb += (cd & kCodeT) ? ct * pt : 0.0;
+  const double d = diag_code<double>(cd, cw, ce, cs, cn, cb, ct);
+  const double pc = p[c];
+  const double q = g.dx[i] * g.dy[j] * g.dz[k] * (d * pc - nb);
+  const double a = sc->alpha;
+  x[c] = x[c] + a * pc;
+  r[c] = r[c] - a * q;
+}
+
+// Convergence checks of the multigrid FCG: FFPM_MG_CHECK iterations (default
+// 5, at most the caller's check interval).
+int mg_check_interval(int check_interval) {
+  const char *e = std::getenv("FFPM_MG_CHECK");
+  int c = e ? std::atoi(e) : 5;
+  if (check_interval > 0 && check_interval < c) c = check_interval;
+  return c < 1 ? 1 : c;
+}
+
+struct MgStats {
+  int levels = 0;
+  int checks = 0;
+};
+thread_local MgStats g_mg;
+
+// Flexible CG with one stored direction (Notay's FCG(1)), FP64, preconditioned
+// by the FP32 multigrid cycle: z = B r; p = z - (z.A p_old / p_old.A p_old)
+// p_old; alpha = z.r / p.A p; x += alpha p; r -= alpha A p. The direction and
+// SpMV run in the fused PCG direction kernel (p = z + beta p_old, q = A p,
+// p.q). Every `check` iterations the FP64 projected true residual replaces r
+// (src/fpm_stairstep_mpi_3d.f90:297-329) and the max-norm residual is tested
+// against the tolerance, as in the reference's PCG; a breakdown restarts
+// once from p = z, and ten checks without a 1% gain stop at the roundoff
+// floor.
+int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
+                   double *residual) {
+  int st = 0;
+  GeomT<float> g32;
+  if ((st = make_geom32(f, &g32))) return st;
+  const GeomT<double> g = geom_t(f.g);
+  Multigrid mg;
+  if ((st = mg_setup(mg, g, g32, f.code))) return st;
+  g_mg = MgStats();
+  g_mg.levels = mg.levels;
+  const int ci = mg_check_interval(check_interval);
+  const int cl = cells_per_lane<double>(f.nx);
+  const dim3 xr_grid((unsigned)((f.nx + 63) / 64), (unsigned)((f.ny + 3) / 4), (unsigned)f.nz);
+  double *pbuf[2] = {f.next, f.q};
+  int cur = 0, mode = kDirReset;
+  double res;
+  if ((st = true_residual(f, false, &res))) return st;
+  double best = res;
+  int it = 0, restarts = 0, stalled = 0;
+  while (*sweeps < max_sweeps && res > tolerance) {
+    int n = ci - it % ci;
+    if (max_sweeps - *sweeps < n) n = max_sweeps - *sweeps;
+    k_chunk_begin<<<1, 64, 0, stream()>>>(f.sc);
+    FFPM_LAUNCHED("k_chunk_begin");
+    for (int m = 0; m < n; ++m) {
+      if ((st = mg_vcycle(mg, f.r, mode == kDirUpdate ? pbuf[cur] : nullptr, f.z))) return st;
+      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, mg.partials, mg.last_blocks, f.sc);
+      FFPM_LAUNCHED("k_finish");
+      const int dst = cur ^ 1;
+      if ((st = launch_dir<double>(f, cl, g, f.z, pbuf[cur], pbuf[dst], mode))) return st;
+      cur = dst;
+      {
+        TimedLaunch tl("fcg_xr");
+        k_fcg_xr<<<xr_grid, dim3(64, 4), 0, stream()>>>(g, f.code, pbuf[cur], f.phi, f.r, f.sc);
+      }
+      FFPM_LAUNCHED("k_fcg_xr");
+      mode = kDirUpdate;
+    }
+    Scalars h;
+    if ((st = read_scalars(f, &h))) return st;
+    *sweeps += h.iters;
+    it += h.iters;
+    if (h.iters > 0) restarts = 0;
+    if (h.status != 0) {
+      // breakdown (roundoff floor or lost conjugacy): restart once from p = z
+      restarts = restarts + 1;
+      if (restarts > 1) break;
+      k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
+      FFPM_LAUNCHED("k_clear_status");
+      if ((st = true_residual(f, false, &res))) return st;
+      mode = kDirReset;
+      continue;
+    }
+    if (it % ci == 0 || *sweeps == max_sweeps) {
+      if ((st = true_residual(f, false, &res))) return st;
+      g_mg.checks += 1;
+      if (res < 0.99 * best) {
+        best = res;
+        stalled = 0;
+      } else {
+        stalled = stalled + 1;
+      }
+      if (stalled >= 10 || !(res == res)) break;
+    }
+  }
+  *residual = res;
+  return OK;
+}
+
 // The whole solve on device pointers (src/fpm_stairstep_mpi_3d.f90:106-330).
 int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid,
                         const double *bg, const double *xlo, const double *xhi,
@@ -806,7 +943,8 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
                         int max_sweeps, int check_interval, int method, int *sweeps,
                         double *residual, const FcgArgs *fcg = nullptr) {
   if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI &&
-      method != FFPM_METHOD_PCG_MIXED && !(method == FFPM_METHOD_FCG && fcg)) {
+      method != FFPM_METHOD_PCG_MIXED && method != FFPM_METHOD_FCG_MG &&
+      !(method == FFPM_METHOD_FCG && fcg)) {
     set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
     return ERR_METHOD;
   }
@@ -822,7 +960,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   if (!all_present({phi_ref, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz}))
     return null_error();
   const bool pcg = method == FFPM_METHOD_PCG || method == FFPM_METHOD_PCG_MIXED ||
-                   method == FFPM_METHOD_FCG;
+                   method == FFPM_METHOD_FCG || method == FFPM_METHOD_FCG_MG;
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, pcg);
   if (st) return st;
@@ -847,6 +985,8 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   g_mixed = MixedStats();
   if (method == FFPM_METHOD_FCG) {
     st = fcg_iterate(f, *fcg, tolerance, max_sweeps, check_interval, sweeps, residual);
+  } else if (method == FFPM_METHOD_FCG_MG) {
+    st = fcg_mg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual);
   } else if (method == FFPM_METHOD_PCG_MIXED) {
     st = pcg_mixed(f, tolerance, max_sweeps, check_interval, mixed_inner_rtol(), sweeps,
                    residual);
@@ -1069,6 +1209,12 @@ int ffpm_last_mixed_solve(int *outer, int *inner) {
   return FFPM_OK;
 }
 
+int ffpm_last_mg_solve(int *levels, int *checks) {
+  *levels = g_mg.levels;
+  *checks = g_mg.checks;
+  return FFPM_OK;
+}
+
 int ffpm_exchange_solid_halo_3d(int32_t *solid, int nx, int ny, int nz, int west, int east,
                                 int south, int north) {
   // src/fpm_stairstep_mpi_3d.f90:29-32: zero the halo at physical boundaries.
@@ -1206,7 +1352,7 @@ int ffpm_solve_stairstep_poisson_3d(
     int east, int south, int north, double tolerance, int max_sweeps, int check_interval,
     int method, int *sweeps, double *residual) {
   if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI &&
-      method != FFPM_METHOD_PCG_MIXED) {
+      method != FFPM_METHOD_PCG_MIXED && method != FFPM_METHOD_FCG_MG) {
     set_error(ERR_METHOD, "stair-step method must be pcg or jacobi");
     return ERR_METHOD;
   }

@@ -103,11 +103,13 @@ contains
     if(present(use_openacc))continue
     m=0
     if(present(method))then
-      ! 'pcg-mixed' is this library's FP32-inner / FP64-outer PCG.
-      if(method/='pcg'.and.method/='jacobi'.and.method/='pcg-mixed') &
+      ! 'pcg-mixed' is this library's FP32-inner / FP64-outer PCG, 'fcg-mg'
+      ! its flexible CG with an FP32 multigrid preconditioner.
+      if(method/='pcg'.and.method/='jacobi'.and.method/='pcg-mixed'.and.method/='fcg-mg') &
            error stop 'stair-step method must be pcg or jacobi'
       if(method=='jacobi')m=1
       if(method=='pcg-mixed')m=2
+      if(method=='fcg-mg')m=4
     endif
     call stop_on(c_solve(phi,rhs,solid,body_grad,xlo,xhi,ylo,yhi,zlo,zhi,xc,yc,zc,dx,dy,dz, &
          nx,ny,nz,west,east,south,north,tolerance,max_sweeps,check_interval,m,sweeps,residual))

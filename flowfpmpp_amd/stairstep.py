@@ -25,7 +25,7 @@ from ._lib import FFPMError, check, lib
 from .device import DeviceArray
 
 PROC_NULL = -1
-METHODS = {"pcg": 0, "jacobi": 1, "pcg-mixed": 2}
+METHODS = {"pcg": 0, "jacobi": 1, "pcg-mixed": 2, "fcg-mg": 4}
 
 
 def _is_dev(a) -> bool:
@@ -100,7 +100,10 @@ def solve_stairstep_poisson_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, z
     """src/fpm_stairstep_mpi_3d.f90:106-330; phi is updated in place and
     (sweeps, residual) is returned. method "pcg-mixed" (no reference
     counterpart) runs FP64 defect correction around an FP32 PCG and stops on
-    the same FP64 max-norm residual; sweeps then counts inner iterations."""
+    the same FP64 max-norm residual; sweeps then counts inner iterations.
+    method "fcg-mg" (no reference counterpart) runs FP64 flexible CG with an
+    FP32 geometric-multigrid preconditioner, stopping on the same FP64
+    max-norm residual; sweeps counts FCG iterations."""
     del use_openacc
     if method not in METHODS:
         raise FFPMError(4, "FFPM_ERR_METHOD: stair-step method must be pcg or jacobi")
@@ -170,6 +173,13 @@ def last_mixed_solve():
     o, i = C.c_int(), C.c_int()
     check(lib().ffpm_last_mixed_solve(C.byref(o), C.byref(i)))
     return o.value, i.value
+
+
+def last_mg_solve():
+    """(levels, convergence checks) of the last "fcg-mg" solve on this thread."""
+    lv, ch = C.c_int(), C.c_int()
+    check(lib().ffpm_last_mg_solve(C.byref(lv), C.byref(ch)))
+    return lv.value, ch.value
 
 
 def timing(enable: bool, reset: bool = True) -> None:

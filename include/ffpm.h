@@ -53,12 +53,17 @@ enum {
   FFPM_ERR_DEVICE = 7         /* HIP failure: no device, launch or memory error */
 };
 
-/* Iteration selector (src/fpm_stairstep_mpi_3d.f90:125-129). PCG_MIXED has
- * no counterpart in the reference: FP64 defect correction around an FP32 PCG
- * (FP32 stencil and preconditioner, FP64 dots), stopping on the same FP64
- * max-norm residual; sweeps then counts inner iterations. */
+/* Iteration selector (src/fpm_stairstep_mpi_3d.f90:125-129). PCG_MIXED and
+ * FCG_MG have no counterpart in the reference. PCG_MIXED: FP64 defect
+ * correction around an FP32 PCG (FP32 stencil and preconditioner, FP64 dots),
+ * stopping on the same FP64 max-norm residual; sweeps then counts inner
+ * iterations. FCG_MG: FP64 flexible CG preconditioned by an FP32 geometric
+ * multigrid V-cycle on the same operator, stopping on the same FP64 max-norm
+ * residual (checked every 5 iterations or every check_interval if smaller);
+ * sweeps counts FCG iterations. */
 enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1, FFPM_METHOD_PCG_MIXED = 2,
-       FFPM_METHOD_FCG = 3 /* only through ffpm_solve_stairstep_fcg_dev */ };
+       FFPM_METHOD_FCG = 3 /* only through ffpm_solve_stairstep_fcg_dev */,
+       FFPM_METHOD_FCG_MG = 4 };
 
 const char *ffpm_last_error(void);
 /* Sets the HIP stream (hipStream_t) every later call launches on; 0 = null stream. */
@@ -123,6 +128,9 @@ int ffpm_solve_stairstep_poisson_3d_dev(
 /* Outer defect corrections and inner FP32 iterations of the last
  * FFPM_METHOD_PCG_MIXED solve on the calling thread (no reference counterpart). */
 int ffpm_last_mixed_solve(int *outer, int *inner);
+/* Multigrid levels and convergence checks of the last FFPM_METHOD_FCG_MG
+ * solve on the calling thread (no reference counterpart). */
+int ffpm_last_mg_solve(int *levels, int *checks);
 
 /* Batched solves (no counterpart in the reference, which solves one frame per
  * call of solve_stairstep_poisson_3d, src/fpm_stairstep_mpi_3d.f90:106-330).
