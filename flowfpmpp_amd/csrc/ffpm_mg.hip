@@ -19,6 +19,21 @@ inline dim3 cell_grid(int nx, int ny, int nz) {
   return dim3((unsigned)((nx + kMx - 1) / kMx), (unsigned)((ny + kMy - 1) / kMy), (unsigned)nz);
 }
 
+// Column kernels (one thread per coarse (I, J) walking K): K chunk so that
+// the launch has at least ~4096 wavefronts.
+inline int column_chunk(int cnx, int cny, int cnz) {
+  const long waves_per_plane = (long)((cnx + kMx - 1) / kMx) * ((cny + kMy - 1) / kMy) * 4;
+  long chunks = (4096 + waves_per_plane - 1) / waves_per_plane;
+  if (chunks < 1) chunks = 1;
+  if (chunks > cnz) chunks = cnz;
+  return (int)((cnz + chunks - 1) / chunks);
+}
+
+inline dim3 column_grid(int cnx, int cny, int cnz, int kchunk) {
+  return dim3((unsigned)((cnx + kMx - 1) / kMx), (unsigned)((cny + kMy - 1) / kMy),
+              (unsigned)((cnz + kchunk - 1) / kchunk));
+}
+
 struct Idx {
   int i, j, k;
   size_t c;
@@ -36,32 +51,6 @@ __device__ __forceinline__ Idx cell_idx(int nx, int ny) {
 }
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
-struct Fine {
-  float d, V, nb;  // diag, volume, sum of coefficient * neighbour
-};
-
-__device__ __forceinline__ Fine fine_row(const GeomT<float> &g, unsigned cd,
-                                         const float *__restrict__ x, size_t c, int i, int j,
-                                         int k) {
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
-  const float cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j], cb = g.cb[k], ct = g.ct[k];
-  const float xw = x[i > 0 ? c - 1 : c], xe = x[i < nx - 1 ? c + 1 : c];
-  const float xs = x[j > 0 ? c - sx : c], xn = x[j < ny - 1 ? c + sx : c];
-  const float xb = x[k > 0 ? c - sxy : c], xt = x[k < nz - 1 ? c + sxy : c];
-  Fine f;
-  f.d = diag_code<float>(cd, cw, ce, cs, cn, cb, ct);
-  f.V = g.dx[i] * g.dy[j] * g.dz[k];
-  float s = (cd & kCodeW) ? cw * xw : 0.f;
-  s += (cd & kCodeE) ? ce * xe : 0.f;
-  s += (cd & kCodeS) ? cs * xs : 0.f;
-  s += (cd & kCodeN) ? cn * xn : 0.f;
-  s += (cd & kCodeB) ? cb * xb : 0.f;
-  s += (cd & kCodeT) ? ct * xt : 0.f;
-  f.nb = s;
-  return f;
-}
-
 // x = omega D^-1 r (the first sweep from zero); D = V diag.
 __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t *__restrict__ code,
                                                   const double *__restrict__ r,
@@ -75,48 +64,216 @@ __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t 
   x[t.c] = d > 0.f ? omega * ((float)r[t.c] / V) / d : 0.f;
 }
 
-// One damped-Jacobi sweep on A x = r: out = in + omega D^-1 (r - A in).
-__global__ __launch_bounds__(256) void k_mgf_sweep(GeomT<float> g,
-                                                   const uint8_t *__restrict__ code,
-                                                   const double *__restrict__ r,
-                                                   const float *__restrict__ in,
-                                                   float *__restrict__ out, float omega) {
-  const Idx t = cell_idx(g.nx, g.ny);
-  if (!t.in) return;
-  const unsigned cd = code[t.c];
-  const Fine f = fine_row(g, cd, in, t.c, t.i, t.j, t.k);
-  const float xv = in[t.c];
-  out[t.c] = f.d > 0.f ? xv + omega * ((float)r[t.c] / f.V + f.nb - f.d * xv) / f.d : 0.f;
-}
-
 // Coarse right-hand side: the level-0 residual r - A x summed over the
-// children of each level-1 cell (one thread per level-1 cell).
-__global__ __launch_bounds__(256) void k_mgf_restrict(GeomT<float> g,
-                                                      const uint8_t *__restrict__ code,
-                                                      const double *__restrict__ r,
-                                                      const float *__restrict__ x, int cnx,
-                                                      int cny, float *__restrict__ bc) {
-  const Idx t = cell_idx(cnx, cny);
-  if (!t.in) return;
-  float s = 0.f;
-  for (int dk = 0; dk < 2; ++dk) {
-    const int k = 2 * t.k + dk;
-    if (k >= g.nz) break;
-    for (int dj = 0; dj < 2; ++dj) {
-      const int j = 2 * t.j + dj;
-      if (j >= g.ny) break;
-      for (int di = 0; di < 2; ++di) {
-        const int i = 2 * t.i + di;
-        if (i >= g.nx) break;
-        const size_t c = (size_t)i + (size_t)g.nx * ((size_t)j + (size_t)g.ny * k);
-        const unsigned cd = code[c];
-        if (!(cd & kCodeFluid)) continue;
-        const Fine f = fine_row(g, cd, x, c, i, j, k);
-        s += (float)r[c] - f.V * (f.d * x[c] - f.nb);
+// children of each level-1 cell. One thread per level-1 (I, J) column walks a
+// range of K: each fine plane's 2x2 block of x is loaded once and kept for
+// the planes above and below; only the in-plane halo comes from other
+// threads' cells (cache hits). VEC (even nx): the two x-adjacent children of
+// a row move as one 8-byte (x), 16-byte (r) and 2-byte (code) access.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
+                                                        const uint8_t *__restrict__ code,
+                                                        const double *__restrict__ r,
+                                                        const float *__restrict__ x, int cnx,
+                                                        int cny, int cnz, int kchunk,
+                                                        float *__restrict__ bc) {
+  const int I = blockIdx.x * kMx + threadIdx.x, J = blockIdx.y * kMy + threadIdx.y;
+  if (I >= cnx || J >= cny) return;
+  const int K0 = blockIdx.z * kchunk, K1 = min(cnz, K0 + kchunk);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const size_t sx = (size_t)nx;
+  const int i0 = 2 * I, j0 = 2 * J;
+  const bool va[2] = {true, i0 + 1 < nx}, vb[2] = {true, j0 + 1 < ny};
+  const int ia[2] = {i0, va[1] ? i0 + 1 : i0}, jb[2] = {j0, vb[1] ? j0 + 1 : j0};
+  float cw[2], ce[2], dxa[2], cs[2], cn[2], dyb[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    cw[a] = g.cw[ia[a]];
+    ce[a] = g.ce[ia[a]];
+    dxa[a] = g.dx[ia[a]];
+    cs[a] = g.cs[jb[a]];
+    cn[a] = g.cn[jb[a]];
+    dyb[a] = g.dy[jb[a]];
+  }
+  const bool hw = i0 > 0, he = i0 + 2 < nx, hs = j0 > 0, hn = j0 + 2 < ny;
+  auto at = [&](int i, int j, int k) { return (size_t)i + sx * ((size_t)j + (size_t)ny * k); };
+  auto load = [&](int k, float (&v)[2][2]) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const bool ok = k >= 0 && k < nz && vb[b];
+      if constexpr (VEC) {
+        const float2 t = ok ? *reinterpret_cast<const float2 *>(x + at(i0, jb[b], k))
+                            : make_float2(0.f, 0.f);
+        v[b][0] = t.x;
+        v[b][1] = t.y;
+      } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) v[b][a] = ok && va[a] ? x[at(ia[a], jb[b], k)] : 0.f;
       }
     }
+  };
+  auto plane = [&](int k, const float (&dn)[2][2], const float (&c)[2][2],
+                   const float (&up)[2][2]) {
+    float s = 0.f;
+    if (k >= nz) return s;
+    const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      if (!vb[b]) continue;
+      const size_t c0 = at(i0, jb[b], k);
+      unsigned cds[2];
+      double rs[2];
+      if constexpr (VEC) {
+        const uchar2 cd2 = *reinterpret_cast<const uchar2 *>(code + c0);
+        const double2 r2 = *reinterpret_cast<const double2 *>(r + c0);
+        cds[0] = cd2.x;
+        cds[1] = cd2.y;
+        rs[0] = r2.x;
+        rs[1] = r2.y;
+      } else {
+        cds[0] = code[c0];
+        rs[0] = r[c0];
+        cds[1] = va[1] ? code[c0 + 1] : 0u;
+        rs[1] = va[1] ? r[c0 + 1] : 0.0;
+      }
+      const float xw = hw ? x[c0 - 1] : 0.f, xe = he ? x[c0 + 2] : 0.f;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const unsigned cd = cds[a];
+        if (!(cd & kCodeFluid)) continue;
+        const size_t cc = c0 + a;
+        const float west = a == 1 ? c[b][0] : xw;
+        const float east = a == 0 ? c[b][1] : xe;
+        const float south = b == 1 ? c[0][a] : (hs ? x[cc - sx] : 0.f);
+        const float north = b == 0 ? c[1][a] : (hn ? x[cc + sx] : 0.f);
+        float nb = (cd & kCodeW) ? cw[a] * west : 0.f;
+        nb += (cd & kCodeE) ? ce[a] * east : 0.f;
+        nb += (cd & kCodeS) ? cs[b] * south : 0.f;
+        nb += (cd & kCodeN) ? cn[b] * north : 0.f;
+        nb += (cd & kCodeB) ? cb * dn[b][a] : 0.f;
+        nb += (cd & kCodeT) ? ct * up[b][a] : 0.f;
+        const float d = diag_code<float>(cd, cw[a], ce[a], cs[b], cn[b], cb, ct);
+        s += (float)rs[a] - dxa[a] * dyb[b] * dzk * (d * c[b][a] - nb);
+      }
+    }
+    return s;
+  };
+  float below[2][2], c0[2][2], c1[2][2], above[2][2];
+  load(2 * K0 - 1, below);
+  load(2 * K0, c0);
+  load(2 * K0 + 1, c1);
+  for (int K = K0; K < K1; ++K) {
+    load(2 * K + 2, above);
+    const float s = plane(2 * K, below, c0, c1) + plane(2 * K + 1, c0, c1, above);
+    bc[(size_t)I + (size_t)cnx * ((size_t)J + (size_t)cny * K)] = s;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        below[b][a] = c1[b][a];
+        c0[b][a] = above[b][a];
+      }
+    load(2 * K + 3, c1);
   }
-  bc[t.c] = s;
+}
+
+// Prolongation onto level 0 with one thread per level-1 (I, J) column walking
+// a range of K: the 3x3 coarse patches (values and unknown masks) of planes
+// K-1, K, K+1 stay in registers, and the thread updates the 2x2x2 fine cells
+// of each coarse cell with the trilinear weights of prolong_value (only fine
+// cells that carry unknowns change). VEC (even nx): each fine row pair moves as
+// one 8-byte x and 2-byte code access.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
+                                                       const uint8_t *__restrict__ code,
+                                                       float *__restrict__ x, int cnx, int cny,
+                                                       int cnz, const float *__restrict__ xc,
+                                                       const float *__restrict__ mc,
+                                                       int kchunk) {
+  const int I = blockIdx.x * kMx + threadIdx.x, J = blockIdx.y * kMy + threadIdx.y;
+  if (I >= cnx || J >= cny) return;
+  const int K0 = blockIdx.z * kchunk, K1 = min(cnz, K0 + kchunk);
+  const bool vi[3] = {I > 0, true, I + 1 < cnx}, vj[3] = {J > 0, true, J + 1 < cny};
+  const int ii[3] = {vi[0] ? I - 1 : I, I, vi[2] ? I + 1 : I};
+  const int jj[3] = {vj[0] ? J - 1 : J, J, vj[2] ? J + 1 : J};
+  const size_t csx = (size_t)cnx, fsx = (size_t)nx;
+  const bool has_i1 = 2 * I + 1 < nx;
+  auto cload = [&](int K, float (&v)[3][3], float (&m)[3][3]) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const bool ok = K >= 0 && K < cnz && vi[a] && vj[b];
+        const size_t cc = (size_t)ii[a] + csx * ((size_t)jj[b] + (size_t)cny * K);
+        v[b][a] = ok ? xc[cc] : 0.f;
+        m[b][a] = ok ? mc[cc] : 0.f;
+      }
+  };
+  float xm[3][3], mm[3][3], x0[3][3], m0[3][3], xp[3][3], mp[3][3];
+  cload(K0 - 1, xm, mm);
+  cload(K0, x0, m0);
+  for (int K = K0; K < K1; ++K) {
+    cload(K + 1, xp, mp);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int k = 2 * K + c;
+      if (k >= nz) break;
+      const float(&xz)[3][3] = c ? xp : xm;
+      const float(&mz)[3][3] = c ? mp : mm;
+      const float wk = (c ? K + 1 < cnz : K > 0) ? 0.25f : 0.f;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int j = 2 * J + b;
+        if (j >= ny) break;
+        const int bj = b ? 2 : 0;
+        const float wj = vj[bj] ? 0.25f : 0.f;
+        const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)j + (size_t)ny * k);
+        float xv[2];
+        unsigned cd[2];
+        if constexpr (VEC) {
+          const float2 t = *reinterpret_cast<const float2 *>(x + c0);
+          const uchar2 u = *reinterpret_cast<const uchar2 *>(code + c0);
+          xv[0] = t.x;
+          xv[1] = t.y;
+          cd[0] = u.x;
+          cd[1] = u.y;
+        } else {
+          xv[0] = x[c0];
+          cd[0] = code[c0];
+          xv[1] = has_i1 ? x[c0 + 1] : 0.f;
+          cd[1] = has_i1 ? code[c0 + 1] : 0u;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          if (!(cd[a] & kCodeFluid)) continue;
+          const int ai = a ? 2 : 0;
+          const float wi = vi[ai] ? 0.25f : 0.f;
+          const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
+          const float num0 = w00 * x0[1][1] + w10 * x0[1][ai] + w01 * x0[bj][1] + w11 * x0[bj][ai];
+          const float den0 = w00 * m0[1][1] + w10 * m0[1][ai] + w01 * m0[bj][1] + w11 * m0[bj][ai];
+          const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
+          const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
+          const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
+          xv[a] += den > 0.f ? num / den : 0.f;
+        }
+        if constexpr (VEC) {
+          *reinterpret_cast<float2 *>(x + c0) = make_float2(xv[0], xv[1]);
+        } else {
+          x[c0] = xv[0];
+          if (has_i1) x[c0 + 1] = xv[1];
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        xm[b][a] = x0[b][a];
+        mm[b][a] = m0[b][a];
+        x0[b][a] = xp[b][a];
+        m0[b][a] = mp[b][a];
+      }
+  }
 }
 
 // ---- coarse levels: per-cell conductances ------------------------------------
@@ -233,17 +390,6 @@ __device__ __forceinline__ float prolong_value(int i, int j, int k, int cnx, int
   return den > 0.f ? num / den : 0.f;
 }
 
-__global__ __launch_bounds__(256) void k_mgf_prolong(int nx, int ny,
-                                                     const uint8_t *__restrict__ code,
-                                                     float *__restrict__ x, int cnx, int cny,
-                                                     int cnz, const float *__restrict__ xc,
-                                                     const float *__restrict__ mc) {
-  const Idx t = cell_idx(nx, ny);
-  if (!t.in) return;
-  if (!(code[t.c] & kCodeFluid)) return;
-  x[t.c] += prolong_value(t.i, t.j, t.k, cnx, cny, cnz, xc, mc);
-}
-
 __global__ __launch_bounds__(256) void k_mgc_prolong(int nx, int ny, const float *__restrict__ m,
                                                      float *__restrict__ x, int cnx, int cny,
                                                      int cnz, const float *__restrict__ xc,
@@ -299,53 +445,118 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
   if (on) x[t] = xv;
 }
 
-// Last post-smoothing sweep of level 0, written as z in FP64, with the partial
-// sums z.r and z.(A p_old) (A in FP64 from the code and the FP64
-// coefficients, the operator of the outer iteration).
-__global__ __launch_bounds__(256) void k_mgf_last(GeomT<float> g, GeomT<double> gd,
-                                                  const uint8_t *__restrict__ code,
-                                                  const double *__restrict__ r,
-                                                  const float *__restrict__ in,
-                                                  const double *__restrict__ pold,
-                                                  double *__restrict__ z, double *partials,
-                                                  float omega, int nblocks) {
+// Streaming damped-Jacobi sweep of level 0 in FP32 (the structure of the
+// reference-Jacobi kernel k_jacobi_t: each lane owns CL x-adjacent cells of
+// one row and walks its z range; plane k+1's loads are issued before plane k
+// is computed; west/east neighbours by wave shuffle):
+//   out = (1 - w) a + (w / d) (r / V + sum c a_nb), 0 where d = 0.
+// LAST writes out as z in FP64 and the partial sums z.r (one slot).
+template <int CL, bool LAST>
+__global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
+                                                     const float *__restrict__ a,
+                                                     const double *__restrict__ r,
+                                                     const uint8_t *__restrict__ code,
+                                                     float *__restrict__ b,
+                                                     double *__restrict__ z, double *partials,
+                                                     float omega) {
   __shared__ double lds[16 * kReduceSlots];
-  const int i = blockIdx.x * kMx + threadIdx.x, j = blockIdx.y * kMy + threadIdx.y;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double s[2] = {0.0, 0.0};
-  if (i < nx && j < ny) {
-    const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
-    for (int k = blockIdx.z; k < nz; k += gridDim.z) {
-      const size_t c = (size_t)i + sx * ((size_t)j + (size_t)ny * k);
-      const unsigned cd = code[c];
-      const Fine f = fine_row(g, cd, in, c, i, j, k);
-      const float xv = in[c];
-      const double rv = r[c];
-      const float zf = f.d > 0.f ? xv + omega * ((float)rv / f.V + f.nb - f.d * xv) / f.d : 0.f;
-      const double zv = (double)zf;
-      z[c] = zv;
-      s[0] += zv * rv;
-      if (pold && (cd & kCodeFluid)) {
-        const double cw = gd.cw[i], ce = gd.ce[i], cs = gd.cs[j], cn = gd.cn[j], cb = gd.cb[k],
-                     ct = gd.ct[k];
-        const double pw = pold[i > 0 ? c - 1 : c], pe = pold[i < nx - 1 ? c + 1 : c];
-        const double ps = pold[j > 0 ? c - sx : c], pn = pold[j < ny - 1 ? c + sx : c];
-        const double pb = pold[k > 0 ? c - sxy : c], pt = pold[k < nz - 1 ? c + sxy : c];
-        double nb = (cd & kCodeW) ? cw * pw : 0.0;
-        nb += (cd & kCodeE) ? ce * pe : 0.0;
-        nb += (cd & kCodeS) ? cs * ps : 0.0;
-        nb += (cd & kCodeN) ? cn * pn : 0.0;
-        nb += (cd & kCodeB) ? cb * pb : 0.0;
-        nb += (cd & kCodeT) ? ct * pt : 0.0;
-        const double d = diag_code<double>(cd, cw, ce, cs, cn, cb, ct);
-        const double V = gd.dx[i] * gd.dy[j] * gd.dz[k];
-        s[1] += zv * (V * (d * pold[c] - nb));
+  const Lane l = lane_of<CL>(t, nx, ny, nz, blockIdx.x);
+  double s = 0.0;
+  if (l.active) {
+    using V = Vec<float, CL>;
+    using VD = Vec<double, CL>;
+    const int i = l.i, j = l.j;
+    const size_t nxy = (size_t)nx * ny;
+    float cw[CL], ce[CL], dxdy[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+      cw[m] = g.cw[i + m];
+      ce[m] = g.ce[i + m];
+      dxdy[m] = g.dx[i + m] * g.dy[j];
+    }
+    const float cs = g.cs[j], cn = g.cn[j];
+    const float keep = 1.f - omega;
+    const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+    const int lane = (int)threadIdx.x;
+    const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
+    const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : row0;
+    const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : row0;
+    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+    auto at = [&](int k) { return base + nxy * (size_t)k; };
+    struct Set {
+      V a_next;
+      VD r;
+      Plane<kDirCurrent, float, CL> pl;
+    };
+    Set sa, sb;
+    V a_cur = ldv<float, CL>(a, at(k0));
+    V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
+    sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
+    sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
+    sa.r = ldv<double, CL>(r, at(k0));
+    auto stage = [&](int k, const Set &u, Set &ld) {
+      const int kn = min(k + 1, k1 - 1);
+      ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
+      ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
+      ld.r = ldv<double, CL>(r, at(kn));
+      const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
+      const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
+      const float vw = lane == 0 ? u.pl.w.a : from_w;
+      const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
+      const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
+      V bv;
+      VD zv;
+#pragma unroll
+      for (int m = 0; m < CL; ++m) {
+        const float west = m == 0 ? (hw ? vw : 0.f) : a_cur.v[m - 1];
+        const float east = m == CL - 1 ? (he ? ve : 0.f) : a_cur.v[m + 1];
+        const float south = hs ? u.pl.s.a.v[m] : 0.f, north = hn ? u.pl.n.a.v[m] : 0.f;
+        const float d = diag_code<float>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
+        const float src = (float)u.r.v[m] / (dxdy[m] * dzk);
+        const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
+                         cb * a_prev.v[m] + ct * u.a_next.v[m];
+        const float o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
+        if constexpr (LAST) {
+          zv.v[m] = (double)o;
+          s += zv.v[m] * u.r.v[m];
+        } else {
+          bv.v[m] = o;
+        }
       }
+      if constexpr (LAST) stv<double, CL>(z, at(k), zv);
+      else stv<float, CL>(b, at(k), bv);
+      a_prev = a_cur;
+      a_cur = u.a_next;
+    };
+    for (int k = k0; k < k1; k += 2) {
+      stage(k, sa, sb);
+      if (k + 1 < k1) stage(k + 1, sb, sa);
     }
   }
-  block_sum<2>(s, lds);
-  const int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  store_partials(partials, 2, s, nblocks, block);
+  if constexpr (LAST) {
+    double v[1] = {s};
+    block_sum<1>(v, lds);
+    store_partials(partials, 1, v, gridDim.x, blockIdx.x);
+  }
+}
+
+// x += alpha p, r -= alpha A p (the fused PCG update body) and the first
+// smoothing step of the next cycle, x1 = w D^-1 r, stored in FP32; counts the
+// iteration.
+template <int CL>
+__global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
+                                                   const double *__restrict__ p,
+                                                   const uint8_t *__restrict__ code,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   float *__restrict__ x1, Scalars *sc,
+                                                   double omega) {
+  if (sc->status != 0) return;
+  pcg_upd_body<double, CL, float, false>(t, blockIdx.x, gridDim.x, g, p, code, x, r, x1, nullptr,
+                                         sc, nullptr, omega);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
 }
 
 // ---- setup --------------------------------------------------------------------
@@ -440,10 +651,13 @@ float env_float(const char *name, float fallback) {
 }  // namespace
 
 int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
-             const uint8_t *code) {
+             const uint8_t *code, const Tiling &tl, int cl, double *partials) {
   mg.g64 = g64;
   mg.g32 = g32;
   mg.code = code;
+  mg.tl = tl;
+  mg.cl = cl;
+  mg.partials = partials;
   mg.omega = env_float("FFPM_MG_OMEGA", 0.8f);
   mg.scale = env_float("FFPM_MG_SCALE", 0.5f);
   mg.coarse_sweeps = (int)env_float("FFPM_MG_COARSE_SWEEPS", 30.f);
@@ -471,15 +685,10 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     return ERR_SHAPE;
   }
   mg.levels = L + 1;
-  // last kernel's launch: z planes strided over at most 16 grid rows
-  mg.kz = g64.nz < 16 ? g64.nz : 16;
-  const dim3 lg = cell_grid(g64.nx, g64.ny, mg.kz);
-  mg.last_blocks = (int)(lg.x * lg.y * lg.z);
-  // one arena: level-0 x, y; per coarse level gE gN gT m x y b; partials
+  // one arena: level-0 x, y; per coarse level gE gN gT m x y b
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   size_t bytes = 2 * up(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) bytes += 7 * up(sizeof(float) * mg.lv[l].n);
-  bytes += up(sizeof(double) * 2 * (size_t)mg.last_blocks);
   int st = 0;
   char *base = (char *)workspace().get(S_MG, bytes, &st);
   if (!base) return st;
@@ -497,7 +706,6 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     v.m = (float *)take(b); v.x = (float *)take(b); v.y = (float *)take(b);
     v.b = (float *)take(b);
   }
-  mg.partials = (double *)take(sizeof(double) * 2 * (size_t)mg.last_blocks);
   // conductances and masks of the coarse levels
   {
     const MgLevel &c = mg.lv[1];
@@ -520,26 +728,71 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   return OK;
 }
 
-int mg_vcycle(const Multigrid &mg, const double *r, const double *p_old, double *z) {
+namespace {
+
+// Level-0 streaming sweep launch for the multigrid's cells-per-lane choice.
+template <bool LAST>
+void launch_sweep(const Multigrid &mg, const float *a, const double *r, float *b, double *z) {
+  const dim3 blk(kTileX, mg.tl.ty);
+  const float w = mg.omega;
+  if (mg.cl == 4)
+    k_mg_sweep_t<4, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
+                                                              mg.partials, w);
+  else if (mg.cl == 2)
+    k_mg_sweep_t<2, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
+                                                              mg.partials, w);
+  else
+    k_mg_sweep_t<1, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
+                                                              mg.partials, w);
+}
+
+}  // namespace
+
+int mg_first(const Multigrid &mg, const double *r) {
+  const MgLevel &f0 = mg.lv[0];
+  k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMx, kMy), 0, stream()>>>(mg.g32, mg.code, r,
+                                                                             f0.x, mg.omega);
+  FFPM_LAUNCHED("k_mgf_jac0");
+  return OK;
+}
+
+int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
+              double *r, Scalars *sc) {
+  const dim3 blk(kTileX, t.ty);
+  const double w = (double)mg.omega;
+  float *x1 = mg.lv[0].x;
+  TimedLaunch tl("mg_update");
+  if (cl == 2)
+    k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w);
+  else
+    k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w);
+  FFPM_LAUNCHED("k_mg_upd_t");
+  return OK;
+}
+
+int mg_vcycle(const Multigrid &mg, const double *r, double *z) {
   const dim3 blk(kMx, kMy);
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
   const float w = mg.omega;
   const int L = mg.levels - 1;  // coarsest index
   {
-    TimedLaunch tl("mg_fine_pre");
-    k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), blk, 0, stream()>>>(g, mg.code, r, f0.x, w);
-    k_mgf_sweep<<<cell_grid(f0.nx, f0.ny, f0.nz), blk, 0, stream()>>>(g, mg.code, r, f0.x, f0.y,
-                                                                      w);
+    TimedLaunch tl("mg_fine_sweep");
+    launch_sweep<false>(mg, f0.x, r, f0.y, nullptr);
   }
-  FFPM_LAUNCHED("k_mgf_sweep");
+  FFPM_LAUNCHED("k_mg_sweep_t");
   {
     const MgLevel &c = mg.lv[1];
     TimedLaunch tl("mg_fine_restrict");
-    k_mgf_restrict<<<cell_grid(c.nx, c.ny, c.nz), blk, 0, stream()>>>(g, mg.code, r, f0.y, c.nx,
-                                                                       c.ny, c.b);
+    const int kc = column_chunk(c.nx, c.ny, c.nz);
+    if (f0.nx % 2 == 0)
+      k_mgf_restrict_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
+          g, mg.code, r, f0.y, c.nx, c.ny, c.nz, kc, c.b);
+    else
+      k_mgf_restrict_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
+          g, mg.code, r, f0.y, c.nx, c.ny, c.nz, kc, c.b);
   }
-  FFPM_LAUNCHED("k_mgf_restrict");
+  FFPM_LAUNCHED("k_mgf_restrict_z");
   // down: pre-smoothing (result in y) and restriction
   for (int l = 1; l < L; ++l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
@@ -567,19 +820,26 @@ int mg_vcycle(const Multigrid &mg, const double *r, const double *p_old, double 
   }
   {
     const MgLevel &c = mg.lv[1];
-    TimedLaunch tl("mg_fine_post");
-    k_mgf_prolong<<<cell_grid(f0.nx, f0.ny, f0.nz), blk, 0, stream()>>>(
-        f0.nx, f0.ny, mg.code, f0.y, c.nx, c.ny, c.nz, c.y, c.m);
-    k_mgf_sweep<<<cell_grid(f0.nx, f0.ny, f0.nz), blk, 0, stream()>>>(g, mg.code, r, f0.y, f0.x,
-                                                                      w);
+    TimedLaunch tl("mg_fine_prolong");
+    const int kc = column_chunk(c.nx, c.ny, c.nz);
+    if (f0.nx % 2 == 0)
+      k_mgf_prolong_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
+          f0.nx, f0.ny, f0.nz, mg.code, f0.y, c.nx, c.ny, c.nz, c.y, c.m, kc);
+    else
+      k_mgf_prolong_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
+          f0.nx, f0.ny, f0.nz, mg.code, f0.y, c.nx, c.ny, c.nz, c.y, c.m, kc);
   }
-  FFPM_LAUNCHED("k_mgf_prolong");
+  FFPM_LAUNCHED("k_mgf_prolong_z");
+  {
+    TimedLaunch tl("mg_fine_sweep");
+    launch_sweep<false>(mg, f0.y, r, f0.x, nullptr);
+  }
+  FFPM_LAUNCHED("k_mg_sweep_t");
   {
     TimedLaunch tl("mg_fine_last");
-    k_mgf_last<<<cell_grid(f0.nx, f0.ny, mg.kz), blk, 0, stream()>>>(
-        g, mg.g64, mg.code, r, f0.x, p_old, z, mg.partials, w, mg.last_blocks);
+    launch_sweep<true>(mg, f0.x, r, nullptr, z);
   }
-  FFPM_LAUNCHED("k_mgf_last");
+  FFPM_LAUNCHED("k_mg_sweep_t");
   return OK;
 }
 

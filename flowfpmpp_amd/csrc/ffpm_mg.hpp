@@ -13,12 +13,13 @@
 // its diagonal is positive.
 //
 // Cycle (FP32 arithmetic, FP64 input r and output z): two damped-Jacobi
-// sweeps from zero, restriction of the residual by summation over the
-// children, recursion, trilinear cell-centred prolongation with weights
-// renormalised over the coarse cells that carry unknowns, two damped-Jacobi
-// sweeps. The coarsest level (at most 1024 cells) runs a fixed number of
-// sweeps in one workgroup. Every step is a fixed linear map, so the flexible
-// CG around it sees the same preconditioner at every iteration.
+// sweeps from zero (the first one fused into the outer update), restriction
+// of the residual by summation over the children, recursion, trilinear
+// cell-centred prolongation with weights renormalised over the coarse cells
+// that carry unknowns, two damped-Jacobi sweeps. The coarsest level (at most
+// 1024 cells) runs a fixed number of sweeps in one workgroup. Every step is a
+// fixed linear map, so the CG around it sees the same preconditioner at
+// every iteration.
 #pragma once
 
 #include "ffpm_pcg.hpp"
@@ -42,21 +43,30 @@ struct Multigrid {
   GeomT<float> g32{};
   GeomT<double> g64{};
   const uint8_t *code = nullptr;
+  Tiling tl{};              // level-0 streaming launch shape (CL cells per lane)
+  int cl = 1;
   float omega = 0.8f;
   float scale = 0.5f;
   int coarse_sweeps = 30;
-  int kz = 1;               // z planes per workgroup stride of the last kernel
-  int last_blocks = 0;      // workgroups (partials per slot) of the last kernel
-  double *partials = nullptr;
+  double *partials = nullptr;  // z.r partials of the last sweep (tl.blocks)
 };
 
 // Builds the hierarchy for the frame with FP64 geometry g64, its FP32 copy
-// g32 and the neighbour code (workspace slot S_MG).
+// g32 and the neighbour code (workspace slot S_MG); tl and cl are the
+// level-0 streaming launch shape for FP32 values.
 int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
-             const uint8_t *code);
+             const uint8_t *code, const Tiling &tl, int cl, double *partials);
 
-// z = B r (one cycle). Leaves in mg.partials two slots of mg.last_blocks
-// partial sums: z.r and z.(A p_old) with A in FP64 (zero when p_old is null).
-int mg_vcycle(const Multigrid &mg, const double *r, const double *p_old, double *z);
+// First smoothing step from zero on r: lv[0].x = w D^-1 r.
+int mg_first(const Multigrid &mg, const double *r);
+
+// x += alpha p, r -= alpha A p and lv[0].x = w D^-1 r (alpha in sc; no-op
+// after a breakdown); counts the iteration. t, cl: FP64 launch shape.
+int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
+              double *r, Scalars *sc);
+
+// z = B r (the rest of the cycle after the first step, which must be in
+// lv[0].x). Leaves the z.r partial sums (one slot, tl.blocks) in partials.
+int mg_vcycle(const Multigrid &mg, const double *r, double *z);
 
 }  // namespace ffpm

@@ -247,13 +247,16 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 }
 
 // q = A p (recomputed), x += alpha p, r -= alpha q, z = M^-1 r and z.r.
-template <typename T, int CL>
+// With DOT false (the multigrid-preconditioned CG) z = zscale M^-1 r is stored
+// in ZT (the multigrid's first smoothing step from zero) and no dot is formed.
+template <typename T, int CL, typename ZT = T, bool DOT = true>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const T *__restrict__ p,
                                              const uint8_t *__restrict__ code,
                                              T *__restrict__ x, T *__restrict__ r,
-                                             T *__restrict__ z, double *partials,
-                                             const Scalars *sc, double *lds) {
+                                             ZT *__restrict__ z, double *partials,
+                                             const Scalars *sc, double *lds,
+                                             T zscale = T(1)) {
   if (sc->status != 0) return;
   const T alpha = (T)sc->alpha;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -303,7 +306,8 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       ld.x = ldv<T, CL>(x, at(kn));
       ld.r = ldv<T, CL>(r, at(kn));
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
-      Vec<T, CL> xv = u.x, rv = u.r, zv;
+      Vec<T, CL> xv = u.x, rv = u.r;
+      Vec<ZT, CL> zv;
       const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
       const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
       const T vw = lane == 0 ? u.pl.w.a : from_w;
@@ -320,12 +324,16 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
         const T q = d > 0 ? qr : T(0);
         xv.v[m] = xv.v[m] + alpha * p_cur.v[m];
         rv.v[m] = rv.v[m] - alpha * q;
-        zv.v[m] = rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m];
-        s += (double)zv.v[m] * (double)rv.v[m];
+        if constexpr (DOT) {
+          zv.v[m] = rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m];
+          s += (double)zv.v[m] * (double)rv.v[m];
+        } else {
+          zv.v[m] = (ZT)(zscale * rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m]);
+        }
       }
       stv<T, CL>(x, at(k), xv);
       stv<T, CL>(r, at(k), rv);
-      stv<T, CL>(z, at(k), zv);
+      stv<ZT, CL>(z, at(k), zv);
       p_prev = p_cur;
       p_cur = u.p_next;
     };
@@ -334,9 +342,11 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       if (k + 1 < k1) stage(k + 1, sb, sa);
     }
   }
-  double v[1] = {s};
-  block_sum<1>(v, lds);
-  store_partials(partials, 1, v, nblocks, block);
+  if constexpr (DOT) {
+    double v[1] = {s};
+    block_sum<1>(v, lds);
+    store_partials(partials, 1, v, nblocks, block);
+  }
 }
 
 // Inner right-hand side of a defect correction: r32 = r64 (the projected FP64

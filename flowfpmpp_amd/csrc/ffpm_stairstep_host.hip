@@ -805,41 +805,6 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
   return OK;
 }
 
-// x += alpha p, r -= alpha A p (A p recomputed in FP64 from the neighbour
-// code, as the SpMV of the direction kernel forms it); counts the iteration.
-__global__ __launch_bounds__(256) void k_fcg_xr(GeomT<double> g, const uint8_t *__restrict__ code,
-                                                const double *__restrict__ p,
-                                                double *__restrict__ x, double *__restrict__ r,
-                                                Scalars *sc) {
-  if (sc->status != 0) return;
-  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
-      threadIdx.y == 0)
-    sc->iters += 1;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (i >= nx || j >= ny) return;
-  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
-  const size_t c = (size_t)i + sx * ((size_t)j + (size_t)ny * k);
-  const unsigned cd = code[c];
-  if (!(cd & kCodeFluid)) return;
-  const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j], cb = g.cb[k], ct = g.ct[k];
-  const double pw = p[i > 0 ? c - 1 : c], pe = p[i < nx - 1 ? c + 1 : c];
-  const double ps = p[j > 0 ? c - sx : c], pn = p[j < ny - 1 ? c + sx : c];
-  const double pb = p[k > 0 ? c - sxy : c], pt = p[k < nz - 1 ? c + sxy : c];
-  double nb = (cd & kCodeW) ? cw * pw : 0.0;
-  nb += (cd & kCodeE) ? ce * pe : 0.0;
-  nb += (cd & kCodeS) ? cs * ps : 0.0;
-  nb += (cd & kCodeN) ? cn * pn : 0.0;
-  nb += (cd & kCodeB) ? cb * pb : 0.0;
-  nb += (cd & kCodeT) ? ct * pt : 0.0;
-  const double d = diag_code<double>(cd, cw, ce, cs, cn, cb, ct);
-  const double pc = p[c];
-  const double q = g.dx[i] * g.dy[j] * g.dz[k] * (d * pc - nb);
-  const double a = sc->alpha;
-  x[c] = x[c] + a * pc;
-  r[c] = r[c] - a * q;
-}
-
 // Convergence checks of the multigrid FCG: FFPM_MG_CHECK iterations (default
 // 5, at most the caller's check interval).
 int mg_check_interval(int check_interval) {
@@ -855,11 +820,12 @@ struct MgStats {
 };
 thread_local MgStats g_mg;
 
-// Flexible CG with one stored direction (Notay's FCG(1)), FP64, preconditioned
-// by the FP32 multigrid cycle: z = B r; p = z - (z.A p_old / p_old.A p_old)
-// p_old; alpha = z.r / p.A p; x += alpha p; r -= alpha A p. The direction and
-// SpMV run in the fused PCG direction kernel (p = z + beta p_old, q = A p,
-// p.q). Every `check` iterations the FP64 projected true residual replaces r
+// Conjugate gradients in FP64 preconditioned by the FP32 multigrid cycle:
+// z = B r; beta = z.r / (z.r)_old; p = z + beta p; alpha = z.r / p.A p;
+// x += alpha p; r -= alpha A p. The direction and SpMV run in the fused PCG
+// direction kernel (p = z + beta p_old, q = A p, p.q), the update in the fused
+// PCG update body, which also forms the cycle's first smoothing step. Every
+// `check` iterations the FP64 projected true residual replaces r
 // (src/fpm_stairstep_mpi_3d.f90:297-329) and the max-norm residual is tested
 // against the tolerance, as in the reference's PCG; a breakdown restarts
 // once from p = z, and ten checks without a 1% gain stop at the roundoff
@@ -870,15 +836,15 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   GeomT<float> g32;
   if ((st = make_geom32(f, &g32))) return st;
   const GeomT<double> g = geom_t(f.g);
+  const int cl = cells_per_lane<double>(f.nx), cl32 = cells_per_lane<float>(f.nx);
   Multigrid mg;
-  if ((st = mg_setup(mg, g, g32, f.code))) return st;
+  if ((st = mg_setup(mg, g, g32, f.code, f.tl[cl32], cl32, f.partials))) return st;
   g_mg = MgStats();
   g_mg.levels = mg.levels;
   const int ci = mg_check_interval(check_interval);
-  const int cl = cells_per_lane<double>(f.nx);
-  const dim3 xr_grid((unsigned)((f.nx + 63) / 64), (unsigned)((f.ny + 3) / 4), (unsigned)f.nz);
   double *pbuf[2] = {f.next, f.q};
   int cur = 0, mode = kDirReset;
+  bool first_ready = false;  // lv[0].x holds w D^-1 r for the current r
   double res;
   if ((st = true_residual(f, false, &res))) return st;
   double best = res;
@@ -889,17 +855,15 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     k_chunk_begin<<<1, 64, 0, stream()>>>(f.sc);
     FFPM_LAUNCHED("k_chunk_begin");
     for (int m = 0; m < n; ++m) {
-      if ((st = mg_vcycle(mg, f.r, mode == kDirUpdate ? pbuf[cur] : nullptr, f.z))) return st;
-      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, mg.partials, mg.last_blocks, f.sc);
+      if (!first_ready && (st = mg_first(mg, f.r))) return st;
+      if ((st = mg_vcycle(mg, f.r, f.z))) return st;
+      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 1, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
       const int dst = cur ^ 1;
       if ((st = launch_dir<double>(f, cl, g, f.z, pbuf[cur], pbuf[dst], mode))) return st;
       cur = dst;
-      {
-        TimedLaunch tl("fcg_xr");
-        k_fcg_xr<<<xr_grid, dim3(64, 4), 0, stream()>>>(g, f.code, pbuf[cur], f.phi, f.r, f.sc);
-      }
-      FFPM_LAUNCHED("k_fcg_xr");
+      if ((st = mg_update(mg, f.tl[cl], cl, pbuf[cur], f.phi, f.r, f.sc))) return st;
+      first_ready = true;
       mode = kDirUpdate;
     }
     Scalars h;
@@ -914,11 +878,13 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
       FFPM_LAUNCHED("k_clear_status");
       if ((st = true_residual(f, false, &res))) return st;
+      first_ready = false;
       mode = kDirReset;
       continue;
     }
     if (it % ci == 0 || *sweeps == max_sweeps) {
       if ((st = true_residual(f, false, &res))) return st;
+      first_ready = false;
       g_mg.checks += 1;
       if (res < 0.99 * best) {
         best = res;
