@@ -4,6 +4,7 @@
 // one z plane per grid row, so no index divisions; all loads are
 // unconditional with clamped indices and masked coefficients.
 #include <cstdlib>
+#include <utility>
 
 #include "ffpm_mg.hpp"
 #include "ffpm_pcg_body.hpp"
@@ -20,10 +21,15 @@ inline dim3 cell_grid(int nx, int ny, int nz) {
 }
 
 // Column kernels (one thread per coarse (I, J) walking K): K chunk so that
-// the launch has at least ~4096 wavefronts.
+// the launch has at least FFPM_MG_COLUMN_WAVES (default 16384) wavefronts.
 inline int column_chunk(int cnx, int cny, int cnz) {
+  static const long target = [] {
+    const char *e = std::getenv("FFPM_MG_COLUMN_WAVES");
+    const long v = e ? std::atol(e) : 16384;
+    return v < 1 ? 1L : v;
+  }();
   const long waves_per_plane = (long)((cnx + kMx - 1) / kMx) * ((cny + kMy - 1) / kMy) * 4;
-  long chunks = (4096 + waves_per_plane - 1) / waves_per_plane;
+  long chunks = (target + waves_per_plane - 1) / waves_per_plane;
   if (chunks < 1) chunks = 1;
   if (chunks > cnz) chunks = cnz;
   return (int)((cnz + chunks - 1) / chunks);
@@ -214,38 +220,48 @@ __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
   cload(K0, x0, m0);
   for (int K = K0; K < K1; ++K) {
     cload(K + 1, xp, mp);
+    // the block's four fine rows (2 planes x 2 rows) are loaded together,
+    // updated, and stored together
+    float xv[2][2][2];
+    unsigned cd[2][2][2];
+    bool ok[2][2];
+    size_t c0[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int k = 2 * K + c, j = 2 * J + b;
+        ok[c][b] = k < nz && j < ny;
+        c0[c][b] = (size_t)(2 * I) + fsx * ((size_t)(ok[c][b] ? j : 0) +
+                                            (size_t)ny * (ok[c][b] ? k : 0));
+        if constexpr (VEC) {
+          const float2 t = ok[c][b] ? *reinterpret_cast<const float2 *>(x + c0[c][b])
+                                    : make_float2(0.f, 0.f);
+          const uchar2 u = ok[c][b] ? *reinterpret_cast<const uchar2 *>(code + c0[c][b])
+                                    : make_uchar2(0, 0);
+          xv[c][b][0] = t.x;
+          xv[c][b][1] = t.y;
+          cd[c][b][0] = u.x;
+          cd[c][b][1] = u.y;
+        } else {
+          xv[c][b][0] = ok[c][b] ? x[c0[c][b]] : 0.f;
+          cd[c][b][0] = ok[c][b] ? code[c0[c][b]] : 0u;
+          xv[c][b][1] = ok[c][b] && has_i1 ? x[c0[c][b] + 1] : 0.f;
+          cd[c][b][1] = ok[c][b] && has_i1 ? code[c0[c][b] + 1] : 0u;
+        }
+      }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const int k = 2 * K + c;
-      if (k >= nz) break;
       const float(&xz)[3][3] = c ? xp : xm;
       const float(&mz)[3][3] = c ? mp : mm;
       const float wk = (c ? K + 1 < cnz : K > 0) ? 0.25f : 0.f;
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const int j = 2 * J + b;
-        if (j >= ny) break;
         const int bj = b ? 2 : 0;
         const float wj = vj[bj] ? 0.25f : 0.f;
-        const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)j + (size_t)ny * k);
-        float xv[2];
-        unsigned cd[2];
-        if constexpr (VEC) {
-          const float2 t = *reinterpret_cast<const float2 *>(x + c0);
-          const uchar2 u = *reinterpret_cast<const uchar2 *>(code + c0);
-          xv[0] = t.x;
-          xv[1] = t.y;
-          cd[0] = u.x;
-          cd[1] = u.y;
-        } else {
-          xv[0] = x[c0];
-          cd[0] = code[c0];
-          xv[1] = has_i1 ? x[c0 + 1] : 0.f;
-          cd[1] = has_i1 ? code[c0 + 1] : 0u;
-        }
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-          if (!(cd[a] & kCodeFluid)) continue;
+          if (!(cd[c][b][a] & kCodeFluid)) continue;
           const int ai = a ? 2 : 0;
           const float wi = vi[ai] ? 0.25f : 0.f;
           const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
@@ -254,16 +270,22 @@ __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
           const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
           const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
           const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
-          xv[a] += den > 0.f ? num / den : 0.f;
-        }
-        if constexpr (VEC) {
-          *reinterpret_cast<float2 *>(x + c0) = make_float2(xv[0], xv[1]);
-        } else {
-          x[c0] = xv[0];
-          if (has_i1) x[c0 + 1] = xv[1];
+          xv[c][b][a] += den > 0.f ? num / den : 0.f;
         }
       }
     }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (!ok[c][b]) continue;
+        if constexpr (VEC) {
+          *reinterpret_cast<float2 *>(x + c0[c][b]) = make_float2(xv[c][b][0], xv[c][b][1]);
+        } else {
+          x[c0[c][b]] = xv[c][b][0];
+          if (has_i1) x[c0[c][b] + 1] = xv[c][b][1];
+        }
+      }
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -544,18 +566,19 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
 }
 
 // x += alpha p, r -= alpha A p (the fused PCG update body) and the first
-// smoothing step of the next cycle, x1 = w D^-1 r, stored in FP32; counts the
-// iteration.
+// smoothing step of the next cycle, x1 = w D^-1 r, stored in FP32; with rmax
+// the per-workgroup max |r / V|; counts the iteration.
 template <int CL>
 __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    const double *__restrict__ p,
                                                    const uint8_t *__restrict__ code,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    float *__restrict__ x1, Scalars *sc,
-                                                   double omega) {
+                                                   double omega, double *rmax) {
+  __shared__ double lds[16 * kReduceSlots];
   if (sc->status != 0) return;
   pcg_upd_body<double, CL, float, false>(t, blockIdx.x, gridDim.x, g, p, code, x, r, x1, nullptr,
-                                         sc, nullptr, omega);
+                                         sc, lds, omega, rmax);
   if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
 }
 
@@ -658,10 +681,15 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   mg.tl = tl;
   mg.cl = cl;
   mg.partials = partials;
-  mg.omega = env_float("FFPM_MG_OMEGA", 0.8f);
+  mg.omega = env_float("FFPM_MG_OMEGA", 0.9f);
+  mg.omega_coarse = env_float("FFPM_MG_OMEGA_COARSE", mg.omega);
   mg.scale = env_float("FFPM_MG_SCALE", 0.5f);
   mg.coarse_sweeps = (int)env_float("FFPM_MG_COARSE_SWEEPS", 30.f);
   if (mg.coarse_sweeps < 1) mg.coarse_sweeps = 1;
+  mg.nu = (int)env_float("FFPM_MG_NU", 3.f);
+  mg.nu_coarse = (int)env_float("FFPM_MG_NU_COARSE", 4.f);
+  if (mg.nu < 2) mg.nu = 2;  // the update forms the first sweep, the last one writes z
+  if (mg.nu_coarse < 1) mg.nu_coarse = 1;
   // level extents: halve (ceil) until the level fits one workgroup
   int L = 0;
   int nx = g64.nx, ny = g64.ny, nz = g64.nz;
@@ -757,15 +785,15 @@ int mg_first(const Multigrid &mg, const double *r) {
 }
 
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
-              double *r, Scalars *sc) {
+              double *r, Scalars *sc, double *rmax) {
   const dim3 blk(kTileX, t.ty);
   const double w = (double)mg.omega;
   float *x1 = mg.lv[0].x;
   TimedLaunch tl("mg_update");
   if (cl == 2)
-    k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w);
+    k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax);
   else
-    k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w);
+    k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax);
   FFPM_LAUNCHED("k_mg_upd_t");
   return OK;
 }
@@ -774,11 +802,16 @@ int mg_vcycle(const Multigrid &mg, const double *r, double *z) {
   const dim3 blk(kMx, kMy);
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
-  const float w = mg.omega;
+  const float wc = mg.omega_coarse;
   const int L = mg.levels - 1;  // coarsest index
-  {
+  // cur[l]: the level's current iterate, oth[l]: its ping-pong partner
+  float *cur[kMgMaxLevels], *oth[kMgMaxLevels];
+  cur[0] = f0.x;
+  oth[0] = f0.y;
+  for (int s = 1; s < mg.nu; ++s) {
     TimedLaunch tl("mg_fine_sweep");
-    launch_sweep<false>(mg, f0.x, r, f0.y, nullptr);
+    launch_sweep<false>(mg, cur[0], r, oth[0], nullptr);
+    std::swap(cur[0], oth[0]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
@@ -787,35 +820,46 @@ int mg_vcycle(const Multigrid &mg, const double *r, double *z) {
     const int kc = column_chunk(c.nx, c.ny, c.nz);
     if (f0.nx % 2 == 0)
       k_mgf_restrict_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, f0.y, c.nx, c.ny, c.nz, kc, c.b);
+          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, c.b);
     else
       k_mgf_restrict_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, f0.y, c.nx, c.ny, c.nz, kc, c.b);
+          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, c.b);
   }
   FFPM_LAUNCHED("k_mgf_restrict_z");
-  // down: pre-smoothing (result in y) and restriction
+  // down: pre-smoothing and restriction
   for (int l = 1; l < L; ++l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
     const dim3 gr = cell_grid(v.nx, v.ny, v.nz);
-    k_mgc_jac0<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.gE, v.gN, v.gT, v.b, v.x, w);
-    k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, v.x, v.y, w);
+    cur[l] = v.x;
+    oth[l] = v.y;
+    k_mgc_jac0<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.gE, v.gN, v.gT, v.b, cur[l], wc);
+    for (int s = 1; s < mg.nu_coarse; ++s) {
+      k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l],
+                                            oth[l], wc);
+      std::swap(cur[l], oth[l]);
+    }
     k_mgc_restrict<<<cell_grid(c.nx, c.ny, c.nz), blk, 0, stream()>>>(
-        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, v.y, c.nx, c.ny, c.b);
+        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l], c.nx, c.ny, c.b);
     FFPM_LAUNCHED("k_mgc_restrict");
   }
   {
     const MgLevel &c = mg.lv[L];
+    cur[L] = c.y;
     k_mgc_coarsest<<<1, kMgCoarsestCells, 0, stream()>>>(c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b,
-                                                         c.y, w, mg.coarse_sweeps);
+                                                         cur[L], wc, mg.coarse_sweeps);
     FFPM_LAUNCHED("k_mgc_coarsest");
   }
-  // up: prolongation into y, two sweeps y -> x -> y
+  // up: prolongation into the level's iterate, then post-smoothing
   for (int l = L - 1; l >= 1; --l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
     const dim3 gr = cell_grid(v.nx, v.ny, v.nz);
-    k_mgc_prolong<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.m, v.y, c.nx, c.ny, c.nz, c.y, c.m);
-    k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, v.y, v.x, w);
-    k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, v.x, v.y, w);
+    k_mgc_prolong<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.m, cur[l], c.nx, c.ny, c.nz, cur[l + 1],
+                                            c.m);
+    for (int s = 0; s < mg.nu_coarse; ++s) {
+      k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l],
+                                            oth[l], wc);
+      std::swap(cur[l], oth[l]);
+    }
     FFPM_LAUNCHED("k_mgc_sweep");
   }
   {
@@ -824,20 +868,21 @@ int mg_vcycle(const Multigrid &mg, const double *r, double *z) {
     const int kc = column_chunk(c.nx, c.ny, c.nz);
     if (f0.nx % 2 == 0)
       k_mgf_prolong_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          f0.nx, f0.ny, f0.nz, mg.code, f0.y, c.nx, c.ny, c.nz, c.y, c.m, kc);
+          f0.nx, f0.ny, f0.nz, mg.code, cur[0], c.nx, c.ny, c.nz, cur[1], c.m, kc);
     else
       k_mgf_prolong_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          f0.nx, f0.ny, f0.nz, mg.code, f0.y, c.nx, c.ny, c.nz, c.y, c.m, kc);
+          f0.nx, f0.ny, f0.nz, mg.code, cur[0], c.nx, c.ny, c.nz, cur[1], c.m, kc);
   }
   FFPM_LAUNCHED("k_mgf_prolong_z");
-  {
+  for (int s = 1; s < mg.nu; ++s) {
     TimedLaunch tl("mg_fine_sweep");
-    launch_sweep<false>(mg, f0.y, r, f0.x, nullptr);
+    launch_sweep<false>(mg, cur[0], r, oth[0], nullptr);
+    std::swap(cur[0], oth[0]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
     TimedLaunch tl("mg_fine_last");
-    launch_sweep<true>(mg, f0.x, r, nullptr, z);
+    launch_sweep<true>(mg, cur[0], r, nullptr, z);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   return OK;

@@ -12,14 +12,14 @@
 // null space of the Neumann problem is kept. A cell carries an unknown when
 // its diagonal is positive.
 //
-// Cycle (FP32 arithmetic, FP64 input r and output z): two damped-Jacobi
-// sweeps from zero (the first one fused into the outer update), restriction
-// of the residual by summation over the children, recursion, trilinear
-// cell-centred prolongation with weights renormalised over the coarse cells
-// that carry unknowns, two damped-Jacobi sweeps. The coarsest level (at most
-// 1024 cells) runs a fixed number of sweeps in one workgroup. Every step is a
-// fixed linear map, so the CG around it sees the same preconditioner at
-// every iteration.
+// Cycle (FP32 arithmetic, FP64 input r and output z): nu damped-Jacobi
+// sweeps (weight 0.9) from zero (the first one fused into the outer update),
+// restriction of the residual by summation over the children, recursion,
+// trilinear cell-centred prolongation with weights renormalised over the
+// coarse cells that carry unknowns, nu damped-Jacobi sweeps (nu = 3 on level
+// 0 and 4 on the coarse levels by default, measured fastest at 512x384x256). The coarsest level (at most 1024 cells) runs a fixed
+// number of sweeps in one workgroup. Every step is a fixed linear map, so the
+// CG around it sees the same preconditioner at every iteration.
 #pragma once
 
 #include "ffpm_pcg.hpp"
@@ -45,9 +45,12 @@ struct Multigrid {
   const uint8_t *code = nullptr;
   Tiling tl{};              // level-0 streaming launch shape (CL cells per lane)
   int cl = 1;
-  float omega = 0.8f;
+  float omega = 0.9f;        // damped-Jacobi weight on level 0
+  float omega_coarse = 0.9f; // and on the coarse levels
   float scale = 0.5f;
   int coarse_sweeps = 30;
+  int nu = 3;                // level-0 sweeps before and after the coarse correction
+  int nu_coarse = 4;         // coarse-level sweeps before and after
   double *partials = nullptr;  // z.r partials of the last sweep (tl.blocks)
 };
 
@@ -61,9 +64,10 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 int mg_first(const Multigrid &mg, const double *r);
 
 // x += alpha p, r -= alpha A p and lv[0].x = w D^-1 r (alpha in sc; no-op
-// after a breakdown); counts the iteration. t, cl: FP64 launch shape.
+// after a breakdown); counts the iteration. t, cl: FP64 launch shape. With
+// rmax non-null, one partial max |r / V| per workgroup (t.blocks) goes there.
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
-              double *r, Scalars *sc);
+              double *r, Scalars *sc, double *rmax = nullptr);
 
 // z = B r (the rest of the cycle after the first step, which must be in
 // lv[0].x). Leaves the z.r partial sums (one slot, tl.blocks) in partials.

@@ -248,7 +248,9 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 
 // q = A p (recomputed), x += alpha p, r -= alpha q, z = M^-1 r and z.r.
 // With DOT false (the multigrid-preconditioned CG) z = zscale M^-1 r is stored
-// in ZT (the multigrid's first smoothing step from zero) and no dot is formed.
+// in ZT (the multigrid's first smoothing step from zero) and no dot is formed;
+// with rmax non-null each workgroup also stores max |r / V| over its fluid
+// cells there (an estimate of the max-norm residual).
 template <typename T, int CL, typename ZT = T, bool DOT = true>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const T *__restrict__ p,
@@ -256,12 +258,12 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
                                              T *__restrict__ x, T *__restrict__ r,
                                              ZT *__restrict__ z, double *partials,
                                              const Scalars *sc, double *lds,
-                                             T zscale = T(1)) {
+                                             T zscale = T(1), double *rmax = nullptr) {
   if (sc->status != 0) return;
   const T alpha = (T)sc->alpha;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
-  double s = 0.0;
+  double s = 0.0, rmx = 0.0;
   if (l.active) {
     const int i = l.i, j = l.j;
     const size_t nxy = (size_t)nx * ny;
@@ -329,6 +331,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
           s += (double)zv.v[m] * (double)rv.v[m];
         } else {
           zv.v[m] = (ZT)(zscale * rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m]);
+          if (rmax && d > 0) rmx = fmax(rmx, fabs((double)rv.v[m]) / (double)(dxdy[m] * dzk));
         }
       }
       stv<T, CL>(x, at(k), xv);
@@ -346,6 +349,11 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
     double v[1] = {s};
     block_sum<1>(v, lds);
     store_partials(partials, 1, v, nblocks, block);
+  } else {
+    if (rmax) {
+      block_max(rmx, lds);
+      store_partials(rmax, 1, &rmx, nblocks, block);
+    }
   }
 }
 

@@ -805,11 +805,12 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
   return OK;
 }
 
-// Convergence checks of the multigrid FCG: FFPM_MG_CHECK iterations (default
-// 5, at most the caller's check interval).
+// Convergence-check granularity of the multigrid CG: FFPM_MG_CHECK
+// iterations (default 2, at most the caller's check interval); the FP64 true
+// residual itself is formed only near convergence (see fcg_mg_iterate).
 int mg_check_interval(int check_interval) {
   const char *e = std::getenv("FFPM_MG_CHECK");
-  int c = e ? std::atoi(e) : 5;
+  int c = e ? std::atoi(e) : 2;
   if (check_interval > 0 && check_interval < c) c = check_interval;
   return c < 1 ? 1 : c;
 }
@@ -827,7 +828,8 @@ thread_local MgStats g_mg;
 // PCG update body, which also forms the cycle's first smoothing step. Every
 // `check` iterations the FP64 projected true residual replaces r
 // (src/fpm_stairstep_mpi_3d.f90:297-329) and the max-norm residual is tested
-// against the tolerance, as in the reference's PCG; a breakdown restarts
+// against the tolerance, as in the reference's PCG (formed only when the
+// recursive residual is near the tolerance, see below); a breakdown restarts
 // once from p = z, and ten checks without a 1% gain stop at the roundoff
 // floor.
 int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
@@ -848,7 +850,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   double res;
   if ((st = true_residual(f, false, &res))) return st;
   double best = res;
-  int it = 0, restarts = 0, stalled = 0;
+  int it = 0, restarts = 0, stalled = 0, last_check = 0;
   while (*sweeps < max_sweeps && res > tolerance) {
     int n = ci - it % ci;
     if (max_sweeps - *sweeps < n) n = max_sweeps - *sweeps;
@@ -862,7 +864,16 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       const int dst = cur ^ 1;
       if ((st = launch_dir<double>(f, cl, g, f.z, pbuf[cur], pbuf[dst], mode))) return st;
       cur = dst;
-      if ((st = mg_update(mg, f.tl[cl], cl, pbuf[cur], f.phi, f.r, f.sc))) return st;
+      // the chunk's last update also estimates the max-norm residual from
+      // the recursively updated r
+      const bool last = m == n - 1;
+      if ((st = mg_update(mg, f.tl[cl], cl, pbuf[cur], f.phi, f.r, f.sc,
+                          last ? f.partials : nullptr)))
+        return st;
+      if (last) {
+        k_finish<<<1, 256, 0, stream()>>>(FIN_MAX, 1, f.partials, f.tl[cl].blocks, f.sc);
+        FFPM_LAUNCHED("k_finish");
+      }
       first_ready = true;
       mode = kDirUpdate;
     }
@@ -882,7 +893,12 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       mode = kDirReset;
       continue;
     }
-    if (it % ci == 0 || *sweeps == max_sweeps) {
+    // The FP64 true residual (the stopping test) is formed when the
+    // recursive estimate is within 4x of the tolerance, at the iteration cap,
+    // and at least every 20 iterations (which also refreshes r).
+    const bool near = !(h.res > 4.0 * tolerance);
+    if ((it % ci == 0 && (near || it - last_check >= 20)) || *sweeps == max_sweeps) {
+      last_check = it;
       if ((st = true_residual(f, false, &res))) return st;
       first_ready = false;
       g_mg.checks += 1;

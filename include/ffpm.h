@@ -59,8 +59,8 @@ enum {
  * stopping on the same FP64 max-norm residual; sweeps then counts inner
  * iterations. FCG_MG: FP64 flexible CG preconditioned by an FP32 geometric
  * multigrid V-cycle on the same operator, stopping on the same FP64 max-norm
- * residual (checked every 5 iterations or every check_interval if smaller);
- * sweeps counts FCG iterations. */
+ * residual (tested every 2 iterations, or every check_interval if smaller,
+ * once a recursive estimate is near the tolerance); sweeps counts iterations. */
 enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1, FFPM_METHOD_PCG_MIXED = 2,
        FFPM_METHOD_FCG = 3 /* only through ffpm_solve_stairstep_fcg_dev */,
        FFPM_METHOD_FCG_MG = 4 };
