@@ -472,7 +472,8 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
 // one row and walks its z range; plane k+1's loads are issued before plane k
 // is computed; west/east neighbours by wave shuffle):
 //   out = (1 - w) a + (w / d) (r / V + sum c a_nb), 0 where d = 0.
-// LAST writes out as z in FP64 and the partial sums z.r (one slot).
+// LAST writes out as z in FP64 and the partial sums z.r and z.q (two slots;
+// q = A p_old from the previous update, FP32, or z.q = 0 when q is null).
 template <int CL, bool LAST>
 __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const float *__restrict__ a,
@@ -480,11 +481,12 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const uint8_t *__restrict__ code,
                                                      float *__restrict__ b,
                                                      double *__restrict__ z, double *partials,
-                                                     float omega) {
+                                                     float omega,
+                                                     const float *__restrict__ q) {
   __shared__ double lds[16 * kReduceSlots];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, blockIdx.x);
-  double s = 0.0;
+  double s = 0.0, sq = 0.0;
   if (l.active) {
     using V = Vec<float, CL>;
     using VD = Vec<double, CL>;
@@ -529,8 +531,11 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
       const float vw = lane == 0 ? u.pl.w.a : from_w;
       const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
       const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
-      V bv;
+      V bv, qz;
       VD zv;
+      if constexpr (LAST) {
+        if (q) qz = ldv<float, CL>(q, at(k));
+      }
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
         const float west = m == 0 ? (hw ? vw : 0.f) : a_cur.v[m - 1];
@@ -544,6 +549,7 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
         if constexpr (LAST) {
           zv.v[m] = (double)o;
           s += zv.v[m] * u.r.v[m];
+          if (q) sq += zv.v[m] * (double)qz.v[m];
         } else {
           bv.v[m] = o;
         }
@@ -559,26 +565,27 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
     }
   }
   if constexpr (LAST) {
-    double v[1] = {s};
-    block_sum<1>(v, lds);
-    store_partials(partials, 1, v, gridDim.x, blockIdx.x);
+    double v[2] = {s, sq};
+    block_sum<2>(v, lds);
+    store_partials(partials, 2, v, gridDim.x, blockIdx.x);
   }
 }
 
-// x += alpha p, r -= alpha A p (the fused PCG update body) and the first
-// smoothing step of the next cycle, x1 = w D^-1 r, stored in FP32; with rmax
-// the per-workgroup max |r / V|; counts the iteration.
+// x += alpha p, r -= alpha A p (the fused PCG update body), the first
+// smoothing step of the next cycle, x1 = w D^-1 r, and q = A p, both stored in
+// FP32; with rmax the per-workgroup max |r / V|; counts the iteration.
 template <int CL>
 __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    const double *__restrict__ p,
                                                    const uint8_t *__restrict__ code,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    float *__restrict__ x1, Scalars *sc,
-                                                   double omega, double *rmax) {
+                                                   double omega, double *rmax,
+                                                   float *__restrict__ q32) {
   __shared__ double lds[16 * kReduceSlots];
   if (sc->status != 0) return;
   pcg_upd_body<double, CL, float, false>(t, blockIdx.x, gridDim.x, g, p, code, x, r, x1, nullptr,
-                                         sc, lds, omega, rmax);
+                                         sc, lds, omega, rmax, q32);
   if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
 }
 
@@ -713,9 +720,9 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     return ERR_SHAPE;
   }
   mg.levels = L + 1;
-  // one arena: level-0 x, y; per coarse level gE gN gT m x y b
+  // one arena: level-0 x, y, q; per coarse level gE gN gT m x y b
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  size_t bytes = 2 * up(sizeof(float) * mg.lv[0].n);
+  size_t bytes = 3 * up(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) bytes += 7 * up(sizeof(float) * mg.lv[l].n);
   int st = 0;
   char *base = (char *)workspace().get(S_MG, bytes, &st);
@@ -727,6 +734,7 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   };
   mg.lv[0].x = (float *)take(sizeof(float) * mg.lv[0].n);
   mg.lv[0].y = (float *)take(sizeof(float) * mg.lv[0].n);
+  mg.q32 = (float *)take(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
     const size_t b = sizeof(float) * v.n;
@@ -760,18 +768,19 @@ namespace {
 
 // Level-0 streaming sweep launch for the multigrid's cells-per-lane choice.
 template <bool LAST>
-void launch_sweep(const Multigrid &mg, const float *a, const double *r, float *b, double *z) {
+void launch_sweep(const Multigrid &mg, const float *a, const double *r, float *b, double *z,
+                  const float *q = nullptr) {
   const dim3 blk(kTileX, mg.tl.ty);
   const float w = mg.omega;
   if (mg.cl == 4)
     k_mg_sweep_t<4, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
-                                                              mg.partials, w);
+                                                              mg.partials, w, q);
   else if (mg.cl == 2)
     k_mg_sweep_t<2, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
-                                                              mg.partials, w);
+                                                              mg.partials, w, q);
   else
     k_mg_sweep_t<1, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
-                                                              mg.partials, w);
+                                                              mg.partials, w, q);
 }
 
 }  // namespace
@@ -791,14 +800,16 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, dou
   float *x1 = mg.lv[0].x;
   TimedLaunch tl("mg_update");
   if (cl == 2)
-    k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax);
+    k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
+                                                  mg.q32);
   else
-    k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax);
+    k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
+                                                  mg.q32);
   FFPM_LAUNCHED("k_mg_upd_t");
   return OK;
 }
 
-int mg_vcycle(const Multigrid &mg, const double *r, double *z) {
+int mg_vcycle(const Multigrid &mg, const double *r, double *z, bool with_q) {
   const dim3 blk(kMx, kMy);
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
@@ -882,7 +893,7 @@ int mg_vcycle(const Multigrid &mg, const double *r, double *z) {
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
     TimedLaunch tl("mg_fine_last");
-    launch_sweep<true>(mg, cur[0], r, nullptr, z);
+    launch_sweep<true>(mg, cur[0], r, nullptr, z, with_q ? mg.q32 : nullptr);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   return OK;

@@ -51,7 +51,8 @@ struct Multigrid {
   int coarse_sweeps = 30;
   int nu = 3;                // level-0 sweeps before and after the coarse correction
   int nu_coarse = 4;         // coarse-level sweeps before and after
-  double *partials = nullptr;  // z.r partials of the last sweep (tl.blocks)
+  double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
+  float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
 };
 
 // Builds the hierarchy for the frame with FP64 geometry g64, its FP32 copy
@@ -63,14 +64,16 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 // First smoothing step from zero on r: lv[0].x = w D^-1 r.
 int mg_first(const Multigrid &mg, const double *r);
 
-// x += alpha p, r -= alpha A p and lv[0].x = w D^-1 r (alpha in sc; no-op
-// after a breakdown); counts the iteration. t, cl: FP64 launch shape. With
-// rmax non-null, one partial max |r / V| per workgroup (t.blocks) goes there.
+// x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r and q32 = A p (alpha in
+// sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
+// shape. With rmax non-null, one partial max |r / V| per workgroup (t.blocks)
+// goes there.
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
               double *r, Scalars *sc, double *rmax = nullptr);
 
 // z = B r (the rest of the cycle after the first step, which must be in
-// lv[0].x). Leaves the z.r partial sums (one slot, tl.blocks) in partials.
-int mg_vcycle(const Multigrid &mg, const double *r, double *z);
+// lv[0].x). Leaves two slots of partial sums (tl.blocks each) in partials:
+// z.r, and z.q32 when with_q (else 0).
+int mg_vcycle(const Multigrid &mg, const double *r, double *z, bool with_q);
 
 }  // namespace ffpm

@@ -821,11 +821,15 @@ struct MgStats {
 };
 thread_local MgStats g_mg;
 
-// Conjugate gradients in FP64 preconditioned by the FP32 multigrid cycle:
-// z = B r; beta = z.r / (z.r)_old; p = z + beta p; alpha = z.r / p.A p;
-// x += alpha p; r -= alpha A p. The direction and SpMV run in the fused PCG
-// direction kernel (p = z + beta p_old, q = A p, p.q), the update in the fused
-// PCG update body, which also forms the cycle's first smoothing step. Every
+// Flexible conjugate gradients with one stored direction (Notay's FCG(1)) in
+// FP64, preconditioned by the FP32 multigrid cycle: z = B r;
+// beta = -z.A p_old / p_old.A p_old; p = z + beta p_old; alpha = z.r / p.A p;
+// x += alpha p; r -= alpha A p. (The cycle is not exactly symmetric: the
+// Fletcher-Reeves beta of plain PCG stagnates with it on strongly stretched
+// grids.) The direction and SpMV run in the fused PCG direction kernel
+// (p = z + beta p_old, q = A p, p.q), the update in the fused PCG update body,
+// which also forms the cycle's first smoothing step and stores q = A p in FP32
+// for the next beta. Every
 // `check` iterations the FP64 projected true residual replaces r
 // (src/fpm_stairstep_mpi_3d.f90:297-329) and the max-norm residual is tested
 // against the tolerance, as in the reference's PCG (formed only when the
@@ -858,8 +862,8 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     FFPM_LAUNCHED("k_chunk_begin");
     for (int m = 0; m < n; ++m) {
       if (!first_ready && (st = mg_first(mg, f.r))) return st;
-      if ((st = mg_vcycle(mg, f.r, f.z))) return st;
-      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 1, f.partials, mg.tl.blocks, f.sc);
+      if ((st = mg_vcycle(mg, f.r, f.z, mode == kDirUpdate))) return st;
+      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
       const int dst = cur ^ 1;
       if ((st = launch_dir<double>(f, cl, g, f.z, pbuf[cur], pbuf[dst], mode))) return st;

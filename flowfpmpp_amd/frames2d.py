@@ -12,6 +12,7 @@ omega(nx,ny), vif_x(nx,ny), vif_y(nx,ny), phi_x(nx,ny), phi_y(nx,ny),
 solid(nx,ny)`` (solid as 1.0/0.0).
 
     python flowfpmpp_amd/frames2d.py foil <frame.bin> [--out fields.bin] [--tol 1e-10]
+                                     [--method pcg|fcg-mg]
     python flowfpmpp_amd/frames2d.py cylinder [--nx 360 --ny 240] [--out fields.bin]
 """
 from __future__ import annotations
@@ -92,10 +93,11 @@ def write_fields(path, xc, yc, omega, vif, phi, solid):
 
 
 def _solve_2d(xf, yf, xc, yc, dx, dy, zc, dz, solid, u, v, kin, ring, dt, nu, tolerance,
-              max_sweeps, check_interval, slip_body, log):
+              max_sweeps, check_interval, slip_body, log, method="pcg"):
     """phi_x and phi_y of a two-cell-deep stair-step frame (zero z-face data)
-    by PCG on the GPU, each followed by the force partition of its direction.
-    ``rhs`` is shared by the two directions as in the reference's drivers."""
+    by PCG (or another solve method) on the GPU, each followed by the force
+    partition of its direction. ``rhs`` is shared by the two directions as in
+    the reference's drivers."""
     from flowfpmpp_amd import partition as FP
     from flowfpmpp_amd import stairstep as S
     from flowfpmpp_amd.device import DeviceArray, synchronize
@@ -124,7 +126,7 @@ def _solve_2d(xf, yf, xc, yc, dx, dy, zc, dz, solid, u, v, kin, ring, dt, nu, to
         corr, _ = S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, body, *faces, *axes[3:])
         sw, res = S.solve_stairstep_poisson_3d(phi, d_rhs, d_solid, body, *faces, *axes,
                                                tolerance, max_sweeps, check_interval,
-                                               method="pcg")
+                                               method=method)
         phi2 = np.asfortranarray(phi.download()[1:nx + 1, 1:ny + 1, 0])
         terms, vif, om = FP.fpm_partition_terms_2d(T(phi2), body[:2], d_u, d_v, sol2, d_xf,
                                                    d_yf, kin, d_ring, dt, nu, slip_body)
@@ -136,21 +138,24 @@ def _solve_2d(xf, yf, xc, yc, dx, dy, zc, dz, solid, u, v, kin, ring, dt, nu, to
         out["phi"].append(phi2)
         out["omega"] = om.download()
         if log:
-            log(f"direction {d + 1}: PCG iterations {sw}, residual {res:.3e}")
+            log(f"direction {d + 1}: {method} iterations {sw}, residual {res:.3e}")
     synchronize()
     out["seconds"] = time.perf_counter() - t0
     out["xc"], out["yc"] = np.asarray(xc)[1:nx + 1], np.asarray(yc)[1:ny + 1]
     return out
 
 
-def solve_frame(fr: Frame2D, tolerance: float = 1e-10, re: float = 1.0e4, log=None):
+def solve_frame(fr: Frame2D, tolerance: float = 1e-10, re: float = 1.0e4, log=None,
+                method: str = "pcg"):
     """The real-frame driver (examples/flapping_foil_fpm.f90:55-86): the foil
     as a stair-step body in a two-cell-deep frame of unit span (dz = 0.5, zero
     z-face derivatives), halo centres mirrored across the outer boundary,
     phi_x and phi_y by PCG (check interval 50, cap 2,000,000), then the force
     partition of each direction with nu = 1/Re. Returns a dict with
     iterations, residuals, terms (4 x 2: KINE, VIF, VIS, BOUND by direction),
-    the F_VIF densities, phi, omega and the cell centres."""
+    the F_VIF densities, phi, omega and the cell centres. method "fcg-mg"
+    (the multigrid-preconditioned CG) replaces the reference's PCG; both stop
+    on the same FP64 max-norm residual."""
     nx, ny = fr.nx, fr.ny
     dx, dy = np.diff(fr.xf), np.diff(fr.yf)
     xc = np.empty(nx + 2)
@@ -160,7 +165,8 @@ def solve_frame(fr: Frame2D, tolerance: float = 1e-10, re: float = 1.0e4, log=No
     xc[0], xc[nx + 1] = xc[1] - dx[0], xc[nx] + dx[-1]
     yc[0], yc[ny + 1] = yc[1] - dy[0], yc[ny] + dy[-1]
     return _solve_2d(fr.xf, fr.yf, xc, yc, dx, dy, [0.25, 0.75], [0.5, 0.5], fr.solid, fr.u,
-                     fr.v, fr.kin, fr.ring, fr.dt, 1.0 / re, tolerance, 2000000, 50, False, log)
+                     fr.v, fr.kin, fr.ring, fr.dt, 1.0 / re, tolerance, 2000000, 50, False, log,
+                     method)
 
 
 def cylinder_vortices(gnx: int = 360, gny: int = 240):
@@ -199,14 +205,14 @@ def cylinder_vortices(gnx: int = 360, gny: int = 240):
     return xf, yf, xc, yc, dx, dy, zc, dz, solid, u, v, om, ring
 
 
-def solve_cylinder_vortices(gnx: int = 360, gny: int = 240, log=None):
+def solve_cylinder_vortices(gnx: int = 360, gny: int = 240, log=None, method: str = "pcg"):
     """The demo driver (examples/demo_cylinder_vortices.f90:80-95): PCG to
     1e-9 (check interval 500, cap 5,000,000) and the partition with the body
     faces as slip faces, zero kinematics, dt = 1 and nu = 0. The vorticity
     written out is the prescribed one, as in the reference."""
     xf, yf, xc, yc, dx, dy, zc, dz, solid, u, v, om, ring = cylinder_vortices(gnx, gny)
     r = _solve_2d(xf, yf, xc, yc, dx, dy, zc, dz, solid, u, v, np.zeros(6), ring, 1.0, 0.0,
-                  1e-9, 5000000, 500, True, log)
+                  1e-9, 5000000, 500, True, log, method)
     r["omega"] = om
     r["solid"] = solid
     return r
@@ -224,6 +230,8 @@ def main():  # pragma: no cover - command-line driver
     c.add_argument("--nx", type=int, default=360)
     c.add_argument("--ny", type=int, default=240)
     c.add_argument("--out", default=None)
+    for sp in (f, c):
+        sp.add_argument("--method", default="pcg", choices=["pcg", "fcg-mg", "pcg-mixed"])
     args = ap.parse_args()
 
     def log(m):
@@ -232,17 +240,18 @@ def main():  # pragma: no cover - command-line driver
     if args.case == "foil":
         fr = read_ffpm2d02(args.frame)
         log(f"frame: {fr.nx} x {fr.ny} cells, {int(fr.solid.sum())} solid, t = {fr.time:.3f}")
-        r = solve_frame(fr, args.tol, log=log)
+        r = solve_frame(fr, args.tol, log=log, method=args.method)
         solid, grid = fr.solid, [fr.nx, fr.ny]
     else:
-        r = solve_cylinder_vortices(args.nx, args.ny, log=log)
+        r = solve_cylinder_vortices(args.nx, args.ny, log=log, method=args.method)
         solid, grid = r["solid"], [args.nx, args.ny]
     for q, nm in enumerate(["F_KINE", "F_VIF", "F_VIS", "F_BOUND"]):
         log(f"  {nm:8s} {r['terms'][q, 0]: .5e} {r['terms'][q, 1]: .5e}")
     log(f"  sum      {r['terms'][:, 0].sum(): .5e} {r['terms'][:, 1].sum(): .5e}")
     if args.out:
         write_fields(args.out, r["xc"], r["yc"], r["omega"], r["vif"], r["phi"], solid)
-    print(json.dumps({"case": args.case, "grid": grid, "iterations": r["iterations"],
+    print(json.dumps({"case": args.case, "method": args.method, "grid": grid,
+                      "iterations": r["iterations"],
                       "residuals": r["residuals"], "seconds": round(r["seconds"], 3),
                       "terms_x": r["terms"][:, 0].tolist(), "terms_y": r["terms"][:, 1].tolist(),
                       "sum": r["terms"].sum(axis=0).tolist()}), flush=True)

@@ -144,3 +144,48 @@ def test_fcg_mg_two_cell_deep_frame():
         print(f"direction {d}: PCG {b[1]}, fcg-mg {a[1]} iterations; rel diff {rel:.2e}",
               flush=True)
         assert a[2] <= 1e-10 and rel <= 1e-8 and a[1] <= 100
+
+
+def test_fcg_mg_strongly_stretched_frame():
+    """Cells with aspect ratios up to 50 (a thin body on tanh-stretched axes,
+    width ratio 40 and 50, two cells deep, as in the real flapping-foil
+    frame): the cycle is far from an exact inverse there and not exactly
+    symmetric, and the flexible-CG coefficient keeps the iteration converging
+    (the Fletcher-Reeves coefficient stagnates at 2e-9 / 4e-8 after 600
+    iterations on this frame in a CPU prototype of the same cycle)."""
+    from flowfpmpp_amd.grids import tanh_axis
+    print("case: fcg-mg on a 240x300x2 frame stretched 40:1 / 50:1", flush=True)
+    nx, ny, nz = 240, 300, 2
+    xf, _, dx = tanh_axis(nx, 0.0, 10.0, ratio=40.0)
+    yf, _, dy = tanh_axis(ny, 0.0, 13.0, ratio=50.0)
+    xc = np.empty(nx + 2)
+    yc = np.empty(ny + 2)
+    xc[1:-1] = 0.5 * (xf[:-1] + xf[1:])
+    yc[1:-1] = 0.5 * (yf[:-1] + yf[1:])
+    xc[0], xc[-1] = xc[1] - dx[0], xc[-2] + dx[-1]
+    yc[0], yc[-1] = yc[1] - dy[0], yc[-2] + dy[-1]
+    X, Y = np.meshgrid(xc[1:-1], yc[1:-1], indexing="ij")
+    solid = ((X - 5) / 0.5) ** 2 + ((Y - 6.5) / 0.06) ** 2 < 1
+    dz, zc = np.array([0.5, 0.5]), np.array([0.25, 0.75])
+    solid3 = np.zeros((nx + 2, ny + 2, nz), dtype=np.int32, order="F")
+    for k in range(nz):
+        solid3[1:-1, 1:-1, k] = solid
+    for d in range(2):
+        bg = np.zeros(3)
+        bg[d] = 1.0
+        out = {}
+        for method in ("pcg", "fcg-mg"):
+            rhs = np.zeros((nx, ny, nz), order="F")
+            faces = [np.zeros(s, order="F") for s in
+                     ((ny, nz), (ny, nz), (nx, nz), (nx, nz), (nx, ny), (nx, ny))]
+            S.enforce_stairstep_compatibility_3d(rhs, solid3, bg, *faces, dx, dy, dz)
+            ph = np.zeros((nx + 2, ny + 2, nz), order="F")
+            sw, res = S.solve_stairstep_poisson_3d(ph, rhs, solid3, bg, *faces, xc, yc, zc, dx,
+                                                   dy, dz, 1e-10, 200000, 50, method=method)
+            out[method] = (ph, sw, res)
+        a, b = out["fcg-mg"], out["pcg"]
+        rel = np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()
+        print(f"direction {d}: PCG {b[1]} iterations ({b[2]:.2e}), fcg-mg {a[1]} ({a[2]:.2e}); "
+              f"rel diff {rel:.2e}", flush=True)
+        assert a[2] <= 1e-10 and b[2] <= 1e-10
+        assert a[1] <= 400 and rel <= 1e-8
