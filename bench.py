@@ -131,9 +131,12 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=25.0):
 
 def bench_batch(args):
     """64 uneven frames (grids from 64^3 to 160x128x96, 1-3 bodies each) x 3
-    force directions = 192 FP64 PCG solves to the tolerance in one batched
-    solve (shared operator per frame, per-solve masks, HIP-graph chunks,
-    autotuned launch shape). A step solves all 192 from phi = 0."""
+    force directions = 192 PCG solves to the tolerance in one batched solve
+    (shared operator per frame, per-solve masks, HIP-graph chunks, autotuned
+    launch shape): FP64 with --method pcg, FP32 inner / FP64 outer otherwise.
+    A step solves all 192 from phi = 0. With --method fcg-mg the same 192
+    solves are also run one after another with the multigrid CG
+    ("fcg_mg_sequential")."""
     from flowfpmpp_amd import stairstep as S
     from flowfpmpp_amd import workloads as W
     from flowfpmpp_amd.device import DeviceArray, synchronize
@@ -155,7 +158,7 @@ def bench_batch(args):
     cells = sum(p.cells for p, _ in frames) * 3
     synchronize()
     log(f"64 frames x 3 directions resident: {cells} cells, {time.perf_counter() - t_gen:.1f} s")
-    bmethod = args.method if args.method in ("pcg", "pcg-mixed") else "pcg"
+    bmethod = args.method if args.method in ("pcg", "pcg-mixed") else "pcg-mixed"
     rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20, method=bmethod)
     log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
 
@@ -197,6 +200,35 @@ def bench_batch(args):
         "converged": bool((res <= args.tol).all()),
         "autotune_ms_per_iteration": round(ms_it, 4),
     }
+    if args.method == "fcg-mg":
+        for _ in range(max(1, args.warmup)):
+            for q in probs[:3]:
+                q.phi.zero_()
+                S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
+                                             *q.axes, args.tol, args.max_iters, args.check,
+                                             method="fcg-mg")
+        synchronize()
+        per, its, rs = [], [], []
+        t0 = time.perf_counter()
+        for q in probs:
+            t1 = time.perf_counter()
+            q.phi.zero_()
+            sw, r = S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
+                                                 *q.axes, args.tol, args.max_iters, args.check,
+                                                 method="fcg-mg")
+            per.append((time.perf_counter() - t1) * 1e3)
+            its.append(sw)
+            rs.append(r)
+        synchronize()
+        seq = time.perf_counter() - t0
+        out["fcg_mg_sequential"] = {
+            "ms": round(seq * 1e3, 3), "solves_per_s": round(192 / seq, 3),
+            "solve_ms_p50": round(float(np.percentile(per, 50)), 3),
+            "solve_ms_p99": round(float(np.percentile(per, 99)), 3),
+            "iterations_min_max": [int(min(its)), int(max(its))],
+            "max_residual": float(max(rs)), "converged": bool(max(rs) <= args.tol)}
+        log(f"fcg-mg one after another: {seq * 1e3:.1f} ms for 192 solves, iterations "
+            f"{min(its)}-{max(its)}")
     if args.sequential:
         t0 = time.perf_counter()
         for (p, dirs) in frames:
