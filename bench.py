@@ -57,11 +57,11 @@ BYTES_PER_CELL = {
     "terms": 124,        # phi, solid(4), 9 fields; write 5 outputs
     # CG with the FP32 multigrid cycle (level-0 kernels; the coarse levels add
     # 1/8, 1/64, ... of their own small traffic)
-    "mg_update": 49,         # read p, code; read/write phi, r; write x1 = w D^-1 r, q (FP32)
-    "mg_fine_sweep": 17,     # x (FP32), r, code; write x (FP32)
-    "mg_fine_restrict": 13,  # x, r, code per fine cell (coarse writes 1/8 of 4 B)
+    "mg_update": 53,         # read p, code; read/write phi, r; write x1, q, s = r/V (FP32)
+    "mg_fine_sweep": 13,     # x, s (FP32), code; write x (FP32)
+    "mg_fine_restrict": 9,   # x, s, code per fine cell (coarse writes 1/8 of 4 B)
     "mg_fine_prolong": 9,    # x read/write, code (coarse reads 1/8)
-    "mg_fine_last": 25,      # x, r, code, q (FP32); write z (FP64)
+    "mg_fine_last": 21,      # x, s, q (FP32), code; write z (FP64)
 }
 
 

@@ -57,21 +57,25 @@ __device__ __forceinline__ Idx cell_idx(int nx, int ny) {
 }
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
-// x = omega D^-1 r (the first sweep from zero); D = V diag.
+// The cycle's source s = r / V in FP32 (0 where d = 0) and the first sweep
+// from zero, x = omega D^-1 r = omega s / d; D = V diag.
 __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t *__restrict__ code,
                                                   const double *__restrict__ r,
-                                                  float *__restrict__ x, float omega) {
+                                                  float *__restrict__ x, float *__restrict__ s32,
+                                                  float omega) {
   const Idx t = cell_idx(g.nx, g.ny);
   if (!t.in) return;
   const unsigned cd = code[t.c];
   const float d = diag_code<float>(cd, g.cw[t.i], g.ce[t.i], g.cs[t.j], g.cn[t.j], g.cb[t.k],
                                    g.ct[t.k]);
   const float V = g.dx[t.i] * g.dy[t.j] * g.dz[t.k];
-  x[t.c] = d > 0.f ? omega * ((float)r[t.c] / V) / d : 0.f;
+  const float sv = d > 0.f ? (float)r[t.c] / V : 0.f;
+  s32[t.c] = sv;
+  x[t.c] = d > 0.f ? omega * sv / d : 0.f;
 }
 
-// Coarse right-hand side: the level-0 residual r - A x summed over the
-// children of each level-1 cell. One thread per level-1 (I, J) column walks a
+// Coarse right-hand side: the level-0 residual r - A x = V (s - (d x - sum c
+// x_nb)) summed over the children of each level-1 cell. One thread per level-1 (I, J) column walks a
 // range of K: each fine plane's 2x2 block of x is loaded once and kept for
 // the planes above and below; only the in-plane halo comes from other
 // threads' cells (cache hits). VEC (even nx): the two x-adjacent children of
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t 
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
                                                         const uint8_t *__restrict__ code,
-                                                        const double *__restrict__ r,
+                                                        const float *__restrict__ src,
                                                         const float *__restrict__ x, int cnx,
                                                         int cny, int cnz, int kchunk,
                                                         float *__restrict__ bc) {
@@ -128,19 +132,19 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
       if (!vb[b]) continue;
       const size_t c0 = at(i0, jb[b], k);
       unsigned cds[2];
-      double rs[2];
+      float ss[2];
       if constexpr (VEC) {
         const uchar2 cd2 = *reinterpret_cast<const uchar2 *>(code + c0);
-        const double2 r2 = *reinterpret_cast<const double2 *>(r + c0);
+        const float2 s2 = *reinterpret_cast<const float2 *>(src + c0);
         cds[0] = cd2.x;
         cds[1] = cd2.y;
-        rs[0] = r2.x;
-        rs[1] = r2.y;
+        ss[0] = s2.x;
+        ss[1] = s2.y;
       } else {
         cds[0] = code[c0];
-        rs[0] = r[c0];
+        ss[0] = src[c0];
         cds[1] = va[1] ? code[c0 + 1] : 0u;
-        rs[1] = va[1] ? r[c0 + 1] : 0.0;
+        ss[1] = va[1] ? src[c0 + 1] : 0.f;
       }
       const float xw = hw ? x[c0 - 1] : 0.f, xe = he ? x[c0 + 2] : 0.f;
 #pragma unroll
@@ -159,7 +163,7 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
         nb += (cd & kCodeB) ? cb * dn[b][a] : 0.f;
         nb += (cd & kCodeT) ? ct * up[b][a] : 0.f;
         const float d = diag_code<float>(cd, cw[a], ce[a], cs[b], cn[b], cb, ct);
-        s += (float)rs[a] - dxa[a] * dyb[b] * dzk * (d * c[b][a] - nb);
+        s += dxa[a] * dyb[b] * dzk * (ss[a] - (d * c[b][a] - nb));
       }
     }
     return s;
@@ -471,13 +475,14 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
 // reference-Jacobi kernel k_jacobi_t: each lane owns CL x-adjacent cells of
 // one row and walks its z range; plane k+1's loads are issued before plane k
 // is computed; west/east neighbours by wave shuffle):
-//   out = (1 - w) a + (w / d) (r / V + sum c a_nb), 0 where d = 0.
-// LAST writes out as z in FP64 and the partial sums z.r and z.q (two slots;
-// q = A p_old from the previous update, FP32, or z.q = 0 when q is null).
+//   out = (1 - w) a + (w / d) (s + sum c a_nb), 0 where d = 0, s = r / V.
+// LAST writes out as z in FP64 and the partial sums z.r = z.(s V) and z.q (two
+// slots; q = A p_old from the previous update, FP32, or z.q = 0 when q is
+// null).
 template <int CL, bool LAST>
 __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const float *__restrict__ a,
-                                                     const double *__restrict__ r,
+                                                     const float *__restrict__ r,
                                                      const uint8_t *__restrict__ code,
                                                      float *__restrict__ b,
                                                      double *__restrict__ z, double *partials,
@@ -490,6 +495,7 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
   if (l.active) {
     using V = Vec<float, CL>;
     using VD = Vec<double, CL>;
+    using VS = Vec<float, CL>;
     const int i = l.i, j = l.j;
     const size_t nxy = (size_t)nx * ny;
     float cw[CL], ce[CL], dxdy[CL];
@@ -512,7 +518,7 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
     auto at = [&](int k) { return base + nxy * (size_t)k; };
     struct Set {
       V a_next;
-      VD r;
+      VS r;
       Plane<kDirCurrent, float, CL> pl;
     };
     Set sa, sb;
@@ -520,12 +526,12 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
     V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
     sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
     sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
-    sa.r = ldv<double, CL>(r, at(k0));
+    sa.r = ldv<float, CL>(r, at(k0));
     auto stage = [&](int k, const Set &u, Set &ld) {
       const int kn = min(k + 1, k1 - 1);
       ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
       ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
-      ld.r = ldv<double, CL>(r, at(kn));
+      ld.r = ldv<float, CL>(r, at(kn));
       const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
       const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
       const float vw = lane == 0 ? u.pl.w.a : from_w;
@@ -542,13 +548,13 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
         const float east = m == CL - 1 ? (he ? ve : 0.f) : a_cur.v[m + 1];
         const float south = hs ? u.pl.s.a.v[m] : 0.f, north = hn ? u.pl.n.a.v[m] : 0.f;
         const float d = diag_code<float>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
-        const float src = (float)u.r.v[m] / (dxdy[m] * dzk);
+        const float src = u.r.v[m];
         const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
                          cb * a_prev.v[m] + ct * u.a_next.v[m];
         const float o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
         if constexpr (LAST) {
           zv.v[m] = (double)o;
-          s += zv.v[m] * u.r.v[m];
+          s += zv.v[m] * ((double)u.r.v[m] * (double)(dxdy[m] * dzk));
           if (q) sq += zv.v[m] * (double)qz.v[m];
         } else {
           bv.v[m] = o;
@@ -572,8 +578,9 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
 }
 
 // x += alpha p, r -= alpha A p (the fused PCG update body), the first
-// smoothing step of the next cycle, x1 = w D^-1 r, and q = A p, both stored in
-// FP32; with rmax the per-workgroup max |r / V|; counts the iteration.
+// smoothing step of the next cycle, x1 = w D^-1 r, q = A p and the cycle's
+// source s = r / V, stored in FP32; with rmax the per-workgroup max |r / V|;
+// counts the iteration.
 template <int CL>
 __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    const double *__restrict__ p,
@@ -581,11 +588,12 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    float *__restrict__ x1, Scalars *sc,
                                                    double omega, double *rmax,
-                                                   float *__restrict__ q32) {
+                                                   float *__restrict__ q32,
+                                                   float *__restrict__ s32) {
   __shared__ double lds[16 * kReduceSlots];
   if (sc->status != 0) return;
   pcg_upd_body<double, CL, float, false>(t, blockIdx.x, gridDim.x, g, p, code, x, r, x1, nullptr,
-                                         sc, lds, omega, rmax, q32);
+                                         sc, lds, omega, rmax, q32, s32);
   if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
 }
 
@@ -720,9 +728,9 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     return ERR_SHAPE;
   }
   mg.levels = L + 1;
-  // one arena: level-0 x, y, q; per coarse level gE gN gT m x y b
+  // one arena: level-0 x, y, q, s; per coarse level gE gN gT m x y b
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  size_t bytes = 3 * up(sizeof(float) * mg.lv[0].n);
+  size_t bytes = 4 * up(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) bytes += 7 * up(sizeof(float) * mg.lv[l].n);
   int st = 0;
   char *base = (char *)workspace().get(S_MG, bytes, &st);
@@ -735,6 +743,7 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   mg.lv[0].x = (float *)take(sizeof(float) * mg.lv[0].n);
   mg.lv[0].y = (float *)take(sizeof(float) * mg.lv[0].n);
   mg.q32 = (float *)take(sizeof(float) * mg.lv[0].n);
+  mg.s32 = (float *)take(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
     const size_t b = sizeof(float) * v.n;
@@ -768,7 +777,7 @@ namespace {
 
 // Level-0 streaming sweep launch for the multigrid's cells-per-lane choice.
 template <bool LAST>
-void launch_sweep(const Multigrid &mg, const float *a, const double *r, float *b, double *z,
+void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b, double *z,
                   const float *q = nullptr) {
   const dim3 blk(kTileX, mg.tl.ty);
   const float w = mg.omega;
@@ -787,8 +796,8 @@ void launch_sweep(const Multigrid &mg, const float *a, const double *r, float *b
 
 int mg_first(const Multigrid &mg, const double *r) {
   const MgLevel &f0 = mg.lv[0];
-  k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMx, kMy), 0, stream()>>>(mg.g32, mg.code, r,
-                                                                             f0.x, mg.omega);
+  k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMx, kMy), 0, stream()>>>(
+      mg.g32, mg.code, r, f0.x, mg.s32, mg.omega);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
 }
@@ -801,16 +810,17 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, dou
   TimedLaunch tl("mg_update");
   if (cl == 2)
     k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32);
+                                                  mg.q32, mg.s32);
   else
     k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32);
+                                                  mg.q32, mg.s32);
   FFPM_LAUNCHED("k_mg_upd_t");
   return OK;
 }
 
-int mg_vcycle(const Multigrid &mg, const double *r, double *z, bool with_q) {
+int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
   const dim3 blk(kMx, kMy);
+  const float *r = mg.s32;
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
   const float wc = mg.omega_coarse;

@@ -53,6 +53,7 @@ struct Multigrid {
   int nu_coarse = 4;         // coarse-level sweeps before and after
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
+  float *s32 = nullptr;        // the cycle's source r / V (level-0 cells)
 };
 
 // Builds the hierarchy for the frame with FP64 geometry g64, its FP32 copy
@@ -61,19 +62,20 @@ struct Multigrid {
 int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
              const uint8_t *code, const Tiling &tl, int cl, double *partials);
 
-// First smoothing step from zero on r: lv[0].x = w D^-1 r.
+// The cycle's source s32 = r / V and the first smoothing step from zero,
+// lv[0].x = w D^-1 r.
 int mg_first(const Multigrid &mg, const double *r);
 
-// x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r and q32 = A p (alpha in
-// sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
+// x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r, q32 = A p and s32 = r / V
+// (alpha in sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
 // shape. With rmax non-null, one partial max |r / V| per workgroup (t.blocks)
 // goes there.
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
               double *r, Scalars *sc, double *rmax = nullptr);
 
 // z = B r (the rest of the cycle after the first step, which must be in
-// lv[0].x). Leaves two slots of partial sums (tl.blocks each) in partials:
-// z.r, and z.q32 when with_q (else 0).
-int mg_vcycle(const Multigrid &mg, const double *r, double *z, bool with_q);
+// lv[0].x with its source in s32). Leaves two slots of partial sums
+// (tl.blocks each) in partials: z.r, and z.q32 when with_q (else 0).
+int mg_vcycle(const Multigrid &mg, double *z, bool with_q);
 
 }  // namespace ffpm

@@ -250,8 +250,9 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 // With DOT false (the multigrid-preconditioned CG) z = zscale M^-1 r is stored
 // in ZT (the multigrid's first smoothing step from zero) and no dot is formed;
 // with rmax non-null each workgroup also stores max |r / V| over its fluid
-// cells there (an estimate of the max-norm residual), and with qout non-null
-// q = A p is stored in ZT (for the flexible-CG coefficient of the next step).
+// cells there (an estimate of the max-norm residual), with qout non-null
+// q = A p is stored in ZT (for the flexible-CG coefficient of the next step),
+// and with sout non-null r / V (0 outside fluid cells) is stored in ZT.
 template <typename T, int CL, typename ZT = T, bool DOT = true>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const T *__restrict__ p,
@@ -260,7 +261,8 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
                                              ZT *__restrict__ z, double *partials,
                                              const Scalars *sc, double *lds,
                                              T zscale = T(1), double *rmax = nullptr,
-                                             ZT *__restrict__ qout = nullptr) {
+                                             ZT *__restrict__ qout = nullptr,
+                                             ZT *__restrict__ sout = nullptr) {
   if (sc->status != 0) return;
   const T alpha = (T)sc->alpha;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -311,7 +313,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       ld.r = ldv<T, CL>(r, at(kn));
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
       Vec<T, CL> xv = u.x, rv = u.r;
-      Vec<ZT, CL> zv, qv;
+      Vec<ZT, CL> zv, qv, sv;
       const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
       const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
       const T vw = lane == 0 ? u.pl.w.a : from_w;
@@ -334,6 +336,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
         } else {
           zv.v[m] = (ZT)(zscale * rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m]);
           qv.v[m] = (ZT)q;
+          sv.v[m] = d > 0 ? (ZT)(rv.v[m] / (dxdy[m] * dzk)) : ZT(0);
           if (rmax && d > 0) rmx = fmax(rmx, fabs((double)rv.v[m]) / (double)(dxdy[m] * dzk));
         }
       }
@@ -342,6 +345,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       stv<ZT, CL>(z, at(k), zv);
       if constexpr (!DOT) {
         if (qout) stv<ZT, CL>(qout, at(k), qv);
+        if (sout) stv<ZT, CL>(sout, at(k), sv);
       }
       p_prev = p_cur;
       p_cur = u.p_next;
