@@ -28,29 +28,15 @@ __device__ inline void finish_body(int op, int nslots, const double *partials, i
       sc->red2 = v[1];
     }
   } else if (op == FIN_FCG1) {
-    // CG preconditioned by the multigrid cycle, z.r in slot 0. With one slot
-    // beta = z.r / (z.r)_old (Fletcher-Reeves, the previous z.r in rz_new);
-    // with two slots, slot 1 holding
-    // z.(A p_old), p = z + beta p_old is made A-orthogonal to p_old
-    // (flexible CG with one stored direction: beta = -z.A p_old / p_old.A p_old).
+    // CG preconditioned by the multigrid cycle (flexible CG with one stored
+    // direction): slot 0 z.r, slot 1 z.(A p_old); p = z + beta p_old is made
+    // A-orthogonal to p_old, beta = -z.A p_old / p_old.A p_old, and
     // alpha = z.r / p.A p follows in FIN_PQ.
-    if (nslots >= 2) {
-      double v[2];
-      grid_finish_sum<2>(partials, nblocks, v, lds);
-      if (lead) {
-        sc->beta = sc->pq > 0 ? -v[1] / sc->pq : 0.0;
-        sc->rz = v[0];
-      }
-    } else {
-      double v[1];
-      grid_finish_sum<1>(partials, nblocks, v, lds);
-      if (lead) {
-        // the previous z.r is kept in rz_new: a true-residual check between
-        // two cycles overwrites rz with the diagonal r.M^-1 r
-        sc->beta = sc->rz_new > 0 ? v[0] / sc->rz_new : 0.0;
-        sc->rz = v[0];
-        sc->rz_new = v[0];
-      }
+    double v[2];
+    grid_finish_sum<2>(partials, nblocks, v, lds);
+    if (lead) {
+      sc->beta = sc->pq > 0 ? -v[1] / sc->pq : 0.0;
+      sc->rz = v[0];
     }
   } else if (op == FIN_SUMS) {
     double v[kReduceSlots];

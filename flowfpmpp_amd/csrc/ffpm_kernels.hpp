@@ -39,8 +39,7 @@ enum FinishOp : int {
   FIN_PQ = 3,     // pq = sum(slot 0); breakdown test; alpha = rz / pq
   FIN_RZNEW = 4,  // rz_new = sum(slot 0); rz_new < 0 test; beta; rz; iters += 1
   FIN_SUMS = 5,   // sums[s] = sum(slot s), s < nslots
-  FIN_FCG1 = 6,   // rz = sum(slot 0) = z.r; beta = z.r / rz_old, or with 2 slots
-                  // beta = -sum(slot 1) / pq = -z.(A p_old) / pq
+  FIN_FCG1 = 6,   // rz = sum(slot 0) = z.r; beta = -sum(slot 1) / pq = -z.(A p_old) / pq
 };
 
 struct Bounds {
