@@ -253,6 +253,60 @@ A100_JACOBI_FRAME = {(64, 64, 16): (0.00702, 19.9), (128, 128, 32): (0.0279, 26.
 A100_JACOBI_1E8 = {(128, 128, 32): (226000, 5.329)}
 
 
+def bench_jacobi(args):
+    """Weighted Jacobi to 1e-10 on the 512x384x256 frame (x direction), capped:
+    the solve runs in chunks of --check sweeps continuing from the previous
+    phi (the sweep is memoryless and commutes with the gauge's constant
+    shift), the FP64 max-norm residual is recorded after every chunk, and the
+    sweeps still needed are extrapolated from the asymptotic rate of the last
+    chunks; the time is that count times the measured per-sweep cost."""
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd import workloads as W
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    T = DeviceArray.from_host
+    p = frame(tuple(args.grid))
+    rhs, bg, faces = W.influence_inputs(p, 0)
+    d_solid, d_rhs = T(p.solid), T(rhs)
+    d_faces = [T(f) for f in faces]
+    axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+    S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, bg, *d_faces, *axes[3:])
+    phi = T(p.phi0())
+    chunk, total = args.check, 0
+    hist = []
+    synchronize()
+    t0 = time.perf_counter()
+    while total < args.max_iters:
+        sw, res = S.solve_stairstep_poisson_3d(phi, d_rhs, d_solid, bg, *d_faces, *axes, -1.0,
+                                               chunk, chunk, method="jacobi")
+        total += sw
+        hist.append((total, res))
+        log(f"jacobi: {total} sweeps, residual {res:.3e}")
+        if res <= args.tol:
+            break
+    synchronize()
+    elapsed = time.perf_counter() - t0
+    per_sweep_ms = elapsed * 1e3 / total
+    tail = hist[-4:]
+    rate = (np.log(tail[-1][1]) - np.log(tail[0][1])) / (tail[-1][0] - tail[0][0])
+    res_now = hist[-1][1]
+    more = 0.0 if res_now <= args.tol else (np.log(args.tol) - np.log(res_now)) / rate
+    sweeps = total + more
+    out = {
+        "metric": "weighted Jacobi time to solve phi_x to 1e-10 (ms), extrapolated",
+        "value": round(sweeps * per_sweep_ms, 1), "unit": "ms", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{p.label} frame, x direction, weighted Jacobi (0.2/0.8), "
+                               f"{total} sweeps run in chunks of {chunk}, then the sweeps left "
+                               "to the tolerance from the rate of the last three chunks",
+                   "grid": list(args.grid), "tolerance": args.tol},
+        "sweeps_run": total, "residual_history": [[int(a), float(b)] for a, b in hist],
+        "rate_per_sweep": float(rate), "sweeps_extrapolated": int(round(sweeps)),
+        "ms_per_sweep": round(per_sweep_ms, 4),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_reference(args):
     """The reference's benchmark (tests/benchmark_stairstep_3d.f90:51-77: a
     stair-step sphere on a sinusoidally stretched grid, x influence field,
@@ -338,10 +392,13 @@ def main():
     ap.add_argument("--check", type=int, default=100)
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="frame", choices=["frame", "batch64", "reference"],
+    ap.add_argument("--workload", default="frame",
+                    choices=["frame", "batch64", "reference", "jacobi"],
                     help="frame: the 3-direction solve of one frame (the headline); batch64: "
                          "64 uneven frames x 3 directions in one batched solve; reference: "
-                         "the reference's own benchmark frames, timed as it times them")
+                         "the reference's own benchmark frames, timed as it times them; "
+                         "jacobi: weighted Jacobi on the frame's x direction, capped at "
+                         "--max-iters sweeps in chunks of --check, extrapolated to --tol")
     ap.add_argument("--sequential", action="store_true",
                     help="batch64: also time the 192 solves one after another")
     args = ap.parse_args()
@@ -357,6 +414,8 @@ def main():
         return bench_batch(args)
     if args.workload == "reference":
         return bench_reference(args)
+    if args.workload == "jacobi":
+        return bench_jacobi(args)
 
     t_gen = time.perf_counter()
     p = frame(tuple(args.grid))
