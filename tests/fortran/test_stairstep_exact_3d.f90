@@ -4,8 +4,9 @@ program test_stairstep_exact_3d
   ! four blocked bodies, one touching the outer boundary). With body and outer
   ! face gradients g and f = 0, phi = g.x minus its fluid mean is the exact
   ! discrete solution; gradients equal g and the densities have closed forms.
-  ! Both iterations; the reference's pass criteria (exact_3d.f90:46-50). The
-  ! x influence field then runs 600 fixed Jacobi sweeps and PCG to 1e-11.
+  ! Jacobi, PCG and the library's multigrid-preconditioned CG ('fcg-mg'); the
+  ! reference's pass criteria (exact_3d.f90:46-50). The x influence field then
+  ! runs 600 fixed Jacobi sweeps, PCG and 'fcg-mg' to 1e-11.
   use,intrinsic :: iso_fortran_env,only:real64,output_unit,error_unit
   use,intrinsic :: ieee_arithmetic,only:ieee_is_finite
   use fpm_stairstep_mpi_3d
@@ -22,7 +23,7 @@ program test_stairstep_exact_3d
   real(real64)::cx,cy,cz,ev,p
   integer::solid(0:nx+1,0:ny+1,nz),i,j,k,sweeps,m
   logical::ok,all_ok
-  character(8),parameter::methods(2)=[character(8)::'jacobi','pcg']
+  character(8),parameter::methods(3)=[character(8)::'jacobi','pcg','fcg-mg']
 
   call axis(nx,0.0_real64,1.2_real64,0.21_real64,fx,xc(1:nx),dx)
   call axis(ny,0.0_real64,1.0_real64,-0.17_real64,fy,yc(1:ny),dy)
@@ -42,7 +43,7 @@ program test_stairstep_exact_3d
   bounds=[-1.0_real64,2.0_real64,-1.0_real64,2.0_real64,-1.0_real64,1.0_real64]
   all_ok=.true.
 
-  do m=1,2
+  do m=1,3
     tol=merge(1e-12_real64,1e-10_real64,m==1)
     body=g;xlo=g(1);xhi=g(1);ylo=g(2);yhi=g(2);zlo=g(3);zhi=g(3);rhs=0;phi=0
     call enforce_stairstep_compatibility_3d(rhs,solid,body,xlo,xhi,ylo,yhi,zlo,zhi,dx,dy,dz, &
@@ -95,7 +96,7 @@ program test_stairstep_exact_3d
       ok=sweeps==600
     else
       call solve_stairstep_poisson_3d(phi,rhs,solid,body,xlo,xhi,ylo,yhi,zlo,zhi,xc,yc,zc,dx,dy,dz, &
-           nx,ny,nz,none,none,none,none,comm,1e-11_real64,100000,50,sweeps,res)
+           nx,ny,nz,none,none,none,none,comm,1e-11_real64,100000,50,sweeps,res,method=trim(methods(m)))
       ok=res<=2e-11_real64
     endif
     chk=stairstep_residual_3d(phi,rhs,solid,body,xlo,xhi,ylo,yhi,zlo,zhi,xc,yc,zc,dx,dy,dz, &
