@@ -479,7 +479,7 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
 // LAST writes out as z in FP64 and the partial sums z.r = z.(s V) and z.q (two
 // slots; q = A p_old from the previous update, FP32, or z.q = 0 when q is
 // null).
-template <int CL, bool LAST>
+template <int CL, bool LAST, bool NT>
 __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const float *__restrict__ a,
                                                      const float *__restrict__ r,
@@ -526,12 +526,12 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
     V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
     sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
     sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
-    sa.r = ldv<float, CL>(r, at(k0));
+    sa.r = ldv_s<NT, float, CL>(r, at(k0));
     auto stage = [&](int k, const Set &u, Set &ld) {
       const int kn = min(k + 1, k1 - 1);
       ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
       ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
-      ld.r = ldv<float, CL>(r, at(kn));
+      ld.r = ldv_s<NT, float, CL>(r, at(kn));
       const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
       const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
       const float vw = lane == 0 ? u.pl.w.a : from_w;
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
       V bv, qz;
       VD zv;
       if constexpr (LAST) {
-        if (q) qz = ldv<float, CL>(q, at(k));
+        if (q) qz = ldv_s<NT, float, CL>(q, at(k));
       }
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
@@ -560,8 +560,8 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
           bv.v[m] = o;
         }
       }
-      if constexpr (LAST) stv<double, CL>(z, at(k), zv);
-      else stv<float, CL>(b, at(k), bv);
+      if constexpr (LAST) stv_s<NT, double, CL>(z, at(k), zv);
+      else stv_s<NT, float, CL>(b, at(k), bv);
       a_prev = a_cur;
       a_cur = u.a_next;
     };
@@ -701,6 +701,7 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   mg.scale = env_float("FFPM_MG_SCALE", 0.5f);
   mg.coarse_sweeps = (int)env_float("FFPM_MG_COARSE_SWEEPS", 30.f);
   if (mg.coarse_sweeps < 1) mg.coarse_sweeps = 1;
+  mg.nontemporal = env_float("FFPM_MG_NT", 1.f) != 0.f;
   mg.nu = (int)env_float("FFPM_MG_NU", 3.f);
   mg.nu_coarse = (int)env_float("FFPM_MG_NU_COARSE", 4.f);
   if (mg.nu < 2) mg.nu = 2;  // the update forms the first sweep, the last one writes z
@@ -776,20 +777,27 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 namespace {
 
 // Level-0 streaming sweep launch for the multigrid's cells-per-lane choice.
-template <bool LAST>
-void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b, double *z,
-                  const float *q = nullptr) {
+template <bool LAST, bool NT>
+void launch_sweep_nt(const Multigrid &mg, const float *a, const float *r, float *b, double *z,
+                     const float *q) {
   const dim3 blk(kTileX, mg.tl.ty);
   const float w = mg.omega;
   if (mg.cl == 4)
-    k_mg_sweep_t<4, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
-                                                              mg.partials, w, q);
+    k_mg_sweep_t<4, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
+                                                                  b, z, mg.partials, w, q);
   else if (mg.cl == 2)
-    k_mg_sweep_t<2, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
-                                                              mg.partials, w, q);
+    k_mg_sweep_t<2, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
+                                                                  b, z, mg.partials, w, q);
   else
-    k_mg_sweep_t<1, LAST><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code, b, z,
-                                                              mg.partials, w, q);
+    k_mg_sweep_t<1, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
+                                                                  b, z, mg.partials, w, q);
+}
+
+template <bool LAST>
+void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b, double *z,
+                  const float *q = nullptr) {
+  if (mg.nontemporal) launch_sweep_nt<LAST, true>(mg, a, r, b, z, q);
+  else launch_sweep_nt<LAST, false>(mg, a, r, b, z, q);
 }
 
 }  // namespace

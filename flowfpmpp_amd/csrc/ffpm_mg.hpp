@@ -51,6 +51,7 @@ struct Multigrid {
   int coarse_sweeps = 30;
   int nu = 3;                // level-0 sweeps before and after the coarse correction
   int nu_coarse = 4;         // coarse-level sweeps before and after
+  bool nontemporal = true;   // level-0 sweeps: nontemporal source loads and stores
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
   float *s32 = nullptr;        // the cycle's source r / V (level-0 cells)

@@ -41,6 +41,54 @@ __device__ __forceinline__ void stv(T *__restrict__ a, size_t c, const Vec<T, CL
   *reinterpret_cast<Vec<T, CL> *>(a + c) = x;
 }
 
+// Streaming accesses of values used once (the source, the outputs): the
+// nontemporal form marks the cache lines for early eviction, which keeps
+// the L2 for the neighbour rows the stencil re-reads. NT = false keeps the
+// plain access (FFPM_MG_NT=0 for the multigrid kernels).
+template <typename T, int CL>
+struct NtVec;
+template <> struct NtVec<float, 1> { typedef float type; };
+template <> struct NtVec<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct NtVec<double, 1> { typedef double type; };
+template <> struct NtVec<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<double, 4> { typedef double type __attribute__((ext_vector_type(4))); };
+
+template <bool NT, typename T, int CL>
+__device__ __forceinline__ Vec<T, CL> ldv_s(const T *__restrict__ a, size_t c) {
+  if constexpr (NT) {
+    typedef typename NtVec<T, CL>::type V;
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(a + c));
+    Vec<T, CL> o;
+    if constexpr (CL == 1) {
+      o.v[0] = v;
+    } else {
+#pragma unroll
+      for (int m = 0; m < CL; ++m) o.v[m] = v[m];
+    }
+    return o;
+  } else {
+    return ldv<T, CL>(a, c);
+  }
+}
+
+template <bool NT, typename T, int CL>
+__device__ __forceinline__ void stv_s(T *__restrict__ a, size_t c, const Vec<T, CL> &x) {
+  if constexpr (NT) {
+    typedef typename NtVec<T, CL>::type V;
+    V v;
+    if constexpr (CL == 1) {
+      v = x.v[0];
+    } else {
+#pragma unroll
+      for (int m = 0; m < CL; ++m) v[m] = x.v[m];
+    }
+    __builtin_nontemporal_store(v, reinterpret_cast<V *>(a + c));
+  } else {
+    stv<T, CL>(a, c, x);
+  }
+}
+
 // Raw loads behind one direction vector (z and p_old for kDirUpdate, z for
 // kDirReset, p_old for kDirCurrent), combined only when used, so the loads of
 // a whole plane can be issued one plane ahead. The mode is a template
@@ -340,12 +388,14 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
           if (rmax && d > 0) rmx = fmax(rmx, fabs((double)rv.v[m]) / (double)(dxdy[m] * dzk));
         }
       }
-      stv<T, CL>(x, at(k), xv);
-      stv<T, CL>(r, at(k), rv);
-      stv<ZT, CL>(z, at(k), zv);
+      // the multigrid variant writes its outputs with nontemporal stores:
+      // none is read again by this kernel
+      stv_s<!DOT, T, CL>(x, at(k), xv);
+      stv_s<!DOT, T, CL>(r, at(k), rv);
+      stv_s<!DOT, ZT, CL>(z, at(k), zv);
       if constexpr (!DOT) {
-        if (qout) stv<ZT, CL>(qout, at(k), qv);
-        if (sout) stv<ZT, CL>(sout, at(k), sv);
+        if (qout) stv_s<true, ZT, CL>(qout, at(k), qv);
+        if (sout) stv_s<true, ZT, CL>(sout, at(k), sv);
       }
       p_prev = p_cur;
       p_cur = u.p_next;
