@@ -56,6 +56,28 @@ __device__ __forceinline__ Idx cell_idx(int nx, int ny) {
   return t;
 }
 
+// Diagonal of a coarse level: the sum of the six face conductances.
+__device__ __forceinline__ float coarse_diag(int nx, int ny, const float *__restrict__ gE,
+                                             const float *__restrict__ gN,
+                                             const float *__restrict__ gT, size_t c, int i,
+                                             int j, int k) {
+  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
+  const float gw = i > 0 ? gE[c - 1] : 0.f, gs = j > 0 ? gN[c - sx] : 0.f,
+              gb = k > 0 ? gT[c - sxy] : 0.f;
+  return gw + gE[c] + gs + gN[c] + gb + gT[c];
+}
+
+// The coarse level's first sweep from zero, x = omega b / diag (0 where the
+// diagonal is 0), formed by the restriction that produces b.
+struct CoarseOut {
+  float *b, *x;
+  const float *rd;  // omega / diag, 0 where the diagonal is 0
+  __device__ __forceinline__ void store(size_t c, float bv) const {
+    b[c] = bv;
+    x[c] = rd[c] * bv;
+  }
+};
+
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
 // The cycle's source s = r / V in FP32 (0 where d = 0) and the first sweep
 // from zero, x = omega D^-1 r = omega s / d; D = V diag.
@@ -86,7 +108,7 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
                                                         const float *__restrict__ src,
                                                         const float *__restrict__ x, int cnx,
                                                         int cny, int cnz, int kchunk,
-                                                        float *__restrict__ bc) {
+                                                        CoarseOut out) {
   const int I = blockIdx.x * kMx + threadIdx.x, J = blockIdx.y * kMy + threadIdx.y;
   if (I >= cnx || J >= cny) return;
   const int K0 = blockIdx.z * kchunk, K1 = min(cnz, K0 + kchunk);
@@ -175,7 +197,7 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
   for (int K = K0; K < K1; ++K) {
     load(2 * K + 2, above);
     const float s = plane(2 * K, below, c0, c1) + plane(2 * K + 1, c0, c1, above);
-    bc[(size_t)I + (size_t)cnx * ((size_t)J + (size_t)cny * K)] = s;
+    out.store((size_t)I + (size_t)cnx * ((size_t)J + (size_t)cny * K), s);
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -324,26 +346,8 @@ __device__ __forceinline__ Coarse coarse_row(int nx, int ny, int nz, const float
   return o;
 }
 
-__device__ __forceinline__ float coarse_diag(int nx, int ny, const float *__restrict__ gE,
-                                             const float *__restrict__ gN,
-                                             const float *__restrict__ gT, size_t c, int i,
-                                             int j, int k) {
-  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
-  const float gw = i > 0 ? gE[c - 1] : 0.f, gs = j > 0 ? gN[c - sx] : 0.f,
-              gb = k > 0 ? gT[c - sxy] : 0.f;
-  return gw + gE[c] + gs + gN[c] + gb + gT[c];
-}
 
-__global__ __launch_bounds__(256) void k_mgc_jac0(int nx, int ny, const float *__restrict__ gE,
-                                                  const float *__restrict__ gN,
-                                                  const float *__restrict__ gT,
-                                                  const float *__restrict__ b,
-                                                  float *__restrict__ x, float omega) {
-  const Idx t = cell_idx(nx, ny);
-  if (!t.in) return;
-  const float d = coarse_diag(nx, ny, gE, gN, gT, t.c, t.i, t.j, t.k);
-  x[t.c] = d > 0.f ? omega * b[t.c] / d : 0.f;
-}
+
 
 __global__ __launch_bounds__(256) void k_mgc_sweep(int nx, int ny, int nz,
                                                    const float *__restrict__ gE,
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
                                                       const float *__restrict__ gT,
                                                       const float *__restrict__ b,
                                                       const float *__restrict__ x, int cnx,
-                                                      int cny, float *__restrict__ bc) {
+                                                      int cny, CoarseOut out) {
   const Idx t = cell_idx(cnx, cny);
   if (!t.in) return;
   float s = 0.f;
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
       }
     }
   }
-  bc[t.c] = s;
+  out.store(t.c, s);
 }
 
 // Trilinear cell-centred prolongation from the coarse level (cnx, cny, cnz;
@@ -674,11 +678,13 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(int nx, int ny, int nz,
 
 __global__ __launch_bounds__(256) void k_mg_mask(int nx, int ny, const float *__restrict__ gE,
                                                  const float *__restrict__ gN,
-                                                 const float *__restrict__ gT,
-                                                 float *__restrict__ m) {
+                                                 const float *__restrict__ gT, float omega,
+                                                 float *__restrict__ m, float *__restrict__ rd) {
   const Idx t = cell_idx(nx, ny);
   if (!t.in) return;
-  m[t.c] = coarse_diag(nx, ny, gE, gN, gT, t.c, t.i, t.j, t.k) > 0.f ? 1.f : 0.f;
+  const float d = coarse_diag(nx, ny, gE, gN, gT, t.c, t.i, t.j, t.k);
+  m[t.c] = d > 0.f ? 1.f : 0.f;
+  rd[t.c] = d > 0.f ? omega / d : 0.f;
 }
 
 float env_float(const char *name, float fallback) {
@@ -729,10 +735,10 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     return ERR_SHAPE;
   }
   mg.levels = L + 1;
-  // one arena: level-0 x, y, q, s; per coarse level gE gN gT m x y b
+  // one arena: level-0 x, y, q, s; per coarse level gE gN gT m rd x y b
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   size_t bytes = 4 * up(sizeof(float) * mg.lv[0].n);
-  for (int l = 1; l < mg.levels; ++l) bytes += 7 * up(sizeof(float) * mg.lv[l].n);
+  for (int l = 1; l < mg.levels; ++l) bytes += 8 * up(sizeof(float) * mg.lv[l].n);
   int st = 0;
   char *base = (char *)workspace().get(S_MG, bytes, &st);
   if (!base) return st;
@@ -749,7 +755,8 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     MgLevel &v = mg.lv[l];
     const size_t b = sizeof(float) * v.n;
     v.gE = (float *)take(b); v.gN = (float *)take(b); v.gT = (float *)take(b);
-    v.m = (float *)take(b); v.x = (float *)take(b); v.y = (float *)take(b);
+    v.m = (float *)take(b); v.rd = (float *)take(b); v.x = (float *)take(b);
+    v.y = (float *)take(b);
     v.b = (float *)take(b);
   }
   // conductances and masks of the coarse levels
@@ -767,8 +774,8 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   }
   for (int l = 1; l < mg.levels; ++l) {
     const MgLevel &c = mg.lv[l];
-    k_mg_mask<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMx, kMy), 0, stream()>>>(c.nx, c.ny, c.gE,
-                                                                          c.gN, c.gT, c.m);
+    k_mg_mask<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMx, kMy), 0, stream()>>>(
+        c.nx, c.ny, c.gE, c.gN, c.gT, mg.omega_coarse, c.m, c.rd);
     FFPM_LAUNCHED("k_mg_mask");
   }
   return OK;
@@ -826,6 +833,19 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, dou
   return OK;
 }
 
+namespace {
+
+CoarseOut coarse_out(const Multigrid &mg, int l) {
+  const MgLevel &v = mg.lv[l];
+  CoarseOut o;
+  o.b = v.b;
+  o.x = v.x;
+  o.rd = v.rd;
+  return o;
+}
+
+}  // namespace
+
 int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
   const dim3 blk(kMx, kMy);
   const float *r = mg.s32;
@@ -849,26 +869,25 @@ int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
     const int kc = column_chunk(c.nx, c.ny, c.nz);
     if (f0.nx % 2 == 0)
       k_mgf_restrict_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, c.b);
+          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
     else
       k_mgf_restrict_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, c.b);
+          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
   }
   FFPM_LAUNCHED("k_mgf_restrict_z");
   // down: pre-smoothing and restriction
   for (int l = 1; l < L; ++l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
     const dim3 gr = cell_grid(v.nx, v.ny, v.nz);
-    cur[l] = v.x;
+    cur[l] = v.x;  // the first sweep, formed by the restriction into this level
     oth[l] = v.y;
-    k_mgc_jac0<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.gE, v.gN, v.gT, v.b, cur[l], wc);
     for (int s = 1; s < mg.nu_coarse; ++s) {
       k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l],
                                             oth[l], wc);
       std::swap(cur[l], oth[l]);
     }
     k_mgc_restrict<<<cell_grid(c.nx, c.ny, c.nz), blk, 0, stream()>>>(
-        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l], c.nx, c.ny, c.b);
+        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l], c.nx, c.ny, coarse_out(mg, l + 1));
     FFPM_LAUNCHED("k_mgc_restrict");
   }
   {

@@ -34,6 +34,7 @@ struct MgLevel {
   size_t n = 0;
   float *gE = nullptr, *gN = nullptr, *gT = nullptr;  // coarse levels only
   float *m = nullptr;                                 // coarse levels: 1 where diag > 0
+  float *rd = nullptr;                                // coarse levels: omega / diag (or 0)
   float *x = nullptr, *y = nullptr, *b = nullptr;     // b: coarse levels only
 };
 
