@@ -431,12 +431,14 @@ __global__ __launch_bounds__(256) void k_mgc_prolong(int nx, int ny, const float
 }
 
 // Coarsest level (n <= 1024 cells, one workgroup): `sweeps` damped-Jacobi
-// sweeps from zero with the iterate in LDS.
+// sweeps from zero with the iterate in LDS; the first one is rd b, as the
+// restriction into a level forms it.
 __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
                                                        const float *__restrict__ gE,
                                                        const float *__restrict__ gN,
                                                        const float *__restrict__ gT,
                                                        const float *__restrict__ b,
+                                                       const float *__restrict__ rd,
                                                        float *__restrict__ x, float omega,
                                                        int sweeps) {
   __shared__ float xs[kMgCoarsestCells];
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
     bb = b[t];
   }
   const float d = gw + ge + gs + gn + gb + gt;
-  float xv = on && d > 0.f ? omega * bb / d : 0.f;
+  float xv = on ? rd[t] * bb : 0.f;
   const int tw = i > 0 ? t - 1 : t, te = i < nx - 1 ? t + 1 : t;
   const int ts = j > 0 ? t - nx : t, tn = j < ny - 1 ? t + nx : t;
   const int tb = k > 0 ? t - nx * ny : t, tt = k < nz - 1 ? t + nx * ny : t;
@@ -894,7 +896,7 @@ int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
     const MgLevel &c = mg.lv[L];
     cur[L] = c.y;
     k_mgc_coarsest<<<1, kMgCoarsestCells, 0, stream()>>>(c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b,
-                                                         cur[L], wc, mg.coarse_sweeps);
+                                                         c.rd, cur[L], wc, mg.coarse_sweeps);
     FFPM_LAUNCHED("k_mgc_coarsest");
   }
   // up: prolongation into the level's iterate, then post-smoothing

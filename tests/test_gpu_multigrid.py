@@ -189,3 +189,4 @@ def test_fcg_mg_strongly_stretched_frame():
               f"rel diff {rel:.2e}", flush=True)
         assert a[2] <= 1e-10 and b[2] <= 1e-10
         assert a[1] <= 400 and rel <= 1e-8
+
