@@ -16,6 +16,14 @@
 // each solve also runs the FP64 defect-correction loop of the standalone mixed
 // solve (FP32 inner PCG to a relative rz tolerance, FP64 accumulation and true
 // residual) as a per-solve phase on the device.
+//
+// The multigrid-preconditioned CG (FFPM_METHOD_FCG_MG) batches the same way:
+// every solve gets its own hierarchy state (the level operators are shared by
+// the solves of one frame), each level's cycle step runs as one launch over
+// every solve that has that level, the coarsest levels (at different depths)
+// run in one launch of one workgroup per solve, and the host loop of the
+// standalone fcg_mg_iterate (restart, gated true residual, stall exit) runs
+// per solve on the device.
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
@@ -24,6 +32,8 @@
 
 #include "../../include/ffpm.h"
 #include "ffpm_bodies.hpp"
+#include "ffpm_mg.hpp"
+#include "ffpm_mg_body.hpp"
 #include "ffpm_pcg_body.hpp"
 #include "ffpm_runtime.hpp"
 
@@ -46,10 +56,14 @@ struct BSolve {
   double *x, *r, *z, *p[2];
   double *partials;
   Scalars *sc;
-  // FP32 inner solve of mixed precision
+  // FP32 inner solve of mixed precision; g32 and tl32 also serve the
+  // multigrid cycle's level 0
   GeomT<float> g32;
   Tiling tl32;
   float *e32, *r32, *z32, *p32[2];
+  // multigrid CG: the hierarchy and the K chunk of the level-1 column launches
+  Multigrid mg;
+  int kc1;
 };
 
 // Per-solve phase of the loop: the PCG iterations (FP64, or the FP32 inner
@@ -87,6 +101,8 @@ struct BState {
   int finished_at;
   int phase;         // Phase
   int outer, inner;  // mixed precision: corrections and FP32 iterations
+  int first_ready;   // multigrid CG: the cycle's first step is in place for r
+  int last_check;    // multigrid CG: iteration of the last true residual
   double res, best;
   double target;     // mixed precision: inner stop rz <= target
 };
@@ -373,6 +389,249 @@ __global__ void k_bm_check_end(const BSolve *solves, BState *state, int count, i
   state[s] = st;
 }
 
+// ---- multigrid CG: the standalone fcg_mg_iterate per solve ----------------
+// Cell and column launches of level-l arrays: each workgroup decodes its
+// (x, y, z) coordinates in its own solve's launch from its local index.
+__device__ __forceinline__ mgk::Blk cell_blk(const MgLevel &v, int local) {
+  return mgk::blk_of(local, (v.nx + mgk::kMx - 1) / mgk::kMx, (v.ny + mgk::kMy - 1) / mgk::kMy);
+}
+
+// The cycle's source and first smoothing step after a new r (the first slot
+// of a chunk after a true residual or a restart).
+__global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid grid,
+                                                  const BState *state) {
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!iterating(st, 0) || st.first_ready) return;
+  const BSolve &d = solves[s];
+  const Multigrid &mg = d.mg;
+  mgk::mgf_jac0_body(cell_blk(mg.lv[0], grid.local[blockIdx.x]), mg.g32, mg.code, d.r,
+                     mg.lv[0].x, mg.s32, mg.omega);
+}
+
+// Level-0 sweep from the iterate in x (flip 0) or y (flip 1) into the other;
+// LAST writes z (FP64) with the z.r and z.q partials (z.q only after the first
+// direction, as the standalone cycle).
+template <int CL, bool LAST, bool NT>
+__global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid grid,
+                                                   const BState *state, int slot, int flip) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!iterating(st, slot)) return;
+  const BSolve &d = solves[s];
+  const Multigrid &mg = d.mg;
+  const float *a = flip ? mg.lv[0].y : mg.lv[0].x;
+  float *b = flip ? mg.lv[0].x : mg.lv[0].y;
+  const float *q = LAST && st.mode == kDirUpdate ? mg.q32 : nullptr;
+  mgk::mg_sweep_body<CL, LAST, NT>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl, mg.g32, a,
+                                   mg.s32, mg.code, b, d.z, mg.partials, mg.omega, q);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_bg_restrict0(const BSolve *solves, Grid grid,
+                                                      const BState *state, int slot, int flip) {
+  const int s = grid.solve[blockIdx.x];
+  if (!iterating(state[s], slot)) return;
+  const BSolve &d = solves[s];
+  const Multigrid &mg = d.mg;
+  const MgLevel &c = mg.lv[1];
+  const mgk::CoarseOut out{c.b, c.x, c.rd};
+  mgk::mgf_restrict_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), mg.g32, mg.code, mg.s32,
+                                flip ? mg.lv[0].y : mg.lv[0].x, c.nx, c.ny, c.nz, d.kc1, out);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_bg_prolong0(const BSolve *solves, Grid grid,
+                                                     const BState *state, int slot, int flip) {
+  const int s = grid.solve[blockIdx.x];
+  if (!iterating(state[s], slot)) return;
+  const BSolve &d = solves[s];
+  const Multigrid &mg = d.mg;
+  const MgLevel &f0 = mg.lv[0], &c = mg.lv[1];
+  // the coarse iterate ends in y on every coarse level (the coarsest writes
+  // y; the others end an odd number of sweeps after starting in x)
+  mgk::mgf_prolong_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), f0.nx, f0.ny, f0.nz,
+                               mg.code, flip ? f0.y : f0.x, c.nx, c.ny, c.nz, c.y, c.m, d.kc1);
+}
+
+// Coarse level l (not a solve's coarsest): one damped-Jacobi sweep.
+__global__ __launch_bounds__(256) void k_bg_csweep(const BSolve *solves, Grid grid,
+                                                   const BState *state, int slot, int l,
+                                                   int flip) {
+  const int s = grid.solve[blockIdx.x];
+  if (!iterating(state[s], slot)) return;
+  const Multigrid &mg = solves[s].mg;
+  if (l >= mg.levels - 1) return;
+  const MgLevel &v = mg.lv[l];
+  mgk::mgc_sweep_body(cell_blk(v, grid.local[blockIdx.x]), v.nx, v.ny, v.nz, v.gE, v.gN, v.gT,
+                      v.b, flip ? v.y : v.x, flip ? v.x : v.y, mg.omega_coarse);
+}
+
+// Restriction of level l's residual into level l + 1 (grid of level l + 1).
+__global__ __launch_bounds__(256) void k_bg_crestrict(const BSolve *solves, Grid grid,
+                                                      const BState *state, int slot, int l,
+                                                      int flip) {
+  const int s = grid.solve[blockIdx.x];
+  if (!iterating(state[s], slot)) return;
+  const Multigrid &mg = solves[s].mg;
+  if (l >= mg.levels - 1) return;
+  const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
+  const mgk::CoarseOut out{c.b, c.x, c.rd};
+  mgk::mgc_restrict_body(cell_blk(c, grid.local[blockIdx.x]), v.nx, v.ny, v.nz, v.gE, v.gN,
+                         v.gT, v.b, flip ? v.y : v.x, c.nx, c.ny, out);
+}
+
+// Prolongation from level l + 1 into level l's iterate (grid of level l).
+__global__ __launch_bounds__(256) void k_bg_cprolong(const BSolve *solves, Grid grid,
+                                                     const BState *state, int slot, int l,
+                                                     int flip) {
+  const int s = grid.solve[blockIdx.x];
+  if (!iterating(state[s], slot)) return;
+  const Multigrid &mg = solves[s].mg;
+  if (l >= mg.levels - 1) return;
+  const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
+  mgk::mgc_prolong_body(cell_blk(v, grid.local[blockIdx.x]), v.nx, v.ny, v.m,
+                        flip ? v.y : v.x, c.nx, c.ny, c.nz, c.y, c.m);
+}
+
+// Every solve's coarsest level, one workgroup per solve.
+__global__ __launch_bounds__(1024) void k_bg_coarsest(const BSolve *solves, const BState *state,
+                                                      int slot) {
+  __shared__ float xs[kMgCoarsestCells];
+  const int s = blockIdx.x;
+  if (!iterating(state[s], slot)) return;
+  const Multigrid &mg = solves[s].mg;
+  const MgLevel &c = mg.lv[mg.levels - 1];
+  mgk::mgc_coarsest_body(xs, c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b, c.rd, c.y,
+                         mg.omega_coarse, mg.coarse_sweeps);
+}
+
+// x += alpha p, r -= alpha A p and the next cycle's first step; the slot that
+// ends a solve's chunk also forms the max |r / V| partials.
+template <int CL>
+__global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid,
+                                                 const BState *state, int slot) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!iterating(st, slot)) return;
+  const BSolve &d = solves[s];
+  const Multigrid &mg = d.mg;
+  double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
+  mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g, d.p[st.cur], d.code,
+                       d.x, d.r, mg.lv[0].x, d.sc, (double)mg.omega, rmax, mg.q32, mg.s32);
+}
+
+// One workgroup per solve: the finish of the preceding reduction for the
+// solves that ran it (FIN_FCG1, FIN_PQ: the slot's iterating solves; FIN_MAX:
+// those whose chunk ends at the slot; FIN_TR1, FIN_RZ: those that need a
+// true residual). FIN_PQ also switches the current direction buffer.
+__global__ __launch_bounds__(256) void k_bg_finish(const BSolve *solves, BState *state, int op,
+                                                   int slot) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = blockIdx.x;
+  const BState st = state[s];
+  const BSolve &d = solves[s];
+  int nblocks = d.tl.blocks, nslots = 1;
+  if (op == FIN_FCG1 || op == FIN_PQ || op == FIN_MAX) {
+    if (!iterating(st, slot)) return;
+    if (op == FIN_MAX && slot != st.n_chunk - 1) return;
+    if (op == FIN_FCG1) {
+      nblocks = d.mg.tl.blocks;
+      nslots = 2;
+    }
+  } else {
+    if (!st.need_tr) return;
+    nblocks = op == FIN_TR1 ? d.tt.blocks : d.eb;
+    if (op == FIN_TR1) nslots = 3;
+  }
+  finish_body(op, nslots, d.partials, nblocks, d.sc, lds);
+  if (op == FIN_PQ && threadIdx.x == 0) state[s].cur = st.cur ^ 1;
+}
+
+// After a slot: the cycle's first step is in place; later directions update.
+__global__ void k_bg_slot_end(BState *state, int count, int slot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count || !iterating(state[s], slot)) return;
+  state[s].first_ready = 1;
+  state[s].mode = kDirUpdate;
+}
+
+// After a chunk: iteration count, breakdown restart, and whether the FP64
+// true residual is formed (near the tolerance by the chunk's max |r / V|
+// estimate, 20 iterations after the last one, or at the iteration cap).
+__global__ void k_bg_check_begin(const BSolve *solves, BState *state, int count, int ci,
+                                 int max_sweeps, double tolerance) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  st.need_tr = 0;
+  if (st.active) {
+    Scalars *sc = solves[s].sc;
+    const int iters = sc->iters;
+    st.sweeps += iters;
+    st.it += iters;
+    if (iters > 0) st.restarts = 0;
+    if (sc->status != 0) {
+      st.restarts += 1;
+      if (st.restarts > 1) {
+        st.active = 0;
+      } else {
+        sc->status = 0;
+        sc->iters = 0;
+        st.need_tr = 1;
+        st.first_ready = 0;
+        st.mode = kDirReset;
+      }
+    } else {
+      const bool near = !(sc->res > 4.0 * tolerance);
+      if ((st.it % ci == 0 && (near || st.it - st.last_check >= 20)) ||
+          st.sweeps == max_sweeps) {
+        st.last_check = st.it;
+        st.need_tr = 2;
+        st.first_ready = 0;
+      }
+    }
+  }
+  state[s] = st;
+}
+
+// After the true residual: stall test (ten checks without a 1% gain), loop
+// condition, next chunk length.
+__global__ void k_bg_check_end(const BSolve *solves, BState *state, int count, int ci,
+                               int max_sweeps, double tolerance, const int *replay) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  BState st = state[s];
+  const int was_active = st.active;
+  Scalars *sc = solves[s].sc;
+  if (st.need_tr) {
+    st.res = sc->res;
+    if (st.need_tr == 3) {
+      st.best = st.res;
+    } else if (st.need_tr == 2) {
+      if (st.res < 0.99 * st.best) {
+        st.best = st.res;
+        st.stalled = 0;
+      } else {
+        st.stalled += 1;
+      }
+      if (st.stalled >= 10 || !(st.res == st.res)) st.active = 0;
+    }
+  }
+  if (st.active && !(st.sweeps < max_sweeps && st.res > tolerance)) st.active = 0;
+  if (st.active) {
+    int n = ci - st.it % ci;
+    if (max_sweeps - st.sweeps < n) n = max_sweeps - st.sweeps;
+    st.n_chunk = n;
+    sc->iters = 0;
+  }
+  if (was_active && !st.active) st.finished_at = *replay;
+  st.need_tr = 0;
+  state[s] = st;
+}
+
 // Active solves after a replay; advances the replay counter.
 __global__ __launch_bounds__(256) void k_b_count(const BState *state, int count, int *replay,
                                                  int *n_active) {
@@ -468,6 +727,12 @@ struct Batch {
   int *d_counters = nullptr;  // [0] replay, [1] active count
   Grid g_pcg[3], g_pcg32[5], g_tr, g_ew;  // g_pcg[CL], g_pcg32[CL]
   int *d_isolated = nullptr;
+  // multigrid CG: level-0 cell grid, level-1 column grids (odd / even nx),
+  // cell grids of coarse level l over the solves that have it, and the
+  // largest coarsest-level index
+  bool mgm = false;
+  Grid g_cell0, g_col[2], g_lvl[kMgMaxLevels];
+  int mg_top = 0;
 };
 
 int build_grid(Arena &a, Grid &g, const std::vector<int> &blocks, const std::vector<int> &use,
@@ -495,9 +760,10 @@ int build_grid(Arena &a, Grid &g, const std::vector<int> &blocks, const std::vec
 // Builds every solve's operator and state; arrays live in one arena kept in
 // the workspace (grow-only).
 int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zchunk,
-                bool mixed) {
+                bool mixed, bool mgm = false) {
   B.count = count;
   B.mixed = mixed;
+  B.mgm = mgm;
   B.h.assign(count, BSolve{});
   B.cl.assign(count, 1);
   B.cl32.assign(count, 1);
@@ -525,7 +791,9 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
                  align256(sizeof(int) * 4) + align256(sizeof(int));
   std::vector<Tiling> tl(count), tl32(count), tt(count);
   std::vector<int> eb(count);
+  std::vector<Multigrid> plan(mgm ? count : 0);
   size_t grid_blocks = 0;
+  auto cells3 = [](const dim3 &g) { return (size_t)g.x * g.y * g.z; };
   for (int s = 0; s < count; ++s) {
     const ffpm_stairstep_problem &p = P[s];
     const size_t n = (size_t)p.nx * p.ny * p.nz;
@@ -541,14 +809,33 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     total += 6 * align256(sizeof(double) * n) + align256(sizeof(double) * kReduceSlots * nb) +
              align256(sizeof(Scalars));
     if (mixed) total += 5 * align256(sizeof(float) * n);
+    if (mgm) {
+      // extents only: the pointers are bound once the arrays exist
+      Multigrid &mg = plan[s];
+      const GeomT<double> g64{p.nx, p.ny, p.nz, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, nullptr};
+      const GeomT<float> g32{p.nx, p.ny, p.nz, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr};
+      int st = mg_plan(mg, g64, g32, nullptr, tl32[s], B.cl32[s], nullptr);
+      if (st) return st;
+      total += align256(mg_state_bytes(mg));
+      if (B.owner[s] == s) total += align256(mg_operator_bytes(mg));
+      const MgLevel &c1 = mg.lv[1];
+      grid_blocks += cells3(mgk::cell_grid(p.nx, p.ny, p.nz)) +
+                     cells3(mgk::column_grid(c1.nx, c1.ny, c1.nz,
+                                             mgk::column_chunk(c1.nx, c1.ny, c1.nz)));
+      for (int l = 1; l < mg.levels; ++l)
+        grid_blocks += cells3(mgk::cell_grid(mg.lv[l].nx, mg.lv[l].ny, mg.lv[l].nz));
+      B.mg_top = std::max(B.mg_top, mg.levels - 1);
+    }
     if (B.owner[s] == s)
       total += 2 * align256(sizeof(double) * n) + align256(n) +
                align256(sizeof(double) * 2 * (p.nx + p.ny + p.nz)) +
                align256(sizeof(float) * 3 * (p.nx + p.ny + p.nz));
   }
-  // workgroup tables: two int arrays per grid (at most eight grids), each
-  // 256-aligned
-  total += 2 * align256(sizeof(int) * grid_blocks) + 16 * 256;
+  // workgroup tables: two int arrays per grid (at most eight grids, plus
+  // 3 + kMgMaxLevels for the multigrid), each 256-aligned
+  total += 2 * align256(sizeof(int) * grid_blocks) + 2 * 256 * (8 + 3 + kMgMaxLevels);
   int st = 0;
   Arena a;
   a.base = (char *)workspace().get(S_BATCH, total, &st);
@@ -627,6 +914,31 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
                                                 tt[s].blocks), eb[s]);
     d.partials = a.take<double>(sizeof(double) * kReduceSlots * nb);
     d.sc = a.take<Scalars>(sizeof(Scalars));
+    if (mgm) {
+      // the hierarchy: level operators of the frame's first solve, own state
+      Multigrid mg = plan[s];
+      mg.g64 = d.g;
+      mg.g32 = d.g32;
+      mg.code = d.code;
+      mg.partials = d.partials;
+      if (o == s) {
+        char *ob = a.take<char>(mg_operator_bytes(mg));
+        mg_bind_operator(mg, ob);
+      } else {
+        const Multigrid &om = B.h[o].mg;
+        for (int l = 1; l < mg.levels; ++l) {
+          mg.lv[l].gE = om.lv[l].gE;
+          mg.lv[l].gN = om.lv[l].gN;
+          mg.lv[l].gT = om.lv[l].gT;
+          mg.lv[l].m = om.lv[l].m;
+          mg.lv[l].rd = om.lv[l].rd;
+        }
+      }
+      char *sb = a.take<char>(mg_state_bytes(mg));
+      mg_bind_state(mg, sb);
+      d.mg = mg;
+      d.kc1 = mgk::column_chunk(mg.lv[1].nx, mg.lv[1].ny, mg.lv[1].nz);
+    }
     // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379); a frame's later
     // right-hand sides rewrite the identical diag, rdiag and code.
     k_build_operator<<<d.eb, kThreads, 0, stream()>>>(
@@ -636,6 +948,7 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     FFPM_LAUNCHED("k_build_operator");
     k_pack_phi<<<d.eb, kThreads, 0, stream()>>>(nx, ny, nz, p.solid, p.phi, d.x);
     FFPM_LAUNCHED("k_pack_phi");
+    if (mgm && o == s && (st = mg_build(d.mg))) return st;
     k_b_zero_scalars<<<1, 64, 0, stream()>>>(d.sc);
     FFPM_LAUNCHED("k_b_zero_scalars");
   }
@@ -652,7 +965,7 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     }
     if ((st = build_grid(a, B.g_pcg[c], blocks, use, hs, hl))) return st;
   }
-  if (mixed) {
+  if (mixed || mgm) {
     for (int c : {1, 2, 4}) {
       for (int s = 0; s < count; ++s) {
         blocks[s] = tl32[s].blocks;
@@ -668,6 +981,31 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
   if ((st = build_grid(a, B.g_tr, blocks, use, hs, hl))) return st;
   for (int s = 0; s < count; ++s) blocks[s] = eb[s];
   if ((st = build_grid(a, B.g_ew, blocks, use, hs, hl))) return st;
+  if (mgm) {
+    for (int s = 0; s < count; ++s) {
+      const MgLevel &f0 = B.h[s].mg.lv[0];
+      blocks[s] = (int)cells3(mgk::cell_grid(f0.nx, f0.ny, f0.nz));
+      use[s] = 1;
+    }
+    if ((st = build_grid(a, B.g_cell0, blocks, use, hs, hl))) return st;
+    for (int v = 0; v < 2; ++v) {
+      for (int s = 0; s < count; ++s) {
+        const MgLevel &c1 = B.h[s].mg.lv[1];
+        blocks[s] = (int)cells3(mgk::column_grid(c1.nx, c1.ny, c1.nz, B.h[s].kc1));
+        use[s] = (B.h[s].mg.lv[0].nx % 2 == 0) == (v == 1);
+      }
+      if ((st = build_grid(a, B.g_col[v], blocks, use, hs, hl))) return st;
+    }
+    for (int l = 1; l <= B.mg_top; ++l) {
+      for (int s = 0; s < count; ++s) {
+        const Multigrid &mg = B.h[s].mg;
+        use[s] = mg.levels - 1 >= l;
+        blocks[s] = use[s] ? (int)cells3(mgk::cell_grid(mg.lv[l].nx, mg.lv[l].ny, mg.lv[l].nz))
+                           : 0;
+      }
+      if ((st = build_grid(a, B.g_lvl[l], blocks, use, hs, hl))) return st;
+    }
+  }
   FFPM_HIP(hipStreamSynchronize(stream()));
   if (isolated > 0) {
     set_error(ERR_ISOLATED, "stair-step fluid cell has no fluid neighbour");
@@ -772,6 +1110,146 @@ int launch_chunk(const Batch &B, hipStream_t s, int ci, int max_sweeps, double t
   return OK;
 }
 
+// ---- multigrid CG launches -------------------------------------------------
+// The true residual of the solves that need one, then the loop logic after it.
+int launch_check_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol) {
+  const int cb = (B.count + 63) / 64;
+  k_b_tr1<<<B.g_tr.blocks, dim3(kTileX, B.h[0].tt.ty), 0, s>>>(B.d_solves, B.g_tr, B.d_state);
+  FFPM_LAUNCHED("k_b_tr1");
+  k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_TR1, 0);
+  FFPM_LAUNCHED("k_bg_finish");
+  k_b_tr2<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
+  FFPM_LAUNCHED("k_b_tr2");
+  k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZ, 0);
+  FFPM_LAUNCHED("k_bg_finish");
+  k_bg_check_end<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps, tol,
+                                   B.d_counters);
+  FFPM_LAUNCHED("k_bg_check_end");
+  return OK;
+}
+
+// One V-cycle of every solve iterating in slot m: each level's step is one
+// launch over the solves that have the level. The number of sweeps per level
+// is the same for every solve, so the iterate's place (x or y) is too.
+template <bool NT>
+int launch_cycle_mg(const Batch &B, hipStream_t s, int m) {
+  const Multigrid &h = B.h[0].mg;
+  const dim3 cblk(mgk::kMx, mgk::kMy), sblk(kTileX, B.h[0].tl32.ty);
+  auto sweep0 = [&](bool last, int flip) -> int {
+    for (int c : {1, 2, 4}) {
+      const Grid &g = B.g_pcg32[c];
+      if (!g.blocks) continue;
+      if (c == 4 && last)
+        k_bg_sweep<4, true, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+      else if (c == 4)
+        k_bg_sweep<4, false, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+      else if (c == 2 && last)
+        k_bg_sweep<2, true, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+      else if (c == 2)
+        k_bg_sweep<2, false, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+      else if (last)
+        k_bg_sweep<1, true, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+      else
+        k_bg_sweep<1, false, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+    }
+    FFPM_LAUNCHED("k_bg_sweep");
+    return OK;
+  };
+  int st;
+  int f0 = 0;
+  for (int k = 1; k < h.nu; ++k) {
+    if ((st = sweep0(false, f0))) return st;
+    f0 ^= 1;
+  }
+  if (B.g_col[0].blocks)
+    k_bg_restrict0<false><<<B.g_col[0].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[0], B.d_state,
+                                                             m, f0);
+  if (B.g_col[1].blocks)
+    k_bg_restrict0<true><<<B.g_col[1].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[1], B.d_state,
+                                                            m, f0);
+  FFPM_LAUNCHED("k_bg_restrict0");
+  int fl[kMgMaxLevels] = {};
+  for (int l = 1; l < B.mg_top; ++l) {
+    int f = 0;
+    for (int k = 1; k < h.nu_coarse; ++k) {
+      k_bg_csweep<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l, f);
+      f ^= 1;
+    }
+    k_bg_crestrict<<<B.g_lvl[l + 1].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l + 1],
+                                                          B.d_state, m, l, f);
+    fl[l] = f;
+  }
+  FFPM_LAUNCHED("k_bg_crestrict");
+  k_bg_coarsest<<<B.count, kMgCoarsestCells, 0, s>>>(B.d_solves, B.d_state, m);
+  FFPM_LAUNCHED("k_bg_coarsest");
+  for (int l = B.mg_top - 1; l >= 1; --l) {
+    int f = fl[l];
+    k_bg_cprolong<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l, f);
+    for (int k = 0; k < h.nu_coarse; ++k) {
+      k_bg_csweep<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l, f);
+      f ^= 1;
+    }
+  }
+  FFPM_LAUNCHED("k_bg_csweep");
+  if (B.g_col[0].blocks)
+    k_bg_prolong0<false><<<B.g_col[0].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[0], B.d_state,
+                                                            m, f0);
+  if (B.g_col[1].blocks)
+    k_bg_prolong0<true><<<B.g_col[1].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[1], B.d_state,
+                                                           m, f0);
+  FFPM_LAUNCHED("k_bg_prolong0");
+  for (int k = 1; k < h.nu; ++k) {
+    if ((st = sweep0(false, f0))) return st;
+    f0 ^= 1;
+  }
+  return sweep0(true, f0);
+}
+
+// One chunk: ci slots (first step, cycle, FCG(1) beta, direction and SpMV,
+// alpha, update), then the check logic and the true residual where needed.
+int launch_chunk_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol) {
+  const int cb = (B.count + 63) / 64;
+  const dim3 cblk(mgk::kMx, mgk::kMy), pblk(kTileX, B.h[0].tl.ty);
+  int st;
+  for (int m = 0; m < ci; ++m) {
+    if (m == 0) {
+      k_bg_first<<<B.g_cell0.blocks, cblk, 0, s>>>(B.d_solves, B.g_cell0, B.d_state);
+      FFPM_LAUNCHED("k_bg_first");
+    }
+    st = B.h[0].mg.nontemporal ? launch_cycle_mg<true>(B, s, m)
+                               : launch_cycle_mg<false>(B, s, m);
+    if (st) return st;
+    k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_FCG1, m);
+    FFPM_LAUNCHED("k_bg_finish");
+    for (int c : {1, 2}) {
+      const Grid &g = B.g_pcg[c];
+      if (!g.blocks) continue;
+      if (c == 2) k_b_dir<double, 2><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      else k_b_dir<double, 1><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      FFPM_LAUNCHED("k_b_dir");
+    }
+    k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_PQ, m);
+    FFPM_LAUNCHED("k_bg_finish");
+    for (int c : {1, 2}) {
+      const Grid &g = B.g_pcg[c];
+      if (!g.blocks) continue;
+      if (c == 2) k_bg_upd<2><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      else k_bg_upd<1><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      FFPM_LAUNCHED("k_bg_upd");
+    }
+    k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_MAX, m);
+    FFPM_LAUNCHED("k_bg_finish");
+    k_bg_slot_end<<<cb, 64, 0, s>>>(B.d_state, B.count, m);
+    FFPM_LAUNCHED("k_bg_slot_end");
+  }
+  k_bg_check_begin<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps, tol);
+  FFPM_LAUNCHED("k_bg_check_begin");
+  if ((st = launch_check_mg(B, s, ci, max_sweeps, tol))) return st;
+  k_b_count<<<1, 256, 0, s>>>(B.d_state, B.count, B.d_counters, B.d_counters + 1);
+  FFPM_LAUNCHED("k_b_count");
+  return OK;
+}
+
 struct StreamGuard {
   hipStream_t s = nullptr;
   hipGraph_t graph = nullptr;
@@ -788,7 +1266,8 @@ struct StreamGuard {
 // time the first replay was issued (after the graph is built) in loop_ms.
 int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, int use_graph,
               std::vector<BState> &out, std::vector<double> &replay_ms, double *loop_ms) {
-  const int ci = check_interval > 1 ? check_interval : 1;
+  const int ci = B.mgm ? mg_check_interval(check_interval)
+                       : (check_interval > 1 ? check_interval : 1);
   const int count = B.count;
   int st;
   // All solves must share the PCG workgroup height (one launch per grid).
@@ -807,7 +1286,9 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
   k_b_zero_int<<<1, 64, 0, g.s>>>(B.d_counters, 4);
   FFPM_LAUNCHED("k_b_zero_int");
   // the initial true residual of every solve
-  if ((st = launch_check(B, g.s, ci, max_sweeps, tolerance))) return st;
+  st = B.mgm ? launch_check_mg(B, g.s, ci, max_sweeps, tolerance)
+             : launch_check(B, g.s, ci, max_sweeps, tolerance);
+  if (st) return st;
   k_b_count<<<1, 256, 0, g.s>>>(B.d_state, count, B.d_counters, B.d_counters + 1);
   FFPM_LAUNCHED("k_b_count");
   int active = 0;
@@ -817,7 +1298,8 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
                           std::chrono::steady_clock::now() - t0).count());
   if (use_graph && active > 0) {
     FFPM_HIP(hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal));
-    st = launch_chunk(B, g.s, ci, max_sweeps, tolerance, ty);
+    st = B.mgm ? launch_chunk_mg(B, g.s, ci, max_sweeps, tolerance)
+               : launch_chunk(B, g.s, ci, max_sweeps, tolerance, ty);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(g.s, &graph);
     g.graph = graph;
@@ -830,7 +1312,8 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
   while (active > 0) {
     if (use_graph) {
       FFPM_HIP(hipGraphLaunch(g.exec, g.s));
-    } else if ((st = launch_chunk(B, g.s, ci, max_sweeps, tolerance, ty))) {
+    } else if ((st = B.mgm ? launch_chunk_mg(B, g.s, ci, max_sweeps, tolerance)
+                           : launch_chunk(B, g.s, ci, max_sweeps, tolerance, ty))) {
       return st;
     }
     FFPM_HIP(hipMemcpyAsync(&active, B.d_counters + 1, sizeof(int), hipMemcpyDeviceToHost,
@@ -883,8 +1366,9 @@ int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
     set_error(ERR_ARGUMENT, "empty batch");
     return ERR_ARGUMENT;
   }
-  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_PCG_MIXED) {
-    set_error(ERR_METHOD, "batched solves support pcg and pcg-mixed");
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_PCG_MIXED &&
+      method != FFPM_METHOD_FCG_MG) {
+    set_error(ERR_METHOD, "batched solves support pcg, pcg-mixed and fcg-mg");
     return ERR_METHOD;
   }
   if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8) {
@@ -892,7 +1376,8 @@ int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
     return ERR_ARGUMENT;
   }
   Batch B;
-  int st = batch_setup(B, problems, count, rows, zchunk, method == FFPM_METHOD_PCG_MIXED);
+  int st = batch_setup(B, problems, count, rows, zchunk, method == FFPM_METHOD_PCG_MIXED,
+                       method == FFPM_METHOD_FCG_MG);
   if (st) return st;
   std::vector<BState> out;
   std::vector<double> replay_ms;

@@ -1,107 +1,27 @@
 // Geometric multigrid V-cycle on the stair-step operator (preconditioner of
 // the flexible CG in ffpm_stairstep_host.hip). Layout and cycle as described
-// in ffpm_mg.hpp. Cell kernels run 64 x 4 workgroups over (x, y) tiles with
-// one z plane per grid row, so no index divisions; all loads are
-// unconditional with clamped indices and masked coefficients.
+// in ffpm_mg.hpp; the kernel bodies are in ffpm_mg_body.hpp.
 #include <cstdlib>
 #include <utility>
 
 #include "ffpm_mg.hpp"
-#include "ffpm_pcg_body.hpp"
+#include "ffpm_mg_body.hpp"
 #include "ffpm_runtime.hpp"
 
 namespace ffpm {
 
 namespace {
 
-constexpr int kMx = 64, kMy = 4;
+using namespace mgk;
 
-inline dim3 cell_grid(int nx, int ny, int nz) {
-  return dim3((unsigned)((nx + kMx - 1) / kMx), (unsigned)((ny + kMy - 1) / kMy), (unsigned)nz);
-}
-
-// Column kernels (one thread per coarse (I, J) walking K): K chunk so that
-// the launch has at least FFPM_MG_COLUMN_WAVES (default 16384) wavefronts.
-inline int column_chunk(int cnx, int cny, int cnz) {
-  static const long target = [] {
-    const char *e = std::getenv("FFPM_MG_COLUMN_WAVES");
-    const long v = e ? std::atol(e) : 16384;
-    return v < 1 ? 1L : v;
-  }();
-  const long waves_per_plane = (long)((cnx + kMx - 1) / kMx) * ((cny + kMy - 1) / kMy) * 4;
-  long chunks = (target + waves_per_plane - 1) / waves_per_plane;
-  if (chunks < 1) chunks = 1;
-  if (chunks > cnz) chunks = cnz;
-  return (int)((cnz + chunks - 1) / chunks);
-}
-
-inline dim3 column_grid(int cnx, int cny, int cnz, int kchunk) {
-  return dim3((unsigned)((cnx + kMx - 1) / kMx), (unsigned)((cny + kMy - 1) / kMy),
-              (unsigned)((cnz + kchunk - 1) / kchunk));
-}
-
-struct Idx {
-  int i, j, k;
-  size_t c;
-  bool in;
-};
-
-__device__ __forceinline__ Idx cell_idx(int nx, int ny) {
-  Idx t;
-  t.i = blockIdx.x * kMx + threadIdx.x;
-  t.j = blockIdx.y * kMy + threadIdx.y;
-  t.k = blockIdx.z;
-  t.in = t.i < nx && t.j < ny;
-  t.c = (size_t)t.i + (size_t)nx * ((size_t)t.j + (size_t)ny * t.k);
-  return t;
-}
-
-// Diagonal of a coarse level: the sum of the six face conductances.
-__device__ __forceinline__ float coarse_diag(int nx, int ny, const float *__restrict__ gE,
-                                             const float *__restrict__ gN,
-                                             const float *__restrict__ gT, size_t c, int i,
-                                             int j, int k) {
-  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
-  const float gw = i > 0 ? gE[c - 1] : 0.f, gs = j > 0 ? gN[c - sx] : 0.f,
-              gb = k > 0 ? gT[c - sxy] : 0.f;
-  return gw + gE[c] + gs + gN[c] + gb + gT[c];
-}
-
-// The coarse level's first sweep from zero, x = omega b / diag (0 where the
-// diagonal is 0), formed by the restriction that produces b.
-struct CoarseOut {
-  float *b, *x;
-  const float *rd;  // omega / diag, 0 where the diagonal is 0
-  __device__ __forceinline__ void store(size_t c, float bv) const {
-    b[c] = bv;
-    x[c] = rd[c] * bv;
-  }
-};
-
-// ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
-// The cycle's source s = r / V in FP32 (0 where d = 0) and the first sweep
-// from zero, x = omega D^-1 r = omega s / d; D = V diag.
+// ---- standalone kernels: one solve per launch --------------------------------
 __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t *__restrict__ code,
                                                   const double *__restrict__ r,
                                                   float *__restrict__ x, float *__restrict__ s32,
                                                   float omega) {
-  const Idx t = cell_idx(g.nx, g.ny);
-  if (!t.in) return;
-  const unsigned cd = code[t.c];
-  const float d = diag_code<float>(cd, g.cw[t.i], g.ce[t.i], g.cs[t.j], g.cn[t.j], g.cb[t.k],
-                                   g.ct[t.k]);
-  const float V = g.dx[t.i] * g.dy[t.j] * g.dz[t.k];
-  const float sv = d > 0.f ? (float)r[t.c] / V : 0.f;
-  s32[t.c] = sv;
-  x[t.c] = d > 0.f ? omega * sv / d : 0.f;
+  mgf_jac0_body(blk3(), g, code, r, x, s32, omega);
 }
 
-// Coarse right-hand side: the level-0 residual r - A x = V (s - (d x - sum c
-// x_nb)) summed over the children of each level-1 cell. One thread per level-1 (I, J) column walks a
-// range of K: each fine plane's 2x2 block of x is loaded once and kept for
-// the planes above and below; only the in-plane halo comes from other
-// threads' cells (cache hits). VEC (even nx): the two x-adjacent children of
-// a row move as one 8-byte (x), 16-byte (r) and 2-byte (code) access.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
                                                         const uint8_t *__restrict__ code,
@@ -109,112 +29,9 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
                                                         const float *__restrict__ x, int cnx,
                                                         int cny, int cnz, int kchunk,
                                                         CoarseOut out) {
-  const int I = blockIdx.x * kMx + threadIdx.x, J = blockIdx.y * kMy + threadIdx.y;
-  if (I >= cnx || J >= cny) return;
-  const int K0 = blockIdx.z * kchunk, K1 = min(cnz, K0 + kchunk);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t sx = (size_t)nx;
-  const int i0 = 2 * I, j0 = 2 * J;
-  const bool va[2] = {true, i0 + 1 < nx}, vb[2] = {true, j0 + 1 < ny};
-  const int ia[2] = {i0, va[1] ? i0 + 1 : i0}, jb[2] = {j0, vb[1] ? j0 + 1 : j0};
-  float cw[2], ce[2], dxa[2], cs[2], cn[2], dyb[2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    cw[a] = g.cw[ia[a]];
-    ce[a] = g.ce[ia[a]];
-    dxa[a] = g.dx[ia[a]];
-    cs[a] = g.cs[jb[a]];
-    cn[a] = g.cn[jb[a]];
-    dyb[a] = g.dy[jb[a]];
-  }
-  const bool hw = i0 > 0, he = i0 + 2 < nx, hs = j0 > 0, hn = j0 + 2 < ny;
-  auto at = [&](int i, int j, int k) { return (size_t)i + sx * ((size_t)j + (size_t)ny * k); };
-  auto load = [&](int k, float (&v)[2][2]) {
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const bool ok = k >= 0 && k < nz && vb[b];
-      if constexpr (VEC) {
-        const float2 t = ok ? *reinterpret_cast<const float2 *>(x + at(i0, jb[b], k))
-                            : make_float2(0.f, 0.f);
-        v[b][0] = t.x;
-        v[b][1] = t.y;
-      } else {
-#pragma unroll
-        for (int a = 0; a < 2; ++a) v[b][a] = ok && va[a] ? x[at(ia[a], jb[b], k)] : 0.f;
-      }
-    }
-  };
-  auto plane = [&](int k, const float (&dn)[2][2], const float (&c)[2][2],
-                   const float (&up)[2][2]) {
-    float s = 0.f;
-    if (k >= nz) return s;
-    const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      if (!vb[b]) continue;
-      const size_t c0 = at(i0, jb[b], k);
-      unsigned cds[2];
-      float ss[2];
-      if constexpr (VEC) {
-        const uchar2 cd2 = *reinterpret_cast<const uchar2 *>(code + c0);
-        const float2 s2 = *reinterpret_cast<const float2 *>(src + c0);
-        cds[0] = cd2.x;
-        cds[1] = cd2.y;
-        ss[0] = s2.x;
-        ss[1] = s2.y;
-      } else {
-        cds[0] = code[c0];
-        ss[0] = src[c0];
-        cds[1] = va[1] ? code[c0 + 1] : 0u;
-        ss[1] = va[1] ? src[c0 + 1] : 0.f;
-      }
-      const float xw = hw ? x[c0 - 1] : 0.f, xe = he ? x[c0 + 2] : 0.f;
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const unsigned cd = cds[a];
-        if (!(cd & kCodeFluid)) continue;
-        const size_t cc = c0 + a;
-        const float west = a == 1 ? c[b][0] : xw;
-        const float east = a == 0 ? c[b][1] : xe;
-        const float south = b == 1 ? c[0][a] : (hs ? x[cc - sx] : 0.f);
-        const float north = b == 0 ? c[1][a] : (hn ? x[cc + sx] : 0.f);
-        float nb = (cd & kCodeW) ? cw[a] * west : 0.f;
-        nb += (cd & kCodeE) ? ce[a] * east : 0.f;
-        nb += (cd & kCodeS) ? cs[b] * south : 0.f;
-        nb += (cd & kCodeN) ? cn[b] * north : 0.f;
-        nb += (cd & kCodeB) ? cb * dn[b][a] : 0.f;
-        nb += (cd & kCodeT) ? ct * up[b][a] : 0.f;
-        const float d = diag_code<float>(cd, cw[a], ce[a], cs[b], cn[b], cb, ct);
-        s += dxa[a] * dyb[b] * dzk * (ss[a] - (d * c[b][a] - nb));
-      }
-    }
-    return s;
-  };
-  float below[2][2], c0[2][2], c1[2][2], above[2][2];
-  load(2 * K0 - 1, below);
-  load(2 * K0, c0);
-  load(2 * K0 + 1, c1);
-  for (int K = K0; K < K1; ++K) {
-    load(2 * K + 2, above);
-    const float s = plane(2 * K, below, c0, c1) + plane(2 * K + 1, c0, c1, above);
-    out.store((size_t)I + (size_t)cnx * ((size_t)J + (size_t)cny * K), s);
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        below[b][a] = c1[b][a];
-        c0[b][a] = above[b][a];
-      }
-    load(2 * K + 3, c1);
-  }
+  mgf_restrict_z_body<VEC>(blk3(), g, code, src, x, cnx, cny, cnz, kchunk, out);
 }
 
-// Prolongation onto level 0 with one thread per level-1 (I, J) column walking
-// a range of K: the 3x3 coarse patches (values and unknown masks) of planes
-// K-1, K, K+1 stay in registers, and the thread updates the 2x2x2 fine cells
-// of each coarse cell with the trilinear weights of prolong_value (only fine
-// cells that carry unknowns change). VEC (even nx): each fine row pair moves as
-// one 8-byte x and 2-byte code access.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
                                                        const uint8_t *__restrict__ code,
@@ -222,132 +39,8 @@ __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
                                                        int cnz, const float *__restrict__ xc,
                                                        const float *__restrict__ mc,
                                                        int kchunk) {
-  const int I = blockIdx.x * kMx + threadIdx.x, J = blockIdx.y * kMy + threadIdx.y;
-  if (I >= cnx || J >= cny) return;
-  const int K0 = blockIdx.z * kchunk, K1 = min(cnz, K0 + kchunk);
-  const bool vi[3] = {I > 0, true, I + 1 < cnx}, vj[3] = {J > 0, true, J + 1 < cny};
-  const int ii[3] = {vi[0] ? I - 1 : I, I, vi[2] ? I + 1 : I};
-  const int jj[3] = {vj[0] ? J - 1 : J, J, vj[2] ? J + 1 : J};
-  const size_t csx = (size_t)cnx, fsx = (size_t)nx;
-  const bool has_i1 = 2 * I + 1 < nx;
-  auto cload = [&](int K, float (&v)[3][3], float (&m)[3][3]) {
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const bool ok = K >= 0 && K < cnz && vi[a] && vj[b];
-        const size_t cc = (size_t)ii[a] + csx * ((size_t)jj[b] + (size_t)cny * K);
-        v[b][a] = ok ? xc[cc] : 0.f;
-        m[b][a] = ok ? mc[cc] : 0.f;
-      }
-  };
-  float xm[3][3], mm[3][3], x0[3][3], m0[3][3], xp[3][3], mp[3][3];
-  cload(K0 - 1, xm, mm);
-  cload(K0, x0, m0);
-  for (int K = K0; K < K1; ++K) {
-    cload(K + 1, xp, mp);
-    // the block's four fine rows (2 planes x 2 rows) are loaded together,
-    // updated, and stored together
-    float xv[2][2][2];
-    unsigned cd[2][2][2];
-    bool ok[2][2];
-    size_t c0[2][2];
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int k = 2 * K + c, j = 2 * J + b;
-        ok[c][b] = k < nz && j < ny;
-        c0[c][b] = (size_t)(2 * I) + fsx * ((size_t)(ok[c][b] ? j : 0) +
-                                            (size_t)ny * (ok[c][b] ? k : 0));
-        if constexpr (VEC) {
-          const float2 t = ok[c][b] ? *reinterpret_cast<const float2 *>(x + c0[c][b])
-                                    : make_float2(0.f, 0.f);
-          const uchar2 u = ok[c][b] ? *reinterpret_cast<const uchar2 *>(code + c0[c][b])
-                                    : make_uchar2(0, 0);
-          xv[c][b][0] = t.x;
-          xv[c][b][1] = t.y;
-          cd[c][b][0] = u.x;
-          cd[c][b][1] = u.y;
-        } else {
-          xv[c][b][0] = ok[c][b] ? x[c0[c][b]] : 0.f;
-          cd[c][b][0] = ok[c][b] ? code[c0[c][b]] : 0u;
-          xv[c][b][1] = ok[c][b] && has_i1 ? x[c0[c][b] + 1] : 0.f;
-          cd[c][b][1] = ok[c][b] && has_i1 ? code[c0[c][b] + 1] : 0u;
-        }
-      }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const float(&xz)[3][3] = c ? xp : xm;
-      const float(&mz)[3][3] = c ? mp : mm;
-      const float wk = (c ? K + 1 < cnz : K > 0) ? 0.25f : 0.f;
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int bj = b ? 2 : 0;
-        const float wj = vj[bj] ? 0.25f : 0.f;
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          if (!(cd[c][b][a] & kCodeFluid)) continue;
-          const int ai = a ? 2 : 0;
-          const float wi = vi[ai] ? 0.25f : 0.f;
-          const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
-          const float num0 = w00 * x0[1][1] + w10 * x0[1][ai] + w01 * x0[bj][1] + w11 * x0[bj][ai];
-          const float den0 = w00 * m0[1][1] + w10 * m0[1][ai] + w01 * m0[bj][1] + w11 * m0[bj][ai];
-          const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
-          const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
-          const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
-          xv[c][b][a] += den > 0.f ? num / den : 0.f;
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        if (!ok[c][b]) continue;
-        if constexpr (VEC) {
-          *reinterpret_cast<float2 *>(x + c0[c][b]) = make_float2(xv[c][b][0], xv[c][b][1]);
-        } else {
-          x[c0[c][b]] = xv[c][b][0];
-          if (has_i1) x[c0[c][b] + 1] = xv[c][b][1];
-        }
-      }
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        xm[b][a] = x0[b][a];
-        mm[b][a] = m0[b][a];
-        x0[b][a] = xp[b][a];
-        m0[b][a] = mp[b][a];
-      }
-  }
+  mgf_prolong_z_body<VEC>(blk3(), nx, ny, nz, code, x, cnx, cny, cnz, xc, mc, kchunk);
 }
-
-// ---- coarse levels: per-cell conductances ------------------------------------
-struct Coarse {
-  float d, nb;
-};
-
-__device__ __forceinline__ Coarse coarse_row(int nx, int ny, int nz, const float *__restrict__ gE,
-                                             const float *__restrict__ gN,
-                                             const float *__restrict__ gT,
-                                             const float *__restrict__ x, size_t c, int i,
-                                             int j, int k) {
-  const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
-  const size_t cw = i > 0 ? c - 1 : c, cs = j > 0 ? c - sx : c, cb = k > 0 ? c - sxy : c;
-  const size_t ce = i < nx - 1 ? c + 1 : c, cn = j < ny - 1 ? c + sx : c,
-               ct = k < nz - 1 ? c + sxy : c;
-  const float ge = gE[c], gn = gN[c], gt = gT[c];
-  const float gw = i > 0 ? gE[cw] : 0.f, gs = j > 0 ? gN[cs] : 0.f, gb = k > 0 ? gT[cb] : 0.f;
-  Coarse o;
-  o.d = gw + ge + gs + gn + gb + gt;
-  o.nb = gw * x[cw] + ge * x[ce] + gs * x[cs] + gn * x[cn] + gb * x[cb] + gt * x[ct];
-  return o;
-}
-
-
-
 
 __global__ __launch_bounds__(256) void k_mgc_sweep(int nx, int ny, int nz,
                                                    const float *__restrict__ gE,
@@ -356,11 +49,7 @@ __global__ __launch_bounds__(256) void k_mgc_sweep(int nx, int ny, int nz,
                                                    const float *__restrict__ b,
                                                    const float *__restrict__ in,
                                                    float *__restrict__ out, float omega) {
-  const Idx t = cell_idx(nx, ny);
-  if (!t.in) return;
-  const Coarse o = coarse_row(nx, ny, nz, gE, gN, gT, in, t.c, t.i, t.j, t.k);
-  const float xv = in[t.c];
-  out[t.c] = o.d > 0.f ? xv + omega * (b[t.c] + o.nb - o.d * xv) / o.d : 0.f;
+  mgc_sweep_body(blk3(), nx, ny, nz, gE, gN, gT, b, in, out, omega);
 }
 
 __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
@@ -370,69 +59,16 @@ __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
                                                       const float *__restrict__ b,
                                                       const float *__restrict__ x, int cnx,
                                                       int cny, CoarseOut out) {
-  const Idx t = cell_idx(cnx, cny);
-  if (!t.in) return;
-  float s = 0.f;
-  for (int dk = 0; dk < 2; ++dk) {
-    const int k = 2 * t.k + dk;
-    if (k >= nz) break;
-    for (int dj = 0; dj < 2; ++dj) {
-      const int j = 2 * t.j + dj;
-      if (j >= ny) break;
-      for (int di = 0; di < 2; ++di) {
-        const int i = 2 * t.i + di;
-        if (i >= nx) break;
-        const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-        const Coarse o = coarse_row(nx, ny, nz, gE, gN, gT, x, c, i, j, k);
-        if (o.d > 0.f) s += b[c] - (o.d * x[c] - o.nb);
-      }
-    }
-  }
-  out.store(t.c, s);
-}
-
-// Trilinear cell-centred prolongation from the coarse level (cnx, cny, cnz;
-// values xc, unknown mask mc): fine cell 2I+a takes 3/4 of coarse cell I and
-// 1/4 of its neighbour on the side of a, per axis; weights are renormalised
-// over coarse cells with mc = 1. Only fine cells that carry unknowns change.
-__device__ __forceinline__ float prolong_value(int i, int j, int k, int cnx, int cny, int cnz,
-                                               const float *__restrict__ xc,
-                                               const float *__restrict__ mc) {
-  const int I = i >> 1, J = j >> 1, K = k >> 1;
-  const int I2 = (i & 1) ? I + 1 : I - 1, J2 = (j & 1) ? J + 1 : J - 1,
-            K2 = (k & 1) ? K + 1 : K - 1;
-  const bool vi = I2 >= 0 && I2 < cnx, vj = J2 >= 0 && J2 < cny, vk = K2 >= 0 && K2 < cnz;
-  const int ia[2] = {I, vi ? I2 : I}, ja[2] = {J, vj ? J2 : J}, ka[2] = {K, vk ? K2 : K};
-  const float wi[2] = {0.75f, vi ? 0.25f : 0.f}, wj[2] = {0.75f, vj ? 0.25f : 0.f},
-              wk[2] = {0.75f, vk ? 0.25f : 0.f};
-  float num = 0.f, den = 0.f;
-#pragma unroll
-  for (int c3 = 0; c3 < 2; ++c3)
-#pragma unroll
-    for (int b3 = 0; b3 < 2; ++b3)
-#pragma unroll
-      for (int a3 = 0; a3 < 2; ++a3) {
-        const size_t cc = (size_t)ia[a3] + (size_t)cnx * ((size_t)ja[b3] + (size_t)cny * ka[c3]);
-        const float w = wi[a3] * wj[b3] * wk[c3];
-        num += w * xc[cc];
-        den += w * mc[cc];
-      }
-  return den > 0.f ? num / den : 0.f;
+  mgc_restrict_body(blk3(), nx, ny, nz, gE, gN, gT, b, x, cnx, cny, out);
 }
 
 __global__ __launch_bounds__(256) void k_mgc_prolong(int nx, int ny, const float *__restrict__ m,
                                                      float *__restrict__ x, int cnx, int cny,
                                                      int cnz, const float *__restrict__ xc,
                                                      const float *__restrict__ mc) {
-  const Idx t = cell_idx(nx, ny);
-  if (!t.in) return;
-  if (!(m[t.c] > 0.f)) return;
-  x[t.c] += prolong_value(t.i, t.j, t.k, cnx, cny, cnz, xc, mc);
+  mgc_prolong_body(blk3(), nx, ny, m, x, cnx, cny, cnz, xc, mc);
 }
 
-// Coarsest level (n <= 1024 cells, one workgroup): `sweeps` damped-Jacobi
-// sweeps from zero with the iterate in LDS; the first one is rd b, as the
-// restriction into a level forms it.
 __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
                                                        const float *__restrict__ gE,
                                                        const float *__restrict__ gN,
@@ -442,49 +78,9 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
                                                        float *__restrict__ x, float omega,
                                                        int sweeps) {
   __shared__ float xs[kMgCoarsestCells];
-  const int t = threadIdx.x;
-  const int n = nx * ny * nz;
-  const bool on = t < n;
-  int i = 0, j = 0, k = 0;
-  float gw = 0.f, ge = 0.f, gs = 0.f, gn = 0.f, gb = 0.f, gt = 0.f, bb = 0.f;
-  if (on) {
-    i = t % nx;
-    j = (t / nx) % ny;
-    k = t / (nx * ny);
-    ge = gE[t];
-    gn = gN[t];
-    gt = gT[t];
-    gw = i > 0 ? gE[t - 1] : 0.f;
-    gs = j > 0 ? gN[t - nx] : 0.f;
-    gb = k > 0 ? gT[t - nx * ny] : 0.f;
-    bb = b[t];
-  }
-  const float d = gw + ge + gs + gn + gb + gt;
-  float xv = on ? rd[t] * bb : 0.f;
-  const int tw = i > 0 ? t - 1 : t, te = i < nx - 1 ? t + 1 : t;
-  const int ts = j > 0 ? t - nx : t, tn = j < ny - 1 ? t + nx : t;
-  const int tb = k > 0 ? t - nx * ny : t, tt = k < nz - 1 ? t + nx * ny : t;
-  for (int s = 1; s < sweeps; ++s) {
-    if (on) xs[t] = xv;
-    __syncthreads();
-    if (on && d > 0.f) {
-      const float nb = gw * xs[tw] + ge * xs[te] + gs * xs[ts] + gn * xs[tn] + gb * xs[tb] +
-                       gt * xs[tt];
-      xv = xv + omega * (bb + nb - d * xv) / d;
-    }
-    __syncthreads();
-  }
-  if (on) x[t] = xv;
+  mgc_coarsest_body(xs, nx, ny, nz, gE, gN, gT, b, rd, x, omega, sweeps);
 }
 
-// Streaming damped-Jacobi sweep of level 0 in FP32 (the structure of the
-// reference-Jacobi kernel k_jacobi_t: each lane owns CL x-adjacent cells of
-// one row and walks its z range; plane k+1's loads are issued before plane k
-// is computed; west/east neighbours by wave shuffle):
-//   out = (1 - w) a + (w / d) (s + sum c a_nb), 0 where d = 0, s = r / V.
-// LAST writes out as z in FP64 and the partial sums z.r = z.(s V) and z.q (two
-// slots; q = A p_old from the previous update, FP32, or z.q = 0 when q is
-// null).
 template <int CL, bool LAST, bool NT>
 __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const float *__restrict__ a,
@@ -495,98 +91,10 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      float omega,
                                                      const float *__restrict__ q) {
   __shared__ double lds[16 * kReduceSlots];
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const Lane l = lane_of<CL>(t, nx, ny, nz, blockIdx.x);
-  double s = 0.0, sq = 0.0;
-  if (l.active) {
-    using V = Vec<float, CL>;
-    using VD = Vec<double, CL>;
-    using VS = Vec<float, CL>;
-    const int i = l.i, j = l.j;
-    const size_t nxy = (size_t)nx * ny;
-    float cw[CL], ce[CL], dxdy[CL];
-#pragma unroll
-    for (int m = 0; m < CL; ++m) {
-      cw[m] = g.cw[i + m];
-      ce[m] = g.ce[i + m];
-      dxdy[m] = g.dx[i + m] * g.dy[j];
-    }
-    const float cs = g.cs[j], cn = g.cn[j];
-    const float keep = 1.f - omega;
-    const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
-    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
-    const int lane = (int)threadIdx.x;
-    const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
-    const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : row0;
-    const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : row0;
-    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
-    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
-    auto at = [&](int k) { return base + nxy * (size_t)k; };
-    struct Set {
-      V a_next;
-      VS r;
-      Plane<kDirCurrent, float, CL> pl;
-    };
-    Set sa, sb;
-    V a_cur = ldv<float, CL>(a, at(k0));
-    V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
-    sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
-    sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
-    sa.r = ldv_s<NT, float, CL>(r, at(k0));
-    auto stage = [&](int k, const Set &u, Set &ld) {
-      const int kn = min(k + 1, k1 - 1);
-      ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
-      ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
-      ld.r = ldv_s<NT, float, CL>(r, at(kn));
-      const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
-      const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
-      const float vw = lane == 0 ? u.pl.w.a : from_w;
-      const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
-      const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
-      V bv, qz;
-      VD zv;
-      if constexpr (LAST) {
-        if (q) qz = ldv_s<NT, float, CL>(q, at(k));
-      }
-#pragma unroll
-      for (int m = 0; m < CL; ++m) {
-        const float west = m == 0 ? (hw ? vw : 0.f) : a_cur.v[m - 1];
-        const float east = m == CL - 1 ? (he ? ve : 0.f) : a_cur.v[m + 1];
-        const float south = hs ? u.pl.s.a.v[m] : 0.f, north = hn ? u.pl.n.a.v[m] : 0.f;
-        const float d = diag_code<float>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
-        const float src = u.r.v[m];
-        const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
-                         cb * a_prev.v[m] + ct * u.a_next.v[m];
-        const float o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
-        if constexpr (LAST) {
-          zv.v[m] = (double)o;
-          s += zv.v[m] * ((double)u.r.v[m] * (double)(dxdy[m] * dzk));
-          if (q) sq += zv.v[m] * (double)qz.v[m];
-        } else {
-          bv.v[m] = o;
-        }
-      }
-      if constexpr (LAST) stv_s<NT, double, CL>(z, at(k), zv);
-      else stv_s<NT, float, CL>(b, at(k), bv);
-      a_prev = a_cur;
-      a_cur = u.a_next;
-    };
-    for (int k = k0; k < k1; k += 2) {
-      stage(k, sa, sb);
-      if (k + 1 < k1) stage(k + 1, sb, sa);
-    }
-  }
-  if constexpr (LAST) {
-    double v[2] = {s, sq};
-    block_sum<2>(v, lds);
-    store_partials(partials, 2, v, gridDim.x, blockIdx.x);
-  }
+  mg_sweep_body<CL, LAST, NT>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, z, partials,
+                              omega, q);
 }
 
-// x += alpha p, r -= alpha A p (the fused PCG update body), the first
-// smoothing step of the next cycle, x1 = w D^-1 r, q = A p and the cycle's
-// source s = r / V, stored in FP32; with rmax the per-workgroup max |r / V|;
-// counts the iteration.
 template <int CL>
 __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    const double *__restrict__ p,
@@ -597,10 +105,8 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    float *__restrict__ q32,
                                                    float *__restrict__ s32) {
   __shared__ double lds[16 * kReduceSlots];
-  if (sc->status != 0) return;
-  pcg_upd_body<double, CL, float, false>(t, blockIdx.x, gridDim.x, g, p, code, x, r, x1, nullptr,
-                                         sc, lds, omega, rmax, q32, s32);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
+  mg_upd_body<CL>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32,
+                  s32);
 }
 
 // ---- setup --------------------------------------------------------------------
@@ -612,7 +118,7 @@ __global__ __launch_bounds__(256) void k_mg_coarsen0(GeomT<double> g,
                                                      float *__restrict__ gE,
                                                      float *__restrict__ gN,
                                                      float *__restrict__ gT) {
-  const Idx t = cell_idx(cnx, cny);
+  const Idx t = cell_idx(cnx, cny, blk3());
   if (!t.in) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   auto at = [&](int i, int j, int k) {
@@ -651,7 +157,7 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(int nx, int ny, int nz,
                                                     float *__restrict__ gE,
                                                     float *__restrict__ gN,
                                                     float *__restrict__ gT) {
-  const Idx t = cell_idx(cnx, cny);
+  const Idx t = cell_idx(cnx, cny, blk3());
   if (!t.in) return;
   auto at = [&](int i, int j, int k) {
     return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
@@ -682,7 +188,7 @@ __global__ __launch_bounds__(256) void k_mg_mask(int nx, int ny, const float *__
                                                  const float *__restrict__ gN,
                                                  const float *__restrict__ gT, float omega,
                                                  float *__restrict__ m, float *__restrict__ rd) {
-  const Idx t = cell_idx(nx, ny);
+  const Idx t = cell_idx(nx, ny, blk3());
   if (!t.in) return;
   const float d = coarse_diag(nx, ny, gE, gN, gT, t.c, t.i, t.j, t.k);
   m[t.c] = d > 0.f ? 1.f : 0.f;
@@ -696,8 +202,15 @@ float env_float(const char *name, float fallback) {
 
 }  // namespace
 
-int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
-             const uint8_t *code, const Tiling &tl, int cl, double *partials) {
+int mg_check_interval(int check_interval) {
+  const char *e = std::getenv("FFPM_MG_CHECK");
+  int c = e ? std::atoi(e) : 2;
+  if (check_interval > 0 && check_interval < c) c = check_interval;
+  return c < 1 ? 1 : c;
+}
+
+int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
+            const uint8_t *code, const Tiling &tl, int cl, double *partials) {
   mg.g64 = g64;
   mg.g32 = g32;
   mg.code = code;
@@ -737,35 +250,62 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     return ERR_SHAPE;
   }
   mg.levels = L + 1;
-  // one arena: level-0 x, y, q, s; per coarse level gE gN gT m rd x y b
-  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  size_t bytes = 4 * up(sizeof(float) * mg.lv[0].n);
-  for (int l = 1; l < mg.levels; ++l) bytes += 8 * up(sizeof(float) * mg.lv[l].n);
-  int st = 0;
-  char *base = (char *)workspace().get(S_MG, bytes, &st);
-  if (!base) return st;
-  auto take = [&](size_t b) {
-    char *p = base;
-    base += up(b);
-    return p;
-  };
-  mg.lv[0].x = (float *)take(sizeof(float) * mg.lv[0].n);
-  mg.lv[0].y = (float *)take(sizeof(float) * mg.lv[0].n);
-  mg.q32 = (float *)take(sizeof(float) * mg.lv[0].n);
-  mg.s32 = (float *)take(sizeof(float) * mg.lv[0].n);
+  return OK;
+}
+
+namespace {
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+float *carve(char *&base, size_t n) {
+  float *p = reinterpret_cast<float *>(base);
+  base += up256(sizeof(float) * n);
+  return p;
+}
+
+}  // namespace
+
+size_t mg_operator_bytes(const Multigrid &mg) {
+  size_t bytes = 0;
+  for (int l = 1; l < mg.levels; ++l) bytes += 5 * up256(sizeof(float) * mg.lv[l].n);
+  return bytes;
+}
+
+size_t mg_state_bytes(const Multigrid &mg) {
+  size_t bytes = 4 * up256(sizeof(float) * mg.lv[0].n);
+  for (int l = 1; l < mg.levels; ++l) bytes += 3 * up256(sizeof(float) * mg.lv[l].n);
+  return bytes;
+}
+
+void mg_bind_operator(Multigrid &mg, char *&base) {
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
-    const size_t b = sizeof(float) * v.n;
-    v.gE = (float *)take(b); v.gN = (float *)take(b); v.gT = (float *)take(b);
-    v.m = (float *)take(b); v.rd = (float *)take(b); v.x = (float *)take(b);
-    v.y = (float *)take(b);
-    v.b = (float *)take(b);
+    v.gE = carve(base, v.n);
+    v.gN = carve(base, v.n);
+    v.gT = carve(base, v.n);
+    v.m = carve(base, v.n);
+    v.rd = carve(base, v.n);
   }
-  // conductances and masks of the coarse levels
+}
+
+void mg_bind_state(Multigrid &mg, char *&base) {
+  mg.lv[0].x = carve(base, mg.lv[0].n);
+  mg.lv[0].y = carve(base, mg.lv[0].n);
+  mg.q32 = carve(base, mg.lv[0].n);
+  mg.s32 = carve(base, mg.lv[0].n);
+  for (int l = 1; l < mg.levels; ++l) {
+    MgLevel &v = mg.lv[l];
+    v.x = carve(base, v.n);
+    v.y = carve(base, v.n);
+    v.b = carve(base, v.n);
+  }
+}
+
+int mg_build(const Multigrid &mg) {
   {
     const MgLevel &c = mg.lv[1];
     k_mg_coarsen0<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMx, kMy), 0, stream()>>>(
-        g64, code, c.nx, c.ny, mg.scale, c.gE, c.gN, c.gT);
+        mg.g64, mg.code, c.nx, c.ny, mg.scale, c.gE, c.gN, c.gT);
     FFPM_LAUNCHED("k_mg_coarsen0");
   }
   for (int l = 2; l < mg.levels; ++l) {
@@ -781,6 +321,18 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     FFPM_LAUNCHED("k_mg_mask");
   }
   return OK;
+}
+
+int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
+             const uint8_t *code, const Tiling &tl, int cl, double *partials) {
+  int st = mg_plan(mg, g64, g32, code, tl, cl, partials);
+  if (st) return st;
+  char *base =
+      (char *)workspace().get(S_MG, mg_operator_bytes(mg) + mg_state_bytes(mg), &st);
+  if (!base) return st;
+  mg_bind_state(mg, base);
+  mg_bind_operator(mg, base);
+  return mg_build(mg);
 }
 
 namespace {

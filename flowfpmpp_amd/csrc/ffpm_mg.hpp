@@ -64,6 +64,25 @@ struct Multigrid {
 int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
              const uint8_t *code, const Tiling &tl, int cl, double *partials);
 
+// The steps of mg_setup for callers that place the arrays themselves (the
+// batched solves): parameters and level extents (no allocation); the bytes of
+// the level operators (gE gN gT m rd of the coarse levels, shared by solves
+// of one operator) and of one solve's cycle state (level-0 x y q s, coarse
+// x y b), every array 256-byte aligned; carving them from base (advanced past
+// them); and the launches that fill the level operators.
+int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
+            const uint8_t *code, const Tiling &tl, int cl, double *partials);
+size_t mg_operator_bytes(const Multigrid &mg);
+size_t mg_state_bytes(const Multigrid &mg);
+void mg_bind_operator(Multigrid &mg, char *&base);
+void mg_bind_state(Multigrid &mg, char *&base);
+int mg_build(const Multigrid &mg);
+
+// Convergence-check granularity of the multigrid CG: FFPM_MG_CHECK
+// iterations (default 2, at most the caller's check interval); the FP64 true
+// residual itself is formed only near convergence.
+int mg_check_interval(int check_interval);
+
 // The cycle's source s32 = r / V and the first smoothing step from zero,
 // lv[0].x = w D^-1 r.
 int mg_first(const Multigrid &mg, const double *r);

@@ -805,16 +805,6 @@ int fcg_iterate(Frame &f, const FcgArgs &a, double tolerance, int max_sweeps,
   return OK;
 }
 
-// Convergence-check granularity of the multigrid CG: FFPM_MG_CHECK
-// iterations (default 2, at most the caller's check interval); the FP64 true
-// residual itself is formed only near convergence (see fcg_mg_iterate).
-int mg_check_interval(int check_interval) {
-  const char *e = std::getenv("FFPM_MG_CHECK");
-  int c = e ? std::atoi(e) : 2;
-  if (check_interval > 0 && check_interval < c) c = check_interval;
-  return c < 1 ? 1 : c;
-}
-
 struct MgStats {
   int levels = 0;
   int checks = 0;

@@ -245,13 +245,14 @@ def _structs(problems):
 
 def solve_stairstep_batch(problems, tolerance, max_sweeps, check_interval, rows=0, zchunk=0,
                           use_graph=True, method="pcg"):
-    """PCG ('pcg', FP64) or mixed-precision PCG ('pcg-mixed') on many problems
-    in one batch (per-problem convergence masks, HIP-graph chunks). Each
-    problem ends as its own solve_stairstep_poisson_3d with the same method
-    would; sweeps, residual and the time to its convergence (ms from the batch
-    start) are stored on it."""
-    if method not in ("pcg", "pcg-mixed"):
-        raise ValueError("batched solves support 'pcg' and 'pcg-mixed'")
+    """PCG ('pcg', FP64), mixed-precision PCG ('pcg-mixed') or the
+    multigrid-preconditioned CG ('fcg-mg') on many problems in one batch
+    (per-problem convergence masks, HIP-graph chunks). Each problem ends as its
+    own solve_stairstep_poisson_3d with the same method would; sweeps, residual
+    and the time to its convergence (ms from the batch start) are stored on
+    it. rows and zchunk apply to the PCG methods."""
+    if method not in ("pcg", "pcg-mixed", "fcg-mg"):
+        raise ValueError("batched solves support 'pcg', 'pcg-mixed' and 'fcg-mg'")
     arr = _structs(problems)
     check(lib().ffpm_solve_stairstep_batch_dev(arr, len(problems), C.c_double(tolerance),
                                                 int(max_sweeps), int(check_interval),

@@ -156,3 +156,39 @@ def test_batch_errors():
         S.solve_stairstep_batch(f.problems(), 1e-10, 100, 10, rows=3)
     with pytest.raises(ValueError):
         S.solve_stairstep_batch(f.problems(), 1e-10, 100, 10, method="jacobi")
+
+
+def _multigrid_frames():
+    # hierarchy depths 1 (a frame of at most 1024 cells), 2 and 3; nx a
+    # multiple of 4, even only, and odd (all level-0 launch widths and both
+    # level-1 column variants)
+    return _uneven_frames() + [Frame(12, 10, 8, [W.Sphere((0.0, 0.0, 0.0), 1.2)], seed=15,
+                                     dirs=(0, 2))]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_multigrid_batch_equals_standalone(use_graph):
+    """The multigrid-preconditioned CG in a batch: frames of different sizes
+    and hierarchy depths, one launch per cycle step over every solve that has
+    the level; each problem ends bit-identical to its standalone fcg-mg solve
+    (same iterations, same residual, identical phi)."""
+    print(f"case: uneven fcg-mg batch, graph {use_graph}", flush=True)
+    frames = _multigrid_frames()
+    ref = [r for fr in frames for r in fr.standalone(1e-10, 100000, 25, method="fcg-mg")]
+    probs = [q for fr in frames for q in fr.problems()]
+    S.solve_stairstep_batch(probs, 1e-10, 100000, 25, use_graph=use_graph, method="fcg-mg")
+    _assert_same(probs, ref)
+    assert all(q.residual <= 1e-10 for q in probs)
+    assert all(0 < q.sweeps < 200 for q in probs)
+
+
+def test_multigrid_batch_fixed_work():
+    """Fixed work (negative tolerance) with a cap that is not a multiple of the
+    two-iteration chunk: every problem runs exactly the cap, as standalone."""
+    print("case: fcg-mg batch, fixed work 7 iterations", flush=True)
+    frames = _multigrid_frames()[:3]
+    ref = [r for fr in frames for r in fr.standalone(-1.0, 7, 25, method="fcg-mg")]
+    probs = [q for fr in frames for q in fr.problems()]
+    S.solve_stairstep_batch(probs, -1.0, 7, 25, method="fcg-mg")
+    _assert_same(probs, ref)
+    assert all(q.sweeps == 7 for q in probs)
