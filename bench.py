@@ -131,12 +131,13 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=25.0):
 
 def bench_batch(args):
     """64 uneven frames (grids from 64^3 to 160x128x96, 1-3 bodies each) x 3
-    force directions = 192 PCG solves to the tolerance in one batched solve
-    (shared operator per frame, per-solve masks, HIP-graph chunks, autotuned
-    launch shape): FP64 with --method pcg, FP32 inner / FP64 outer otherwise.
-    A step solves all 192 from phi = 0. With --method fcg-mg the same 192
-    solves are also run one after another with the multigrid CG
-    ("fcg_mg_sequential")."""
+    force directions = 192 solves to the tolerance in one batched solve
+    (shared operator per frame, per-solve masks, HIP-graph chunks). A step
+    solves all 192 from phi = 0. With --method fcg-mg (the default) the line's
+    value is the batched multigrid-preconditioned CG, and the same 192 solves
+    are also timed as one batched mixed-precision PCG (autotuned launch shape,
+    "pcg_mixed_batched") and as 192 fcg-mg solves one after another
+    ("fcg_mg_sequential"); --method pcg / pcg-mixed time that batched PCG."""
     from flowfpmpp_amd import stairstep as S
     from flowfpmpp_amd import workloads as W
     from flowfpmpp_amd.device import DeviceArray, synchronize
@@ -158,55 +159,65 @@ def bench_batch(args):
     cells = sum(p.cells for p, _ in frames) * 3
     synchronize()
     log(f"64 frames x 3 directions resident: {cells} cells, {time.perf_counter() - t_gen:.1f} s")
-    bmethod = args.method if args.method in ("pcg", "pcg-mixed") else "pcg-mixed"
-    rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20, method=bmethod)
-    log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
+    bmethod = args.method if args.method in ("pcg", "pcg-mixed", "fcg-mg") else "pcg-mixed"
 
-    def step():
-        for q in probs:
-            q.phi.zero_()
-        S.solve_stairstep_batch(probs, args.tol, args.max_iters, args.check, rows=rows,
-                                zchunk=zchunk, method=bmethod)
-
-    for _ in range(args.warmup):
-        step()
-    synchronize()
-    times = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        step()
+    def run_batch(method, steps, warmup, rows=0, zchunk=0):
+        def step():
+            for q in probs:
+                q.phi.zero_()
+            S.solve_stairstep_batch(probs, args.tol, args.max_iters, args.check, rows=rows,
+                                    zchunk=zchunk, method=method)
+        for _ in range(warmup):
+            step()
         synchronize()
-        times.append(time.perf_counter() - t0)
-    ms_per_step = 1e3 * sum(times) / len(times)
-    solve_ms = np.array([q.solve_ms for q in probs])
-    sweeps = np.array([q.sweeps for q in probs])
-    res = np.array([q.residual for q in probs])
-    log(f"batch: {ms_per_step:.1f} ms for 192 solves; iterations {sweeps.min()}-{sweeps.max()}")
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            step()
+            synchronize()
+            times.append(time.perf_counter() - t0)
+        ms = 1e3 * sum(times) / len(times)
+        solve_ms = np.array([q.solve_ms for q in probs])
+        sweeps = np.array([q.sweeps for q in probs])
+        res = np.array([q.residual for q in probs])
+        log(f"batched {method}: {ms:.1f} ms for 192 solves; iterations "
+            f"{sweeps.min()}-{sweeps.max()}")
+        return {"ms": round(ms, 3), "solves_per_s": round(192 * 1e3 / ms, 3),
+                "solve_ms_p50": round(float(np.percentile(solve_ms, 50)), 3),
+                "solve_ms_p99": round(float(np.percentile(solve_ms, 99)), 3),
+                "iterations_min_max": [int(sweeps.min()), int(sweeps.max())],
+                "max_residual": float(res.max()), "converged": bool((res <= args.tol).all())}
+
+    rows, zchunk, ms_it = S.autotune_stairstep_batch(
+        probs, iterations=10 if bmethod == "fcg-mg" else 20, method=bmethod)
+    log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
+    main = run_batch(bmethod, args.steps, args.warmup, rows, zchunk)
     out = {
         "metric": f"batched solves/s: 64 uneven frames x 3 directions, {bmethod} to 1e-10",
-        "value": round(192 * 1e3 / ms_per_step, 3), "unit": "solves/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "value": main["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": main["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if bmethod == "pcg" else "f32+f64",
+        "dtype": {"pcg": "f64", "pcg-mixed": "f32+f64", "fcg-mg": "f64+f32"}[bmethod],
         "data": "synthetic",
         "config": {"workload": "64 frames (seeds 4000-4063), grids 64^3/96x80x64/128x96x64/"
                                "160x128x96, 1-3 bodies, 3 directions each; one batched "
-                               f"{bmethod} solve, check 100",
+                               f"{bmethod} solve (autotuned launch shape), check {args.check}",
                    "cells": cells, "tolerance": args.tol, "rows": rows, "zchunk": zchunk},
-        "solve_ms_p50": round(float(np.percentile(solve_ms, 50)), 3),
-        "solve_ms_p99": round(float(np.percentile(solve_ms, 99)), 3),
-        "iterations_min_max": [int(sweeps.min()), int(sweeps.max())],
-        "max_residual": float(res.max()),
-        "converged": bool((res <= args.tol).all()),
-        "autotune_ms_per_iteration": round(ms_it, 4),
     }
-    if args.method == "fcg-mg":
-        for _ in range(max(1, args.warmup)):
-            for q in probs[:3]:
-                q.phi.zero_()
-                S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
-                                             *q.axes, args.tol, args.max_iters, args.check,
-                                             method="fcg-mg")
+    out.update({k: main[k] for k in ("solve_ms_p50", "solve_ms_p99", "iterations_min_max",
+                                     "max_residual", "converged")})
+    out["autotune_ms_per_iteration"] = round(ms_it, 4)
+    if bmethod == "fcg-mg":
+        rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20,
+                                                         method="pcg-mixed")
+        mixed = run_batch("pcg-mixed", 1, 1, rows, zchunk)
+        mixed.update(rows=rows, zchunk=zchunk)
+        out["pcg_mixed_batched"] = mixed
+        for q in probs[:3]:
+            q.phi.zero_()
+            S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
+                                         *q.axes, args.tol, args.max_iters, args.check,
+                                         method="fcg-mg")
         synchronize()
         per, its, rs = [], [], []
         t0 = time.perf_counter()
@@ -229,7 +240,8 @@ def bench_batch(args):
             "max_residual": float(max(rs)), "converged": bool(max(rs) <= args.tol)}
         log(f"fcg-mg one after another: {seq * 1e3:.1f} ms for 192 solves, iterations "
             f"{min(its)}-{max(its)}")
-    if args.sequential:
+        out["batch_speedup_vs_sequential"] = round(seq * 1e3 / main["ms"], 3)
+    elif args.sequential:
         t0 = time.perf_counter()
         for (p, dirs) in frames:
             for q in dirs:
@@ -240,7 +252,7 @@ def bench_batch(args):
         synchronize()
         seq = time.perf_counter() - t0
         out["sequential_ms"] = round(seq * 1e3, 3)
-        out["batch_speedup_vs_sequential"] = round(seq * 1e3 / ms_per_step, 3)
+        out["batch_speedup_vs_sequential"] = round(seq * 1e3 / main["ms"], 3)
     print(json.dumps(out), flush=True)
 
 

@@ -1405,8 +1405,9 @@ int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int coun
     set_error(ERR_ARGUMENT, "autotune needs a batch, an iteration count and outputs");
     return ERR_ARGUMENT;
   }
-  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_PCG_MIXED) {
-    set_error(ERR_METHOD, "batched solves support pcg and pcg-mixed");
+  if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_PCG_MIXED &&
+      method != FFPM_METHOD_FCG_MG) {
+    set_error(ERR_METHOD, "batched solves support pcg, pcg-mixed and fcg-mg");
     return ERR_METHOD;
   }
   static const int kRows[] = {4, 8};
@@ -1416,7 +1417,8 @@ int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int coun
   for (int r : kRows) {
     for (int zc : kChunks) {
       Batch B;
-      int st = batch_setup(B, problems, count, r, zc, method == FFPM_METHOD_PCG_MIXED);
+      int st = batch_setup(B, problems, count, r, zc, method == FFPM_METHOD_PCG_MIXED,
+                           method == FFPM_METHOD_FCG_MG);
       if (st) return st;
       std::vector<BState> out;
       std::vector<double> replay_ms;

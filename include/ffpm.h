@@ -139,11 +139,14 @@ int ffpm_last_mg_solve(int *levels, int *checks);
  * convergence masks, one HIP graph per check interval) and each ends exactly
  * as its own FP64 PCG solve would: same iterations, same phi. Problems with
  * the same solid and axis pointers (e.g. the three force directions of a
- * frame) share one operator. method is FFPM_METHOD_PCG or _PCG_MIXED (each
+ * frame) share one operator. method is FFPM_METHOD_PCG, _PCG_MIXED (each
  * problem then ends as its standalone mixed solve, sweeps = FP32 inner
- * iterations). rows/zchunk fix the PCG workgroup height (0, 4 or 8) and z
- * chunk (0 = the standalone defaults); solve_ms is the time from the batch
- * start until the problem converged. */
+ * iterations) or _FCG_MG (each problem ends as its standalone multigrid CG;
+ * a frame's problems also share the multigrid level operators). rows/zchunk
+ * fix the workgroup height (0, 4 or 8) and z chunk of the streaming kernels
+ * (0 = the standalone defaults, with which each problem is bit-identical to
+ * its standalone solve; other shapes change only the reduction order);
+ * solve_ms is the time from the batch start until the problem converged. */
 typedef struct ffpm_stairstep_problem {
   int nx, ny, nz;
   double *phi;             /* (0:nx+1,0:ny+1,nz), in/out */
@@ -159,9 +162,10 @@ typedef struct ffpm_stairstep_problem {
 int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
                                    double tolerance, int max_sweeps, int check_interval,
                                    int method, int rows, int zchunk, int use_graph);
-/* Times `iterations` batched iterations (FP64 PCG, or FP32 inner iterations
- * for PCG_MIXED) for each candidate launch shape (rows 4, 8 x z chunks 8, 16,
- * 32) and returns the fastest; phi is not written. */
+/* Times `iterations` batched iterations (FP64 PCG, FP32 inner iterations for
+ * PCG_MIXED, multigrid CG iterations for FCG_MG) for each candidate launch
+ * shape (rows 4, 8 x z chunks 8, 16, 32) and returns the fastest; phi is not
+ * written. */
 int ffpm_autotune_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count, int method,
                                       int iterations, int *rows, int *zchunk,
                                       double *ms_per_iteration);

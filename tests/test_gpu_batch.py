@@ -192,3 +192,22 @@ def test_multigrid_batch_fixed_work():
     S.solve_stairstep_batch(probs, -1.0, 7, 25, method="fcg-mg")
     _assert_same(probs, ref)
     assert all(q.sweeps == 7 for q in probs)
+
+
+def test_multigrid_batch_autotuned_shape():
+    """The autotuner times the multigrid CG's launch shapes too; a batch solved
+    with the chosen shape (other reduction order than the standalone solves)
+    converges to the same fields within 1e-10 relative."""
+    print("case: autotune uneven fcg-mg batch", flush=True)
+    frames = _multigrid_frames()
+    probs = [q for fr in frames for q in fr.problems()]
+    rows, zchunk, ms = S.autotune_stairstep_batch(probs, iterations=6, method="fcg-mg")
+    assert rows in (4, 8) and zchunk in (8, 16, 32) and ms > 0
+    assert all(not np.any(q.phi.download()) for q in probs)  # phi untouched
+    S.solve_stairstep_batch(probs, 1e-10, 100000, 25, rows=rows, zchunk=zchunk,
+                            method="fcg-mg")
+    ref = [r for fr in frames for r in fr.standalone(1e-10, 100000, 25, method="fcg-mg")]
+    for q, (phi, sw, r) in zip(probs, ref):
+        assert q.residual <= 1e-10
+        assert abs(q.sweeps - sw) <= 2
+        assert np.abs(q.phi.download() - phi).max() <= 1e-10 * np.abs(phi).max()
