@@ -393,7 +393,7 @@ __global__ void k_bm_check_end(const BSolve *solves, BState *state, int count, i
 // Cell and column launches of level-l arrays: each workgroup decodes its
 // (x, y, z) coordinates in its own solve's launch from its local index.
 __device__ __forceinline__ mgk::Blk cell_blk(const MgLevel &v, int local) {
-  return mgk::blk_of(local, (v.nx + mgk::kMx - 1) / mgk::kMx, (v.ny + mgk::kMy - 1) / mgk::kMy);
+  return mgk::blk_of(local, mgk::plane_blocks(v.nx, v.ny), 1);
 }
 
 // The cycle's source and first smoothing step after a new r (the first slot
@@ -792,11 +792,12 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
   std::vector<Tiling> tl(count), tl32(count), tt(count);
   std::vector<int> eb(count);
   std::vector<Multigrid> plan(mgm ? count : 0);
-  size_t grid_blocks = 0;
+  size_t grid_blocks = 0, all_cells = 0;
   auto cells3 = [](const dim3 &g) { return (size_t)g.x * g.y * g.z; };
   for (int s = 0; s < count; ++s) {
     const ffpm_stairstep_problem &p = P[s];
     const size_t n = (size_t)p.nx * p.ny * p.nz;
+    all_cells += n;
     B.cl[s] = cells_per_lane_f64(p.nx);
     B.cl32[s] = cells_per_lane_f32(p.nx);
     tl[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl[s], rows, zchunk);
@@ -937,7 +938,11 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
       char *sb = a.take<char>(mg_state_bytes(mg));
       mg_bind_state(mg, sb);
       d.mg = mg;
-      d.kc1 = mgk::column_chunk(mg.lv[1].nx, mg.lv[1].ny, mg.lv[1].nz);
+      // the column launches' wavefront target, shared among the solves by
+      // cells (the K chunk does not change any value)
+      const long target = std::max(1L, (long)((double)mgk::column_waves() * (double)n /
+                                              (double)all_cells));
+      d.kc1 = mgk::column_chunk(mg.lv[1].nx, mg.lv[1].ny, mg.lv[1].nz, target);
     }
     // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379); a frame's later
     // right-hand sides rewrite the identical diag, rdiag and code.
@@ -1134,7 +1139,7 @@ int launch_check_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, doubl
 template <bool NT>
 int launch_cycle_mg(const Batch &B, hipStream_t s, int m) {
   const Multigrid &h = B.h[0].mg;
-  const dim3 cblk(mgk::kMx, mgk::kMy), sblk(kTileX, B.h[0].tl32.ty);
+  const dim3 cblk(mgk::kMt), sblk(kTileX, B.h[0].tl32.ty);
   auto sweep0 = [&](bool last, int flip) -> int {
     for (int c : {1, 2, 4}) {
       const Grid &g = B.g_pcg32[c];
@@ -1209,7 +1214,7 @@ int launch_cycle_mg(const Batch &B, hipStream_t s, int m) {
 // alpha, update), then the check logic and the true residual where needed.
 int launch_chunk_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol) {
   const int cb = (B.count + 63) / 64;
-  const dim3 cblk(mgk::kMx, mgk::kMy), pblk(kTileX, B.h[0].tl.ty);
+  const dim3 cblk(mgk::kMt), pblk(kTileX, B.h[0].tl.ty);
   int st;
   for (int m = 0; m < ci; ++m) {
     if (m == 0) {

@@ -304,19 +304,19 @@ void mg_bind_state(Multigrid &mg, char *&base) {
 int mg_build(const Multigrid &mg) {
   {
     const MgLevel &c = mg.lv[1];
-    k_mg_coarsen0<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMx, kMy), 0, stream()>>>(
+    k_mg_coarsen0<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMt), 0, stream()>>>(
         mg.g64, mg.code, c.nx, c.ny, mg.scale, c.gE, c.gN, c.gT);
     FFPM_LAUNCHED("k_mg_coarsen0");
   }
   for (int l = 2; l < mg.levels; ++l) {
     const MgLevel &f = mg.lv[l - 1], &c = mg.lv[l];
-    k_mg_coarsen<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMx, kMy), 0, stream()>>>(
+    k_mg_coarsen<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMt), 0, stream()>>>(
         f.nx, f.ny, f.nz, f.gE, f.gN, f.gT, c.nx, c.ny, mg.scale, c.gE, c.gN, c.gT);
     FFPM_LAUNCHED("k_mg_coarsen");
   }
   for (int l = 1; l < mg.levels; ++l) {
     const MgLevel &c = mg.lv[l];
-    k_mg_mask<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMx, kMy), 0, stream()>>>(
+    k_mg_mask<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMt), 0, stream()>>>(
         c.nx, c.ny, c.gE, c.gN, c.gT, mg.omega_coarse, c.m, c.rd);
     FFPM_LAUNCHED("k_mg_mask");
   }
@@ -365,7 +365,7 @@ void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b,
 
 int mg_first(const Multigrid &mg, const double *r) {
   const MgLevel &f0 = mg.lv[0];
-  k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMx, kMy), 0, stream()>>>(
+  k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMt), 0, stream()>>>(
       mg.g32, mg.code, r, f0.x, mg.s32, mg.omega);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
@@ -401,7 +401,7 @@ CoarseOut coarse_out(const Multigrid &mg, int l) {
 }  // namespace
 
 int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
-  const dim3 blk(kMx, kMy);
+  const dim3 blk(kMt);
   const float *r = mg.s32;
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;

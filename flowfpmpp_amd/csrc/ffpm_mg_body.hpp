@@ -3,9 +3,10 @@
 // one solve per launch) and the batched ones (ffpm_batch.hip, many solves per
 // launch): every body takes its workgroup's coordinates within its own
 // solve's launch, so a batched solve computes exactly what its standalone
-// solve computes. Cell bodies run 64 x 4 workgroups over (x, y) tiles with
-// one z plane per grid row, so no index divisions; all loads are
-// unconditional with clamped indices and masked coefficients.
+// solve computes. Cell and column bodies run 256-thread workgroups over the
+// cells (or coarse columns) of one z plane taken in storage order, so every
+// lane has work on narrow levels too; all loads are unconditional with
+// clamped indices and masked coefficients.
 #pragma once
 
 #include <cstdlib>
@@ -16,21 +17,30 @@
 namespace ffpm {
 namespace mgk {
 
-constexpr int kMx = 64, kMy = 4;
+constexpr int kMt = 256;  // threads of a cell or column workgroup
+
+// Workgroups per z plane of an nx x ny level (the grid is that x 1 x planes).
+__host__ __device__ inline int plane_blocks(int nx, int ny) { return (nx * ny + kMt - 1) / kMt; }
 
 inline dim3 cell_grid(int nx, int ny, int nz) {
-  return dim3((unsigned)((nx + kMx - 1) / kMx), (unsigned)((ny + kMy - 1) / kMy), (unsigned)nz);
+  return dim3((unsigned)plane_blocks(nx, ny), 1u, (unsigned)nz);
 }
 
-// Column kernels (one thread per coarse (I, J) walking K): K chunk so that
-// the launch has at least FFPM_MG_COLUMN_WAVES (default 16384) wavefronts.
-inline int column_chunk(int cnx, int cny, int cnz) {
+// Wavefronts the column launches aim for: FFPM_MG_COLUMN_WAVES, default 16384.
+inline long column_waves() {
   static const long target = [] {
     const char *e = std::getenv("FFPM_MG_COLUMN_WAVES");
     const long v = e ? std::atol(e) : 16384;
     return v < 1 ? 1L : v;
   }();
-  const long waves_per_plane = (long)((cnx + kMx - 1) / kMx) * ((cny + kMy - 1) / kMy) * 4;
+  return target;
+}
+
+// Column kernels (one thread per coarse (I, J) walking K): K chunk so that
+// the launch has at least `target` wavefronts (column_waves() for one solve
+// per launch; a batch shares the target among its solves).
+inline int column_chunk(int cnx, int cny, int cnz, long target = column_waves()) {
+  const long waves_per_plane = (long)plane_blocks(cnx, cny) * (kMt / 64);
   long chunks = (target + waves_per_plane - 1) / waves_per_plane;
   if (chunks < 1) chunks = 1;
   if (chunks > cnz) chunks = cnz;
@@ -38,8 +48,7 @@ inline int column_chunk(int cnx, int cny, int cnz) {
 }
 
 inline dim3 column_grid(int cnx, int cny, int cnz, int kchunk) {
-  return dim3((unsigned)((cnx + kMx - 1) / kMx), (unsigned)((cny + kMy - 1) / kMy),
-              (unsigned)((cnz + kchunk - 1) / kchunk));
+  return dim3((unsigned)plane_blocks(cnx, cny), 1u, (unsigned)((cnz + kchunk - 1) / kchunk));
 }
 
 // Workgroup coordinates of a cell or column launch: blockIdx in the
@@ -67,10 +76,11 @@ struct Idx {
 
 __device__ __forceinline__ Idx cell_idx(int nx, int ny, Blk bk) {
   Idx t;
-  t.i = bk.x * kMx + threadIdx.x;
-  t.j = bk.y * kMy + threadIdx.y;
+  const int p = bk.x * kMt + (int)threadIdx.x;  // index in the plane
+  t.j = p / nx;
+  t.i = p - t.j * nx;
   t.k = bk.z;
-  t.in = t.i < nx && t.j < ny;
+  t.in = t.j < ny;
   t.c = (size_t)t.i + (size_t)nx * ((size_t)t.j + (size_t)ny * t.k);
   return t;
 }
@@ -128,8 +138,9 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
                                                     const float *__restrict__ src,
                                                     const float *__restrict__ x, int cnx, int cny,
                                                     int cnz, int kchunk, CoarseOut out) {
-  const int I = bk.x * kMx + threadIdx.x, J = bk.y * kMy + threadIdx.y;
-  if (I >= cnx || J >= cny) return;
+  const int pc = bk.x * kMt + (int)threadIdx.x;  // coarse column in the plane
+  const int J = pc / cnx, I = pc - J * cnx;
+  if (J >= cny) return;
   const int K0 = bk.z * kchunk, K1 = min(cnz, K0 + kchunk);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t sx = (size_t)nx;
@@ -240,8 +251,9 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
                                                    float *__restrict__ x, int cnx, int cny, int cnz,
                                                    const float *__restrict__ xc,
                                                    const float *__restrict__ mc, int kchunk) {
-  const int I = bk.x * kMx + threadIdx.x, J = bk.y * kMy + threadIdx.y;
-  if (I >= cnx || J >= cny) return;
+  const int pc = bk.x * kMt + (int)threadIdx.x;  // coarse column in the plane
+  const int J = pc / cnx, I = pc - J * cnx;
+  if (J >= cny) return;
   const int K0 = bk.z * kchunk, K1 = min(cnz, K0 + kchunk);
   const bool vi[3] = {I > 0, true, I + 1 < cnx}, vj[3] = {J > 0, true, J + 1 < cny};
   const int ii[3] = {vi[0] ? I - 1 : I, I, vi[2] ? I + 1 : I};
