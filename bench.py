@@ -188,9 +188,12 @@ def bench_batch(args):
                 "iterations_min_max": [int(sweeps.min()), int(sweeps.max())],
                 "max_residual": float(res.max()), "converged": bool((res <= args.tol).all())}
 
-    rows, zchunk, ms_it = S.autotune_stairstep_batch(
-        probs, iterations=10 if bmethod == "fcg-mg" else 20, method=bmethod)
-    log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
+    if args.shape:
+        (rows, zchunk), ms_it = args.shape, None
+    else:
+        rows, zchunk, ms_it = S.autotune_stairstep_batch(
+            probs, iterations=10 if bmethod == "fcg-mg" else 20, method=bmethod)
+        log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
     main = run_batch(bmethod, args.steps, args.warmup, rows, zchunk)
     out = {
         "metric": f"batched solves/s: 64 uneven frames x 3 directions, {bmethod} to 1e-10",
@@ -201,13 +204,15 @@ def bench_batch(args):
         "data": "synthetic",
         "config": {"workload": "64 frames (seeds 4000-4063), grids 64^3/96x80x64/128x96x64/"
                                "160x128x96, 1-3 bodies, 3 directions each; one batched "
-                               f"{bmethod} solve (autotuned launch shape), check {args.check}",
+                               f"{bmethod} solve ({'given' if args.shape else 'autotuned'} "
+                               f"launch shape), check {args.check}",
                    "cells": cells, "tolerance": args.tol, "rows": rows, "zchunk": zchunk},
     }
     out.update({k: main[k] for k in ("solve_ms_p50", "solve_ms_p99", "iterations_min_max",
                                      "max_residual", "converged")})
-    out["autotune_ms_per_iteration"] = round(ms_it, 4)
-    if bmethod == "fcg-mg":
+    if ms_it is not None:
+        out["autotune_ms_per_iteration"] = round(ms_it, 4)
+    if bmethod == "fcg-mg" and not args.no_compare:
         rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20,
                                                          method="pcg-mixed")
         mixed = run_batch("pcg-mixed", 1, 1, rows, zchunk)
@@ -413,6 +418,10 @@ def main():
                          "--max-iters sweeps in chunks of --check, extrapolated to --tol")
     ap.add_argument("--sequential", action="store_true",
                     help="batch64: also time the 192 solves one after another")
+    ap.add_argument("--shape", type=int, nargs=2, metavar=("ROWS", "ZCHUNK"),
+                    help="batch64: this launch shape instead of the autotuned one")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch64 with fcg-mg: skip the batched mixed-PCG and sequential runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole frame lives on one MI355X and this build "
