@@ -61,7 +61,8 @@ BYTES_PER_CELL = {
     "mg_fine_sweep": 13,     # x, s (FP32), code; write x (FP32)
     "mg_fine_restrict": 9,   # x, s, code per fine cell (coarse writes 1/8 of 4 B)
     "mg_fine_prolong": 9,    # x read/write, code (coarse reads 1/8)
-    "mg_fine_last": 21,      # x, s, q (FP32), code; write z (FP64)
+    "mg_fine_last": 17,      # x, s, q (FP32), code; write z (FP32)
+    "mg_direction": 21,      # read z (FP32), p_old, code; write p
 }
 
 

@@ -410,8 +410,8 @@ __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid gri
 }
 
 // Level-0 sweep from the iterate in x (flip 0) or y (flip 1) into the other;
-// LAST writes z (FP64) with the z.r and z.q partials (z.q only after the first
-// direction, as the standalone cycle).
+// LAST writes the cycle's output z32 with the z.r and z.q partials (z.q only
+// after the first direction, as the standalone cycle).
 template <int CL, bool LAST, bool NT>
 __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid grid,
                                                    const BState *state, int slot, int flip) {
@@ -422,10 +422,10 @@ __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid gr
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   const float *a = flip ? mg.lv[0].y : mg.lv[0].x;
-  float *b = flip ? mg.lv[0].x : mg.lv[0].y;
+  float *b = LAST ? mg.z32 : (flip ? mg.lv[0].x : mg.lv[0].y);
   const float *q = LAST && st.mode == kDirUpdate ? mg.q32 : nullptr;
   mgk::mg_sweep_body<CL, LAST, NT>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl, mg.g32, a,
-                                   mg.s32, mg.code, b, d.z, mg.partials, mg.omega, q);
+                                   mg.s32, mg.code, b, mg.partials, mg.omega, q);
 }
 
 template <bool VEC>
@@ -505,6 +505,27 @@ __global__ __launch_bounds__(1024) void k_bg_coarsest(const BSolve *solves, cons
   const MgLevel &c = mg.lv[mg.levels - 1];
   mgk::mgc_coarsest_body(xs, c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b, c.rd, c.y,
                          mg.omega_coarse, mg.coarse_sweeps);
+}
+
+// The direction step on the cycle's FP32 output (p = z or z + beta p_old,
+// q = A p, p.q), as the standalone mg_direction.
+template <int CL>
+__global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
+                                                const BState *state, int slot) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BState &st = state[s];
+  if (!iterating(st, slot)) return;
+  const BSolve &d = solves[s];
+  const int lb = grid.local[blockIdx.x];
+  const double *pin = d.p[st.cur];
+  double *pout = d.p[st.cur ^ 1];
+  if (st.mode == kDirUpdate)
+    pcg_dir_body<double, CL, kDirUpdate, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32, d.code,
+                                                pin, pout, d.partials, d.sc, lds);
+  else
+    pcg_dir_body<double, CL, kDirReset, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32, d.code,
+                                               pin, pout, d.partials, d.sc, lds);
 }
 
 // x += alpha p, r -= alpha A p and the next cycle's first step; the slot that
@@ -1229,9 +1250,9 @@ int launch_chunk_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, doubl
     for (int c : {1, 2}) {
       const Grid &g = B.g_pcg[c];
       if (!g.blocks) continue;
-      if (c == 2) k_b_dir<double, 2><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
-      else k_b_dir<double, 1><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
-      FFPM_LAUNCHED("k_b_dir");
+      if (c == 2) k_bg_dir<2><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      else k_bg_dir<1><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      FFPM_LAUNCHED("k_bg_dir");
     }
     k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_PQ, m);
     FFPM_LAUNCHED("k_bg_finish");

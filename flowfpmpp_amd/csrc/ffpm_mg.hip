@@ -86,13 +86,11 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const float *__restrict__ a,
                                                      const float *__restrict__ r,
                                                      const uint8_t *__restrict__ code,
-                                                     float *__restrict__ b,
-                                                     double *__restrict__ z, double *partials,
-                                                     float omega,
-                                                     const float *__restrict__ q) {
+                                                     float *__restrict__ b, double *partials,
+                                                     float omega, const float *__restrict__ q) {
   __shared__ double lds[16 * kReduceSlots];
-  mg_sweep_body<CL, LAST, NT>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, z, partials,
-                              omega, q);
+  mg_sweep_body<CL, LAST, NT>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, partials, omega,
+                              q);
 }
 
 template <int CL>
@@ -107,6 +105,20 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
   __shared__ double lds[16 * kReduceSlots];
   mg_upd_body<CL>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32,
                   s32);
+}
+
+// The flexible CG's direction step on the cycle's FP32 output (the PCG
+// direction body with z widened on load).
+template <int CL, int MODE>
+__global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<double> g,
+                                                   const float *__restrict__ z,
+                                                   const uint8_t *__restrict__ code,
+                                                   const double *__restrict__ pin,
+                                                   double *__restrict__ pout, double *partials,
+                                                   Scalars *sc) {
+  __shared__ double lds[16 * kReduceSlots];
+  pcg_dir_body<double, CL, MODE, float>(t, blockIdx.x, gridDim.x, g, z, code, pin, pout,
+                                        partials, sc, lds);
 }
 
 // ---- setup --------------------------------------------------------------------
@@ -272,7 +284,7 @@ size_t mg_operator_bytes(const Multigrid &mg) {
 }
 
 size_t mg_state_bytes(const Multigrid &mg) {
-  size_t bytes = 4 * up256(sizeof(float) * mg.lv[0].n);
+  size_t bytes = 5 * up256(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) bytes += 3 * up256(sizeof(float) * mg.lv[l].n);
   return bytes;
 }
@@ -293,6 +305,7 @@ void mg_bind_state(Multigrid &mg, char *&base) {
   mg.lv[0].y = carve(base, mg.lv[0].n);
   mg.q32 = carve(base, mg.lv[0].n);
   mg.s32 = carve(base, mg.lv[0].n);
+  mg.z32 = carve(base, mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
     v.x = carve(base, v.n);
@@ -339,26 +352,26 @@ namespace {
 
 // Level-0 streaming sweep launch for the multigrid's cells-per-lane choice.
 template <bool LAST, bool NT>
-void launch_sweep_nt(const Multigrid &mg, const float *a, const float *r, float *b, double *z,
+void launch_sweep_nt(const Multigrid &mg, const float *a, const float *r, float *b,
                      const float *q) {
   const dim3 blk(kTileX, mg.tl.ty);
   const float w = mg.omega;
   if (mg.cl == 4)
     k_mg_sweep_t<4, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
-                                                                  b, z, mg.partials, w, q);
+                                                                  b, mg.partials, w, q);
   else if (mg.cl == 2)
     k_mg_sweep_t<2, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
-                                                                  b, z, mg.partials, w, q);
+                                                                  b, mg.partials, w, q);
   else
     k_mg_sweep_t<1, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
-                                                                  b, z, mg.partials, w, q);
+                                                                  b, mg.partials, w, q);
 }
 
 template <bool LAST>
-void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b, double *z,
+void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b,
                   const float *q = nullptr) {
-  if (mg.nontemporal) launch_sweep_nt<LAST, true>(mg, a, r, b, z, q);
-  else launch_sweep_nt<LAST, false>(mg, a, r, b, z, q);
+  if (mg.nontemporal) launch_sweep_nt<LAST, true>(mg, a, r, b, q);
+  else launch_sweep_nt<LAST, false>(mg, a, r, b, q);
 }
 
 }  // namespace
@@ -400,7 +413,7 @@ CoarseOut coarse_out(const Multigrid &mg, int l) {
 
 }  // namespace
 
-int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
+int mg_vcycle(const Multigrid &mg, bool with_q) {
   const dim3 blk(kMt);
   const float *r = mg.s32;
   const MgLevel &f0 = mg.lv[0];
@@ -413,7 +426,7 @@ int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
   oth[0] = f0.y;
   for (int s = 1; s < mg.nu; ++s) {
     TimedLaunch tl("mg_fine_sweep");
-    launch_sweep<false>(mg, cur[0], r, oth[0], nullptr);
+    launch_sweep<false>(mg, cur[0], r, oth[0]);
     std::swap(cur[0], oth[0]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
@@ -478,15 +491,38 @@ int mg_vcycle(const Multigrid &mg, double *z, bool with_q) {
   FFPM_LAUNCHED("k_mgf_prolong_z");
   for (int s = 1; s < mg.nu; ++s) {
     TimedLaunch tl("mg_fine_sweep");
-    launch_sweep<false>(mg, cur[0], r, oth[0], nullptr);
+    launch_sweep<false>(mg, cur[0], r, oth[0]);
     std::swap(cur[0], oth[0]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
     TimedLaunch tl("mg_fine_last");
-    launch_sweep<true>(mg, cur[0], r, nullptr, z, with_q ? mg.q32 : nullptr);
+    launch_sweep<true>(mg, cur[0], r, mg.z32, with_q ? mg.q32 : nullptr);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
+  return OK;
+}
+
+int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const double *pin, double *pout,
+                 int mode, Scalars *sc) {
+  if (mode != kDirUpdate && mode != kDirReset) {
+    set_error(ERR_ARGUMENT, "multigrid direction mode");
+    return ERR_ARGUMENT;
+  }
+  const dim3 blk(kTileX, t.ty);
+  TimedLaunch tl("mg_direction");
+  auto go = [&](auto kern) {
+    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, mg.z32, mg.code, pin, pout, mg.partials,
+                                         sc);
+  };
+  if (cl == 2) {
+    if (mode == kDirUpdate) go(k_mg_dir_t<2, kDirUpdate>);
+    else go(k_mg_dir_t<2, kDirReset>);
+  } else {
+    if (mode == kDirUpdate) go(k_mg_dir_t<1, kDirUpdate>);
+    else go(k_mg_dir_t<1, kDirReset>);
+  }
+  FFPM_LAUNCHED("k_mg_dir_t");
   return OK;
 }
 

@@ -12,13 +12,14 @@
 // null space of the Neumann problem is kept. A cell carries an unknown when
 // its diagonal is positive.
 //
-// Cycle (FP32 arithmetic, FP64 input r and output z): nu damped-Jacobi
+// Cycle (FP32 arithmetic and output z, FP64 input r): nu damped-Jacobi
 // sweeps (weight 0.9) from zero (the first one fused into the outer update),
 // restriction of the residual by summation over the children, recursion,
 // trilinear cell-centred prolongation with weights renormalised over the
 // coarse cells that carry unknowns, nu damped-Jacobi sweeps (nu = 3 on level
-// 0 and 4 on the coarse levels by default, measured fastest at 512x384x256). The coarsest level (at most 1024 cells) runs a fixed
-// number of sweeps in one workgroup. Every step is a fixed linear map, so the
+// 0 and 4 on the coarse levels by default, measured fastest at 512x384x256).
+// The coarsest level (at most 1024 cells) runs a fixed number of sweeps in
+// one workgroup. Every step is a fixed linear map, so the
 // CG around it sees the same preconditioner at every iteration.
 #pragma once
 
@@ -56,6 +57,7 @@ struct Multigrid {
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
   float *s32 = nullptr;        // the cycle's source r / V (level-0 cells)
+  float *z32 = nullptr;        // the cycle's output z = B r (level-0 cells)
 };
 
 // Builds the hierarchy for the frame with FP64 geometry g64, its FP32 copy
@@ -67,7 +69,7 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 // The steps of mg_setup for callers that place the arrays themselves (the
 // batched solves): parameters and level extents (no allocation); the bytes of
 // the level operators (gE gN gT m rd of the coarse levels, shared by solves
-// of one operator) and of one solve's cycle state (level-0 x y q s, coarse
+// of one operator) and of one solve's cycle state (level-0 x y q s z, coarse
 // x y b), every array 256-byte aligned; carving them from base (advanced past
 // them); and the launches that fill the level operators.
 int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
@@ -94,9 +96,15 @@ int mg_first(const Multigrid &mg, const double *r);
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
               double *r, Scalars *sc, double *rmax = nullptr);
 
-// z = B r (the rest of the cycle after the first step, which must be in
+// z32 = B r (the rest of the cycle after the first step, which must be in
 // lv[0].x with its source in s32). Leaves two slots of partial sums
 // (tl.blocks each) in partials: z.r, and z.q32 when with_q (else 0).
-int mg_vcycle(const Multigrid &mg, double *z, bool with_q);
+int mg_vcycle(const Multigrid &mg, bool with_q);
+
+// The flexible CG's direction step with the cycle's FP32 output: p = z32 (MODE
+// kDirReset) or z32 + beta p_old (kDirUpdate), q = A p and the p.q partials
+// (t.blocks, FP64 launch shape t with cl cells per lane).
+int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const double *pin, double *pout,
+                 int mode, Scalars *sc);
 
 }  // namespace ffpm

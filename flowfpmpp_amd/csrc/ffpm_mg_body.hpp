@@ -511,23 +511,21 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
 // one row and walks its z range; plane k+1's loads are issued before plane k
 // is computed; west/east neighbours by wave shuffle):
 //   out = (1 - w) a + (w / d) (s + sum c a_nb), 0 where d = 0, s = r / V.
-// LAST writes out as z in FP64 and the partial sums z.r = z.(s V) and z.q (two
-// slots; q = A p_old from the previous update, FP32, or z.q = 0 when q is
-// null).
+// LAST: out is the cycle's output z (FP32; the FP64 iteration widens it
+// exactly), and the partial sums z.r = z.(s V) and z.q go to two slots (q = A
+// p_old from the previous update, FP32, or z.q = 0 when q is null).
 template <int CL, bool LAST, bool NT>
 __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *lds, Tiling t,
                                               GeomT<float> g, const float *__restrict__ a,
                                               const float *__restrict__ r,
                                               const uint8_t *__restrict__ code,
-                                              float *__restrict__ b, double *__restrict__ z,
-                                              double *partials, float omega,
-                                              const float *__restrict__ q) {
+                                              float *__restrict__ b, double *partials,
+                                              float omega, const float *__restrict__ q) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
   double s = 0.0, sq = 0.0;
   if (l.active) {
     using V = Vec<float, CL>;
-    using VD = Vec<double, CL>;
     using VS = Vec<float, CL>;
     const int i = l.i, j = l.j;
     const size_t nxy = (size_t)nx * ny;
@@ -571,7 +569,6 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
       const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
       const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
       V bv, qz;
-      VD zv;
       if constexpr (LAST) {
         if (q) qz = ldv_s<NT, float, CL>(q, at(k));
       }
@@ -585,16 +582,14 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
         const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
                          cb * a_prev.v[m] + ct * u.a_next.v[m];
         const float o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
+        bv.v[m] = o;
         if constexpr (LAST) {
-          zv.v[m] = (double)o;
-          s += zv.v[m] * ((double)u.r.v[m] * (double)(dxdy[m] * dzk));
-          if (q) sq += zv.v[m] * (double)qz.v[m];
-        } else {
-          bv.v[m] = o;
+          const double zd = (double)o;
+          s += zd * ((double)u.r.v[m] * (double)(dxdy[m] * dzk));
+          if (q) sq += zd * (double)qz.v[m];
         }
       }
-      if constexpr (LAST) stv_s<NT, double, CL>(z, at(k), zv);
-      else stv_s<NT, float, CL>(b, at(k), bv);
+      stv_s<NT, float, CL>(b, at(k), bv);
       a_prev = a_cur;
       a_cur = u.a_next;
     };

@@ -16,6 +16,8 @@
 // computed, and every value moves in 16-byte vectors where the layout allows.
 #pragma once
 
+#include <type_traits>
+
 #include "ffpm_pcg.hpp"
 
 namespace ffpm {
@@ -93,37 +95,40 @@ __device__ __forceinline__ void stv_s(T *__restrict__ a, size_t c, const Vec<T, 
 // kDirReset, p_old for kDirCurrent), combined only when used, so the loads of
 // a whole plane can be issued one plane ahead. The mode is a template
 // argument: with a run-time mode the compiler puts every load pair in its own
-// branch and waits for it there, which serialises the loads of a plane.
-template <int MODE, typename T, int CL>
+// branch and waits for it there, which serialises the loads of a plane. z may
+// be stored in a narrower type ZT (the multigrid cycle's FP32 output), widened
+// exactly on use (kDirCurrent, which loads p_old into a, needs ZT = T).
+template <int MODE, typename T, int CL, typename ZT = T>
 struct RawV {
-  Vec<T, CL> a, b;
-  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+  static_assert(MODE != kDirCurrent || std::is_same<ZT, T>::value, "p_old is T");
+  Vec<ZT, CL> a;  // z (p_old for kDirCurrent)
+  Vec<T, CL> b;   // p_old for kDirUpdate
+  __device__ __forceinline__ void load(const ZT *__restrict__ z, const T *__restrict__ pin,
                                        size_t c) {
     if constexpr (MODE == kDirCurrent) {
-      a = ldv<T, CL>(pin, c);
+      a = ldv<ZT, CL>(pin, c);
     } else if constexpr (MODE == kDirReset) {
-      a = ldv<T, CL>(z, c);
+      a = ldv<ZT, CL>(z, c);
     } else {
-      a = ldv<T, CL>(z, c);
+      a = ldv<ZT, CL>(z, c);
       b = ldv<T, CL>(pin, c);
     }
   }
   __device__ __forceinline__ Vec<T, CL> value(T beta) const {
-    if constexpr (MODE != kDirUpdate) {
-      return a;
-    } else {
-      Vec<T, CL> v;
+    Vec<T, CL> v;
 #pragma unroll
-      for (int m = 0; m < CL; ++m) v.v[m] = a.v[m] + beta * b.v[m];
-      return v;
-    }
+    for (int m = 0; m < CL; ++m)
+      v.v[m] = MODE == kDirUpdate ? (T)a.v[m] + beta * b.v[m] : (T)a.v[m];
+    return v;
   }
 };
 
-template <int MODE, typename T>
+template <int MODE, typename T, typename ZT = T>
 struct Raw1 {
-  T a, b;
-  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+  static_assert(MODE != kDirCurrent || std::is_same<ZT, T>::value, "p_old is T");
+  ZT a;  // z (p_old for kDirCurrent)
+  T b;   // p_old for kDirUpdate
+  __device__ __forceinline__ void load(const ZT *__restrict__ z, const T *__restrict__ pin,
                                        size_t c) {
     if constexpr (MODE == kDirCurrent) {
       a = pin[c];
@@ -135,19 +140,19 @@ struct Raw1 {
     }
   }
   __device__ __forceinline__ T value(T beta) const {
-    if constexpr (MODE != kDirUpdate) return a;
-    else return a + beta * b;
+    if constexpr (MODE != kDirUpdate) return (T)a;
+    else return (T)a + beta * b;
   }
 };
 
 // The in-plane neighbours of a lane's cells (south and north rows, the west
 // and east cells) and their neighbour codes, for one plane.
-template <int MODE, typename T, int CL>
+template <int MODE, typename T, int CL, typename ZT = T>
 struct Plane {
-  RawV<MODE, T, CL> s, n;
-  Raw1<MODE, T> w, e;
+  RawV<MODE, T, CL, ZT> s, n;
+  Raw1<MODE, T, ZT> w, e;
   Codes<CL> code;
-  __device__ __forceinline__ void load(const T *__restrict__ z, const T *__restrict__ pin,
+  __device__ __forceinline__ void load(const ZT *__restrict__ z, const T *__restrict__ pin,
                                        const uint8_t *__restrict__ cd, size_t c, ptrdiff_t os,
                                        ptrdiff_t on, ptrdiff_t ow, ptrdiff_t oe) {
     s.load(z, pin, c + os);
@@ -203,10 +208,11 @@ __device__ __forceinline__ Lane lane_of(const Tiling &t, int nx, int ny, int nz,
 
 
 // p = z + beta p_old (or z, or p_old, by MODE), q = A p and p.q for the
-// workgroup `block` of a launch of nblocks workgroups.
-template <typename T, int CL, int MODE>
+// workgroup `block` of a launch of nblocks workgroups; z in T or in a
+// narrower ZT.
+template <typename T, int CL, int MODE, typename ZT = T>
 __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nblocks,
-                                             const GeomT<T> &g, const T *__restrict__ z,
+                                             const GeomT<T> &g, const ZT *__restrict__ z,
                                              const uint8_t *__restrict__ code,
                                              const T *__restrict__ pin, T *__restrict__ pout,
                                              double *partials, const Scalars *sc, double *lds) {
@@ -245,8 +251,8 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
     const size_t base = (size_t)i + (size_t)nx * (size_t)j;
     const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
     auto at = [&](int k) { return base + nxy * (size_t)k; };
-    RawV<MODE, T, CL> own_a, own_b;
-    Plane<MODE, T, CL> pl_a, pl_b;
+    RawV<MODE, T, CL, ZT> own_a, own_b;
+    Plane<MODE, T, CL, ZT> pl_a, pl_b;
     own_a.load(z, pin, at(k0));
     Vec<T, CL> p_cur = own_a.value(beta);
     Vec<T, CL> p_prev = p_cur;
@@ -256,8 +262,9 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
     }
     own_a.load(z, pin, at(min(k0 + 1, last)));
     pl_a.load(z, pin, code, at(k0), os, on, ow, oe);
-    auto stage = [&](int k, const RawV<MODE, T, CL> &own_use, const Plane<MODE, T, CL> &pl,
-                     RawV<MODE, T, CL> &own_load, Plane<MODE, T, CL> &pl_load) {
+    auto stage = [&](int k, const RawV<MODE, T, CL, ZT> &own_use,
+                     const Plane<MODE, T, CL, ZT> &pl, RawV<MODE, T, CL, ZT> &own_load,
+                     Plane<MODE, T, CL, ZT> &pl_load) {
       own_load.load(z, pin, at(min(k + 2, last)));
       pl_load.load(z, pin, code, at(min(k + 1, k1 - 1)), os, on, ow, oe);
       const Vec<T, CL> p_next = own_use.value(beta);

@@ -852,11 +852,12 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     FFPM_LAUNCHED("k_chunk_begin");
     for (int m = 0; m < n; ++m) {
       if (!first_ready && (st = mg_first(mg, f.r))) return st;
-      if ((st = mg_vcycle(mg, f.z, mode == kDirUpdate))) return st;
+      if ((st = mg_vcycle(mg, mode == kDirUpdate))) return st;
       k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
       const int dst = cur ^ 1;
-      if ((st = launch_dir<double>(f, cl, g, f.z, pbuf[cur], pbuf[dst], mode))) return st;
+      if ((st = mg_direction(mg, f.tl[cl], cl, pbuf[cur], pbuf[dst], mode, f.sc))) return st;
+      if ((st = finish(f, FIN_PQ, 1, f.tl[cl].blocks))) return st;
       cur = dst;
       // the chunk's last update also estimates the max-norm residual from
       // the recursively updated r
