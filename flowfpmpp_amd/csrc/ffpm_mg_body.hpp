@@ -240,11 +240,13 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
 }
 
 // Prolongation onto level 0 with one thread per level-1 (I, J) column walking
-// a range of K: the 3x3 coarse patches (values and unknown masks) of planes
-// K-1, K, K+1 stay in registers, and the thread updates the 2x2x2 fine cells
-// of each coarse cell with the trilinear weights of prolong_value (only fine
-// cells that carry unknowns change). VEC (even nx): each fine row pair moves as
-// one 8-byte x and 2-byte code access.
+// a range of K: fine plane 2K + c takes coarse planes K and K - 1 (c = 0) or
+// K + 1 (c = 1), so two 3x3 coarse patches (values and unknown masks) are
+// held at a time, the next plane's patch loaded over the one just retired;
+// each
+// fine cell takes the trilinear weights of prolong_value (only fine cells that
+// carry unknowns change). VEC (even nx): each fine row pair moves as one
+// 8-byte x and 2-byte code access.
 template <bool VEC>
 __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int nz,
                                                    const uint8_t *__restrict__ code,
@@ -271,86 +273,81 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
         m[b][a] = ok ? mc[cc] : 0.f;
       }
   };
-  float xm[3][3], mm[3][3], x0[3][3], m0[3][3], xp[3][3], mp[3][3];
-  cload(K0 - 1, xm, mm);
-  cload(K0, x0, m0);
-  for (int K = K0; K < K1; ++K) {
-    cload(K + 1, xp, mp);
-    // the block's four fine rows (2 planes x 2 rows) are loaded together,
-    // updated, and stored together
-    float xv[2][2][2];
-    unsigned cd[2][2][2];
-    bool ok[2][2];
-    size_t c0[2][2];
+  // fine plane k = 2K + c from the patches of plane K (x0) and of its z
+  // neighbour on the side of c (xz); wk: that neighbour's weight
+  auto fine_plane = [&](int k, float wk, const float (&x0)[3][3], const float (&m0)[3][3],
+                        const float (&xz)[3][3], const float (&mz)[3][3]) {
+    if (k >= nz) return;
+    float xv[2][2];
+    unsigned cd[2][2];
+    bool ok[2];
+    size_t c0[2];
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int k = 2 * K + c, j = 2 * J + b;
-        ok[c][b] = k < nz && j < ny;
-        c0[c][b] = (size_t)(2 * I) + fsx * ((size_t)(ok[c][b] ? j : 0) +
-                                            (size_t)ny * (ok[c][b] ? k : 0));
-        if constexpr (VEC) {
-          const float2 t = ok[c][b] ? *reinterpret_cast<const float2 *>(x + c0[c][b])
-                                    : make_float2(0.f, 0.f);
-          const uchar2 u = ok[c][b] ? *reinterpret_cast<const uchar2 *>(code + c0[c][b])
-                                    : make_uchar2(0, 0);
-          xv[c][b][0] = t.x;
-          xv[c][b][1] = t.y;
-          cd[c][b][0] = u.x;
-          cd[c][b][1] = u.y;
-        } else {
-          xv[c][b][0] = ok[c][b] ? x[c0[c][b]] : 0.f;
-          cd[c][b][0] = ok[c][b] ? code[c0[c][b]] : 0u;
-          xv[c][b][1] = ok[c][b] && has_i1 ? x[c0[c][b] + 1] : 0.f;
-          cd[c][b][1] = ok[c][b] && has_i1 ? code[c0[c][b] + 1] : 0u;
-        }
-      }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const float(&xz)[3][3] = c ? xp : xm;
-      const float(&mz)[3][3] = c ? mp : mm;
-      const float wk = (c ? K + 1 < cnz : K > 0) ? 0.25f : 0.f;
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int bj = b ? 2 : 0;
-        const float wj = vj[bj] ? 0.25f : 0.f;
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          if (!(cd[c][b][a] & kCodeFluid)) continue;
-          const int ai = a ? 2 : 0;
-          const float wi = vi[ai] ? 0.25f : 0.f;
-          const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
-          const float num0 = w00 * x0[1][1] + w10 * x0[1][ai] + w01 * x0[bj][1] + w11 * x0[bj][ai];
-          const float den0 = w00 * m0[1][1] + w10 * m0[1][ai] + w01 * m0[bj][1] + w11 * m0[bj][ai];
-          const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
-          const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
-          const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
-          xv[c][b][a] += den > 0.f ? num / den : 0.f;
-        }
+    for (int b = 0; b < 2; ++b) {
+      const int j = 2 * J + b;
+      ok[b] = j < ny;
+      c0[b] = (size_t)(2 * I) + fsx * ((size_t)(ok[b] ? j : 0) + (size_t)ny * k);
+      if constexpr (VEC) {
+        const float2 t = ok[b] ? *reinterpret_cast<const float2 *>(x + c0[b])
+                               : make_float2(0.f, 0.f);
+        const uchar2 u = ok[b] ? *reinterpret_cast<const uchar2 *>(code + c0[b])
+                               : make_uchar2(0, 0);
+        xv[b][0] = t.x;
+        xv[b][1] = t.y;
+        cd[b][0] = u.x;
+        cd[b][1] = u.y;
+      } else {
+        xv[b][0] = ok[b] ? x[c0[b]] : 0.f;
+        cd[b][0] = ok[b] ? code[c0[b]] : 0u;
+        xv[b][1] = ok[b] && has_i1 ? x[c0[b] + 1] : 0.f;
+        cd[b][1] = ok[b] && has_i1 ? code[c0[b] + 1] : 0u;
       }
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int b = 0; b < 2; ++b) {
+      const int bj = b ? 2 : 0;
+      const float wj = vj[bj] ? 0.25f : 0.f;
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        if (!ok[c][b]) continue;
-        if constexpr (VEC) {
-          *reinterpret_cast<float2 *>(x + c0[c][b]) = make_float2(xv[c][b][0], xv[c][b][1]);
-        } else {
-          x[c0[c][b]] = xv[c][b][0];
-          if (has_i1) x[c0[c][b] + 1] = xv[c][b][1];
-        }
+      for (int a = 0; a < 2; ++a) {
+        if (!(cd[b][a] & kCodeFluid)) continue;
+        const int ai = a ? 2 : 0;
+        const float wi = vi[ai] ? 0.25f : 0.f;
+        const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
+        const float num0 = w00 * x0[1][1] + w10 * x0[1][ai] + w01 * x0[bj][1] + w11 * x0[bj][ai];
+        const float den0 = w00 * m0[1][1] + w10 * m0[1][ai] + w01 * m0[bj][1] + w11 * m0[bj][ai];
+        const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
+        const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
+        const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
+        xv[b][a] += den > 0.f ? num / den : 0.f;
       }
+    }
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        xm[b][a] = x0[b][a];
-        mm[b][a] = m0[b][a];
-        x0[b][a] = xp[b][a];
-        m0[b][a] = mp[b][a];
+    for (int b = 0; b < 2; ++b) {
+      if (!ok[b]) continue;
+      if constexpr (VEC) {
+        *reinterpret_cast<float2 *>(x + c0[b]) = make_float2(xv[b][0], xv[b][1]);
+      } else {
+        x[c0[b]] = xv[b][0];
+        if (has_i1) x[c0[b] + 1] = xv[b][1];
       }
+    }
+  };
+  // pa: the patch of plane K, pb: of plane K - 1 (before fine plane 2K) or
+  // K + 1 (before fine plane 2K + 1); the two swap roles each step
+  float xa[3][3], ma[3][3], xb[3][3], mb[3][3];
+  cload(K0 - 1, xb, mb);
+  cload(K0, xa, ma);
+  int K = K0;
+  while (K < K1) {
+    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xa, ma, xb, mb);
+    cload(K + 1, xb, mb);
+    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xa, ma, xb, mb);
+    if (++K >= K1) break;
+    // plane K (old K + 1) is in xb: the roles swap
+    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xb, mb, xa, ma);
+    cload(K + 1, xa, ma);
+    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xb, mb, xa, ma);
+    ++K;
   }
 }
 
