@@ -406,31 +406,31 @@ __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid gri
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   mgk::mgf_jac0_body(cell_blk(mg.lv[0], grid.local[blockIdx.x]), mg.g32, mg.code, d.r,
-                     mg.lv[0].x, mg.s32, mg.omega);
+                     mg.lv[0].x, mg.s32, mg.first);
 }
 
-// Level-0 sweep from the iterate in x (flip 0) or y (flip 1) into the other;
+// One level-0 smoothing step (MgStep: buffers ia, ip, io, coefficients);
 // LAST writes the cycle's output z32 with the z.r and z.q partials (z.q only
 // after the first direction, as the standalone cycle).
-template <int CL, bool LAST, bool NT>
+template <int CL, bool LAST, bool NT, bool CHEB>
 __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid grid,
-                                                   const BState *state, int slot, int flip) {
+                                                   const BState *state, int slot, MgStep step) {
   __shared__ double lds[16 * kReduceSlots];
   const int s = grid.solve[blockIdx.x];
   const BState &st = state[s];
   if (!iterating(st, slot)) return;
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
-  const float *a = flip ? mg.lv[0].y : mg.lv[0].x;
-  float *b = LAST ? mg.z32 : (flip ? mg.lv[0].x : mg.lv[0].y);
   const float *q = LAST && st.mode == kDirUpdate ? mg.q32 : nullptr;
-  mgk::mg_sweep_body<CL, LAST, NT>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl, mg.g32, a,
-                                   mg.s32, mg.code, b, mg.partials, mg.omega, q);
+  mgk::mg_sweep_body<CL, LAST, NT, CHEB>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl,
+                                         mg.g32, mg.buf0(step.a), mg.s32, mg.code,
+                                         mg.buf0(step.o), mg.partials, step.c2, q,
+                                         step.p >= 0 ? mg.buf0(step.p) : nullptr, step.c1);
 }
 
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_bg_restrict0(const BSolve *solves, Grid grid,
-                                                      const BState *state, int slot, int flip) {
+                                                      const BState *state, int slot, int ia) {
   const int s = grid.solve[blockIdx.x];
   if (!iterating(state[s], slot)) return;
   const BSolve &d = solves[s];
@@ -438,40 +438,45 @@ __global__ __launch_bounds__(256) void k_bg_restrict0(const BSolve *solves, Grid
   const MgLevel &c = mg.lv[1];
   const mgk::CoarseOut out{c.b, c.x, c.rd};
   mgk::mgf_restrict_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), mg.g32, mg.code, mg.s32,
-                                flip ? mg.lv[0].y : mg.lv[0].x, c.nx, c.ny, c.nz, d.kc1, out);
+                                mg.buf0(ia), c.nx, c.ny, c.nz, d.kc1, out);
 }
 
+// Prolongation from level 1's result (buffer ic; the coarsest level writes
+// the same buffer as the others) into level 0's iterate (buffer ia).
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_bg_prolong0(const BSolve *solves, Grid grid,
-                                                     const BState *state, int slot, int flip) {
+                                                     const BState *state, int slot, int ia,
+                                                     int ic) {
   const int s = grid.solve[blockIdx.x];
   if (!iterating(state[s], slot)) return;
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   const MgLevel &f0 = mg.lv[0], &c = mg.lv[1];
-  // the coarse iterate ends in y on every coarse level (the coarsest writes
-  // y; the others end an odd number of sweeps after starting in x)
   mgk::mgf_prolong_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), f0.nx, f0.ny, f0.nz,
-                               mg.code, flip ? f0.y : f0.x, c.nx, c.ny, c.nz, c.y, c.m, d.kc1);
+                               mg.code, mg.buf0(ia), c.nx, c.ny, c.nz, Multigrid::bufc(c, ic),
+                               c.m, d.kc1);
 }
 
-// Coarse level l (not a solve's coarsest): one damped-Jacobi sweep.
+// Coarse level l (not a solve's coarsest): one smoothing step.
 __global__ __launch_bounds__(256) void k_bg_csweep(const BSolve *solves, Grid grid,
                                                    const BState *state, int slot, int l,
-                                                   int flip) {
+                                                   MgStep step) {
   const int s = grid.solve[blockIdx.x];
   if (!iterating(state[s], slot)) return;
   const Multigrid &mg = solves[s].mg;
   if (l >= mg.levels - 1) return;
   const MgLevel &v = mg.lv[l];
   mgk::mgc_sweep_body(cell_blk(v, grid.local[blockIdx.x]), v.nx, v.ny, v.nz, v.gE, v.gN, v.gT,
-                      v.b, flip ? v.y : v.x, flip ? v.x : v.y, mg.omega_coarse);
+                      v.b, Multigrid::bufc(v, step.a),
+                      step.p >= 0 ? Multigrid::bufc(v, step.p) : nullptr,
+                      Multigrid::bufc(v, step.o), step.c1, step.c2);
 }
 
-// Restriction of level l's residual into level l + 1 (grid of level l + 1).
+// Restriction of level l's residual (buffer ia) into level l + 1 (grid of
+// level l + 1).
 __global__ __launch_bounds__(256) void k_bg_crestrict(const BSolve *solves, Grid grid,
                                                       const BState *state, int slot, int l,
-                                                      int flip) {
+                                                      int ia) {
   const int s = grid.solve[blockIdx.x];
   if (!iterating(state[s], slot)) return;
   const Multigrid &mg = solves[s].mg;
@@ -479,32 +484,33 @@ __global__ __launch_bounds__(256) void k_bg_crestrict(const BSolve *solves, Grid
   const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
   const mgk::CoarseOut out{c.b, c.x, c.rd};
   mgk::mgc_restrict_body(cell_blk(c, grid.local[blockIdx.x]), v.nx, v.ny, v.nz, v.gE, v.gN,
-                         v.gT, v.b, flip ? v.y : v.x, c.nx, c.ny, out);
+                         v.gT, v.b, Multigrid::bufc(v, ia), c.nx, c.ny, out);
 }
 
-// Prolongation from level l + 1 into level l's iterate (grid of level l).
+// Prolongation from level l + 1's result (buffer ic) into level l's iterate
+// (buffer ia; grid of level l).
 __global__ __launch_bounds__(256) void k_bg_cprolong(const BSolve *solves, Grid grid,
                                                      const BState *state, int slot, int l,
-                                                     int flip) {
+                                                     int ia, int ic) {
   const int s = grid.solve[blockIdx.x];
   if (!iterating(state[s], slot)) return;
   const Multigrid &mg = solves[s].mg;
   if (l >= mg.levels - 1) return;
   const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
   mgk::mgc_prolong_body(cell_blk(v, grid.local[blockIdx.x]), v.nx, v.ny, v.m,
-                        flip ? v.y : v.x, c.nx, c.ny, c.nz, c.y, c.m);
+                        Multigrid::bufc(v, ia), c.nx, c.ny, c.nz, Multigrid::bufc(c, ic), c.m);
 }
 
-// Every solve's coarsest level, one workgroup per solve.
+// Every solve's coarsest level, one workgroup per solve, into buffer io.
 __global__ __launch_bounds__(1024) void k_bg_coarsest(const BSolve *solves, const BState *state,
-                                                      int slot) {
+                                                      int slot, int io) {
   __shared__ float xs[kMgCoarsestCells];
   const int s = blockIdx.x;
   if (!iterating(state[s], slot)) return;
   const Multigrid &mg = solves[s].mg;
   const MgLevel &c = mg.lv[mg.levels - 1];
-  mgk::mgc_coarsest_body(xs, c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b, c.rd, c.y,
-                         mg.omega_coarse, mg.coarse_sweeps);
+  mgk::mgc_coarsest_body(xs, c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b, c.rd,
+                         Multigrid::bufc(c, io), mg.omega_coarse, mg.coarse_sweeps);
 }
 
 // The direction step on the cycle's FP32 output (p = z or z + beta p_old,
@@ -541,7 +547,7 @@ __global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid
   const Multigrid &mg = d.mg;
   double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
   mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g, d.p[st.cur], d.code,
-                       d.x, d.r, mg.lv[0].x, d.sc, (double)mg.omega, rmax, mg.q32, mg.s32);
+                       d.x, d.r, mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32, mg.s32);
 }
 
 // One workgroup per solve: the finish of the preceding reduction for the
@@ -1155,80 +1161,65 @@ int launch_check_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, doubl
 }
 
 // One V-cycle of every solve iterating in slot m: each level's step is one
-// launch over the solves that have the level. The number of sweeps per level
-// is the same for every solve, so the iterate's place (x or y) is too.
-template <bool NT>
+// launch over the solves that have the level. The steps (buffers and
+// coefficients) are the same for every solve.
+template <bool NT, bool CHEB>
 int launch_cycle_mg(const Batch &B, hipStream_t s, int m) {
   const Multigrid &h = B.h[0].mg;
+  const MgSchedule &s0 = h.s0, &sc = h.sc;
   const dim3 cblk(mgk::kMt), sblk(kTileX, B.h[0].tl32.ty);
-  auto sweep0 = [&](bool last, int flip) -> int {
+  auto step0 = [&](const MgStep &stp, bool last) -> int {
     for (int c : {1, 2, 4}) {
       const Grid &g = B.g_pcg32[c];
       if (!g.blocks) continue;
-      if (c == 4 && last)
-        k_bg_sweep<4, true, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
-      else if (c == 4)
-        k_bg_sweep<4, false, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
-      else if (c == 2 && last)
-        k_bg_sweep<2, true, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
-      else if (c == 2)
-        k_bg_sweep<2, false, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
-      else if (last)
-        k_bg_sweep<1, true, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
-      else
-        k_bg_sweep<1, false, NT><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, flip);
+      auto go = [&](auto kern) {
+        kern<<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m, stp);
+      };
+      if (c == 4) last ? go(k_bg_sweep<4, true, NT, CHEB>) : go(k_bg_sweep<4, false, NT, CHEB>);
+      else if (c == 2) last ? go(k_bg_sweep<2, true, NT, CHEB>) : go(k_bg_sweep<2, false, NT, CHEB>);
+      else last ? go(k_bg_sweep<1, true, NT, CHEB>) : go(k_bg_sweep<1, false, NT, CHEB>);
     }
     FFPM_LAUNCHED("k_bg_sweep");
     return OK;
   };
   int st;
-  int f0 = 0;
-  for (int k = 1; k < h.nu; ++k) {
-    if ((st = sweep0(false, f0))) return st;
-    f0 ^= 1;
-  }
+  for (int k = 0; k < s0.npre; ++k)
+    if ((st = step0(s0.pre[k], false))) return st;
   if (B.g_col[0].blocks)
     k_bg_restrict0<false><<<B.g_col[0].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[0], B.d_state,
-                                                             m, f0);
+                                                             m, s0.mid);
   if (B.g_col[1].blocks)
     k_bg_restrict0<true><<<B.g_col[1].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[1], B.d_state,
-                                                            m, f0);
+                                                            m, s0.mid);
   FFPM_LAUNCHED("k_bg_restrict0");
-  int fl[kMgMaxLevels] = {};
   for (int l = 1; l < B.mg_top; ++l) {
-    int f = 0;
-    for (int k = 1; k < h.nu_coarse; ++k) {
-      k_bg_csweep<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l, f);
-      f ^= 1;
-    }
+    for (int k = 0; k < sc.npre; ++k)
+      k_bg_csweep<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l,
+                                                     sc.pre[k]);
     k_bg_crestrict<<<B.g_lvl[l + 1].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l + 1],
-                                                          B.d_state, m, l, f);
-    fl[l] = f;
+                                                          B.d_state, m, l, sc.mid);
   }
   FFPM_LAUNCHED("k_bg_crestrict");
-  k_bg_coarsest<<<B.count, kMgCoarsestCells, 0, s>>>(B.d_solves, B.d_state, m);
+  k_bg_coarsest<<<B.count, kMgCoarsestCells, 0, s>>>(B.d_solves, B.d_state, m, sc.out);
   FFPM_LAUNCHED("k_bg_coarsest");
   for (int l = B.mg_top - 1; l >= 1; --l) {
-    int f = fl[l];
-    k_bg_cprolong<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l, f);
-    for (int k = 0; k < h.nu_coarse; ++k) {
-      k_bg_csweep<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l, f);
-      f ^= 1;
-    }
+    k_bg_cprolong<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l,
+                                                     sc.mid, sc.out);
+    for (int k = 0; k < sc.npost; ++k)
+      k_bg_csweep<<<B.g_lvl[l].blocks, cblk, 0, s>>>(B.d_solves, B.g_lvl[l], B.d_state, m, l,
+                                                     sc.post[k]);
   }
   FFPM_LAUNCHED("k_bg_csweep");
   if (B.g_col[0].blocks)
     k_bg_prolong0<false><<<B.g_col[0].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[0], B.d_state,
-                                                            m, f0);
+                                                            m, s0.mid, sc.out);
   if (B.g_col[1].blocks)
     k_bg_prolong0<true><<<B.g_col[1].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[1], B.d_state,
-                                                           m, f0);
+                                                           m, s0.mid, sc.out);
   FFPM_LAUNCHED("k_bg_prolong0");
-  for (int k = 1; k < h.nu; ++k) {
-    if ((st = sweep0(false, f0))) return st;
-    f0 ^= 1;
-  }
-  return sweep0(true, f0);
+  for (int k = 0; k < s0.npost; ++k)
+    if ((st = step0(s0.post[k], k == s0.npost - 1))) return st;
+  return OK;
 }
 
 // One chunk: ci slots (first step, cycle, FCG(1) beta, direction and SpMV,
@@ -1242,8 +1233,13 @@ int launch_chunk_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, doubl
       k_bg_first<<<B.g_cell0.blocks, cblk, 0, s>>>(B.d_solves, B.g_cell0, B.d_state);
       FFPM_LAUNCHED("k_bg_first");
     }
-    st = B.h[0].mg.nontemporal ? launch_cycle_mg<true>(B, s, m)
-                               : launch_cycle_mg<false>(B, s, m);
+    const Multigrid &h = B.h[0].mg;
+    if (h.cheb)
+      st = h.nontemporal ? launch_cycle_mg<true, true>(B, s, m)
+                         : launch_cycle_mg<false, true>(B, s, m);
+    else
+      st = h.nontemporal ? launch_cycle_mg<true, false>(B, s, m)
+                         : launch_cycle_mg<false, false>(B, s, m);
     if (st) return st;
     k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_FCG1, m);
     FFPM_LAUNCHED("k_bg_finish");

@@ -48,8 +48,9 @@ __global__ __launch_bounds__(256) void k_mgc_sweep(int nx, int ny, int nz,
                                                    const float *__restrict__ gT,
                                                    const float *__restrict__ b,
                                                    const float *__restrict__ in,
-                                                   float *__restrict__ out, float omega) {
-  mgc_sweep_body(blk3(), nx, ny, nz, gE, gN, gT, b, in, out, omega);
+                                                   const float *__restrict__ prev,
+                                                   float *__restrict__ out, float c1, float c2) {
+  mgc_sweep_body(blk3(), nx, ny, nz, gE, gN, gT, b, in, prev, out, c1, c2);
 }
 
 __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
@@ -81,16 +82,17 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
   mgc_coarsest_body(xs, nx, ny, nz, gE, gN, gT, b, rd, x, omega, sweeps);
 }
 
-template <int CL, bool LAST, bool NT>
+template <int CL, bool LAST, bool NT, bool CHEB>
 __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const float *__restrict__ a,
                                                      const float *__restrict__ r,
                                                      const uint8_t *__restrict__ code,
                                                      float *__restrict__ b, double *partials,
-                                                     float omega, const float *__restrict__ q) {
+                                                     float c2, const float *__restrict__ q,
+                                                     const float *__restrict__ prev, float c1) {
   __shared__ double lds[16 * kReduceSlots];
-  mg_sweep_body<CL, LAST, NT>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, partials, omega,
-                              q);
+  mg_sweep_body<CL, LAST, NT, CHEB>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, partials,
+                                    c2, q, prev, c1);
 }
 
 template <int CL>
@@ -212,6 +214,60 @@ float env_float(const char *name, float fallback) {
   return e ? (float)std::atof(e) : fallback;
 }
 
+// The steps of one level (MgSchedule) for nu smoothing steps before and after
+// the coarse correction; `first` receives the scale of the first step from
+// zero. Chebyshev (Saad's three-term recurrence on D^-1 A over [lo, hi], hi
+// = 2.1 above the bound 2 of the spectrum, lo = 2 / ratio): step 0 is
+// x1 = x0 + (1 / theta) D^-1 (b - A x0); step k >= 1 adds
+// rho_k rho_(k-1) (x_k - x_(k-1)) + (2 rho_k / delta) D^-1 (b - A x_k), with
+// theta and delta the interval's centre and half-width, rho_0 = delta / theta
+// and rho_k = 1 / (2 theta / delta - rho_(k-1)). Damped Jacobi: every step
+// x + omega D^-1 (b - A x). Buffers rotate over 0..2 (the step writes the
+// one holding neither its iterate nor its previous iterate); with to_z the
+// last step writes buffer 3.
+MgSchedule make_schedule(int nu, bool cheb, float omega, double ratio, bool to_z,
+                         float *first) {
+  const double hi = 2.1, lo = 2.0 / (ratio > 1.0 ? ratio : 25.0);
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  float c1[8] = {}, c2[8] = {};
+  double rho = 1.0 / sigma;
+  c2[0] = cheb ? (float)(1.0 / theta) : omega;
+  for (int k = 1; k < nu; ++k) {
+    if (cheb) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      c1[k] = (float)(rn * rho);
+      c2[k] = (float)(2.0 * rn / delta);
+      rho = rn;
+    } else {
+      c2[k] = omega;
+    }
+  }
+  *first = c2[0];
+  auto other = [](int a, int p) {
+    for (int i = 0; i < 3; ++i)
+      if (i != a && i != p) return i;
+    return 2;
+  };
+  MgSchedule s;
+  int cur = 0, prev = -1;  // the first step from zero leaves buffer 0
+  for (int k = 1; k < nu; ++k) {
+    const int o = other(cur, prev);
+    s.pre[s.npre++] = MgStep{cur, prev, o, c1[k], c2[k]};
+    prev = cur;
+    cur = o;
+  }
+  s.mid = cur;
+  prev = -1;
+  for (int k = 0; k < nu; ++k) {
+    const int o = (to_z && k == nu - 1) ? 3 : other(cur, prev);
+    s.post[s.npost++] = MgStep{cur, k == 0 ? -1 : prev, o, c1[k], c2[k]};
+    prev = cur;
+    cur = o;
+  }
+  s.out = cur;
+  return s;
+}
+
 }  // namespace
 
 int mg_check_interval(int check_interval) {
@@ -235,10 +291,17 @@ int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   mg.coarse_sweeps = (int)env_float("FFPM_MG_COARSE_SWEEPS", 30.f);
   if (mg.coarse_sweeps < 1) mg.coarse_sweeps = 1;
   mg.nontemporal = env_float("FFPM_MG_NT", 1.f) != 0.f;
+  mg.cheb = env_float("FFPM_MG_CHEB", 1.f) != 0.f;
   mg.nu = (int)env_float("FFPM_MG_NU", 3.f);
-  mg.nu_coarse = (int)env_float("FFPM_MG_NU_COARSE", 4.f);
+  mg.nu_coarse = (int)env_float("FFPM_MG_NU_COARSE", mg.cheb ? 3.f : 4.f);
   if (mg.nu < 2) mg.nu = 2;  // the update forms the first sweep, the last one writes z
+  if (mg.nu > 8) mg.nu = 8;
   if (mg.nu_coarse < 1) mg.nu_coarse = 1;
+  if (mg.nu_coarse > 8) mg.nu_coarse = 8;
+  const double ratio = env_float("FFPM_MG_CHEB_RATIO", 25.f);
+  mg.s0 = make_schedule(mg.nu, mg.cheb, mg.omega, ratio, true, &mg.first);
+  mg.sc = make_schedule(mg.nu_coarse, mg.cheb, mg.omega_coarse, ratio, false,
+                        &mg.first_coarse);
   // level extents: halve (ceil) until the level fits one workgroup
   int L = 0;
   int nx = g64.nx, ny = g64.ny, nz = g64.nz;
@@ -284,8 +347,8 @@ size_t mg_operator_bytes(const Multigrid &mg) {
 }
 
 size_t mg_state_bytes(const Multigrid &mg) {
-  size_t bytes = 5 * up256(sizeof(float) * mg.lv[0].n);
-  for (int l = 1; l < mg.levels; ++l) bytes += 3 * up256(sizeof(float) * mg.lv[l].n);
+  size_t bytes = 6 * up256(sizeof(float) * mg.lv[0].n);
+  for (int l = 1; l < mg.levels; ++l) bytes += 4 * up256(sizeof(float) * mg.lv[l].n);
   return bytes;
 }
 
@@ -303,6 +366,7 @@ void mg_bind_operator(Multigrid &mg, char *&base) {
 void mg_bind_state(Multigrid &mg, char *&base) {
   mg.lv[0].x = carve(base, mg.lv[0].n);
   mg.lv[0].y = carve(base, mg.lv[0].n);
+  mg.lv[0].w = carve(base, mg.lv[0].n);
   mg.q32 = carve(base, mg.lv[0].n);
   mg.s32 = carve(base, mg.lv[0].n);
   mg.z32 = carve(base, mg.lv[0].n);
@@ -310,6 +374,7 @@ void mg_bind_state(Multigrid &mg, char *&base) {
     MgLevel &v = mg.lv[l];
     v.x = carve(base, v.n);
     v.y = carve(base, v.n);
+    v.w = carve(base, v.n);
     v.b = carve(base, v.n);
   }
 }
@@ -330,7 +395,7 @@ int mg_build(const Multigrid &mg) {
   for (int l = 1; l < mg.levels; ++l) {
     const MgLevel &c = mg.lv[l];
     k_mg_mask<<<cell_grid(c.nx, c.ny, c.nz), dim3(kMt), 0, stream()>>>(
-        c.nx, c.ny, c.gE, c.gN, c.gT, mg.omega_coarse, c.m, c.rd);
+        c.nx, c.ny, c.gE, c.gN, c.gT, mg.first_coarse, c.m, c.rd);
     FFPM_LAUNCHED("k_mg_mask");
   }
   return OK;
@@ -350,28 +415,31 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 
 namespace {
 
-// Level-0 streaming sweep launch for the multigrid's cells-per-lane choice.
-template <bool LAST, bool NT>
-void launch_sweep_nt(const Multigrid &mg, const float *a, const float *r, float *b,
-                     const float *q) {
+// Level-0 streaming sweep launch (one step of a schedule) for the
+// multigrid's cells-per-lane choice.
+template <bool LAST, bool NT, bool CHEB>
+void launch_step0(const Multigrid &mg, const MgStep &st, const float *q) {
   const dim3 blk(kTileX, mg.tl.ty);
-  const float w = mg.omega;
-  if (mg.cl == 4)
-    k_mg_sweep_t<4, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
-                                                                  b, mg.partials, w, q);
-  else if (mg.cl == 2)
-    k_mg_sweep_t<2, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
-                                                                  b, mg.partials, w, q);
-  else
-    k_mg_sweep_t<1, LAST, NT><<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, r, mg.code,
-                                                                  b, mg.partials, w, q);
+  const float *a = mg.buf0(st.a), *p = st.p >= 0 ? mg.buf0(st.p) : nullptr;
+  float *o = mg.buf0(st.o);
+  auto go = [&](auto kern) {
+    kern<<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, mg.s32, mg.code, o, mg.partials,
+                                             st.c2, q, p, st.c1);
+  };
+  if (mg.cl == 4) go(k_mg_sweep_t<4, LAST, NT, CHEB>);
+  else if (mg.cl == 2) go(k_mg_sweep_t<2, LAST, NT, CHEB>);
+  else go(k_mg_sweep_t<1, LAST, NT, CHEB>);
 }
 
 template <bool LAST>
-void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b,
-                  const float *q = nullptr) {
-  if (mg.nontemporal) launch_sweep_nt<LAST, true>(mg, a, r, b, q);
-  else launch_sweep_nt<LAST, false>(mg, a, r, b, q);
+void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr) {
+  if (mg.cheb) {
+    if (mg.nontemporal) launch_step0<LAST, true, true>(mg, st, q);
+    else launch_step0<LAST, false, true>(mg, st, q);
+  } else {
+    if (mg.nontemporal) launch_step0<LAST, true, false>(mg, st, q);
+    else launch_step0<LAST, false, false>(mg, st, q);
+  }
 }
 
 }  // namespace
@@ -379,7 +447,7 @@ void launch_sweep(const Multigrid &mg, const float *a, const float *r, float *b,
 int mg_first(const Multigrid &mg, const double *r) {
   const MgLevel &f0 = mg.lv[0];
   k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMt), 0, stream()>>>(
-      mg.g32, mg.code, r, f0.x, mg.s32, mg.omega);
+      mg.g32, mg.code, r, f0.x, mg.s32, mg.first);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
 }
@@ -387,7 +455,7 @@ int mg_first(const Multigrid &mg, const double *r) {
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
               double *r, Scalars *sc, double *rmax) {
   const dim3 blk(kTileX, t.ty);
-  const double w = (double)mg.omega;
+  const double w = (double)mg.first;
   float *x1 = mg.lv[0].x;
   TimedLaunch tl("mg_update");
   if (cl == 2)
@@ -418,16 +486,11 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
   const float *r = mg.s32;
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
-  const float wc = mg.omega_coarse;
+  const MgSchedule &s0 = mg.s0, &sc = mg.sc;
   const int L = mg.levels - 1;  // coarsest index
-  // cur[l]: the level's current iterate, oth[l]: its ping-pong partner
-  float *cur[kMgMaxLevels], *oth[kMgMaxLevels];
-  cur[0] = f0.x;
-  oth[0] = f0.y;
-  for (int s = 1; s < mg.nu; ++s) {
+  for (int k = 0; k < s0.npre; ++k) {
     TimedLaunch tl("mg_fine_sweep");
-    launch_sweep<false>(mg, cur[0], r, oth[0]);
-    std::swap(cur[0], oth[0]);
+    launch_step<false>(mg, s0.pre[k]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
@@ -436,68 +499,67 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     const int kc = column_chunk(c.nx, c.ny, c.nz);
     if (f0.nx % 2 == 0)
       k_mgf_restrict_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
+          g, mg.code, r, mg.buf0(s0.mid), c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
     else
       k_mgf_restrict_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, cur[0], c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
+          g, mg.code, r, mg.buf0(s0.mid), c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
   }
   FFPM_LAUNCHED("k_mgf_restrict_z");
-  // down: pre-smoothing and restriction
+  auto csweep = [&](const MgLevel &v, const MgStep &st) {
+    k_mgc_sweep<<<cell_grid(v.nx, v.ny, v.nz), blk, 0, stream()>>>(
+        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, Multigrid::bufc(v, st.a),
+        st.p >= 0 ? Multigrid::bufc(v, st.p) : nullptr, Multigrid::bufc(v, st.o), st.c1, st.c2);
+  };
+  // down: pre-smoothing (after the first step, formed by the restriction
+  // into the level) and restriction
   for (int l = 1; l < L; ++l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
-    const dim3 gr = cell_grid(v.nx, v.ny, v.nz);
-    cur[l] = v.x;  // the first sweep, formed by the restriction into this level
-    oth[l] = v.y;
-    for (int s = 1; s < mg.nu_coarse; ++s) {
-      k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l],
-                                            oth[l], wc);
-      std::swap(cur[l], oth[l]);
-    }
+    for (int k = 0; k < sc.npre; ++k) csweep(v, sc.pre[k]);
     k_mgc_restrict<<<cell_grid(c.nx, c.ny, c.nz), blk, 0, stream()>>>(
-        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l], c.nx, c.ny, coarse_out(mg, l + 1));
+        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, Multigrid::bufc(v, sc.mid), c.nx, c.ny,
+        coarse_out(mg, l + 1));
     FFPM_LAUNCHED("k_mgc_restrict");
   }
   {
+    // the coarsest level leaves its solution where the other coarse levels
+    // leave theirs
     const MgLevel &c = mg.lv[L];
-    cur[L] = c.y;
     k_mgc_coarsest<<<1, kMgCoarsestCells, 0, stream()>>>(c.nx, c.ny, c.nz, c.gE, c.gN, c.gT, c.b,
-                                                         c.rd, cur[L], wc, mg.coarse_sweeps);
+                                                         c.rd, Multigrid::bufc(c, sc.out),
+                                                         mg.omega_coarse, mg.coarse_sweeps);
     FFPM_LAUNCHED("k_mgc_coarsest");
   }
   // up: prolongation into the level's iterate, then post-smoothing
   for (int l = L - 1; l >= 1; --l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
-    const dim3 gr = cell_grid(v.nx, v.ny, v.nz);
-    k_mgc_prolong<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.m, cur[l], c.nx, c.ny, c.nz, cur[l + 1],
-                                            c.m);
-    for (int s = 0; s < mg.nu_coarse; ++s) {
-      k_mgc_sweep<<<gr, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, cur[l],
-                                            oth[l], wc);
-      std::swap(cur[l], oth[l]);
-    }
+    k_mgc_prolong<<<cell_grid(v.nx, v.ny, v.nz), blk, 0, stream()>>>(
+        v.nx, v.ny, v.m, Multigrid::bufc(v, sc.mid), c.nx, c.ny, c.nz,
+        Multigrid::bufc(c, sc.out), c.m);
+    for (int k = 0; k < sc.npost; ++k) csweep(v, sc.post[k]);
     FFPM_LAUNCHED("k_mgc_sweep");
   }
   {
     const MgLevel &c = mg.lv[1];
     TimedLaunch tl("mg_fine_prolong");
     const int kc = column_chunk(c.nx, c.ny, c.nz);
+    float *x0 = mg.buf0(s0.mid);
+    const float *xc = Multigrid::bufc(c, sc.out);
     if (f0.nx % 2 == 0)
       k_mgf_prolong_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          f0.nx, f0.ny, f0.nz, mg.code, cur[0], c.nx, c.ny, c.nz, cur[1], c.m, kc);
+          f0.nx, f0.ny, f0.nz, mg.code, x0, c.nx, c.ny, c.nz, xc, c.m, kc);
     else
       k_mgf_prolong_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          f0.nx, f0.ny, f0.nz, mg.code, cur[0], c.nx, c.ny, c.nz, cur[1], c.m, kc);
+          f0.nx, f0.ny, f0.nz, mg.code, x0, c.nx, c.ny, c.nz, xc, c.m, kc);
   }
   FFPM_LAUNCHED("k_mgf_prolong_z");
-  for (int s = 1; s < mg.nu; ++s) {
+  for (int k = 0; k + 1 < s0.npost; ++k) {
     TimedLaunch tl("mg_fine_sweep");
-    launch_sweep<false>(mg, cur[0], r, oth[0]);
-    std::swap(cur[0], oth[0]);
+    launch_step<false>(mg, s0.post[k]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
     TimedLaunch tl("mg_fine_last");
-    launch_sweep<true>(mg, cur[0], r, mg.z32, with_q ? mg.q32 : nullptr);
+    launch_step<true>(mg, s0.post[s0.npost - 1], with_q ? mg.q32 : nullptr);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   return OK;

@@ -12,14 +12,16 @@
 // null space of the Neumann problem is kept. A cell carries an unknown when
 // its diagonal is positive.
 //
-// Cycle (FP32 arithmetic and output z, FP64 input r): nu damped-Jacobi
-// sweeps (weight 0.9) from zero (the first one fused into the outer update),
-// restriction of the residual by summation over the children, recursion,
-// trilinear cell-centred prolongation with weights renormalised over the
-// coarse cells that carry unknowns, nu damped-Jacobi sweeps (nu = 3 on level
-// 0 and 4 on the coarse levels by default, measured fastest at 512x384x256).
-// The coarsest level (at most 1024 cells) runs a fixed number of sweeps in
-// one workgroup. Every step is a fixed linear map, so the
+// Cycle (FP32 arithmetic and output z, FP64 input r): nu smoothing steps
+// from zero (the first one fused into the outer update or the restriction
+// that forms the level's right-hand side), restriction of the residual by
+// summation over the children, recursion, trilinear cell-centred
+// prolongation with weights renormalised over the coarse cells that carry
+// unknowns, nu smoothing steps. The smoother is Chebyshev of degree nu on
+// D^-1 A over [2.1 / ratio, 2.1] (rows sum to zero, so the spectrum of D^-1 A
+// lies in [0, 2]; ratio 25), or damped Jacobi (weight 0.9); nu = 3 on every
+// level. The coarsest level (at most 1024 cells) runs a fixed number of
+// damped-Jacobi sweeps in one workgroup. Every step is a fixed linear map, so the
 // CG around it sees the same preconditioner at every iteration.
 #pragma once
 
@@ -35,8 +37,27 @@ struct MgLevel {
   size_t n = 0;
   float *gE = nullptr, *gN = nullptr, *gT = nullptr;  // coarse levels only
   float *m = nullptr;                                 // coarse levels: 1 where diag > 0
-  float *rd = nullptr;                                // coarse levels: omega / diag (or 0)
-  float *x = nullptr, *y = nullptr, *b = nullptr;     // b: coarse levels only
+  float *rd = nullptr;                                // coarse levels: first-step scale / diag
+  float *x = nullptr, *y = nullptr, *w = nullptr;     // iterate buffers
+  float *b = nullptr;                                 // coarse levels only
+};
+
+// One smoothing step of a level: iterate buffer a (and the previous iterate
+// p, -1 for none) into buffer o, out = a + c2 (b - A a) / d + c1 (a - p).
+// Level-0 buffers: 0 x, 1 y, 2 w, 3 the cycle's output z32; coarse: 0 x, 1 y,
+// 2 w.
+struct MgStep {
+  int a, p, o;
+  float c1, c2;
+};
+
+// The steps of one level: `pre` after the first step (which leaves buffer 0),
+// then the restriction from buffer `mid`, and after the prolongation into
+// `mid` the `post` steps, ending in buffer `out`. The same for every level
+// of a kind and every solve, so batched solves share it.
+struct MgSchedule {
+  int npre = 0, npost = 0, mid = 0, out = 0;
+  MgStep pre[8], post[8];
 };
 
 struct Multigrid {
@@ -54,10 +75,21 @@ struct Multigrid {
   int nu = 3;                // level-0 sweeps before and after the coarse correction
   int nu_coarse = 4;         // coarse-level sweeps before and after
   bool nontemporal = true;   // level-0 sweeps: nontemporal source loads and stores
+  bool cheb = true;          // Chebyshev smoothing (else damped Jacobi)
+  float first = 0.9f;        // scale of the first step from zero, x = first b / d
+  float first_coarse = 0.9f; // the same on the coarse levels
+  MgSchedule s0, sc;         // level 0 (its last step writes z32), coarse levels
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
   float *s32 = nullptr;        // the cycle's source r / V (level-0 cells)
   float *z32 = nullptr;        // the cycle's output z = B r (level-0 cells)
+  // the iterate buffers of level 0 (index as in MgStep)
+  __host__ __device__ float *buf0(int i) const {
+    return i == 0 ? lv[0].x : i == 1 ? lv[0].y : i == 2 ? lv[0].w : z32;
+  }
+  __host__ __device__ static float *bufc(const MgLevel &v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : v.w;
+  }
 };
 
 // Builds the hierarchy for the frame with FP64 geometry g64, its FP32 copy
@@ -69,8 +101,8 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 // The steps of mg_setup for callers that place the arrays themselves (the
 // batched solves): parameters and level extents (no allocation); the bytes of
 // the level operators (gE gN gT m rd of the coarse levels, shared by solves
-// of one operator) and of one solve's cycle state (level-0 x y q s z, coarse
-// x y b), every array 256-byte aligned; carving them from base (advanced past
+// of one operator) and of one solve's cycle state (level-0 x y w q s z,
+// coarse x y w b), every array 256-byte aligned; carving them from base (advanced past
 // them); and the launches that fill the level operators.
 int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
             const uint8_t *code, const Tiling &tl, int cl, double *partials);

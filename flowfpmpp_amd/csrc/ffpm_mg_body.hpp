@@ -376,18 +376,24 @@ __device__ __forceinline__ Coarse coarse_row(int nx, int ny, int nz, const float
 
 
 
+// One smoothing step of a coarse level: out = x + c2 (b - A x) / d, plus
+// c1 (x - prev) for a Chebyshev step after the first (prev null: a zero
+// previous iterate); damped Jacobi is c2 = omega, c1 = 0.
 __device__ __forceinline__ void mgc_sweep_body(Blk bk, int nx, int ny, int nz,
                                                const float *__restrict__ gE,
                                                const float *__restrict__ gN,
                                                const float *__restrict__ gT,
                                                const float *__restrict__ b,
                                                const float *__restrict__ in,
-                                               float *__restrict__ out, float omega) {
+                                               const float *__restrict__ prev,
+                                               float *__restrict__ out, float c1, float c2) {
   const Idx t = cell_idx(nx, ny, bk);
   if (!t.in) return;
   const Coarse o = coarse_row(nx, ny, nz, gE, gN, gT, in, t.c, t.i, t.j, t.k);
   const float xv = in[t.c];
-  out[t.c] = o.d > 0.f ? xv + omega * (b[t.c] + o.nb - o.d * xv) / o.d : 0.f;
+  float v = o.d > 0.f ? xv + c2 * (b[t.c] + o.nb - o.d * xv) / o.d : 0.f;
+  if (c1 != 0.f && o.d > 0.f) v += c1 * (xv - (prev ? prev[t.c] : 0.f));
+  out[t.c] = v;
 }
 
 __device__ __forceinline__ void mgc_restrict_body(Blk bk, int nx, int ny, int nz,
@@ -511,13 +517,18 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
 // LAST: out is the cycle's output z (FP32; the FP64 iteration widens it
 // exactly), and the partial sums z.r = z.(s V) and z.q go to two slots (q = A
 // p_old from the previous update, FP32, or z.q = 0 when q is null).
-template <int CL, bool LAST, bool NT>
+//
+// CHEB: a Chebyshev step, out = a + c2 (s - A a) / d + c1 (a - prev) (prev
+// null: a zero previous iterate); otherwise damped Jacobi with omega = c2.
+template <int CL, bool LAST, bool NT, bool CHEB = false>
 __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *lds, Tiling t,
                                               GeomT<float> g, const float *__restrict__ a,
                                               const float *__restrict__ r,
                                               const uint8_t *__restrict__ code,
                                               float *__restrict__ b, double *partials,
-                                              float omega, const float *__restrict__ q) {
+                                              float omega, const float *__restrict__ q,
+                                              const float *__restrict__ prev = nullptr,
+                                              float c1 = 0.f) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
   double s = 0.0, sq = 0.0;
@@ -546,7 +557,7 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
     auto at = [&](int k) { return base + nxy * (size_t)k; };
     struct Set {
       V a_next;
-      VS r;
+      VS r, p;
       Plane<kDirCurrent, float, CL> pl;
     };
     Set sa, sb;
@@ -555,11 +566,17 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
     sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
     sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
     sa.r = ldv_s<NT, float, CL>(r, at(k0));
+    if constexpr (CHEB) {
+      if (prev) sa.p = ldv_s<NT, float, CL>(prev, at(k0));
+    }
     auto stage = [&](int k, const Set &u, Set &ld) {
       const int kn = min(k + 1, k1 - 1);
       ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
       ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
       ld.r = ldv_s<NT, float, CL>(r, at(kn));
+      if constexpr (CHEB) {
+        if (prev) ld.p = ldv_s<NT, float, CL>(prev, at(kn));
+      }
       const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
       const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
       const float vw = lane == 0 ? u.pl.w.a : from_w;
@@ -578,7 +595,14 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
         const float src = u.r.v[m];
         const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
                          cb * a_prev.v[m] + ct * u.a_next.v[m];
-        const float o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
+        float o;
+        if constexpr (CHEB) {
+          const float av = a_cur.v[m];
+          o = d > 0.f ? av + omega * (src + nb - d * av) / d : 0.f;
+          if (c1 != 0.f && d > 0.f) o += c1 * (av - (prev ? u.p.v[m] : 0.f));
+        } else {
+          o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
+        }
         bv.v[m] = o;
         if constexpr (LAST) {
           const double zd = (double)o;
