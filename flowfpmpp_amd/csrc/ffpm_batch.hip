@@ -32,6 +32,7 @@
 
 #include "../../include/ffpm.h"
 #include "ffpm_bodies.hpp"
+#include "ffpm_frame_body.hpp"
 #include "ffpm_mg.hpp"
 #include "ffpm_mg_body.hpp"
 #include "ffpm_pcg_body.hpp"
@@ -51,6 +52,8 @@ struct BSolve {
   Tiling tt;        // launch shape of the true-residual kernel
   size_t n;         // cells
   int eb;           // workgroups of the elementwise kernels
+  int pb;           // workgroups of the gauge shift (halo layout)
+  int own;          // first solve of its operator (builds the shared arrays)
   const uint8_t *code;
   const double *src, *diag, *rdiag;
   double *x, *r, *z, *p[2];
@@ -693,8 +696,125 @@ __global__ void k_b_zero_int(int *p, int n) {
   if (t < n) p[t] = 0;
 }
 
-__global__ void k_b_zero_scalars(Scalars *sc) {
-  if (threadIdx.x == 0) *sc = Scalars{};
+// ---- set-up and finish of every solve (the standalone solve's steps, one
+// launch each over the batch) ------------------------------------------------
+// 1-D coefficients of every operator (solve blockIdx.y, entry blockIdx.x *
+// 256 + threadIdx.x), then their FP32 copies and those of dx, dy, dz.
+__global__ __launch_bounds__(256) void k_bs_axes(const BSolve *solves) {
+  const BSolve &d = solves[blockIdx.y];
+  if (!d.own) return;
+  const Geom &g = d.geom;
+  axis_coefficients_body(blockIdx.x * blockDim.x + threadIdx.x, g.nx, g.ny, g.nz, g.xc, g.yc,
+                         g.zc, g.dx, g.dy, g.dz, (double *)g.cw, (double *)g.ce,
+                         (double *)g.cs, (double *)g.cn, (double *)g.cb, (double *)g.ct);
+}
+
+__global__ __launch_bounds__(256) void k_bs_axes32(const BSolve *solves) {
+  const BSolve &d = solves[blockIdx.y];
+  if (!d.own) return;
+  const Geom &g = d.geom;
+  const int tot = g.nx + g.ny + g.nz, t = blockIdx.x * blockDim.x + threadIdx.x;
+  float *c32 = (float *)d.g32.cw;  // coefficients, then dx, dy, dz
+  if (t < 2 * tot) c32[t] = (float)g.cw[t];
+  else if (t < 2 * tot + g.nx) c32[t] = (float)g.dx[t - 2 * tot];
+  else if (t < 2 * tot + g.nx + g.ny) c32[t] = (float)g.dy[t - 2 * tot - g.nx];
+  else if (t < 3 * tot) c32[t] = (float)g.dz[t - 2 * tot - g.nx - g.ny];
+}
+
+// build_operator for every solve; the first solve of a frame also writes the
+// shared diag, rdiag and neighbour code.
+__global__ __launch_bounds__(256) void k_bs_build(const BSolve *solves,
+                                                  const ffpm_stairstep_problem *probs, Grid grid,
+                                                  int *isolated) {
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  const ffpm_stairstep_problem &p = probs[s];
+  build_operator_body(grid.local[blockIdx.x], d.eb, d.geom, p.solid, p.rhs, p.xlo, p.xhi, p.ylo,
+                      p.yhi, p.zlo, p.zhi, p.body_grad[0], p.body_grad[1], p.body_grad[2], 1,
+                      (double *)d.src, (double *)d.diag, (double *)d.rdiag, (uint8_t *)d.code,
+                      isolated, d.own != 0);
+}
+
+__global__ __launch_bounds__(256) void k_bs_pack(const BSolve *solves,
+                                                 const ffpm_stairstep_problem *probs, Grid grid) {
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  const ffpm_stairstep_problem &p = probs[s];
+  pack_phi_body(grid.local[blockIdx.x], d.eb, p.nx, p.ny, p.nz, p.solid, p.phi, d.x);
+}
+
+__global__ void k_bs_zero_scalars(const BSolve *solves, int count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < count) *solves[s].sc = Scalars{};
+}
+
+// Multigrid level operators of every frame, level l (grid of level l).
+__global__ __launch_bounds__(256) void k_bs_coarsen(const BSolve *solves, Grid grid, int l) {
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  if (!d.own) return;
+  const Multigrid &mg = d.mg;
+  const MgLevel &c = mg.lv[l];
+  const mgk::Blk bk = cell_blk(c, grid.local[blockIdx.x]);
+  if (l == 1) {
+    mgk::mg_coarsen0_body(bk, mg.g64, mg.code, c.nx, c.ny, mg.scale, c.gE, c.gN, c.gT);
+  } else {
+    const MgLevel &f = mg.lv[l - 1];
+    mgk::mg_coarsen_body(bk, f.nx, f.ny, f.nz, f.gE, f.gN, f.gT, c.nx, c.ny, mg.scale, c.gE,
+                         c.gN, c.gT);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bs_mask(const BSolve *solves, Grid grid, int l) {
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  if (!d.own) return;
+  const Multigrid &mg = d.mg;
+  const MgLevel &c = mg.lv[l];
+  mgk::mg_mask_body(cell_blk(c, grid.local[blockIdx.x]), c.nx, c.ny, c.gE, c.gN, c.gT,
+                    mg.first_coarse, c.m, c.rd);
+}
+
+// Gauge (src/fpm_stairstep_mpi_3d.f90:170-185) and the final residual
+// (src/fpm_stairstep_mpi_3d.f90:186-190) of every solve.
+__global__ __launch_bounds__(256) void k_bf_gauge_sums(const BSolve *solves,
+                                                       const ffpm_stairstep_problem *probs,
+                                                       Grid grid) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  gauge_sums_body(grid.local[blockIdx.x], d.eb, d.geom, probs[s].solid, d.x, d.partials, lds);
+}
+
+__global__ __launch_bounds__(256) void k_bf_gauge_apply(const BSolve *solves,
+                                                        const ffpm_stairstep_problem *probs,
+                                                        Grid grid) {
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  const ffpm_stairstep_problem &p = probs[s];
+  gauge_apply_body(grid.local[blockIdx.x], d.pb, p.nx, p.ny, p.nz, p.solid, d.x, p.phi, d.sc);
+}
+
+__global__ __launch_bounds__(512) void k_bf_residual_max(const BSolve *solves, Grid grid) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int s = grid.solve[blockIdx.x];
+  const BSolve &d = solves[s];
+  residual_max_body(d.tt, grid.local[blockIdx.x], d.tt.blocks, d.geom, d.x, d.src, d.diag,
+                    d.partials, lds);
+}
+
+// One workgroup per solve: FIN_SUMS of the gauge (eb partials, two slots) or
+// FIN_MAX of the final residual (tt.blocks partials); with res, the residual
+// of every solve is gathered there.
+__global__ __launch_bounds__(256) void k_bf_finish(const BSolve *solves, int op, double *res) {
+  __shared__ double lds[16 * kReduceSlots];
+  const BSolve &d = solves[blockIdx.x];
+  if (op == FIN_SUMS) finish_body(op, 2, d.partials, d.eb, d.sc, lds);
+  else finish_body(op, 1, d.partials, d.tt.blocks, d.sc, lds);
+  if (res) {
+    __syncthreads();
+    if (threadIdx.x == 0) res[blockIdx.x] = d.sc->res;
+  }
 }
 
 // ------------------------------------------------------------- host --------
@@ -760,6 +880,11 @@ struct Batch {
   bool mgm = false;
   Grid g_cell0, g_col[2], g_lvl[kMgMaxLevels];
   int mg_top = 0;
+  // the problems on the device, the final residuals, the gauge-shift grid
+  ffpm_stairstep_problem *d_probs = nullptr;
+  double *d_res = nullptr;
+  Grid g_pad;
+  int max_axis = 0;  // largest nx + ny + nz
 };
 
 int build_grid(Arena &a, Grid &g, const std::vector<int> &blocks, const std::vector<int> &use,
@@ -815,7 +940,9 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
   }
   // sizes
   size_t total = align256(sizeof(BSolve) * count) + align256(sizeof(BState) * count) +
-                 align256(sizeof(int) * 4) + align256(sizeof(int));
+                 align256(sizeof(int) * 4) + align256(sizeof(int)) +
+                 align256(sizeof(ffpm_stairstep_problem) * count) +
+                 align256(sizeof(double) * count);
   std::vector<Tiling> tl(count), tl32(count), tt(count);
   std::vector<int> eb(count);
   std::vector<Multigrid> plan(mgm ? count : 0);
@@ -833,7 +960,9 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     eb[s] = elementwise_blocks(n);
     const size_t nb = (size_t)std::max(std::max(std::max(tl[s].blocks, tl32[s].blocks),
                                                 tt[s].blocks), eb[s]);
-    grid_blocks += (size_t)tl[s].blocks + tl32[s].blocks + tt[s].blocks + eb[s];
+    grid_blocks += (size_t)tl[s].blocks + tl32[s].blocks + tt[s].blocks + eb[s] +
+                   elementwise_blocks((size_t)(p.nx + 2) * (p.ny + 2) * p.nz);
+    B.max_axis = std::max(B.max_axis, p.nx + p.ny + p.nz);
     total += 6 * align256(sizeof(double) * n) + align256(sizeof(double) * kReduceSlots * nb) +
              align256(sizeof(Scalars));
     if (mixed) total += 5 * align256(sizeof(float) * n);
@@ -861,9 +990,9 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
                align256(sizeof(double) * 2 * (p.nx + p.ny + p.nz)) +
                align256(sizeof(float) * 3 * (p.nx + p.ny + p.nz));
   }
-  // workgroup tables: two int arrays per grid (at most eight grids, plus
+  // workgroup tables: two int arrays per grid (at most nine grids, plus
   // 3 + kMgMaxLevels for the multigrid), each 256-aligned
-  total += 2 * align256(sizeof(int) * grid_blocks) + 2 * 256 * (8 + 3 + kMgMaxLevels);
+  total += 2 * align256(sizeof(int) * grid_blocks) + 2 * 256 * (9 + 3 + kMgMaxLevels);
   int st = 0;
   Arena a;
   a.base = (char *)workspace().get(S_BATCH, total, &st);
@@ -872,6 +1001,8 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
   B.d_state = a.take<BState>(sizeof(BState) * count);
   B.d_counters = a.take<int>(sizeof(int) * 4);
   B.d_isolated = a.take<int>(sizeof(int));
+  B.d_probs = a.take<ffpm_stairstep_problem>(sizeof(ffpm_stairstep_problem) * count);
+  B.d_res = a.take<double>(sizeof(double) * count);
   k_b_zero_int<<<1, 64, 0, stream()>>>(B.d_isolated, 1);
   FFPM_LAUNCHED("k_b_zero_int");
   for (int s = 0; s < count; ++s) {
@@ -884,7 +1015,9 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     d.tl32 = tl32[s];
     d.tt = tt[s];
     d.eb = eb[s];
+    d.pb = elementwise_blocks((size_t)(nx + 2) * (ny + 2) * nz);
     const int o = B.owner[s];
+    d.own = o == s;
     if (o == s) {
       double *coef = a.take<double>(sizeof(double) * 2 * (nx + ny + nz));
       Geom g{};
@@ -894,21 +1027,12 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
       g.dx = p.dx; g.dy = p.dy; g.dz = p.dz;
       g.xc = p.xc; g.yc = p.yc; g.zc = p.zc;
       const int tot = nx + ny + nz;
-      k_axis_coefficients<<<(tot + 255) / 256, 256, 0, stream()>>>(
-          nx, ny, nz, p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, (double *)g.cw, (double *)g.ce,
-          (double *)g.cs, (double *)g.cn, (double *)g.cb, (double *)g.ct);
-      FFPM_LAUNCHED("k_axis_coefficients");
       d.geom = g;
       d.g = GeomT<double>{nx, ny, nz, g.cw, g.ce, g.cs, g.cn, g.cb, g.ct, g.dx, g.dy, g.dz};
-      // FP32 copies for the mixed inner solve (the standalone mixed solve's
-      // conversions): coefficients, then dx, dy, dz
+      // FP32 copies for the mixed inner solve and the multigrid (the
+      // standalone solves' conversions): coefficients, then dx, dy, dz
       float *c32 = a.take<float>(sizeof(float) * 3 * tot);
       const int ncoef = 2 * tot;
-      k_to_float<<<(ncoef + 255) / 256, 256, 0, stream()>>>(ncoef, coef, c32);
-      k_to_float<<<(nx + 255) / 256, 256, 0, stream()>>>(nx, p.dx, c32 + ncoef);
-      k_to_float<<<(ny + 255) / 256, 256, 0, stream()>>>(ny, p.dy, c32 + ncoef + nx);
-      k_to_float<<<(nz + 255) / 256, 256, 0, stream()>>>(nz, p.dz, c32 + ncoef + nx + ny);
-      FFPM_LAUNCHED("k_to_float");
       d.g32 = GeomT<float>{nx, ny, nz, c32, c32 + nx, c32 + 2 * nx, c32 + 2 * nx + ny,
                            c32 + 2 * nx + 2 * ny, c32 + 2 * nx + 2 * ny + nz, c32 + ncoef,
                            c32 + ncoef + nx, c32 + ncoef + nx + ny};
@@ -971,23 +1095,10 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
                                               (double)all_cells));
       d.kc1 = mgk::column_chunk(mg.lv[1].nx, mg.lv[1].ny, mg.lv[1].nz, target);
     }
-    // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379); a frame's later
-    // right-hand sides rewrite the identical diag, rdiag and code.
-    k_build_operator<<<d.eb, kThreads, 0, stream()>>>(
-        d.geom, p.solid, p.rhs, p.xlo, p.xhi, p.ylo, p.yhi, p.zlo, p.zhi, p.body_grad[0],
-        p.body_grad[1], p.body_grad[2], 1, src, (double *)d.diag, (double *)d.rdiag,
-        (uint8_t *)d.code, B.d_isolated);
-    FFPM_LAUNCHED("k_build_operator");
-    k_pack_phi<<<d.eb, kThreads, 0, stream()>>>(nx, ny, nz, p.solid, p.phi, d.x);
-    FFPM_LAUNCHED("k_pack_phi");
-    if (mgm && o == s && (st = mg_build(d.mg))) return st;
-    k_b_zero_scalars<<<1, 64, 0, stream()>>>(d.sc);
-    FFPM_LAUNCHED("k_b_zero_scalars");
   }
-  int isolated = 0;
-  FFPM_HIP(hipMemcpyAsync(&isolated, B.d_isolated, sizeof(int), hipMemcpyDeviceToHost,
-                          stream()));
   FFPM_HIP(hipMemcpyAsync(B.d_solves, B.h.data(), sizeof(BSolve) * count,
+                          hipMemcpyHostToDevice, stream()));
+  FFPM_HIP(hipMemcpyAsync(B.d_probs, P, sizeof(ffpm_stairstep_problem) * count,
                           hipMemcpyHostToDevice, stream()));
   std::vector<int> blocks(count), use(count), hs, hl;
   for (int c = 1; c <= 2; ++c) {
@@ -1038,6 +1149,38 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
       if ((st = build_grid(a, B.g_lvl[l], blocks, use, hs, hl))) return st;
     }
   }
+  for (int s = 0; s < count; ++s) {
+    blocks[s] = B.h[s].pb;
+    use[s] = 1;
+  }
+  if ((st = build_grid(a, B.g_pad, blocks, use, hs, hl))) return st;
+  // the standalone solve's set-up, one launch per step over the batch:
+  // coefficients of every operator, build_operator
+  // (src/fpm_stairstep_mpi_3d.f90:336-379), the dense phi, zero scalars, and
+  // the multigrid level operators
+  const unsigned ax = (unsigned)((3 * B.max_axis + 255) / 256);
+  k_bs_axes<<<dim3(ax, count), 256, 0, stream()>>>(B.d_solves);
+  FFPM_LAUNCHED("k_bs_axes");
+  k_bs_axes32<<<dim3(ax, count), 256, 0, stream()>>>(B.d_solves);
+  FFPM_LAUNCHED("k_bs_axes32");
+  k_bs_build<<<B.g_ew.blocks, kThreads, 0, stream()>>>(B.d_solves, B.d_probs, B.g_ew,
+                                                       B.d_isolated);
+  FFPM_LAUNCHED("k_bs_build");
+  k_bs_pack<<<B.g_ew.blocks, kThreads, 0, stream()>>>(B.d_solves, B.d_probs, B.g_ew);
+  FFPM_LAUNCHED("k_bs_pack");
+  k_bs_zero_scalars<<<(count + 63) / 64, 64, 0, stream()>>>(B.d_solves, count);
+  FFPM_LAUNCHED("k_bs_zero_scalars");
+  if (mgm) {
+    for (int l = 1; l <= B.mg_top; ++l)
+      k_bs_coarsen<<<B.g_lvl[l].blocks, mgk::kMt, 0, stream()>>>(B.d_solves, B.g_lvl[l], l);
+    FFPM_LAUNCHED("k_bs_coarsen");
+    for (int l = 1; l <= B.mg_top; ++l)
+      k_bs_mask<<<B.g_lvl[l].blocks, mgk::kMt, 0, stream()>>>(B.d_solves, B.g_lvl[l], l);
+    FFPM_LAUNCHED("k_bs_mask");
+  }
+  int isolated = 0;
+  FFPM_HIP(hipMemcpyAsync(&isolated, B.d_isolated, sizeof(int), hipMemcpyDeviceToHost,
+                          stream()));
   FFPM_HIP(hipStreamSynchronize(stream()));
   if (isolated > 0) {
     set_error(ERR_ISOLATED, "stair-step fluid cell has no fluid neighbour");
@@ -1351,24 +1494,23 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
   return OK;
 }
 
-// Gauge, unpack and the final residual of one solve (as the standalone solve
-// ends, src/fpm_stairstep_mpi_3d.f90:170-190).
-int batch_finalize(const Batch &B, ffpm_stairstep_problem *P, int s, double *residual) {
-  const BSolve &d = B.h[s];
-  const ffpm_stairstep_problem &p = P[s];
-  k_gauge_sums<<<d.eb, kThreads, 0, stream()>>>(d.geom, p.solid, d.x, d.partials, d.sc);
-  FFPM_LAUNCHED("k_gauge_sums");
-  k_finish<<<1, 256, 0, stream()>>>(FIN_SUMS, 2, d.partials, d.eb, d.sc);
-  FFPM_LAUNCHED("k_finish");
-  const int pb = elementwise_blocks((size_t)(p.nx + 2) * (p.ny + 2) * p.nz);
-  k_gauge_apply<<<pb, kThreads, 0, stream()>>>(p.nx, p.ny, p.nz, p.solid, d.x, p.phi, d.sc);
-  FFPM_LAUNCHED("k_gauge_apply");
-  k_residual_max<<<d.tt.blocks, dim3(kTileX, d.tt.ty), 0, stream()>>>(d.tt, d.geom, d.x, d.src,
-                                                                        d.diag, d.partials, d.sc);
-  FFPM_LAUNCHED("k_residual_max");
-  k_finish<<<1, 256, 0, stream()>>>(FIN_MAX, 1, d.partials, d.tt.blocks, d.sc);
-  FFPM_LAUNCHED("k_finish");
-  FFPM_HIP(hipMemcpyAsync(residual, &d.sc->res, sizeof(double), hipMemcpyDeviceToHost,
+// Gauge, unpack and the final residual of every solve (as the standalone
+// solve ends, src/fpm_stairstep_mpi_3d.f90:170-190), one launch per step;
+// the residuals land in res (host).
+int batch_finalize(const Batch &B, std::vector<double> &res) {
+  k_bf_gauge_sums<<<B.g_ew.blocks, kThreads, 0, stream()>>>(B.d_solves, B.d_probs, B.g_ew);
+  FFPM_LAUNCHED("k_bf_gauge_sums");
+  k_bf_finish<<<B.count, 256, 0, stream()>>>(B.d_solves, FIN_SUMS, nullptr);
+  FFPM_LAUNCHED("k_bf_finish");
+  k_bf_gauge_apply<<<B.g_pad.blocks, kThreads, 0, stream()>>>(B.d_solves, B.d_probs, B.g_pad);
+  FFPM_LAUNCHED("k_bf_gauge_apply");
+  k_bf_residual_max<<<B.g_tr.blocks, dim3(kTileX, B.h[0].tt.ty), 0, stream()>>>(B.d_solves,
+                                                                                 B.g_tr);
+  FFPM_LAUNCHED("k_bf_residual_max");
+  k_bf_finish<<<B.count, 256, 0, stream()>>>(B.d_solves, FIN_MAX, B.d_res);
+  FFPM_LAUNCHED("k_bf_finish");
+  res.resize(B.count);
+  FFPM_HIP(hipMemcpyAsync(res.data(), B.d_res, sizeof(double) * B.count, hipMemcpyDeviceToHost,
                           stream()));
   return OK;
 }
@@ -1407,9 +1549,8 @@ int ffpm_solve_stairstep_batch_dev(ffpm_stairstep_problem *problems, int count,
   if ((st = batch_run(B, tolerance, max_sweeps, check_interval, use_graph, out, replay_ms,
                       &loop_ms)))
     return st;
-  std::vector<double> res(count);
-  for (int s = 0; s < count; ++s)
-    if ((st = batch_finalize(B, problems, s, &res[s]))) return st;
+  std::vector<double> res;
+  if ((st = batch_finalize(B, res))) return st;
   FFPM_HIP(hipStreamSynchronize(stream()));
   for (int s = 0; s < count; ++s) {
     problems[s].sweeps = B.mixed ? out[s].inner : out[s].sweeps;

@@ -123,44 +123,14 @@ __global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<double> g,
                                         partials, sc, lds);
 }
 
-// ---- setup --------------------------------------------------------------------
-// Level-1 conductances from level 0 (FP64 sums, stored FP32): half the sum of
-// V c over the fluid-fluid level-0 faces on each aggregate face.
+// ---- setup ----------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_mg_coarsen0(GeomT<double> g,
                                                      const uint8_t *__restrict__ code, int cnx,
                                                      int cny, float scale,
                                                      float *__restrict__ gE,
                                                      float *__restrict__ gN,
                                                      float *__restrict__ gT) {
-  const Idx t = cell_idx(cnx, cny, blk3());
-  if (!t.in) return;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  auto at = [&](int i, int j, int k) {
-    return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-  };
-  double se = 0.0, sn = 0.0, st = 0.0;
-  const int i1 = 2 * t.i + 1, j1 = 2 * t.j + 1, k1 = 2 * t.k + 1;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      {  // +x face: level-0 cells (i1, j, k), j = 2J+a, k = 2K+b
-        const int j = 2 * t.j + a, k = 2 * t.k + b;
-        if (i1 < nx && j < ny && k < nz && (code[at(i1, j, k)] & kCodeE))
-          se += g.dx[i1] * g.dy[j] * g.dz[k] * g.ce[i1];
-      }
-      {  // +y face
-        const int i = 2 * t.i + a, k = 2 * t.k + b;
-        if (j1 < ny && i < nx && k < nz && (code[at(i, j1, k)] & kCodeN))
-          sn += g.dx[i] * g.dy[j1] * g.dz[k] * g.cn[j1];
-      }
-      {  // +z face
-        const int i = 2 * t.i + a, j = 2 * t.j + b;
-        if (k1 < nz && i < nx && j < ny && (code[at(i, j, k1)] & kCodeT))
-          st += g.dx[i] * g.dy[j] * g.dz[k1] * g.ct[k1];
-      }
-    }
-  gE[t.c] = (float)(scale * se);
-  gN[t.c] = (float)(scale * sn);
-  gT[t.c] = (float)(scale * st);
+  mg_coarsen0_body(blk3(), g, code, cnx, cny, scale, gE, gN, gT);
 }
 
 __global__ __launch_bounds__(256) void k_mg_coarsen(int nx, int ny, int nz,
@@ -171,42 +141,14 @@ __global__ __launch_bounds__(256) void k_mg_coarsen(int nx, int ny, int nz,
                                                     float *__restrict__ gE,
                                                     float *__restrict__ gN,
                                                     float *__restrict__ gT) {
-  const Idx t = cell_idx(cnx, cny, blk3());
-  if (!t.in) return;
-  auto at = [&](int i, int j, int k) {
-    return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
-  };
-  float se = 0.f, sn = 0.f, st = 0.f;
-  const int i1 = 2 * t.i + 1, j1 = 2 * t.j + 1, k1 = 2 * t.k + 1;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      {
-        const int j = 2 * t.j + a, k = 2 * t.k + b;
-        if (i1 < nx && j < ny && k < nz) se += fE[at(i1, j, k)];
-      }
-      {
-        const int i = 2 * t.i + a, k = 2 * t.k + b;
-        if (j1 < ny && i < nx && k < nz) sn += fN[at(i, j1, k)];
-      }
-      {
-        const int i = 2 * t.i + a, j = 2 * t.j + b;
-        if (k1 < nz && i < nx && j < ny) st += fT[at(i, j, k1)];
-      }
-    }
-  gE[t.c] = scale * se;
-  gN[t.c] = scale * sn;
-  gT[t.c] = scale * st;
+  mg_coarsen_body(blk3(), nx, ny, nz, fE, fN, fT, cnx, cny, scale, gE, gN, gT);
 }
 
 __global__ __launch_bounds__(256) void k_mg_mask(int nx, int ny, const float *__restrict__ gE,
                                                  const float *__restrict__ gN,
-                                                 const float *__restrict__ gT, float omega,
+                                                 const float *__restrict__ gT, float first,
                                                  float *__restrict__ m, float *__restrict__ rd) {
-  const Idx t = cell_idx(nx, ny, blk3());
-  if (!t.in) return;
-  const float d = coarse_diag(nx, ny, gE, gN, gT, t.c, t.i, t.j, t.k);
-  m[t.c] = d > 0.f ? 1.f : 0.f;
-  rd[t.c] = d > 0.f ? omega / d : 0.f;
+  mg_mask_body(blk3(), nx, ny, gE, gN, gT, first, m, rd);
 }
 
 float env_float(const char *name, float fallback) {

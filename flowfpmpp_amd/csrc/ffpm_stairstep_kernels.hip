@@ -10,13 +10,9 @@
 // mask-free: solid cells hold zero in every iterate and zero src/rdiag/diag,
 // and the zero physical halo is substituted instead of stored.
 #include "ffpm_bodies.hpp"
+#include "ffpm_frame_body.hpp"
 
 namespace ffpm {
-
-// (0:nx+1, 0:ny+1, 1:nz) halo layout, 0-based z
-__device__ __forceinline__ size_t ph(int i, int j, int k, int nx, int ny) {
-  return (size_t)i + (size_t)(nx + 2) * ((size_t)j + (size_t)(ny + 2) * (size_t)k);
-}
 
 // axis_coefficients (src/fpm_stairstep_mpi_3d.f90:499-510) and the z
 // coefficients (src/fpm_stairstep_mpi_3d.f90:138-142), one thread per entry.
@@ -24,25 +20,12 @@ __global__ void k_axis_coefficients(int nx, int ny, int nz, const double *xc, co
                                     const double *zc, const double *dx, const double *dy,
                                     const double *dz, double *cw, double *ce, double *cs,
                                     double *cn, double *cb, double *ct) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nx) {
-    const int m = t + 1;
-    cw[t] = (m == 1) ? 0.0 : 1 / (dx[t] * (xc[m] - xc[m - 1]));
-    ce[t] = (m == nx) ? 0.0 : 1 / (dx[t] * (xc[m + 1] - xc[m]));
-  } else if (t < nx + ny) {
-    const int u = t - nx, m = u + 1;
-    cs[u] = (m == 1) ? 0.0 : 1 / (dy[u] * (yc[m] - yc[m - 1]));
-    cn[u] = (m == ny) ? 0.0 : 1 / (dy[u] * (yc[m + 1] - yc[m]));
-  } else if (t < nx + ny + nz) {
-    const int k = t - nx - ny;  // 0-based; zc is (1:nz)
-    cb[k] = (k > 0) ? 1 / (dz[k] * (zc[k] - zc[k - 1])) : 0.0;
-    ct[k] = (k < nz - 1) ? 1 / (dz[k] * (zc[k + 1] - zc[k])) : 0.0;
-  }
+  axis_coefficients_body(blockIdx.x * blockDim.x + threadIdx.x, nx, ny, nz, xc, yc, zc, dx, dy,
+                         dz, cw, ce, cs, cn, cb, ct);
 }
 
-// build_operator (src/fpm_stairstep_mpi_3d.f90:336-379): folds the outer and
-// body face gradients into src, sums fluid-neighbour coefficients into diag,
-// and sets rdiag = 1/(V diag) (PCG) or 0.8/diag (Jacobi); 0 in solids.
+// build_operator (src/fpm_stairstep_mpi_3d.f90:336-379); body in
+// ffpm_frame_body.hpp.
 __global__ __launch_bounds__(512) void k_build_operator(
     Geom g, const int32_t *__restrict__ solid, const double *__restrict__ rhs,
     const double *__restrict__ xlo, const double *__restrict__ xhi,
@@ -50,64 +33,14 @@ __global__ __launch_bounds__(512) void k_build_operator(
     const double *__restrict__ zlo, const double *__restrict__ zhi, double bg0, double bg1,
     double bg2, int for_pcg, double *__restrict__ src, double *__restrict__ diag,
     double *__restrict__ rdiag, uint8_t *__restrict__ code, int *__restrict__ isolated) {
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t n = (size_t)nx * ny * nz;
-  int local_isolated = 0;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx);
-    const int j = (int)((c / nx) % ny);
-    const int k = (int)(c / ((size_t)nx * ny));
-    const int I = i + 1, J = j + 1;
-    double s = 0, d = 0;
-    unsigned cb = 0;
-    if (solid[ph(I, J, k, nx, ny)] == 0) {
-      const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
-      cb = kCodeFluid;
-      s = -rhs[c];
-      if (I == 1) s = s - xlo[j + (size_t)ny * k] / dxi;
-      else if (solid[ph(I - 1, J, k, nx, ny)] != 0) s = s - bg0 / dxi;
-      else { d = d + g.cw[i]; cb |= kCodeW; }
-      if (I == nx) s = s + xhi[j + (size_t)ny * k] / dxi;
-      else if (solid[ph(I + 1, J, k, nx, ny)] != 0) s = s + bg0 / dxi;
-      else { d = d + g.ce[i]; cb |= kCodeE; }
-      if (J == 1) s = s - ylo[i + (size_t)nx * k] / dyj;
-      else if (solid[ph(I, J - 1, k, nx, ny)] != 0) s = s - bg1 / dyj;
-      else { d = d + g.cs[j]; cb |= kCodeS; }
-      if (J == ny) s = s + yhi[i + (size_t)nx * k] / dyj;
-      else if (solid[ph(I, J + 1, k, nx, ny)] != 0) s = s + bg1 / dyj;
-      else { d = d + g.cn[j]; cb |= kCodeN; }
-      if (k == 0) s = s - zlo[i + (size_t)nx * j] / dzk;
-      else if (solid[ph(I, J, k - 1, nx, ny)] != 0) s = s - bg2 / dzk;
-      else { d = d + g.cb[k]; cb |= kCodeB; }
-      if (k == nz - 1) s = s + zhi[i + (size_t)nx * j] / dzk;
-      else if (solid[ph(I, J, k + 1, nx, ny)] != 0) s = s + bg2 / dzk;
-      else { d = d + g.ct[k]; cb |= kCodeT; }
-      if (d <= 0) local_isolated += 1;
-    }
-    src[c] = s;
-    diag[c] = d;
-    code[c] = (uint8_t)cb;
-    double rd;
-    if (d <= 0) rd = 0;
-    else if (for_pcg) rd = 1 / (g.dx[i] * g.dy[j] * g.dz[k] * d);
-    else rd = 0.8 / d;
-    rdiag[c] = rd;
-  }
-  if (local_isolated) atomicAdd(isolated, local_isolated);
+  build_operator_body(blockIdx.x, gridDim.x, g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi, bg0,
+                      bg1, bg2, for_pcg, src, diag, rdiag, code, isolated, true);
 }
 
-// Init loop of src/fpm_stairstep_mpi_3d.f90:154-160 on the dense copy: phi in
-// solid cells becomes 0 (the halo is substituted, not stored).
+// Init loop of src/fpm_stairstep_mpi_3d.f90:154-160 on the dense copy.
 __global__ void k_pack_phi(int nx, int ny, int nz, const int32_t *__restrict__ solid,
                            const double *__restrict__ phi_ref, double *__restrict__ phi) {
-  const size_t n = (size_t)nx * ny * nz;
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
-    const size_t p = ph(i + 1, j + 1, k, nx, ny);
-    phi[c] = solid[p] != 0 ? 0.0 : phi_ref[p];
-  }
+  pack_phi_body(blockIdx.x, gridDim.x, nx, ny, nz, solid, phi_ref, phi);
 }
 
 // Writes the dense field back to the caller's halo layout with the physical
@@ -155,44 +88,15 @@ __global__ __launch_bounds__(512) void k_jacobi(Tiling t, Geom g, const double *
   }
 }
 
-// operator_residual (src/fpm_stairstep_mpi_3d.f90:514-530): max-norm of
-// src + sum(c phi_nb) - diag phi over diag > 0. Result -> sc->res.
+// operator_residual (src/fpm_stairstep_mpi_3d.f90:514-530); the partials'
+// maximum goes to sc->res (k_finish FIN_MAX).
 __global__ __launch_bounds__(512) void k_residual_max(Tiling t, Geom g,
                                                       const double *__restrict__ phi,
                                                       const double *__restrict__ src,
                                                       const double *__restrict__ diag,
                                                       double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz);
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double m = 0.0;
-  if (tc.valid && tc.i < nx && tc.j < ny) {
-    const int i = tc.i, j = tc.j;
-    const size_t nxy = (size_t)nx * ny;
-    size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * tc.k0);
-    const double cw = g.cw[i], ce = g.ce[i], cs = g.cs[j], cn = g.cn[j];
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    double p_cur = phi[c];
-    double p_prev = tc.k0 > 0 ? phi[c - nxy] : p_cur;
-    for (int k = tc.k0; k < tc.k1; ++k) {
-      const double p_next = (k + 1 < nz) ? phi[c + nxy] : p_cur;
-      const double d = diag[c];
-      if (d > 0) {
-        const double pw = hw ? phi[c - 1] : 0.0;
-        const double pe = he ? phi[c + 1] : 0.0;
-        const double ps = hs ? phi[c - nx] : 0.0;
-        const double pn = hn ? phi[c + nx] : 0.0;
-        const double v = fabs(src[c] + cw * pw + ce * pe + cs * ps + cn * pn + g.cb[k] * p_prev +
-                              g.ct[k] * p_next - d * p_cur);
-        m = fmax(m, v);
-      }
-      p_prev = p_cur;
-      p_cur = p_next;
-      c += nxy;
-    }
-  }
-  block_max(m, lds);
-  store_partials(partials, 1, &m, gridDim.x);
+  residual_max_body(t, blockIdx.x, gridDim.x, g, phi, src, diag, partials, lds);
 }
 
 // true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319); body in
@@ -317,49 +221,21 @@ __global__ void k_clear_status(Scalars *sc) {
   }
 }
 
-// Gauge sums (src/fpm_stairstep_mpi_3d.f90:170-179): fluid-volume integral
-// of phi and the fluid volume -> sc->sums[0..1].
+// Gauge sums (src/fpm_stairstep_mpi_3d.f90:170-179) -> sc->sums[0..1] after
+// k_finish FIN_SUMS.
 __global__ __launch_bounds__(512) void k_gauge_sums(Geom g, const int32_t *__restrict__ solid,
                                                    const double *__restrict__ phi,
                                                    double *partials, Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const size_t n = (size_t)nx * ny * nz;
-  double s[2] = {0.0, 0.0};
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
-    if (solid[ph(i + 1, j + 1, k, nx, ny)] == 0) {
-      s[0] += phi[c] * g.dx[i] * g.dy[j] * g.dz[k];
-      s[1] += g.dx[i] * g.dy[j] * g.dz[k];
-    }
-  }
-  block_sum<2>(s, lds);
-  store_partials(partials, 2, s, gridDim.x);
+  gauge_sums_body(blockIdx.x, gridDim.x, g, solid, phi, partials, lds);
 }
 
-// Gauge shift (src/fpm_stairstep_mpi_3d.f90:180-185) on the dense field, with
-// the caller's halo-layout copy written in the same pass.
+// Gauge shift (src/fpm_stairstep_mpi_3d.f90:180-185) with the caller's
+// halo-layout copy written in the same pass.
 __global__ void k_gauge_apply(int nx, int ny, int nz, const int32_t *__restrict__ solid,
                               double *__restrict__ phi, double *__restrict__ phi_ref,
                               const Scalars *sc) {
-  const double mean_value = sc->sums[0] / sc->sums[1];
-  const size_t n = (size_t)(nx + 2) * (ny + 2) * nz;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
-       p += (size_t)gridDim.x * blockDim.x) {
-    const int I = (int)(p % (nx + 2)), J = (int)((p / (nx + 2)) % (ny + 2));
-    const int k = (int)(p / ((size_t)(nx + 2) * (ny + 2)));
-    double v = 0.0;
-    if (I >= 1 && I <= nx && J >= 1 && J <= ny) {
-      const size_t c = (size_t)(I - 1) + (size_t)nx * ((size_t)(J - 1) + (size_t)ny * k);
-      v = phi[c];
-      if (solid[p] == 0) {
-        v = v - mean_value;
-        phi[c] = v;
-      }
-    }
-    phi_ref[p] = v;
-  }
+  gauge_apply_body(blockIdx.x, gridDim.x, nx, ny, nz, solid, phi, phi_ref, sc);
 }
 
 // Flux balance of src/fpm_stairstep_mpi_3d.f90:49-52,69-91: optional fluid
