@@ -369,6 +369,14 @@ def bench_reference(args):
                      "reference_iterations": it_ref, "residual": res, "frame_s": round(t, 4),
                      "a100_frame_s": a100, "speedup_vs_a100": round(a100 / t, 2)})
         log(f"PCG {g}: {sw} iterations, {t:.3f} s (A100 {a100} s)")
+        # the same frame to the same tolerance with the multigrid CG (no
+        # counterpart in the reference): the time a caller of this library
+        # waits for the frame
+        t, sw, res = frame_time(p, "fcg-mg", 1e-8, 1000000, 100)
+        rows.append({"grid": list(g), "method": "fcg-mg", "tol": 1e-8, "iterations": sw,
+                     "residual": res, "frame_s": round(t, 4), "a100_pcg_frame_s": a100,
+                     "speedup_vs_a100_pcg": round(a100 / t, 2)})
+        log(f"fcg-mg {g}: {sw} iterations, {t:.3f} s (A100 PCG {a100} s)")
     for g, (a100, a100_sweep) in A100_JACOBI_FRAME.items():
         p = W.reference_benchmark(*g)
         t, sw, res = frame_time(p, "jacobi", -1.0, 250, 250)
