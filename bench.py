@@ -96,20 +96,32 @@ def frame(grid):
                              label=f"{nx}x{ny}x{nz}")
 
 
-def cpu_baseline(p, gpu_iters, tol, method, budget_s=25.0):
-    """Times the CPU oracle (single thread) on this host: the marginal cost of
-    one iteration from two fixed-work runs of the x-direction solve, plus the
-    fixed setup/gauge cost, scaled to the GPU's iteration counts."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, gpu_iters, tol, method, budget_s=20.0, parallel=False):
+    """Times the CPU oracle on this host (one thread, or the OpenMP build on
+    every thread OMP_NUM_THREADS allows): the marginal cost of one iteration
+    from two fixed-work runs of the x-direction solve, plus the fixed
+    setup/gauge cost, scaled to the GPU's iteration counts."""
     from oracle import oracle as orc
     from flowfpmpp_amd import workloads as W
     rhs, bg, faces = W.influence_inputs(p, 0)
     orc.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
 
     def run(k):
         phi = p.phi0()
         t0 = time.perf_counter()
         orc.solve_stairstep_poisson_3d(phi, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc, p.dx,
-                                       p.dy, p.dz, -1.0, k, k, method)
+                                       p.dy, p.dz, -1.0, k, k, method, parallel=parallel)
         return time.perf_counter() - t0
 
     t1 = run(1)
@@ -119,15 +131,18 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=25.0):
     per_iter = (t2 - t1) / (k2 - 1)
     setup = max(t1 - per_iter, 0.0)
     est_s = sum(setup + per_iter * it for it in gpu_iters)
+    cores = threads if parallel else 1
+    how = (f"OpenMP over z planes, {cores} threads" if parallel else "1 thread")
     return {
         "value": est_s * 1e3,
         "unit": "ms",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": (f"CPU oracle (gcc -O2, 1 thread), {method} on this frame, direction x: "
-                   f"1 and {k2} fixed iterations -> {per_iter * 1e3:.1f} ms/iteration + "
-                   f"{setup * 1e3:.0f} ms setup/gauge, scaled to the GPU's iteration counts "
-                   f"{list(gpu_iters)} over the three directions"),
+        "sample": (f"CPU oracle (gcc -O2, {how}; {cpu_model()}, nproc {os.cpu_count()}), "
+                   f"{method} on this frame, direction x: 1 and {k2} fixed iterations -> "
+                   f"{per_iter * 1e3:.1f} ms/iteration + {setup * 1e3:.0f} ms setup/gauge, "
+                   f"scaled to the GPU's iteration counts {list(gpu_iters)} over the three "
+                   f"directions"),
     }
 
 
@@ -574,9 +589,17 @@ def main():
                            "speedup": round(pcg_ms / ms_per_step, 2),
                            "max_rel_diff_of_force_totals": dtot}
     if not args.no_cpu and pcg_iters:
-        log("timing the CPU oracle baseline")
-        out["cpu_baseline"] = cpu_baseline(p, pcg_iters, args.tol,
-                                           "jacobi" if args.method == "jacobi" else "pcg")
+        # the reference's CPU path runs many MPI ranks on a node: the baseline
+        # is the all-thread OpenMP build of the oracle, the serial one beside it
+        cm = "jacobi" if args.method == "jacobi" else "pcg"
+        if cm == "pcg":
+            log("timing the CPU oracle baseline (all threads)")
+            out["cpu_baseline"] = cpu_baseline(p, pcg_iters, args.tol, cm, parallel=True)
+            log("timing the CPU oracle baseline (one thread)")
+            out["cpu_baseline_serial"] = cpu_baseline(p, pcg_iters, args.tol, cm)
+        else:
+            log("timing the CPU oracle baseline")
+            out["cpu_baseline"] = cpu_baseline(p, pcg_iters, args.tol, cm)
     print(json.dumps(out), flush=True)
 
 

@@ -18,6 +18,14 @@
 /* (0:nx+1, 0:ny+1, 1:nz) */
 #define PH(i, j, k) ((size_t)(i) + (size_t)(nx + 2) * ((size_t)(j) + (size_t)(ny + 2) * (size_t)((k) - 1)))
 /* (1:nx, 1:ny, 1:nz) */
+/* The parallel CPU-baseline build (-fopenmp -DFFPM_ORACLE_OMP) splits the
+ * z-plane loops of the PCG path over threads; the checker build is serial, so
+ * its sums keep the reference's order. */
+#ifdef FFPM_ORACLE_OMP
+#define ORC_OMP(x) _Pragma(#x)
+#else
+#define ORC_OMP(x)
+#endif
 #define CL(i, j, k) ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)ny * (size_t)((k) - 1)))
 /* (0:nx+1, 0:ny+1, 0:nz+1) */
 #define PL(i, j, k) ((size_t)(i) + (size_t)(nx + 2) * ((size_t)(j) + (size_t)(ny + 2) * (size_t)(k)))
@@ -200,6 +208,7 @@ double orc_operator_residual_3d(const double *phi, const double *src, const doub
                                 const double *cn, const double *cb, const double *ct, int nx,
                                 int ny, int nz) {
   double local = 0;
+  ORC_OMP(omp parallel for reduction(max : local) schedule(static))
   for (int k = 1; k <= nz; ++k) {
     const int km = k > 1 ? k - 1 : 1, kp = k < nz ? k + 1 : nz;
     for (int j = 1; j <= ny; ++j)
@@ -251,6 +260,7 @@ static void true_residual(solve_state *st, int reset, double *rz_out, double *re
   *res_out = orc_operator_residual_3d(phi, src, diag, st->cw, st->ce, st->cs, st->cn, st->cb,
                                       st->ct, nx, ny, nz);
   double red1 = 0, red2 = 0;
+  ORC_OMP(omp parallel for reduction(+ : red1, red2) schedule(static))
   for (int k = 1; k <= nz; ++k) {
     const int km = k > 1 ? k - 1 : 1, kp = k < nz ? k + 1 : nz;
     for (int j = 1; j <= ny; ++j)
@@ -270,6 +280,7 @@ static void true_residual(solve_state *st, int reset, double *rz_out, double *re
   }
   const double shift = red1 / red2;
   double rz = 0;
+  ORC_OMP(omp parallel for reduction(+ : rz) schedule(static))
   for (int k = 1; k <= nz; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
@@ -294,6 +305,7 @@ static void pcg_iterate(solve_state *st, double tolerance, int max_sweeps, int c
   best = res;
   while (*sweeps < max_sweeps && res > tolerance) {
     pq = 0;
+    ORC_OMP(omp parallel for reduction(+ : pq) schedule(static))
     for (int k = 1; k <= nz; ++k) {
       const int km = k > 1 ? k - 1 : 1, kp = k < nz ? k + 1 : nz;
       for (int j = 1; j <= ny; ++j)
@@ -319,6 +331,7 @@ static void pcg_iterate(solve_state *st, double tolerance, int max_sweeps, int c
     restarts = 0;
     alpha = rz / pq;
     rz_new = 0;
+    ORC_OMP(omp parallel for reduction(+ : rz_new) schedule(static))
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) {
@@ -330,6 +343,7 @@ static void pcg_iterate(solve_state *st, double tolerance, int max_sweeps, int c
     if (!(rz_new >= 0)) break;
     beta = rz_new / rz;
     rz = rz_new;
+    ORC_OMP(omp parallel for schedule(static))
     for (int k = 1; k <= nz; ++k)
       for (int j = 1; j <= ny; ++j)
         for (int i = 1; i <= nx; ++i) {

@@ -19,6 +19,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libffpm_oracle.so")
+# the same restatement with the PCG path's plane loops over OpenMP threads
+# (CPU baseline only: its sums are not in the reference's order)
+LIB_OMP_PATH = os.path.join(HERE, "_build", "libffpm_oracle_omp.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_laplace.so")
 REF_PART_PATH = os.path.join(HERE, "_ref", "libref_partition.so")
 
@@ -32,6 +35,7 @@ ERRORS = {
 }
 
 _lib = None
+_lib_omp = None
 _ref = None
 _ref_part = None
 
@@ -56,6 +60,16 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_operator_residual_3d.restype = C.c_double
     return _lib
+
+
+def lib_omp():
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(LIB_OMP_PATH):
+            build()
+        _lib_omp = C.CDLL(LIB_OMP_PATH)
+        _lib_omp.orc_operator_residual_3d.restype = C.c_double
+    return _lib_omp
 
 
 def ref_available() -> bool:
@@ -116,11 +130,12 @@ def enforce_stairstep_compatibility_3d(rhs, solid, body_grad, xlo, xhi, ylo, yhi
 def solve_stairstep_poisson_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc,
                                yc, zc, dx, dy, dz, tolerance, max_sweeps, check_interval,
                                method="pcg", west=PROC_NULL, east=PROC_NULL, south=PROC_NULL,
-                               north=PROC_NULL):
+                               north=PROC_NULL, parallel=False):
+    """parallel: the OpenMP build (timing only; the checker is serial)."""
     nx, ny, nz = _dims(solid)
     sweeps, res = C.c_int(), C.c_double()
     bg = np.asfortranarray(body_grad, dtype=np.float64)
-    _check(lib().orc_solve_stairstep_poisson_3d(
+    _check((lib_omp() if parallel else lib()).orc_solve_stairstep_poisson_3d(
         _p(phi), _p(rhs), _p(solid, np.int32), _p(bg), _p(xlo), _p(xhi), _p(ylo), _p(yhi),
         _p(zlo), _p(zhi), _p(xc), _p(yc), _p(zc), _p(dx), _p(dy), _p(dz), nx, ny, nz, west,
         east, south, north, _d(tolerance), int(max_sweeps), int(check_interval),
