@@ -12,7 +12,7 @@ the errors e_k = phi* - phi_k of its iterates after k iterations, with the
 exact residual r_k = A e_k (so the network learns A^-1 on the residuals FCG
 meets, from smooth early ones to the late ones).
 
-No counterpart in the reference (north-star addition).
+No counterpart in the reference.
 """
 from __future__ import annotations
 
