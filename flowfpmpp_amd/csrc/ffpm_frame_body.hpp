@@ -16,6 +16,36 @@ __device__ __forceinline__ size_t ph(int i, int j, int k, int nx, int ny) {
   return (size_t)i + (size_t)(nx + 2) * ((size_t)j + (size_t)(ny + 2) * (size_t)k);
 }
 
+// (i, j, k) of the cells a grid-stride loop visits in an nx x ny x (any)
+// box: the first from one division, every later one by adding the stride's
+// own (i, j, k) with a carry per axis (no 64-bit division per cell).
+struct Walk3 {
+  unsigned i, j, k, si, sj, sk, nx, ny;
+  __device__ __forceinline__ Walk3(size_t c, size_t stride, int nx_, int ny_)
+      : nx((unsigned)nx_), ny((unsigned)ny_) {
+    const size_t nxy = (size_t)nx_ * ny_;
+    i = (unsigned)(c % nx_);
+    j = (unsigned)((c / nx_) % ny_);
+    k = (unsigned)(c / nxy);
+    si = (unsigned)(stride % nx_);
+    sj = (unsigned)((stride / nx_) % ny_);
+    sk = (unsigned)(stride / nxy);
+  }
+  __device__ __forceinline__ void next() {
+    i += si;
+    j += sj;
+    k += sk;
+    if (i >= nx) {
+      i -= nx;
+      ++j;
+    }
+    if (j >= ny) {
+      j -= ny;
+      ++k;
+    }
+  }
+};
+
 // axis_coefficients (src/fpm_stairstep_mpi_3d.f90:499-510) and the z
 // coefficients (src/fpm_stairstep_mpi_3d.f90:138-142): entry t of the
 // nx + ny + nz.
@@ -56,36 +86,44 @@ __device__ __forceinline__ void build_operator_body(
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)nx * ny * nz;
   int local_isolated = 0;
-  for (size_t c = block * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)nblocks * blockDim.x) {
-    const int i = (int)(c % nx);
-    const int j = (int)((c / nx) % ny);
-    const int k = (int)(c / ((size_t)nx * ny));
+  const size_t c0 = block * (size_t)blockDim.x + threadIdx.x, stride = (size_t)nblocks * blockDim.x;
+  Walk3 w(c0, stride, nx, ny);
+  for (size_t c = c0; c < n; c += stride, w.next()) {
+    const int i = (int)w.i, j = (int)w.j, k = (int)w.k;
     const int I = i + 1, J = j + 1;
+    // every operand is loaded before the branches (a load under each branch
+    // waits for the previous one); the arithmetic keeps the reference's order
+    const size_t pc = ph(I, J, k, nx, ny), pxy = (size_t)(nx + 2) * (ny + 2);
+    const int32_t s0 = solid[pc], sw = solid[pc - 1], se = solid[pc + 1],
+                  ss = solid[pc - (nx + 2)], sn = solid[pc + (nx + 2)],
+                  sb = k > 0 ? solid[pc - pxy] : 0, st = k < nz - 1 ? solid[pc + pxy] : 0;
+    const double rv = rhs[c];
+    const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
+    const double cwi = g.cw[i], cei = g.ce[i], csj = g.cs[j], cnj = g.cn[j], cbk = g.cb[k],
+                 ctk = g.ct[k];
     double s = 0, d = 0;
     unsigned cb = 0;
-    if (solid[ph(I, J, k, nx, ny)] == 0) {
-      const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
+    if (s0 == 0) {
       cb = kCodeFluid;
-      s = -rhs[c];
+      s = -rv;
       if (I == 1) s = s - xlo[j + (size_t)ny * k] / dxi;
-      else if (solid[ph(I - 1, J, k, nx, ny)] != 0) s = s - bg0 / dxi;
-      else { d = d + g.cw[i]; cb |= kCodeW; }
+      else if (sw != 0) s = s - bg0 / dxi;
+      else { d = d + cwi; cb |= kCodeW; }
       if (I == nx) s = s + xhi[j + (size_t)ny * k] / dxi;
-      else if (solid[ph(I + 1, J, k, nx, ny)] != 0) s = s + bg0 / dxi;
-      else { d = d + g.ce[i]; cb |= kCodeE; }
+      else if (se != 0) s = s + bg0 / dxi;
+      else { d = d + cei; cb |= kCodeE; }
       if (J == 1) s = s - ylo[i + (size_t)nx * k] / dyj;
-      else if (solid[ph(I, J - 1, k, nx, ny)] != 0) s = s - bg1 / dyj;
-      else { d = d + g.cs[j]; cb |= kCodeS; }
+      else if (ss != 0) s = s - bg1 / dyj;
+      else { d = d + csj; cb |= kCodeS; }
       if (J == ny) s = s + yhi[i + (size_t)nx * k] / dyj;
-      else if (solid[ph(I, J + 1, k, nx, ny)] != 0) s = s + bg1 / dyj;
-      else { d = d + g.cn[j]; cb |= kCodeN; }
+      else if (sn != 0) s = s + bg1 / dyj;
+      else { d = d + cnj; cb |= kCodeN; }
       if (k == 0) s = s - zlo[i + (size_t)nx * j] / dzk;
-      else if (solid[ph(I, J, k - 1, nx, ny)] != 0) s = s - bg2 / dzk;
-      else { d = d + g.cb[k]; cb |= kCodeB; }
+      else if (sb != 0) s = s - bg2 / dzk;
+      else { d = d + cbk; cb |= kCodeB; }
       if (k == nz - 1) s = s + zhi[i + (size_t)nx * j] / dzk;
-      else if (solid[ph(I, J, k + 1, nx, ny)] != 0) s = s + bg2 / dzk;
-      else { d = d + g.ct[k]; cb |= kCodeT; }
+      else if (st != 0) s = s + bg2 / dzk;
+      else { d = d + ctk; cb |= kCodeT; }
       if (d <= 0) local_isolated += 1;
     }
     src[c] = s;
@@ -94,7 +132,7 @@ __device__ __forceinline__ void build_operator_body(
       code[c] = (uint8_t)cb;
       double rd;
       if (d <= 0) rd = 0;
-      else if (for_pcg) rd = 1 / (g.dx[i] * g.dy[j] * g.dz[k] * d);
+      else if (for_pcg) rd = 1 / (dxi * dyj * dzk * d);
       else rd = 0.8 / d;
       rdiag[c] = rd;
     }
@@ -109,10 +147,10 @@ __device__ __forceinline__ void pack_phi_body(int block, int nblocks, int nx, in
                                               const double *__restrict__ phi_ref,
                                               double *__restrict__ phi) {
   const size_t n = (size_t)nx * ny * nz;
-  for (size_t c = block * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)nblocks * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
-    const size_t p = ph(i + 1, j + 1, k, nx, ny);
+  const size_t c0 = block * (size_t)blockDim.x + threadIdx.x, stride = (size_t)nblocks * blockDim.x;
+  Walk3 w(c0, stride, nx, ny);
+  for (size_t c = c0; c < n; c += stride, w.next()) {
+    const size_t p = ph((int)w.i + 1, (int)w.j + 1, (int)w.k, nx, ny);
     phi[c] = solid[p] != 0 ? 0.0 : phi_ref[p];
   }
 }
@@ -165,9 +203,10 @@ __device__ __forceinline__ void gauge_sums_body(int block, int nblocks, const Ge
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)nx * ny * nz;
   double s[2] = {0.0, 0.0};
-  for (size_t c = block * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)nblocks * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+  const size_t c0 = block * (size_t)blockDim.x + threadIdx.x, stride = (size_t)nblocks * blockDim.x;
+  Walk3 w(c0, stride, nx, ny);
+  for (size_t c = c0; c < n; c += stride, w.next()) {
+    const int i = (int)w.i, j = (int)w.j, k = (int)w.k;
     if (solid[ph(i + 1, j + 1, k, nx, ny)] == 0) {
       s[0] += phi[c] * g.dx[i] * g.dy[j] * g.dz[k];
       s[1] += g.dx[i] * g.dy[j] * g.dz[k];
@@ -187,10 +226,10 @@ __device__ __forceinline__ void gauge_apply_body(int block, int nblocks, int nx,
                                                  const Scalars *sc) {
   const double mean_value = sc->sums[0] / sc->sums[1];
   const size_t n = (size_t)(nx + 2) * (ny + 2) * nz;
-  for (size_t p = block * (size_t)blockDim.x + threadIdx.x; p < n;
-       p += (size_t)nblocks * blockDim.x) {
-    const int I = (int)(p % (nx + 2)), J = (int)((p / (nx + 2)) % (ny + 2));
-    const int k = (int)(p / ((size_t)(nx + 2) * (ny + 2)));
+  const size_t p0 = block * (size_t)blockDim.x + threadIdx.x, stride = (size_t)nblocks * blockDim.x;
+  Walk3 w(p0, stride, nx + 2, ny + 2);
+  for (size_t p = p0; p < n; p += stride, w.next()) {
+    const int I = (int)w.i, J = (int)w.j, k = (int)w.k;
     double v = 0.0;
     if (I >= 1 && I <= nx && J >= 1 && J <= ny) {
       const size_t c = (size_t)(I - 1) + (size_t)nx * ((size_t)(J - 1) + (size_t)ny * k);

@@ -251,29 +251,37 @@ __global__ __launch_bounds__(512) void k_compat_sums(
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)nx * ny * nz;
   double s[3] = {0.0, 0.0, 0.0};
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+  const size_t c0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x,
+               stride = (size_t)gridDim.x * blockDim.x;
+  Walk3 w(c0, stride, nx, ny);
+  for (size_t c = c0; c < n; c += stride, w.next()) {
+    const int i = (int)w.i, j = (int)w.j, k = (int)w.k;
     const int I = i + 1, J = j + 1;
-    if (solid[ph(I, J, k, nx, ny)] != 0) continue;
+    // operands loaded before the branches, as in build_operator_body
+    const size_t pc = ph(I, J, k, nx, ny), pxy = (size_t)(nx + 2) * (ny + 2);
+    const int32_t s0 = solid[pc], sw = solid[pc - 1], se = solid[pc + 1],
+                  ss = solid[pc - (nx + 2)], sn = solid[pc + (nx + 2)],
+                  sb = k > 0 ? solid[pc - pxy] : 0, st = k < nz - 1 ? solid[pc + pxy] : 0;
+    const double rv = rhs[c];
     const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
-    s[0] += rhs[c] * dxi * dyj * dzk;
+    if (s0 != 0) continue;
+    s[0] += rv * dxi * dyj * dzk;
     double f = 0.0, ar = 0.0, a;
     a = dyj * dzk;
     if (I == 1) { f = f - xlo[j + (size_t)ny * k] * a; ar = ar + a; }
-    else if (solid[ph(I - 1, J, k, nx, ny)] != 0) { f = f - bg0 * a; }
+    else if (sw != 0) { f = f - bg0 * a; }
     if (I == nx) { f = f + xhi[j + (size_t)ny * k] * a; ar = ar + a; }
-    else if (solid[ph(I + 1, J, k, nx, ny)] != 0) { f = f + bg0 * a; }
+    else if (se != 0) { f = f + bg0 * a; }
     a = dxi * dzk;
     if (J == 1) { f = f - ylo[i + (size_t)nx * k] * a; ar = ar + a; }
-    else if (solid[ph(I, J - 1, k, nx, ny)] != 0) { f = f - bg1 * a; }
+    else if (ss != 0) { f = f - bg1 * a; }
     if (J == ny) { f = f + yhi[i + (size_t)nx * k] * a; ar = ar + a; }
-    else if (solid[ph(I, J + 1, k, nx, ny)] != 0) { f = f + bg1 * a; }
+    else if (sn != 0) { f = f + bg1 * a; }
     a = dxi * dyj;
     if (k == 0) { f = f - zlo[i + (size_t)nx * j] * a; ar = ar + a; }
-    else if (solid[ph(I, J, k - 1, nx, ny)] != 0) { f = f - bg2 * a; }
+    else if (sb != 0) { f = f - bg2 * a; }
     if (k == nz - 1) { f = f + zhi[i + (size_t)nx * j] * a; ar = ar + a; }
-    else if (solid[ph(I, J, k + 1, nx, ny)] != 0) { f = f + bg2 * a; }
+    else if (st != 0) { f = f + bg2 * a; }
     s[1] += f;
     s[2] += ar;
   }
@@ -317,27 +325,36 @@ __device__ __forceinline__ void face_gradients(const Geom &g, const double *__re
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int i = I - 1, j = J - 1;
   const size_t pc = ph(I, J, k, nx, ny);
-  const double p0 = phi[pc];
+  const size_t sy = (size_t)(nx + 2), sz = (size_t)(nx + 2) * (ny + 2);
+  const bool kb = k > 0, kt = k < nz - 1;
+  // every operand is loaded before the branches (the halo layout holds the
+  // x and y neighbours of every cell; z neighbours only inside); the
+  // arithmetic is the reference's
+  const double p0 = phi[pc], pw = phi[pc - 1], pe = phi[pc + 1], ps = phi[pc - sy],
+               pn = phi[pc + sy], pb = kb ? phi[pc - sz] : 0.0, pt = kt ? phi[pc + sz] : 0.0;
+  const int32_t sw = solid[pc - 1], se = solid[pc + 1], ss = solid[pc - sy],
+                sn = solid[pc + sy], sb = kb ? solid[pc - sz] : 0, st = kt ? solid[pc + sz] : 0;
+  const double xm = g.xc[I - 1], x0 = g.xc[I], xp = g.xc[I + 1];
+  const double ym = g.yc[J - 1], y0 = g.yc[J], yp = g.yc[J + 1];
+  const double zm = kb ? g.zc[k - 1] : 0.0, z0 = g.zc[k], zp = kt ? g.zc[k + 1] : 0.0;
   if (I == 1) gr[0] = xlo[j + (size_t)ny * k];
-  else if (solid[pc - 1] != 0) gr[0] = bg0;
-  else gr[0] = (p0 - phi[pc - 1]) / (g.xc[I] - g.xc[I - 1]);
+  else if (sw != 0) gr[0] = bg0;
+  else gr[0] = (p0 - pw) / (x0 - xm);
   if (I == nx) gr[1] = xhi[j + (size_t)ny * k];
-  else if (solid[pc + 1] != 0) gr[1] = bg0;
-  else gr[1] = (phi[pc + 1] - p0) / (g.xc[I + 1] - g.xc[I]);
-  const size_t sy = (size_t)(nx + 2);
+  else if (se != 0) gr[1] = bg0;
+  else gr[1] = (pe - p0) / (xp - x0);
   if (J == 1) gr[2] = ylo[i + (size_t)nx * k];
-  else if (solid[pc - sy] != 0) gr[2] = bg1;
-  else gr[2] = (p0 - phi[pc - sy]) / (g.yc[J] - g.yc[J - 1]);
+  else if (ss != 0) gr[2] = bg1;
+  else gr[2] = (p0 - ps) / (y0 - ym);
   if (J == ny) gr[3] = yhi[i + (size_t)nx * k];
-  else if (solid[pc + sy] != 0) gr[3] = bg1;
-  else gr[3] = (phi[pc + sy] - p0) / (g.yc[J + 1] - g.yc[J]);
-  const size_t sz = (size_t)(nx + 2) * (ny + 2);
+  else if (sn != 0) gr[3] = bg1;
+  else gr[3] = (pn - p0) / (yp - y0);
   if (k == 0) gr[4] = zlo[i + (size_t)nx * j];
-  else if (solid[pc - sz] != 0) gr[4] = bg2;
-  else gr[4] = (p0 - phi[pc - sz]) / (g.zc[k] - g.zc[k - 1]);
+  else if (sb != 0) gr[4] = bg2;
+  else gr[4] = (p0 - pb) / (z0 - zm);
   if (k == nz - 1) gr[5] = zhi[i + (size_t)nx * j];
-  else if (solid[pc + sz] != 0) gr[5] = bg2;
-  else gr[5] = (phi[pc + sz] - p0) / (g.zc[k + 1] - g.zc[k]);
+  else if (st != 0) gr[5] = bg2;
+  else gr[5] = (pt - p0) / (zp - z0);
 }
 
 // stairstep_residual_3d (src/fpm_stairstep_mpi_3d.f90:403-419) from face
