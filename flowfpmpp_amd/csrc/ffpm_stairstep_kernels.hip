@@ -403,9 +403,14 @@ __global__ __launch_bounds__(512) void k_terms(
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const size_t n = (size_t)nx * ny * nz;
   double s[2] = {0.0, 0.0};
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+  const size_t c0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x,
+               stride = (size_t)gridDim.x * blockDim.x;
+  Walk3 wk(c0, stride, nx, ny);
+  for (size_t c = c0; c < n; c += stride, wk.next()) {
+    const int i = (int)wk.i, j = (int)wk.j, k = (int)wk.k;
+    // the flow fields are loaded with the gradients' operands
+    const double uu = u[c], vv = v[c], ww = w[c], ox_ = ox[c], oy_ = oy[c], oz_ = oz[c];
+    const double axc = ax[c], ayc = ay[c], azc = az[c];
     double gr[6];
     face_gradients(g, phi, solid, xlo, xhi, ylo, yhi, zlo, zhi, bg0, bg1, bg2, i + 1, j + 1, k,
                    gr);
@@ -416,12 +421,11 @@ __global__ __launch_bounds__(512) void k_terms(
     gx[c] = gxv;
     gy[c] = gyv;
     gz[c] = gzv;
-    const double uu = u[c], vv = v[c], ww = w[c], ox_ = ox[c], oy_ = oy[c], oz_ = oz[c];
     const double cx = ww * oy_ - vv * oz_;
     const double cy = uu * oz_ - ww * ox_;
     const double cz = vv * ox_ - uu * oy_;
     const double vo = -(cx * gxv + cy * gyv + cz * gzv);
-    const double ad = -(ax[c] * gxv + ay[c] * gyv + az[c] * gzv);
+    const double ad = -(axc * gxv + ayc * gyv + azc * gzv);
     vortex[c] = vo;
     advection[c] = ad;
     const double X = g.xc[i + 1], Y = g.yc[j + 1], Z = g.zc[k];
