@@ -408,9 +408,16 @@ __global__ __launch_bounds__(512) void k_terms(
   Walk3 wk(c0, stride, nx, ny);
   for (size_t c = c0; c < n; c += stride, wk.next()) {
     const int i = (int)wk.i, j = (int)wk.j, k = (int)wk.k;
-    // the flow fields are loaded with the gradients' operands
-    const double uu = u[c], vv = v[c], ww = w[c], ox_ = ox[c], oy_ = oy[c], oz_ = oz[c];
-    const double axc = ax[c], ayc = ay[c], azc = az[c];
+    // the flow fields are loaded with the gradients' operands; they and the
+    // outputs are touched once, so they bypass the caches that hold phi
+    const double uu = __builtin_nontemporal_load(u + c), vv = __builtin_nontemporal_load(v + c),
+                 ww = __builtin_nontemporal_load(w + c),
+                 ox_ = __builtin_nontemporal_load(ox + c),
+                 oy_ = __builtin_nontemporal_load(oy + c),
+                 oz_ = __builtin_nontemporal_load(oz + c);
+    const double axc = __builtin_nontemporal_load(ax + c),
+                 ayc = __builtin_nontemporal_load(ay + c),
+                 azc = __builtin_nontemporal_load(az + c);
     double gr[6];
     face_gradients(g, phi, solid, xlo, xhi, ylo, yhi, zlo, zhi, bg0, bg1, bg2, i + 1, j + 1, k,
                    gr);
@@ -418,16 +425,16 @@ __global__ __launch_bounds__(512) void k_terms(
     const double gxv = mask * 0.5 * (gr[0] + gr[1]);
     const double gyv = mask * 0.5 * (gr[2] + gr[3]);
     const double gzv = mask * 0.5 * (gr[4] + gr[5]);
-    gx[c] = gxv;
-    gy[c] = gyv;
-    gz[c] = gzv;
+    __builtin_nontemporal_store(gxv, gx + c);
+    __builtin_nontemporal_store(gyv, gy + c);
+    __builtin_nontemporal_store(gzv, gz + c);
     const double cx = ww * oy_ - vv * oz_;
     const double cy = uu * oz_ - ww * ox_;
     const double cz = vv * ox_ - uu * oy_;
     const double vo = -(cx * gxv + cy * gyv + cz * gzv);
     const double ad = -(axc * gxv + ayc * gyv + azc * gzv);
-    vortex[c] = vo;
-    advection[c] = ad;
+    __builtin_nontemporal_store(vo, vortex + c);
+    __builtin_nontemporal_store(ad, advection + c);
     const double X = g.xc[i + 1], Y = g.yc[j + 1], Z = g.zc[k];
     const bool inside = X > bd.b[0] && X < bd.b[1] && Y > bd.b[2] && Y < bd.b[3] &&
                         Z > bd.b[4] && Z < bd.b[5];
