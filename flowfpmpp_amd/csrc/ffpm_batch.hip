@@ -818,18 +818,10 @@ __global__ __launch_bounds__(256) void k_bf_finish(const BSolve *solves, int op,
 }
 
 // ------------------------------------------------------------- host --------
-int cells_per_lane_f64(int nx) {
-  const char *e = std::getenv("FFPM_X2");
-  if (e && std::strcmp(e, "0") == 0) return 1;
-  return nx % 2 == 0 ? 2 : 1;
-}
-
-// FP32 kernels: 16-byte vectors of four cells where nx allows.
-int cells_per_lane_f32(int nx) {
-  const char *e = std::getenv("FFPM_X2");
-  if (e && std::strcmp(e, "0") == 0) return 1;
-  return nx % 4 == 0 ? 4 : (nx % 2 == 0 ? 2 : 1);
-}
+// Cells per lane as the standalone solves choose them (so that each solve's
+// tiles and reduction order are its standalone solve's).
+int cells_per_lane_f64(int nx) { return cells_per_lane(nx, 2); }
+int cells_per_lane_f32(int nx) { return cells_per_lane(nx, 4); }
 
 double inner_rtol() {
   const char *e = std::getenv("FFPM_MIXED_RTOL");

@@ -11,6 +11,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 
 namespace ffpm {
@@ -191,6 +193,25 @@ __device__ inline double grid_finish_max(const double *partials, int slot, int n
   for (int b = tid; b < nblocks; b += nthreads) x = fmax(x, p[b]);
   block_max(x, lds);
   return x;
+}
+
+// Cells per lane of the streaming kernels (a 64-lane row of a tile covers
+// 64 x cells cells of an x row), at most vmax (the 16-byte vector) and
+// dividing nx. FFPM_LANE_FIT=1: the width that leaves the fewest lanes of a
+// row idle (ties to the wider), else the widest; FFPM_X2=0: one cell.
+inline int cells_per_lane(int nx, int vmax) {
+  const char *e = std::getenv("FFPM_X2");
+  if (e && std::strcmp(e, "0") == 0) return 1;
+  const char *f = std::getenv("FFPM_LANE_FIT");
+  const bool fit = f && std::strcmp(f, "1") == 0;
+  int best = 1;
+  long best_idle = -1;
+  for (int c = 1; c <= vmax; c *= 2) {
+    if (nx % c) continue;
+    const long row = 64L * c, idle = (nx + row - 1) / row * row - nx;
+    if (!fit || best_idle < 0 || idle <= best_idle) best = c, best_idle = idle;
+  }
+  return best;
 }
 
 // Elementwise launch shape: grid-stride over n cells.

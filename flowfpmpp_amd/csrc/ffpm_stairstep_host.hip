@@ -255,15 +255,10 @@ int pcg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, 
   return OK;
 }
 
-// Cells per lane of the fused kernels: 16-byte vectors where nx allows.
+// Cells per lane of the fused kernels: up to a 16-byte vector.
 template <typename T>
 int cells_per_lane(int nx) {
-  const char *e = std::getenv("FFPM_X2");
-  if (e && std::strcmp(e, "0") == 0) return 1;
-  const int w = (int)(16 / sizeof(T));
-  if (w >= 4 && nx % 4 == 0) return 4;
-  if (nx % 2 == 0) return 2;
-  return 1;
+  return ffpm::cells_per_lane(nx, (int)(16 / sizeof(T)));
 }
 
 template <typename T>
