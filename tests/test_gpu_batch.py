@@ -182,6 +182,26 @@ def test_multigrid_batch_equals_standalone(use_graph):
     assert all(0 < q.sweeps < 200 for q in probs)
 
 
+@pytest.mark.parametrize("method", ["fcg-mg", "pcg"])
+def test_lane_fit_rule_batch_equals_standalone(method, monkeypatch):
+    """FFPM_LANE_FIT=1 gives the frames of these tests one cell per lane (in
+    place of the widest vector dividing nx); the batched and standalone solves
+    use the same rule, so each problem stays bit-identical to its standalone
+    solve, and the fields agree with the default rule's to 1e-10 relative."""
+    print(f"case: lane-fit rule, {method} batch", flush=True)
+    frames = _multigrid_frames()
+    wide = [r for fr in frames for r in fr.standalone(1e-10, 100000, 25, method=method)]
+    monkeypatch.setenv("FFPM_LANE_FIT", "1")
+    ref = [r for fr in frames for r in fr.standalone(1e-10, 100000, 25, method=method)]
+    probs = [q for fr in frames for q in fr.problems()]
+    S.solve_stairstep_batch(probs, 1e-10, 100000, 25, method=method)
+    monkeypatch.delenv("FFPM_LANE_FIT")
+    _assert_same(probs, ref)
+    for q, (phi, sw, r) in zip(probs, wide):
+        assert q.residual <= 1e-10
+        assert np.abs(q.phi.download() - phi).max() <= 1e-10 * np.abs(phi).max()
+
+
 def test_multigrid_batch_fixed_work():
     """Fixed work (negative tolerance) with a cap that is not a multiple of the
     two-iteration chunk: every problem runs exactly the cap, as standalone."""
