@@ -16,10 +16,10 @@ using namespace mgk;
 
 // ---- standalone kernels: one solve per launch --------------------------------
 __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t *__restrict__ code,
-                                                  const double *__restrict__ r,
+                                                  double *__restrict__ r,
                                                   float *__restrict__ x, float *__restrict__ s32,
-                                                  float omega) {
-  mgf_jac0_body(blk3(), g, code, r, x, s32, omega);
+                                                  float omega, const Scalars *proj) {
+  mgf_jac0_body(blk3(), g, code, r, x, s32, omega, proj);
 }
 
 template <bool VEC>
@@ -386,10 +386,10 @@ void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr
 
 }  // namespace
 
-int mg_first(const Multigrid &mg, const double *r) {
+int mg_first(const Multigrid &mg, double *r, const Scalars *proj) {
   const MgLevel &f0 = mg.lv[0];
   k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMt), 0, stream()>>>(
-      mg.g32, mg.code, r, f0.x, mg.s32, mg.first);
+      mg.g32, mg.code, r, f0.x, mg.s32, mg.first, proj);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
 }

@@ -118,8 +118,10 @@ int mg_build(const Multigrid &mg);
 int mg_check_interval(int check_interval);
 
 // The cycle's source s32 = r / V and the first smoothing step from zero,
-// lv[0].x = w D^-1 r.
-int mg_first(const Multigrid &mg, const double *r);
+// lv[0].x = w D^-1 r. With proj non-null (the scalars of a true residual's
+// first pass) r is first projected off the constants in place, r -= red1 /
+// red2 where diag > 0, which completes that true residual.
+int mg_first(const Multigrid &mg, double *r, const Scalars *proj = nullptr);
 
 // x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r, q32 = A p and s32 = r / V
 // (alpha in sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch

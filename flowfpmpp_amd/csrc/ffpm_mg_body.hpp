@@ -109,18 +109,27 @@ struct CoarseOut {
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
 // The cycle's source s = r / V in FP32 (0 where d = 0) and the first sweep
-// from zero, x = omega D^-1 r = omega s / d; D = V diag.
+// from zero, x = omega D^-1 r = omega s / d; D = V diag. With proj non-null
+// the true residual's projection (src/fpm_stairstep_mpi_3d.f90:319-322) is
+// applied first: r -= red1 / red2 in the cells with a fluid neighbour (those
+// whose diag is positive), and the projected r is stored.
 __device__ __forceinline__ void mgf_jac0_body(Blk bk, GeomT<float> g,
                                               const uint8_t *__restrict__ code,
-                                              const double *__restrict__ r, float *__restrict__ x,
-                                              float *__restrict__ s32, float omega) {
+                                              double *__restrict__ r, float *__restrict__ x,
+                                              float *__restrict__ s32, float omega,
+                                              const Scalars *proj = nullptr) {
   const Idx t = cell_idx(g.nx, g.ny, bk);
   if (!t.in) return;
   const unsigned cd = code[t.c];
+  double rv = r[t.c];
+  if (proj && (cd & (kCodeW | kCodeE | kCodeS | kCodeN | kCodeB | kCodeT))) {
+    rv = rv - proj->red1 / proj->red2;
+    r[t.c] = rv;
+  }
   const float d = diag_code<float>(cd, g.cw[t.i], g.ce[t.i], g.cs[t.j], g.cn[t.j], g.cb[t.k],
                                    g.ct[t.k]);
   const float V = g.dx[t.i] * g.dy[t.j] * g.dz[t.k];
-  const float sv = d > 0.f ? (float)r[t.c] / V : 0.f;
+  const float sv = d > 0.f ? (float)rv / V : 0.f;
   s32[t.c] = sv;
   x[t.c] = d > 0.f ? omega * sv / d : 0.f;
 }

@@ -149,15 +149,21 @@ int read_scalars(const Frame &f, Scalars *h) {
 // true_residual (src/fpm_stairstep_mpi_3d.f90:297-329); res read back. With
 // reset the direction p = M^-1 r is written to f.next (the three-kernel
 // iteration); the fused iteration applies the reset in its next SpMV instead.
-int true_residual(Frame &f, bool reset, double *res) {
-  const int eb = elementwise_blocks(f.n);
+// true_residual's first pass: r = V (src - A phi) unprojected, the max-norm
+// residual in sc->res and the projection sums in sc->red1, sc->red2.
+int true_residual_pass1(Frame &f) {
   {
     TimedLaunch tl("true_residual_1");
     k_true_residual_1<<<f.tiling.blocks, dim3(kTileX, f.tiling.ty), 0, stream()>>>(
         f.tiling, f.g, f.phi, f.src, f.diag, f.r, f.partials, f.sc);
     FFPM_LAUNCHED("k_true_residual_1");
   }
-  finish(f, FIN_TR1, 3, f.tiling.blocks);
+  return finish(f, FIN_TR1, 3, f.tiling.blocks);
+}
+
+int true_residual(Frame &f, bool reset, double *res) {
+  const int eb = elementwise_blocks(f.n);
+  if (int st = true_residual_pass1(f)) return st;
   {
     TimedLaunch tl("true_residual_2");
     k_true_residual_2<<<eb, kThreads, 0, stream()>>>(f.n, f.r, f.rdiag, f.diag, f.next,
@@ -836,8 +842,19 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   double *pbuf[2] = {f.next, f.q};
   int cur = 0, mode = kDirReset;
   bool first_ready = false;  // lv[0].x holds w D^-1 r for the current r
+  // The true residual's second pass (projection of r) runs inside the next
+  // first smoothing step: its z and r.z are not used here, and the projected
+  // r is needed only by that step and what follows it.
+  auto check = [&](double *res_out) -> int {
+    Scalars h;
+    int s2 = true_residual_pass1(f);
+    if (s2 || (s2 = read_scalars(f, &h))) return s2;
+    *res_out = h.res;
+    first_ready = false;
+    return OK;
+  };
   double res;
-  if ((st = true_residual(f, false, &res))) return st;
+  if ((st = check(&res))) return st;
   double best = res;
   int it = 0, restarts = 0, stalled = 0, last_check = 0;
   while (*sweeps < max_sweeps && res > tolerance) {
@@ -846,7 +863,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     k_chunk_begin<<<1, 64, 0, stream()>>>(f.sc);
     FFPM_LAUNCHED("k_chunk_begin");
     for (int m = 0; m < n; ++m) {
-      if (!first_ready && (st = mg_first(mg, f.r))) return st;
+      if (!first_ready && (st = mg_first(mg, f.r, f.sc))) return st;
       if ((st = mg_vcycle(mg, mode == kDirUpdate))) return st;
       k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
@@ -878,8 +895,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       if (restarts > 1) break;
       k_clear_status<<<1, 64, 0, stream()>>>(f.sc);
       FFPM_LAUNCHED("k_clear_status");
-      if ((st = true_residual(f, false, &res))) return st;
-      first_ready = false;
+      if ((st = check(&res))) return st;
       mode = kDirReset;
       continue;
     }
@@ -889,8 +905,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     const bool near = !(h.res > 4.0 * tolerance);
     if ((it % ci == 0 && (near || it - last_check >= 20)) || *sweeps == max_sweeps) {
       last_check = it;
-      if ((st = true_residual(f, false, &res))) return st;
-      first_ready = false;
+      if ((st = check(&res))) return st;
       g_mg.checks += 1;
       if (res < 0.99 * best) {
         best = res;
