@@ -400,7 +400,9 @@ __device__ __forceinline__ mgk::Blk cell_blk(const MgLevel &v, int local) {
 }
 
 // The cycle's source and first smoothing step after a new r (the first slot
-// of a chunk after a true residual or a restart).
+// of a chunk after a true residual or a restart), with the true residual's
+// projection of r: a solve's first step is pending exactly when a first pass
+// has run since its last one.
 __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid grid,
                                                   const BState *state) {
   const int s = grid.solve[blockIdx.x];
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid gri
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   mgk::mgf_jac0_body(cell_blk(mg.lv[0], grid.local[blockIdx.x]), mg.g32, mg.code, d.r,
-                     mg.lv[0].x, mg.s32, mg.first);
+                     mg.lv[0].x, mg.s32, mg.first, d.sc);
 }
 
 // One level-0 smoothing step (MgStep: buffers ia, ip, io, coefficients);
@@ -1285,10 +1287,8 @@ int launch_check_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, doubl
   FFPM_LAUNCHED("k_b_tr1");
   k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_TR1, 0);
   FFPM_LAUNCHED("k_bg_finish");
-  k_b_tr2<<<B.g_ew.blocks, kThreads, 0, s>>>(B.d_solves, B.g_ew, B.d_state);
-  FFPM_LAUNCHED("k_b_tr2");
-  k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_RZ, 0);
-  FFPM_LAUNCHED("k_bg_finish");
+  // the projection of r (the true residual's second pass) runs in the
+  // solve's next k_bg_first
   k_bg_check_end<<<cb, 64, 0, s>>>(B.d_solves, B.d_state, B.count, ci, max_sweeps, tol,
                                    B.d_counters);
   FFPM_LAUNCHED("k_bg_check_end");
