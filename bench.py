@@ -55,6 +55,8 @@ BYTES_PER_CELL = {
     "true_residual_2": 32,  # r (read, write), rdiag, diag
     "residual_max": 24,  # phi, src, diag
     "terms": 124,        # phi, solid(4), 9 fields; write 5 outputs
+    "terms_dirs": 220,   # the three directions in one pass: 3 phi, solid(4), 9 fields;
+                         # write 15 outputs
     # CG with the FP32 multigrid cycle (level-0 kernels; the coarse levels add
     # 1/8, 1/64, ... of their own small traffic)
     "mg_update": 53,         # read p, code; read/write phi, r; write x1, q, s = r/V (FP32)
@@ -492,17 +494,17 @@ def main():
             sw, r = S.solve_stairstep_poisson_3d(d_phi[d], d_rhs, d_solid, bgs[d], *d_faces[d],
                                                  *axes, args.tol, args.max_iters, args.check,
                                                  method=method)
-            o = S.compute_stairstep_fpm_terms_3d(d_phi[d], *[dev[k] for k in
-                                                             ("u", "v", "w", "ox", "oy", "oz",
-                                                              "ax", "ay", "az")],
-                                                 d_solid, bgs[d], *axes, *d_faces[d], p.bounds,
-                                                 *outs[d])
             iters.append(sw)
             res.append(r)
             if method == "pcg-mixed":
                 mixed_outer.append(S.last_mixed_solve()[0])
-            totals.append([float(x) for x in o[5]])
             log(f"{method} direction {d}: {sw} iterations, residual {r:.3e}")
+        # the three directions' densities in one pass over the flow fields
+        # (each equal bit for bit to its own compute_stairstep_fpm_terms_3d)
+        o = S.compute_stairstep_fpm_terms_3d_dirs(
+            d_phi, *[dev[k] for k in ("u", "v", "w", "ox", "oy", "oz", "ax", "ay", "az")],
+            d_solid, bgs, *axes, d_faces, p.bounds, outs)
+        totals = [[float(x) for x in o[d][5]] for d in range(3)]
         return iters, res, totals
 
     for _ in range(args.warmup):

@@ -98,4 +98,19 @@ __global__ void k_terms(Geom g, const double *phi, const int32_t *solid, const d
                         double bg1, double bg2, Bounds bd, double *gx, double *gy, double *gz,
                         double *vortex, double *advection, double *partials, Scalars *sc);
 
+// The per-direction arguments of k_terms_dirs (directions 0..ND-1 used).
+struct TermsDirs {
+  const double *phi[3];
+  const double *xlo[3], *xhi[3], *ylo[3], *yhi[3], *zlo[3], *zhi[3];
+  double bg[3][3];
+  double *gx[3], *gy[3], *gz[3], *vortex[3], *advection[3];
+};
+
+template <int ND>
+__global__ void k_terms_dirs(Geom g, TermsDirs a, const int32_t *solid, const double *u,
+                             const double *v, const double *w, const double *ox,
+                             const double *oy, const double *oz, const double *ax,
+                             const double *ay, const double *az, Bounds bd, double *partials);
+__global__ void k_terms_dirs_finish(int nd, const double *partials, int nblocks, double *out);
+
 }  // namespace ffpm

@@ -52,6 +52,8 @@ enum Slot : int {
   S_P2_AXES = 100, S_P2_WORK, S_P2_PART, S_P2_SC, S_P2_HOST, S_P2_SOLID,
   // multigrid hierarchy of the flexible CG (one arena)
   S_MG = 110,
+  // densities of several directions in one pass: partials, totals
+  S_TD_PART = 120, S_TD_OUT,
 };
 
 // ---- kernel timing with HIP events on the library stream --------------------

@@ -1103,6 +1103,79 @@ int terms_dev(const double *phi, const double *u, const double *v, const double 
   return OK;
 }
 
+// compute_stairstep_fpm_terms_3d for nd directions (1-3) of one frame in one
+// pass over the flow fields; per-direction arguments are host arrays of
+// device pointers, body_grad holds 3 nd values and totals 2 nd.
+int terms_dirs_dev(int nd, const double *const *phi, const double *u, const double *v,
+                   const double *w, const double *ox, const double *oy, const double *oz,
+                   const double *ax, const double *ay, const double *az, const int32_t *solid,
+                   const double *bg, const double *xc, const double *yc, const double *zc,
+                   const double *dx, const double *dy, const double *dz,
+                   const double *const *xlo, const double *const *xhi,
+                   const double *const *ylo, const double *const *yhi,
+                   const double *const *zlo, const double *const *zhi, const double *bounds,
+                   double *const *gx, double *const *gy, double *const *gz,
+                   double *const *vortex, double *const *advection, double *totals, int nx,
+                   int ny, int nz, int west, int east, int south, int north) {
+  if (nd < 1 || nd > 3) {
+    set_error(ERR_ARGUMENT, "densities: 1 to 3 directions");
+    return ERR_ARGUMENT;
+  }
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error(ERR_SHAPE, "empty grid");
+    return ERR_SHAPE;
+  }
+  if (any_neighbour(west, east, south, north)) return decomposition_error();
+  if (!all_present({phi, u, v, w, ox, oy, oz, ax, ay, az, solid, bg, xc, yc, zc, dx, dy, dz, xlo,
+                    xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex, advection, totals}))
+    return null_error();
+  TermsDirs a{};
+  for (int d = 0; d < nd; ++d) {
+    if (!all_present({phi[d], xlo[d], xhi[d], ylo[d], yhi[d], zlo[d], zhi[d], gx[d], gy[d],
+                      gz[d], vortex[d], advection[d]}))
+      return null_error();
+    a.phi[d] = phi[d];
+    a.xlo[d] = xlo[d];
+    a.xhi[d] = xhi[d];
+    a.ylo[d] = ylo[d];
+    a.yhi[d] = yhi[d];
+    a.zlo[d] = zlo[d];
+    a.zhi[d] = zhi[d];
+    for (int e = 0; e < 3; ++e) a.bg[d][e] = bg[3 * d + e];
+    a.gx[d] = gx[d];
+    a.gy[d] = gy[d];
+    a.gz[d] = gz[d];
+    a.vortex[d] = vortex[d];
+    a.advection[d] = advection[d];
+  }
+  Frame f;
+  int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
+  if (st) return st;
+  Bounds bd;
+  for (int i = 0; i < 6; ++i) bd.b[i] = bounds[i];
+  const int eb = elementwise_blocks(f.n);
+  double *part = (double *)workspace().get(S_TD_PART, sizeof(double) * 6 * eb, &st);
+  if (!part) return st;
+  double *out = (double *)workspace().get(S_TD_OUT, sizeof(double) * 6, &st);
+  if (!out) return st;
+  {
+    TimedLaunch tl("terms_dirs");
+    auto go = [&](auto kern) {
+      kern<<<eb, kThreads, 0, stream()>>>(f.g, a, solid, u, v, w, ox, oy, oz, ax, ay, az, bd,
+                                         part);
+    };
+    if (nd == 3) go(k_terms_dirs<3>);
+    else if (nd == 2) go(k_terms_dirs<2>);
+    else go(k_terms_dirs<1>);
+  }
+  FFPM_LAUNCHED("k_terms_dirs");
+  k_terms_dirs_finish<<<1, 256, 0, stream()>>>(nd, part, eb, out);
+  FFPM_LAUNCHED("k_terms_dirs_finish");
+  FFPM_HIP(hipMemcpyAsync(totals, out, sizeof(double) * 2 * nd, hipMemcpyDeviceToHost, stream()));
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  return OK;
+}
+
 // ---- host-pointer staging ---------------------------------------------------
 namespace {
 
@@ -1412,6 +1485,21 @@ int ffpm_compute_stairstep_fpm_terms_3d_dev(
   return terms_dev(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, body_grad, xc, yc, zc, dx, dy, dz,
                    xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex, advection, totals,
                    nx, ny, nz, west, east, south, north);
+}
+
+int ffpm_compute_stairstep_fpm_terms_3d_dirs_dev(
+    int ndir, const double *const *phi, const double *u, const double *v, const double *w,
+    const double *ox, const double *oy, const double *oz, const double *ax, const double *ay,
+    const double *az, const int32_t *solid, const double *body_grad, const double *xc,
+    const double *yc, const double *zc, const double *dx, const double *dy, const double *dz,
+    const double *const *xlo, const double *const *xhi, const double *const *ylo,
+    const double *const *yhi, const double *const *zlo, const double *const *zhi,
+    const double *bounds, double *const *gx, double *const *gy, double *const *gz,
+    double *const *vortex, double *const *advection, double *totals, int nx, int ny, int nz,
+    int west, int east, int south, int north) {
+  return terms_dirs_dev(ndir, phi, u, v, w, ox, oy, oz, ax, ay, az, solid, body_grad, xc, yc, zc,
+                        dx, dy, dz, xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex,
+                        advection, totals, nx, ny, nz, west, east, south, north);
 }
 
 int ffpm_compute_stairstep_fpm_terms_3d(

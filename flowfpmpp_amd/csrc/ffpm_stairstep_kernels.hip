@@ -446,4 +446,102 @@ __global__ __launch_bounds__(512) void k_terms(
   store_partials(partials, 2, s, gridDim.x);
 }
 
+// compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469) for
+// ND directions in one pass: the nine flow fields of a cell are loaded once
+// and each direction's face gradients, densities and bounded integrals are
+// formed as k_terms forms them (the same cells per thread in the same order,
+// one block_sum per direction), so every direction's outputs and partial
+// sums equal those of its own k_terms launch bit for bit. Partials:
+// direction d's two slots at partials + 2 d gridDim.x.
+template <int ND>
+__global__ __launch_bounds__(512) void k_terms_dirs(
+    Geom g, TermsDirs a, const int32_t *__restrict__ solid, const double *__restrict__ u,
+    const double *__restrict__ v, const double *__restrict__ w, const double *__restrict__ ox,
+    const double *__restrict__ oy, const double *__restrict__ oz, const double *__restrict__ ax,
+    const double *__restrict__ ay, const double *__restrict__ az, Bounds bd, double *partials) {
+  __shared__ double lds[16 * kReduceSlots];
+  const int nx = g.nx, ny = g.ny;
+  const size_t n = (size_t)nx * ny * g.nz;
+  double s[ND][2];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) s[d][0] = s[d][1] = 0.0;
+  const size_t c0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x,
+               stride = (size_t)gridDim.x * blockDim.x;
+  Walk3 wk(c0, stride, nx, ny);
+  for (size_t c = c0; c < n; c += stride, wk.next()) {
+    const int i = (int)wk.i, j = (int)wk.j, k = (int)wk.k;
+    const double uu = __builtin_nontemporal_load(u + c), vv = __builtin_nontemporal_load(v + c),
+                 ww = __builtin_nontemporal_load(w + c),
+                 ox_ = __builtin_nontemporal_load(ox + c),
+                 oy_ = __builtin_nontemporal_load(oy + c),
+                 oz_ = __builtin_nontemporal_load(oz + c);
+    const double axc = __builtin_nontemporal_load(ax + c),
+                 ayc = __builtin_nontemporal_load(ay + c),
+                 azc = __builtin_nontemporal_load(az + c);
+    const double mask = solid[ph(i + 1, j + 1, k, nx, ny)] == 0 ? 1.0 : 0.0;
+    const double cx = ww * oy_ - vv * oz_;
+    const double cy = uu * oz_ - ww * ox_;
+    const double cz = vv * ox_ - uu * oy_;
+    const double X = g.xc[i + 1], Y = g.yc[j + 1], Z = g.zc[k];
+    const bool inside = X > bd.b[0] && X < bd.b[1] && Y > bd.b[2] && Y < bd.b[3] &&
+                        Z > bd.b[4] && Z < bd.b[5];
+    const double volume = inside ? g.dx[i] * g.dy[j] * g.dz[k] : 0.0;
+    // every direction's gradients before any store: the outputs may alias
+    // the potentials as far as the compiler knows, so a store between them
+    // would hold the next direction's loads back
+    double gr[ND][6];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+      face_gradients(g, a.phi[d], solid, a.xlo[d], a.xhi[d], a.ylo[d], a.yhi[d], a.zlo[d],
+                     a.zhi[d], a.bg[d][0], a.bg[d][1], a.bg[d][2], i + 1, j + 1, k, gr[d]);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const double gxv = mask * 0.5 * (gr[d][0] + gr[d][1]);
+      const double gyv = mask * 0.5 * (gr[d][2] + gr[d][3]);
+      const double gzv = mask * 0.5 * (gr[d][4] + gr[d][5]);
+      __builtin_nontemporal_store(gxv, a.gx[d] + c);
+      __builtin_nontemporal_store(gyv, a.gy[d] + c);
+      __builtin_nontemporal_store(gzv, a.gz[d] + c);
+      const double vo = -(cx * gxv + cy * gyv + cz * gzv);
+      const double ad = -(axc * gxv + ayc * gyv + azc * gzv);
+      __builtin_nontemporal_store(vo, a.vortex[d] + c);
+      __builtin_nontemporal_store(ad, a.advection[d] + c);
+      s[d][0] += volume * vo;
+      s[d][1] += volume * ad;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    block_sum<2>(s[d], lds);
+    store_partials(partials + (size_t)2 * d * gridDim.x, 2, s[d], gridDim.x);
+  }
+}
+
+template __global__ void k_terms_dirs<1>(Geom, TermsDirs, const int32_t *, const double *,
+                                         const double *, const double *, const double *,
+                                         const double *, const double *, const double *,
+                                         const double *, const double *, Bounds, double *);
+template __global__ void k_terms_dirs<2>(Geom, TermsDirs, const int32_t *, const double *,
+                                         const double *, const double *, const double *,
+                                         const double *, const double *, const double *,
+                                         const double *, const double *, Bounds, double *);
+template __global__ void k_terms_dirs<3>(Geom, TermsDirs, const int32_t *, const double *,
+                                         const double *, const double *, const double *,
+                                         const double *, const double *, const double *,
+                                         const double *, const double *, Bounds, double *);
+
+// Totals of k_terms_dirs: direction d's two sums into out[2d], out[2d + 1],
+// summed as k_finish sums FIN_SUMS partials (one workgroup of the same size).
+__global__ void k_terms_dirs_finish(int nd, const double *partials, int nblocks, double *out) {
+  __shared__ double lds[16 * kReduceSlots];
+  for (int d = 0; d < nd; ++d) {
+    double v[2];
+    grid_finish_sum<2>(partials + (size_t)2 * d * nblocks, nblocks, v, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      out[2 * d] = v[0];
+      out[2 * d + 1] = v[1];
+    }
+  }
+}
+
 }  // namespace ffpm

@@ -162,6 +162,49 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
     return (*outs, totals)
 
 
+def compute_stairstep_fpm_terms_3d_dirs(phis, u, v, w, ox, oy, oz, ax, ay, az, solid,
+                                        body_grads, xc, yc, zc, dx, dy, dz, faces, bounds,
+                                        outs=None, west=PROC_NULL, east=PROC_NULL,
+                                        south=PROC_NULL, north=PROC_NULL):
+    """compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469)
+    for 1-3 directions of one frame in one pass over the flow fields (device
+    arrays only). phis: the directions' potentials; body_grads: one 3-vector
+    per direction; faces: per direction its six outer face arrays (xlo, xhi,
+    ylo, yhi, zlo, zhi); outs: per direction five output arrays (gx, gy, gz,
+    vortex, advection), allocated when None. Returns per direction (gx, gy,
+    gz, vortex, advection, totals), each equal bit for bit to what
+    compute_stairstep_fpm_terms_3d returns for that direction."""
+    nd = len(phis)
+    if not 1 <= nd <= 3 or len(body_grads) != nd or len(faces) != nd:
+        raise ValueError("1 to 3 directions, each with a body gradient and six face arrays")
+    if any(len(f) != 6 for f in faces):
+        raise ValueError("six outer face arrays per direction")
+    nx, ny, nz = _shape(solid)
+    flat = [*phis, u, v, w, ox, oy, oz, ax, ay, az, solid, xc, yc, zc, dx, dy, dz,
+            *[a for f in faces for a in f]]
+    if outs is None:
+        outs = [[DeviceArray((nx, ny, nz)) for _ in range(5)] for _ in range(nd)]
+    if len(outs) != nd or any(len(o) != 5 for o in outs):
+        raise ValueError("five output arrays per direction")
+    if _mode(*flat, *[a for o in outs for a in o]) != "_dev":
+        raise TypeError("the multi-direction densities take device arrays")
+    P = C.c_void_p * nd
+
+    def arr(items):
+        return P(*[C.c_void_p(_d(a).value) for a in items])
+
+    bg, bgp = _vec(np.asarray(body_grads, dtype=np.float64).reshape(-1), 3 * nd)
+    bd, bdp = _vec(bounds, 6)
+    totals = np.zeros(2 * nd)
+    ptrs = [arr(phis)] + [arr([f[m] for f in faces]) for m in range(6)] + \
+        [arr([o[m] for o in outs]) for m in range(5)]
+    check(lib().ffpm_compute_stairstep_fpm_terms_3d_dirs_dev(
+        nd, ptrs[0], _d(u), _d(v), _d(w), _d(ox), _d(oy), _d(oz), _d(ax), _d(ay), _d(az),
+        _i(solid), bgp, _d(xc), _d(yc), _d(zc), _d(dx), _d(dy), _d(dz), *ptrs[1:7], bdp,
+        *ptrs[7:12], totals.ctypes.data_as(C.c_void_p), nx, ny, nz, west, east, south, north))
+    return [(*outs[d], totals[2 * d:2 * d + 2].copy()) for d in range(nd)]
+
+
 def set_stream(handle) -> None:
     """Launch on the given raw hipStream_t handle (None: the null stream)."""
     check(lib().ffpm_set_stream(C.c_void_p(0 if handle is None else int(handle))))

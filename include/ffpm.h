@@ -272,6 +272,26 @@ int ffpm_compute_stairstep_fpm_terms_3d_dev(
     double *advection, double *totals, int nx, int ny, int nz, int west, int east, int south,
     int north);
 
+/* compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469) for
+ * ndir (1-3) directions of one frame in one pass: the nine flow fields are
+ * read once for all the directions' potentials (the reference calls the
+ * routine once per direction). Per-direction arguments are host arrays of
+ * ndir device pointers (phi, the outer face data xlo..zhi, the outputs
+ * gx..advection); body_grad holds 3 ndir values (direction d at 3d) and
+ * totals receives 2 ndir (direction d at 2d). Each direction's outputs and
+ * totals equal those of ffpm_compute_stairstep_fpm_terms_3d_dev for it, bit
+ * for bit. */
+int ffpm_compute_stairstep_fpm_terms_3d_dirs_dev(
+    int ndir, const double *const *phi, const double *u, const double *v, const double *w,
+    const double *ox, const double *oy, const double *oz, const double *ax, const double *ay,
+    const double *az, const int32_t *solid, const double *body_grad, const double *xc,
+    const double *yc, const double *zc, const double *dx, const double *dy, const double *dz,
+    const double *const *xlo, const double *const *xhi, const double *const *ylo,
+    const double *const *yhi, const double *const *zlo, const double *const *zhi,
+    const double *bounds, double *const *gx, double *const *gy, double *const *gz,
+    double *const *vortex, double *const *advection, double *totals, int nx, int ny, int nz,
+    int west, int east, int south, int north);
+
 /* ------------------------------------------------------------------------ *
  * No-body pure-Neumann solvers, src/fpm_laplace_3d.f90
  * ------------------------------------------------------------------------ */

@@ -403,3 +403,33 @@ def test_mixed_precision_on_the_128x96x64_frame():
               f"FP32 iterations; max rel |phi - phi_FP64| {e_phi:.2e}, vortex density "
               f"{e_vort:.2e}, advection density {e_adv:.2e}", flush=True)
         assert e_phi <= 1e-8 and e_vort <= 1e-6 and e_adv <= 1e-6
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_terms_of_several_directions_in_one_pass(nd):
+    """The directions' densities from one pass over the flow fields equal the
+    per-direction compute_stairstep_fpm_terms_3d outputs and totals bit for
+    bit (random potentials and outer face data per direction, a body touching
+    the outer boundary, integration bounds cutting the domain)."""
+    print(f"case: densities of {nd} directions in one pass, 40x30x20", flush=True)
+    T = DeviceArray.from_host
+    p = W.stretched_frame(40, 30, 20, [W.Sphere((0.0, 0.0, 0.0), 0.9),
+                                       W.Box((3.0, -1.0, -0.7), (4.5, 1.0, 0.7))], seed=5)
+    rng = np.random.default_rng(31)
+    solid, axes = T(p.solid), [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+    fl = [T(p.fields[k]) for k in ("u", "v", "w", "ox", "oy", "oz", "ax", "ay", "az")]
+    bounds = np.array([-2.0, 3.0, -1.5, 2.0, -1.0, 1.5])
+    phis, faces, bgs = [], [], []
+    for d in range(nd):
+        phis.append(T(np.asfortranarray(rng.standard_normal(p.solid.shape))))
+        faces.append([T(np.asfortranarray(rng.standard_normal(f.shape))) for f in p.faces()])
+        bgs.append(np.eye(3)[d] * (1.0 + 0.5 * d))
+    one = [S.compute_stairstep_fpm_terms_3d(phis[d], *fl, solid, bgs[d], *axes, *faces[d],
+                                            bounds) for d in range(nd)]
+    many = S.compute_stairstep_fpm_terms_3d_dirs(phis, *fl, solid, bgs, *axes, faces, bounds)
+    assert len(many) == nd
+    for d in range(nd):
+        for a, b in zip(one[d][:5], many[d][:5]):
+            assert np.array_equal(a.download(), b.download())
+        assert np.array_equal(one[d][5], many[d][5])
+        assert np.all(np.isfinite(many[d][5])) and np.any(many[d][5] != 0)
