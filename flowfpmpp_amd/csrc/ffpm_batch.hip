@@ -949,7 +949,10 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     B.cl[s] = cells_per_lane_f64(p.nx);
     B.cl32[s] = cells_per_lane_f32(p.nx);
     tl[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl[s], rows, zchunk);
-    tl32[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl32[s], rows, zchunk);
+    // FP32 kernels: the mixed inner PCG's, or the multigrid cycle's (their
+    // own rows unless a shape is given, as the standalone solves choose)
+    tl32[s] = pcg_tiling(p.nx, p.ny, p.nz, B.cl32[s], mgm && rows <= 0 ? mg_rows() : rows,
+                         zchunk);
     tt[s] = make_tiling(p.nx, p.ny, p.nz, 0);
     eb[s] = elementwise_blocks(n);
     const size_t nb = (size_t)std::max(std::max(std::max(tl[s].blocks, tl32[s].blocks),
@@ -1427,10 +1430,12 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
                        : (check_interval > 1 ? check_interval : 1);
   const int count = B.count;
   int st;
-  // All solves must share the PCG workgroup height (one launch per grid).
-  const int ty = B.h[0].tl.ty;
-  for (int s = 1; s < count; ++s)
-    if (B.h[s].tl.ty != ty || B.h[s].tl32.ty != ty || B.h[s].tt.ty != B.h[0].tt.ty) {
+  // All solves must share the PCG workgroup height (one launch per grid);
+  // the multigrid cycle's FP32 sweeps have their own (sblk), the mixed inner
+  // PCG's FP32 kernels launch with the FP64 one.
+  const int ty = B.h[0].tl.ty, ty32 = B.mgm ? B.h[0].tl32.ty : ty;
+  for (int s = 0; s < count; ++s)
+    if (B.h[s].tl.ty != ty || B.h[s].tl32.ty != ty32 || B.h[s].tt.ty != B.h[0].tt.ty) {
       set_error(ERR_ARGUMENT, "batched solves need one workgroup height");
       return ERR_ARGUMENT;
     }

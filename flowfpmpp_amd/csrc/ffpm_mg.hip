@@ -219,6 +219,12 @@ int mg_check_interval(int check_interval) {
   return c < 1 ? 1 : c;
 }
 
+int mg_rows() {
+  const char *e = std::getenv("FFPM_MG_ROWS");
+  const int r = e ? std::atoi(e) : 4;
+  return r == 1 || r == 2 || r == 4 || r == 8 || r == 16 ? r : 4;
+}
+
 int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
             const uint8_t *code, const Tiling &tl, int cl, double *partials) {
   mg.g64 = g64;

@@ -112,6 +112,11 @@ void mg_bind_operator(Multigrid &mg, char *&base);
 void mg_bind_state(Multigrid &mg, char *&base);
 int mg_build(const Multigrid &mg);
 
+// Rows per workgroup of the level-0 FP32 kernels (the cycle's sweeps):
+// FFPM_MG_ROWS (1, 2, 4, 8, 16), default 4, measured faster than the 8 rows
+// of the FP64 kernels at 512x384x256 (141 against 145 us per sweep).
+int mg_rows();
+
 // Convergence-check granularity of the multigrid CG: FFPM_MG_CHECK
 // iterations (default 2, at most the caller's check interval); the FP64 true
 // residual itself is formed only near convergence.

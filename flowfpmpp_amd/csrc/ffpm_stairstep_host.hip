@@ -84,6 +84,9 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   for (int cl : {1, 2, 4}) {
     f.tl[cl] = make_tiling(nx, ny, nz, 0, pcg_rows, kTileX * cl);
     if ((size_t)f.tl[cl].blocks > nb) nb = (size_t)f.tl[cl].blocks;
+    // the multigrid cycle's FP32 level-0 tiling (its sweeps' partials)
+    const Tiling tm = make_tiling(nx, ny, nz, 0, mg_rows(), kTileX * cl);
+    if ((size_t)tm.blocks > nb) nb = (size_t)tm.blocks;
   }
   double *coef = (double *)ws.get(S_COEF, sizeof(double) * 2 * (nx + ny + nz), &st);
   if (!coef) return st;
@@ -835,7 +838,8 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   const GeomT<double> g = geom_t(f.g);
   const int cl = cells_per_lane<double>(f.nx), cl32 = cells_per_lane<float>(f.nx);
   Multigrid mg;
-  if ((st = mg_setup(mg, g, g32, f.code, f.tl[cl32], cl32, f.partials))) return st;
+  const Tiling tm = make_tiling(f.nx, f.ny, f.nz, 0, mg_rows(), kTileX * cl32);
+  if ((st = mg_setup(mg, g, g32, f.code, tm, cl32, f.partials))) return st;
   g_mg = MgStats();
   g_mg.levels = mg.levels;
   const int ci = mg_check_interval(check_interval);
