@@ -148,6 +148,81 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=20.0, parallel=False):
     }
 
 
+def batch64_frames():
+    """The 64 uneven frames x 3 directions of the batched workload, resident on
+    the device with the compatibility correction applied; returns
+    (problems, cells)."""
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd import workloads as W
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    T = DeviceArray.from_host
+    probs, cells = [], 0
+    for i in range(64):
+        p = W.batch_instance(i)
+        solid = T(p.solid)
+        axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+        for d in range(3):
+            rhs, bg, faces = W.influence_inputs(p, d)
+            d_rhs, d_faces = T(rhs), [T(f) for f in faces]
+            S.enforce_stairstep_compatibility_3d(d_rhs, solid, bg, *d_faces, *axes[3:])
+            probs.append(S.BatchProblem(T(p.phi0()), d_rhs, solid, bg, *d_faces, *axes))
+        cells += 3 * p.cells
+    synchronize()
+    return probs, cells
+
+
+def run_batch(probs, method, tol, max_iters, check, steps, warmup, rows=0, zchunk=0):
+    """Times `steps` batched solves of all problems from phi = 0 (after
+    `warmup` untimed ones); returns solves/s, p50/p99 of the per-solve time
+    to convergence, iteration range and residuals."""
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd.device import synchronize
+
+    def step():
+        for q in probs:
+            q.phi.zero_()
+        S.solve_stairstep_batch(probs, tol, max_iters, check, rows=rows, zchunk=zchunk,
+                                method=method)
+    for _ in range(warmup):
+        step()
+    synchronize()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        step()
+        synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = 1e3 * sum(times) / len(times)
+    solve_ms = np.array([q.solve_ms for q in probs])
+    sweeps = np.array([q.sweeps for q in probs])
+    res = np.array([q.residual for q in probs])
+    log(f"batched {method}: {ms:.1f} ms for {len(probs)} solves; iterations "
+        f"{sweeps.min()}-{sweeps.max()}")
+    return {"ms": round(ms, 3), "solves_per_s": round(len(probs) * 1e3 / ms, 3),
+            "solve_ms_p50": round(float(np.percentile(solve_ms, 50)), 3),
+            "solve_ms_p99": round(float(np.percentile(solve_ms, 99)), 3),
+            "iterations_min_max": [int(sweeps.min()), int(sweeps.max())],
+            "max_residual": float(res.max()), "converged": bool((res <= tol).all())}
+
+
+def batch64_summary(tol, max_iters, check, steps=2, warmup=1):
+    """The batched workload as one object of the headline line: 192 solves by
+    the batched multigrid CG with the autotuned launch shape."""
+    from flowfpmpp_amd import stairstep as S
+    t0 = time.perf_counter()
+    probs, cells = batch64_frames()
+    log(f"batch64: 64 frames x 3 directions resident ({cells} cells, "
+        f"{time.perf_counter() - t0:.1f} s)")
+    rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=10, method="fcg-mg")
+    out = run_batch(probs, "fcg-mg", tol, max_iters, check, steps, warmup, rows, zchunk)
+    out.update({"workload": "64 frames (seeds 4000-4063), grids 64^3/96x80x64/128x96x64/"
+                            "160x128x96, 1-3 bodies, 3 directions each: 192 solves in one "
+                            "batched fcg-mg solve to 1e-10",
+                "method": "fcg-mg", "cells": cells, "rows": rows, "zchunk": zchunk,
+                "steps": steps, "warmup": warmup})
+    return out
+
+
 def bench_batch(args):
     """64 uneven frames (grids from 64^3 to 160x128x96, 1-3 bodies each) x 3
     force directions = 192 solves to the tolerance in one batched solve
@@ -158,62 +233,19 @@ def bench_batch(args):
     "pcg_mixed_batched") and as 192 fcg-mg solves one after another
     ("fcg_mg_sequential"); --method pcg / pcg-mixed time that batched PCG."""
     from flowfpmpp_amd import stairstep as S
-    from flowfpmpp_amd import workloads as W
-    from flowfpmpp_amd.device import DeviceArray, synchronize
-    T = DeviceArray.from_host
+    from flowfpmpp_amd.device import synchronize
     t_gen = time.perf_counter()
-    frames = []
-    for i in range(64):
-        p = W.batch_instance(i)
-        solid = T(p.solid)
-        axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
-        dirs = []
-        for d in range(3):
-            rhs, bg, faces = W.influence_inputs(p, d)
-            d_rhs, d_faces = T(rhs), [T(f) for f in faces]
-            S.enforce_stairstep_compatibility_3d(d_rhs, solid, bg, *d_faces, *axes[3:])
-            dirs.append(S.BatchProblem(T(p.phi0()), d_rhs, solid, bg, *d_faces, *axes))
-        frames.append((p, dirs))
-    probs = [q for _, dirs in frames for q in dirs]
-    cells = sum(p.cells for p, _ in frames) * 3
-    synchronize()
+    probs, cells = batch64_frames()
     log(f"64 frames x 3 directions resident: {cells} cells, {time.perf_counter() - t_gen:.1f} s")
     bmethod = args.method if args.method in ("pcg", "pcg-mixed", "fcg-mg") else "pcg-mixed"
-
-    def run_batch(method, steps, warmup, rows=0, zchunk=0):
-        def step():
-            for q in probs:
-                q.phi.zero_()
-            S.solve_stairstep_batch(probs, args.tol, args.max_iters, args.check, rows=rows,
-                                    zchunk=zchunk, method=method)
-        for _ in range(warmup):
-            step()
-        synchronize()
-        times = []
-        for _ in range(steps):
-            t0 = time.perf_counter()
-            step()
-            synchronize()
-            times.append(time.perf_counter() - t0)
-        ms = 1e3 * sum(times) / len(times)
-        solve_ms = np.array([q.solve_ms for q in probs])
-        sweeps = np.array([q.sweeps for q in probs])
-        res = np.array([q.residual for q in probs])
-        log(f"batched {method}: {ms:.1f} ms for 192 solves; iterations "
-            f"{sweeps.min()}-{sweeps.max()}")
-        return {"ms": round(ms, 3), "solves_per_s": round(192 * 1e3 / ms, 3),
-                "solve_ms_p50": round(float(np.percentile(solve_ms, 50)), 3),
-                "solve_ms_p99": round(float(np.percentile(solve_ms, 99)), 3),
-                "iterations_min_max": [int(sweeps.min()), int(sweeps.max())],
-                "max_residual": float(res.max()), "converged": bool((res <= args.tol).all())}
-
     if args.shape:
         (rows, zchunk), ms_it = args.shape, None
     else:
         rows, zchunk, ms_it = S.autotune_stairstep_batch(
             probs, iterations=10 if bmethod == "fcg-mg" else 20, method=bmethod)
         log(f"autotuned launch shape: {rows} rows, z chunk {zchunk} ({ms_it:.3f} ms/iteration)")
-    main = run_batch(bmethod, args.steps, args.warmup, rows, zchunk)
+    main = run_batch(probs, bmethod, args.tol, args.max_iters, args.check, args.steps,
+                     args.warmup, rows, zchunk)
     out = {
         "metric": f"batched solves/s: 64 uneven frames x 3 directions, {bmethod} to 1e-10",
         "value": main["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
@@ -234,7 +266,8 @@ def bench_batch(args):
     if bmethod == "fcg-mg" and not args.no_compare:
         rows, zchunk, ms_it = S.autotune_stairstep_batch(probs, iterations=20,
                                                          method="pcg-mixed")
-        mixed = run_batch("pcg-mixed", 1, 1, rows, zchunk)
+        mixed = run_batch(probs, "pcg-mixed", args.tol, args.max_iters, args.check, 1, 1, rows,
+                          zchunk)
         mixed.update(rows=rows, zchunk=zchunk)
         out["pcg_mixed_batched"] = mixed
         for q in probs[:3]:
@@ -267,12 +300,11 @@ def bench_batch(args):
         out["batch_speedup_vs_sequential"] = round(seq * 1e3 / main["ms"], 3)
     elif args.sequential:
         t0 = time.perf_counter()
-        for (p, dirs) in frames:
-            for q in dirs:
-                q.phi.zero_()
-                S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
-                                             *q.axes, args.tol, args.max_iters, args.check,
-                                             method=bmethod)
+        for q in probs:
+            q.phi.zero_()
+            S.solve_stairstep_poisson_3d(q.phi, q.rhs, q.solid, q.body_grad, *q.faces,
+                                         *q.axes, args.tol, args.max_iters, args.check,
+                                         method=bmethod)
         synchronize()
         seq = time.perf_counter() - t0
         out["sequential_ms"] = round(seq * 1e3, 3)
@@ -436,6 +468,8 @@ def main():
     ap.add_argument("--check", type=int, default=100)
     ap.add_argument("--max-iters", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="frame workload: skip the batch64 object (192 batched solves)")
     ap.add_argument("--workload", default="frame",
                     choices=["frame", "batch64", "reference", "jacobi"],
                     help="frame: the 3-direction solve of one frame (the headline); batch64: "
@@ -590,6 +624,9 @@ def main():
                            "residuals": pcg_res, "check_interval": args.check,
                            "speedup": round(pcg_ms / ms_per_step, 2),
                            "max_rel_diff_of_force_totals": dtot}
+    if not args.no_batch and tuple(args.grid) == (512, 384, 256):
+        log("batched workload: 64 uneven frames x 3 directions")
+        out["batch64"] = batch64_summary(args.tol, args.max_iters, args.check)
     if not args.no_cpu and pcg_iters:
         # the reference's CPU path runs many MPI ranks on a node: the baseline
         # is the all-thread OpenMP build of the oracle, the serial one beside it

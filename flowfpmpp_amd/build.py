@@ -39,7 +39,7 @@ FORTRAN_DIR = os.path.join(PKG, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
 FORTRAN_MODULES = ["fpm_ffpm_status.f90", "fpm_stairstep_mpi_3d.f90", "fpm_laplace_3d.f90",
                    "fpm_terms_3d.f90", "fpm_partition_2d.f90"]
-FORTRAN_TESTS = ["test_stairstep_exact_3d.f90", "test_fpm_3d.f90", "test_fpm_partition_2d.f90"]
+FORTRAN_TESTS = ["test_stairstep_exact_3d.f90", "test_fpm_3d.f90", "partition_frame_driver.f90"]
 
 
 def build(verbose: bool = False) -> str:

@@ -5,15 +5,19 @@ module fpm_stairstep_mpi_3d
   ! (include/ffpm.h). One GPU holds the whole domain, so every neighbour rank
   ! must be MPI_PROC_NULL (any negative value); comm is accepted and unused.
   ! use_openacc is accepted for source compatibility: the work always runs on
-  ! the GPU. Errors of the reference's own checks stop with the reference's
-  ! message text.
-  use,intrinsic :: iso_fortran_env,only:real64
+  ! the GPU, and use_openacc=.false. prints a one-time notice on standard
+  ! error saying so (a caller's CPU-against-GPU check would otherwise compare
+  ! the GPU with itself without knowing). Errors of the reference's own checks
+  ! stop with the reference's message text.
+  use,intrinsic :: iso_fortran_env,only:real64,error_unit
   use,intrinsic :: iso_c_binding
   use fpm_ffpm_status,only:stop_on
   implicit none
   private
   public::exchange_solid_halo_3d,enforce_stairstep_compatibility_3d,solve_stairstep_poisson_3d
   public::stairstep_residual_3d,compute_stairstep_fpm_terms_3d
+
+  logical,save::backend_noticed=.false.
 
   interface
     integer(c_int) function c_exchange_solid_halo(solid,nx,ny,nz,west,east,south,north) &
@@ -65,6 +69,19 @@ module fpm_stairstep_mpi_3d
   end interface
 
 contains
+  ! The reference's use_openacc=.false. runs its host loops
+  ! (src/fpm_stairstep_mpi_3d.f90:124,440); this library has only the GPU
+  ! backend, so the call runs on the GPU and the first such call says so.
+  subroutine backend_notice(use_openacc,routine)
+    logical,intent(in)::use_openacc
+    character(*),intent(in)::routine
+    if(use_openacc.or.backend_noticed)return
+    backend_noticed=.true.
+    write(error_unit,'(3a)')'fpm_stairstep_mpi_3d: ',routine, &
+         ': use_openacc=.false. requested, but this library has only the GPU backend; '// &
+         'the call runs on the GPU'
+  end subroutine backend_notice
+
   ! src/fpm_stairstep_mpi_3d.f90:25-36
   subroutine exchange_solid_halo_3d(solid,nx,ny,nz,west,east,south,north,comm)
     integer,intent(in)::nx,ny,nz,west,east,south,north,comm
@@ -100,7 +117,7 @@ contains
     character(*),intent(in),optional::method
     integer(c_int)::m
     if(comm==comm)continue
-    if(present(use_openacc))continue
+    if(present(use_openacc))call backend_notice(use_openacc,'solve_stairstep_poisson_3d')
     m=0
     if(present(method))then
       ! 'pcg-mixed' is this library's FP32-inner / FP64-outer PCG, 'fcg-mg'
@@ -141,7 +158,7 @@ contains
     real(real64),intent(out)::totals(2)
     logical,intent(in),optional::use_openacc
     if(comm==comm)continue
-    if(present(use_openacc))continue
+    if(present(use_openacc))call backend_notice(use_openacc,'compute_stairstep_fpm_terms_3d')
     call stop_on(c_terms(phi,u,v,w,ox,oy,oz,ax,ay,az,solid,body_grad,xc,yc,zc,dx,dy,dz, &
          xlo,xhi,ylo,yhi,zlo,zhi,bounds,gx,gy,gz,vortex,advection,totals,nx,ny,nz, &
          west,east,south,north))

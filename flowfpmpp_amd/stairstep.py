@@ -13,7 +13,12 @@ scalars are returned instead of passed. Arrays may be
   resident on the device.
 
 ``use_openacc`` is accepted for source compatibility: this library has one
-backend, the GPU, and runs there whatever the flag says.
+backend, the GPU. ``use_openacc=False`` still runs on the GPU and warns once
+(a ``RuntimeWarning``), so a CPU-against-GPU comparison is never silently a
+GPU-against-GPU one.
+
+Every array's shape is checked against the grid (taken from ``solid``)
+before the library is called: a mismatch raises ValueError.
 """
 from __future__ import annotations
 
@@ -26,6 +31,21 @@ from .device import DeviceArray
 
 PROC_NULL = -1
 METHODS = {"pcg": 0, "jacobi": 1, "pcg-mixed": 2, "fcg-mg": 4}
+
+
+_NOTICED = set()
+
+
+def _backend_notice(use_openacc, routine):
+    """The reference's use_openacc switch (src/fpm_stairstep_mpi_3d.f90:116,124,
+    436,440) selects its host loops when false. This library has only the GPU
+    backend: the call still runs on the GPU, and a warning says so once per
+    routine."""
+    if use_openacc is False and routine not in _NOTICED:
+        import warnings
+        _NOTICED.add(routine)
+        warnings.warn(f"{routine}: use_openacc=False requested, but this library has only the "
+                      "GPU backend; the call runs on the GPU", RuntimeWarning, stacklevel=3)
 
 
 def _is_dev(a) -> bool:
@@ -65,8 +85,52 @@ def _mode(*arrays) -> str:
 
 
 def _shape(solid):
+    if len(solid.shape) != 3:
+        raise ValueError(f"solid must be 3-D (nx+2, ny+2, nz), got shape {solid.shape}")
     nxp, nyp, nz = solid.shape
     return nxp - 2, nyp - 2, nz
+
+
+# Fortran shapes of the reference's dummy arguments (src/fpm_stairstep_mpi_3d.f90:
+# 106-123, 426-439), as functions of the grid taken from solid.
+_SHAPES = {
+    "padded": lambda nx, ny, nz: (nx + 2, ny + 2, nz),
+    "cells": lambda nx, ny, nz: (nx, ny, nz),
+    "xface": lambda nx, ny, nz: (ny, nz),
+    "yface": lambda nx, ny, nz: (nx, nz),
+    "zface": lambda nx, ny, nz: (nx, ny),
+    "xc": lambda nx, ny, nz: (nx + 2,),
+    "yc": lambda nx, ny, nz: (ny + 2,),
+    "zc": lambda nx, ny, nz: (nz,),
+    "dx": lambda nx, ny, nz: (nx,),
+    "dy": lambda nx, ny, nz: (ny,),
+    "dz": lambda nx, ny, nz: (nz,),
+}
+_FACE_KINDS = ("xface", "xface", "yface", "yface", "zface", "zface")
+_FACE_NAMES = ("xlo", "xhi", "ylo", "yhi", "zlo", "zhi")
+_AXIS_NAMES = ("xc", "yc", "zc", "dx", "dy", "dz")
+
+
+def _check_shapes(grid, **named):
+    """Raises ValueError unless every array has the shape its reference dummy
+    argument has for this grid: the library reads and writes exactly that many
+    elements, so a wrongly sized array would be read or written past its end.
+    Each keyword is name=(array, kind); kind is a key of _SHAPES."""
+    for name, (a, kind) in named.items():
+        if a is None:
+            continue
+        want = _SHAPES[kind](*grid)
+        got = tuple(int(x) for x in a.shape)
+        if got != want:
+            raise ValueError(f"{name} has shape {got}; the grid {grid} needs {want}")
+
+
+def _check_frame(grid, faces=None, axes=None, **named):
+    if faces is not None:
+        named.update({n: (f, k) for n, f, k in zip(_FACE_NAMES, faces, _FACE_KINDS)})
+    if axes is not None:
+        named.update({n: (a, n) for n, a in zip(_AXIS_NAMES, axes) if a is not None})
+    _check_shapes(grid, **named)
 
 
 def exchange_solid_halo_3d(solid, west=PROC_NULL, east=PROC_NULL, south=PROC_NULL,
@@ -83,6 +147,8 @@ def enforce_stairstep_compatibility_3d(rhs, solid, body_grad, xlo, xhi, ylo, yhi
     """src/fpm_stairstep_mpi_3d.f90:41-92; updates the face arrays in place and
     returns (correction, imbalance)."""
     nx, ny, nz = _shape(solid)
+    _check_frame((nx, ny, nz), (xlo, xhi, ylo, yhi, zlo, zhi), (None, None, None, dx, dy, dz),
+                 rhs=(rhs, "cells"))
     mode = _mode(rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, dx, dy, dz)
     bg, bgp = _vec(body_grad, 3)
     corr, imb = C.c_double(), C.c_double()
@@ -104,10 +170,12 @@ def solve_stairstep_poisson_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, z
     method "fcg-mg" (no reference counterpart) runs FP64 flexible CG with an
     FP32 geometric-multigrid preconditioner, stopping on the same FP64
     max-norm residual; sweeps counts FCG iterations."""
-    del use_openacc
+    _backend_notice(use_openacc, "solve_stairstep_poisson_3d")
     if method not in METHODS:
         raise FFPMError(4, "FFPM_ERR_METHOD: stair-step method must be pcg or jacobi")
     nx, ny, nz = _shape(solid)
+    _check_frame((nx, ny, nz), (xlo, xhi, ylo, yhi, zlo, zhi), (xc, yc, zc, dx, dy, dz),
+                 phi=(phi, "padded"), rhs=(rhs, "cells"))
     mode = _mode(phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz)
     bg, bgp = _vec(body_grad, 3)
     sweeps, res = C.c_int(), C.c_double()
@@ -124,6 +192,8 @@ def stairstep_residual_3d(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, z
                           north=PROC_NULL):
     """src/fpm_stairstep_mpi_3d.f90:403-419"""
     nx, ny, nz = _shape(solid)
+    _check_frame((nx, ny, nz), (xlo, xhi, ylo, yhi, zlo, zhi), (xc, yc, zc, dx, dy, dz),
+                 phi=(phi, "padded"), rhs=(rhs, "cells"))
     mode = _mode(phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz)
     bg, bgp = _vec(body_grad, 3)
     val = C.c_double()
@@ -142,11 +212,17 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
     """src/fpm_stairstep_mpi_3d.f90:426-469; returns
     (gx, gy, gz, vortex, advection, totals). Output arrays are allocated
     when not given (numpy for host inputs, DeviceArray for device inputs)."""
-    del use_openacc
+    _backend_notice(use_openacc, "compute_stairstep_fpm_terms_3d")
     nx, ny, nz = _shape(solid)
+    outs = [gx, gy, gz, vortex, advection]
+    _check_frame((nx, ny, nz), (xlo, xhi, ylo, yhi, zlo, zhi), (xc, yc, zc, dx, dy, dz),
+                 phi=(phi, "padded"),
+                 **{n: (a, "cells") for n, a in zip(("u", "v", "w", "ox", "oy", "oz", "ax", "ay",
+                                                     "az"), (u, v, w, ox, oy, oz, ax, ay, az))},
+                 **{n: (a, "cells") for n, a in zip(("gx", "gy", "gz", "vortex", "advection"),
+                                                    outs)})
     mode = _mode(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, xc, yc, zc, dx, dy, dz, xlo, xhi,
                  ylo, yhi, zlo, zhi)
-    outs = [gx, gy, gz, vortex, advection]
     for m in range(5):
         if outs[m] is None:
             outs[m] = (DeviceArray((nx, ny, nz)) if mode == "_dev"
@@ -186,6 +262,16 @@ def compute_stairstep_fpm_terms_3d_dirs(phis, u, v, w, ox, oy, oz, ax, ay, az, s
         outs = [[DeviceArray((nx, ny, nz)) for _ in range(5)] for _ in range(nd)]
     if len(outs) != nd or any(len(o) != 5 for o in outs):
         raise ValueError("five output arrays per direction")
+    g = (nx, ny, nz)
+    _check_frame(g, None, (xc, yc, zc, dx, dy, dz),
+                 **{f"phi[{d}]": (phis[d], "padded") for d in range(nd)},
+                 **{n: (a, "cells") for n, a in zip(("u", "v", "w", "ox", "oy", "oz", "ax", "ay",
+                                                     "az"), (u, v, w, ox, oy, oz, ax, ay, az))},
+                 **{f"outs[{d}][{m}]": (outs[d][m], "cells") for d in range(nd)
+                    for m in range(5)})
+    for d in range(nd):
+        _check_shapes(g, **{f"{n}[{d}]": (f, k) for n, f, k in
+                            zip(_FACE_NAMES, faces[d], _FACE_KINDS)})
     if _mode(*flat, *[a for o in outs for a in o]) != "_dev":
         raise TypeError("the multi-direction densities take device arrays")
     P = C.c_void_p * nd
@@ -262,6 +348,15 @@ class BatchProblem:
         arrays = (phi, rhs, solid, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc, zc, dx, dy, dz)
         if not all(_is_dev(a) for a in arrays):
             raise TypeError("batched solves take device arrays")
+        _check_frame(_shape(solid), (xlo, xhi, ylo, yhi, zlo, zhi), (xc, yc, zc, dx, dy, dz),
+                     phi=(phi, "padded"), rhs=(rhs, "cells"))
+        for name, a in (("phi", phi), ("rhs", rhs), *zip(_FACE_NAMES, (xlo, xhi, ylo, yhi, zlo,
+                                                                       zhi)),
+                        *zip(_AXIS_NAMES, (xc, yc, zc, dx, dy, dz))):
+            if a.dtype != np.float64:
+                raise TypeError(f"{name} must hold float64")
+        if solid.dtype != np.int32:
+            raise TypeError("solid must hold int32")
         self.phi, self.rhs, self.solid = phi, rhs, solid
         self.body_grad = np.asarray(body_grad, dtype=np.float64).reshape(3)
         self.faces = (xlo, xhi, ylo, yhi, zlo, zhi)

@@ -270,6 +270,13 @@ def influence_inputs(p: Problem, direction: int):
 BATCH_GRIDS = ((64, 64, 64), (96, 80, 64), (128, 96, 64), (160, 128, 96))
 
 
+def batch_instance_grid(i: int):
+    """The grid of batch instance i (the first draw of its seed), without
+    building the frame."""
+    rng = np.random.Generator(np.random.PCG64(4000 + i))
+    return BATCH_GRIDS[int(rng.integers(len(BATCH_GRIDS)))]
+
+
 def batch_instance(i: int) -> Problem:
     """Instance i of the batched workload (seed 4000 + i): a grid drawn from
     BATCH_GRIDS and 1-3 spheres or boxes (radius / sides ~ U[0.3, 1.0]) on the

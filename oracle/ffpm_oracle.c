@@ -19,8 +19,9 @@
 #define PH(i, j, k) ((size_t)(i) + (size_t)(nx + 2) * ((size_t)(j) + (size_t)(ny + 2) * (size_t)((k) - 1)))
 /* (1:nx, 1:ny, 1:nz) */
 /* The parallel CPU-baseline build (-fopenmp -DFFPM_ORACLE_OMP) splits the
- * z-plane loops of the PCG path over threads; the checker build is serial, so
- * its sums keep the reference's order. */
+ * z-plane loops of the PCG path and of the Jacobi sweep over threads; the
+ * checker build is serial, so its sums keep the reference's order (the Jacobi
+ * sweep has no sums: both builds write the same bits). */
 #ifdef FFPM_ORACLE_OMP
 #define ORC_OMP(x) _Pragma(#x)
 #else
@@ -187,6 +188,9 @@ static int build_operator(const double *rhs, const int32_t *solid, const double 
 int orc_jacobi_sweep_3d(const double *a, double *b, const double *src, const double *rdiag,
                         const double *cw, const double *ce, const double *cs, const double *cn,
                         const double *cb, const double *ct, int nx, int ny, int nz) {
+  /* every b cell is formed from a alone, so the threaded build writes the
+   * same bits as the serial one */
+  ORC_OMP(omp parallel for schedule(static))
   for (int k = 1; k <= nz; ++k) {
     const int km = k > 1 ? k - 1 : 1, kp = k < nz ? k + 1 : nz;
     for (int j = 1; j <= ny; ++j)

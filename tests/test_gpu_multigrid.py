@@ -4,10 +4,11 @@
 The method is judged like the reference's own solves: the FP64 max-norm
 residual of the reference's formula (src/fpm_stairstep_mpi_3d.f90:514-530)
 must reach the tolerance, and the field must agree with the FP64 PCG field of
-the CPU oracle (or of the GPU PCG, itself pinned to the oracle) to what two
-solves converged to a 1e-10 residual can differ by: 1e-8 relative max-norm
-here, with the densities within 1e-6 relative. The exact discrete solution
-phi = g.x is reproduced with the reference test's own criteria
+the CPU oracle (or of the GPU PCG, itself pinned to the oracle) within the
+north star's bound for two solves converged to the same 1e-10 residual:
+1e-10 relative max-norm for phi and for the densities. The exact discrete
+solution phi = g.x is reproduced to roundoff (1e-12; the reference reports
+<= 1e-13, README.md:114-119) and with the reference test's own criteria
 (tests/test_stairstep_exact_3d.f90:46-50)."""
 import numpy as np
 import pytest
@@ -52,10 +53,11 @@ def test_fcg_mg_matches_oracle_pcg(orc, grid):
           f"(oracle PCG {so}); max rel |phi - phi_PCG| = {rel:.2e}", flush=True)
     assert rm <= 1e-10 and ro <= 1e-10
     assert levels >= 2 and 0 < sm <= 100
-    assert rel <= 1e-8
+    assert rel <= 1e-10
     chk = S.stairstep_residual_3d(pm, rhs, p.solid, bg, *faces, p.xc, p.yc, p.zc, p.dx, p.dy,
                                   p.dz)
-    assert chk <= 1e-8  # the independent face-gradient residual agrees
+    # the independent face-gradient residual agrees (tests/test_stairstep_exact_3d.f90:68)
+    assert abs(chk - rm) <= 1e-8 * rm + 1e-12
 
 
 def test_fcg_mg_exact_solution():
@@ -72,38 +74,21 @@ def test_fcg_mg_exact_solution():
         phi, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"], f["az"],
         c.solid, c.body, c.xc, c.yc, c.zc, c.dx, c.dy, c.dz, *faces, c.bounds)
     errs, ierr = cases.exact_expectations(c, phi, gx, gy, gz, vo, tot)
-    print(f"fcg-mg exact: {sw} iterations, residual {res:.2e}, errors {errs}", flush=True)
+    print(f"fcg-mg exact: {sw} iterations, residual {res:.2e}, errors {errs}, integral "
+          f"{ierr:.2e}", flush=True)
     assert abs(corr) < 1e-13 and abs(imb) < 1e-13
     assert res <= 2e-10
     assert errs[0] < 1e-9 and errs[1] < 1e-9 and errs[2] < 1e-9 and errs[3] == 0
+    assert errs[0] <= 1e-12 and errs[1] <= 1e-12 and errs[2] <= 1e-12
     assert ierr < 1e-10
 
 
-def test_fcg_mg_on_the_128x96x64_frame():
-    """All three directions of the 128x96x64 frame (sphere and wall-touching
-    box, 4:1 stretching) against the GPU FP64 PCG solve, device arrays."""
-    print("case: fcg-mg, 128x96x64 frame, 3 directions", flush=True)
-    p = W.frame_128x96x64()
-    f = p.fields
-    for d in range(3):
-        out = {}
-        for method in ("pcg", "fcg-mg"):
-            ph, sw, res, rhs, bg, faces = _frame_solve(p, d, method)
-            dens = S.compute_stairstep_fpm_terms_3d(
-                ph, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"],
-                f["az"], p.solid, bg, p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, *faces, p.bounds)
-            out[method] = (ph, sw, res, dens)
-        a, b = out["fcg-mg"], out["pcg"]
-        e_phi = np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()
-        e_vort = np.abs(a[3][3] - b[3][3]).max() / np.abs(b[3][3]).max()
-        e_adv = np.abs(a[3][4] - b[3][4]).max() / np.abs(b[3][4]).max()
-        print(f"direction {d}: FP64 PCG {b[1]} iterations; fcg-mg {a[1]} iterations, residual "
-              f"{a[2]:.2e}; max rel |phi - phi_PCG| {e_phi:.2e}, vortex density {e_vort:.2e}, "
-              f"advection density {e_adv:.2e}", flush=True)
-        assert a[2] <= 1e-10 and b[2] <= 1e-10
-        assert a[1] <= 100
-        assert e_phi <= 1e-8 and e_vort <= 1e-6 and e_adv <= 1e-6
-    # the device entry point gives the same field as the host one
+def test_fcg_mg_device_entry_equals_host_entry():
+    """On the 128x96x64 frame the device entry point gives the host entry
+    point's field bit for bit (the three directions against the oracle are in
+    test_gpu_headline.py)."""
+    print("case: fcg-mg, 128x96x64 frame, device vs host entry point", flush=True)
+    p = W.frame_128x96x64(with_fields=False)
     rhs, bg, faces = W.influence_inputs(p, 1)
     S.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
     T = DeviceArray.from_host
@@ -113,6 +98,7 @@ def test_fcg_mg_on_the_128x96x64_frame():
         *[T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)], 1e-10, 100000, 100,
         method="fcg-mg")
     hphi, hsw, hres, _, _, _ = _frame_solve(p, 1, "fcg-mg")
+    assert res <= 1e-10
     assert sw == hsw and res == hres and np.array_equal(dphi.download(), hphi)
 
 
@@ -143,7 +129,7 @@ def test_fcg_mg_two_cell_deep_frame():
         rel = np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()
         print(f"direction {d}: PCG {b[1]}, fcg-mg {a[1]} iterations; rel diff {rel:.2e}",
               flush=True)
-        assert a[2] <= 1e-10 and rel <= 1e-8 and a[1] <= 100
+        assert a[2] <= 1e-10 and b[2] <= 1e-10 and rel <= 1e-10 and a[1] <= 100
 
 
 def test_fcg_mg_strongly_stretched_frame():
@@ -188,7 +174,7 @@ def test_fcg_mg_strongly_stretched_frame():
         print(f"direction {d}: PCG {b[1]} iterations ({b[2]:.2e}), fcg-mg {a[1]} ({a[2]:.2e}); "
               f"rel diff {rel:.2e}", flush=True)
         assert a[2] <= 1e-10 and b[2] <= 1e-10
-        assert a[1] <= 400 and rel <= 1e-8
+        assert a[1] <= 400 and rel <= 1e-10
 
 
 
@@ -213,4 +199,4 @@ def test_chebyshev_and_jacobi_smoothing_agree(monkeypatch):
           f"max rel diff {rel:.2e}", flush=True)
     assert rc <= 1e-10 and rj <= 1e-10
     assert sc_ < sj
-    assert rel <= 1e-8
+    assert rel <= 1e-10

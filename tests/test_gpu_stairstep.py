@@ -130,9 +130,14 @@ def test_exact_case_pass_criteria_on_gpu(orc):
             phi, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"], f["az"],
             c.solid, c.body, c.xc, c.yc, c.zc, c.dx, c.dy, c.dz, *faces, c.bounds)
         errs, ierr = cases.exact_expectations(c, phi, gx, gy, gz, vo, tot)
+        print(f"  {method}: {sw} iterations, residual {res:.2e}; errors phi {errs[0]:.1e}, "
+              f"gradient {errs[1]:.1e}, vortex {errs[2]:.1e}, integrals {ierr:.1e}", flush=True)
         assert abs(corr) < 1e-13 and abs(imb) < 1e-13
         assert res <= 2 * tol and chk < 1e-9
         assert errs[0] < 1e-9 and errs[1] < 1e-9 and errs[2] < 1e-9 and errs[3] == 0
+        # roundoff, as the reference reports (README.md:114-119: <= 1e-13 phi,
+        # 5.9e-13 gradient, 7e-13 vortex density)
+        assert errs[0] <= 1e-12 and errs[1] <= 1e-12 and errs[2] <= 1e-12
         assert ierr < 1e-10
         assert np.all(np.isfinite(errs))
 
@@ -313,9 +318,9 @@ def test_fused_and_three_kernel_pcg_agree(orc, monkeypatch):
 @pytest.mark.parametrize("nx_mult", [4, 2, 1])
 def test_mixed_precision_reaches_fp64_tolerance(orc, nx_mult):
     """FP32-inner / FP64-outer PCG stops on the FP64 max-norm residual of the
-    reference's formula; its field matches the FP64 oracle to the accuracy a
-    1e-10 residual allows (bound 1e-8 relative max-norm here, measured values
-    are far smaller), and the densities follow."""
+    reference's formula; its field matches the FP64 oracle
+    and the densities follow: within 1e-10 relative max-norm, the bound for two
+    solves converged to the same residual."""
     grids = {4: (48, 36, 24), 2: (46, 36, 24), 1: (45, 35, 23)}
     nx, ny, nz = grids[nx_mult]
     print(f"case: mixed precision on {nx}x{ny}x{nz}", flush=True)
@@ -335,14 +340,14 @@ def test_mixed_precision_reaches_fp64_tolerance(orc, nx_mult):
     rel = np.abs(pm - po).max() / np.abs(po).max()
     print(f"mixed: {outer} corrections, {inner} FP32 iterations (FP64 PCG {so}); "
           f"max rel |phi - phi_FP64| = {rel:.2e}", flush=True)
-    assert rel <= 1e-8
+    assert rel <= 1e-10
     f = p.fields
     args = lambda ph: (ph, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"],
                        f["az"], p.solid, bg, p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, *faces, p.bounds)
     gm = S.compute_stairstep_fpm_terms_3d(*args(pm))
     go = orc.compute_stairstep_fpm_terms_3d(*args(po))
     for a, b in zip(gm[3:5], go[3:5]):
-        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max()
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
 
 
 
@@ -368,41 +373,6 @@ def test_streaming_jacobi_matches_one_cell_kernel(monkeypatch, grid):
     a, b = out["classic"], out["streaming"]
     assert a[1] == b[1] == 301 and a[2] == b[2]
     assert np.array_equal(a[0], b[0])
-
-
-def test_mixed_precision_on_the_128x96x64_frame():
-    """The mixed-precision workload: the 128x96x64 frame (sphere r=0.5 and a
-    box touching the outer boundary), all three directions, FP32-inner /
-    FP64-outer PCG to the FP64 residual 1e-10, against the FP64 PCG solve of
-    the same frame (which matches the CPU oracle to 1e-10, see the tests
-    above). Reported: max |phi - phi_FP64| relative to max |phi_FP64| and the
-    vortex/advection density errors; bounds 1e-8 and 1e-6 relative."""
-    print("case: mixed precision, 128x96x64 frame, 3 directions", flush=True)
-    p = W.frame_128x96x64()
-    f = p.fields
-    for d in range(3):
-        rhs, bg, faces = W.influence_inputs(p, d)
-        S.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
-        out = {}
-        for method in ("pcg", "pcg-mixed"):
-            ph = p.phi0()
-            sw, res = S.solve_stairstep_poisson_3d(ph, rhs, p.solid, bg, *faces, p.xc, p.yc,
-                                                   p.zc, p.dx, p.dy, p.dz, 1e-10, 100000, 100,
-                                                   method=method)
-            dens = S.compute_stairstep_fpm_terms_3d(
-                ph, f["u"], f["v"], f["w"], f["ox"], f["oy"], f["oz"], f["ax"], f["ay"],
-                f["az"], p.solid, bg, p.xc, p.yc, p.zc, p.dx, p.dy, p.dz, *faces, p.bounds)
-            out[method] = (ph, sw, res, dens)
-        a, b = out["pcg-mixed"], out["pcg"]
-        assert a[2] <= 1e-10 and b[2] <= 1e-10
-        e_phi = np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()
-        e_vort = np.abs(a[3][3] - b[3][3]).max() / np.abs(b[3][3]).max()
-        e_adv = np.abs(a[3][4] - b[3][4]).max() / np.abs(b[3][4]).max()
-        outer, inner = S.last_mixed_solve()
-        print(f"direction {d}: FP64 PCG {b[1]} iterations; mixed {outer} corrections, {inner} "
-              f"FP32 iterations; max rel |phi - phi_FP64| {e_phi:.2e}, vortex density "
-              f"{e_vort:.2e}, advection density {e_adv:.2e}", flush=True)
-        assert e_phi <= 1e-8 and e_vort <= 1e-6 and e_adv <= 1e-6
 
 
 @pytest.mark.parametrize("nd", [1, 2, 3])
