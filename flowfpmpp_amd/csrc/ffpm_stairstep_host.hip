@@ -5,6 +5,7 @@
 #include <cfloat>
 #include <cstdlib>
 #include <initializer_list>
+#include <mutex>
 #include <cstring>
 #include <string>
 
@@ -1110,17 +1111,19 @@ int terms_dev(const double *phi, const double *u, const double *v, const double 
 // compute_stairstep_fpm_terms_3d for nd directions (1-3) of one frame in one
 // pass over the flow fields; per-direction arguments are host arrays of
 // device pointers, body_grad holds 3 nd values and totals 2 nd.
-int terms_dirs_dev(int nd, const double *const *phi, const double *u, const double *v,
-                   const double *w, const double *ox, const double *oy, const double *oz,
-                   const double *ax, const double *ay, const double *az, const int32_t *solid,
-                   const double *bg, const double *xc, const double *yc, const double *zc,
-                   const double *dx, const double *dy, const double *dz,
-                   const double *const *xlo, const double *const *xhi,
-                   const double *const *ylo, const double *const *yhi,
-                   const double *const *zlo, const double *const *zhi, const double *bounds,
-                   double *const *gx, double *const *gy, double *const *gz,
-                   double *const *vortex, double *const *advection, double *totals, int nx,
-                   int ny, int nz, int west, int east, int south, int north) {
+// Checks the arguments of the nd-direction densities and gathers the
+// per-direction pointers and body gradients.
+int pack_terms_dirs(int nd, const double *const *phi, const double *u, const double *v,
+                    const double *w, const double *ox, const double *oy, const double *oz,
+                    const double *ax, const double *ay, const double *az, const int32_t *solid,
+                    const double *bg, const double *xc, const double *yc, const double *zc,
+                    const double *dx, const double *dy, const double *dz,
+                    const double *const *xlo, const double *const *xhi,
+                    const double *const *ylo, const double *const *yhi,
+                    const double *const *zlo, const double *const *zhi, const double *bounds,
+                    double *const *gx, double *const *gy, double *const *gz,
+                    double *const *vortex, double *const *advection, double *totals, int nx,
+                    int ny, int nz, int west, int east, int south, int north, TermsDirs &a) {
   if (nd < 1 || nd > 3) {
     set_error(ERR_ARGUMENT, "densities: 1 to 3 directions");
     return ERR_ARGUMENT;
@@ -1133,7 +1136,7 @@ int terms_dirs_dev(int nd, const double *const *phi, const double *u, const doub
   if (!all_present({phi, u, v, w, ox, oy, oz, ax, ay, az, solid, bg, xc, yc, zc, dx, dy, dz, xlo,
                     xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex, advection, totals}))
     return null_error();
-  TermsDirs a{};
+  a = TermsDirs{};
   for (int d = 0; d < nd; ++d) {
     if (!all_present({phi[d], xlo[d], xhi[d], ylo[d], yhi[d], zlo[d], zhi[d], gx[d], gy[d],
                       gz[d], vortex[d], advection[d]}))
@@ -1152,25 +1155,33 @@ int terms_dirs_dev(int nd, const double *const *phi, const double *u, const doub
     a.vortex[d] = vortex[d];
     a.advection[d] = advection[d];
   }
-  Frame f;
-  int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
-  if (st) return st;
+  return OK;
+}
+
+// The densities pass and its integral finish: per-workgroup partials (6 per
+// workgroup of elementwise_blocks(n)) in `part`, the 2 nd totals in `out`
+// (device), copied to the host array `totals`.
+int launch_terms_dirs(int nd, const TermsDirs &a, const Geom &g, size_t n, const int32_t *solid,
+                      const double *u, const double *v, const double *w, const double *ox,
+                      const double *oy, const double *oz, const double *ax, const double *ay,
+                      const double *az, const double *bounds, double *part, double *out,
+                      double *totals, bool timed) {
   Bounds bd;
   for (int i = 0; i < 6; ++i) bd.b[i] = bounds[i];
-  const int eb = elementwise_blocks(f.n);
-  double *part = (double *)workspace().get(S_TD_PART, sizeof(double) * 6 * eb, &st);
-  if (!part) return st;
-  double *out = (double *)workspace().get(S_TD_OUT, sizeof(double) * 6, &st);
-  if (!out) return st;
-  {
-    TimedLaunch tl("terms_dirs");
-    auto go = [&](auto kern) {
-      kern<<<eb, kThreads, 0, stream()>>>(f.g, a, solid, u, v, w, ox, oy, oz, ax, ay, az, bd,
-                                         part);
-    };
+  const int eb = elementwise_blocks(n);
+  auto go = [&](auto kern) {
+    kern<<<eb, kThreads, 0, stream()>>>(g, a, solid, u, v, w, ox, oy, oz, ax, ay, az, bd, part);
+  };
+  auto launch = [&]() {
     if (nd == 3) go(k_terms_dirs<3>);
     else if (nd == 2) go(k_terms_dirs<2>);
     else go(k_terms_dirs<1>);
+  };
+  if (timed) {
+    TimedLaunch tl("terms_dirs");
+    launch();
+  } else {
+    launch();
   }
   FFPM_LAUNCHED("k_terms_dirs");
   k_terms_dirs_finish<<<1, 256, 0, stream()>>>(nd, part, eb, out);
@@ -1179,6 +1190,127 @@ int terms_dirs_dev(int nd, const double *const *phi, const double *u, const doub
   FFPM_HIP(hipStreamSynchronize(stream()));
   return OK;
 }
+
+int terms_dirs_dev(int nd, const double *const *phi, const double *u, const double *v,
+                   const double *w, const double *ox, const double *oy, const double *oz,
+                   const double *ax, const double *ay, const double *az, const int32_t *solid,
+                   const double *bg, const double *xc, const double *yc, const double *zc,
+                   const double *dx, const double *dy, const double *dz,
+                   const double *const *xlo, const double *const *xhi,
+                   const double *const *ylo, const double *const *yhi,
+                   const double *const *zlo, const double *const *zhi, const double *bounds,
+                   double *const *gx, double *const *gy, double *const *gz,
+                   double *const *vortex, double *const *advection, double *totals, int nx,
+                   int ny, int nz, int west, int east, int south, int north) {
+  TermsDirs a;
+  int st = pack_terms_dirs(nd, phi, u, v, w, ox, oy, oz, ax, ay, az, solid, bg, xc, yc, zc, dx,
+                           dy, dz, xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex,
+                           advection, totals, nx, ny, nz, west, east, south, north, a);
+  if (st) return st;
+  Frame f;
+  if ((st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false))) return st;
+  const int eb = elementwise_blocks(f.n);
+  double *part = (double *)workspace().get(S_TD_PART, sizeof(double) * 6 * eb, &st);
+  if (!part) return st;
+  double *out = (double *)workspace().get(S_TD_OUT, sizeof(double) * 6, &st);
+  if (!out) return st;
+  return launch_terms_dirs(nd, a, f.g, f.n, solid, u, v, w, ox, oy, oz, ax, ay, az, bounds, part,
+                           out, totals, true);
+}
+
+}  // namespace ffpm
+
+// ---- densities without allocation --------------------------------------------
+// The scratch memory of ffpm_compute_stairstep_fpm_terms_3d_dirs_noalloc_dev:
+// the 1-D stencil coefficients (2 (nx+ny+nz) doubles), the per-workgroup
+// partials of the integrals (6 per workgroup, at most 8192 workgroups) and the
+// 6 totals. Each GPU holds its own copy at its own address; host code uses only
+// the address hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr int kNoAllocMaxAxisCells = 16384;  // nx + ny + nz
+constexpr int kNoAllocMaxBlocks = 8192;      // the cap of ffpm::elementwise_blocks
+constexpr size_t kNoAllocCoef = 2 * (size_t)kNoAllocMaxAxisCells;
+constexpr size_t kNoAllocPart = 6 * (size_t)kNoAllocMaxBlocks;
+constexpr size_t kNoAllocDoubles = kNoAllocCoef + kNoAllocPart + 32;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace ffpm {
+namespace {
+
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+int scratch_on_current_device(double **ptr) {
+  int dev = 0;
+  FFPM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error(ERR_DEVICE, "device index beyond the library's scratch table");
+    return ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                hipGetErrorString(e));
+      return ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return OK;
+}
+
+int terms_dirs_noalloc_dev(int nd, const double *const *phi, const double *u, const double *v,
+                           const double *w, const double *ox, const double *oy,
+                           const double *oz, const double *ax, const double *ay,
+                           const double *az, const int32_t *solid, const double *bg,
+                           const double *xc, const double *yc, const double *zc,
+                           const double *dx, const double *dy, const double *dz,
+                           const double *const *xlo, const double *const *xhi,
+                           const double *const *ylo, const double *const *yhi,
+                           const double *const *zlo, const double *const *zhi,
+                           const double *bounds, double *const *gx, double *const *gy,
+                           double *const *gz, double *const *vortex,
+                           double *const *advection, double *totals, int nx, int ny, int nz,
+                           int west, int east, int south, int north) {
+  TermsDirs a;
+  int st = pack_terms_dirs(nd, phi, u, v, w, ox, oy, oz, ax, ay, az, solid, bg, xc, yc, zc, dx,
+                           dy, dz, xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex,
+                           advection, totals, nx, ny, nz, west, east, south, north, a);
+  if (st) return st;
+  if ((long long)nx + ny + nz > kNoAllocMaxAxisCells) {
+    set_error(ERR_ARGUMENT, "densities without allocation: nx+ny+nz exceeds 16384, the size of "
+                            "the library's fixed scratch memory");
+    return ERR_ARGUMENT;
+  }
+  double *base = nullptr;
+  if ((st = scratch_on_current_device(&base))) return st;
+  double *cw = base, *ce = cw + nx, *cs = ce + nx, *cn = cs + ny, *cb = cn + ny, *ct = cb + nz;
+  double *part = base + kNoAllocCoef;
+  double *out = part + kNoAllocPart;
+  const int tot = nx + ny + nz;
+  k_axis_coefficients<<<(tot + 255) / 256, 256, 0, stream()>>>(nx, ny, nz, xc, yc, zc, dx, dy, dz,
+                                                               cw, ce, cs, cn, cb, ct);
+  FFPM_LAUNCHED("k_axis_coefficients");
+  Geom g{};
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.cw = cw; g.ce = ce; g.cs = cs; g.cn = cn; g.cb = cb; g.ct = ct;
+  g.dx = dx; g.dy = dy; g.dz = dz;
+  g.xc = xc; g.yc = yc; g.zc = zc;
+  return launch_terms_dirs(nd, a, g, (size_t)nx * ny * nz, solid, u, v, w, ox, oy, oz, ax, ay,
+                           az, bounds, part, out, totals, false);
+}
+
+}  // namespace
 
 // ---- host-pointer staging ---------------------------------------------------
 namespace {
@@ -1504,6 +1636,22 @@ int ffpm_compute_stairstep_fpm_terms_3d_dirs_dev(
   return terms_dirs_dev(ndir, phi, u, v, w, ox, oy, oz, ax, ay, az, solid, body_grad, xc, yc, zc,
                         dx, dy, dz, xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex,
                         advection, totals, nx, ny, nz, west, east, south, north);
+}
+
+int ffpm_compute_stairstep_fpm_terms_3d_dirs_noalloc_dev(
+    int ndir, const double *const *phi, const double *u, const double *v, const double *w,
+    const double *ox, const double *oy, const double *oz, const double *ax, const double *ay,
+    const double *az, const int32_t *solid, const double *body_grad, const double *xc,
+    const double *yc, const double *zc, const double *dx, const double *dy, const double *dz,
+    const double *const *xlo, const double *const *xhi, const double *const *ylo,
+    const double *const *yhi, const double *const *zlo, const double *const *zhi,
+    const double *bounds, double *const *gx, double *const *gy, double *const *gz,
+    double *const *vortex, double *const *advection, double *totals, int nx, int ny, int nz,
+    int west, int east, int south, int north) {
+  return terms_dirs_noalloc_dev(ndir, phi, u, v, w, ox, oy, oz, ax, ay, az, solid, body_grad, xc,
+                                yc, zc, dx, dy, dz, xlo, xhi, ylo, yhi, zlo, zhi, bounds, gx, gy,
+                                gz, vortex, advection, totals, nx, ny, nz, west, east, south,
+                                north);
 }
 
 int ffpm_compute_stairstep_fpm_terms_3d(

@@ -241,7 +241,7 @@ def compute_stairstep_fpm_terms_3d(phi, u, v, w, ox, oy, oz, ax, ay, az, solid, 
 def compute_stairstep_fpm_terms_3d_dirs(phis, u, v, w, ox, oy, oz, ax, ay, az, solid,
                                         body_grads, xc, yc, zc, dx, dy, dz, faces, bounds,
                                         outs=None, west=PROC_NULL, east=PROC_NULL,
-                                        south=PROC_NULL, north=PROC_NULL):
+                                        south=PROC_NULL, north=PROC_NULL, noalloc=False):
     """compute_stairstep_fpm_terms_3d (src/fpm_stairstep_mpi_3d.f90:426-469)
     for 1-3 directions of one frame in one pass over the flow fields (device
     arrays only). phis: the directions' potentials; body_grads: one 3-vector
@@ -249,7 +249,9 @@ def compute_stairstep_fpm_terms_3d_dirs(phis, u, v, w, ox, oy, oz, ax, ay, az, s
     ylo, yhi, zlo, zhi); outs: per direction five output arrays (gx, gy, gz,
     vortex, advection), allocated when None. Returns per direction (gx, gy,
     gz, vortex, advection, totals), each equal bit for bit to what
-    compute_stairstep_fpm_terms_3d returns for that direction."""
+    compute_stairstep_fpm_terms_3d returns for that direction. noalloc: call the
+    library's entry point that allocates no device memory (fixed scratch,
+    nx + ny + nz <= 16384), with the same results."""
     nd = len(phis)
     if not 1 <= nd <= 3 or len(body_grads) != nd or len(faces) != nd:
         raise ValueError("1 to 3 directions, each with a body gradient and six face arrays")
@@ -284,7 +286,9 @@ def compute_stairstep_fpm_terms_3d_dirs(phis, u, v, w, ox, oy, oz, ax, ay, az, s
     totals = np.zeros(2 * nd)
     ptrs = [arr(phis)] + [arr([f[m] for f in faces]) for m in range(6)] + \
         [arr([o[m] for o in outs]) for m in range(5)]
-    check(lib().ffpm_compute_stairstep_fpm_terms_3d_dirs_dev(
+    fn = (lib().ffpm_compute_stairstep_fpm_terms_3d_dirs_noalloc_dev if noalloc
+          else lib().ffpm_compute_stairstep_fpm_terms_3d_dirs_dev)
+    check(fn(
         nd, ptrs[0], _d(u), _d(v), _d(w), _d(ox), _d(oy), _d(oz), _d(ax), _d(ay), _d(az),
         _i(solid), bgp, _d(xc), _d(yc), _d(zc), _d(dx), _d(dy), _d(dz), *ptrs[1:7], bdp,
         *ptrs[7:12], totals.ctypes.data_as(C.c_void_p), nx, ny, nz, west, east, south, north))

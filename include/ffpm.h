@@ -292,6 +292,27 @@ int ffpm_compute_stairstep_fpm_terms_3d_dirs_dev(
     double *const *vortex, double *const *advection, double *totals, int nx, int ny, int nz,
     int west, int east, int south, int north);
 
+/* No counterpart in the reference: the variant of
+ * ffpm_compute_stairstep_fpm_terms_3d_dirs_dev that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * same kernels, same results bit for bit. Its scratch memory (the 1-D stencil
+ * coefficients, the integrals' per-workgroup partials and totals; 640 KB) is
+ * a fixed array the library holds on each GPU it runs on until the process
+ * ends; grids with nx + ny + nz > 16384 return FFPM_ERR_ARGUMENT. Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * threads or streams) race and must be serialised by the caller. */
+int ffpm_compute_stairstep_fpm_terms_3d_dirs_noalloc_dev(
+    int ndir, const double *const *phi, const double *u, const double *v, const double *w,
+    const double *ox, const double *oy, const double *oz, const double *ax, const double *ay,
+    const double *az, const int32_t *solid, const double *body_grad, const double *xc,
+    const double *yc, const double *zc, const double *dx, const double *dy, const double *dz,
+    const double *const *xlo, const double *const *xhi, const double *const *ylo,
+    const double *const *yhi, const double *const *zlo, const double *const *zhi,
+    const double *bounds, double *const *gx, double *const *gy, double *const *gz,
+    double *const *vortex, double *const *advection, double *totals, int nx, int ny, int nz,
+    int west, int east, int south, int north);
+
 /* ------------------------------------------------------------------------ *
  * No-body pure-Neumann solvers, src/fpm_laplace_3d.f90
  * ------------------------------------------------------------------------ */
