@@ -34,19 +34,25 @@ FLAGS = [
 ]
 
 
+# roctx ranges (the reference's trace module's names) for rocprofv3 timelines
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+LIBS = ["-L", ROCM_LIB, "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM_LIB}"]
+
 FC = os.environ.get("FFPM_FC", "/opt/rocm/bin/amdflang")
 FORTRAN_DIR = os.path.join(PKG, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
-FORTRAN_MODULES = ["fpm_ffpm_status.f90", "fpm_stairstep_mpi_3d.f90", "fpm_laplace_3d.f90",
-                   "fpm_terms_3d.f90", "fpm_partition_2d.f90"]
-FORTRAN_TESTS = ["test_stairstep_exact_3d.f90", "test_fpm_3d.f90", "partition_frame_driver.f90"]
+FORTRAN_MODULES = ["fpm_ffpm_status.f90", "fpm_ffpm_frame.f90", "fpm_trace.f90",
+                   "fpm_stairstep_mpi_3d.f90", "fpm_laplace_3d.f90", "fpm_terms_3d.f90",
+                   "fpm_partition_2d.f90"]
+FORTRAN_TESTS = ["test_stairstep_exact_3d.f90", "test_fpm_3d.f90", "partition_frame_driver.f90",
+                 "test_frame_residency.f90"]
 
 
 def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
+    cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, *LIBS, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

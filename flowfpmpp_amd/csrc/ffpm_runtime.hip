@@ -1,14 +1,73 @@
 // Error state, stream, workspaces and kernel timing of the library.
 #include "ffpm_runtime.hpp"
 
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/ffpm.h"
 
+// One bound frame: host arrays with their device copies.
+struct ffpm_frame {
+  struct Array {
+    const char *host;
+    size_t bytes;
+    void *dev;
+    int copy_out;
+  };
+  std::vector<Array> arrays;
+};
+
 namespace ffpm {
+
+namespace {
+std::mutex g_frames_mutex;
+std::vector<ffpm_frame *> g_frames;
+std::atomic<long long> g_h2d_count{0}, g_h2d_bytes{0}, g_d2h_count{0}, g_d2h_bytes{0};
+
+// the bound array holding [p, p + bytes) in any live frame; overlap without
+// containment is reported through *partial
+const ffpm_frame::Array *find_bound(const char *p, size_t bytes, bool *partial) {
+  *partial = false;
+  for (const ffpm_frame *f : g_frames)
+    for (const auto &a : f->arrays) {
+      const char *lo = a.host, *hi = a.host + a.bytes;
+      if (p >= lo && p + bytes <= hi) return &a;
+      if (p < hi && p + bytes > lo) *partial = true;
+    }
+  return nullptr;
+}
+}  // namespace
+
+void *present_device(const void *host, size_t bytes, int *status) {
+  if (!host || bytes == 0) return nullptr;
+  std::lock_guard<std::mutex> lock(g_frames_mutex);
+  if (g_frames.empty()) return nullptr;
+  bool partial = false;
+  const char *p = static_cast<const char *>(host);
+  const ffpm_frame::Array *a = find_bound(p, bytes, &partial);
+  if (a) return static_cast<char *>(a->dev) + (p - a->host);
+  if (partial) {
+    set_error(ERR_ARGUMENT, "an array argument overlaps an array bound to a frame without lying "
+                            "inside it");
+    *status = ERR_ARGUMENT;
+  }
+  return nullptr;
+}
+
+void count_h2d(size_t bytes) {
+  g_h2d_count += 1;
+  g_h2d_bytes += (long long)bytes;
+}
+
+void count_d2h(size_t bytes) {
+  g_d2h_count += 1;
+  g_d2h_bytes += (long long)bytes;
+}
 
 namespace {
 thread_local std::string g_error;
@@ -176,6 +235,133 @@ __global__ __launch_bounds__(256) void k_copy(size_t n2, const double2 *__restri
 
 extern "C" {
 
+int ffpm_frame_create(ffpm_frame **frame) {
+  if (!frame) {
+    ffpm::set_error(ffpm::ERR_ARGUMENT, "null frame pointer");
+    return ffpm::ERR_ARGUMENT;
+  }
+  auto *f = new ffpm_frame();
+  std::lock_guard<std::mutex> lock(ffpm::g_frames_mutex);
+  ffpm::g_frames.push_back(f);
+  *frame = f;
+  return FFPM_OK;
+}
+
+int ffpm_frame_bind(ffpm_frame *frame, void *host, size_t bytes, int copy_out) {
+  using namespace ffpm;
+  if (!frame || !host || bytes == 0) {
+    set_error(ERR_ARGUMENT, "frame binding needs a frame, a host array and its size");
+    return ERR_ARGUMENT;
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_frames_mutex);
+    bool partial = false;
+    if (find_bound(static_cast<const char *>(host), bytes, &partial) || partial) {
+      set_error(ERR_ARGUMENT, "array already bound to a frame");
+      return ERR_ARGUMENT;
+    }
+  }
+  void *dev = nullptr;
+  FFPM_HIP(hipMalloc(&dev, bytes));
+  const hipError_t e = hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream());
+  const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(stream()) : e;
+  if (e2 != hipSuccess) {
+    (void)hipFree(dev);
+    return hip_fail(e2, "frame upload", __FILE__, __LINE__);
+  }
+  count_h2d(bytes);
+  std::lock_guard<std::mutex> lock(g_frames_mutex);
+  frame->arrays.push_back({static_cast<const char *>(host), bytes, dev, copy_out ? 1 : 0});
+  return FFPM_OK;
+}
+
+namespace {
+const ffpm_frame::Array *frame_array(ffpm_frame *frame, const void *host) {
+  if (!frame) return nullptr;
+  for (const auto &a : frame->arrays)
+    if (a.host == static_cast<const char *>(host)) return &a;
+  return nullptr;
+}
+}  // namespace
+
+int ffpm_frame_upload(ffpm_frame *frame, const void *host) {
+  using namespace ffpm;
+  const ffpm_frame::Array *a = frame_array(frame, host);
+  if (!a) {
+    set_error(ERR_ARGUMENT, "array not bound to this frame");
+    return ERR_ARGUMENT;
+  }
+  FFPM_HIP(hipMemcpyAsync(a->dev, a->host, a->bytes, hipMemcpyHostToDevice, stream()));
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  count_h2d(a->bytes);
+  return FFPM_OK;
+}
+
+int ffpm_frame_download(ffpm_frame *frame, void *host) {
+  using namespace ffpm;
+  const ffpm_frame::Array *a = frame_array(frame, host);
+  if (!a) {
+    set_error(ERR_ARGUMENT, "array not bound to this frame");
+    return ERR_ARGUMENT;
+  }
+  FFPM_HIP(hipMemcpyAsync(host, a->dev, a->bytes, hipMemcpyDeviceToHost, stream()));
+  FFPM_HIP(hipStreamSynchronize(stream()));
+  count_d2h(a->bytes);
+  return FFPM_OK;
+}
+
+int ffpm_frame_destroy(ffpm_frame *frame) {
+  using namespace ffpm;
+  if (!frame) return FFPM_OK;
+  int status = FFPM_OK;
+  for (const auto &a : frame->arrays) {
+    if (a.copy_out && status == FFPM_OK) {
+      hipError_t e = hipMemcpyAsync(const_cast<char *>(a.host), a.dev, a.bytes,
+                                    hipMemcpyDeviceToHost, stream());
+      if (e == hipSuccess) e = hipStreamSynchronize(stream());
+      if (e != hipSuccess) status = hip_fail(e, "frame download", __FILE__, __LINE__);
+      else count_d2h(a.bytes);
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_frames_mutex);
+    for (size_t i = 0; i < g_frames.size(); ++i)
+      if (g_frames[i] == frame) {
+        g_frames.erase(g_frames.begin() + (long)i);
+        break;
+      }
+  }
+  for (const auto &a : frame->arrays) (void)hipFree(a.dev);
+  delete frame;
+  return status;
+}
+
+int ffpm_transfer_counts(long long *counts) {
+  if (!counts) {
+    ffpm::set_error(ffpm::ERR_ARGUMENT, "null output pointer");
+    return ffpm::ERR_ARGUMENT;
+  }
+  counts[0] = ffpm::g_h2d_count;
+  counts[1] = ffpm::g_h2d_bytes;
+  counts[2] = ffpm::g_d2h_count;
+  counts[3] = ffpm::g_d2h_bytes;
+  return FFPM_OK;
+}
+
+int ffpm_trace_push(const char *name) {
+  if (!name) {
+    ffpm::set_error(ffpm::ERR_ARGUMENT, "null trace range name");
+    return ffpm::ERR_ARGUMENT;
+  }
+  roctxRangePushA(name);
+  return FFPM_OK;
+}
+
+int ffpm_trace_pop(void) {
+  roctxRangePop();
+  return FFPM_OK;
+}
+
 int ffpm_copy_bandwidth(size_t bytes, int reps, double *gbps, double *avg_us) {
   void *a = nullptr, *b = nullptr;
   const size_t n2 = bytes / 16;
@@ -236,12 +422,14 @@ int ffpm_free(void *ptr) {
 int ffpm_memcpy_htod(void *dst, const void *src, size_t bytes) {
   FFPM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ffpm::stream()));
   FFPM_HIP(hipStreamSynchronize(ffpm::stream()));
+  ffpm::count_h2d(bytes);
   return FFPM_OK;
 }
 
 int ffpm_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   FFPM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ffpm::stream()));
   FFPM_HIP(hipStreamSynchronize(ffpm::stream()));
+  ffpm::count_d2h(bytes);
   return FFPM_OK;
 }
 

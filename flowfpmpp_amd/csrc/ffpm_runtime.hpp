@@ -10,11 +10,47 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "ffpm_kernels.hpp"
 
 namespace ffpm {
 
+// A named roctx range over a scope (rocprofv3 --marker-trace shows it on the
+// timeline): the ranges the reference emits through its trace module,
+// 'stair-step setup', 'stair-step iterations', 'stair-step gauge' and
+// 'stair-step FPM terms' (src/fpm_stairstep_mpi_3d.f90:131,162,169,441).
+class TraceRange {
+ public:
+  explicit TraceRange(const char *name) { roctxRangePushA(name); }
+  ~TraceRange() {
+    if (open_) roctxRangePop();
+  }
+  // ends this range and opens the next one
+  void next(const char *name) {
+    if (open_) roctxRangePop();
+    roctxRangePushA(name);
+    open_ = true;
+  }
+  TraceRange(const TraceRange &) = delete;
+  TraceRange &operator=(const TraceRange &) = delete;
+
+ private:
+  bool open_ = true;
+};
+
 hipStream_t stream();
+
+// Host arrays bound to device copies by ffpm_frame_bind (the role of the
+// caller's OpenACC data region, src/fpm_stairstep_mpi_3d.f90:145-148):
+// the device copy of [host, host + bytes) when that range lies inside one
+// bound array, else nullptr. A range that overlaps a bound array without
+// lying inside it sets *status to ERR_ARGUMENT.
+void *present_device(const void *host, size_t bytes, int *status);
+// Host<->device copies made by the library (staging and frames), counted
+// for ffpm_transfer_counts.
+void count_h2d(size_t bytes);
+void count_d2h(size_t bytes);
 
 // Grow-only device buffers addressed by slot id; reallocated only when a
 // larger size is requested. One workspace per process (the library is not

@@ -953,6 +953,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
     return null_error();
   const bool pcg = method == FFPM_METHOD_PCG || method == FFPM_METHOD_PCG_MIXED ||
                    method == FFPM_METHOD_FCG || method == FFPM_METHOD_FCG_MG;
+  TraceRange trace("stair-step setup");
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, true, pcg);
   if (st) return st;
@@ -975,6 +976,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   FFPM_LAUNCHED("k_pack_phi");
   double *cur = f.phi;
   g_mixed = MixedStats();
+  trace.next("stair-step iterations");
   if (method == FFPM_METHOD_FCG) {
     st = fcg_iterate(f, *fcg, tolerance, max_sweeps, check_interval, sweeps, residual);
   } else if (method == FFPM_METHOD_FCG_MG) {
@@ -991,6 +993,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   }
   if (st) return st;
   // gauge (src/fpm_stairstep_mpi_3d.f90:170-187)
+  trace.next("stair-step gauge");
   k_gauge_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, cur, f.partials, f.sc);
   FFPM_LAUNCHED("k_gauge_sums");
   finish(f, FIN_SUMS, 2, eb);
@@ -1088,6 +1091,7 @@ int terms_dev(const double *phi, const double *u, const double *v, const double 
   if (!all_present({phi, u, v, w, ox, oy, oz, ax, ay, az, solid, bg, xc, yc, zc, dx, dy, dz, xlo,
                     xhi, ylo, yhi, zlo, zhi, bounds, gx, gy, gz, vortex, advection, totals}))
     return null_error();
+  TraceRange trace("stair-step FPM terms");
   Frame f;
   int st = frame_setup(f, nx, ny, nz, xc, yc, zc, dx, dy, dz, false, false);
   if (st) return st;
@@ -1166,6 +1170,7 @@ int launch_terms_dirs(int nd, const TermsDirs &a, const Geom &g, size_t n, const
                       const double *oy, const double *oz, const double *ax, const double *ay,
                       const double *az, const double *bounds, double *part, double *out,
                       double *totals, bool timed) {
+  TraceRange trace("stair-step FPM terms");
   Bounds bd;
   for (int i = 0; i < 6; ++i) bd.b[i] = bounds[i];
   const int eb = elementwise_blocks(n);
@@ -1315,23 +1320,40 @@ int terms_dirs_noalloc_dev(int nd, const double *const *phi, const double *u, co
 // ---- host-pointer staging ---------------------------------------------------
 namespace {
 
+// A host array bound to a frame (ffpm_frame_bind) is already resident: its
+// device copy is used in place and nothing moves, as inside the caller's
+// OpenACC data region (src/fpm_stairstep_mpi_3d.f90:145-148).
 template <class T>
 int stage_in(int slot, const T *host, size_t count, T **dev) {
   int st = 0;
+  if ((*dev = (T *)present_device(host, sizeof(T) * count, &st))) return OK;
+  if (st) return st;
   *dev = (T *)workspace().get(slot, sizeof(T) * count, &st);
   if (!*dev) return st;
   FFPM_HIP(hipMemcpyAsync(*dev, host, sizeof(T) * count, hipMemcpyHostToDevice, stream()));
+  count_h2d(sizeof(T) * count);
   return OK;
 }
 
 template <class T>
 int stage_out(T *host, const T *dev, size_t count) {
+  int st = 0;
+  if (T *p = (T *)present_device(host, sizeof(T) * count, &st)) {
+    if (p != dev)
+      FFPM_HIP(hipMemcpyAsync(p, dev, sizeof(T) * count, hipMemcpyDeviceToDevice, stream()));
+    return OK;
+  }
+  if (st) return st;
   FFPM_HIP(hipMemcpyAsync(host, dev, sizeof(T) * count, hipMemcpyDeviceToHost, stream()));
+  count_d2h(sizeof(T) * count);
   return OK;
 }
 
-int alloc_out(int slot, size_t count, double **dev) {
+// an output array: its bound device copy, else a workspace slot
+int alloc_out(int slot, const double *host, size_t count, double **dev) {
   int st = 0;
+  if ((*dev = (double *)present_device(host, sizeof(double) * count, &st))) return OK;
+  if (st) return st;
   *dev = (double *)workspace().get(slot, sizeof(double) * count, &st);
   return *dev ? OK : st;
 }
@@ -1428,7 +1450,12 @@ int ffpm_exchange_solid_halo_3d(int32_t *solid, int nx, int ny, int nz, int west
       plane[sx * (ny + 1) + i] = 0;
     }
   }
-  return FFPM_OK;
+  // a solid array bound to a frame gets the same halo in its device copy
+  int st = 0;
+  const size_t np = sx * sy * nz;
+  if (int32_t *d = (int32_t *)present_device(solid, sizeof(int32_t) * np, &st))
+    return ffpm_exchange_solid_halo_3d_dev(d, nx, ny, nz, west, east, south, north);
+  return st;
 }
 
 int ffpm_exchange_solid_halo_3d_dev(int32_t *solid, int nx, int ny, int nz, int west, int east,
@@ -1675,8 +1702,9 @@ int ffpm_compute_stairstep_fpm_terms_3d(
   if ((st = stage_in(S_U_SOLID, solid, np, &d_solid))) return st;
   for (int m = 0; m < 9; ++m)
     if ((st = stage_in(S_U_F0 + m, fin[m], n, &f[m]))) return st;
+  double *hout[5] = {gx, gy, gz, vortex, advection};
   for (int m = 0; m < 5; ++m)
-    if ((st = alloc_out(S_U_O0 + m, n, &o[m]))) return st;
+    if ((st = alloc_out(S_U_O0 + m, hout[m], n, &o[m]))) return st;
   if ((st = stage_faces(nx, ny, nz, xlo, xhi, ylo, yhi, zlo, zhi, s))) return st;
   if ((st = stage_axes(nx, ny, nz, xc, yc, zc, dx, dy, dz, s))) return st;
   st = terms_dev(d_phi, f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], d_solid, body_grad,
@@ -1684,7 +1712,6 @@ int ffpm_compute_stairstep_fpm_terms_3d(
                  bounds, o[0], o[1], o[2], o[3], o[4], totals, nx, ny, nz, west, east, south,
                  north);
   if (st) return st;
-  double *hout[5] = {gx, gy, gz, vortex, advection};
   for (int m = 0; m < 5; ++m)
     if ((st = stage_out(hout[m], o[m], n))) return st;
   return sync();

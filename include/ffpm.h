@@ -84,6 +84,38 @@ int ffpm_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int ffpm_memset_zero(void *ptr, size_t bytes);
 int ffpm_synchronize(void);
 
+/* trace_push / trace_pop of the reference's trace module (src/fpm_trace.f90:
+ * 7-13, NVTX version src/fpm_trace_nvtx.f90:8-15): a named range on the
+ * rocprofv3 timeline (roctx; shown with --marker-trace). The solver emits
+ * the reference's own four ranges itself: 'stair-step setup', 'stair-step
+ * iterations', 'stair-step gauge' and 'stair-step FPM terms'
+ * (src/fpm_stairstep_mpi_3d.f90:131,162,169,441). name is NUL-terminated. */
+int ffpm_trace_push(const char *name);
+int ffpm_trace_pop(void);
+
+/* Frames: host arrays kept resident in HBM across calls, the role the
+ * caller's OpenACC data region plays for the reference (`!$acc data copy(phi)
+ * copyin(...)`, src/fpm_stairstep_mpi_3d.f90:145-148; the benchmark's frame
+ * region, tests/benchmark_stairstep_3d.f90:151). ffpm_frame_bind uploads a
+ * host array once; until ffpm_frame_destroy every host-pointer entry point
+ * uses that array's device copy in place instead of staging it (inputs are
+ * not uploaded again, outputs are not downloaded). ffpm_frame_destroy copies
+ * the arrays bound with copy_out != 0 back to the host and frees the device
+ * copies. ffpm_frame_upload / ffpm_frame_download refresh one bound array in
+ * either direction (`!$acc update device / self`). An array may be bound to
+ * one frame at a time; an argument that overlaps a bound array without lying
+ * inside it is an error. No counterpart in the reference's own API. */
+typedef struct ffpm_frame ffpm_frame;
+int ffpm_frame_create(ffpm_frame **frame);
+int ffpm_frame_bind(ffpm_frame *frame, void *host, size_t bytes, int copy_out);
+int ffpm_frame_upload(ffpm_frame *frame, const void *host);
+int ffpm_frame_download(ffpm_frame *frame, void *host);
+int ffpm_frame_destroy(ffpm_frame *frame);
+/* Host-to-device and device-to-host copies made by the library since the
+ * process started: counts[0] uploads, counts[1] bytes uploaded, counts[2]
+ * downloads, counts[3] bytes downloaded. */
+int ffpm_transfer_counts(long long counts[4]);
+
 /* ------------------------------------------------------------------------ *
  * Stair-step solve, src/fpm_stairstep_mpi_3d.f90
  * ------------------------------------------------------------------------ */

@@ -31,6 +31,22 @@ def test_fortran_dropin_program(program):
     assert "PASS" in out.stdout
 
 
+def test_fortran_frame_keeps_arrays_resident():
+    """compat -> solve -> densities through fpm_stairstep_mpi_3d, unbound and
+    inside an fpm_ffpm_frame frame: bit-identical results, no transfers
+    inside the frame, one upload per bound array and one download per
+    copy_out array (the program checks these and prints PASS); the unbound
+    run passes use_openacc=.false., which prints the backend notice once."""
+    exe = _exe("test_frame_residency")
+    print("case: Fortran frame residency, 40x32x24", flush=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(out.stdout)
+    print(out.stderr)
+    assert out.returncode == 0, out.stderr
+    assert "PASS" in out.stdout
+    assert out.stderr.count("use_openacc=.false. requested") == 1
+
+
 def _write_frame(path, c, slip):
     nx, ny = c["phi"].shape
     f64 = lambda a: np.asarray(a, dtype="<f8").ravel(order="F").tobytes()
