@@ -312,6 +312,67 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def unet_flops(cells):
+    """Multiply-add FLOPs of one forward pass of the U-Net on `cells` cells: the
+    3x3x3 convolutions of its three levels (channels in x out per level:
+    16x16, 16x16, 48x16, 16x16 at full resolution; 16x32, 32x32, 96x32, 32x32
+    at 1/8; 32x64, 64x64 at 1/64)."""
+    return 27 * 2 * cells * ((16 * 16 + 16 * 16 + 48 * 16 + 16 * 16)
+                             + (16 * 32 + 32 * 32 + 96 * 32 + 32 * 32) / 8
+                             + (32 * 64 + 64 * 64) / 64)
+
+
+def bench_unet(args):
+    """The learned preconditioner's network: forward passes on a frame of
+    --grid (default 256x192x128) and training steps (forward, backward, Adam)
+    at 128^3; MFMA TFLOP/s of the forward pass against the dense BF16 peak."""
+    from flowfpmpp_amd import learned as L
+    from flowfpmpp_amd import workloads as W
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    grid = tuple(args.grid) if tuple(args.grid) != (512, 384, 256) else (256, 192, 128)
+    reps = max(1, args.steps)
+    net = L.UNet(seed=1)
+    rng = np.random.Generator(np.random.PCG64(1))
+    p = W.stretched_frame(*grid, L.random_bodies(rng), seed=1, with_fields=False)
+    feats = DeviceArray.from_host(L.features(p, 0))
+    src = DeviceArray.from_host(rng.standard_normal(grid).astype(np.float32))
+    out = DeviceArray(grid, np.float32)
+    for _ in range(max(1, args.warmup)):
+        net.forward(feats, src, out)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(feats, src, out)
+    synchronize()
+    fwd = (time.perf_counter() - t0) / reps
+    tg = (128, 128, 128)
+    q = W.stretched_frame(*tg, L.random_bodies(rng), seed=2, with_fields=False)
+    tf = DeviceArray.from_host(L.features(q, 1))
+    ts = DeviceArray.from_host(rng.standard_normal(tg).astype(np.float32))
+    tt = DeviceArray.from_host(rng.standard_normal(tg).astype(np.float32))
+    tm = DeviceArray.from_host(np.ones(tg, dtype=np.float32))
+    net.train_step(tf, ts, tt, tm, 1e-4)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.train_step(tf, ts, tt, tm, 1e-4)
+    synchronize()
+    step = (time.perf_counter() - t0) / reps
+    flop = unet_flops(int(np.prod(grid)))
+    tflops = flop / fwd / 1e12
+    out = {"metric": "U-Net forward time (ms) and MFMA TFLOP/s", "value": round(fwd * 1e3, 3),
+           "unit": "ms", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+           "ms_per_step": round(fwd * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+           "config": {"workload": f"U-Net forward on {grid[0]}x{grid[1]}x{grid[2]}, training "
+                                  "steps on 128^3", "grid": list(grid)},
+           "forward_tflop": round(flop / 1e12, 4), "forward_tflops": round(tflops, 1),
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0,
+                        "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4), "traffic": None},
+           "train_grid": list(tg), "train_step_ms": round(step * 1e3, 3)}
+    print(json.dumps(out), flush=True)
+
+
 # The reference's published A100 times (seconds) for the same frames:
 # benchmarks/pcg_time_to_solution.csv:5-9 and benchmarks/stairstep_benchmark.csv:9-12.
 A100_PCG_1E8 = {(128, 128, 32): (1000, 0.1759), (256, 256, 64): (1900, 1.249),
@@ -471,12 +532,14 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="frame workload: skip the batch64 object (192 batched solves)")
     ap.add_argument("--workload", default="frame",
-                    choices=["frame", "batch64", "reference", "jacobi"],
+                    choices=["frame", "batch64", "reference", "jacobi", "unet"],
                     help="frame: the 3-direction solve of one frame (the headline); batch64: "
                          "64 uneven frames x 3 directions in one batched solve; reference: "
                          "the reference's own benchmark frames, timed as it times them; "
                          "jacobi: weighted Jacobi on the frame's x direction, capped at "
-                         "--max-iters sweeps in chunks of --check, extrapolated to --tol")
+                         "--max-iters sweeps in chunks of --check, extrapolated to --tol; "
+                         "unet: the learned preconditioner's network, forward passes on "
+                         "--grid (256x192x128 by default) and training steps at 128^3")
     ap.add_argument("--sequential", action="store_true",
                     help="batch64: also time the 192 solves one after another")
     ap.add_argument("--shape", type=int, nargs=2, metavar=("ROWS", "ZCHUNK"),
@@ -498,6 +561,8 @@ def main():
         return bench_reference(args)
     if args.workload == "jacobi":
         return bench_jacobi(args)
+    if args.workload == "unet":
+        return bench_unet(args)
 
     t_gen = time.perf_counter()
     p = frame(tuple(args.grid))

@@ -30,6 +30,7 @@ namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef uint16_t bf16;
 
 // BF16 <-> FP32: BF16 is the upper half of the FP32 encoding; rounding to
@@ -65,68 +66,122 @@ __device__ inline Seg seg_of(long seg, int segx, int ny) {
 // y = act(conv(x, w) + b); bias may be null. With a gate tensor (and no bias
 // or ReLU), y = conv(x, w) * (gate > 0): the data gradient of a convolution
 // times the ReLU derivative of the layer below (gate = that layer's output).
-template <int NB, int MT>
-__global__ __launch_bounds__(64 * kW) void k_conv3(int nx, int ny, int nz, int cin,
-                                                   const bf16 *__restrict__ x,
-                                                   const bf16 *__restrict__ w,
-                                                   const float *__restrict__ bias, int relu,
-                                                   const bf16 *__restrict__ gate,
-                                                   bf16 *__restrict__ y) {
+//
+// Implicit GEMM on v_mfma_f32_16x16x32_bf16 with the input brick in LDS: a
+// workgroup (4 wavefronts) computes a 16 x 4 x 4 brick of output cells; every
+// wavefront owns one z-plane of it, four 16-cell x-rows (the MFMA's 16 rows).
+// The input channels are taken 16 at a time: the workgroup stages the
+// 18 x 6 x 6 input brick (one-cell halo, zeros outside the box) of 16 channels
+// in LDS (31 KB, so five workgroups fit a CU whatever the layer's width), then
+// runs over the taps two at a time, K = 2 taps x 16 channels = 32: lane
+// group g = lane / 16 takes tap 2c + g / 2, channels 8 (g % 2)..+7, reads them
+// for its row (cell lane % 16 shifted by the tap) from LDS as one 16-byte
+// access and the matching 16 bytes of the weights [tap][cout][CIN] for
+// output channel lane % 16 from global memory (cached, shared by the four
+// rows, loaded one step ahead). The 27th tap pairs with zeros. A cell's 16
+// channels sit at a stride of 24 halves in LDS, which spreads 8 consecutive
+// cells over distinct 16-byte bank groups.
+constexpr int kBx = 16, kBy = 4, kBz = 4;                  // output brick
+constexpr int kHx = kBx + 2, kHy = kBy + 2, kHz = kBz + 2;  // input brick
+constexpr int kHalo = kHx * kHy * kHz;
+constexpr int kCg = 16;                                     // channels per stage
+constexpr int kLdsStride = kCg + 8;                         // halves per cell
+constexpr int kPairs = 14;                                  // ceil(27 / 2) tap pairs
+
+//
+// With CA > 0 the input is the decoder's skip concatenation [up(xc) | x]
+// formed while staging: channels 0..CA-1 of a cell are the CA channels of
+// its parent in the half-resolution tensor xc (nearest upsampling; the fine
+// extents are even), channels CA..CIN-1 the CIN - CA channels of x.
+template <int CIN, int NB, int CA>
+__global__ __launch_bounds__(256) void k_conv3(int nx, int ny, int nz,
+                                               const bf16 *__restrict__ x,
+                                               const bf16 *__restrict__ xc,
+                                               const bf16 *__restrict__ w,
+                                               const float *__restrict__ bias, int relu,
+                                               const bf16 *__restrict__ gate,
+                                               bf16 *__restrict__ y) {
   constexpr int cout = NB * 16;
+  constexpr int cb = CIN - CA;  // channels of x
+  static_assert(CIN % kCg == 0 && CA % kCg == 0, "channel counts are multiples of 16");
+  __shared__ __align__(16) bf16 lds[kHalo * kLdsStride];
+  const int tiles_x = (nx + kBx - 1) / kBx, tiles_y = (ny + kBy - 1) / kBy;
+  const int bx = blockIdx.x % tiles_x, by = (blockIdx.x / tiles_x) % tiles_y,
+            bz = blockIdx.x / (tiles_x * tiles_y);
+  const int x0 = bx * kBx, y0 = by * kBy, z0 = bz * kBz;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int segx = (nx + 15) / 16;
-  const long nseg = (long)segx * ny * nz;
-  const long seg0 = ((long)blockIdx.x * kW + wave) * MT;
-  if (seg0 >= nseg) return;
   const int m = lane & 15, g = lane >> 4;
-  v4f acc[MT][NB];
+  v4f acc[kBy][NB];
 #pragma unroll
-  for (int a = 0; a < MT; ++a)
+  for (int a = 0; a < kBy; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  Seg sg[MT];
-  bool live[MT];
+  // the weights of tap pair c of channel group G
+  auto load_b = [&](int G, int c, uint4 (&bv)[NB]) {
+    const int t = 2 * c + (g >> 1), ci = G * kCg + 8 * (g & 1);
 #pragma unroll
-  for (int a = 0; a < MT; ++a) {
-    live[a] = seg0 + a < nseg;
-    sg[a] = seg_of(live[a] ? seg0 + a : seg0, segx, ny);
-  }
-  const int chunks = cin >> 4;
-  for (int t = 0; t < 27; ++t) {
-    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-    long base[MT];
-    bool ok[MT];
-#pragma unroll
-    for (int a = 0; a < MT; ++a) {
-      const int i = sg[a].i0 + m + dx, j = sg[a].j + dy, k = sg[a].k + dz;
-      ok[a] = live[a] && sg[a].i0 + m < nx && i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 &&
-              k < nz;
-      base[a] = ok[a] ? ((long)i + (long)nx * ((long)j + (long)ny * k)) * cin : 0;
+    for (int b = 0; b < NB; ++b)
+      bv[b] = t < 27 ? *reinterpret_cast<const uint4 *>(w + ((size_t)t * cout + b * 16 + m) *
+                                                               CIN + ci)
+                     : make_uint4(0u, 0u, 0u, 0u);
+  };
+  for (int G = 0; G < CIN / kCg; ++G) {
+    if (G > 0) __syncthreads();  // every wavefront is done with the previous group
+    for (int e = threadIdx.x; e < kHalo * 2; e += 256) {
+      const int hc = e >> 1, part = e & 1;
+      const int hx = hc % kHx, hy = (hc / kHx) % kHy, hz = hc / (kHx * kHy);
+      const int i = x0 - 1 + hx, j = y0 - 1 + hy, k = z0 - 1 + hz;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
+        const int ch = G * kCg + part * 8;
+        if (ch < CA) {
+          const int cx = nx >> 1, cy = ny >> 1;
+          v = *reinterpret_cast<const uint4 *>(
+              xc + ((size_t)(i >> 1) + (size_t)cx * ((size_t)(j >> 1) + (size_t)cy * (k >> 1))) *
+                       CA + ch);
+        } else {
+          v = *reinterpret_cast<const uint4 *>(
+              x + ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)) * cb + (ch - CA));
+        }
+      }
+      *reinterpret_cast<uint4 *>(lds + hc * kLdsStride + part * 8) = v;
     }
-    const bf16 *wt = w + (size_t)t * cout * cin;
-    for (int c = 0; c < chunks; ++c) {
-      v4s bfr[NB];
+    uint4 bnext[NB];
+    load_b(G, 0, bnext);
+    __syncthreads();
+    for (int c = 0; c < kPairs; ++c) {
+      const int t = 2 * c + (g >> 1);
+      const bool valid = t < 27;
+      const int tt = valid ? t : 0;
+      const int dz = tt / 9 - 1, dy = (tt / 3) % 3 - 1, dx = tt % 3 - 1;
+      v8bf bfr[NB];
 #pragma unroll
-      for (int b = 0; b < NB; ++b)
-        bfr[b] = *reinterpret_cast<const v4s *>(wt + (size_t)(b * 16 + m) * cin + c * 16 + 4 * g);
+      for (int b = 0; b < NB; ++b) bfr[b] = __builtin_bit_cast(v8bf, bnext[b]);
+      if (c + 1 < kPairs) load_b(G, c + 1, bnext);
+      const int base = (m + dx + 1) + kHx * ((dy + 1) + kHy * (wave + dz + 1));
 #pragma unroll
-      for (int a = 0; a < MT; ++a) {
-        v4s afr = v4s{0, 0, 0, 0};
-        if (ok[a]) afr = *reinterpret_cast<const v4s *>(x + base[a] + c * 16 + 4 * g);
+      for (int a = 0; a < kBy; ++a) {
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (valid)
+          v = *reinterpret_cast<const uint4 *>(lds + (base + kHx * a) * kLdsStride + 8 * (g & 1));
+        const v8bf afr = __builtin_bit_cast(v8bf, v);
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afr, bfr[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[b], acc[a][b], 0, 0, 0);
       }
     }
   }
+  const int k = z0 + wave;
+  if (k >= nz) return;
 #pragma unroll
-  for (int a = 0; a < MT; ++a) {
-    if (!live[a]) continue;
+  for (int a = 0; a < kBy; ++a) {
+    const int j = y0 + a;
+    if (j >= ny) continue;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int i = sg[a].i0 + 4 * g + v;
+      const int i = x0 + 4 * g + v;
       if (i >= nx) continue;
-      const size_t cell = (size_t)i + (size_t)nx * ((size_t)sg[a].j + (size_t)ny * sg[a].k);
+      const size_t cell = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const int co = b * 16 + m;
@@ -541,24 +596,46 @@ int net_alloc_acts(UNet &N, int nx, int ny, int nz, int train) {
   return OK;
 }
 
-template <int NB>
-void launch_conv_nb(int nx, int ny, int nz, int cin, const bf16 *x, const bf16 *w,
-                    const float *b, int relu, const bf16 *gate, bf16 *y, hipStream_t s) {
-  constexpr int MT = NB >= 4 ? 2 : 4;
-  const long nseg = (long)((nx + 15) / 16) * ny * nz;
-  const long waves = (nseg + MT - 1) / MT;
-  const int blocks = (int)((waves + kW - 1) / kW);
-  k_conv3<NB, MT><<<blocks, 64 * kW, 0, s>>>(nx, ny, nz, cin, x, w, b, relu, gate, y);
+template <int CIN, int NB, int CA = 0>
+void launch_conv(int nx, int ny, int nz, const bf16 *x, const bf16 *w, const float *b, int relu,
+                 const bf16 *gate, bf16 *y, const bf16 *xc = nullptr) {
+  const long tiles = (long)((nx + kBx - 1) / kBx) * ((ny + kBy - 1) / kBy) * ((nz + kBz - 1) / kBz);
+  k_conv3<CIN, NB, CA><<<(unsigned)tiles, 256, 0, stream()>>>(nx, ny, nz, x, xc, w, b, relu,
+                                                             gate, y);
 }
 
+// The decoder convolutions on [up(coarse) | skip] without the concatenated
+// tensor: layer 6 (64 up + 32 skip -> 32) and layer 8 (32 up + 16 skip -> 16).
+int conv_concat(int nx, int ny, int nz, int ca, int cin, int cout, const bf16 *coarse,
+                const bf16 *skip, const bf16 *w, const float *b, bf16 *y) {
+  if (ca == 64 && cin == 96 && cout == 32)
+    launch_conv<96, 2, 64>(nx, ny, nz, skip, w, b, 1, nullptr, y, coarse);
+  else if (ca == 32 && cin == 48 && cout == 16)
+    launch_conv<48, 1, 32>(nx, ny, nz, skip, w, b, 1, nullptr, y, coarse);
+  else {
+    set_error(ERR_ARGUMENT, "unsupported concatenation");
+    return ERR_ARGUMENT;
+  }
+  FFPM_LAUNCHED("k_conv3 concat");
+  return OK;
+}
+
+// The (input, output) channel pairs of the network's forward passes and data
+// gradients.
 int conv(int nx, int ny, int nz, int cin, int cout, const bf16 *x, const bf16 *w,
          const float *b, int relu, const bf16 *gate, bf16 *y) {
-  switch (cout) {
-    case 16: launch_conv_nb<1>(nx, ny, nz, cin, x, w, b, relu, gate, y, stream()); break;
-    case 32: launch_conv_nb<2>(nx, ny, nz, cin, x, w, b, relu, gate, y, stream()); break;
-    case 48: launch_conv_nb<3>(nx, ny, nz, cin, x, w, b, relu, gate, y, stream()); break;
-    case 64: launch_conv_nb<4>(nx, ny, nz, cin, x, w, b, relu, gate, y, stream()); break;
-    case 96: launch_conv_nb<6>(nx, ny, nz, cin, x, w, b, relu, gate, y, stream()); break;
+  switch (cin * 1000 + cout) {
+    case 16016: launch_conv<16, 1>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 16032: launch_conv<16, 2>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 16048: launch_conv<16, 3>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 32016: launch_conv<32, 1>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 32032: launch_conv<32, 2>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 32064: launch_conv<32, 4>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 32096: launch_conv<32, 6>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 48016: launch_conv<48, 1>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 64032: launch_conv<64, 2>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 64064: launch_conv<64, 4>(nx, ny, nz, x, w, b, relu, gate, y); break;
+    case 96032: launch_conv<96, 2>(nx, ny, nz, x, w, b, relu, gate, y); break;
     default:
       set_error(ERR_ARGUMENT, "unsupported channel count");
       return ERR_ARGUMENT;
@@ -613,15 +690,26 @@ int net_forward(UNet &N) {
   FFPM_LAUNCHED("k_pool");
   if ((st = conv(x2, y2, z2, 32, 64, N.p2, Wb(N, 4), Bp(N, 4), 1, nullptr, N.a2))) return st;
   if ((st = conv(x2, y2, z2, 64, 64, N.a2, Wb(N, 5), Bp(N, 5), 1, nullptr, N.b2))) return st;
-  k_up<<<ew_blocks(c1 * 64), 256, 0, s>>>(x2, y2, z2, 64, 96, 0, N.b2, N.c1, 0);
-  k_slice<<<ew_blocks(c1 * 32), 256, 0, s>>>(c1, 32, N.s1, 32, 0, N.c1, 96, 64);
-  FFPM_LAUNCHED("k_up/k_slice");
-  if ((st = conv(x1, y1, z1, 96, 32, N.c1, Wb(N, 6), Bp(N, 6), 1, nullptr, N.a1c))) return st;
+  // Training keeps the concatenations (the backward pass reads them);
+  // inference forms them while staging each convolution's input brick.
+  if (N.train) {
+    k_up<<<ew_blocks(c1 * 64), 256, 0, s>>>(x2, y2, z2, 64, 96, 0, N.b2, N.c1, 0);
+    k_slice<<<ew_blocks(c1 * 32), 256, 0, s>>>(c1, 32, N.s1, 32, 0, N.c1, 96, 64);
+    FFPM_LAUNCHED("k_up/k_slice");
+    if ((st = conv(x1, y1, z1, 96, 32, N.c1, Wb(N, 6), Bp(N, 6), 1, nullptr, N.a1c))) return st;
+  } else if ((st = conv_concat(x1, y1, z1, 64, 96, 32, N.b2, N.s1, Wb(N, 6), Bp(N, 6), N.a1c))) {
+    return st;
+  }
   if ((st = conv(x1, y1, z1, 32, 32, N.a1c, Wb(N, 7), Bp(N, 7), 1, nullptr, N.d1))) return st;
-  k_up<<<ew_blocks(c0 * 32), 256, 0, s>>>(x1, y1, z1, 32, 48, 0, N.d1, N.c0, 0);
-  k_slice<<<ew_blocks(c0 * 16), 256, 0, s>>>(c0, 16, N.s0, 16, 0, N.c0, 48, 32);
-  FFPM_LAUNCHED("k_up/k_slice");
-  if ((st = conv(nx, ny, nz, 48, 16, N.c0, Wb(N, 8), Bp(N, 8), 1, nullptr, N.a0c))) return st;
+  if (N.train) {
+    k_up<<<ew_blocks(c0 * 32), 256, 0, s>>>(x1, y1, z1, 32, 48, 0, N.d1, N.c0, 0);
+    k_slice<<<ew_blocks(c0 * 16), 256, 0, s>>>(c0, 16, N.s0, 16, 0, N.c0, 48, 32);
+    FFPM_LAUNCHED("k_up/k_slice");
+    if ((st = conv(nx, ny, nz, 48, 16, N.c0, Wb(N, 8), Bp(N, 8), 1, nullptr, N.a0c))) return st;
+  } else if ((st = conv_concat(nx, ny, nz, 32, 48, 16, N.d1, N.s0, Wb(N, 8), Bp(N, 8),
+                               N.a0c))) {
+    return st;
+  }
   if ((st = conv(nx, ny, nz, 16, 16, N.a0c, Wb(N, 9), Bp(N, 9), 1, nullptr, N.d0))) return st;
   k_head<<<ew_blocks(c0), 256, 0, s>>>(c0, N.d0, N.p + N.L.head_w, N.p + N.L.head_b, N.out);
   FFPM_LAUNCHED("k_head");

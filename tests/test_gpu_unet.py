@@ -61,6 +61,23 @@ def test_forward_matches_cpu_fp64():
     assert rel <= 1e-2
 
 
+def test_inference_and_training_forward_identical():
+    """Inference forms the decoder's skip concatenations while staging each
+    convolution's input; training materialises them for the backward pass.
+    Both feed the same bits to the same MFMA sequence: outputs are identical."""
+    print("case: unet forward, fused vs materialised concatenation, 64x48x32", flush=True)
+    feats, source, target, mask = _sample(64, 48, 32, seed=4)
+    net = L.UNet(seed=9)
+    out_a = DeviceArray(source.shape, np.float32)
+    net.forward(T(feats), T(source), out_a)
+    a = out_a.download()
+    net.train_step(T(feats), T(source), T(target), T(mask), 0.0)  # lr 0: weights unchanged
+    out_b = DeviceArray(source.shape, np.float32)
+    net.forward(T(feats), T(source), out_b)
+    assert np.array_equal(a, out_b.download())
+    assert np.isfinite(a).all() and np.abs(a).max() > 0
+
+
 def test_gradients_match_cpu_fp64():
     print("case: unet gradients 32x24x16 vs CPU", flush=True)
     feats, source, target, mask = _sample(seed=1)
