@@ -148,6 +148,69 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=20.0, parallel=False):
     }
 
 
+def step_learned_args(p, d_phi, d_rhs, d_solid, bgs, axes, d_faces, dev, outs):
+    return dict(p=p, d_phi=d_phi, d_rhs=d_rhs, d_solid=d_solid, bgs=bgs, axes=axes,
+                d_faces=d_faces, dev=dev, outs=outs)
+
+
+def learned_summary(p, args, a, fcg_mg_ms, pcg_ms):
+    """The headline step (3 directions: compatibility, solve to the tolerance,
+    gauge, densities) with the multigrid CG whose coarse correction on level 2
+    (128x96x64 here) includes the trained U-Net (flowfpmpp_amd/data/
+    unet_weights.npz): "augment" adds the network's A-optimal correction to
+    the recursive one, "replace" lets the network stand in for everything
+    below the level. Features on the level are built on the host first
+    (reported apart); the network's activations are allocated by a warm-up
+    solve."""
+    from flowfpmpp_amd import learned as L
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    net = L.UNet.load()
+    level = L.learned_level(p.nx, p.ny, p.nz)
+    t0 = time.perf_counter()
+    feats = [DeviceArray.from_host(L.level_features(p, level, d)) for d in range(3)]
+    feat_ms = (time.perf_counter() - t0) * 1e3
+    fields = [a["dev"][k] for k in ("u", "v", "w", "ox", "oy", "oz", "ax", "ay", "az")]
+
+    def step(mode):
+        its, res, fbs = [], [], []
+        for d in range(3):
+            a["d_phi"][d].zero_()
+            for f in a["d_faces"][d]:
+                f.zero_()
+            S.enforce_stairstep_compatibility_3d(a["d_rhs"], a["d_solid"], a["bgs"][d],
+                                                 *a["d_faces"][d], *a["axes"][3:])
+            sw, r, fb = L.solve_fcg_mg_learned(a["d_phi"][d], a["d_rhs"], a["d_solid"],
+                                               a["bgs"][d], *a["d_faces"][d], *a["axes"],
+                                               args.tol, args.max_iters, args.check, net,
+                                               feats[d], level, mode)
+            its.append(sw)
+            res.append(r)
+            fbs.append(fb)
+        S.compute_stairstep_fpm_terms_3d_dirs(a["d_phi"], *fields, a["d_solid"], a["bgs"],
+                                              *a["axes"], a["d_faces"], p.bounds, a["outs"])
+        return its, res, fbs
+
+    out = {"network": "U-Net 16/32/64, BF16 MFMA, trained on 2001 fields at 128^3 "
+                      "(flowfpmpp_amd/data/unet_weights.npz)",
+           "level": level, "level_grid": list(L.mg_levels(p.nx, p.ny, p.nz)[level]),
+           "features_host_ms": round(feat_ms, 1), "fcg_mg_ms_per_step": fcg_mg_ms}
+    for mode in ("augment", "replace"):
+        if mode == "augment":
+            step(mode)  # warm-up: the network's buffers at this level (shared by both modes)
+        synchronize()
+        t1 = time.perf_counter()
+        its, res, fbs = step(mode)
+        synchronize()
+        ms = (time.perf_counter() - t1) * 1e3
+        log(f"learned coarse correction ({mode}): {ms:.1f} ms, iterations {its}, fallback {fbs}")
+        out[mode] = {"ms_per_step": round(ms, 3), "iterations": its, "residuals": res,
+                     "fallback_at": fbs, "converged": all(r <= args.tol for r in res),
+                     "speedup_vs_fp64_pcg": round(pcg_ms / ms, 2) if pcg_ms else None,
+                     "time_vs_fcg_mg": round(ms / fcg_mg_ms, 3)}
+    return out
+
+
 def batch64_frames():
     """The 64 uneven frames x 3 directions of the batched workload, resident on
     the device with the compatibility correction applied; returns
@@ -531,6 +594,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="frame workload: skip the batch64 object (192 batched solves)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="frame workload: skip the learned_fcg object (the multigrid CG "
+                         "with the trained network in its coarse correction)")
     ap.add_argument("--workload", default="frame",
                     choices=["frame", "batch64", "reference", "jacobi", "unet"],
                     help="frame: the 3-direction solve of one frame (the headline); batch64: "
@@ -689,6 +755,11 @@ def main():
                            "residuals": pcg_res, "check_interval": args.check,
                            "speedup": round(pcg_ms / ms_per_step, 2),
                            "max_rel_diff_of_force_totals": dtot}
+    if not args.no_learned and tuple(args.grid) == (512, 384, 256):
+        log("multigrid CG with the learned coarse correction on the same frame")
+        out["learned_fcg"] = learned_summary(p, args, step_learned_args(
+            p, d_phi, d_rhs, d_solid, bgs, axes, d_faces, dev, outs), ms_per_step,
+            out.get("fp64_pcg", {}).get("ms_per_step"))
     if not args.no_batch and tuple(args.grid) == (512, 384, 256):
         log("batched workload: 64 uneven frames x 3 directions")
         out["batch64"] = batch64_summary(args.tol, args.max_iters, args.check)

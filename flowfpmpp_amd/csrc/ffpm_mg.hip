@@ -7,6 +7,7 @@
 #include "ffpm_mg.hpp"
 #include "ffpm_mg_body.hpp"
 #include "ffpm_runtime.hpp"
+#include "ffpm_unet.hpp"
 
 namespace ffpm {
 
@@ -121,6 +122,114 @@ __global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<double> g,
   __shared__ double lds[16 * kReduceSlots];
   pcg_dir_body<double, CL, MODE, float>(t, blockIdx.x, gridDim.x, g, z, code, pin, pout,
                                         partials, sc, lds);
+}
+
+// ---- the learned coarse correction -----------------------------------------------
+// Cell volumes of level L: a level-L cell aggregates level-0 cells
+// [I 2^L, (I + 1) 2^L) per axis (clipped to the grid: ceil halving).
+__global__ __launch_bounds__(256) void k_lrn_volume(int L, int nx, int ny, int nz,
+                                                    const double *__restrict__ dx,
+                                                    const double *__restrict__ dy,
+                                                    const double *__restrict__ dz, int cnx,
+                                                    int cny, float *__restrict__ vol) {
+  const Idx t = cell_idx(cnx, cny, blk3());
+  if (!t.in) return;
+  const int s = 1 << L;
+  auto width = [&](const double *d, int n, int I) {
+    double w = 0.0;
+    for (int i = I * s; i < (I + 1) * s && i < n; ++i) w += d[i];
+    return w;
+  };
+  vol[t.c] = (float)(width(dx, nx, t.i) * width(dy, ny, t.j) * width(dz, nz, t.k));
+}
+
+// res = b - A x where the level carries an unknown (else 0), and one partial
+// max |res| / V per workgroup.
+__global__ __launch_bounds__(256) void k_lrn_residual(int nx, int ny, int nz,
+                                                      const float *__restrict__ gE,
+                                                      const float *__restrict__ gN,
+                                                      const float *__restrict__ gT,
+                                                      const float *__restrict__ b,
+                                                      const float *__restrict__ x,
+                                                      const float *__restrict__ vol,
+                                                      float *__restrict__ res,
+                                                      double *__restrict__ part) {
+  __shared__ double lds[16];
+  const Idx t = cell_idx(nx, ny, blk3());
+  double mx = 0.0;
+  if (t.in) {
+    const Coarse o = coarse_row(nx, ny, nz, gE, gN, gT, x, t.c, t.i, t.j, t.k);
+    const float r = o.d > 0.f ? b[t.c] - (o.d * x[t.c] - o.nb) : 0.f;
+    res[t.c] = r;
+    mx = fabs((double)r) / (double)vol[t.c];
+  }
+  block_max(mx, lds);
+  if (threadIdx.x == 0) part[blockIdx.x + (size_t)gridDim.x * blockIdx.z] = mx;
+}
+
+// sc[0] = sigma = the largest partial; sc[1] = alpha from the c.res and c.A c
+// partials (slots 1, 2): c.res / c.A c, 0 unless c.A c > 0.
+__global__ __launch_bounds__(256) void k_lrn_finish(int what, const double *__restrict__ part,
+                                                    int blocks, double *__restrict__ sc) {
+  __shared__ double lds[32];
+  if (what == 0) {
+    const double m = grid_finish_max(part, 0, blocks, lds);
+    if (threadIdx.x == 0) sc[0] = m;
+  } else {
+    double v[2];
+    grid_finish_sum<2>(part + blocks, blocks, v, lds);
+    if (threadIdx.x == 0) sc[1] = v[1] > 0.0 && v[0] == v[0] ? v[0] / v[1] : 0.0;
+  }
+}
+
+// The network's source density s = res / (V sigma) (0 when sigma is 0).
+__global__ __launch_bounds__(256) void k_lrn_source(size_t n, const float *__restrict__ res,
+                                                    const float *__restrict__ vol,
+                                                    const double *__restrict__ sc,
+                                                    float *__restrict__ src) {
+  const double sig = sc[0];
+  const float inv = sig > 0.0 ? (float)(1.0 / sig) : 0.f;
+  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * blockDim.x)
+    src[c] = res[c] / vol[c] * inv;
+}
+
+// q = A c where the level carries an unknown; partials of c.res and c.q.
+__global__ __launch_bounds__(256) void k_lrn_dots(int nx, int ny, int nz,
+                                                  const float *__restrict__ gE,
+                                                  const float *__restrict__ gN,
+                                                  const float *__restrict__ gT,
+                                                  const float *__restrict__ c,
+                                                  const float *__restrict__ res,
+                                                  double *__restrict__ part) {
+  __shared__ double lds[32];
+  const Idx t = cell_idx(nx, ny, blk3());
+  double v[2] = {0.0, 0.0};
+  if (t.in) {
+    const Coarse o = coarse_row(nx, ny, nz, gE, gN, gT, c, t.c, t.i, t.j, t.k);
+    if (o.d > 0.f) {
+      const float q = o.d * c[t.c] - o.nb;
+      v[0] = (double)c[t.c] * (double)res[t.c];
+      v[1] = (double)c[t.c] * (double)q;
+    }
+  }
+  block_sum<2>(v, lds);
+  const int blocks = gridDim.x * gridDim.z, b = blockIdx.x + gridDim.x * blockIdx.z;
+  if (threadIdx.x == 0) {
+    part[(size_t)blocks + b] = v[0];
+    part[(size_t)2 * blocks + b] = v[1];
+  }
+}
+
+// x += alpha c where the level carries an unknown.
+__global__ __launch_bounds__(256) void k_lrn_add(size_t n, const float *__restrict__ m,
+                                                 const float *__restrict__ c,
+                                                 const double *__restrict__ sc,
+                                                 float *__restrict__ x) {
+  const float a = (float)sc[1];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    if (m[i] > 0.f) x[i] += a * c[i];
 }
 
 // ---- setup ----------------------------------------------------------------------
@@ -361,6 +470,49 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   return mg_build(mg);
 }
 
+int mg_learned_setup(Multigrid &mg, UNet *net, const float *features, int level, int mode) {
+  if (mode != 0 && mode != 1) {
+    set_error(ERR_ARGUMENT, "learned correction mode must be 0 (replace) or 1 (augment)");
+    return ERR_ARGUMENT;
+  }
+  if (!net || !features) {
+    set_error(ERR_ARGUMENT, "learned coarse correction needs a network and its features");
+    return ERR_ARGUMENT;
+  }
+  if (level < 1 || level >= mg.levels - 1) {
+    set_error(ERR_ARGUMENT, "learned level must lie between level 1 and the coarsest level");
+    return ERR_ARGUMENT;
+  }
+  const MgLevel &v = mg.lv[level];
+  if (v.nx < 16 || v.nx % 4 || v.ny % 4 || v.nz % 4) {
+    set_error(ERR_SHAPE, "the learned level needs nx >= 16 and every extent a multiple of 4");
+    return ERR_SHAPE;
+  }
+  MgLearned &L = mg.lrn;
+  L.net = net;
+  L.features = features;
+  L.level = level;
+  L.mode = mode;
+  const dim3 grid = cell_grid(v.nx, v.ny, v.nz);
+  L.blocks = (int)(grid.x * grid.z);
+  const size_t fb = (v.n * sizeof(float) + 255) / 256 * 256;
+  const size_t pb = ((size_t)3 * L.blocks * sizeof(double) + 255) / 256 * 256;
+  int st = 0;
+  char *base = (char *)workspace().get(S_MG_LRN, 4 * fb + pb + 256, &st);
+  if (!base) return st;
+  L.res = (float *)base;
+  L.src = (float *)(base + fb);
+  L.q = (float *)(base + 2 * fb);
+  L.vol = (float *)(base + 3 * fb);
+  L.part = (double *)(base + 4 * fb);
+  L.sc = (double *)(base + 4 * fb + pb);
+  k_lrn_volume<<<grid, dim3(kMt), 0, stream()>>>(level, mg.g64.nx, mg.g64.ny, mg.g64.nz,
+                                                 mg.g64.dx, mg.g64.dy, mg.g64.dz, v.nx, v.ny,
+                                                 L.vol);
+  FFPM_LAUNCHED("k_lrn_volume");
+  return OK;
+}
+
 namespace {
 
 // Level-0 streaming sweep launch (one step of a schedule) for the
@@ -458,9 +610,34 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
         v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, Multigrid::bufc(v, st.a),
         st.p >= 0 ? Multigrid::bufc(v, st.p) : nullptr, Multigrid::bufc(v, st.o), st.c1, st.c2);
   };
+  // The learned correction either replaces everything below its level (the
+  // cycle stops descending there) or, in augment mode, adds its A-optimal
+  // correction of the level's remaining residual after the recursive coarse
+  // correction, before the level's post-smoothing.
+  const bool replace = mg.lrn.net && mg.lrn.mode == 0;
+  const bool augment = mg.lrn.net && mg.lrn.mode == 1;
+  const int Ln = replace ? mg.lrn.level : L;
+  auto learned_correct = [&](const MgLevel &v, float *xm) -> int {
+    const MgLearned &N = mg.lrn;
+    const dim3 grid = cell_grid(v.nx, v.ny, v.nz);
+    k_lrn_residual<<<grid, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, xm,
+                                               N.vol, N.res, N.part);
+    k_lrn_finish<<<1, 256, 0, stream()>>>(0, N.part, N.blocks, N.sc);
+    const int eb = elementwise_blocks(v.n);
+    k_lrn_source<<<eb, 256, 0, stream()>>>(v.n, N.res, N.vol, N.sc, N.src);
+    FFPM_LAUNCHED("k_lrn_source");
+    const float *c = nullptr;
+    int st = unet_apply(*N.net, v.nx, v.ny, v.nz, N.features, N.src, &c);
+    if (st) return st;
+    k_lrn_dots<<<grid, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, c, N.res, N.part);
+    k_lrn_finish<<<1, 256, 0, stream()>>>(1, N.part, N.blocks, N.sc);
+    k_lrn_add<<<eb, 256, 0, stream()>>>(v.n, v.m, c, N.sc, xm);
+    FFPM_LAUNCHED("k_lrn_add");
+    return OK;
+  };
   // down: pre-smoothing (after the first step, formed by the restriction
   // into the level) and restriction
-  for (int l = 1; l < L; ++l) {
+  for (int l = 1; l < Ln; ++l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
     for (int k = 0; k < sc.npre; ++k) csweep(v, sc.pre[k]);
     k_mgc_restrict<<<cell_grid(c.nx, c.ny, c.nz), blk, 0, stream()>>>(
@@ -468,7 +645,18 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
         coarse_out(mg, l + 1));
     FFPM_LAUNCHED("k_mgc_restrict");
   }
-  {
+  if (replace) {
+    // the learned level: pre-smoothing, the network's correction of the
+    // smoothed iterate, post-smoothing
+    const MgLevel &v = mg.lv[Ln];
+    TimedLaunch tl("mg_learned");
+    for (int k = 0; k < sc.npre; ++k) csweep(v, sc.pre[k]);
+    float *xm = Multigrid::bufc(v, sc.mid);
+    int st = learned_correct(v, xm);
+    if (st) return st;
+    for (int k = 0; k < sc.npost; ++k) csweep(v, sc.post[k]);
+    FFPM_LAUNCHED("k_mgc_sweep");
+  } else {
     // the coarsest level leaves its solution where the other coarse levels
     // leave theirs
     const MgLevel &c = mg.lv[L];
@@ -478,11 +666,16 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     FFPM_LAUNCHED("k_mgc_coarsest");
   }
   // up: prolongation into the level's iterate, then post-smoothing
-  for (int l = L - 1; l >= 1; --l) {
+  for (int l = Ln - 1; l >= 1; --l) {
     const MgLevel &v = mg.lv[l], &c = mg.lv[l + 1];
     k_mgc_prolong<<<cell_grid(v.nx, v.ny, v.nz), blk, 0, stream()>>>(
         v.nx, v.ny, v.m, Multigrid::bufc(v, sc.mid), c.nx, c.ny, c.nz,
         Multigrid::bufc(c, sc.out), c.m);
+    if (augment && l == mg.lrn.level) {
+      TimedLaunch tl("mg_learned");
+      int st = learned_correct(v, Multigrid::bufc(v, sc.mid));
+      if (st) return st;
+    }
     for (int k = 0; k < sc.npost; ++k) csweep(v, sc.post[k]);
     FFPM_LAUNCHED("k_mgc_sweep");
   }

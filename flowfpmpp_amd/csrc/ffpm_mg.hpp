@@ -60,6 +60,28 @@ struct MgSchedule {
   MgStep pre[8], post[8];
 };
 
+struct UNet;
+
+// The learned coarse correction (no counterpart in the reference) on level
+// `level` of the cycle: the level's residual res = b - A x is scaled to a
+// source density s = res / (V sigma), sigma = max |res / V|; the network maps
+// s (with the level's geometry features) to a correction c, whose length is
+// the A-optimal one, alpha = c.res / c.A c; x += alpha c, then the level's
+// usual post-smoothing. Mode 0 (replace): the network takes the place of
+// everything coarser, applied after the level's pre-smoothing. Mode 1
+// (augment): the recursion runs as usual and the network corrects what is
+// left after the prolongation from below. Buffers are the level's cells.
+struct MgLearned {
+  UNet *net = nullptr;               // null: the plain cycle
+  const float *features = nullptr;   // 8 FP32 per level cell, channels-last
+  int level = 0;
+  int mode = 0;
+  float *res = nullptr, *src = nullptr, *q = nullptr, *vol = nullptr;
+  double *part = nullptr;            // per-workgroup partials, 3 slots of `blocks`
+  double *sc = nullptr;              // sigma, alpha
+  int blocks = 0;                    // workgroups of the level's cell launch
+};
+
 struct Multigrid {
   int levels = 0;
   MgLevel lv[kMgMaxLevels];
@@ -79,6 +101,7 @@ struct Multigrid {
   float first = 0.9f;        // scale of the first step from zero, x = first b / d
   float first_coarse = 0.9f; // the same on the coarse levels
   MgSchedule s0, sc;         // level 0 (its last step writes z32), coarse levels
+  MgLearned lrn;               // the learned coarse correction, when net is set
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
   float *s32 = nullptr;        // the cycle's source r / V (level-0 cells)
@@ -111,6 +134,12 @@ size_t mg_state_bytes(const Multigrid &mg);
 void mg_bind_operator(Multigrid &mg, char *&base);
 void mg_bind_state(Multigrid &mg, char *&base);
 int mg_build(const Multigrid &mg);
+
+// Puts the network on `features` at `level` (1 <= level < levels - 1, whose
+// extents are multiples of 4 with nx >= 16) in mode 0 (replace) or 1
+// (augment); allocates the level's work buffers (workspace slot S_MG_LRN) and
+// computes its cell volumes.
+int mg_learned_setup(Multigrid &mg, UNet *net, const float *features, int level, int mode);
 
 // Rows per workgroup of the level-0 FP32 kernels (the cycle's sweeps):
 // FFPM_MG_ROWS (1, 2, 4, 8, 16), default 4, measured faster than the 8 rows

@@ -86,8 +86,8 @@ enum Slot : int {
   S_FCG_SRC = 91, S_FCG_U, S_FCG_AU, S_FCG_AN, S_FCG_N, S_FCG_RING,
   // 2-D force partition
   S_P2_AXES = 100, S_P2_WORK, S_P2_PART, S_P2_SC, S_P2_HOST, S_P2_SOLID,
-  // multigrid hierarchy of the flexible CG (one arena)
-  S_MG = 110,
+  // multigrid hierarchy of the flexible CG (one arena), the learned level's buffers
+  S_MG = 110, S_MG_LRN,
   // densities of several directions in one pass: partials, totals
   S_TD_PART = 120, S_TD_OUT,
 };

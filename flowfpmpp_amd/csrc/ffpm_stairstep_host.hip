@@ -831,8 +831,18 @@ thread_local MgStats g_mg;
 // recursive residual is near the tolerance, see below); a breakdown restarts
 // once from p = z, and ten checks without a 1% gain stop at the roundoff
 // floor.
+// The learned coarse correction of the multigrid CG (no reference
+// counterpart): the network, its features on the level and the level.
+struct MgLearnedArgs {
+  UNet *net;
+  const float *features;
+  int level;
+  int mode;          // 0 replace the recursion below the level, 1 augment it
+  int *fallback_at;  // iterations before the switch to diagonal PCG, or -1
+};
+
 int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interval, int *sweeps,
-                   double *residual) {
+                   double *residual, const MgLearnedArgs *lrn = nullptr) {
   int st = 0;
   GeomT<float> g32;
   if ((st = make_geom32(f, &g32))) return st;
@@ -841,6 +851,12 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   Multigrid mg;
   const Tiling tm = make_tiling(f.nx, f.ny, f.nz, 0, mg_rows(), kTileX * cl32);
   if ((st = mg_setup(mg, g, g32, f.code, tm, cl32, f.partials))) return st;
+  if (lrn && (st = mg_learned_setup(mg, lrn->net, lrn->features, lrn->level, lrn->mode)))
+    return st;
+  if (lrn) *lrn->fallback_at = -1;
+  // the learned cycle is not a fixed linear map; three convergence checks
+  // without a 1% gain hand the solve to diagonal PCG
+  const int stall_limit = lrn ? 3 : 10;
   g_mg = MgStats();
   g_mg.levels = mg.levels;
   const int ci = mg_check_interval(check_interval);
@@ -918,8 +934,18 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       } else {
         stalled = stalled + 1;
       }
-      if (stalled >= 10 || !(res == res)) break;
+      if (stalled >= stall_limit || !(res == res)) break;
     }
+  }
+  if (lrn && *sweeps < max_sweeps && res > tolerance) {
+    // fall back to diagonal PCG from the current phi (its first step forms
+    // the true residual afresh)
+    *lrn->fallback_at = *sweeps;
+    int more = 0;
+    if ((st = pcg_iterate_fused(f, tolerance, max_sweeps - *sweeps, check_interval, &more,
+                                &res)))
+      return st;
+    *sweeps += more;
   }
   *residual = res;
   return OK;
@@ -933,7 +959,8 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
                         const double *dx, const double *dy, const double *dz, int nx, int ny,
                         int nz, int west, int east, int south, int north, double tolerance,
                         int max_sweeps, int check_interval, int method, int *sweeps,
-                        double *residual, const FcgArgs *fcg = nullptr) {
+                        double *residual, const FcgArgs *fcg = nullptr,
+                        const MgLearnedArgs *lrn = nullptr) {
   if (method != FFPM_METHOD_PCG && method != FFPM_METHOD_JACOBI &&
       method != FFPM_METHOD_PCG_MIXED && method != FFPM_METHOD_FCG_MG &&
       !(method == FFPM_METHOD_FCG && fcg)) {
@@ -980,7 +1007,7 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   if (method == FFPM_METHOD_FCG) {
     st = fcg_iterate(f, *fcg, tolerance, max_sweeps, check_interval, sweeps, residual);
   } else if (method == FFPM_METHOD_FCG_MG) {
-    st = fcg_mg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual);
+    st = fcg_mg_iterate(f, tolerance, max_sweeps, check_interval, sweeps, residual, lrn);
   } else if (method == FFPM_METHOD_PCG_MIXED) {
     st = pcg_mixed(f, tolerance, max_sweeps, check_interval, mixed_inner_rtol(), sweeps,
                    residual);
@@ -1563,6 +1590,25 @@ int ffpm_solve_stairstep_fcg_dev(
   int st = solve_stairstep_dev(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc,
                                zc, dx, dy, dz, nx, ny, nz, -1, -1, -1, -1, tolerance, max_sweeps,
                                check_interval, FFPM_METHOD_FCG, sweeps, residual, &a);
+  if (st) return st;
+  return sync();
+}
+
+int ffpm_solve_stairstep_fcg_mg_learned_dev(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+    double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, int level, int mode,
+    const float *features, int *sweeps, double *residual, int *fallback_at) {
+  if (!net || !features || !fallback_at) {
+    set_error(ERR_ARGUMENT, "learned multigrid CG needs a network, its features and an output");
+    return ERR_ARGUMENT;
+  }
+  MgLearnedArgs a{reinterpret_cast<UNet *>(net), features, level, mode, fallback_at};
+  int st = solve_stairstep_dev(phi, rhs, solid, body_grad, xlo, xhi, ylo, yhi, zlo, zhi, xc, yc,
+                               zc, dx, dy, dz, nx, ny, nz, -1, -1, -1, -1, tolerance, max_sweeps,
+                               check_interval, FFPM_METHOD_FCG_MG, sweeps, residual, nullptr, &a);
   if (st) return st;
   return sync();
 }

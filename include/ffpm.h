@@ -250,6 +250,27 @@ int ffpm_solve_stairstep_fcg_dev(
     double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, const float *features,
     int *sweeps, double *residual, int *fallback_at);
 
+/* The multigrid-preconditioned CG (FFPM_METHOD_FCG_MG) with a learned coarse
+ * correction (no counterpart in the reference) on multigrid level `level`
+ * (1 <= level < the coarsest; extents multiples of 4, nx >= 16): the network
+ * maps the level's scaled residual density to a correction, added with the
+ * A-optimal length before the level's post-smoothing. mode 0: it replaces
+ * the recursive coarse solve below the level (applied after the level's
+ * pre-smoothing); mode 1: the recursion runs as usual and the network
+ * corrects the residual left after the prolongation from below. features: device
+ * FP32, 8 per cell of that level (channels-last; fluid fraction, signed
+ * distance, three cell widths over 8/128, the direction one-hot). Stops on
+ * the reference's FP64 max-norm residual; after three convergence checks
+ * without a 1% gain it falls back to diagonal PCG from the current phi
+ * (*fallback_at: iterations before the switch, -1 if none). */
+int ffpm_solve_stairstep_fcg_mg_learned_dev(
+    double *phi, const double *rhs, const int32_t *solid, const double *body_grad,
+    const double *xlo, const double *xhi, const double *ylo, const double *yhi,
+    const double *zlo, const double *zhi, const double *xc, const double *yc, const double *zc,
+    const double *dx, const double *dy, const double *dz, int nx, int ny, int nz,
+    double tolerance, int max_sweeps, int check_interval, ffpm_unet *net, int level, int mode,
+    const float *features, int *sweeps, double *residual, int *fallback_at);
+
 /* fpm_partition_terms_2d, src/fpm_partition_2d.f90:35-154: force partition of
  * a 2-D stair-step body with rigid in-plane motion, per unit span and density.
  * phi, u, v (nx,ny); solid (nx,ny) int32, nonzero = solid (the reference's
