@@ -69,15 +69,35 @@ BYTES_PER_CELL = {
 }
 
 
+# The sources of the measured kernels' device code: the committed traffic
+# figures hold only while these are the sources that were measured.
+KERNEL_SOURCES = ("ffpm_common.hpp", "ffpm_kernels.hpp", "ffpm_pcg.hpp", "ffpm_pcg_body.hpp",
+                  "ffpm_bodies.hpp", "ffpm_frame_body.hpp", "ffpm_mg.hpp", "ffpm_mg_body.hpp",
+                  "ffpm_mg.hip", "ffpm_pcg_kernels.hip", "ffpm_stairstep_kernels.hip")
+
+
+def kernel_sources_sha256():
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(HERE, "flowfpmpp_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
 def measured_traffic(kernel, grid):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
     / WRITE_SIZE measurement of this workload (profiles/hbm_traffic.json), or
-    None when the kernel or grid was not measured."""
+    None when the kernel or grid was not measured or the kernel sources have
+    changed since the measurement (their SHA-256 is stored with it)."""
     path = os.path.join(HERE, "profiles", "hbm_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
+        return None
+    if t.get("kernel_sources_sha256") != kernel_sources_sha256():
+        log("profiles/hbm_traffic.json was measured on other kernel sources: traffic null")
         return None
     k = t.get("kernels", {}).get(kernel)
     return k["traffic_bytes"] if k and t.get("grid") == grid else None
