@@ -168,6 +168,58 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=20.0, parallel=False):
     }
 
 
+def plain_stencil(p, args, pcg_kern, d_rhs, d_solid, axes, faces, phi, bg, sweeps=200):
+    """The plain FP64 stencil kernels on this frame against the HBM peak: the
+    fused PCG iteration's two kernels from the FP64 PCG step just run, and
+    the weighted-Jacobi sweep timed here over `sweeps` fixed sweeps of the x
+    direction (HIP events on the library's stream around every launch).
+    GB/s are given by the kernels' own algorithmic bytes and by the
+    reference's (its three-kernel PCG iteration: 112 B/cell; its Jacobi
+    sweep: 32 B/cell), with the rocprofv3-measured traffic where the
+    committed measurement covers the current sources."""
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd.device import synchronize
+    cells = p.cells
+    faces_now = faces
+    phi.zero_()
+    for f in faces_now:
+        f.zero_()
+    S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, bg, *faces_now, *axes[3:])
+    synchronize()
+    S.timing(True)
+    S.solve_stairstep_poisson_3d(phi, d_rhs, d_solid, bg, *faces_now, *axes, -1.0, sweeps,
+                                 sweeps, method="jacobi")
+    synchronize()
+    S.timing(False)
+    kern = dict(pcg_kern)
+    kern["jacobi"] = S.timing_query("jacobi")
+    out = {}
+    for name, (cnt, ms) in kern.items():
+        if not cnt:
+            continue
+        avg_us = ms * 1e3 / cnt
+        # this Jacobi kernel reads the 1-byte neighbour code instead of rdiag
+        own = 25 if name == "jacobi" else BYTES_PER_CELL[name]
+        e = {"launches": cnt, "avg_us": round(avg_us, 2), "bytes_per_cell": own,
+             "gbps": round(own * cells / avg_us / 1e3, 1),
+             "frac": round(own * cells / avg_us / 1e3 / HBM_PEAK_GBS, 4)}
+        traffic = measured_traffic(name, list(args.grid))
+        if traffic:
+            e["traffic_gbps"] = round(traffic / avg_us / 1e3, 1)
+        out[name] = e
+    if "jacobi" in out:
+        out["jacobi"]["reference_bytes_per_cell"] = 32
+        out["jacobi"]["reference_frac"] = round(
+            32 * cells / out["jacobi"]["avg_us"] / 1e3 / HBM_PEAK_GBS, 4)
+    if "pcg_dir_spmv" in out and "pcg_update" in out:
+        it_us = out["pcg_dir_spmv"]["avg_us"] + out["pcg_update"]["avg_us"]
+        out["pcg_iteration"] = {"us": round(it_us, 2), "reference_bytes_per_cell": 112,
+                                "reference_frac": round(112 * cells / it_us / 1e3 / HBM_PEAK_GBS,
+                                                        4)}
+    log(f"plain stencil: {json.dumps(out)}")
+    return out
+
+
 def step_learned_args(p, d_phi, d_rhs, d_solid, bgs, axes, d_faces, dev, outs):
     return dict(p=p, d_phi=d_phi, d_rhs=d_rhs, d_solid=d_solid, bgs=bgs, axes=axes,
                 d_faces=d_faces, dev=dev, outs=outs)
@@ -765,16 +817,21 @@ def main():
     if args.method != "pcg" and not args.no_pcg:
         log("FP64 diagonal PCG on the same frame (one step)")
         synchronize()
+        S.timing(True)
         t1 = time.perf_counter()
         pcg_iters, pcg_res, pcg_totals = step("pcg")
         synchronize()
         pcg_ms = (time.perf_counter() - t1) * 1e3
+        S.timing(False)
+        pcg_kern = {n: S.timing_query(n) for n in ("pcg_dir_spmv", "pcg_update")}
         dtot = max(abs(a - b) / max(abs(b), 1e-300) for ta, tb in zip(totals, pcg_totals)
                    for a, b in zip(ta, tb))
         out["fp64_pcg"] = {"ms_per_step": round(pcg_ms, 3), "iterations": pcg_iters,
                            "residuals": pcg_res, "check_interval": args.check,
                            "speedup": round(pcg_ms / ms_per_step, 2),
                            "max_rel_diff_of_force_totals": dtot}
+        out["plain_stencil"] = plain_stencil(p, args, pcg_kern, d_rhs, d_solid, axes,
+                                             d_faces[0], d_phi[0], bgs[0])
     if not args.no_learned and tuple(args.grid) == (512, 384, 256):
         log("multigrid CG with the learned coarse correction on the same frame")
         out["learned_fcg"] = learned_summary(p, args, step_learned_args(
