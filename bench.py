@@ -508,6 +508,104 @@ def bench_unet(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_laplace(args):
+    """The no-body Neumann solvers of fpm_laplace_3d on --grid (512x384x256 by
+    default): the uniform solver (undamped Jacobi with wall ghosts) and the
+    non-uniform one (0.2/0.8-damped Jacobi on the frame's stretched axes),
+    each for a fixed --steps x 100 sweeps from zero on a random source and
+    random face data (compatibility enforced), the max-norm residual checked
+    every --check sweeps. Per-sweep times come from HIP events on the
+    library's stream, for the streaming sweeps and for the one-cell-per-thread
+    sweeps (FFPM_LAP=classic), whose gauged fields must be bit-identical."""
+    from flowfpmpp_amd import laplace as LP
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    T = DeviceArray.from_host
+    from flowfpmpp_amd.grids import tanh_axis
+    nx, ny, nz = args.grid
+    cells = nx * ny * nz
+    # the stretched axes of the influence-potential frames (width ratio 4)
+    ax = [tanh_axis(n, -e, e, 4.0) for n, e in ((nx, 4.0), (ny, 3.0), (nz, 2.0))]
+    rng = np.random.Generator(np.random.PCG64(7))
+    rhs = np.asfortranarray(rng.standard_normal((nx, ny, nz)))
+    faces = [np.asfortranarray(rng.standard_normal(s)) for s in
+             ((ny, nz), (ny, nz), (nx, nz), (nx, nz), (nx, ny), (nx, ny))]
+    centres = [np.ascontiguousarray(a[1], dtype=np.float64) for a in ax]
+    widths = [np.ascontiguousarray(a[2], dtype=np.float64) for a in ax]
+    h = (float(widths[0].mean()), float(widths[1].mean()), float(widths[2].mean()))
+    sweeps = 100 * max(1, args.steps)
+    check = max(1, min(args.check, sweeps))
+    d = {}
+    for kind in ("uniform", "nonuniform"):
+        r, fs = T(rhs), [T(f) for f in faces]
+        if kind == "uniform":
+            LP.enforce_neumann_compatibility_3d(r, *fs, nx, ny, nz, *h)
+        else:
+            LP.enforce_neumann_compatibility_nonuniform_3d(r, *fs, *[T(w) for w in widths],
+                                                           nx, ny, nz)
+        d[kind] = (r, fs)
+    axes = [T(a) for a in centres] + [T(w) for w in widths]
+    res = {}
+    prev = os.environ.get("FFPM_LAP")
+    try:
+        for mode in ("streaming", "classic"):
+            if mode == "classic":
+                os.environ["FFPM_LAP"] = "classic"
+            else:
+                os.environ.pop("FFPM_LAP", None)
+            for kind in ("uniform", "nonuniform"):
+                r, fs = d[kind]
+                shape = (nx + 2, ny + 2, nz + 2) if kind == "uniform" else (nx, ny, nz)
+                phi = DeviceArray(shape)
+                phi.zero_()
+                synchronize()
+                S.timing(True)
+                t0 = time.perf_counter()
+                if kind == "uniform":
+                    it, rr = LP.solve_neumann_poisson_3d(phi, r, *fs, nx, ny, nz, *h, -1.0,
+                                                         sweeps, check)
+                else:
+                    it, rr = LP.solve_neumann_poisson_nonuniform_3d(phi, r, *fs, *axes, nx, ny,
+                                                                    nz, -1.0, sweeps, check)
+                synchronize()
+                wall = time.perf_counter() - t0
+                cnt, ms = S.timing_query("lap_sweep")
+                S.timing(False)
+                res[(mode, kind)] = (phi.download(), it, rr, ms / cnt, wall)
+                log(f"laplace {kind} ({mode} sweeps): {it} sweeps, {ms / cnt * 1e3:.1f} us per "
+                    f"sweep, {wall * 1e3:.1f} ms for the solve, residual {rr:.3e}")
+    finally:
+        if prev is None:
+            os.environ.pop("FFPM_LAP", None)
+        else:
+            os.environ["FFPM_LAP"] = prev
+    per = {}
+    for kind in ("uniform", "nonuniform"):
+        a, b = res[("streaming", kind)], res[("classic", kind)]
+        same = bool(np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2])
+        if not same:
+            raise SystemExit(f"laplace {kind}: streaming and classic sweeps differ")
+        us = a[3] * 1e3
+        per[kind] = {"us_per_sweep": round(us, 1), "classic_us_per_sweep": round(b[3] * 1e3, 1),
+                     "solve_ms": round(a[4] * 1e3, 1), "classic_solve_ms": round(b[4] * 1e3, 1),
+                     "gbs": round(24 * cells / (us * 1e-6) / 1e9, 0),
+                     "residual": a[2], "bit_identical_to_classic": same}
+    worst = max(per[k]["us_per_sweep"] for k in per)
+    gbs = 24 * cells / (worst * 1e-6) / 1e9
+    out = {"metric": "no-body Jacobi sweep time (us, the slower of the two solvers)",
+           "value": worst, "unit": "us", "n_gpus": 1, "steps": sweeps, "warmup": 0,
+           "ms_per_step": round(worst / 1e3, 4), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"fpm_laplace_3d no-body solvers on {nx}x{ny}x{nz} "
+                                  f"(uniform spacing {h[0]:.4g}/{h[1]:.4g}/{h[2]:.4g}; tanh-stretched "
+                                  f"axes, width ratio 4), {sweeps} sweeps from zero, "
+                                  f"residual every {check}", "grid": list(args.grid)},
+           "solvers": per,
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 0), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(gbs / 8000.0, 4), "traffic": None}}
+    print(json.dumps(out), flush=True)
+
+
 # The reference's published A100 times (seconds) for the same frames:
 # benchmarks/pcg_time_to_solution.csv:5-9 and benchmarks/stairstep_benchmark.csv:9-12.
 A100_PCG_1E8 = {(128, 128, 32): (1000, 0.1759), (256, 256, 64): (1900, 1.249),
@@ -670,14 +768,16 @@ def main():
                     help="frame workload: skip the learned_fcg object (the multigrid CG "
                          "with the trained network in its coarse correction)")
     ap.add_argument("--workload", default="frame",
-                    choices=["frame", "batch64", "reference", "jacobi", "unet"],
+                    choices=["frame", "batch64", "reference", "jacobi", "unet", "laplace"],
                     help="frame: the 3-direction solve of one frame (the headline); batch64: "
                          "64 uneven frames x 3 directions in one batched solve; reference: "
                          "the reference's own benchmark frames, timed as it times them; "
                          "jacobi: weighted Jacobi on the frame's x direction, capped at "
                          "--max-iters sweeps in chunks of --check, extrapolated to --tol; "
                          "unet: the learned preconditioner's network, forward passes on "
-                         "--grid (256x192x128 by default) and training steps at 128^3")
+                         "--grid (256x192x128 by default) and training steps at 128^3; "
+                         "laplace: the no-body Neumann solvers (uniform and non-uniform), "
+                         "--steps x 100 sweeps on --grid, streaming against classic sweeps")
     ap.add_argument("--sequential", action="store_true",
                     help="batch64: also time the 192 solves one after another")
     ap.add_argument("--shape", type=int, nargs=2, metavar=("ROWS", "ZCHUNK"),
@@ -701,6 +801,8 @@ def main():
         return bench_jacobi(args)
     if args.workload == "unet":
         return bench_unet(args)
+    if args.workload == "laplace":
+        return bench_laplace(args)
 
     t_gen = time.perf_counter()
     p = frame(tuple(args.grid))

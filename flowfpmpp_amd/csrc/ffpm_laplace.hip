@@ -7,8 +7,13 @@
 // reference's copy-back kernel becomes a pointer swap; values are identical)
 // and check the max-norm residual every check_interval sweeps.
 #include <cfloat>
+#include <cstdint>
+#include <cstdlib>
+#include <initializer_list>
+#include <string>
 
 #include "../../include/ffpm.h"
+#include "ffpm_pcg_body.hpp"
 #include "ffpm_runtime.hpp"
 
 namespace ffpm {
@@ -222,6 +227,226 @@ __global__ __launch_bounds__(256) void k_lap_nonuniform_sweep(NonUniform g, Face
   }
 }
 
+// FFPM_LAP=classic: the one-cell-per-thread sweeps (the same bits).
+bool lap_classic() {
+  const char *e = std::getenv("FFPM_LAP");
+  return e && std::string(e) == "classic";
+}
+
+// Both sweeps, and both max-norm residuals, in the streaming scheme of the
+// stair-step kernels: a workgroup owns a tile of 64*CL x rows cells (one
+// wavefront per x row, CL adjacent cells per lane in one 16-byte vector for
+// CL = 2) and a chunk of z planes, which it walks holding the planes below
+// and above in registers. The loads of plane k+1 (south and north rows, the
+// edge lanes' west and east cells, rhs, the face data of boundary rows) and
+// the own row of plane k+2 are issued before plane k is computed; west and
+// east neighbours inside the row come by wave shuffle. Expressions and their
+// order are those of k_lap_uniform_sweep / k_lap_uniform_residual
+// (NU = false) and k_lap_nonuniform_sweep / k_lap_nonuniform_residual
+// (NU = true), so every cell gets the same bits as there, and the residual
+// (a maximum, which no order changes) is the same. RES = false writes the
+// next iterate to b; RES = true writes this workgroup's largest |residual|
+// to partials[blockIdx.x] and nothing else.
+template <int CL>
+struct LapSet {
+  Vec<double, CL> a_next, s, n, r, yl, yh;
+  double w, e, xl, xh;
+};
+
+template <int CL, bool NU, bool RES>
+__global__ __launch_bounds__(512) void k_lap_stream_t(Tiling t, Uniform u, NonUniform g, Faces f,
+                                                      const double *__restrict__ a,
+                                                      double *__restrict__ b,
+                                                      const double *__restrict__ rhs,
+                                                      double *partials) {
+  const int nx = u.nx, ny = u.ny, nz = u.nz;
+  const Lane l = lane_of<CL>(t, nx, ny, nz, blockIdx.x);
+  double rmax = 0.0;
+  if (l.active) {
+    using V = Vec<double, CL>;
+    const int i = l.i, j = l.j, lane = (int)threadIdx.x;
+    const size_t nxy = (size_t)nx * ny;
+    const bool hs = j > 0, hn = j < ny - 1;
+    const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+    // The edge lanes of a row fetch the cell beyond it; the others re-read
+    // their own first cell (a cached load, unused).
+    const ptrdiff_t ow = lane == 0 && i > 0 ? -1 : 0;
+    const ptrdiff_t oe = lane == kTileX - 1 && i + CL < nx ? CL : 0;
+    // Whether this wavefront's row holds the x = 0 or x = nx-1 cell (uniform
+    // across the wavefront, so the face loads below are skipped as a whole).
+    const int row0 = i - lane * CL;
+    const bool fx0 = row0 == 0, fx1 = row0 + kTileX * CL >= nx;
+    const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+    const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+    auto at = [&](int k) { return base + nxy * (size_t)k; };
+    // Per-lane x and y data: the sweep's weights and the widths, or the
+    // residual's centre distances and the widths.
+    double cw[CL], ce[CL], rdx[CL];
+    double cs = 0.0, cn = 0.0, dyj = 0.0;
+    if constexpr (NU) {
+#pragma unroll
+      for (int m = 0; m < CL; ++m) {
+        const int ii = i + m;
+        if constexpr (RES) {
+          cw[m] = ii > 0 ? g.xc[ii] - g.xc[ii - 1] : 1.0;
+          ce[m] = ii < nx - 1 ? g.xc[ii + 1] - g.xc[ii] : 1.0;
+        } else {
+          cw[m] = g.aw[ii];
+          ce[m] = g.ae[ii];
+        }
+        rdx[m] = g.dx[ii];
+      }
+      if constexpr (RES) {
+        cs = hs ? g.yc[j] - g.yc[j - 1] : 1.0;
+        cn = hn ? g.yc[j + 1] - g.yc[j] : 1.0;
+      } else {
+        cs = g.as[j];
+        cn = g.an[j];
+      }
+      dyj = g.dy[j];
+    }
+    V zl, zh;
+    if (k0 == 0) zl = ldv<double, CL>(f.zlo, base);
+    if (k1 == nz) zh = ldv<double, CL>(f.zhi, base);
+    auto load = [&](LapSet<CL> &ld, int k, int knext) {
+      const size_t c = at(k);
+      ld.a_next = ldv<double, CL>(a, at(knext));
+      ld.s = ldv<double, CL>(a, c + os);
+      ld.n = ldv<double, CL>(a, c + on);
+      ld.w = a[c + ow];
+      ld.e = a[c + oe];
+      ld.r = ldv<double, CL>(rhs, c);
+      if (fx0) ld.xl = f.xlo[j + (size_t)ny * k];
+      if (fx1) ld.xh = f.xhi[j + (size_t)ny * k];
+      if (j == 0) ld.yl = ldv<double, CL>(f.ylo, (size_t)i + (size_t)nx * k);
+      if (j == ny - 1) ld.yh = ldv<double, CL>(f.yhi, (size_t)i + (size_t)nx * k);
+    };
+    LapSet<CL> sa, sb;
+    V a_cur = ldv<double, CL>(a, at(k0));
+    V a_prev = k0 > 0 ? ldv<double, CL>(a, at(k0 - 1)) : a_cur;
+    load(sa, k0, min(k0 + 1, last));
+    auto stage = [&](int k, const LapSet<CL> &u_, LapSet<CL> &ld) {
+      const int kn = min(k + 1, k1 - 1);
+      load(ld, kn, min(k + 2, last));
+      const double from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
+      const double from_e = __shfl_down(a_cur.v[0], 1, kTileX);
+      const double vw = lane == 0 ? u_.w : from_w;
+      const double ve = lane == kTileX - 1 ? u_.e : from_e;
+      double cb = 0.0, ct = 0.0, dzk = 0.0;
+      if constexpr (NU) {
+        if constexpr (RES) {
+          cb = k > 0 ? g.zc[k] - g.zc[k - 1] : 1.0;
+          ct = k < nz - 1 ? g.zc[k + 1] - g.zc[k] : 1.0;
+        } else {
+          cb = g.ab[k];
+          ct = g.at[k];
+        }
+        dzk = g.dz[k];
+      }
+      V bv;
+#pragma unroll
+      for (int m = 0; m < CL; ++m) {
+        const int ii = i + m;
+        const double p = a_cur.v[m];
+        const double west = m == 0 ? vw : a_cur.v[m - 1];
+        const double east = m == CL - 1 ? ve : a_cur.v[m + 1];
+        const double below = a_prev.v[m], above = u_.a_next.v[m];
+        if constexpr (!NU) {
+          double xm = ii > 0 ? west : p, xp = ii < nx - 1 ? east : p;
+          double ym = hs ? u_.s.v[m] : p, yp = hn ? u_.n.v[m] : p;
+          double zm = k > 0 ? below : p, zp = k < nz - 1 ? above : p;
+          if (ii == 0) xm = p - u_.xl * u.dx;
+          if (ii == nx - 1) xp = p + u_.xh * u.dx;
+          if (j == 0) ym = p - u_.yl.v[m] * u.dy;
+          if (j == ny - 1) yp = p + u_.yh.v[m] * u.dy;
+          if (k == 0) zm = p - zl.v[m] * u.dz;
+          if (k == nz - 1) zp = p + zh.v[m] * u.dz;
+          if constexpr (RES) {
+            const double res = (xm - 2 * p + xp) / u.rdx2 + (ym - 2 * p + yp) / u.rdy2 +
+                               (zm - 2 * p + zp) / u.rdz2 - u_.r.v[m];
+            rmax = fmax(rmax, fabs(res));
+          } else {
+            bv.v[m] = ((xm + xp) * u.idx2 + (ym + yp) * u.idy2 + (zm + zp) * u.idz2 -
+                       u_.r.v[m]) *
+                      u.invdiag;
+          }
+        } else if constexpr (RES) {
+          const double g0 = ii == 0 ? u_.xl : (p - west) / cw[m];
+          const double g1 = ii == nx - 1 ? u_.xh : (east - p) / ce[m];
+          const double g2 = j == 0 ? u_.yl.v[m] : (p - u_.s.v[m]) / cs;
+          const double g3 = j == ny - 1 ? u_.yh.v[m] : (u_.n.v[m] - p) / cn;
+          const double g4 = k == 0 ? zl.v[m] : (p - below) / cb;
+          const double g5 = k == nz - 1 ? zh.v[m] : (above - p) / ct;
+          const double res = (g1 - g0) / rdx[m] + (g3 - g2) / dyj + (g5 - g4) / dzk - u_.r.v[m];
+          rmax = fmax(rmax, fabs(res));
+        } else {
+          const double diag = cw[m] + ce[m] + cs + cn + cb + ct;
+          double numer = -u_.r.v[m];
+          if (ii > 0) numer = numer + cw[m] * west;
+          if (ii < nx - 1) numer = numer + ce[m] * east;
+          if (hs) numer = numer + cs * u_.s.v[m];
+          if (hn) numer = numer + cn * u_.n.v[m];
+          if (k > 0) numer = numer + cb * below;
+          if (k < nz - 1) numer = numer + ct * above;
+          if (ii == 0) numer = numer - u_.xl / rdx[m];
+          if (ii == nx - 1) numer = numer + u_.xh / rdx[m];
+          if (j == 0) numer = numer - u_.yl.v[m] / dyj;
+          if (j == ny - 1) numer = numer + u_.yh.v[m] / dyj;
+          if (k == 0) numer = numer - zl.v[m] / dzk;
+          if (k == nz - 1) numer = numer + zh.v[m] / dzk;
+          bv.v[m] = 0.2 * p + 0.8 * numer / diag;
+        }
+      }
+      if constexpr (!RES) stv<double, CL>(b, at(k), bv);
+      a_prev = a_cur;
+      a_cur = u_.a_next;
+    };
+    for (int k = k0; k < k1; k += 2) {
+      stage(k, sa, sb);
+      if (k + 1 < k1) stage(k + 1, sb, sa);
+    }
+  }
+  if constexpr (RES) {
+    __shared__ double lds[16];
+    block_max(rmax, lds);
+    store_partials(partials, 1, &rmax, gridDim.x);
+  }
+}
+
+// Tiling and cells per lane of the streaming sweeps: CL = 2 where nx is even
+// and every array the kernel reads or writes in 16-byte vectors is 16-byte
+// aligned, else 1; tiles of 8 rows (FFPM_LAP_ROWS overrides: 1, 2, 4 or 8).
+int lap_cells_per_lane(int nx, std::initializer_list<const void *> ptrs) {
+  if (nx % 2) return 1;
+  for (const void *q : ptrs)
+    if (reinterpret_cast<uintptr_t>(q) % 16) return 1;
+  return 2;
+}
+
+Tiling lap_tiling(int nx, int ny, int nz, int cl) {
+  const char *e = std::getenv("FFPM_LAP_ROWS");
+  int rows = e ? std::atoi(e) : 8;
+  if (rows != 1 && rows != 2 && rows != 4 && rows != 8) rows = 8;
+  return make_tiling(nx, ny, nz, 0, rows, kTileX * cl);
+}
+
+// Workgroups of the larger of the two streaming launches (partials slots).
+size_t lap_max_blocks(int nx, int ny, int nz) {
+  const int b1 = lap_tiling(nx, ny, nz, 1).blocks, b2 = lap_tiling(nx, ny, nz, 2).blocks;
+  return (size_t)(b1 > b2 ? b1 : b2);
+}
+
+template <bool NU, bool RES>
+void launch_lap_stream(const Tiling &t, int cl, const Uniform &u, const NonUniform &g,
+                       const Faces &f, const double *a, double *b, const double *rhs,
+                       double *partials) {
+  const dim3 blk(kTileX, t.ty);
+  if (cl == 2)
+    k_lap_stream_t<2, NU, RES><<<t.blocks, blk, 0, stream()>>>(t, u, g, f, a, b, rhs, partials);
+  else
+    k_lap_stream_t<1, NU, RES><<<t.blocks, blk, 0, stream()>>>(t, u, g, f, a, b, rhs, partials);
+}
+
 // Face gradients of a dense field with prescribed outer-face data
 // (src/fpm_laplace_3d.f90:305-310, src/fpm_terms_3d.f90:243-248).
 __device__ __forceinline__ void dense_face_gradients(const NonUniform &g, const Faces &f,
@@ -372,10 +597,13 @@ struct Ctx {
   Scalars *sc;
 };
 
-int ctx_setup(Ctx &x, size_t n) {
+// blocks: the largest launch whose partials the solve stores (beyond the
+// elementwise launches).
+int ctx_setup(Ctx &x, size_t n, size_t blocks = 0) {
   int st = 0;
   Workspace &ws = workspace();
-  const size_t nb = (size_t)elementwise_blocks(n) + 8192;
+  const size_t eb = (size_t)elementwise_blocks(n);
+  const size_t nb = (blocks > eb ? blocks : eb) + 8192;
   x.partials = (double *)ws.get(S_PARTIALS, sizeof(double) * kReduceSlots * nb, &st);
   if (!x.partials) return st;
   x.sc = (Scalars *)ws.get(S_SCALARS, sizeof(Scalars), &st);
@@ -456,7 +684,7 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
   if ((st = bad(nx, ny, nz))) return st;
   Ctx x;
   const size_t n = (size_t)nx * ny * nz;
-  if ((st = ctx_setup(x, n))) return st;
+  if ((st = ctx_setup(x, n, lap_max_blocks(nx, ny, nz)))) return st;
   double *a = (double *)workspace().get(S_PHI, sizeof(double) * n, &st);
   if (!a) return st;
   double *b = (double *)workspace().get(S_L_NEXT, sizeof(double) * n, &st);
@@ -472,6 +700,20 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
   u.rdz2 = dz * dz;
   Faces f{xlo, xhi, ylo, yhi, zlo, zhi};
   const int eb = elementwise_blocks(n);
+  const bool classic = lap_classic();
+  const int cl = lap_cells_per_lane(nx, {a, b, rhs, ylo, yhi, zlo, zhi});
+  const Tiling tz = lap_tiling(nx, ny, nz, cl);
+  NonUniform g0{};
+  // max-norm residual of a -> sc->res
+  auto residual = [&](const double *v) {
+    if (classic) {
+      k_lap_uniform_residual<<<eb, kThreads, 0, stream()>>>(u, f, v, rhs, x.partials, x.sc);
+      finish(x, FIN_MAX, 1, eb);
+    } else {
+      launch_lap_stream<false, true>(tz, cl, u, g0, f, v, nullptr, rhs, x.partials);
+      finish(x, FIN_MAX, 1, tz.blocks);
+    }
+  };
   k_lap_pack<<<eb, kThreads, 0, stream()>>>(nx, ny, nz, phi_ref, a);
   FFPM_LAUNCHED("k_lap_pack");
   *iterations = 0;
@@ -483,7 +725,10 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
     for (int s = 0; s < nsteps; ++s) {
       {
         TimedLaunch tl("lap_sweep");
-        k_lap_uniform_sweep<<<eb, kThreads, 0, stream()>>>(u, f, a, b, rhs);
+        if (classic)
+          k_lap_uniform_sweep<<<eb, kThreads, 0, stream()>>>(u, f, a, b, rhs);
+        else
+          launch_lap_stream<false, false>(tz, cl, u, g0, f, a, b, rhs, nullptr);
       }
       double *t = a;
       a = b;
@@ -491,9 +736,8 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
       *iterations += 1;
     }
     FFPM_LAUNCHED("k_lap_uniform_sweep");
-    k_lap_uniform_residual<<<eb, kThreads, 0, stream()>>>(u, f, a, rhs, x.partials, x.sc);
+    residual(a);
     FFPM_LAUNCHED("k_lap_uniform_residual");
-    finish(x, FIN_MAX, 1, eb);
     if ((st = read_sc(x, &h))) return st;
     res = h.res;
   }
@@ -507,9 +751,8 @@ int lap_uniform_solve_dev(double *phi_ref, const double *rhs, const double *xlo,
   k_lap_unpack_ghosts<<<elementwise_blocks(np), kThreads, 0, stream()>>>(nx, ny, nz, f, dx, dy,
                                                                          dz, a, phi_ref);
   FFPM_LAUNCHED("k_lap_unpack_ghosts");
-  k_lap_uniform_residual<<<eb, kThreads, 0, stream()>>>(u, f, a, rhs, x.partials, x.sc);
+  residual(a);
   FFPM_LAUNCHED("k_lap_uniform_residual");
-  finish(x, FIN_MAX, 1, eb);
   if ((st = read_sc(x, &h))) return st;
   *final_residual = h.res;
   return OK;
@@ -543,13 +786,28 @@ int lap_nonuniform_solve_dev(double *phi, const double *rhs, const double *xlo,
   if ((st = bad(nx, ny, nz))) return st;
   Ctx x;
   const size_t n = (size_t)nx * ny * nz;
-  if ((st = ctx_setup(x, n))) return st;
+  if ((st = ctx_setup(x, n, lap_max_blocks(nx, ny, nz)))) return st;
   NonUniform g;
   if ((st = nonuniform_setup(g, nx, ny, nz, xc, yc, zc, dx, dy, dz))) return st;
   double *b = (double *)workspace().get(S_L_NEXT, sizeof(double) * n, &st);
   if (!b) return st;
   Faces f{xlo, xhi, ylo, yhi, zlo, zhi};
   const int eb = elementwise_blocks(n);
+  const bool classic = lap_classic();
+  const int cl = lap_cells_per_lane(nx, {phi, b, rhs, ylo, yhi, zlo, zhi});
+  const Tiling tz = lap_tiling(nx, ny, nz, cl);
+  Uniform u0{};
+  u0.nx = nx; u0.ny = ny; u0.nz = nz;
+  // max-norm residual of v -> sc->res
+  auto residual = [&](const double *v) {
+    if (classic) {
+      k_lap_nonuniform_residual<<<eb, kThreads, 0, stream()>>>(g, f, v, rhs, x.partials, x.sc);
+      finish(x, FIN_MAX, 1, eb);
+    } else {
+      launch_lap_stream<true, true>(tz, cl, u0, g, f, v, nullptr, rhs, x.partials);
+      finish(x, FIN_MAX, 1, tz.blocks);
+    }
+  };
   double *a = phi;
   *iterations = 0;
   double res = DBL_MAX;
@@ -560,7 +818,10 @@ int lap_nonuniform_solve_dev(double *phi, const double *rhs, const double *xlo,
     for (int s = 0; s < nsteps; ++s) {
       {
         TimedLaunch tl("lap_sweep");
-        k_lap_nonuniform_sweep<<<eb, kThreads, 0, stream()>>>(g, f, a, b, rhs);
+        if (classic)
+          k_lap_nonuniform_sweep<<<eb, kThreads, 0, stream()>>>(g, f, a, b, rhs);
+        else
+          launch_lap_stream<true, false>(tz, cl, u0, g, f, a, b, rhs, nullptr);
       }
       double *t = a;
       a = b;
@@ -568,9 +829,8 @@ int lap_nonuniform_solve_dev(double *phi, const double *rhs, const double *xlo,
       *iterations += 1;
     }
     FFPM_LAUNCHED("k_lap_nonuniform_sweep");
-    k_lap_nonuniform_residual<<<eb, kThreads, 0, stream()>>>(g, f, a, rhs, x.partials, x.sc);
+    residual(a);
     FFPM_LAUNCHED("k_lap_nonuniform_residual");
-    finish(x, FIN_MAX, 1, eb);
     if ((st = read_sc(x, &h))) return st;
     res = h.res;
   }
@@ -583,9 +843,8 @@ int lap_nonuniform_solve_dev(double *phi, const double *rhs, const double *xlo,
   finish(x, FIN_SUMS, 2, eb);
   k_lap_subtract_mean<<<eb, kThreads, 0, stream()>>>(n, a, -1.0, x.sc);
   FFPM_LAUNCHED("k_lap_subtract_mean");
-  k_lap_nonuniform_residual<<<eb, kThreads, 0, stream()>>>(g, f, a, rhs, x.partials, x.sc);
+  residual(a);
   FFPM_LAUNCHED("k_lap_nonuniform_residual");
-  finish(x, FIN_MAX, 1, eb);
   if ((st = read_sc(x, &h))) return st;
   *final_residual = h.res;
   return OK;
