@@ -59,13 +59,13 @@ BYTES_PER_CELL = {
                          # write 15 outputs
     # CG with the FP32 multigrid cycle (level-0 kernels; the coarse levels add
     # 1/8, 1/64, ... of their own small traffic)
-    "mg_update": 53,         # read p, code; read/write phi, r; write x1, q, s = r/V (FP32)
+    "mg_update": 49,         # read p (FP32), code; read/write phi, r; write x1, q, s = r/V (FP32)
     "mg_fine_sweep": 15,     # x, s (FP32), code; write x (FP32); half the Chebyshev
                              # steps also read the previous iterate (13 and 17 B)
     "mg_fine_restrict": 9,   # x, s, code per fine cell (coarse writes 1/8 of 4 B)
     "mg_fine_prolong": 9,    # x read/write, code (coarse reads 1/8)
     "mg_fine_last": 21,      # x, previous iterate, s, q (FP32), code; write z (FP32)
-    "mg_direction": 21,      # read z (FP32), p_old, code; write p
+    "mg_direction": 13,      # read z, p_old (FP32), code; write p (FP32)
 }
 
 

@@ -529,14 +529,15 @@ __global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
   if (!iterating(st, slot)) return;
   const BSolve &d = solves[s];
   const int lb = grid.local[blockIdx.x];
-  const double *pin = d.p[st.cur];
-  double *pout = d.p[st.cur ^ 1];
+  // FP32 directions in the FP64 buffers' memory, as the standalone solve
+  const float *pin = reinterpret_cast<const float *>(d.p[st.cur]);
+  float *pout = reinterpret_cast<float *>(d.p[st.cur ^ 1]);
   if (st.mode == kDirUpdate)
-    pcg_dir_body<double, CL, kDirUpdate, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32, d.code,
-                                                pin, pout, d.partials, d.sc, lds);
+    pcg_dir_body<double, CL, kDirUpdate, float, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32,
+                                                       d.code, pin, pout, d.partials, d.sc, lds);
   else
-    pcg_dir_body<double, CL, kDirReset, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32, d.code,
-                                               pin, pout, d.partials, d.sc, lds);
+    pcg_dir_body<double, CL, kDirReset, float, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32,
+                                                      d.code, pin, pout, d.partials, d.sc, lds);
 }
 
 // x += alpha p, r -= alpha A p and the next cycle's first step; the slot that
@@ -551,8 +552,9 @@ __global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
-  mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g, d.p[st.cur], d.code,
-                       d.x, d.r, mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32, mg.s32);
+  mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g,
+                       reinterpret_cast<const float *>(d.p[st.cur]), d.code, d.x, d.r,
+                       mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32, mg.s32);
 }
 
 // One workgroup per solve: the finish of the preceding reduction for the

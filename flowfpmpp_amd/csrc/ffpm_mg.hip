@@ -98,7 +98,7 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
 
 template <int CL>
 __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
-                                                   const double *__restrict__ p,
+                                                   const float *__restrict__ p,
                                                    const uint8_t *__restrict__ code,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    float *__restrict__ x1, Scalars *sc,
@@ -111,17 +111,17 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
 }
 
 // The flexible CG's direction step on the cycle's FP32 output (the PCG
-// direction body with z widened on load).
+// direction body with z widened on load and p stored in FP32).
 template <int CL, int MODE>
 __global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<double> g,
                                                    const float *__restrict__ z,
                                                    const uint8_t *__restrict__ code,
-                                                   const double *__restrict__ pin,
-                                                   double *__restrict__ pout, double *partials,
+                                                   const float *__restrict__ pin,
+                                                   float *__restrict__ pout, double *partials,
                                                    Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  pcg_dir_body<double, CL, MODE, float>(t, blockIdx.x, gridDim.x, g, z, code, pin, pout,
-                                        partials, sc, lds);
+  pcg_dir_body<double, CL, MODE, float, float>(t, blockIdx.x, gridDim.x, g, z, code, pin, pout,
+                                               partials, sc, lds);
 }
 
 // ---- the learned coarse correction -----------------------------------------------
@@ -552,7 +552,7 @@ int mg_first(const Multigrid &mg, double *r, const Scalars *proj) {
   return OK;
 }
 
-int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
+int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, double *x,
               double *r, Scalars *sc, double *rmax) {
   const dim3 blk(kTileX, t.ty);
   const double w = (double)mg.first;
@@ -706,7 +706,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
   return OK;
 }
 
-int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const double *pin, double *pout,
+int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin, float *pout,
                  int mode, Scalars *sc) {
   if (mode != kDirUpdate && mode != kDirReset) {
     set_error(ERR_ARGUMENT, "multigrid direction mode");

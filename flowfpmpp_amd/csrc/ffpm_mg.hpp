@@ -159,9 +159,10 @@ int mg_first(const Multigrid &mg, double *r, const Scalars *proj = nullptr);
 
 // x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r, q32 = A p and s32 = r / V
 // (alpha in sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
-// shape. With rmax non-null, one partial max |r / V| per workgroup (t.blocks)
-// goes there.
-int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, double *x,
+// shape. The direction p is stored in FP32 (mg_direction rounds it) and
+// widened exactly; x, r and the arithmetic stay FP64. With rmax non-null, one
+// partial max |r / V| per workgroup (t.blocks) goes there.
+int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, double *x,
               double *r, Scalars *sc, double *rmax = nullptr);
 
 // z32 = B r (the rest of the cycle after the first step, which must be in
@@ -170,9 +171,10 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const double *p, dou
 int mg_vcycle(const Multigrid &mg, bool with_q);
 
 // The flexible CG's direction step with the cycle's FP32 output: p = z32 (MODE
-// kDirReset) or z32 + beta p_old (kDirUpdate), q = A p and the p.q partials
-// (t.blocks, FP64 launch shape t with cl cells per lane).
-int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const double *pin, double *pout,
+// kDirReset) or z32 + beta p_old (kDirUpdate), formed in FP64 and stored
+// rounded to FP32, q = A p of the rounded p and the p.q partials (t.blocks,
+// FP64 launch shape t with cl cells per lane).
+int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin, float *pout,
                  int mode, Scalars *sc);
 
 }  // namespace ffpm

@@ -641,15 +641,15 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
 // counts the iteration.
 template <int CL>
 __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds, Tiling t,
-                                            GeomT<double> g, const double *__restrict__ p,
+                                            GeomT<double> g, const float *__restrict__ p,
                                             const uint8_t *__restrict__ code,
                                             double *__restrict__ x, double *__restrict__ r,
                                             float *__restrict__ x1, Scalars *sc, double omega,
                                             double *rmax, float *__restrict__ q32,
                                             float *__restrict__ s32) {
   if (sc->status != 0) return;
-  pcg_upd_body<double, CL, float, false>(t, block, nblocks, g, p, code, x, r, x1, nullptr,
-                                         sc, lds, omega, rmax, q32, s32);
+  pcg_upd_body<double, CL, float, false, float>(t, block, nblocks, g, p, code, x, r, x1,
+                                                nullptr, sc, lds, omega, rmax, q32, s32);
   if (block == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
 }
 

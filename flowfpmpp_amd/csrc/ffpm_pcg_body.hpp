@@ -43,6 +43,20 @@ __device__ __forceinline__ void stv(T *__restrict__ a, size_t c, const Vec<T, CL
   *reinterpret_cast<Vec<T, CL> *>(a + c) = x;
 }
 
+// Element-wise conversion of a vector (exact where the values are already
+// representable in U, as the rounded directions are).
+template <typename U, typename T, int CL>
+__device__ __forceinline__ Vec<U, CL> convert(const Vec<T, CL> &x) {
+  if constexpr (std::is_same<U, T>::value) {
+    return x;
+  } else {
+    Vec<U, CL> o;
+#pragma unroll
+    for (int m = 0; m < CL; ++m) o.v[m] = (U)x.v[m];
+    return o;
+  }
+}
+
 // Streaming accesses of values used once (the source, the outputs): the
 // nontemporal form marks the cache lines for early eviction, which keeps
 // the L2 for the neighbour rows the stencil re-reads. NT = false keeps the
@@ -97,13 +111,25 @@ __device__ __forceinline__ void stv_s(T *__restrict__ a, size_t c, const Vec<T, 
 // argument: with a run-time mode the compiler puts every load pair in its own
 // branch and waits for it there, which serialises the loads of a plane. z may
 // be stored in a narrower type ZT (the multigrid cycle's FP32 output), widened
-// exactly on use (kDirCurrent, which loads p_old into a, needs ZT = T).
-template <int MODE, typename T, int CL, typename ZT = T>
+// exactly on use (kDirCurrent, which loads p_old into a, needs ZT = T). The
+// directions may be stored in a narrower type PT (the multigrid CG's FP32
+// directions): p = z + beta p_old is formed in T and rounded to PT, and every
+// use of p (its own value, its neighbours in A p) takes the rounded value,
+// so the kernels that read the stored p later see exactly what was used here.
+template <typename T, typename PT>
+__device__ __forceinline__ T round_to(T v) {
+  if constexpr (std::is_same<T, PT>::value) return v;
+  else return (T)(PT)v;
+}
+
+template <int MODE, typename T, int CL, typename ZT = T, typename PT = T>
 struct RawV {
-  static_assert(MODE != kDirCurrent || std::is_same<ZT, T>::value, "p_old is T");
+  static_assert(MODE != kDirCurrent ||
+                    (std::is_same<ZT, T>::value && std::is_same<PT, T>::value),
+                "p_old is T");
   Vec<ZT, CL> a;  // z (p_old for kDirCurrent)
-  Vec<T, CL> b;   // p_old for kDirUpdate
-  __device__ __forceinline__ void load(const ZT *__restrict__ z, const T *__restrict__ pin,
+  Vec<PT, CL> b;  // p_old for kDirUpdate
+  __device__ __forceinline__ void load(const ZT *__restrict__ z, const PT *__restrict__ pin,
                                        size_t c) {
     if constexpr (MODE == kDirCurrent) {
       a = ldv<ZT, CL>(pin, c);
@@ -111,24 +137,26 @@ struct RawV {
       a = ldv<ZT, CL>(z, c);
     } else {
       a = ldv<ZT, CL>(z, c);
-      b = ldv<T, CL>(pin, c);
+      b = ldv<PT, CL>(pin, c);
     }
   }
   __device__ __forceinline__ Vec<T, CL> value(T beta) const {
     Vec<T, CL> v;
 #pragma unroll
     for (int m = 0; m < CL; ++m)
-      v.v[m] = MODE == kDirUpdate ? (T)a.v[m] + beta * b.v[m] : (T)a.v[m];
+      v.v[m] = round_to<T, PT>(MODE == kDirUpdate ? (T)a.v[m] + beta * (T)b.v[m] : (T)a.v[m]);
     return v;
   }
 };
 
-template <int MODE, typename T, typename ZT = T>
+template <int MODE, typename T, typename ZT = T, typename PT = T>
 struct Raw1 {
-  static_assert(MODE != kDirCurrent || std::is_same<ZT, T>::value, "p_old is T");
+  static_assert(MODE != kDirCurrent ||
+                    (std::is_same<ZT, T>::value && std::is_same<PT, T>::value),
+                "p_old is T");
   ZT a;  // z (p_old for kDirCurrent)
-  T b;   // p_old for kDirUpdate
-  __device__ __forceinline__ void load(const ZT *__restrict__ z, const T *__restrict__ pin,
+  PT b;  // p_old for kDirUpdate
+  __device__ __forceinline__ void load(const ZT *__restrict__ z, const PT *__restrict__ pin,
                                        size_t c) {
     if constexpr (MODE == kDirCurrent) {
       a = pin[c];
@@ -140,19 +168,19 @@ struct Raw1 {
     }
   }
   __device__ __forceinline__ T value(T beta) const {
-    if constexpr (MODE != kDirUpdate) return (T)a;
-    else return (T)a + beta * b;
+    if constexpr (MODE != kDirUpdate) return round_to<T, PT>((T)a);
+    else return round_to<T, PT>((T)a + beta * (T)b);
   }
 };
 
 // The in-plane neighbours of a lane's cells (south and north rows, the west
 // and east cells) and their neighbour codes, for one plane.
-template <int MODE, typename T, int CL, typename ZT = T>
+template <int MODE, typename T, int CL, typename ZT = T, typename PT = T>
 struct Plane {
-  RawV<MODE, T, CL, ZT> s, n;
-  Raw1<MODE, T, ZT> w, e;
+  RawV<MODE, T, CL, ZT, PT> s, n;
+  Raw1<MODE, T, ZT, PT> w, e;
   Codes<CL> code;
-  __device__ __forceinline__ void load(const ZT *__restrict__ z, const T *__restrict__ pin,
+  __device__ __forceinline__ void load(const ZT *__restrict__ z, const PT *__restrict__ pin,
                                        const uint8_t *__restrict__ cd, size_t c, ptrdiff_t os,
                                        ptrdiff_t on, ptrdiff_t ow, ptrdiff_t oe) {
     s.load(z, pin, c + os);
@@ -210,11 +238,11 @@ __device__ __forceinline__ Lane lane_of(const Tiling &t, int nx, int ny, int nz,
 // p = z + beta p_old (or z, or p_old, by MODE), q = A p and p.q for the
 // workgroup `block` of a launch of nblocks workgroups; z in T or in a
 // narrower ZT.
-template <typename T, int CL, int MODE, typename ZT = T>
+template <typename T, int CL, int MODE, typename ZT = T, typename PT = T>
 __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const ZT *__restrict__ z,
                                              const uint8_t *__restrict__ code,
-                                             const T *__restrict__ pin, T *__restrict__ pout,
+                                             const PT *__restrict__ pin, PT *__restrict__ pout,
                                              double *partials, const Scalars *sc, double *lds) {
   if (sc->status != 0) return;
   const T beta = (T)sc->beta;
@@ -251,8 +279,8 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
     const size_t base = (size_t)i + (size_t)nx * (size_t)j;
     const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
     auto at = [&](int k) { return base + nxy * (size_t)k; };
-    RawV<MODE, T, CL, ZT> own_a, own_b;
-    Plane<MODE, T, CL, ZT> pl_a, pl_b;
+    RawV<MODE, T, CL, ZT, PT> own_a, own_b;
+    Plane<MODE, T, CL, ZT, PT> pl_a, pl_b;
     own_a.load(z, pin, at(k0));
     Vec<T, CL> p_cur = own_a.value(beta);
     Vec<T, CL> p_prev = p_cur;
@@ -262,9 +290,9 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
     }
     own_a.load(z, pin, at(min(k0 + 1, last)));
     pl_a.load(z, pin, code, at(k0), os, on, ow, oe);
-    auto stage = [&](int k, const RawV<MODE, T, CL, ZT> &own_use,
-                     const Plane<MODE, T, CL, ZT> &pl, RawV<MODE, T, CL, ZT> &own_load,
-                     Plane<MODE, T, CL, ZT> &pl_load) {
+    auto stage = [&](int k, const RawV<MODE, T, CL, ZT, PT> &own_use,
+                     const Plane<MODE, T, CL, ZT, PT> &pl, RawV<MODE, T, CL, ZT, PT> &own_load,
+                     Plane<MODE, T, CL, ZT, PT> &pl_load) {
       own_load.load(z, pin, at(min(k + 2, last)));
       pl_load.load(z, pin, code, at(min(k + 1, k1 - 1)), os, on, ow, oe);
       const Vec<T, CL> p_next = own_use.value(beta);
@@ -287,7 +315,7 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
                      cb * p_prev.v[m] - ct * p_next.v[m]);
         s += (double)p_cur.v[m] * (double)(d > 0 ? q : T(0));
       }
-      if constexpr (MODE != kDirCurrent) stv<T, CL>(pout, at(k), p_cur);
+      if constexpr (MODE != kDirCurrent) stv<PT, CL>(pout, at(k), convert<PT>(p_cur));
       p_prev = p_cur;
       p_cur = p_next;
     };
@@ -308,9 +336,9 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 // cells there (an estimate of the max-norm residual), with qout non-null
 // q = A p is stored in ZT (for the flexible-CG coefficient of the next step),
 // and with sout non-null r / V (0 outside fluid cells) is stored in ZT.
-template <typename T, int CL, typename ZT = T, bool DOT = true>
+template <typename T, int CL, typename ZT = T, bool DOT = true, typename PT = T>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
-                                             const GeomT<T> &g, const T *__restrict__ p,
+                                             const GeomT<T> &g, const PT *__restrict__ p,
                                              const uint8_t *__restrict__ code,
                                              T *__restrict__ x, T *__restrict__ r,
                                              ZT *__restrict__ z, double *partials,
@@ -349,39 +377,43 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
     const size_t base = (size_t)i + (size_t)nx * (size_t)j;
     const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
     auto at = [&](int k) { return base + nxy * (size_t)k; };
+    // p may be stored narrower (PT): it is widened where it is used, so the
+    // loads stay in flight across the computation of a plane.
     struct Set {
-      Vec<T, CL> p_next, x, r;
-      Plane<kDirCurrent, T, CL> pl;
+      Vec<PT, CL> p_next;
+      Vec<T, CL> x, r;
+      Plane<kDirCurrent, PT, CL> pl;
     };
     Set sa, sb;
-    Vec<T, CL> p_cur = ldv<T, CL>(p, at(k0));
-    Vec<T, CL> p_prev = k0 > 0 ? ldv<T, CL>(p, at(k0 - 1)) : p_cur;
-    sa.p_next = ldv<T, CL>(p, at(min(k0 + 1, last)));
+    Vec<T, CL> p_cur = convert<T>(ldv<PT, CL>(p, at(k0)));
+    Vec<T, CL> p_prev = k0 > 0 ? convert<T>(ldv<PT, CL>(p, at(k0 - 1))) : p_cur;
+    sa.p_next = ldv<PT, CL>(p, at(min(k0 + 1, last)));
     sa.pl.load(p, p, code, at(k0), os, on, ow, oe);
     sa.x = ldv<T, CL>(x, at(k0));
     sa.r = ldv<T, CL>(r, at(k0));
     auto stage = [&](int k, const Set &u, Set &ld) {
       const int kn = min(k + 1, k1 - 1);
-      ld.p_next = ldv<T, CL>(p, at(min(k + 2, last)));
+      ld.p_next = ldv<PT, CL>(p, at(min(k + 2, last)));
       ld.pl.load(p, p, code, at(kn), os, on, ow, oe);
       ld.x = ldv<T, CL>(x, at(kn));
       ld.r = ldv<T, CL>(r, at(kn));
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
       Vec<T, CL> xv = u.x, rv = u.r;
       Vec<ZT, CL> zv, qv, sv;
+      const Vec<T, CL> p_next = convert<T>(u.p_next);
       const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
       const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
-      const T vw = lane == 0 ? u.pl.w.a : from_w;
-      const T ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
+      const T vw = lane == 0 ? (T)u.pl.w.a : from_w;
+      const T ve = lane == kTileX - 1 ? (T)u.pl.e.a : from_e;
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
         const T west = m == 0 ? (hw ? vw : T(0)) : p_cur.v[m - 1];
         const T east = m == CL - 1 ? (he ? ve : T(0)) : p_cur.v[m + 1];
-        const T south = hs ? u.pl.s.a.v[m] : T(0), north = hn ? u.pl.n.a.v[m] : T(0);
+        const T south = hs ? (T)u.pl.s.a.v[m] : T(0), north = hn ? (T)u.pl.n.a.v[m] : T(0);
         const T d = diag_code<T>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
         const T qr = dxdy[m] * dzk *
                      (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
-                      cb * p_prev.v[m] - ct * u.p_next.v[m]);
+                      cb * p_prev.v[m] - ct * p_next.v[m]);
         const T q = d > 0 ? qr : T(0);
         xv.v[m] = xv.v[m] + alpha * p_cur.v[m];
         rv.v[m] = rv.v[m] - alpha * q;
@@ -405,7 +437,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
         if (sout) stv_s<true, ZT, CL>(sout, at(k), sv);
       }
       p_prev = p_cur;
-      p_cur = u.p_next;
+      p_cur = p_next;
     };
     for (int k = k0; k < k1; k += 2) {
       stage(k, sa, sb);

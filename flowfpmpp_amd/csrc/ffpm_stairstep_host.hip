@@ -860,7 +860,8 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   g_mg = MgStats();
   g_mg.levels = mg.levels;
   const int ci = mg_check_interval(check_interval);
-  double *pbuf[2] = {f.next, f.q};
+  // the directions, stored in FP32 (in the FP64 buffers' memory)
+  float *pbuf[2] = {reinterpret_cast<float *>(f.next), reinterpret_cast<float *>(f.q)};
   int cur = 0, mode = kDirReset;
   bool first_ready = false;  // lv[0].x holds w D^-1 r for the current r
   // The true residual's second pass (projection of r) runs inside the next
