@@ -446,6 +446,21 @@ __global__ __launch_bounds__(256) void k_bg_restrict0(const BSolve *solves, Grid
                                 mg.buf0(ia), c.nx, c.ny, c.nz, d.kc1, out);
 }
 
+// The streaming restriction from level 0's iterate (buffer ia) onto level 1,
+// over each solve's restriction tiles (the standalone mg_restrict_body).
+template <int CL, bool NT>
+__global__ __launch_bounds__(512) void k_bg_restrict_t(const BSolve *solves, Grid grid,
+                                                       const BState *state, int slot, int ia) {
+  extern __shared__ float mg_rs_lds[];  // mg_restrict_lds(ty, CL) bytes
+  const int s = grid.solve[blockIdx.x];
+  if (!iterating(state[s], slot)) return;
+  const Multigrid &mg = solves[s].mg;
+  const MgLevel &c = mg.lv[1];
+  const mgk::CoarseOut out{c.b, c.x, c.rd};
+  mgk::mg_restrict_body<CL, NT>(grid.local[blockIdx.x], mg_rs_lds, mg.tlr, mg.g32, mg.buf0(ia),
+                                mg.s32, mg.code, c.nx, c.ny, out);
+}
+
 // Prolongation from level 1's result (buffer ic; the coarsest level writes
 // the same buffer as the others) into level 0's iterate (buffer ia).
 template <bool VEC>
@@ -875,6 +890,7 @@ struct Batch {
   // largest coarsest-level index
   bool mgm = false;
   Grid g_cell0, g_col[2], g_lvl[kMgMaxLevels];
+  Grid g_res[5];  // streaming restriction tiles, by cells per lane (1, 2, 4)
   int mg_top = 0;
   // the problems on the device, the final residuals, the gauge-shift grid
   ffpm_stairstep_problem *d_probs = nullptr;
@@ -977,6 +993,7 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
       total += align256(mg_state_bytes(mg));
       if (B.owner[s] == s) total += align256(mg_operator_bytes(mg));
       const MgLevel &c1 = mg.lv[1];
+      grid_blocks += (size_t)mg.tlr.blocks;
       grid_blocks += cells3(mgk::cell_grid(p.nx, p.ny, p.nz)) +
                      cells3(mgk::column_grid(c1.nx, c1.ny, c1.nz,
                                              mgk::column_chunk(c1.nx, c1.ny, c1.nz)));
@@ -990,8 +1007,8 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
                align256(sizeof(float) * 3 * (p.nx + p.ny + p.nz));
   }
   // workgroup tables: two int arrays per grid (at most nine grids, plus
-  // 3 + kMgMaxLevels for the multigrid), each 256-aligned
-  total += 2 * align256(sizeof(int) * grid_blocks) + 2 * 256 * (9 + 3 + kMgMaxLevels);
+  // 6 + kMgMaxLevels for the multigrid), each 256-aligned
+  total += 2 * align256(sizeof(int) * grid_blocks) + 2 * 256 * (9 + 6 + kMgMaxLevels);
   int st = 0;
   Arena a;
   a.base = (char *)workspace().get(S_BATCH, total, &st);
@@ -1137,6 +1154,13 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
         use[s] = (B.h[s].mg.lv[0].nx % 2 == 0) == (v == 1);
       }
       if ((st = build_grid(a, B.g_col[v], blocks, use, hs, hl))) return st;
+    }
+    for (int c : {1, 2, 4}) {
+      for (int s = 0; s < count; ++s) {
+        blocks[s] = B.h[s].mg.tlr.blocks;
+        use[s] = B.cl32[s] == c;
+      }
+      if ((st = build_grid(a, B.g_res[c], blocks, use, hs, hl))) return st;
     }
     for (int l = 1; l <= B.mg_top; ++l) {
       for (int s = 0; s < count; ++s) {
@@ -1325,12 +1349,27 @@ int launch_cycle_mg(const Batch &B, hipStream_t s, int m) {
   int st;
   for (int k = 0; k < s0.npre; ++k)
     if ((st = step0(s0.pre[k], false))) return st;
-  if (B.g_col[0].blocks)
-    k_bg_restrict0<false><<<B.g_col[0].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[0], B.d_state,
-                                                             m, s0.mid);
-  if (B.g_col[1].blocks)
-    k_bg_restrict0<true><<<B.g_col[1].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[1], B.d_state,
-                                                            m, s0.mid);
+  if (h.stream_restrict) {
+    const dim3 rblk(kTileX, h.tlr.ty);
+    for (int c : {1, 2, 4}) {
+      const Grid &g = B.g_res[c];
+      if (!g.blocks) continue;
+      const size_t lb = mgk::mg_restrict_lds(h.tlr.ty, c);
+      auto go = [&](auto kern) {
+        kern<<<g.blocks, rblk, lb, s>>>(B.d_solves, g, B.d_state, m, s0.mid);
+      };
+      if (c == 4) go(k_bg_restrict_t<4, NT>);
+      else if (c == 2) go(k_bg_restrict_t<2, NT>);
+      else go(k_bg_restrict_t<1, NT>);
+    }
+  } else {
+    if (B.g_col[0].blocks)
+      k_bg_restrict0<false><<<B.g_col[0].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[0],
+                                                               B.d_state, m, s0.mid);
+    if (B.g_col[1].blocks)
+      k_bg_restrict0<true><<<B.g_col[1].blocks, cblk, 0, s>>>(B.d_solves, B.g_col[1],
+                                                              B.d_state, m, s0.mid);
+  }
   FFPM_LAUNCHED("k_bg_restrict0");
   for (int l = 1; l < B.mg_top; ++l) {
     for (int k = 0; k < sc.npre; ++k)

@@ -33,6 +33,16 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
   mgf_restrict_z_body<VEC>(blk3(), g, code, src, x, cnx, cny, cnz, kchunk, out);
 }
 
+template <int CL, bool NT>
+__global__ __launch_bounds__(512) void k_mg_restrict_t(Tiling t, GeomT<float> g,
+                                                       const float *__restrict__ x,
+                                                       const float *__restrict__ src,
+                                                       const uint8_t *__restrict__ code, int cnx,
+                                                       int cny, CoarseOut out) {
+  extern __shared__ float mg_rs_lds[];  // mg_restrict_lds(t.ty, CL) bytes
+  mg_restrict_body<CL, NT>(blockIdx.x, mg_rs_lds, t, g, x, src, code, cnx, cny, out);
+}
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
                                                        const uint8_t *__restrict__ code,
@@ -342,6 +352,14 @@ int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   mg.tl = tl;
   mg.cl = cl;
   mg.partials = partials;
+  {
+    const int rows = tl.ty < 2 ? 2 : tl.ty > 8 ? 8 : tl.ty;
+    int zc = tl.zchunk + (tl.zchunk & 1);
+    if (zc > kMgRestrictPlanes) zc = kMgRestrictPlanes;
+    mg.tlr = make_tiling(g64.nx, g64.ny, g64.nz, zc, rows, kTileX * cl);
+    const char *e = std::getenv("FFPM_MG_RESTRICT");
+    mg.stream_restrict = !(e && std::strcmp(e, "column") == 0);
+  }
   mg.omega = env_float("FFPM_MG_OMEGA", 0.9f);
   mg.omega_coarse = env_float("FFPM_MG_OMEGA_COARSE", mg.omega);
   mg.scale = env_float("FFPM_MG_SCALE", 0.5f);
@@ -593,7 +611,26 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     launch_step<false>(mg, s0.pre[k]);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
-  {
+  if (mg.stream_restrict) {
+    const MgLevel &c = mg.lv[1];
+    TimedLaunch tl("mg_fine_restrict");
+    const dim3 rb(kTileX, mg.tlr.ty);
+    const float *xm = mg.buf0(s0.mid);
+    const size_t lb = mg_restrict_lds(mg.tlr.ty, mg.cl);
+    auto go = [&](auto kern) {
+      kern<<<mg.tlr.blocks, rb, lb, stream()>>>(mg.tlr, g, xm, r, mg.code, c.nx, c.ny,
+                                                coarse_out(mg, 1));
+    };
+    if (mg.nontemporal) {
+      if (mg.cl == 4) go(k_mg_restrict_t<4, true>);
+      else if (mg.cl == 2) go(k_mg_restrict_t<2, true>);
+      else go(k_mg_restrict_t<1, true>);
+    } else {
+      if (mg.cl == 4) go(k_mg_restrict_t<4, false>);
+      else if (mg.cl == 2) go(k_mg_restrict_t<2, false>);
+      else go(k_mg_restrict_t<1, false>);
+    }
+  } else {
     const MgLevel &c = mg.lv[1];
     TimedLaunch tl("mg_fine_restrict");
     const int kc = column_chunk(c.nx, c.ny, c.nz);

@@ -89,6 +89,10 @@ struct Multigrid {
   GeomT<double> g64{};
   const uint8_t *code = nullptr;
   Tiling tl{};              // level-0 streaming launch shape (CL cells per lane)
+  Tiling tlr{};             // the restriction's: tl with an even number of rows (2-8)
+                            // and z chunks of an even number of planes
+  bool stream_restrict = true;  // streaming restriction (FFPM_MG_RESTRICT=column: the
+                                // column kernel)
   int cl = 1;
   float omega = 0.9f;        // damped-Jacobi weight on level 0
   float omega_coarse = 0.9f; // and on the coarse levels

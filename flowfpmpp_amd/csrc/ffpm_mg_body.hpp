@@ -248,6 +248,138 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
   }
 }
 
+// Coarse right-hand side in the streaming scheme of the level-0 sweeps: the
+// level-0 residual V (s - (d x - sum c x_nb)) of each fluid cell, summed over
+// the 2x2x2 children of each level-1 cell. The workgroups walk the tiles of
+// tlr (mg_plan: an even number of rows, z chunks of an even number of planes,
+// so every child block lies in one workgroup); each lane forms its cells'
+// residuals plane by plane with the loads of the next plane in flight, adds
+// x-adjacent pairs (in its registers, or from the next lane when it owns one
+// cell), then the two planes of a pair, and parks the sum in LDS; after the
+// chunk (at most kMgRestrictPlanes planes) one barrier, then the even rows
+// add the odd rows' sums and store. The sums are taken in that fixed order
+// whatever the tile shape and cells per lane, so standalone and batched
+// solves get the same bits. lds: mg_restrict_lds(t.ty, CL) bytes. (A
+// barrier per plane pair would wait for the next plane's loads in flight.)
+template <int CL, bool NT>
+__device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Tiling &t,
+                                                 const GeomT<float> &g,
+                                                 const float *__restrict__ a,
+                                                 const float *__restrict__ r,
+                                                 const uint8_t *__restrict__ code, int cnx,
+                                                 int cny, CoarseOut out) {
+  constexpr int RC = CL >= 2 ? CL / 2 : 1;  // coarse cells per lane
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const TileCoord tc = tile_coord(t, nx, ny, nz, block);
+  if (!tc.valid) return;  // the whole workgroup
+  using V = Vec<float, CL>;
+  const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
+  const int i = (tc.i - lane) * CL + CL * lane, j = tc.j;
+  const bool act = i < nx && j < ny;
+  // Lanes past the grid load from a clamped cell (in bounds, unused) and add
+  // nothing, so every wave of the workgroup reaches every barrier.
+  const int ic = i < nx ? i : nx - CL, jc = j < ny ? j : ny - 1;
+  const size_t nxy = (size_t)nx * ny;
+  float cw[CL], ce[CL], dxv[CL];
+#pragma unroll
+  for (int m = 0; m < CL; ++m) {
+    cw[m] = g.cw[ic + m];
+    ce[m] = g.ce[ic + m];
+    dxv[m] = g.dx[ic + m];
+  }
+  const float cs = g.cs[jc], cn = g.cn[jc], dyj = g.dy[jc];
+  const bool hw = ic > 0, he = ic + CL < nx, hs = jc > 0, hn = jc < ny - 1;
+  const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+  // only the edge lanes' west / east loads leave their own cells (a clamped
+  // lane's row start could lie before the array)
+  const ptrdiff_t ow = lane == 0 ? (hw ? -1 : 0) : 0;
+  const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : 0;
+  const size_t base = (size_t)ic + (size_t)nx * (size_t)jc;
+  const int k0 = tc.k0, k1 = tc.k1, last = min(k1, nz - 1);
+  auto at = [&](int k) { return base + nxy * (size_t)k; };
+  struct Set {
+    V a_next, r;
+    Plane<kDirCurrent, float, CL> pl;
+  };
+  Set sa, sb;
+  V a_cur = ldv<float, CL>(a, at(k0));
+  V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
+  sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
+  sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
+  sa.r = ldv_s<NT, float, CL>(r, at(k0));
+  float acc[RC];
+  const int stride = kTileX * RC, ty = (int)blockDim.y;
+  auto stage = [&](int k, const Set &u, Set &ld) {
+    const int kn = min(k + 1, k1 - 1);
+    ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
+    ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
+    ld.r = ldv_s<NT, float, CL>(r, at(kn));
+    const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
+    const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
+    const float vw = lane == 0 ? u.pl.w.a : from_w;
+    const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
+    const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
+    float con[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+      const unsigned cd = act ? u.pl.code.b[m] : 0u;
+      const float west = m == 0 ? vw : a_cur.v[m - 1];
+      const float east = m == CL - 1 ? ve : a_cur.v[m + 1];
+      float nb = (cd & kCodeW) ? cw[m] * west : 0.f;
+      nb += (cd & kCodeE) ? ce[m] * east : 0.f;
+      nb += (cd & kCodeS) ? cs * u.pl.s.a.v[m] : 0.f;
+      nb += (cd & kCodeN) ? cn * u.pl.n.a.v[m] : 0.f;
+      nb += (cd & kCodeB) ? cb * a_prev.v[m] : 0.f;
+      nb += (cd & kCodeT) ? ct * u.a_next.v[m] : 0.f;
+      const float d = diag_code<float>(cd, cw[m], ce[m], cs, cn, cb, ct);
+      con[m] = (cd & kCodeFluid) ? dxv[m] * dyj * dzk * (u.r.v[m] - (d * a_cur.v[m] - nb))
+                                 : 0.f;
+    }
+    float px[RC];
+    if constexpr (CL >= 2) {
+#pragma unroll
+      for (int q = 0; q < RC; ++q) px[q] = con[2 * q] + con[2 * q + 1];
+    } else {
+      // the odd lane's cell joins the even lane's (whose sum is the one kept)
+      px[0] = con[0] + __shfl_down(con[0], 1, kTileX);
+    }
+#pragma unroll
+    for (int q = 0; q < RC; ++q) acc[q] = (k & 1) ? acc[q] + px[q] : px[q];
+    if ((k & 1) || k == nz - 1) {
+      float *slot = lds + (size_t)(((k - k0) >> 1) * ty + row) * stride + lane * RC;
+#pragma unroll
+      for (int q = 0; q < RC; ++q) slot[q] = acc[q];
+    }
+    a_prev = a_cur;
+    a_cur = u.a_next;
+  };
+  for (int k = k0; k < k1; k += 2) {
+    stage(k, sa, sb);
+    if (k + 1 < k1) stage(k + 1, sb, sa);
+  }
+  __syncthreads();
+  if (!(row & 1) && act && (CL >= 2 || !(lane & 1))) {
+    const int J = j >> 1;
+    for (int kc = 0; kc < (k1 - k0 + 1) >> 1; ++kc) {
+      const float *e = lds + (size_t)(kc * ty + row) * stride + lane * RC;
+      const int K = (k0 >> 1) + kc;
+#pragma unroll
+      for (int q = 0; q < RC; ++q) {
+        const int I = (i >> 1) + q;
+        if (I < cnx && J < cny)
+          out.store((size_t)I + (size_t)cnx * ((size_t)J + (size_t)cny * K), e[q] + e[stride + q]);
+      }
+    }
+  }
+}
+
+// Planes per z chunk of the streaming restriction (its LDS holds the sums of
+// half as many coarse planes) and its LDS bytes for tiles of ty rows.
+constexpr int kMgRestrictPlanes = 16;
+__host__ __device__ inline size_t mg_restrict_lds(int ty, int cl) {
+  return sizeof(float) * (size_t)(kMgRestrictPlanes / 2) * ty * kTileX * (cl >= 2 ? cl / 2 : 1);
+}
+
 // Prolongation onto level 0 with one thread per level-1 (I, J) column walking
 // a range of K: fine plane 2K + c takes coarse planes K and K - 1 (c = 0) or
 // K + 1 (c = 1), so two 3x3 coarse patches (values and unknown masks) are

@@ -200,3 +200,35 @@ def test_chebyshev_and_jacobi_smoothing_agree(monkeypatch):
     assert rc <= 1e-10 and rj <= 1e-10
     assert sc_ < sj
     assert rel <= 1e-10
+
+
+@pytest.mark.parametrize("shape", [(64, 48, 32), (61, 37, 23)])
+def test_streaming_and_column_restriction_agree(monkeypatch, shape):
+    """The level-0 restriction in the streaming scheme (the default) and the
+    column kernel (FFPM_MG_RESTRICT=column) sum the same residuals in another
+    order: the cycles converge in the same iterations (within one) to the
+    same field, on even and odd extents (odd ones leave coarse cells with a
+    single child layer)."""
+    nx, ny, nz = shape
+    print(f"case: fcg-mg streaming vs column restriction on {nx}x{ny}x{nz}", flush=True)
+    p = W.stretched_frame(nx, ny, nz, [W.Sphere((-1.0, 0.0, 0.0), 0.7),
+                                       W.Box((2.0, -1.0, -0.7), (3.5, 1.0, 0.7))], seed=15,
+                          with_fields=False)
+    out = {}
+    for mode in ("stream", "column"):
+        if mode == "column":
+            monkeypatch.setenv("FFPM_MG_RESTRICT", "column")
+        else:
+            monkeypatch.delenv("FFPM_MG_RESTRICT", raising=False)
+        for d in range(3):
+            ph, sw, res, *_ = _frame_solve(p, d, "fcg-mg", check=1)
+            out[(mode, d)] = (ph, sw, res)
+    monkeypatch.delenv("FFPM_MG_RESTRICT", raising=False)
+    for d in range(3):
+        (a, sa, ra), (b, sb, rb) = out[("stream", d)], out[("column", d)]
+        rel = np.abs(a - b).max() / np.abs(b).max()
+        print(f"  direction {d}: streaming {sa} iterations ({ra:.2e}), column {sb} "
+              f"({rb:.2e}); max rel diff {rel:.2e}", flush=True)
+        assert ra <= 1e-10 and rb <= 1e-10
+        assert abs(sa - sb) <= 1
+        assert rel <= 1e-10
