@@ -59,12 +59,13 @@ BYTES_PER_CELL = {
                          # write 15 outputs
     # CG with the FP32 multigrid cycle (level-0 kernels; the coarse levels add
     # 1/8, 1/64, ... of their own small traffic)
-    "mg_update": 49,         # read p (FP32), code; read/write phi, r; write x1, q, s = r/V (FP32)
-    "mg_fine_sweep": 15,     # x, s (FP32), code; write x (FP32); half the Chebyshev
-                             # steps also read the previous iterate (13 and 17 B)
-    "mg_fine_restrict": 9,   # x, s, code per fine cell (coarse writes 1/8 of 4 B)
+    "mg_update": 45,         # read p (FP32), code; read/write phi, r; write x1, q (FP32)
+    "mg_fine_sweep": 14,     # x, the first step x1 (the source), code; write x (FP32);
+                             # the step after the first reads x1 once (9 B), the next
+                             # ones 13 B, or 17 B with the previous iterate
+    "mg_fine_restrict": 9,   # x, x1, code per fine cell (coarse writes 1/8 of 4 B)
     "mg_fine_prolong": 9,    # x read/write, code (coarse reads 1/8)
-    "mg_fine_last": 21,      # x, previous iterate, s, q (FP32), code; write z (FP32)
+    "mg_fine_last": 21,      # x, previous iterate, x1, q (FP32), code; write z (FP32)
     "mg_direction": 13,      # read z, p_old (FP32), code; write p (FP32)
 }
 

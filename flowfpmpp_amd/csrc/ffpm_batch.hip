@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid gri
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   mgk::mgf_jac0_body(cell_blk(mg.lv[0], grid.local[blockIdx.x]), mg.g32, mg.code, d.r,
-                     mg.lv[0].x, mg.s32, mg.first, d.sc);
+                     mg.lv[0].x, nullptr, mg.first, d.sc);
 }
 
 // One level-0 smoothing step (MgStep: buffers ia, ip, io, coefficients);
@@ -428,9 +428,10 @@ __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid gr
   const Multigrid &mg = d.mg;
   const float *q = LAST && st.mode == kDirUpdate ? mg.q32 : nullptr;
   mgk::mg_sweep_body<CL, LAST, NT, CHEB>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl,
-                                         mg.g32, mg.buf0(step.a), mg.s32, mg.code,
+                                         mg.g32, mg.buf0(step.a), mg.lv[0].x, mg.code,
                                          mg.buf0(step.o), mg.partials, step.c2, q,
-                                         step.p >= 0 ? mg.buf0(step.p) : nullptr, step.c1);
+                                         step.p >= 0 ? mg.buf0(step.p) : nullptr, step.c1,
+                                         mg.sscale);
 }
 
 template <bool VEC>
@@ -442,8 +443,9 @@ __global__ __launch_bounds__(256) void k_bg_restrict0(const BSolve *solves, Grid
   const Multigrid &mg = d.mg;
   const MgLevel &c = mg.lv[1];
   const mgk::CoarseOut out{c.b, c.x, c.rd};
-  mgk::mgf_restrict_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), mg.g32, mg.code, mg.s32,
-                                mg.buf0(ia), c.nx, c.ny, c.nz, d.kc1, out);
+  mgk::mgf_restrict_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), mg.g32, mg.code,
+                                mg.lv[0].x, mg.buf0(ia), c.nx, c.ny, c.nz, d.kc1, out,
+                                mg.sscale);
 }
 
 // The streaming restriction from level 0's iterate (buffer ia) onto level 1,
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(512) void k_bg_restrict_t(const BSolve *solves, Gri
   const MgLevel &c = mg.lv[1];
   const mgk::CoarseOut out{c.b, c.x, c.rd};
   mgk::mg_restrict_body<CL, NT>(grid.local[blockIdx.x], mg_rs_lds, mg.tlr, mg.g32, mg.buf0(ia),
-                                mg.s32, mg.code, c.nx, c.ny, out);
+                                mg.lv[0].x, mg.code, c.nx, c.ny, out, mg.sscale);
 }
 
 // Prolongation from level 1's result (buffer ic; the coarsest level writes
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid
   double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
   mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g,
                        reinterpret_cast<const float *>(d.p[st.cur]), d.code, d.x, d.r,
-                       mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32, mg.s32);
+                       mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32, nullptr);
 }
 
 // One workgroup per solve: the finish of the preceding reduction for the

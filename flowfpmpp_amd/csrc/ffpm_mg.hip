@@ -29,8 +29,8 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
                                                         const float *__restrict__ src,
                                                         const float *__restrict__ x, int cnx,
                                                         int cny, int cnz, int kchunk,
-                                                        CoarseOut out) {
-  mgf_restrict_z_body<VEC>(blk3(), g, code, src, x, cnx, cny, cnz, kchunk, out);
+                                                        CoarseOut out, float sscale) {
+  mgf_restrict_z_body<VEC>(blk3(), g, code, src, x, cnx, cny, cnz, kchunk, out, sscale);
 }
 
 template <int CL, bool NT>
@@ -38,9 +38,9 @@ __global__ __launch_bounds__(512) void k_mg_restrict_t(Tiling t, GeomT<float> g,
                                                        const float *__restrict__ x,
                                                        const float *__restrict__ src,
                                                        const uint8_t *__restrict__ code, int cnx,
-                                                       int cny, CoarseOut out) {
+                                                       int cny, CoarseOut out, float sscale) {
   extern __shared__ float mg_rs_lds[];  // mg_restrict_lds(t.ty, CL) bytes
-  mg_restrict_body<CL, NT>(blockIdx.x, mg_rs_lds, t, g, x, src, code, cnx, cny, out);
+  mg_restrict_body<CL, NT>(blockIdx.x, mg_rs_lds, t, g, x, src, code, cnx, cny, out, sscale);
 }
 
 template <bool VEC>
@@ -100,10 +100,11 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const uint8_t *__restrict__ code,
                                                      float *__restrict__ b, double *partials,
                                                      float c2, const float *__restrict__ q,
-                                                     const float *__restrict__ prev, float c1) {
+                                                     const float *__restrict__ prev, float c1,
+                                                     float sscale) {
   __shared__ double lds[16 * kReduceSlots];
   mg_sweep_body<CL, LAST, NT, CHEB>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, partials,
-                                    c2, q, prev, c1);
+                                    c2, q, prev, c1, sscale);
 }
 
 template <int CL>
@@ -304,10 +305,14 @@ MgSchedule make_schedule(int nu, bool cheb, float omega, double ratio, bool to_z
     }
   }
   *first = c2[0];
-  auto other = [](int a, int p) {
+  // Level 0 (to_z) keeps the first step in buffer 0 for the whole cycle (it
+  // also stands for the source) and rotates 1, 2 and 4; the coarse levels
+  // rotate 0, 1 and 2.
+  auto other = [to_z](int a, int p) {
+    const int set[3] = {to_z ? 1 : 0, to_z ? 2 : 1, to_z ? 4 : 2};
     for (int i = 0; i < 3; ++i)
-      if (i != a && i != p) return i;
-    return 2;
+      if (set[i] != a && set[i] != p) return set[i];
+    return set[2];
   };
   MgSchedule s;
   int cur = 0, prev = -1;  // the first step from zero leaves buffer 0
@@ -375,6 +380,7 @@ int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   if (mg.nu_coarse > 8) mg.nu_coarse = 8;
   const double ratio = env_float("FFPM_MG_CHEB_RATIO", 25.f);
   mg.s0 = make_schedule(mg.nu, mg.cheb, mg.omega, ratio, true, &mg.first);
+  mg.sscale = 1.f / mg.first;
   mg.sc = make_schedule(mg.nu_coarse, mg.cheb, mg.omega_coarse, ratio, false,
                         &mg.first_coarse);
   // level extents: halve (ceil) until the level fits one workgroup
@@ -443,7 +449,7 @@ void mg_bind_state(Multigrid &mg, char *&base) {
   mg.lv[0].y = carve(base, mg.lv[0].n);
   mg.lv[0].w = carve(base, mg.lv[0].n);
   mg.q32 = carve(base, mg.lv[0].n);
-  mg.s32 = carve(base, mg.lv[0].n);
+  mg.v = carve(base, mg.lv[0].n);
   mg.z32 = carve(base, mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
@@ -541,8 +547,8 @@ void launch_step0(const Multigrid &mg, const MgStep &st, const float *q) {
   const float *a = mg.buf0(st.a), *p = st.p >= 0 ? mg.buf0(st.p) : nullptr;
   float *o = mg.buf0(st.o);
   auto go = [&](auto kern) {
-    kern<<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, mg.s32, mg.code, o, mg.partials,
-                                             st.c2, q, p, st.c1);
+    kern<<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, mg.lv[0].x, mg.code, o,
+                                             mg.partials, st.c2, q, p, st.c1, mg.sscale);
   };
   if (mg.cl == 4) go(k_mg_sweep_t<4, LAST, NT, CHEB>);
   else if (mg.cl == 2) go(k_mg_sweep_t<2, LAST, NT, CHEB>);
@@ -565,7 +571,7 @@ void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr
 int mg_first(const Multigrid &mg, double *r, const Scalars *proj) {
   const MgLevel &f0 = mg.lv[0];
   k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMt), 0, stream()>>>(
-      mg.g32, mg.code, r, f0.x, mg.s32, mg.first, proj);
+      mg.g32, mg.code, r, f0.x, nullptr, mg.first, proj);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
 }
@@ -578,10 +584,10 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
   TimedLaunch tl("mg_update");
   if (cl == 2)
     k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32, mg.s32);
+                                                  mg.q32, nullptr);
   else
     k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32, mg.s32);
+                                                  mg.q32, nullptr);
   FFPM_LAUNCHED("k_mg_upd_t");
   return OK;
 }
@@ -601,7 +607,7 @@ CoarseOut coarse_out(const Multigrid &mg, int l) {
 
 int mg_vcycle(const Multigrid &mg, bool with_q) {
   const dim3 blk(kMt);
-  const float *r = mg.s32;
+  const float *r = mg.lv[0].x;  // the source, read as x1 d sscale
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
   const MgSchedule &s0 = mg.s0, &sc = mg.sc;
@@ -619,7 +625,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     const size_t lb = mg_restrict_lds(mg.tlr.ty, mg.cl);
     auto go = [&](auto kern) {
       kern<<<mg.tlr.blocks, rb, lb, stream()>>>(mg.tlr, g, xm, r, mg.code, c.nx, c.ny,
-                                                coarse_out(mg, 1));
+                                                coarse_out(mg, 1), mg.sscale);
     };
     if (mg.nontemporal) {
       if (mg.cl == 4) go(k_mg_restrict_t<4, true>);
@@ -636,10 +642,10 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     const int kc = column_chunk(c.nx, c.ny, c.nz);
     if (f0.nx % 2 == 0)
       k_mgf_restrict_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, mg.buf0(s0.mid), c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
+          g, mg.code, r, mg.buf0(s0.mid), c.nx, c.ny, c.nz, kc, coarse_out(mg, 1), mg.sscale);
     else
       k_mgf_restrict_z<false><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
-          g, mg.code, r, mg.buf0(s0.mid), c.nx, c.ny, c.nz, kc, coarse_out(mg, 1));
+          g, mg.code, r, mg.buf0(s0.mid), c.nx, c.ny, c.nz, kc, coarse_out(mg, 1), mg.sscale);
   }
   FFPM_LAUNCHED("k_mgf_restrict_z");
   auto csweep = [&](const MgLevel &v, const MgStep &st) {

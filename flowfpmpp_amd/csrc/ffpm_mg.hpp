@@ -44,8 +44,9 @@ struct MgLevel {
 
 // One smoothing step of a level: iterate buffer a (and the previous iterate
 // p, -1 for none) into buffer o, out = a + c2 (b - A a) / d + c1 (a - p).
-// Level-0 buffers: 0 x, 1 y, 2 w, 3 the cycle's output z32; coarse: 0 x, 1 y,
-// 2 w.
+// Level-0 buffers: 0 x (the first step, kept through the cycle: it also
+// stands for the source), 1 y, 2 w, 3 the cycle's output z32, 4 v; coarse:
+// 0 x, 1 y, 2 w.
 struct MgStep {
   int a, p, o;
   float c1, c2;
@@ -108,11 +109,13 @@ struct Multigrid {
   MgLearned lrn;               // the learned coarse correction, when net is set
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
-  float *s32 = nullptr;        // the cycle's source r / V (level-0 cells)
+  float *v = nullptr;          // a fifth level-0 iterate buffer (index 4)
+  float sscale = 1.f;          // 1 / first: the cycle's source r / V is read as
+                               // lv[0].x d sscale (lv[0].x = first D^-1 r)
   float *z32 = nullptr;        // the cycle's output z = B r (level-0 cells)
   // the iterate buffers of level 0 (index as in MgStep)
   __host__ __device__ float *buf0(int i) const {
-    return i == 0 ? lv[0].x : i == 1 ? lv[0].y : i == 2 ? lv[0].w : z32;
+    return i == 0 ? lv[0].x : i == 1 ? lv[0].y : i == 2 ? lv[0].w : i == 3 ? z32 : v;
   }
   __host__ __device__ static float *bufc(const MgLevel &v, int i) {
     return i == 0 ? v.x : i == 1 ? v.y : v.w;
@@ -155,13 +158,13 @@ int mg_rows();
 // residual itself is formed only near convergence.
 int mg_check_interval(int check_interval);
 
-// The cycle's source s32 = r / V and the first smoothing step from zero,
-// lv[0].x = w D^-1 r. With proj non-null (the scalars of a true residual's
+// The first smoothing step from zero, lv[0].x = w D^-1 r (which also carries
+// the cycle's source r / V = lv[0].x d / w). With proj non-null (the scalars of a true residual's
 // first pass) r is first projected off the constants in place, r -= red1 /
 // red2 where diag > 0, which completes that true residual.
 int mg_first(const Multigrid &mg, double *r, const Scalars *proj = nullptr);
 
-// x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r, q32 = A p and s32 = r / V
+// x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r and q32 = A p
 // (alpha in sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
 // shape. The direction p is stored in FP32 (mg_direction rounds it) and
 // widened exactly; x, r and the arithmetic stay FP64. With rmax non-null, one
@@ -170,7 +173,7 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
               double *r, Scalars *sc, double *rmax = nullptr);
 
 // z32 = B r (the rest of the cycle after the first step, which must be in
-// lv[0].x with its source in s32). Leaves two slots of partial sums
+// lv[0].x). Leaves two slots of partial sums
 // (tl.blocks each) in partials: z.r, and z.q32 when with_q (else 0).
 int mg_vcycle(const Multigrid &mg, bool with_q);
 

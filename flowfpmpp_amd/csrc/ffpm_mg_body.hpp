@@ -130,7 +130,7 @@ __device__ __forceinline__ void mgf_jac0_body(Blk bk, GeomT<float> g,
                                    g.ct[t.k]);
   const float V = g.dx[t.i] * g.dy[t.j] * g.dz[t.k];
   const float sv = d > 0.f ? (float)rv / V : 0.f;
-  s32[t.c] = sv;
+  if (s32) s32[t.c] = sv;
   x[t.c] = d > 0.f ? omega * sv / d : 0.f;
 }
 
@@ -146,7 +146,8 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
                                                     const uint8_t *__restrict__ code,
                                                     const float *__restrict__ src,
                                                     const float *__restrict__ x, int cnx, int cny,
-                                                    int cnz, int kchunk, CoarseOut out) {
+                                                    int cnz, int kchunk, CoarseOut out,
+                                                    float sscale) {
   const int pc = bk.x * kMt + (int)threadIdx.x;  // coarse column in the plane
   const int J = pc / cnx, I = pc - J * cnx;
   if (J >= cny) return;
@@ -224,7 +225,7 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
         nb += (cd & kCodeB) ? cb * dn[b][a] : 0.f;
         nb += (cd & kCodeT) ? ct * up[b][a] : 0.f;
         const float d = diag_code<float>(cd, cw[a], ce[a], cs[b], cn[b], cb, ct);
-        s += dxa[a] * dyb[b] * dzk * (ss[a] - (d * c[b][a] - nb));
+        s += dxa[a] * dyb[b] * dzk * (ss[a] * d * sscale - (d * c[b][a] - nb));
       }
     }
     return s;
@@ -267,7 +268,7 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
                                                  const float *__restrict__ a,
                                                  const float *__restrict__ r,
                                                  const uint8_t *__restrict__ code, int cnx,
-                                                 int cny, CoarseOut out) {
+                                                 int cny, CoarseOut out, float sscale) {
   constexpr int RC = CL >= 2 ? CL / 2 : 1;  // coarse cells per lane
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const TileCoord tc = tile_coord(t, nx, ny, nz, block);
@@ -332,8 +333,8 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
       nb += (cd & kCodeB) ? cb * a_prev.v[m] : 0.f;
       nb += (cd & kCodeT) ? ct * u.a_next.v[m] : 0.f;
       const float d = diag_code<float>(cd, cw[m], ce[m], cs, cn, cb, ct);
-      con[m] = (cd & kCodeFluid) ? dxv[m] * dyj * dzk * (u.r.v[m] - (d * a_cur.v[m] - nb))
-                                 : 0.f;
+      const float src = u.r.v[m] * d * sscale;
+      con[m] = (cd & kCodeFluid) ? dxv[m] * dyj * dzk * (src - (d * a_cur.v[m] - nb)) : 0.f;
     }
     float px[RC];
     if constexpr (CL >= 2) {
@@ -661,6 +662,9 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
 //
 // CHEB: a Chebyshev step, out = a + c2 (s - A a) / d + c1 (a - prev) (prev
 // null: a zero previous iterate); otherwise damped Jacobi with omega = c2.
+// The source is read from the cycle's first step x1 = first D^-1 r (r holds
+// x1): s = x1 d sscale, sscale = 1 / first. When a is x1 itself (the step
+// after the first) the one load serves both.
 template <int CL, bool LAST, bool NT, bool CHEB = false>
 __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *lds, Tiling t,
                                               GeomT<float> g, const float *__restrict__ a,
@@ -668,8 +672,8 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
                                               const uint8_t *__restrict__ code,
                                               float *__restrict__ b, double *partials,
                                               float omega, const float *__restrict__ q,
-                                              const float *__restrict__ prev = nullptr,
-                                              float c1 = 0.f) {
+                                              const float *__restrict__ prev, float c1,
+                                              float sscale) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
   double s = 0.0, sq = 0.0;
@@ -702,11 +706,12 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
       Plane<kDirCurrent, float, CL> pl;
     };
     Set sa, sb;
+    const bool same = r == a;  // uniform
     V a_cur = ldv<float, CL>(a, at(k0));
     V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
     sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
     sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
-    sa.r = ldv_s<NT, float, CL>(r, at(k0));
+    if (!same) sa.r = ldv_s<NT, float, CL>(r, at(k0));
     if constexpr (CHEB) {
       if (prev) sa.p = ldv_s<NT, float, CL>(prev, at(k0));
     }
@@ -714,7 +719,7 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
       const int kn = min(k + 1, k1 - 1);
       ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
       ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
-      ld.r = ldv_s<NT, float, CL>(r, at(kn));
+      if (!same) ld.r = ldv_s<NT, float, CL>(r, at(kn));
       if constexpr (CHEB) {
         if (prev) ld.p = ldv_s<NT, float, CL>(prev, at(kn));
       }
@@ -733,7 +738,7 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
         const float east = m == CL - 1 ? (he ? ve : 0.f) : a_cur.v[m + 1];
         const float south = hs ? u.pl.s.a.v[m] : 0.f, north = hn ? u.pl.n.a.v[m] : 0.f;
         const float d = diag_code<float>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
-        const float src = u.r.v[m];
+        const float src = (same ? a_cur.v[m] : u.r.v[m]) * d * sscale;
         const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
                          cb * a_prev.v[m] + ct * u.a_next.v[m];
         float o;
@@ -747,7 +752,7 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
         bv.v[m] = o;
         if constexpr (LAST) {
           const double zd = (double)o;
-          s += zd * ((double)u.r.v[m] * (double)(dxdy[m] * dzk));
+          s += zd * ((double)src * (double)(dxdy[m] * dzk));
           if (q) sq += zd * (double)qz.v[m];
         }
       }
