@@ -64,6 +64,19 @@ __global__ __launch_bounds__(256) void k_mgc_sweep(int nx, int ny, int nz,
   mgc_sweep_body(blk3(), nx, ny, nz, gE, gN, gT, b, in, prev, out, c1, c2);
 }
 
+template <int CL>
+__global__ __launch_bounds__(256) void k_mgc_sweep_t(Tiling t, int nx, int ny, int nz,
+                                                     const float *__restrict__ gE,
+                                                     const float *__restrict__ gN,
+                                                     const float *__restrict__ gT,
+                                                     const float *__restrict__ b,
+                                                     const float *__restrict__ in,
+                                                     const float *__restrict__ prev,
+                                                     float *__restrict__ out, float c1,
+                                                     float c2) {
+  mgc_sweep_t_body<CL>(blockIdx.x, t, nx, ny, nz, gE, gN, gT, b, in, prev, out, c1, c2);
+}
+
 __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
                                                       const float *__restrict__ gE,
                                                       const float *__restrict__ gN,
@@ -401,6 +414,17 @@ int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
     mg.lv[1].nx = (nx + 1) / 2; mg.lv[1].ny = (ny + 1) / 2; mg.lv[1].nz = (nz + 1) / 2;
     mg.lv[1].n = (size_t)mg.lv[1].nx * mg.lv[1].ny * mg.lv[1].nz;
   }
+  // the streaming sweep on the coarse levels that fill the chip (4-row tiles
+  // of 64 lanes; FFPM_MGC_STREAM=0: the one-cell sweep everywhere)
+  const bool cstream = env_float("FFPM_MGC_STREAM", 1.f) != 0.f;
+  for (int l = 1; l <= L; ++l) {
+    MgLevel &v = mg.lv[l];
+    v.cl = 0;
+    if (cstream && l < L && v.n >= kMgStreamCells) {
+      v.cl = cells_per_lane(v.nx, 4);
+      v.tl = make_tiling(v.nx, v.ny, v.nz, 0, 4, kTileX * v.cl);
+    }
+  }
   if ((size_t)mg.lv[L].n > (size_t)kMgCoarsestCells) {
     set_error(ERR_SHAPE, "multigrid hierarchy does not reach a one-workgroup level");
     return ERR_SHAPE;
@@ -649,9 +673,20 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
   }
   FFPM_LAUNCHED("k_mgf_restrict_z");
   auto csweep = [&](const MgLevel &v, const MgStep &st) {
-    k_mgc_sweep<<<cell_grid(v.nx, v.ny, v.nz), blk, 0, stream()>>>(
-        v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, Multigrid::bufc(v, st.a),
-        st.p >= 0 ? Multigrid::bufc(v, st.p) : nullptr, Multigrid::bufc(v, st.o), st.c1, st.c2);
+    const float *a = Multigrid::bufc(v, st.a), *p = st.p >= 0 ? Multigrid::bufc(v, st.p) : nullptr;
+    float *o = Multigrid::bufc(v, st.o);
+    if (v.cl) {
+      auto go = [&](auto kern) {
+        kern<<<v.tl.blocks, dim3(kTileX, v.tl.ty), 0, stream()>>>(
+            v.tl, v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, a, p, o, st.c1, st.c2);
+      };
+      if (v.cl == 4) go(k_mgc_sweep_t<4>);
+      else if (v.cl == 2) go(k_mgc_sweep_t<2>);
+      else go(k_mgc_sweep_t<1>);
+    } else {
+      k_mgc_sweep<<<cell_grid(v.nx, v.ny, v.nz), blk, 0, stream()>>>(
+          v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, a, p, o, st.c1, st.c2);
+    }
   };
   // The learned correction either replaces everything below its level (the
   // cycle stops descending there) or, in augment mode, adds its A-optimal

@@ -40,7 +40,13 @@ struct MgLevel {
   float *rd = nullptr;                                // coarse levels: first-step scale / diag
   float *x = nullptr, *y = nullptr, *w = nullptr;     // iterate buffers
   float *b = nullptr;                                 // coarse levels only
+  // coarse levels of at least kMgStreamCells cells: the streaming sweep's
+  // launch shape and cells per lane (cl = 0: the one-cell sweep)
+  Tiling tl{};
+  int cl = 0;
 };
+
+constexpr size_t kMgStreamCells = 1u << 18;
 
 // One smoothing step of a level: iterate buffer a (and the previous iterate
 // p, -1 for none) into buffer o, out = a + c2 (b - A a) / d + c1 (a - p).

@@ -538,6 +538,93 @@ __device__ __forceinline__ void mgc_sweep_body(Blk bk, int nx, int ny, int nz,
   out[t.c] = v;
 }
 
+// The coarse-level smoothing step of mgc_sweep_body in the streaming scheme
+// of the level-0 kernels, for the levels large enough to fill the chip
+// (MgLevel::tl): a lane owns CL x-adjacent cells of a row (16-byte loads for
+// CL = 4) and walks its z range with the planes below and above in registers
+// (x, and gT of the plane below); the next plane's loads are issued before
+// the current one is computed; west/east values by wave shuffle. The
+// neighbour values, clamped at the walls, and the order of every sum are
+// those of coarse_row and mgc_sweep_body, so both give the same bits.
+template <int CL>
+__device__ __forceinline__ void mgc_sweep_t_body(int block, const Tiling &t, int nx, int ny,
+                                                 int nz, const float *__restrict__ gE,
+                                                 const float *__restrict__ gN,
+                                                 const float *__restrict__ gT,
+                                                 const float *__restrict__ b,
+                                                 const float *__restrict__ in,
+                                                 const float *__restrict__ prev,
+                                                 float *__restrict__ out, float c1, float c2) {
+  const Lane l = lane_of<CL>(t, nx, ny, nz, block);
+  if (!l.active) return;
+  using V = Vec<float, CL>;
+  const int i = l.i, j = l.j, lane = (int)threadIdx.x;
+  const size_t nxy = (size_t)nx * ny;
+  const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
+  const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
+  const ptrdiff_t ow = lane == 0 && hw ? -1 : 0;
+  const ptrdiff_t oe = lane == kTileX - 1 ? (he ? CL : CL - 1) : 0;
+  const size_t base = (size_t)i + (size_t)nx * (size_t)j;
+  const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
+  auto at = [&](int k) { return base + nxy * (size_t)k; };
+  struct Set {
+    V x_next, xs, xn, bb, pp, ge, gn, gt, gs;
+    float xw, xe, gw;
+  };
+  auto load = [&](Set &s, int k, int knext) {
+    const size_t c = at(k);
+    s.x_next = ldv<float, CL>(in, at(knext));
+    s.xs = ldv<float, CL>(in, c + os);
+    s.xn = ldv<float, CL>(in, c + on);
+    s.xw = in[c + ow];
+    s.xe = in[c + oe];
+    s.bb = ldv<float, CL>(b, c);
+    if (prev) s.pp = ldv<float, CL>(prev, c);
+    s.ge = ldv<float, CL>(gE, c);
+    s.gn = ldv<float, CL>(gN, c);
+    s.gt = ldv<float, CL>(gT, c);
+    s.gs = ldv<float, CL>(gN, c + os);
+    s.gw = gE[c + ow];
+  };
+  Set sa, sb;
+  V x_cur = ldv<float, CL>(in, at(k0));
+  V x_prev = k0 > 0 ? ldv<float, CL>(in, at(k0 - 1)) : x_cur;
+  V gt_prev = k0 > 0 ? ldv<float, CL>(gT, at(k0 - 1)) : x_cur;
+  load(sa, k0, min(k0 + 1, last));
+  auto stage = [&](int k, const Set &u, Set &ld) {
+    load(ld, min(k + 1, k1 - 1), min(k + 2, last));
+    const float from_w = __shfl_up(x_cur.v[CL - 1], 1, kTileX);
+    const float from_e = __shfl_down(x_cur.v[0], 1, kTileX);
+    const float gfrom_w = __shfl_up(u.ge.v[CL - 1], 1, kTileX);
+    V o;
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+      const float xv = x_cur.v[m];
+      const float xw = m > 0 ? x_cur.v[m - 1] : hw ? (lane == 0 ? u.xw : from_w) : xv;
+      const float xe = m < CL - 1 ? x_cur.v[m + 1] : he ? (lane == kTileX - 1 ? u.xe : from_e)
+                                                        : xv;
+      const float xs = hs ? u.xs.v[m] : xv, xn = hn ? u.xn.v[m] : xv;
+      const float xb = k > 0 ? x_prev.v[m] : xv, xt = k < nz - 1 ? u.x_next.v[m] : xv;
+      const float ge = u.ge.v[m], gn = u.gn.v[m], gt = u.gt.v[m];
+      const float gw = m > 0 ? u.ge.v[m - 1] : hw ? (lane == 0 ? u.gw : gfrom_w) : 0.f;
+      const float gs = hs ? u.gs.v[m] : 0.f, gb = k > 0 ? gt_prev.v[m] : 0.f;
+      const float d = gw + ge + gs + gn + gb + gt;
+      const float nb = gw * xw + ge * xe + gs * xs + gn * xn + gb * xb + gt * xt;
+      float v = d > 0.f ? xv + c2 * (u.bb.v[m] + nb - d * xv) / d : 0.f;
+      if (c1 != 0.f && d > 0.f) v += c1 * (xv - (prev ? u.pp.v[m] : 0.f));
+      o.v[m] = v;
+    }
+    stv<float, CL>(out, at(k), o);
+    x_prev = x_cur;
+    x_cur = u.x_next;
+    gt_prev = u.gt;
+  };
+  for (int k = k0; k < k1; k += 2) {
+    stage(k, sa, sb);
+    if (k + 1 < k1) stage(k + 1, sb, sa);
+  }
+}
+
 __device__ __forceinline__ void mgc_restrict_body(Blk bk, int nx, int ny, int nz,
                                                   const float *__restrict__ gE,
                                                   const float *__restrict__ gN,
