@@ -234,15 +234,17 @@ def test_streaming_and_column_restriction_agree(monkeypatch, shape):
         assert rel <= 1e-10
 
 
-def test_streaming_coarse_sweep_is_bit_identical(monkeypatch):
+@pytest.mark.parametrize("shape", [(192, 160, 96), (204, 150, 100), (206, 160, 96)])
+def test_streaming_coarse_sweep_is_bit_identical(monkeypatch, shape):
     """The coarse levels of at least 2^18 cells smooth with the streaming
     sweep (the default) or the one-cell sweep (FFPM_MGC_STREAM=0): the same
     arithmetic per cell, so the solves are bit-identical (fields, iteration
-    counts, residuals) on a 192x160x96 frame, whose level 1 (96x80x48 =
-    368,640 cells) uses it."""
-    print("case: fcg-mg streaming vs one-cell coarse sweep on 192x160x96", flush=True)
-    p = W.stretched_frame(192, 160, 96, [W.Sphere((-1.0, 0.0, 0.0), 0.7),
-                                         W.Box((2.0, -1.0, -0.7), (3.5, 1.0, 0.7))], seed=16,
+    counts, residuals). Level 1 of these frames (96x80x48, 102x75x50,
+    103x80x48) has 4, 2 and 1 cells per lane."""
+    nx, ny, nz = shape
+    print(f"case: fcg-mg streaming vs one-cell coarse sweep on {nx}x{ny}x{nz}", flush=True)
+    p = W.stretched_frame(nx, ny, nz, [W.Sphere((-1.0, 0.0, 0.0), 0.7),
+                                       W.Box((2.0, -1.0, -0.7), (3.5, 1.0, 0.7))], seed=16,
                           with_fields=False)
     out = {}
     for mode in ("1", "0"):
