@@ -169,6 +169,65 @@ def cpu_baseline(p, gpu_iters, tol, method, budget_s=20.0, parallel=False):
     }
 
 
+def cpu_time_to_solution_c1(tol, check):
+    """A measured (not scaled) CPU baseline: the 128x96x64 frame's three
+    directions solved to the tolerance, compatibility included, by the CPU
+    oracle's FP64 diagonal PCG on every thread OMP_NUM_THREADS allows, timed
+    end to end; beside it the same three solves on the GPU with FP64 PCG and
+    with fcg-mg (inputs resident in HBM, phi from zero). Iterations and
+    residuals of each are reported."""
+    from oracle import oracle as orc
+    from flowfpmpp_amd import stairstep as S
+    from flowfpmpp_amd import workloads as W
+    from flowfpmpp_amd.device import DeviceArray, synchronize
+    T = DeviceArray.from_host
+    p = W.frame_128x96x64(with_fields=False)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    inputs = [W.influence_inputs(p, d) for d in range(3)]
+    cpu_it, cpu_res = [], []
+    t0 = time.perf_counter()
+    for rhs, bg, faces in inputs:
+        rhs, faces = rhs.copy(order="F"), [f.copy(order="F") for f in faces]
+        orc.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
+        phi = p.phi0()
+        it, res = orc.solve_stairstep_poisson_3d(phi, rhs, p.solid, bg, *faces, p.xc, p.yc,
+                                                 p.zc, p.dx, p.dy, p.dz, tol, 100000, check,
+                                                 "pcg", parallel=True)
+        cpu_it.append(int(it))
+        cpu_res.append(float(res))
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    solid = T(p.solid)
+    axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+    dev = [(T(rhs), bg, [T(f) for f in faces]) for rhs, bg, faces in inputs]
+    out = {"cpu": {"ms": round(cpu_ms, 1), "iterations": cpu_it, "residuals": cpu_res,
+                   "cores": threads, "kind": "port",
+                   "sample": f"CPU oracle FP64 diagonal PCG (gcc -O2, OpenMP over z planes, "
+                             f"{threads} threads; {cpu_model()}), compatibility + solve of the "
+                             f"three directions, measured end to end"}}
+    for method in ("pcg", "fcg-mg"):
+        times, its, rs = [], [], []
+        for rep in range(2):  # the first run includes the library's setup
+            work = [(T(r.download()), bg, [T(f.download()) for f in fs]) for r, bg, fs in dev]
+            phis = [T(p.phi0()) for _ in range(3)]
+            synchronize()
+            t0 = time.perf_counter()
+            its, rs = [], []
+            for (r, bg, fs), ph in zip(work, phis):
+                S.enforce_stairstep_compatibility_3d(r, solid, bg, *fs, *axes[3:])
+                it, res = S.solve_stairstep_poisson_3d(ph, r, solid, bg, *fs, *axes, tol, 100000,
+                                                       check, method=method)
+                its.append(int(it))
+                rs.append(float(res))
+            synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        out["gpu_" + method.replace("-", "_")] = {"ms": round(times[-1], 3), "iterations": its,
+                                                  "residuals": rs,
+                                                  "speedup_vs_cpu": round(cpu_ms / times[-1], 1)}
+    out["workload"] = ("128x96x64 frame (seed 1, sphere + box), 3 directions: compatibility + "
+                       f"solve to {tol:g} at check interval {check}, phi from zero")
+    return out
+
+
 def plain_stencil(p, args, pcg_kern, d_rhs, d_solid, axes, faces, phi, bg, sweeps=200):
     """The plain FP64 stencil kernels on this frame against the HBM peak: the
     fused PCG iteration's two kernels from the FP64 PCG step just run, and
@@ -955,6 +1014,10 @@ def main():
         else:
             log("timing the CPU oracle baseline")
             out["cpu_baseline"] = cpu_baseline(p, pcg_iters, args.tol, cm)
+        if cm == "pcg" and tuple(args.grid) == (512, 384, 256):
+            log("measured CPU and GPU time to solution on the 128x96x64 frame")
+            out["cpu_time_to_solution_128x96x64"] = cpu_time_to_solution_c1(args.tol,
+                                                                            args.check)
     print(json.dumps(out), flush=True)
 
 
