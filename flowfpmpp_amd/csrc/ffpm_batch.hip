@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid gri
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   mgk::mgf_jac0_body(cell_blk(mg.lv[0], grid.local[blockIdx.x]), mg.g32, mg.code, d.r,
-                     mg.lv[0].x, nullptr, mg.first, d.sc);
+                     mg.lv[0].x, mg.first, d.sc);
 }
 
 // One level-0 smoothing step (MgStep: buffers ia, ip, io, coefficients);
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid
   double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
   mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g,
                        reinterpret_cast<const float *>(d.p[st.cur]), d.code, d.x, d.r,
-                       mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32, nullptr);
+                       mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32);
 }
 
 // One workgroup per solve: the finish of the preceding reduction for the

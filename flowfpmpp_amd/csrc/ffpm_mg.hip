@@ -18,9 +18,9 @@ using namespace mgk;
 // ---- standalone kernels: one solve per launch --------------------------------
 __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t *__restrict__ code,
                                                   double *__restrict__ r,
-                                                  float *__restrict__ x, float *__restrict__ s32,
-                                                  float omega, const Scalars *proj) {
-  mgf_jac0_body(blk3(), g, code, r, x, s32, omega, proj);
+                                                  float *__restrict__ x, float omega,
+                                                  const Scalars *proj) {
+  mgf_jac0_body(blk3(), g, code, r, x, omega, proj);
 }
 
 template <bool VEC>
@@ -127,11 +127,9 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    float *__restrict__ x1, Scalars *sc,
                                                    double omega, double *rmax,
-                                                   float *__restrict__ q32,
-                                                   float *__restrict__ s32) {
+                                                   float *__restrict__ q32) {
   __shared__ double lds[16 * kReduceSlots];
-  mg_upd_body<CL>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32,
-                  s32);
+  mg_upd_body<CL>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32);
 }
 
 // The flexible CG's direction step on the cycle's FP32 output (the PCG
@@ -595,7 +593,7 @@ void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr
 int mg_first(const Multigrid &mg, double *r, const Scalars *proj) {
   const MgLevel &f0 = mg.lv[0];
   k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMt), 0, stream()>>>(
-      mg.g32, mg.code, r, f0.x, nullptr, mg.first, proj);
+      mg.g32, mg.code, r, f0.x, mg.first, proj);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
 }
@@ -608,10 +606,10 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
   TimedLaunch tl("mg_update");
   if (cl == 2)
     k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32, nullptr);
+                                                  mg.q32);
   else
     k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32, nullptr);
+                                                  mg.q32);
   FFPM_LAUNCHED("k_mg_upd_t");
   return OK;
 }

@@ -108,16 +108,16 @@ struct CoarseOut {
 };
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
-// The cycle's source s = r / V in FP32 (0 where d = 0) and the first sweep
-// from zero, x = omega D^-1 r = omega s / d; D = V diag. With proj non-null
+// The first sweep from zero, x = omega D^-1 r = omega s / d (D = V diag,
+// s = r / V rounded to FP32, 0 where d = 0), which the rest of the cycle also
+// reads as its source s = x d / omega. With proj non-null
 // the true residual's projection (src/fpm_stairstep_mpi_3d.f90:319-322) is
 // applied first: r -= red1 / red2 in the cells with a fluid neighbour (those
 // whose diag is positive), and the projected r is stored.
 __device__ __forceinline__ void mgf_jac0_body(Blk bk, GeomT<float> g,
                                               const uint8_t *__restrict__ code,
                                               double *__restrict__ r, float *__restrict__ x,
-                                              float *__restrict__ s32, float omega,
-                                              const Scalars *proj = nullptr) {
+                                              float omega, const Scalars *proj = nullptr) {
   const Idx t = cell_idx(g.nx, g.ny, bk);
   if (!t.in) return;
   const unsigned cd = code[t.c];
@@ -130,7 +130,6 @@ __device__ __forceinline__ void mgf_jac0_body(Blk bk, GeomT<float> g,
                                    g.ct[t.k]);
   const float V = g.dx[t.i] * g.dy[t.j] * g.dz[t.k];
   const float sv = d > 0.f ? (float)rv / V : 0.f;
-  if (s32) s32[t.c] = sv;
   x[t.c] = d > 0.f ? omega * sv / d : 0.f;
 }
 
@@ -742,7 +741,8 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
 // reference-Jacobi kernel k_jacobi_t: each lane owns CL x-adjacent cells of
 // one row and walks its z range; plane k+1's loads are issued before plane k
 // is computed; west/east neighbours by wave shuffle):
-//   out = (1 - w) a + (w / d) (s + sum c a_nb), 0 where d = 0, s = r / V.
+//   out = (1 - w) a + (w / d) (s + sum c a_nb), 0 where d = 0, s = r / V
+// (read from the first step, see below).
 // LAST: out is the cycle's output z (FP32; the FP64 iteration widens it
 // exactly), and the partial sums z.r = z.(s V) and z.q go to two slots (q = A
 // p_old from the previous update, FP32, or z.q = 0 when q is null).
@@ -859,21 +859,20 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
   }
 }
 
-// x += alpha p, r -= alpha A p (the fused PCG update body), the first
-// smoothing step of the next cycle, x1 = w D^-1 r, q = A p and the cycle's
-// source s = r / V, stored in FP32; with rmax the per-workgroup max |r / V|;
-// counts the iteration.
+// x += alpha p, r -= alpha A p (the fused PCG update body, p in FP32), the
+// first smoothing step of the next cycle, x1 = w D^-1 r (which also carries
+// the cycle's source), and q = A p, both stored in FP32; with rmax the
+// per-workgroup max |r / V|; counts the iteration.
 template <int CL>
 __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds, Tiling t,
                                             GeomT<double> g, const float *__restrict__ p,
                                             const uint8_t *__restrict__ code,
                                             double *__restrict__ x, double *__restrict__ r,
                                             float *__restrict__ x1, Scalars *sc, double omega,
-                                            double *rmax, float *__restrict__ q32,
-                                            float *__restrict__ s32) {
+                                            double *rmax, float *__restrict__ q32) {
   if (sc->status != 0) return;
   pcg_upd_body<double, CL, float, false, float>(t, block, nblocks, g, p, code, x, r, x1,
-                                                nullptr, sc, lds, omega, rmax, q32, s32);
+                                                nullptr, sc, lds, omega, rmax, q32);
   if (block == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
 }
 

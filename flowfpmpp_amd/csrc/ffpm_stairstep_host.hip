@@ -1064,8 +1064,9 @@ int compat_dev(const double *rhs, const int32_t *solid, const double *bg, double
   k_shift_faces<<<elementwise_blocks(nf), kThreads, 0, stream()>>>(nx, ny, nz, xlo, xhi, ylo,
                                                                    yhi, zlo, zhi, c);
   FFPM_LAUNCHED("k_shift_faces");
-  k_compat_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
-                                               bg[0], bg[1], bg[2], f.partials, f.sc);
+  // the flux after the correction (the source integral is kept)
+  k_compat_sums<<<eb, kThreads, 0, stream()>>>(f.g, solid, nullptr, xlo, xhi, ylo, yhi, zlo,
+                                               zhi, bg[0], bg[1], bg[2], f.partials, f.sc);
   FFPM_LAUNCHED("k_compat_sums");
   finish(f, FIN_SUMS, 3, eb);
   if ((st = read_scalars(f, &h))) return st;

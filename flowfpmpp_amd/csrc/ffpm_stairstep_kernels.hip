@@ -240,7 +240,9 @@ __global__ void k_gauge_apply(int nx, int ny, int nz, const int32_t *__restrict_
 
 // Flux balance of src/fpm_stairstep_mpi_3d.f90:49-52,69-91: optional fluid
 // source integral, net outward flux through fluid outer faces and body faces,
-// and fluid outer-face area -> sc->sums[0..2] (source, flux, area).
+// and fluid outer-face area -> sc->sums[0..2] (source, flux, area). rhs null:
+// the source is not formed (sums[0] = 0) and rhs is not read, as in the
+// recomputation of the flux after the correction.
 __global__ __launch_bounds__(512) void k_compat_sums(
     Geom g, const int32_t *__restrict__ solid, const double *__restrict__ rhs,
     const double *__restrict__ xlo, const double *__restrict__ xhi,
@@ -262,10 +264,10 @@ __global__ __launch_bounds__(512) void k_compat_sums(
     const int32_t s0 = solid[pc], sw = solid[pc - 1], se = solid[pc + 1],
                   ss = solid[pc - (nx + 2)], sn = solid[pc + (nx + 2)],
                   sb = k > 0 ? solid[pc - pxy] : 0, st = k < nz - 1 ? solid[pc + pxy] : 0;
-    const double rv = rhs[c];
+    const double rv = rhs ? rhs[c] : 0.0;
     const double dxi = g.dx[i], dyj = g.dy[j], dzk = g.dz[k];
     if (s0 != 0) continue;
-    s[0] += rv * dxi * dyj * dzk;
+    if (rhs) s[0] += rv * dxi * dyj * dzk;
     double f = 0.0, ar = 0.0, a;
     a = dyj * dzk;
     if (I == 1) { f = f - xlo[j + (size_t)ny * k] * a; ar = ar + a; }
