@@ -28,6 +28,7 @@ struct Scalars {
   double sums[4];     // generic reduction results
   int status;         // 0, kBreakdown or kNegative
   int iters;          // iterations completed since the last chunk start
+  double alpha_pend;  // step length of a direction whose x update is deferred
 };
 
 // Finish operations of k_finish (what the partials of the preceding

@@ -120,16 +120,18 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                     c2, q, prev, c1, sscale);
 }
 
-template <int CL>
+template <int CL, int XM>
 __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    const float *__restrict__ p,
                                                    const uint8_t *__restrict__ code,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    float *__restrict__ x1, Scalars *sc,
                                                    double omega, double *rmax,
-                                                   float *__restrict__ q32) {
+                                                   float *__restrict__ q32,
+                                                   const float *__restrict__ ppend) {
   __shared__ double lds[16 * kReduceSlots];
-  mg_upd_body<CL>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32);
+  mg_upd_body<CL, XM>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32,
+                      ppend);
 }
 
 // The flexible CG's direction step on the cycle's FP32 output (the PCG
@@ -601,17 +603,28 @@ int mg_first(const Multigrid &mg, double *r, const Scalars *proj) {
 }
 
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, double *x,
-              double *r, Scalars *sc, double *rmax) {
+              double *r, Scalars *sc, double *rmax, int xmode, const float *ppend) {
   const dim3 blk(kTileX, t.ty);
   const double w = (double)mg.first;
   float *x1 = mg.lv[0].x;
+  if (xmode == XM_TWO && !ppend) {
+    set_error(ERR_ARGUMENT, "multigrid update: the deferred step needs its direction");
+    return ERR_ARGUMENT;
+  }
   TimedLaunch tl("mg_update");
-  if (cl == 2)
-    k_mg_upd_t<2><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32);
-  else
-    k_mg_upd_t<1><<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax,
-                                                  mg.q32);
+  auto go = [&](auto kern) {
+    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax, mg.q32,
+                                         ppend);
+  };
+  if (cl == 2) {
+    if (xmode == XM_DEFER) go(k_mg_upd_t<2, XM_DEFER>);
+    else if (xmode == XM_TWO) go(k_mg_upd_t<2, XM_TWO>);
+    else go(k_mg_upd_t<2, XM_NOW>);
+  } else {
+    if (xmode == XM_DEFER) go(k_mg_upd_t<1, XM_DEFER>);
+    else if (xmode == XM_TWO) go(k_mg_upd_t<1, XM_TWO>);
+    else go(k_mg_upd_t<1, XM_NOW>);
+  }
   FFPM_LAUNCHED("k_mg_upd_t");
   return OK;
 }

@@ -174,9 +174,14 @@ int mg_first(const Multigrid &mg, double *r, const Scalars *proj = nullptr);
 // (alpha in sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
 // shape. The direction p is stored in FP32 (mg_direction rounds it) and
 // widened exactly; x, r and the arithmetic stay FP64. With rmax non-null, one
-// partial max |r / V| per workgroup (t.blocks) goes there.
+// partial max |r / V| per workgroup (t.blocks) goes there. xmode XM_DEFER
+// leaves x alone and keeps alpha in sc->alpha_pend; the next update, XM_TWO
+// with ppend the deferred direction, applies both steps in order (after a
+// breakdown only the deferred one), so x ends bit-identical to XM_NOW at
+// every iteration while the skipped iteration neither reads nor writes it.
 int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, double *x,
-              double *r, Scalars *sc, double *rmax = nullptr);
+              double *r, Scalars *sc, double *rmax = nullptr, int xmode = XM_NOW,
+              const float *ppend = nullptr);
 
 // z32 = B r (the rest of the cycle after the first step, which must be in
 // lv[0].x). Leaves two slots of partial sums

@@ -863,17 +863,26 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
 // first smoothing step of the next cycle, x1 = w D^-1 r (which also carries
 // the cycle's source), and q = A p, both stored in FP32; with rmax the
 // per-workgroup max |r / V|; counts the iteration.
-template <int CL>
+// XM (XM_NOW, XM_DEFER, XM_TWO) and ppend: the treatment of x (see
+// pcg_upd_body); a deferring update keeps its alpha in sc->alpha_pend for the
+// next one, which applies it with p_pend.
+template <int CL, int XM = XM_NOW>
 __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds, Tiling t,
                                             GeomT<double> g, const float *__restrict__ p,
                                             const uint8_t *__restrict__ code,
                                             double *__restrict__ x, double *__restrict__ r,
                                             float *__restrict__ x1, Scalars *sc, double omega,
-                                            double *rmax, float *__restrict__ q32) {
-  if (sc->status != 0) return;
-  pcg_upd_body<double, CL, float, false, float>(t, block, nblocks, g, p, code, x, r, x1,
-                                                nullptr, sc, lds, omega, rmax, q32);
-  if (block == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc->iters += 1;
+                                            double *rmax, float *__restrict__ q32,
+                                            const float *__restrict__ ppend = nullptr) {
+  const bool broke = sc->status != 0;
+  pcg_upd_body<double, CL, float, false, float, XM>(t, block, nblocks, g, p, code, x, r, x1,
+                                                    nullptr, sc, lds, omega, rmax, q32,
+                                                    nullptr, ppend);
+  if (block == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    // a deferring iteration that broke down leaves no step behind
+    if constexpr (XM == XM_DEFER) sc->alpha_pend = broke ? 0.0 : sc->alpha;
+    if (!broke) sc->iters += 1;
+  }
 }
 
 // ---- setup --------------------------------------------------------------------

@@ -18,6 +18,13 @@ inline GeomT<double> geom_t(const Geom &g) {
   return GeomT<double>{g.nx, g.ny, g.nz, g.cw, g.ce, g.cs, g.cn, g.cb, g.ct, g.dx, g.dy, g.dz};
 }
 
+// How an update kernel treats the iterate x: XM_NOW, x += alpha p; XM_DEFER,
+// x is neither read nor written (its step alpha p is applied with the next
+// direction's); XM_TWO, x = (x + alpha_pend p_pend) + alpha p, the deferred
+// step and this one in the order two updates take them (so x is
+// bit-identical to updating it at every iteration).
+constexpr int XM_NOW = 0, XM_DEFER = 1, XM_TWO = 2;
+
 // MODE is kDirUpdate, kDirReset or kDirCurrent.
 template <typename T, int CL, int MODE>
 __global__ void k_pcg_dir_t(Tiling t, GeomT<T> g, const T *z, const uint8_t *code, const T *pin,

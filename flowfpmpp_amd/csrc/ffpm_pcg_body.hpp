@@ -336,7 +336,8 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 // cells there (an estimate of the max-norm residual), with qout non-null
 // q = A p is stored in ZT (for the flexible-CG coefficient of the next step),
 // and with sout non-null r / V (0 outside fluid cells) is stored in ZT.
-template <typename T, int CL, typename ZT = T, bool DOT = true, typename PT = T>
+template <typename T, int CL, typename ZT = T, bool DOT = true, typename PT = T,
+          int XM = XM_NOW>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const PT *__restrict__ p,
                                              const uint8_t *__restrict__ code,
@@ -345,9 +346,30 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
                                              const Scalars *sc, double *lds,
                                              T zscale = T(1), double *rmax = nullptr,
                                              ZT *__restrict__ qout = nullptr,
-                                             ZT *__restrict__ sout = nullptr) {
-  if (sc->status != 0) return;
+                                             ZT *__restrict__ sout = nullptr,
+                                             const PT *__restrict__ ppend = nullptr) {
+  if (sc->status != 0) {
+    if constexpr (XM == XM_TWO) {
+      // the iteration broke down: only the deferred step is applied (none
+      // when the deferring iteration broke down too: alpha_pend = 0)
+      const T ap = (T)sc->alpha_pend;
+      const Lane l = lane_of<CL>(t, g.nx, g.ny, g.nz, block);
+      if (l.active && ap != T(0)) {
+        const size_t nxy = (size_t)g.nx * g.ny;
+        for (int k = l.k0; k < l.k1; ++k) {
+          const size_t c = (size_t)l.i + (size_t)g.nx * (size_t)l.j + nxy * (size_t)k;
+          Vec<T, CL> xv = ldv<T, CL>(x, c);
+          const Vec<T, CL> pp = convert<T>(ldv<PT, CL>(ppend, c));
+#pragma unroll
+          for (int m = 0; m < CL; ++m) xv.v[m] = xv.v[m] + ap * pp.v[m];
+          stv<T, CL>(x, c, xv);
+        }
+      }
+    }
+    return;
+  }
   const T alpha = (T)sc->alpha;
+  const T alpha_pend = XM == XM_TWO ? (T)sc->alpha_pend : T(0);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
   double s = 0.0, rmx = 0.0;
@@ -380,7 +402,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
     // p may be stored narrower (PT): it is widened where it is used, so the
     // loads stay in flight across the computation of a plane.
     struct Set {
-      Vec<PT, CL> p_next;
+      Vec<PT, CL> p_next, pp;
       Vec<T, CL> x, r;
       Plane<kDirCurrent, PT, CL> pl;
     };
@@ -389,13 +411,15 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
     Vec<T, CL> p_prev = k0 > 0 ? convert<T>(ldv<PT, CL>(p, at(k0 - 1))) : p_cur;
     sa.p_next = ldv<PT, CL>(p, at(min(k0 + 1, last)));
     sa.pl.load(p, p, code, at(k0), os, on, ow, oe);
-    sa.x = ldv<T, CL>(x, at(k0));
+    if constexpr (XM != XM_DEFER) sa.x = ldv<T, CL>(x, at(k0));
+    if constexpr (XM == XM_TWO) sa.pp = ldv<PT, CL>(ppend, at(k0));
     sa.r = ldv<T, CL>(r, at(k0));
     auto stage = [&](int k, const Set &u, Set &ld) {
       const int kn = min(k + 1, k1 - 1);
       ld.p_next = ldv<PT, CL>(p, at(min(k + 2, last)));
       ld.pl.load(p, p, code, at(kn), os, on, ow, oe);
-      ld.x = ldv<T, CL>(x, at(kn));
+      if constexpr (XM != XM_DEFER) ld.x = ldv<T, CL>(x, at(kn));
+      if constexpr (XM == XM_TWO) ld.pp = ldv<PT, CL>(ppend, at(kn));
       ld.r = ldv<T, CL>(r, at(kn));
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
       Vec<T, CL> xv = u.x, rv = u.r;
@@ -415,6 +439,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
                      (d * p_cur.v[m] - cw[m] * west - ce[m] * east - cs * south - cn * north -
                       cb * p_prev.v[m] - ct * p_next.v[m]);
         const T q = d > 0 ? qr : T(0);
+        if constexpr (XM == XM_TWO) xv.v[m] = xv.v[m] + alpha_pend * (T)u.pp.v[m];
         xv.v[m] = xv.v[m] + alpha * p_cur.v[m];
         rv.v[m] = rv.v[m] - alpha * q;
         if constexpr (DOT) {
@@ -429,7 +454,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       }
       // the multigrid variant writes its outputs with nontemporal stores:
       // none is read again by this kernel
-      stv_s<!DOT, T, CL>(x, at(k), xv);
+      if constexpr (XM != XM_DEFER) stv_s<!DOT, T, CL>(x, at(k), xv);
       stv_s<!DOT, T, CL>(r, at(k), rv);
       stv_s<!DOT, ZT, CL>(z, at(k), zv);
       if constexpr (!DOT) {

@@ -860,6 +860,9 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   g_mg = MgStats();
   g_mg.levels = mg.levels;
   const int ci = mg_check_interval(check_interval);
+  // FFPM_MG_DEFER_X=0: x += alpha p at every iteration
+  const char *dx_env = std::getenv("FFPM_MG_DEFER_X");
+  const bool defer_x = !(dx_env && std::strcmp(dx_env, "0") == 0);
   // the directions, stored in FP32 (in the FP64 buffers' memory)
   float *pbuf[2] = {reinterpret_cast<float *>(f.next), reinterpret_cast<float *>(f.q)};
   int cur = 0, mode = kDirReset;
@@ -896,8 +899,12 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       // the chunk's last update also estimates the max-norm residual from
       // the recursively updated r
       const bool last = m == n - 1;
+      // x moves every second iteration: the even iterations of a chunk defer
+      // their step to the odd ones (whose direction's buffer leaves the
+      // deferred one intact), so x is current at every chunk end
+      const int xm = !defer_x ? XM_NOW : (m & 1) ? XM_TWO : last ? XM_NOW : XM_DEFER;
       if ((st = mg_update(mg, f.tl[cl], cl, pbuf[cur], f.phi, f.r, f.sc,
-                          last ? f.partials : nullptr)))
+                          last ? f.partials : nullptr, xm, pbuf[cur ^ 1])))
         return st;
       if (last) {
         k_finish<<<1, 256, 0, stream()>>>(FIN_MAX, 1, f.partials, f.tl[cl].blocks, f.sc);
