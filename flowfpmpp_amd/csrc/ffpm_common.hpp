@@ -170,6 +170,9 @@ __device__ inline void store_partials(double *partials, int nslots, const double
 }
 
 // Called by every thread of the finishing workgroup; thread 0 gets the sums.
+// Each thread adds partials tid, tid + nthreads, ... in that order; the loads
+// of four consecutive terms are issued together (the finish kernels would
+// otherwise wait one memory latency per term).
 template <int N>
 __device__ inline void grid_finish_sum(const double *partials, int nblocks, double (&v)[N],
                                        double *lds) {
@@ -177,8 +180,18 @@ __device__ inline void grid_finish_sum(const double *partials, int nblocks, doub
   const int nthreads = blockDim.x * blockDim.y;
 #pragma unroll
   for (int s = 0; s < N; ++s) {
+    const double *p = partials + (size_t)s * nblocks;
     double x = 0.0;
-    for (int b = tid; b < nblocks; b += nthreads) x += partials[(size_t)s * nblocks + b];
+    int b = tid;
+    for (; b + 3 * nthreads < nblocks; b += 4 * nthreads) {
+      const double t0 = p[b], t1 = p[b + nthreads], t2 = p[b + 2 * nthreads],
+                   t3 = p[b + 3 * nthreads];
+      x += t0;
+      x += t1;
+      x += t2;
+      x += t3;
+    }
+    for (; b < nblocks; b += nthreads) x += p[b];
     v[s] = x;
   }
   block_sum<N>(v, lds);
@@ -190,7 +203,16 @@ __device__ inline double grid_finish_max(const double *partials, int slot, int n
   const int nthreads = blockDim.x * blockDim.y;
   const double *p = partials + (size_t)slot * nblocks;
   double x = 0.0;
-  for (int b = tid; b < nblocks; b += nthreads) x = fmax(x, p[b]);
+  int b = tid;
+  for (; b + 3 * nthreads < nblocks; b += 4 * nthreads) {
+    const double t0 = p[b], t1 = p[b + nthreads], t2 = p[b + 2 * nthreads],
+                 t3 = p[b + 3 * nthreads];
+    x = fmax(x, t0);
+    x = fmax(x, t1);
+    x = fmax(x, t2);
+    x = fmax(x, t3);
+  }
+  for (; b < nblocks; b += nthreads) x = fmax(x, p[b]);
   block_max(x, lds);
   return x;
 }
