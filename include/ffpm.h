@@ -467,7 +467,8 @@ int ffpm_timing_reset(void);
  * "true_residual", "residual_max", "terms", "lap_sweep". count launches, total ms. */
 int ffpm_timing_query(const char *kernel, long *count, double *total_ms);
 /* Bandwidth of a plain device copy (read + write) of the given size, averaged
- * over reps launches; the achievable streaming figure on this device. */
+ * over reps launches (16-byte nontemporal loads and stores, one per thread):
+ * the achievable streaming figure on this device. */
 int ffpm_copy_bandwidth(size_t bytes, int reps, double *gbps, double *avg_us);
 
 #ifdef __cplusplus
