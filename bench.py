@@ -59,7 +59,9 @@ BYTES_PER_CELL = {
                          # write 15 outputs
     # CG with the FP32 multigrid cycle (level-0 kernels; the coarse levels add
     # 1/8, 1/64, ... of their own small traffic)
-    "mg_update": 45,         # read p (FP32), code; read/write phi, r; write x1, q (FP32)
+    "mg_update": 39,         # read p (FP32), code, r; write r, x1, q (FP32); every
+                             # second update also reads/writes phi and reads the
+                             # deferred direction: 29 or 49 B, 39 on average
     "mg_fine_sweep": 14,     # x, the first step x1 (the source), code; write x (FP32);
                              # the step after the first reads x1 once (9 B), the next
                              # ones 13 B, or 17 B with the previous iterate

@@ -549,12 +549,14 @@ __global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
   // FP32 directions in the FP64 buffers' memory, as the standalone solve
   const float *pin = reinterpret_cast<const float *>(d.p[st.cur]);
   float *pout = reinterpret_cast<float *>(d.p[st.cur ^ 1]);
+  // on the cycle's FP32 tiling, as the standalone solve
+  const Tiling &t = d.mg.tl;
   if (st.mode == kDirUpdate)
-    pcg_dir_body<double, CL, kDirUpdate, float, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32,
-                                                       d.code, pin, pout, d.partials, d.sc, lds);
+    pcg_dir_body<double, CL, kDirUpdate, float, float>(t, lb, t.blocks, d.g, d.mg.z32, d.code,
+                                                       pin, pout, d.partials, d.sc, lds);
   else
-    pcg_dir_body<double, CL, kDirReset, float, float>(d.tl, lb, d.tl.blocks, d.g, d.mg.z32,
-                                                      d.code, pin, pout, d.partials, d.sc, lds);
+    pcg_dir_body<double, CL, kDirReset, float, float>(t, lb, t.blocks, d.g, d.mg.z32, d.code,
+                                                      pin, pout, d.partials, d.sc, lds);
 }
 
 // x += alpha p, r -= alpha A p and the next cycle's first step; the slot that
@@ -591,6 +593,8 @@ __global__ __launch_bounds__(256) void k_bg_finish(const BSolve *solves, BState 
     if (op == FIN_FCG1) {
       nblocks = d.mg.tl.blocks;
       nslots = 2;
+    } else if (op == FIN_PQ) {
+      nblocks = d.mg.tl.blocks;  // the direction runs on the FP32 tiling
     }
   } else {
     if (!st.need_tr) return;
@@ -1407,7 +1411,7 @@ int launch_cycle_mg(const Batch &B, hipStream_t s, int m) {
 // alpha, update), then the check logic and the true residual where needed.
 int launch_chunk_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, double tol) {
   const int cb = (B.count + 63) / 64;
-  const dim3 cblk(mgk::kMt), pblk(kTileX, B.h[0].tl.ty);
+  const dim3 cblk(mgk::kMt), pblk(kTileX, B.h[0].tl.ty), sblk(kTileX, B.h[0].tl32.ty);
   int st;
   for (int m = 0; m < ci; ++m) {
     if (m == 0) {
@@ -1424,11 +1428,12 @@ int launch_chunk_mg(const Batch &B, hipStream_t s, int ci, int max_sweeps, doubl
     if (st) return st;
     k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_FCG1, m);
     FFPM_LAUNCHED("k_bg_finish");
-    for (int c : {1, 2}) {
-      const Grid &g = B.g_pcg[c];
+    for (int c : {1, 2, 4}) {
+      const Grid &g = B.g_pcg32[c];
       if (!g.blocks) continue;
-      if (c == 2) k_bg_dir<2><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
-      else k_bg_dir<1><<<g.blocks, pblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      if (c == 4) k_bg_dir<4><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      else if (c == 2) k_bg_dir<2><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m);
+      else k_bg_dir<1><<<g.blocks, sblk, 0, s>>>(B.d_solves, g, B.d_state, m);
       FFPM_LAUNCHED("k_bg_dir");
     }
     k_bg_finish<<<B.count, kThreads, 0, s>>>(B.d_solves, B.d_state, FIN_PQ, m);

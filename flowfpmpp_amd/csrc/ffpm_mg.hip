@@ -809,7 +809,10 @@ int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin,
     kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, mg.z32, mg.code, pin, pout, mg.partials,
                                          sc);
   };
-  if (cl == 2) {
+  if (cl == 4) {
+    if (mode == kDirUpdate) go(k_mg_dir_t<4, kDirUpdate>);
+    else go(k_mg_dir_t<4, kDirReset>);
+  } else if (cl == 2) {
     if (mode == kDirUpdate) go(k_mg_dir_t<2, kDirUpdate>);
     else go(k_mg_dir_t<2, kDirReset>);
   } else {

@@ -191,7 +191,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q);
 // The flexible CG's direction step with the cycle's FP32 output: p = z32 (MODE
 // kDirReset) or z32 + beta p_old (kDirUpdate), formed in FP64 and stored
 // rounded to FP32, q = A p of the rounded p and the p.q partials (t.blocks,
-// FP64 launch shape t with cl cells per lane).
+// launch shape t with cl cells per lane: the cycle's FP32 shape tl, cl).
 int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin, float *pout,
                  int mode, Scalars *sc);
 

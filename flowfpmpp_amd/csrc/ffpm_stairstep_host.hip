@@ -893,8 +893,9 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
       const int dst = cur ^ 1;
-      if ((st = mg_direction(mg, f.tl[cl], cl, pbuf[cur], pbuf[dst], mode, f.sc))) return st;
-      if ((st = finish(f, FIN_PQ, 1, f.tl[cl].blocks))) return st;
+      // the direction's streams are FP32: it runs on the cycle's FP32 tiling
+      if ((st = mg_direction(mg, mg.tl, mg.cl, pbuf[cur], pbuf[dst], mode, f.sc))) return st;
+      if ((st = finish(f, FIN_PQ, 1, mg.tl.blocks))) return st;
       cur = dst;
       // the chunk's last update also estimates the max-norm residual from
       // the recursively updated r
