@@ -7,7 +7,9 @@ Workload: the 512x384x256 tanh-stretched frame (4:1 width ratio) on
 the three force directions: compatibility correction, the solve to 1e-10 on
 the reference's FP64 max-norm residual, gauge, and the FPM densities. Every
 input is resident in HBM before the timed region; phi starts from zero each
-step.
+step. The three directions' solves run at once, each on its own host thread
+and HIP stream (--sequential-directions: one after another on one stream);
+the densities of all three follow in one pass.
 
 The default method is "fcg-mg" (FP64 conjugate gradients preconditioned by
 the FP32 multigrid cycle); the same frame is then solved once with the
@@ -16,7 +18,9 @@ metric asks for ("fp64_pcg"), unless --no-pcg.
 
 Prints ONE JSON line. ``value`` is milliseconds per step (lower is better).
 ``roofline`` covers the kernel with the largest share of device time, timed
-live with HIP events on the library's stream; ``cpu_baseline`` times the CPU
+live with HIP events on the library's stream (with concurrent directions, in
+an extra step that solves them one after another, where each launch has the
+device to itself); ``cpu_baseline`` times the CPU
 oracle (the reference's PCG) on this machine's host on a bounded sample of
 the same frame, scaled to the PCG iteration counts.
 
@@ -230,11 +234,11 @@ def cpu_time_to_solution_c1(tol, check):
     return out
 
 
-def plain_stencil(p, args, pcg_kern, d_rhs, d_solid, axes, faces, phi, bg, sweeps=200):
+def plain_stencil(p, args, d_rhs, d_solid, axes, faces, phi, bg, sweeps=200):
     """The plain FP64 stencil kernels on this frame against the HBM peak: the
-    fused PCG iteration's two kernels from the FP64 PCG step just run, and
-    the weighted-Jacobi sweep timed here over `sweeps` fixed sweeps of the x
-    direction (HIP events on the library's stream around every launch).
+    fused PCG iteration's two kernels and the weighted-Jacobi sweep, timed
+    here over `sweeps` fixed iterations or sweeps of the x direction (HIP
+    events on the library's stream around every launch).
     GB/s are given by the kernels' own algorithmic bytes and by the
     reference's (its three-kernel PCG iteration: 112 B/cell; its Jacobi
     sweep: 32 B/cell), with the rocprofv3-measured traffic where the
@@ -250,11 +254,13 @@ def plain_stencil(p, args, pcg_kern, d_rhs, d_solid, axes, faces, phi, bg, sweep
     synchronize()
     S.timing(True)
     S.solve_stairstep_poisson_3d(phi, d_rhs, d_solid, bg, *faces_now, *axes, -1.0, sweeps,
+                                 sweeps, method="pcg")
+    phi.zero_()
+    S.solve_stairstep_poisson_3d(phi, d_rhs, d_solid, bg, *faces_now, *axes, -1.0, sweeps,
                                  sweeps, method="jacobi")
     synchronize()
     S.timing(False)
-    kern = dict(pcg_kern)
-    kern["jacobi"] = S.timing_query("jacobi")
+    kern = {n: S.timing_query(n) for n in ("pcg_dir_spmv", "pcg_update", "jacobi")}
     out = {}
     for name, (cnt, ms) in kern.items():
         if not cnt:
@@ -844,6 +850,9 @@ def main():
                     help="batch64: also time the 192 solves one after another")
     ap.add_argument("--shape", type=int, nargs=2, metavar=("ROWS", "ZCHUNK"),
                     help="batch64: this launch shape instead of the autotuned one")
+    ap.add_argument("--sequential-directions", action="store_true",
+                    help="frame workload: solve the three directions one after another on one "
+                         "stream (default: at once, one host thread and stream each)")
     ap.add_argument("--no-compare", action="store_true",
                     help="batch64 with fcg-mg: skip the batched mixed-PCG and sequential runs")
     args = ap.parse_args()
@@ -883,23 +892,42 @@ def main():
     log(f"inputs resident on the device: {p.label}, {p.cells} cells, {gen_s:.1f} s")
 
     mixed_outer = []
+    # The three directions are independent solves: by default each runs on
+    # its own host thread and HIP stream (the library keeps one workspace per
+    # stream), so one direction's launch-bound coarse levels and launch gaps
+    # overlap the others' level-0 kernels; the results are bit-identical to
+    # solving them one after another.
+    concurrent = not args.sequential_directions
+    streams, pool = [None, None, None], None
+    if concurrent:
+        from concurrent.futures import ThreadPoolExecutor
+        streams = [S.stream_create() for _ in range(3)]
+        pool = ThreadPoolExecutor(max_workers=3)
 
-    def step(method=args.method):
-        iters, res, totals = [], [], []
-        mixed_outer.clear()
-        for d in range(3):
-            d_phi[d].zero_()
-            for f in d_faces[d]:
-                f.zero_()
-            S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, bgs[d], *d_faces[d], *axes[3:])
-            sw, r = S.solve_stairstep_poisson_3d(d_phi[d], d_rhs, d_solid, bgs[d], *d_faces[d],
-                                                 *axes, args.tol, args.max_iters, args.check,
-                                                 method=method)
-            iters.append(sw)
-            res.append(r)
-            if method == "pcg-mixed":
-                mixed_outer.append(S.last_mixed_solve()[0])
-            log(f"{method} direction {d}: {sw} iterations, residual {r:.3e}")
+    def solve_dir(d, method, stream):
+        if stream is not None:
+            S.set_stream(stream)
+        d_phi[d].zero_()
+        for f in d_faces[d]:
+            f.zero_()
+        S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, bgs[d], *d_faces[d], *axes[3:])
+        sw, r = S.solve_stairstep_poisson_3d(d_phi[d], d_rhs, d_solid, bgs[d], *d_faces[d],
+                                             *axes, args.tol, args.max_iters, args.check,
+                                             method=method)
+        mo = S.last_mixed_solve()[0] if method == "pcg-mixed" else None
+        if stream is not None:
+            synchronize()
+        log(f"{method} direction {d}: {sw} iterations, residual {r:.3e}")
+        return sw, r, mo
+
+    def step(method=args.method, conc=concurrent):
+        if conc:
+            futs = [pool.submit(solve_dir, d, method, streams[d]) for d in range(3)]
+            rs = [f.result() for f in futs]
+        else:
+            rs = [solve_dir(d, method, None) for d in range(3)]
+        iters, res = [x[0] for x in rs], [x[1] for x in rs]
+        mixed_outer[:] = [x[2] for x in rs] if method == "pcg-mixed" else []
         # the three directions' densities in one pass over the flow fields
         # (each equal bit for bit to its own compute_stairstep_fpm_terms_3d)
         o = S.compute_stairstep_fpm_terms_3d_dirs(
@@ -911,12 +939,28 @@ def main():
     for _ in range(args.warmup):
         step()
     synchronize()
-    S.timing(True)
+    if not concurrent:
+        S.timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         iters, res, totals = step()
     synchronize()
     elapsed = time.perf_counter() - t0
+    seq_ms = None
+    if concurrent:
+        # One more step with the directions one after another: its time (the
+        # like-for-like reference for the learned variant, which runs on one
+        # stream), then once more with every hot launch timed by HIP events
+        # for the roofline (in the concurrent step the launches of three
+        # streams overlap, so per-launch event times there would measure the
+        # sharing, not the kernel).
+        t1 = time.perf_counter()
+        step(conc=False)
+        synchronize()
+        seq_ms = (time.perf_counter() - t1) * 1e3
+        S.timing(True)
+        step(conc=False)
+        synchronize()
     S.timing(False)
 
     ms_per_step = elapsed * 1e3 / max(1, args.steps)
@@ -959,6 +1003,8 @@ def main():
                          f"max-norm, check {args.check} + gauge + FPM densities)"),
             "grid": list(args.grid), "cells": cells, "method": args.method,
             "tolerance": args.tol, "check_interval": args.check,
+            "directions": ("at once (three host threads, one HIP stream each)" if concurrent
+                           else "one after another (one stream)"),
         },
         "iterations": iters,
         "mixed_outer_corrections": mixed_outer if args.method == "pcg-mixed" else None,
@@ -970,6 +1016,9 @@ def main():
         "roofline": roofline,
         "input_setup_s": round(gen_s, 2),
     }
+    if concurrent:
+        out["sequential_directions_ms"] = round(seq_ms, 3)
+        out["kernels_timed_in"] = "a step with the directions one after another"
     import ctypes
     from flowfpmpp_amd._lib import lib
     gb, us = ctypes.c_double(), ctypes.c_double()
@@ -979,28 +1028,25 @@ def main():
         roofline["frac_of_copy"] = round(roofline["achieved"] / gb.value, 4)
     pcg_iters = iters if args.method in ("pcg", "jacobi") else None
     if args.method != "pcg" and not args.no_pcg:
-        log("FP64 diagonal PCG on the same frame (one step)")
+        log("FP64 diagonal PCG on the same frame (one step, directions as above)")
         synchronize()
-        S.timing(True)
         t1 = time.perf_counter()
         pcg_iters, pcg_res, pcg_totals = step("pcg")
         synchronize()
         pcg_ms = (time.perf_counter() - t1) * 1e3
-        S.timing(False)
-        pcg_kern = {n: S.timing_query(n) for n in ("pcg_dir_spmv", "pcg_update")}
         dtot = max(abs(a - b) / max(abs(b), 1e-300) for ta, tb in zip(totals, pcg_totals)
                    for a, b in zip(ta, tb))
         out["fp64_pcg"] = {"ms_per_step": round(pcg_ms, 3), "iterations": pcg_iters,
                            "residuals": pcg_res, "check_interval": args.check,
                            "speedup": round(pcg_ms / ms_per_step, 2),
                            "max_rel_diff_of_force_totals": dtot}
-        out["plain_stencil"] = plain_stencil(p, args, pcg_kern, d_rhs, d_solid, axes,
-                                             d_faces[0], d_phi[0], bgs[0])
+        out["plain_stencil"] = plain_stencil(p, args, d_rhs, d_solid, axes, d_faces[0],
+                                             d_phi[0], bgs[0])
     if not args.no_learned and tuple(args.grid) == (512, 384, 256):
         log("multigrid CG with the learned coarse correction on the same frame")
         out["learned_fcg"] = learned_summary(p, args, step_learned_args(
-            p, d_phi, d_rhs, d_solid, bgs, axes, d_faces, dev, outs), ms_per_step,
-            out.get("fp64_pcg", {}).get("ms_per_step"))
+            p, d_phi, d_rhs, d_solid, bgs, axes, d_faces, dev, outs),
+            seq_ms if seq_ms else ms_per_step, out.get("fp64_pcg", {}).get("ms_per_step"))
     if not args.no_batch and tuple(args.grid) == (512, 384, 256):
         log("batched workload: 64 uneven frames x 3 directions")
         out["batch64"] = batch64_summary(args.tol, args.max_iters, args.check)
@@ -1020,6 +1066,10 @@ def main():
             log("measured CPU and GPU time to solution on the 128x96x64 frame")
             out["cpu_time_to_solution_128x96x64"] = cpu_time_to_solution_c1(args.tol,
                                                                             args.check)
+    if concurrent:
+        pool.shutdown()
+        for h in streams:
+            S.stream_destroy(h)
     print(json.dumps(out), flush=True)
 
 

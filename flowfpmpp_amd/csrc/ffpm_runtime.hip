@@ -71,7 +71,8 @@ void count_d2h(size_t bytes) {
 
 namespace {
 thread_local std::string g_error;
-hipStream_t g_stream = nullptr;
+// each thread launches on its own stream (the null stream until it sets one)
+thread_local hipStream_t g_stream = nullptr;
 }  // namespace
 
 void set_error(int code, const std::string &msg) {
@@ -170,9 +171,24 @@ Workspace::~Workspace() {
   // to the driver.
 }
 
+// One workspace per stream, kept until the process ends: a solve's buffers
+// belong to the stream it runs on, so solves on different streams (from
+// different threads) never share them.
+namespace {
+std::mutex g_ws_mutex;
+std::map<hipStream_t, Workspace *> g_workspaces;
+}  // namespace
+
 Workspace &workspace() {
-  static Workspace w;
-  return w;
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  Workspace *&w = g_workspaces[stream()];
+  if (!w) w = new Workspace();
+  return *w;
+}
+
+void release_workspaces() {
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  for (auto &kv : g_workspaces) kv.second->release_all();
 }
 
 KernelTimer &timer() {
@@ -193,23 +209,31 @@ hipEvent_t KernelTimer::take() {
 
 void KernelTimer::start(const char *name, hipEvent_t *a) {
   (void)name;
+  std::lock_guard<std::mutex> lock(m);
   *a = take();
   if (*a) (void)hipEventRecord(*a, stream());
 }
 
 void KernelTimer::stop(const char *name, hipEvent_t a) {
   if (!a) return;
+  std::lock_guard<std::mutex> lock(m);
   hipEvent_t b = take();
   if (!b) return;
   (void)hipEventRecord(b, stream());
   pending.push_back({name, a, b});
-  if (pending.size() > 4096) (void)collect();
+  if (pending.size() > 4096) (void)collect_locked();
 }
 
 int KernelTimer::collect() {
+  std::lock_guard<std::mutex> lock(m);
+  return collect_locked();
+}
+
+int KernelTimer::collect_locked() {
   if (pending.empty()) return 0;
-  FFPM_HIP(hipEventSynchronize(pending.back().b));
   for (auto &r : pending) {
+    // records of several streams: wait for each one's end
+    FFPM_HIP(hipEventSynchronize(r.b));
     float ms = 0.f;
     FFPM_HIP(hipEventElapsedTime(&ms, r.a, r.b));
     auto &t = totals[r.name];
@@ -408,7 +432,27 @@ int ffpm_set_stream(void *s) {
 
 int ffpm_release(void) {
   FFPM_HIP(hipDeviceSynchronize());
-  ffpm::workspace().release_all();
+  ffpm::release_workspaces();
+  return FFPM_OK;
+}
+
+int ffpm_stream_create(void **stream) {
+  if (!stream) {
+    ffpm::set_error(FFPM_ERR_ARGUMENT, "null output pointer");
+    return FFPM_ERR_ARGUMENT;
+  }
+  hipStream_t s = nullptr;
+  // default priority; non-blocking: no implicit ordering with the null stream
+  FFPM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void *)s;
+  return FFPM_OK;
+}
+
+int ffpm_stream_destroy(void *stream) {
+  if (stream) {
+    FFPM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    FFPM_HIP(hipStreamDestroy((hipStream_t)stream));
+  }
   return FFPM_OK;
 }
 
@@ -470,6 +514,7 @@ int ffpm_timing_enable(int on) {
 
 int ffpm_timing_reset(void) {
   int st = ffpm::timer().collect();
+  std::lock_guard<std::mutex> lock(ffpm::timer().m);
   ffpm::timer().totals.clear();
   return st;
 }
@@ -477,6 +522,7 @@ int ffpm_timing_reset(void) {
 int ffpm_timing_query(const char *kernel, long *count, double *total_ms) {
   int st = ffpm::timer().collect();
   if (st) return st;
+  std::lock_guard<std::mutex> lock(ffpm::timer().m);
   auto it = ffpm::timer().totals.find(kernel ? kernel : "");
   if (it == ffpm::timer().totals.end()) {
     *count = 0;

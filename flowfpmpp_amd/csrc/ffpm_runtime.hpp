@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <atomic>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,6 +72,7 @@ class Workspace {
 };
 
 Workspace &workspace();
+void release_workspaces();  // every stream's (no call may be in flight)
 
 enum Slot : int {
   S_COEF = 1, S_PHI, S_NEXT, S_SRC, S_DIAG, S_RDIAG, S_R, S_Q, S_PARTIALS, S_SCALARS,
@@ -93,8 +96,11 @@ enum Slot : int {
 };
 
 // ---- kernel timing with HIP events on the library stream --------------------
+// One timer for the process: the events of a launch are recorded on the
+// launching thread's stream, the records are kept under a mutex.
 struct KernelTimer {
-  bool enabled = false;
+  std::atomic<bool> enabled{false};
+  std::mutex m;
   struct Rec {
     std::string name;
     hipEvent_t a, b;
@@ -106,9 +112,13 @@ struct KernelTimer {
   void start(const char *name, hipEvent_t *a);
   void stop(const char *name, hipEvent_t a);
   int collect();
+  int collect_locked();  // with m held
 };
 
 KernelTimer &timer();
+
+// The calling thread's stream (ffpm_set_stream sets it per thread) and the
+// grow-only workspace of that stream: calls on different streams may overlap.
 
 // Brackets one launch with events when timing is on.
 struct TimedLaunch {

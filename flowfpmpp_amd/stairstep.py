@@ -296,8 +296,22 @@ def compute_stairstep_fpm_terms_3d_dirs(phis, u, v, w, ox, oy, oz, ax, ay, az, s
 
 
 def set_stream(handle) -> None:
-    """Launch on the given raw hipStream_t handle (None: the null stream)."""
+    """Launch this thread's later calls on the given raw hipStream_t handle
+    (None: the null stream). Each stream has its own workspace, so threads on
+    different streams may call the library at the same time."""
     check(lib().ffpm_set_stream(C.c_void_p(0 if handle is None else int(handle))))
+
+
+def stream_create() -> int:
+    """A new non-blocking HIP stream of default priority (raw handle)."""
+    h = C.c_void_p()
+    check(lib().ffpm_stream_create(C.byref(h)))
+    return int(h.value or 0)
+
+
+def stream_destroy(handle) -> None:
+    """Waits for the stream's work and destroys it."""
+    check(lib().ffpm_stream_destroy(C.c_void_p(int(handle))))
 
 
 def last_mixed_solve():

@@ -28,9 +28,12 @@
  * returns the message of the last failure on the calling thread. Messages of
  * the reference's own checks are reproduced verbatim.
  *
- * Calls are blocking and run on one HIP stream (ffpm_set_stream). The
- * library keeps grow-only device workspaces between calls; calls that
- * overlap on one device must be serialised by the caller.
+ * Calls are blocking and run on the calling thread's HIP stream
+ * (ffpm_set_stream, per thread; the null stream until set). The library keeps
+ * a grow-only device workspace per stream between calls, so calls from
+ * different threads on different streams may overlap on one device (the
+ * three force directions of a frame solved at once, for example); calls that
+ * overlap on one stream must be serialised by the caller.
  */
 #ifndef FFPM_H
 #define FFPM_H
@@ -66,9 +69,16 @@ enum { FFPM_METHOD_PCG = 0, FFPM_METHOD_JACOBI = 1, FFPM_METHOD_PCG_MIXED = 2,
        FFPM_METHOD_FCG_MG = 4 };
 
 const char *ffpm_last_error(void);
-/* Sets the HIP stream (hipStream_t) every later call launches on; 0 = null stream. */
+/* Sets the HIP stream (hipStream_t) every later call of the calling thread
+ * launches on; 0 = null stream. No counterpart in the reference. */
 int ffpm_set_stream(void *stream);
-/* Frees the library's device workspaces. */
+/* A new non-blocking HIP stream of default priority for ffpm_set_stream, and
+ * its release (after its work has finished). No counterpart in the
+ * reference: for callers that run independent solves concurrently. */
+int ffpm_stream_create(void **stream);
+int ffpm_stream_destroy(void *stream);
+/* Frees the library's device workspaces (every stream's; no call may be in
+ * flight). */
 int ffpm_release(void);
 
 /* Device memory for callers that hold their arrays in HBM (no counterpart in

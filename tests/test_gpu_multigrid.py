@@ -306,3 +306,50 @@ def test_deferred_iterate_update_is_bit_identical(monkeypatch):
         assert sa == sb and ra == rb
         assert np.array_equal(a, b)
     assert runs["1"][0][2] <= 1e-10 and runs["1"][2][1] == 7
+
+
+def test_directions_on_concurrent_streams_are_bit_identical():
+    """The three directions of a frame solved at once, each from its own host
+    thread on its own HIP stream (the library keeps a workspace per stream),
+    give the fields, iteration counts and residuals of the same solves run
+    one after another on one stream, for fcg-mg and FP64 PCG."""
+    from concurrent.futures import ThreadPoolExecutor
+    from flowfpmpp_amd.device import synchronize
+    print("case: three directions on three streams vs one after another, 128x96x64", flush=True)
+    p = W.frame_128x96x64(with_fields=False)
+    T = DeviceArray.from_host
+    d_solid, d_rhs = T(p.solid), T(p.zeros())
+    axes = [T(a) for a in (p.xc, p.yc, p.zc, p.dx, p.dy, p.dz)]
+    phis = [T(p.phi0()) for _ in range(3)]
+    faces = [[T(f) for f in p.faces()] for _ in range(3)]
+    bgs = [np.eye(3)[d] for d in range(3)]
+
+    def one(d, method, stream):
+        if stream is not None:
+            S.set_stream(stream)
+        phis[d].zero_()
+        for f in faces[d]:
+            f.zero_()
+        S.enforce_stairstep_compatibility_3d(d_rhs, d_solid, bgs[d], *faces[d], *axes[3:])
+        sw, r = S.solve_stairstep_poisson_3d(phis[d], d_rhs, d_solid, bgs[d], *faces[d], *axes,
+                                             1e-10, 100000, 100, method=method)
+        synchronize()
+        return sw, r, phis[d].download()
+
+    streams = [S.stream_create() for _ in range(3)]
+    try:
+        with ThreadPoolExecutor(max_workers=3) as pool:
+            for method in ("fcg-mg", "pcg"):
+                seq = [one(d, method, None) for d in range(3)]
+                for _ in range(2):
+                    conc = [f.result() for f in [pool.submit(one, d, method, streams[d])
+                                                 for d in range(3)]]
+                    for d in range(3):
+                        print(f"{method} direction {d}: {seq[d][0]} / {conc[d][0]} iterations",
+                              flush=True)
+                        assert seq[d][0] == conc[d][0] and seq[d][1] == conc[d][1]
+                        assert seq[d][1] <= 1e-10
+                        assert np.array_equal(seq[d][2], conc[d][2])
+    finally:
+        for h in streams:
+            S.stream_destroy(h)
