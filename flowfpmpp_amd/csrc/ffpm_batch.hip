@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_b_finish(const BSolve *solves, BState *
   } else {
     if (!st.need_tr) return;
     nblocks = op == FIN_TR1 ? d.tt.blocks : d.eb;
-    if (op == FIN_TR1) nslots = 3;
+    if (op == FIN_TR1) nslots = 4;
   }
   finish_body(op, nslots, d.partials, nblocks, d.sc, lds);
   if (threadIdx.x == 0) {
@@ -214,6 +214,7 @@ __global__ void k_b_check_begin(const BSolve *solves, BState *state, int count, 
       } else {
         sc->status = 0;
         sc->iters = 0;
+        sc->sig = 0.0;  // the cycle scale starts afresh, as k_clear_status
         st.need_tr = 1;
         st.mode = kDirReset;
       }
@@ -411,12 +412,12 @@ __global__ __launch_bounds__(256) void k_bg_first(const BSolve *solves, Grid gri
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
   mgk::mgf_jac0_body(cell_blk(mg.lv[0], grid.local[blockIdx.x]), mg.g32, mg.code, d.r,
-                     mg.lv[0].x, mg.first, d.sc);
+                     mg.buf0(0), mg.first, d.sc);
 }
 
 // One level-0 smoothing step (MgStep: buffers ia, ip, io, coefficients);
-// LAST writes the cycle's output z32 with the z.r and z.q partials (z.q only
-// after the first direction, as the standalone cycle).
+// LAST writes the cycle's output z with the z.r, z.q (z.q only after the
+// first direction, as the standalone cycle) and max |z| partials.
 template <int CL, bool LAST, bool NT, bool CHEB>
 __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid grid,
                                                    const BState *state, int slot, MgStep step) {
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid gr
   const Multigrid &mg = d.mg;
   const float *q = LAST && st.mode == kDirUpdate ? mg.q32 : nullptr;
   mgk::mg_sweep_body<CL, LAST, NT, CHEB>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl,
-                                         mg.g32, mg.buf0(step.a), mg.lv[0].x, mg.code,
+                                         mg.g32, mg.buf0(step.a), mg.buf0(0), mg.code,
                                          mg.buf0(step.o), mg.partials, step.c2, q,
                                          step.p >= 0 ? mg.buf0(step.p) : nullptr, step.c1,
                                          mg.sscale);
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(256) void k_bg_restrict0(const BSolve *solves, Grid
   const MgLevel &c = mg.lv[1];
   const mgk::CoarseOut out{c.b, c.x, c.rd};
   mgk::mgf_restrict_z_body<VEC>(cell_blk(c, grid.local[blockIdx.x]), mg.g32, mg.code,
-                                mg.lv[0].x, mg.buf0(ia), c.nx, c.ny, c.nz, d.kc1, out,
+                                mg.buf0(0), mg.buf0(ia), c.nx, c.ny, c.nz, d.kc1, out,
                                 mg.sscale);
 }
 
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(512) void k_bg_restrict_t(const BSolve *solves, Gri
   const MgLevel &c = mg.lv[1];
   const mgk::CoarseOut out{c.b, c.x, c.rd};
   mgk::mg_restrict_body<CL, NT>(grid.local[blockIdx.x], mg_rs_lds, mg.tlr, mg.g32, mg.buf0(ia),
-                                mg.lv[0].x, mg.code, c.nx, c.ny, out, mg.sscale);
+                                mg.buf0(0), mg.code, c.nx, c.ny, out, mg.sscale);
 }
 
 // Prolongation from level 1's result (buffer ic; the coarsest level writes
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(1024) void k_bg_coarsest(const BSolve *solves, cons
                          Multigrid::bufc(c, io), mg.omega_coarse, mg.coarse_sweeps);
 }
 
-// The direction step on the cycle's FP32 output (p = z or z + beta p_old,
+// The direction step on the cycle's output (p = z or z + beta p_old,
 // q = A p, p.q), as the standalone mg_direction.
 template <int CL>
 __global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
@@ -552,11 +553,13 @@ __global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
   // on the cycle's FP32 tiling, as the standalone solve
   const Tiling &t = d.mg.tl;
   if (st.mode == kDirUpdate)
-    pcg_dir_body<double, CL, kDirUpdate, float, float>(t, lb, t.blocks, d.g, d.mg.z32, d.code,
-                                                       pin, pout, d.partials, d.sc, lds);
+    pcg_dir_body<double, CL, kDirUpdate, MgStore, float>(t, lb, t.blocks, d.g, d.mg.buf0(3),
+                                                         d.code, pin, pout, d.partials, d.sc,
+                                                         lds);
   else
-    pcg_dir_body<double, CL, kDirReset, float, float>(t, lb, t.blocks, d.g, d.mg.z32, d.code,
-                                                      pin, pout, d.partials, d.sc, lds);
+    pcg_dir_body<double, CL, kDirReset, MgStore, float>(t, lb, t.blocks, d.g, d.mg.buf0(3),
+                                                        d.code, pin, pout, d.partials, d.sc,
+                                                        lds);
 }
 
 // x += alpha p, r -= alpha A p and the next cycle's first step; the slot that
@@ -573,7 +576,7 @@ __global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid
   double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
   mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g,
                        reinterpret_cast<const float *>(d.p[st.cur]), d.code, d.x, d.r,
-                       mg.lv[0].x, d.sc, (double)mg.first, rmax, mg.q32);
+                       mg.buf0(0), d.sc, (double)mg.first, rmax, mg.q32);
 }
 
 // One workgroup per solve: the finish of the preceding reduction for the
@@ -592,14 +595,14 @@ __global__ __launch_bounds__(256) void k_bg_finish(const BSolve *solves, BState 
     if (op == FIN_MAX && slot != st.n_chunk - 1) return;
     if (op == FIN_FCG1) {
       nblocks = d.mg.tl.blocks;
-      nslots = 2;
+      nslots = 3;
     } else if (op == FIN_PQ) {
       nblocks = d.mg.tl.blocks;  // the direction runs on the FP32 tiling
     }
   } else {
     if (!st.need_tr) return;
     nblocks = op == FIN_TR1 ? d.tt.blocks : d.eb;
-    if (op == FIN_TR1) nslots = 3;
+    if (op == FIN_TR1) nslots = 4;
   }
   finish_body(op, nslots, d.partials, nblocks, d.sc, lds);
   if (op == FIN_PQ && threadIdx.x == 0) state[s].cur = st.cur ^ 1;
@@ -635,6 +638,7 @@ __global__ void k_bg_check_begin(const BSolve *solves, BState *state, int count,
       } else {
         sc->status = 0;
         sc->iters = 0;
+        sc->sig = 0.0;  // the cycle scale starts afresh, as k_clear_status
         st.need_tr = 1;
         st.first_ready = 0;
         st.mode = kDirReset;

@@ -9,6 +9,29 @@
 
 namespace ffpm {
 
+// Scale of the multigrid cycle's FP16 level-0 vectors (Scalars::sig): the
+// largest power of two s with s max <= target, times the current scale cur
+// (max measured in units of cur), within 2^-120..2^120; cur when max is 0
+// or not finite (a non-finite output ends in a breakdown, whose restart
+// starts the scale afresh). A cycle's output is the next cycle's times its
+// contraction (0.1-0.4 measured), so kSigTarget = 2^10 leaves the next
+// output 2^6 below the FP16 limit 65504 with its values at full precision
+// (11 bits) down to 2^-14; a solve's first cycle is scaled from its first
+// step x1 (FIN_TR1), whose output exceeds x1 by at most the condition of
+// D^-1 A (20-45 measured on stretched frames, at most a few times 10^4 at
+// 512 cells per axis): kSigFirst = 2^-4 keeps it below 2^12 times that.
+constexpr double kSigTarget = 1024.0, kSigFirst = 0.0625;
+
+__host__ __device__ inline double cycle_scale(double cur, double target, double mx) {
+  if (!(mx > 0.0) || !(mx <= 1.7976931348623157e308)) return cur;
+  int e = 0;
+  frexp(target / mx, &e);  // target / mx in [2^(e-1), 2^e)
+  const int e0 = ilogb(cur);
+  int en = e0 + e - 1;
+  en = en < -120 ? -120 : en > 120 ? 120 : en;
+  return ldexp(1.0, en);
+}
+
 // Totals the partials of the preceding reduction kernel and applies the PCG
 // scalar logic (src/fpm_stairstep_mpi_3d.f90:257-278). One workgroup.
 __device__ inline void finish_body(int op, int nslots, const double *partials, int nblocks,
@@ -20,23 +43,32 @@ __device__ inline void finish_body(int op, int nslots, const double *partials, i
     if (lead) sc->res = m;
   } else if (op == FIN_TR1) {
     const double m = grid_finish_max(partials, 2, nblocks, lds);
+    const double mx = grid_finish_max(partials, 3, nblocks, lds + 16);
     double v[2];
-    grid_finish_sum<2>(partials, nblocks, v, lds + 16);
+    grid_finish_sum<2>(partials, nblocks, v, lds + 32);
     if (lead) {
       sc->res = m;
       sc->red1 = v[0];
       sc->red2 = v[1];
+      // a solve's first cycle (or the first after a restart): its first step
+      // is about mx (times first < 1)
+      if (!(sc->sig > 0)) sc->sig = cycle_scale(1.0, kSigFirst, mx);
     }
   } else if (op == FIN_FCG1) {
     // CG preconditioned by the multigrid cycle (flexible CG with one stored
     // direction): slot 0 z.r, slot 1 z.(A p_old); p = z + beta p_old is made
     // A-orthogonal to p_old, beta = -z.A p_old / p_old.A p_old, and
-    // alpha = z.r / p.A p follows in FIN_PQ.
+    // alpha = z.r / p.A p follows in FIN_PQ. The slots hold the cycle's
+    // scaled output (sig z; slot 0 also the scaled source): exact powers of
+    // two divide them out.
+    const double zmax = grid_finish_max(partials, 2, nblocks, lds);
     double v[2];
-    grid_finish_sum<2>(partials, nblocks, v, lds);
+    grid_finish_sum<2>(partials, nblocks, v, lds + 16);
     if (lead) {
-      sc->beta = sc->pq > 0 ? -v[1] / sc->pq : 0.0;
-      sc->rz = v[0];
+      const double sg = sc->sig, inv = 1.0 / sg;
+      sc->beta = sc->pq > 0 ? -(v[1] * inv) / sc->pq : 0.0;
+      sc->rz = v[0] * inv * inv;
+      sc->sig_next = cycle_scale(sg, kSigTarget, zmax);
     }
   } else if (op == FIN_SUMS) {
     double v[kReduceSlots];
@@ -71,7 +103,8 @@ __device__ inline void finish_body(int op, int nslots, const double *partials, i
 
 // true_residual, first pass (src/fpm_stairstep_mpi_3d.f90:303-319): the
 // max-norm residual and r = V (src + sum c phi_nb - diag phi) in fluid cells
-// share one evaluation of the bracket; sums of r and of the fluid count.
+// share one evaluation of the bracket; sums of r and of the fluid count, and
+// max |r / V| / diag (the scale of a multigrid cycle's first step).
 __device__ inline void true_residual_1_body(const Tiling &t, int block, int nblocks,
                                             const Geom &g, const double *__restrict__ phi,
                                             const double *__restrict__ src,
@@ -80,7 +113,7 @@ __device__ inline void true_residual_1_body(const Tiling &t, int block, int nblo
                                             double *lds) {
   const TileCoord tc = tile_coord(t, g.nx, g.ny, g.nz, block);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  double m = 0.0;
+  double m = 0.0, mx = 0.0;
   double s[2] = {0.0, 0.0};
   if (tc.valid && tc.i < nx && tc.j < ny) {
     const int i = tc.i, j = tc.j;
@@ -103,6 +136,7 @@ __device__ inline void true_residual_1_body(const Tiling &t, int block, int nblo
         const double v = src[c] + cw * pw + ce * pe + cs * ps + cn * pn + g.cb[k] * p_prev +
                          g.ct[k] * p_next - d * p_cur;
         m = fmax(m, fabs(v));
+        mx = fmax(mx, fabs(v) / d);
         rv = dxdy * g.dz[k] * v;
         s[0] += rv;
         s[1] += 1.0;
@@ -114,9 +148,10 @@ __device__ inline void true_residual_1_body(const Tiling &t, int block, int nblo
     }
   }
   block_max(m, lds);
-  block_sum<2>(s, lds + 16);
-  const double vals[3] = {s[0], s[1], m};
-  store_partials(partials, 3, vals, nblocks, block);
+  block_max(mx, lds + 16);
+  block_sum<2>(s, lds + 32);
+  const double vals[4] = {s[0], s[1], m, mx};
+  store_partials(partials, 4, vals, nblocks, block);
 }
 
 // true_residual, second pass (src/fpm_stairstep_mpi_3d.f90:319-328): project

@@ -29,18 +29,25 @@ struct Scalars {
   int status;         // 0, kBreakdown or kNegative
   int iters;          // iterations completed since the last chunk start
   double alpha_pend;  // step length of a direction whose x update is deferred
+  // The multigrid cycle's level-0 vectors are stored in FP16 scaled by a
+  // power of two: sig for the cycle in flight (its first step, iterates and
+  // output z carry that factor), sig_next for the next one (from the max of
+  // the last output). 0: not set yet (the next FIN_TR1 sets it).
+  double sig, sig_next;
 };
 
 // Finish operations of k_finish (what the partials of the preceding
 // reduction kernel mean and where their total goes).
 enum FinishOp : int {
   FIN_MAX = 0,    // res = max(slot 0)
-  FIN_TR1 = 1,    // red1 = sum(slot 0), red2 = sum(slot 1), res = max(slot 2)
+  FIN_TR1 = 1,    // red1 = sum(slot 0), red2 = sum(slot 1), res = max(slot 2); an unset
+                  // sig from max(slot 3) = max |r / V| / diag
   FIN_RZ = 2,     // rz = sum(slot 0)
   FIN_PQ = 3,     // pq = sum(slot 0); breakdown test; alpha = rz / pq
   FIN_RZNEW = 4,  // rz_new = sum(slot 0); rz_new < 0 test; beta; rz; iters += 1
   FIN_SUMS = 5,   // sums[s] = sum(slot s), s < nslots
-  FIN_FCG1 = 6,   // rz = sum(slot 0) = z.r; beta = -sum(slot 1) / pq = -z.(A p_old) / pq
+  FIN_FCG1 = 6,   // rz = sum(slot 0) / sig^2 = z.r; beta = -sum(slot 1) / (sig pq) =
+                  // -z.(A p_old) / pq (slots of the scaled z); sig_next from max(slot 2)
 };
 
 struct Bounds {

@@ -18,16 +18,16 @@ using namespace mgk;
 // ---- standalone kernels: one solve per launch --------------------------------
 __global__ __launch_bounds__(256) void k_mgf_jac0(GeomT<float> g, const uint8_t *__restrict__ code,
                                                   double *__restrict__ r,
-                                                  float *__restrict__ x, float omega,
-                                                  const Scalars *proj) {
-  mgf_jac0_body(blk3(), g, code, r, x, omega, proj);
+                                                  MgStore *__restrict__ x, float omega,
+                                                  const Scalars *sc) {
+  mgf_jac0_body(blk3(), g, code, r, x, omega, sc);
 }
 
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
                                                         const uint8_t *__restrict__ code,
-                                                        const float *__restrict__ src,
-                                                        const float *__restrict__ x, int cnx,
+                                                        const MgStore *__restrict__ src,
+                                                        const MgStore *__restrict__ x, int cnx,
                                                         int cny, int cnz, int kchunk,
                                                         CoarseOut out, float sscale) {
   mgf_restrict_z_body<VEC>(blk3(), g, code, src, x, cnx, cny, cnz, kchunk, out, sscale);
@@ -35,8 +35,8 @@ __global__ __launch_bounds__(256) void k_mgf_restrict_z(GeomT<float> g,
 
 template <int CL, bool NT>
 __global__ __launch_bounds__(512) void k_mg_restrict_t(Tiling t, GeomT<float> g,
-                                                       const float *__restrict__ x,
-                                                       const float *__restrict__ src,
+                                                       const MgStore *__restrict__ x,
+                                                       const MgStore *__restrict__ src,
                                                        const uint8_t *__restrict__ code, int cnx,
                                                        int cny, CoarseOut out, float sscale) {
   extern __shared__ float mg_rs_lds[];  // mg_restrict_lds(t.ty, CL) bytes
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(512) void k_mg_restrict_t(Tiling t, GeomT<float> g,
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
                                                        const uint8_t *__restrict__ code,
-                                                       float *__restrict__ x, int cnx, int cny,
+                                                       MgStore *__restrict__ x, int cnx, int cny,
                                                        int cnz, const float *__restrict__ xc,
                                                        const float *__restrict__ mc,
                                                        int kchunk) {
@@ -108,12 +108,12 @@ __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
 
 template <int CL, bool LAST, bool NT, bool CHEB>
 __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
-                                                     const float *__restrict__ a,
-                                                     const float *__restrict__ r,
+                                                     const MgStore *__restrict__ a,
+                                                     const MgStore *__restrict__ r,
                                                      const uint8_t *__restrict__ code,
-                                                     float *__restrict__ b, double *partials,
+                                                     MgStore *__restrict__ b, double *partials,
                                                      float c2, const float *__restrict__ q,
-                                                     const float *__restrict__ prev, float c1,
+                                                     const MgStore *__restrict__ prev, float c1,
                                                      float sscale) {
   __shared__ double lds[16 * kReduceSlots];
   mg_sweep_body<CL, LAST, NT, CHEB>(blockIdx.x, gridDim.x, lds, t, g, a, r, code, b, partials,
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    const float *__restrict__ p,
                                                    const uint8_t *__restrict__ code,
                                                    double *__restrict__ x, double *__restrict__ r,
-                                                   float *__restrict__ x1, Scalars *sc,
+                                                   MgStore *__restrict__ x1, Scalars *sc,
                                                    double omega, double *rmax,
                                                    float *__restrict__ q32,
                                                    const float *__restrict__ ppend) {
@@ -134,18 +134,18 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                       ppend);
 }
 
-// The flexible CG's direction step on the cycle's FP32 output (the PCG
-// direction body with z widened on load and p stored in FP32).
+// The flexible CG's direction step on the cycle's output (the PCG direction
+// body with the FP16 z widened and unscaled on load and p stored in FP32).
 template <int CL, int MODE>
 __global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<double> g,
-                                                   const float *__restrict__ z,
+                                                   const MgStore *__restrict__ z,
                                                    const uint8_t *__restrict__ code,
                                                    const float *__restrict__ pin,
                                                    float *__restrict__ pout, double *partials,
                                                    Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  pcg_dir_body<double, CL, MODE, float, float>(t, blockIdx.x, gridDim.x, g, z, code, pin, pout,
-                                               partials, sc, lds);
+  pcg_dir_body<double, CL, MODE, MgStore, float>(t, blockIdx.x, gridDim.x, g, z, code, pin,
+                                                 pout, partials, sc, lds);
 }
 
 // ---- the learned coarse correction -----------------------------------------------
@@ -454,7 +454,7 @@ size_t mg_operator_bytes(const Multigrid &mg) {
 }
 
 size_t mg_state_bytes(const Multigrid &mg) {
-  size_t bytes = 6 * up256(sizeof(float) * mg.lv[0].n);
+  size_t bytes = 5 * up256(sizeof(MgStore) * mg.lv[0].n) + up256(sizeof(float) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) bytes += 4 * up256(sizeof(float) * mg.lv[l].n);
   return bytes;
 }
@@ -471,12 +471,11 @@ void mg_bind_operator(Multigrid &mg, char *&base) {
 }
 
 void mg_bind_state(Multigrid &mg, char *&base) {
-  mg.lv[0].x = carve(base, mg.lv[0].n);
-  mg.lv[0].y = carve(base, mg.lv[0].n);
-  mg.lv[0].w = carve(base, mg.lv[0].n);
+  for (int i = 0; i < 5; ++i) {
+    mg.f0[i] = reinterpret_cast<MgStore *>(base);
+    base += up256(sizeof(MgStore) * mg.lv[0].n);
+  }
   mg.q32 = carve(base, mg.lv[0].n);
-  mg.v = carve(base, mg.lv[0].n);
-  mg.z32 = carve(base, mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
     v.x = carve(base, v.n);
@@ -570,10 +569,10 @@ namespace {
 template <bool LAST, bool NT, bool CHEB>
 void launch_step0(const Multigrid &mg, const MgStep &st, const float *q) {
   const dim3 blk(kTileX, mg.tl.ty);
-  const float *a = mg.buf0(st.a), *p = st.p >= 0 ? mg.buf0(st.p) : nullptr;
-  float *o = mg.buf0(st.o);
+  const MgStore *a = mg.buf0(st.a), *p = st.p >= 0 ? mg.buf0(st.p) : nullptr;
+  MgStore *o = mg.buf0(st.o);
   auto go = [&](auto kern) {
-    kern<<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, mg.lv[0].x, mg.code, o,
+    kern<<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, mg.buf0(0), mg.code, o,
                                              mg.partials, st.c2, q, p, st.c1, mg.sscale);
   };
   if (mg.cl == 4) go(k_mg_sweep_t<4, LAST, NT, CHEB>);
@@ -594,10 +593,10 @@ void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr
 
 }  // namespace
 
-int mg_first(const Multigrid &mg, double *r, const Scalars *proj) {
+int mg_first(const Multigrid &mg, double *r, const Scalars *sc) {
   const MgLevel &f0 = mg.lv[0];
   k_mgf_jac0<<<cell_grid(f0.nx, f0.ny, f0.nz), dim3(kMt), 0, stream()>>>(
-      mg.g32, mg.code, r, f0.x, mg.first, proj);
+      mg.g32, mg.code, r, mg.buf0(0), mg.first, sc);
   FFPM_LAUNCHED("k_mgf_jac0");
   return OK;
 }
@@ -606,7 +605,7 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
               double *r, Scalars *sc, double *rmax, int xmode, const float *ppend) {
   const dim3 blk(kTileX, t.ty);
   const double w = (double)mg.first;
-  float *x1 = mg.lv[0].x;
+  MgStore *x1 = mg.buf0(0);
   if (xmode == XM_TWO && !ppend) {
     set_error(ERR_ARGUMENT, "multigrid update: the deferred step needs its direction");
     return ERR_ARGUMENT;
@@ -644,7 +643,7 @@ CoarseOut coarse_out(const Multigrid &mg, int l) {
 
 int mg_vcycle(const Multigrid &mg, bool with_q) {
   const dim3 blk(kMt);
-  const float *r = mg.lv[0].x;  // the source, read as x1 d sscale
+  const MgStore *r = mg.buf0(0);  // the source, read as x1 d sscale
   const MgLevel &f0 = mg.lv[0];
   const GeomT<float> &g = mg.g32;
   const MgSchedule &s0 = mg.s0, &sc = mg.sc;
@@ -658,7 +657,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     const MgLevel &c = mg.lv[1];
     TimedLaunch tl("mg_fine_restrict");
     const dim3 rb(kTileX, mg.tlr.ty);
-    const float *xm = mg.buf0(s0.mid);
+    const MgStore *xm = mg.buf0(s0.mid);
     const size_t lb = mg_restrict_lds(mg.tlr.ty, mg.cl);
     auto go = [&](auto kern) {
       kern<<<mg.tlr.blocks, rb, lb, stream()>>>(mg.tlr, g, xm, r, mg.code, c.nx, c.ny,
@@ -774,7 +773,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     const MgLevel &c = mg.lv[1];
     TimedLaunch tl("mg_fine_prolong");
     const int kc = column_chunk(c.nx, c.ny, c.nz);
-    float *x0 = mg.buf0(s0.mid);
+    MgStore *x0 = mg.buf0(s0.mid);
     const float *xc = Multigrid::bufc(c, sc.out);
     if (f0.nx % 2 == 0)
       k_mgf_prolong_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
@@ -806,7 +805,7 @@ int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin,
   const dim3 blk(kTileX, t.ty);
   TimedLaunch tl("mg_direction");
   auto go = [&](auto kern) {
-    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, mg.z32, mg.code, pin, pout, mg.partials,
+    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, mg.buf0(3), mg.code, pin, pout, mg.partials,
                                          sc);
   };
   if (cl == 4) {

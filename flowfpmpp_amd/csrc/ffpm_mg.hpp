@@ -12,7 +12,7 @@
 // null space of the Neumann problem is kept. A cell carries an unknown when
 // its diagonal is positive.
 //
-// Cycle (FP32 arithmetic and output z, FP64 input r): nu smoothing steps
+// Cycle (FP32 arithmetic, FP64 input r): nu smoothing steps
 // from zero (the first one fused into the outer update or the restriction
 // that forms the level's right-hand side), restriction of the residual by
 // summation over the children, recursion, trilinear cell-centred
@@ -23,11 +23,25 @@
 // level. The coarsest level (at most 1024 cells) runs a fixed number of
 // damped-Jacobi sweeps in one workgroup. Every step is a fixed linear map, so the
 // CG around it sees the same preconditioner at every iteration.
+//
+// Storage: the level-0 vectors of the cycle (its first step, which also
+// carries the source, the iterates and the output z) are FP16, scaled by a
+// power of two per cycle (Scalars::sig, from the previous cycle's largest
+// output) so that their values sit near 2^10: the cycle is linear, so the
+// scale divides out exactly where z leaves it (the direction step, the z.r
+// and z.q sums). That halves the level-0 bytes of the cycle; with the
+// rounding (11 bits) the 512x384x256 frame takes the same 20 iterations per
+// direction as with FP32 vectors (a CPU model of the cycle: one more of 18
+// in one direction of a 256x192x128 frame). The coarse levels, whose values
+// carry the same scale, stay FP32.
 #pragma once
 
 #include "ffpm_pcg.hpp"
 
 namespace ffpm {
+
+// storage of the cycle's level-0 vectors
+using MgStore = _Float16;
 
 constexpr int kMgMaxLevels = 20;
 constexpr int kMgCoarsestCells = 1024;
@@ -38,7 +52,7 @@ struct MgLevel {
   float *gE = nullptr, *gN = nullptr, *gT = nullptr;  // coarse levels only
   float *m = nullptr;                                 // coarse levels: 1 where diag > 0
   float *rd = nullptr;                                // coarse levels: first-step scale / diag
-  float *x = nullptr, *y = nullptr, *w = nullptr;     // iterate buffers
+  float *x = nullptr, *y = nullptr, *w = nullptr;     // coarse levels: iterate buffers
   float *b = nullptr;                                 // coarse levels only
   // coarse levels of at least kMgStreamCells cells: the streaming sweep's
   // launch shape and cells per lane (cl = 0: the one-cell sweep)
@@ -50,9 +64,9 @@ constexpr size_t kMgStreamCells = 1u << 18;
 
 // One smoothing step of a level: iterate buffer a (and the previous iterate
 // p, -1 for none) into buffer o, out = a + c2 (b - A a) / d + c1 (a - p).
-// Level-0 buffers: 0 x (the first step, kept through the cycle: it also
-// stands for the source), 1 y, 2 w, 3 the cycle's output z32, 4 v; coarse:
-// 0 x, 1 y, 2 w.
+// Level-0 buffers (Multigrid::f0): 0 the first step x1 (kept through the
+// cycle: it also stands for the source), 1 and 2 iterates, 3 the cycle's
+// output z, 4 a fifth iterate; coarse: 0 x, 1 y, 2 w.
 struct MgStep {
   int a, p, o;
   float c1, c2;
@@ -111,18 +125,16 @@ struct Multigrid {
   bool cheb = true;          // Chebyshev smoothing (else damped Jacobi)
   float first = 0.9f;        // scale of the first step from zero, x = first b / d
   float first_coarse = 0.9f; // the same on the coarse levels
-  MgSchedule s0, sc;         // level 0 (its last step writes z32), coarse levels
+  MgSchedule s0, sc;         // level 0 (its last step writes z), coarse levels
   MgLearned lrn;               // the learned coarse correction, when net is set
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
   float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
-  float *v = nullptr;          // a fifth level-0 iterate buffer (index 4)
-  float sscale = 1.f;          // 1 / first: the cycle's source r / V is read as
-                               // lv[0].x d sscale (lv[0].x = first D^-1 r)
-  float *z32 = nullptr;        // the cycle's output z = B r (level-0 cells)
-  // the iterate buffers of level 0 (index as in MgStep)
-  __host__ __device__ float *buf0(int i) const {
-    return i == 0 ? lv[0].x : i == 1 ? lv[0].y : i == 2 ? lv[0].w : i == 3 ? z32 : v;
-  }
+  float sscale = 1.f;          // 1 / first: the cycle's (scaled) source r / V is
+                               // read as f0[0] d sscale (f0[0] = sig first D^-1 r)
+  // the level-0 vectors, FP16 times Scalars::sig (index as in MgStep; 3 is
+  // the cycle's output z = B r)
+  MgStore *f0[5] = {};
+  __host__ __device__ MgStore *buf0(int i) const { return f0[i]; }
   __host__ __device__ static float *bufc(const MgLevel &v, int i) {
     return i == 0 ? v.x : i == 1 ? v.y : v.w;
   }
@@ -137,9 +149,10 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 // The steps of mg_setup for callers that place the arrays themselves (the
 // batched solves): parameters and level extents (no allocation); the bytes of
 // the level operators (gE gN gT m rd of the coarse levels, shared by solves
-// of one operator) and of one solve's cycle state (level-0 x y w q s z,
-// coarse x y w b), every array 256-byte aligned; carving them from base (advanced past
-// them); and the launches that fill the level operators.
+// of one operator) and of one solve's cycle state (the five FP16 level-0
+// vectors and q32, coarse x y w b), every array 256-byte aligned; carving
+// them from base (advanced past them); and the launches that fill the level
+// operators.
 int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
             const uint8_t *code, const Tiling &tl, int cl, double *partials);
 size_t mg_operator_bytes(const Multigrid &mg);
@@ -164,14 +177,16 @@ int mg_rows();
 // residual itself is formed only near convergence.
 int mg_check_interval(int check_interval);
 
-// The first smoothing step from zero, lv[0].x = w D^-1 r (which also carries
-// the cycle's source r / V = lv[0].x d / w). With proj non-null (the scalars of a true residual's
-// first pass) r is first projected off the constants in place, r -= red1 /
-// red2 where diag > 0, which completes that true residual.
-int mg_first(const Multigrid &mg, double *r, const Scalars *proj = nullptr);
+// The first smoothing step from zero after a true residual's first pass,
+// f0[0] = sig w D^-1 r (which also carries the cycle's source r / V =
+// f0[0] d / (w sig)): r is first projected off the constants in place, r -=
+// red1 / red2 where diag > 0, which completes that true residual (sc: the
+// scalars of that first pass, with the cycle scale sig).
+int mg_first(const Multigrid &mg, double *r, const Scalars *sc);
 
-// x += alpha p, r -= alpha A p, lv[0].x = w D^-1 r and q32 = A p
-// (alpha in sc; no-op after a breakdown); counts the iteration. t, cl: FP64 launch
+// x += alpha p, r -= alpha A p, f0[0] = sig w D^-1 r (sig = sc->sig_next,
+// which becomes sc->sig) and q32 = A p (alpha in sc; no-op after a
+// breakdown); counts the iteration. t, cl: FP64 launch
 // shape. The direction p is stored in FP32 (mg_direction rounds it) and
 // widened exactly; x, r and the arithmetic stay FP64. With rmax non-null, one
 // partial max |r / V| per workgroup (t.blocks) goes there. xmode XM_DEFER
@@ -183,13 +198,14 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
               double *r, Scalars *sc, double *rmax = nullptr, int xmode = XM_NOW,
               const float *ppend = nullptr);
 
-// z32 = B r (the rest of the cycle after the first step, which must be in
-// lv[0].x). Leaves two slots of partial sums
-// (tl.blocks each) in partials: z.r, and z.q32 when with_q (else 0).
+// f0[3] = sig B r (the rest of the cycle after the first step, which must be
+// in f0[0]). Leaves three slots of partials (tl.blocks each) of the scaled
+// output: z.r and z.q32 (0 without with_q) times sig^2 and sig, and max |z|
+// times sig (k_finish FIN_FCG1 divides the scale out).
 int mg_vcycle(const Multigrid &mg, bool with_q);
 
-// The flexible CG's direction step with the cycle's FP32 output: p = z32 (MODE
-// kDirReset) or z32 + beta p_old (kDirUpdate), formed in FP64 and stored
+// The flexible CG's direction step with the cycle's output: p = z (MODE
+// kDirReset) or z + beta p_old (kDirUpdate), z = f0[3] / sig, formed in FP64 and stored
 // rounded to FP32, q = A p of the rounded p and the p.q partials (t.blocks,
 // launch shape t with cl cells per lane: the cycle's FP32 shape tl, cl).
 int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin, float *pout,

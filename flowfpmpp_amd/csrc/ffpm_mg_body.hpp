@@ -108,29 +108,31 @@ struct CoarseOut {
 };
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
-// The first sweep from zero, x = omega D^-1 r = omega s / d (D = V diag,
-// s = r / V rounded to FP32, 0 where d = 0), which the rest of the cycle also
-// reads as its source s = x d / omega. With proj non-null
-// the true residual's projection (src/fpm_stairstep_mpi_3d.f90:319-322) is
-// applied first: r -= red1 / red2 in the cells with a fluid neighbour (those
-// whose diag is positive), and the projected r is stored.
+// The first sweep from zero, x = sig omega D^-1 r = sig omega s / d (D = V
+// diag, s = r / V rounded to FP32, 0 where d = 0; sig = sc->sig, the cycle
+// scale), stored in FP16, which the rest of the cycle also reads as its
+// (scaled) source s = x d / omega. The true residual's projection
+// (src/fpm_stairstep_mpi_3d.f90:319-322) is applied first: r -= red1 / red2
+// (from sc) in the cells with a fluid neighbour (those whose diag is
+// positive), and the projected r is stored.
 __device__ __forceinline__ void mgf_jac0_body(Blk bk, GeomT<float> g,
                                               const uint8_t *__restrict__ code,
-                                              double *__restrict__ r, float *__restrict__ x,
-                                              float omega, const Scalars *proj = nullptr) {
+                                              double *__restrict__ r, MgStore *__restrict__ x,
+                                              float omega, const Scalars *sc) {
   const Idx t = cell_idx(g.nx, g.ny, bk);
   if (!t.in) return;
   const unsigned cd = code[t.c];
   double rv = r[t.c];
-  if (proj && (cd & (kCodeW | kCodeE | kCodeS | kCodeN | kCodeB | kCodeT))) {
-    rv = rv - proj->red1 / proj->red2;
+  if (cd & (kCodeW | kCodeE | kCodeS | kCodeN | kCodeB | kCodeT)) {
+    rv = rv - sc->red1 / sc->red2;
     r[t.c] = rv;
   }
+  const float sig = (float)sc->sig;
   const float d = diag_code<float>(cd, g.cw[t.i], g.ce[t.i], g.cs[t.j], g.cn[t.j], g.cb[t.k],
                                    g.ct[t.k]);
   const float V = g.dx[t.i] * g.dy[t.j] * g.dz[t.k];
   const float sv = d > 0.f ? (float)rv / V : 0.f;
-  x[t.c] = d > 0.f ? omega * sv / d : 0.f;
+  x[t.c] = d > 0.f ? (MgStore)(omega * sv / d * sig) : (MgStore)0.f;
 }
 
 // Coarse right-hand side: the level-0 residual r - A x = V (s - (d x - sum c
@@ -143,8 +145,8 @@ __device__ __forceinline__ void mgf_jac0_body(Blk bk, GeomT<float> g,
 template <bool VEC>
 __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
                                                     const uint8_t *__restrict__ code,
-                                                    const float *__restrict__ src,
-                                                    const float *__restrict__ x, int cnx, int cny,
+                                                    const MgStore *__restrict__ src,
+                                                    const MgStore *__restrict__ x, int cnx, int cny,
                                                     int cnz, int kchunk, CoarseOut out,
                                                     float sscale) {
   const int pc = bk.x * kMt + (int)threadIdx.x;  // coarse column in the plane
@@ -173,13 +175,13 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
     for (int b = 0; b < 2; ++b) {
       const bool ok = k >= 0 && k < nz && vb[b];
       if constexpr (VEC) {
-        const float2 t = ok ? *reinterpret_cast<const float2 *>(x + at(i0, jb[b], k))
-                            : make_float2(0.f, 0.f);
-        v[b][0] = t.x;
-        v[b][1] = t.y;
+        const Vec<MgStore, 2> t =
+            ok ? ldv<MgStore, 2>(x, at(i0, jb[b], k)) : Vec<MgStore, 2>{};
+        v[b][0] = (float)t.v[0];
+        v[b][1] = (float)t.v[1];
       } else {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) v[b][a] = ok && va[a] ? x[at(ia[a], jb[b], k)] : 0.f;
+        for (int a = 0; a < 2; ++a) v[b][a] = ok && va[a] ? (float)x[at(ia[a], jb[b], k)] : 0.f;
       }
     }
   };
@@ -196,18 +198,18 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
       float ss[2];
       if constexpr (VEC) {
         const uchar2 cd2 = *reinterpret_cast<const uchar2 *>(code + c0);
-        const float2 s2 = *reinterpret_cast<const float2 *>(src + c0);
+        const Vec<MgStore, 2> s2 = ldv<MgStore, 2>(src, c0);
         cds[0] = cd2.x;
         cds[1] = cd2.y;
-        ss[0] = s2.x;
-        ss[1] = s2.y;
+        ss[0] = (float)s2.v[0];
+        ss[1] = (float)s2.v[1];
       } else {
         cds[0] = code[c0];
-        ss[0] = src[c0];
+        ss[0] = (float)src[c0];
         cds[1] = va[1] ? code[c0 + 1] : 0u;
-        ss[1] = va[1] ? src[c0 + 1] : 0.f;
+        ss[1] = va[1] ? (float)src[c0 + 1] : 0.f;
       }
-      const float xw = hw ? x[c0 - 1] : 0.f, xe = he ? x[c0 + 2] : 0.f;
+      const float xw = hw ? (float)x[c0 - 1] : 0.f, xe = he ? (float)x[c0 + 2] : 0.f;
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
         const unsigned cd = cds[a];
@@ -215,8 +217,8 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
         const size_t cc = c0 + a;
         const float west = a == 1 ? c[b][0] : xw;
         const float east = a == 0 ? c[b][1] : xe;
-        const float south = b == 1 ? c[0][a] : (hs ? x[cc - sx] : 0.f);
-        const float north = b == 0 ? c[1][a] : (hn ? x[cc + sx] : 0.f);
+        const float south = b == 1 ? c[0][a] : (hs ? (float)x[cc - sx] : 0.f);
+        const float north = b == 0 ? c[1][a] : (hn ? (float)x[cc + sx] : 0.f);
         float nb = (cd & kCodeW) ? cw[a] * west : 0.f;
         nb += (cd & kCodeE) ? ce[a] * east : 0.f;
         nb += (cd & kCodeS) ? cs[b] * south : 0.f;
@@ -264,8 +266,8 @@ __device__ __forceinline__ void mgf_restrict_z_body(Blk bk, GeomT<float> g,
 template <int CL, bool NT>
 __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Tiling &t,
                                                  const GeomT<float> &g,
-                                                 const float *__restrict__ a,
-                                                 const float *__restrict__ r,
+                                                 const MgStore *__restrict__ a,
+                                                 const MgStore *__restrict__ r,
                                                  const uint8_t *__restrict__ code, int cnx,
                                                  int cny, CoarseOut out, float sscale) {
   constexpr int RC = CL >= 2 ? CL / 2 : 1;  // coarse cells per lane
@@ -273,6 +275,7 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
   const TileCoord tc = tile_coord(t, nx, ny, nz, block);
   if (!tc.valid) return;  // the whole workgroup
   using V = Vec<float, CL>;
+  using H = Vec<MgStore, CL>;  // loads stay FP16 until used (in flight meanwhile)
   const int lane = (int)threadIdx.x, row = (int)threadIdx.y;
   const int i = (tc.i - lane) * CL + CL * lane, j = tc.j;
   const bool act = i < nx && j < ny;
@@ -298,26 +301,26 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
   const int k0 = tc.k0, k1 = tc.k1, last = min(k1, nz - 1);
   auto at = [&](int k) { return base + nxy * (size_t)k; };
   struct Set {
-    V a_next, r;
-    Plane<kDirCurrent, float, CL> pl;
+    H a_next, r;
+    Plane<kDirCurrent, float, CL, MgStore, MgStore> pl;
   };
   Set sa, sb;
-  V a_cur = ldv<float, CL>(a, at(k0));
-  V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
-  sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
+  V a_cur = convert<float>(ldv<MgStore, CL>(a, at(k0)));
+  V a_prev = k0 > 0 ? convert<float>(ldv<MgStore, CL>(a, at(k0 - 1))) : a_cur;
+  sa.a_next = ldv<MgStore, CL>(a, at(min(k0 + 1, last)));
   sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
-  sa.r = ldv_s<NT, float, CL>(r, at(k0));
+  sa.r = ldv_s<NT, MgStore, CL>(r, at(k0));
   float acc[RC];
   const int stride = kTileX * RC, ty = (int)blockDim.y;
   auto stage = [&](int k, const Set &u, Set &ld) {
     const int kn = min(k + 1, k1 - 1);
-    ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
+    ld.a_next = ldv<MgStore, CL>(a, at(min(k + 2, last)));
     ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
-    ld.r = ldv_s<NT, float, CL>(r, at(kn));
+    ld.r = ldv_s<NT, MgStore, CL>(r, at(kn));
     const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
     const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
-    const float vw = lane == 0 ? u.pl.w.a : from_w;
-    const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
+    const float vw = lane == 0 ? (float)u.pl.w.a : from_w;
+    const float ve = lane == kTileX - 1 ? (float)u.pl.e.a : from_e;
     const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
     float con[CL];
 #pragma unroll
@@ -327,12 +330,12 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
       const float east = m == CL - 1 ? ve : a_cur.v[m + 1];
       float nb = (cd & kCodeW) ? cw[m] * west : 0.f;
       nb += (cd & kCodeE) ? ce[m] * east : 0.f;
-      nb += (cd & kCodeS) ? cs * u.pl.s.a.v[m] : 0.f;
-      nb += (cd & kCodeN) ? cn * u.pl.n.a.v[m] : 0.f;
+      nb += (cd & kCodeS) ? cs * (float)u.pl.s.a.v[m] : 0.f;
+      nb += (cd & kCodeN) ? cn * (float)u.pl.n.a.v[m] : 0.f;
       nb += (cd & kCodeB) ? cb * a_prev.v[m] : 0.f;
-      nb += (cd & kCodeT) ? ct * u.a_next.v[m] : 0.f;
+      nb += (cd & kCodeT) ? ct * (float)u.a_next.v[m] : 0.f;
       const float d = diag_code<float>(cd, cw[m], ce[m], cs, cn, cb, ct);
-      const float src = u.r.v[m] * d * sscale;
+      const float src = (float)u.r.v[m] * d * sscale;
       con[m] = (cd & kCodeFluid) ? dxv[m] * dyj * dzk * (src - (d * a_cur.v[m] - nb)) : 0.f;
     }
     float px[RC];
@@ -351,7 +354,7 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
       for (int q = 0; q < RC; ++q) slot[q] = acc[q];
     }
     a_prev = a_cur;
-    a_cur = u.a_next;
+    a_cur = convert<float>(u.a_next);
   };
   for (int k = k0; k < k1; k += 2) {
     stage(k, sa, sb);
@@ -386,12 +389,12 @@ __host__ __device__ inline size_t mg_restrict_lds(int ty, int cl) {
 // held at a time, the next plane's patch loaded over the one just retired;
 // each
 // fine cell takes the trilinear weights of prolong_value (only fine cells that
-// carry unknowns change). VEC (even nx): each fine row pair moves as one
-// 8-byte x and 2-byte code access.
+// carry unknowns change; the sum is formed in FP32 and stored in FP16). VEC
+// (even nx): each fine row pair moves as one 4-byte x and 2-byte code access.
 template <bool VEC>
 __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int nz,
                                                    const uint8_t *__restrict__ code,
-                                                   float *__restrict__ x, int cnx, int cny, int cnz,
+                                                   MgStore *__restrict__ x, int cnx, int cny, int cnz,
                                                    const float *__restrict__ xc,
                                                    const float *__restrict__ mc, int kchunk) {
   const int pc = bk.x * kMt + (int)threadIdx.x;  // coarse column in the plane
@@ -429,18 +432,17 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
       ok[b] = j < ny;
       c0[b] = (size_t)(2 * I) + fsx * ((size_t)(ok[b] ? j : 0) + (size_t)ny * k);
       if constexpr (VEC) {
-        const float2 t = ok[b] ? *reinterpret_cast<const float2 *>(x + c0[b])
-                               : make_float2(0.f, 0.f);
+        const Vec<MgStore, 2> t = ok[b] ? ldv<MgStore, 2>(x, c0[b]) : Vec<MgStore, 2>{};
         const uchar2 u = ok[b] ? *reinterpret_cast<const uchar2 *>(code + c0[b])
                                : make_uchar2(0, 0);
-        xv[b][0] = t.x;
-        xv[b][1] = t.y;
+        xv[b][0] = (float)t.v[0];
+        xv[b][1] = (float)t.v[1];
         cd[b][0] = u.x;
         cd[b][1] = u.y;
       } else {
-        xv[b][0] = ok[b] ? x[c0[b]] : 0.f;
+        xv[b][0] = ok[b] ? (float)x[c0[b]] : 0.f;
         cd[b][0] = ok[b] ? code[c0[b]] : 0u;
-        xv[b][1] = ok[b] && has_i1 ? x[c0[b] + 1] : 0.f;
+        xv[b][1] = ok[b] && has_i1 ? (float)x[c0[b] + 1] : 0.f;
         cd[b][1] = ok[b] && has_i1 ? code[c0[b] + 1] : 0u;
       }
     }
@@ -466,10 +468,13 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
     for (int b = 0; b < 2; ++b) {
       if (!ok[b]) continue;
       if constexpr (VEC) {
-        *reinterpret_cast<float2 *>(x + c0[b]) = make_float2(xv[b][0], xv[b][1]);
+        Vec<MgStore, 2> t;
+        t.v[0] = (MgStore)xv[b][0];
+        t.v[1] = (MgStore)xv[b][1];
+        stv<MgStore, 2>(x, c0[b], t);
       } else {
-        x[c0[b]] = xv[b][0];
-        if (has_i1) x[c0[b] + 1] = xv[b][1];
+        x[c0[b]] = (MgStore)xv[b][0];
+        if (has_i1) x[c0[b] + 1] = (MgStore)xv[b][1];
       }
     }
   };
@@ -737,15 +742,18 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
   if (on) x[t] = xv;
 }
 
-// Streaming damped-Jacobi sweep of level 0 in FP32 (the structure of the
-// reference-Jacobi kernel k_jacobi_t: each lane owns CL x-adjacent cells of
-// one row and walks its z range; plane k+1's loads are issued before plane k
-// is computed; west/east neighbours by wave shuffle):
+// Streaming damped-Jacobi sweep of level 0 in FP32 arithmetic on FP16
+// vectors (the structure of the reference-Jacobi kernel k_jacobi_t: each lane
+// owns CL x-adjacent cells of one row and walks its z range; plane k+1's
+// loads are issued before plane k is computed and stay FP16 until used;
+// west/east neighbours by wave shuffle):
 //   out = (1 - w) a + (w / d) (s + sum c a_nb), 0 where d = 0, s = r / V
-// (read from the first step, see below).
-// LAST: out is the cycle's output z (FP32; the FP64 iteration widens it
-// exactly), and the partial sums z.r = z.(s V) and z.q go to two slots (q = A
-// p_old from the previous update, FP32, or z.q = 0 when q is null).
+// (read from the first step, see below). Every vector carries the cycle
+// scale sig.
+// LAST: out is the cycle's output z (times sig), and the partials of the
+// stored (rounded) values go to three slots: z.r = z.(s V) and z.q (q = A
+// p_old from the previous update, FP32, or z.q = 0 when q is null), and
+// max |z|.
 //
 // CHEB: a Chebyshev step, out = a + c2 (s - A a) / d + c1 (a - prev) (prev
 // null: a zero previous iterate); otherwise damped Jacobi with omega = c2.
@@ -754,19 +762,19 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
 // after the first) the one load serves both.
 template <int CL, bool LAST, bool NT, bool CHEB = false>
 __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *lds, Tiling t,
-                                              GeomT<float> g, const float *__restrict__ a,
-                                              const float *__restrict__ r,
+                                              GeomT<float> g, const MgStore *__restrict__ a,
+                                              const MgStore *__restrict__ r,
                                               const uint8_t *__restrict__ code,
-                                              float *__restrict__ b, double *partials,
+                                              MgStore *__restrict__ b, double *partials,
                                               float omega, const float *__restrict__ q,
-                                              const float *__restrict__ prev, float c1,
+                                              const MgStore *__restrict__ prev, float c1,
                                               float sscale) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
-  double s = 0.0, sq = 0.0;
+  double s = 0.0, sq = 0.0, zmx = 0.0;
   if (l.active) {
     using V = Vec<float, CL>;
-    using VS = Vec<float, CL>;
+    using H = Vec<MgStore, CL>;
     const int i = l.i, j = l.j;
     const size_t nxy = (size_t)nx * ny;
     float cw[CL], ce[CL], dxdy[CL];
@@ -788,34 +796,34 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
     const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
     auto at = [&](int k) { return base + nxy * (size_t)k; };
     struct Set {
-      V a_next;
-      VS r, p;
-      Plane<kDirCurrent, float, CL> pl;
+      H a_next, r, p;
+      Plane<kDirCurrent, float, CL, MgStore, MgStore> pl;
     };
     Set sa, sb;
     const bool same = r == a;  // uniform
-    V a_cur = ldv<float, CL>(a, at(k0));
-    V a_prev = k0 > 0 ? ldv<float, CL>(a, at(k0 - 1)) : a_cur;
-    sa.a_next = ldv<float, CL>(a, at(min(k0 + 1, last)));
+    V a_cur = convert<float>(ldv<MgStore, CL>(a, at(k0)));
+    V a_prev = k0 > 0 ? convert<float>(ldv<MgStore, CL>(a, at(k0 - 1))) : a_cur;
+    sa.a_next = ldv<MgStore, CL>(a, at(min(k0 + 1, last)));
     sa.pl.load(a, a, code, at(k0), os, on, ow, oe);
-    if (!same) sa.r = ldv_s<NT, float, CL>(r, at(k0));
+    if (!same) sa.r = ldv_s<NT, MgStore, CL>(r, at(k0));
     if constexpr (CHEB) {
-      if (prev) sa.p = ldv_s<NT, float, CL>(prev, at(k0));
+      if (prev) sa.p = ldv_s<NT, MgStore, CL>(prev, at(k0));
     }
     auto stage = [&](int k, const Set &u, Set &ld) {
       const int kn = min(k + 1, k1 - 1);
-      ld.a_next = ldv<float, CL>(a, at(min(k + 2, last)));
+      ld.a_next = ldv<MgStore, CL>(a, at(min(k + 2, last)));
       ld.pl.load(a, a, code, at(kn), os, on, ow, oe);
-      if (!same) ld.r = ldv_s<NT, float, CL>(r, at(kn));
+      if (!same) ld.r = ldv_s<NT, MgStore, CL>(r, at(kn));
       if constexpr (CHEB) {
-        if (prev) ld.p = ldv_s<NT, float, CL>(prev, at(kn));
+        if (prev) ld.p = ldv_s<NT, MgStore, CL>(prev, at(kn));
       }
       const float from_w = __shfl_up(a_cur.v[CL - 1], 1, kTileX);
       const float from_e = __shfl_down(a_cur.v[0], 1, kTileX);
-      const float vw = lane == 0 ? u.pl.w.a : from_w;
-      const float ve = lane == kTileX - 1 ? u.pl.e.a : from_e;
+      const float vw = lane == 0 ? (float)u.pl.w.a : from_w;
+      const float ve = lane == kTileX - 1 ? (float)u.pl.e.a : from_e;
       const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
-      V bv, qz;
+      H bv;
+      V qz;
       if constexpr (LAST) {
         if (q) qz = ldv_s<NT, float, CL>(q, at(k));
       }
@@ -823,29 +831,32 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
       for (int m = 0; m < CL; ++m) {
         const float west = m == 0 ? (hw ? vw : 0.f) : a_cur.v[m - 1];
         const float east = m == CL - 1 ? (he ? ve : 0.f) : a_cur.v[m + 1];
-        const float south = hs ? u.pl.s.a.v[m] : 0.f, north = hn ? u.pl.n.a.v[m] : 0.f;
+        const float south = hs ? (float)u.pl.s.a.v[m] : 0.f,
+                    north = hn ? (float)u.pl.n.a.v[m] : 0.f;
         const float d = diag_code<float>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
-        const float src = (same ? a_cur.v[m] : u.r.v[m]) * d * sscale;
+        const float src = (same ? a_cur.v[m] : (float)u.r.v[m]) * d * sscale;
         const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
-                         cb * a_prev.v[m] + ct * u.a_next.v[m];
+                         cb * a_prev.v[m] + ct * (float)u.a_next.v[m];
         float o;
         if constexpr (CHEB) {
           const float av = a_cur.v[m];
           o = d > 0.f ? av + omega * (src + nb - d * av) / d : 0.f;
-          if (c1 != 0.f && d > 0.f) o += c1 * (av - (prev ? u.p.v[m] : 0.f));
+          if (c1 != 0.f && d > 0.f) o += c1 * (av - (prev ? (float)u.p.v[m] : 0.f));
         } else {
           o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
         }
-        bv.v[m] = o;
+        bv.v[m] = (MgStore)o;
         if constexpr (LAST) {
-          const double zd = (double)o;
+          // the sums take z as stored
+          const double zd = (double)(float)bv.v[m];
           s += zd * ((double)src * (double)(dxdy[m] * dzk));
           if (q) sq += zd * (double)qz.v[m];
+          zmx = fmax(zmx, fabs(zd));
         }
       }
-      stv_s<NT, float, CL>(b, at(k), bv);
+      stv_s<NT, MgStore, CL>(b, at(k), bv);
       a_prev = a_cur;
-      a_cur = u.a_next;
+      a_cur = convert<float>(u.a_next);
     };
     for (int k = k0; k < k1; k += 2) {
       stage(k, sa, sb);
@@ -855,13 +866,16 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
   if constexpr (LAST) {
     double v[2] = {s, sq};
     block_sum<2>(v, lds);
-    store_partials(partials, 2, v, nblocks, block);
+    block_max(zmx, lds + 32);
+    const double vals[3] = {v[0], v[1], zmx};
+    store_partials(partials, 3, vals, nblocks, block);
   }
 }
 
 // x += alpha p, r -= alpha A p (the fused PCG update body, p in FP32), the
-// first smoothing step of the next cycle, x1 = w D^-1 r (which also carries
-// the cycle's source), and q = A p, both stored in FP32; with rmax the
+// first smoothing step of the next cycle, x1 = sig w D^-1 r (FP16 with the
+// next cycle's scale sig = sc->sig_next, which becomes sc->sig; x1 also
+// carries the cycle's source), and q = A p in FP32; with rmax the
 // per-workgroup max |r / V|; counts the iteration.
 // XM (XM_NOW, XM_DEFER, XM_TWO) and ppend: the treatment of x (see
 // pcg_upd_body); a deferring update keeps its alpha in sc->alpha_pend for the
@@ -871,17 +885,19 @@ __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds,
                                             GeomT<double> g, const float *__restrict__ p,
                                             const uint8_t *__restrict__ code,
                                             double *__restrict__ x, double *__restrict__ r,
-                                            float *__restrict__ x1, Scalars *sc, double omega,
+                                            MgStore *__restrict__ x1, Scalars *sc, double omega,
                                             double *rmax, float *__restrict__ q32,
                                             const float *__restrict__ ppend = nullptr) {
   const bool broke = sc->status != 0;
-  pcg_upd_body<double, CL, float, false, float, XM>(t, block, nblocks, g, p, code, x, r, x1,
-                                                    nullptr, sc, lds, omega, rmax, q32,
-                                                    nullptr, ppend);
+  const double sig = sc->sig_next;  // no workgroup of this kernel reads sc->sig
+  pcg_upd_body<double, CL, float, false, float, XM, MgStore>(
+      t, block, nblocks, g, p, code, x, r, x1, nullptr, sc, lds, omega * sig, rmax, q32,
+      nullptr, ppend);
   if (block == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     // a deferring iteration that broke down leaves no step behind
     if constexpr (XM == XM_DEFER) sc->alpha_pend = broke ? 0.0 : sc->alpha;
     if (!broke) sc->iters += 1;
+    sc->sig = sig;
   }
 }
 

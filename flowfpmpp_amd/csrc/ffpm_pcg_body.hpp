@@ -69,6 +69,9 @@ template <> struct NtVec<float, 4> { typedef float type __attribute__((ext_vecto
 template <> struct NtVec<double, 1> { typedef double type; };
 template <> struct NtVec<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct NtVec<double, 4> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct NtVec<_Float16, 1> { typedef _Float16 type; };
+template <> struct NtVec<_Float16, 2> { typedef _Float16 type __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<_Float16, 4> { typedef _Float16 type __attribute__((ext_vector_type(4))); };
 
 template <bool NT, typename T, int CL>
 __device__ __forceinline__ Vec<T, CL> ldv_s(const T *__restrict__ a, size_t c) {
@@ -116,6 +119,10 @@ __device__ __forceinline__ void stv_s(T *__restrict__ a, size_t c, const Vec<T, 
 // directions): p = z + beta p_old is formed in T and rounded to PT, and every
 // use of p (its own value, its neighbours in A p) takes the rounded value,
 // so the kernels that read the stored p later see exactly what was used here.
+// z stored in FP16 is the multigrid cycle's output scaled by a power of two
+// (Scalars::sig): value() multiplies it by zinv = 1 / sig (exact). In a
+// kDirCurrent plane (a stencil operand a read alone) a and the array share
+// one type, which may be narrower than T.
 template <typename T, typename PT>
 __device__ __forceinline__ T round_to(T v) {
   if constexpr (std::is_same<T, PT>::value) return v;
@@ -124,9 +131,7 @@ __device__ __forceinline__ T round_to(T v) {
 
 template <int MODE, typename T, int CL, typename ZT = T, typename PT = T>
 struct RawV {
-  static_assert(MODE != kDirCurrent ||
-                    (std::is_same<ZT, T>::value && std::is_same<PT, T>::value),
-                "p_old is T");
+  static_assert(MODE != kDirCurrent || std::is_same<ZT, PT>::value, "a is read from pin");
   Vec<ZT, CL> a;  // z (p_old for kDirCurrent)
   Vec<PT, CL> b;  // p_old for kDirUpdate
   __device__ __forceinline__ void load(const ZT *__restrict__ z, const PT *__restrict__ pin,
@@ -140,20 +145,21 @@ struct RawV {
       b = ldv<PT, CL>(pin, c);
     }
   }
-  __device__ __forceinline__ Vec<T, CL> value(T beta) const {
+  __device__ __forceinline__ Vec<T, CL> value(T beta, T zinv) const {
     Vec<T, CL> v;
 #pragma unroll
-    for (int m = 0; m < CL; ++m)
-      v.v[m] = round_to<T, PT>(MODE == kDirUpdate ? (T)a.v[m] + beta * (T)b.v[m] : (T)a.v[m]);
+    for (int m = 0; m < CL; ++m) {
+      T zv = (T)a.v[m];
+      if constexpr (std::is_same<ZT, _Float16>::value && MODE != kDirCurrent) zv = zv * zinv;
+      v.v[m] = round_to<T, PT>(MODE == kDirUpdate ? zv + beta * (T)b.v[m] : zv);
+    }
     return v;
   }
 };
 
 template <int MODE, typename T, typename ZT = T, typename PT = T>
 struct Raw1 {
-  static_assert(MODE != kDirCurrent ||
-                    (std::is_same<ZT, T>::value && std::is_same<PT, T>::value),
-                "p_old is T");
+  static_assert(MODE != kDirCurrent || std::is_same<ZT, PT>::value, "a is read from pin");
   ZT a;  // z (p_old for kDirCurrent)
   PT b;  // p_old for kDirUpdate
   __device__ __forceinline__ void load(const ZT *__restrict__ z, const PT *__restrict__ pin,
@@ -167,9 +173,11 @@ struct Raw1 {
       b = pin[c];
     }
   }
-  __device__ __forceinline__ T value(T beta) const {
-    if constexpr (MODE != kDirUpdate) return round_to<T, PT>((T)a);
-    else return round_to<T, PT>((T)a + beta * (T)b);
+  __device__ __forceinline__ T value(T beta, T zinv) const {
+    T zv = (T)a;
+    if constexpr (std::is_same<ZT, _Float16>::value && MODE != kDirCurrent) zv = zv * zinv;
+    if constexpr (MODE != kDirUpdate) return round_to<T, PT>(zv);
+    else return round_to<T, PT>(zv + beta * (T)b);
   }
 };
 
@@ -246,6 +254,8 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
                                              double *partials, const Scalars *sc, double *lds) {
   if (sc->status != 0) return;
   const T beta = (T)sc->beta;
+  // an FP16 z carries the cycle scale sig (a power of two)
+  const T zinv = std::is_same<ZT, _Float16>::value ? (T)(1.0 / sc->sig) : T(1);
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
   double s = 0.0;
@@ -282,11 +292,11 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
     RawV<MODE, T, CL, ZT, PT> own_a, own_b;
     Plane<MODE, T, CL, ZT, PT> pl_a, pl_b;
     own_a.load(z, pin, at(k0));
-    Vec<T, CL> p_cur = own_a.value(beta);
+    Vec<T, CL> p_cur = own_a.value(beta, zinv);
     Vec<T, CL> p_prev = p_cur;
     if (k0 > 0) {
       own_a.load(z, pin, at(k0 - 1));
-      p_prev = own_a.value(beta);
+      p_prev = own_a.value(beta, zinv);
     }
     own_a.load(z, pin, at(min(k0 + 1, last)));
     pl_a.load(z, pin, code, at(k0), os, on, ow, oe);
@@ -295,14 +305,14 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
                      Plane<MODE, T, CL, ZT, PT> &pl_load) {
       own_load.load(z, pin, at(min(k + 2, last)));
       pl_load.load(z, pin, code, at(min(k + 1, k1 - 1)), os, on, ow, oe);
-      const Vec<T, CL> p_next = own_use.value(beta);
-      const Vec<T, CL> vs = pl.s.value(beta), vn = pl.n.value(beta);
+      const Vec<T, CL> p_next = own_use.value(beta, zinv);
+      const Vec<T, CL> vs = pl.s.value(beta, zinv), vn = pl.n.value(beta, zinv);
       // The shuffles run on every active lane (a source lane that is masked
       // off reads as 0), then the edge lanes take their loaded values.
       const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
       const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
-      const T vw = lane == 0 ? pl.w.value(beta) : from_w;
-      const T ve = lane == kTileX - 1 ? pl.e.value(beta) : from_e;
+      const T vw = lane == 0 ? pl.w.value(beta, zinv) : from_w;
+      const T ve = lane == kTileX - 1 ? pl.e.value(beta, zinv) : from_e;
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
@@ -335,14 +345,15 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 // with rmax non-null each workgroup also stores max |r / V| over its fluid
 // cells there (an estimate of the max-norm residual), with qout non-null
 // q = A p is stored in ZT (for the flexible-CG coefficient of the next step),
-// and with sout non-null r / V (0 outside fluid cells) is stored in ZT.
+// and with sout non-null r / V (0 outside fluid cells) is stored in ZT. XT:
+// the type z is stored in (ZT unless given).
 template <typename T, int CL, typename ZT = T, bool DOT = true, typename PT = T,
-          int XM = XM_NOW>
+          int XM = XM_NOW, typename XT = ZT>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const PT *__restrict__ p,
                                              const uint8_t *__restrict__ code,
                                              T *__restrict__ x, T *__restrict__ r,
-                                             ZT *__restrict__ z, double *partials,
+                                             XT *__restrict__ z, double *partials,
                                              const Scalars *sc, double *lds,
                                              T zscale = T(1), double *rmax = nullptr,
                                              ZT *__restrict__ qout = nullptr,
@@ -423,7 +434,8 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       ld.r = ldv<T, CL>(r, at(kn));
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
       Vec<T, CL> xv = u.x, rv = u.r;
-      Vec<ZT, CL> zv, qv, sv;
+      Vec<XT, CL> zv;
+      Vec<ZT, CL> qv, sv;
       const Vec<T, CL> p_next = convert<T>(u.p_next);
       const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
       const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
@@ -446,7 +458,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
           zv.v[m] = rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m];
           s += (double)zv.v[m] * (double)rv.v[m];
         } else {
-          zv.v[m] = (ZT)(zscale * rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m]);
+          zv.v[m] = (XT)(zscale * rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m]);
           qv.v[m] = (ZT)q;
           sv.v[m] = d > 0 ? (ZT)(rv.v[m] / (dxdy[m] * dzk)) : ZT(0);
           if (rmax && d > 0) rmx = fmax(rmx, fabs((double)rv.v[m]) / (double)(dxdy[m] * dzk));
@@ -456,7 +468,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       // none is read again by this kernel
       if constexpr (XM != XM_DEFER) stv_s<!DOT, T, CL>(x, at(k), xv);
       stv_s<!DOT, T, CL>(r, at(k), rv);
-      stv_s<!DOT, ZT, CL>(z, at(k), zv);
+      stv_s<!DOT, XT, CL>(z, at(k), zv);
       if constexpr (!DOT) {
         if (qout) stv_s<true, ZT, CL>(qout, at(k), qv);
         if (sout) stv_s<true, ZT, CL>(sout, at(k), sv);

@@ -162,7 +162,7 @@ int true_residual_pass1(Frame &f) {
         f.tiling, f.g, f.phi, f.src, f.diag, f.r, f.partials, f.sc);
     FFPM_LAUNCHED("k_true_residual_1");
   }
-  return finish(f, FIN_TR1, 3, f.tiling.blocks);
+  return finish(f, FIN_TR1, 4, f.tiling.blocks);
 }
 
 int true_residual(Frame &f, bool reset, double *res) {
@@ -866,7 +866,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   // the directions, stored in FP32 (in the FP64 buffers' memory)
   float *pbuf[2] = {reinterpret_cast<float *>(f.next), reinterpret_cast<float *>(f.q)};
   int cur = 0, mode = kDirReset;
-  bool first_ready = false;  // lv[0].x holds w D^-1 r for the current r
+  bool first_ready = false;  // the cycle's first step holds sig w D^-1 r for the current r
   // The true residual's second pass (projection of r) runs inside the next
   // first smoothing step: its z and r.z are not used here, and the projected
   // r is needed only by that step and what follows it.
@@ -890,7 +890,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
     for (int m = 0; m < n; ++m) {
       if (!first_ready && (st = mg_first(mg, f.r, f.sc))) return st;
       if ((st = mg_vcycle(mg, mode == kDirUpdate))) return st;
-      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 2, f.partials, mg.tl.blocks, f.sc);
+      k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 3, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
       const int dst = cur ^ 1;
       // the direction's streams are FP32: it runs on the cycle's FP32 tiling

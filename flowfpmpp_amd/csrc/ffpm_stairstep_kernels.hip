@@ -214,10 +214,13 @@ __global__ void k_chunk_begin(Scalars *sc) {
   if (threadIdx.x == 0) sc->iters = 0;
 }
 
+// After a breakdown: the multigrid cycle scale also starts afresh (a
+// non-finite cycle output ends in a breakdown).
 __global__ void k_clear_status(Scalars *sc) {
   if (threadIdx.x == 0) {
     sc->status = 0;
     sc->iters = 0;
+    sc->sig = 0.0;
   }
 }
 

@@ -353,3 +353,34 @@ def test_directions_on_concurrent_streams_are_bit_identical():
     finally:
         for h in streams:
             S.stream_destroy(h)
+
+
+def test_cycle_scale_follows_the_data():
+    """The cycle's FP16 vectors carry a power-of-two scale taken from the
+    data (the first step's max at the start, then each cycle's largest
+    output), so a problem multiplied by 2^40 or 2^-40 (right-hand side,
+    boundary fluxes, start field and tolerance) takes the same iterations
+    and gives the same field times that factor, bit for bit: no value of
+    the cycle leaves the FP16 range at either end, and nothing in the solve
+    depends on the data's absolute size."""
+    print("case: fcg-mg on data scaled by 2^40 and 2^-40", flush=True)
+    p = W.stretched_frame(64, 48, 32, [W.Sphere((-1.0, 0.0, 0.0), 0.7),
+                                       W.Box((2.0, -1.0, -0.7), (3.5, 1.0, 0.7))], seed=18,
+                          with_fields=False)
+    rhs, bg, faces = W.influence_inputs(p, 1)
+    S.enforce_stairstep_compatibility_3d(rhs, p.solid, bg, *faces, p.dx, p.dy, p.dz)
+    out = {}
+    for e in (0, 40, -40):
+        f = 2.0 ** e
+        ph = np.asfortranarray(p.phi0() * f)
+        sw, res = S.solve_stairstep_poisson_3d(ph, np.asfortranarray(rhs * f), p.solid, bg * f,
+                                               *(np.asfortranarray(x * f) for x in faces), p.xc,
+                                               p.yc, p.zc, p.dx, p.dy, p.dz, 1e-10 * f, 100000,
+                                               25, method="fcg-mg")
+        print(f"scale 2^{e}: {sw} iterations, residual {res / f:.3e} (unscaled)", flush=True)
+        out[e] = (ph / f, sw, res / f)
+    base = out[0]
+    assert base[2] <= 1e-10
+    for e in (40, -40):
+        assert out[e][1] == base[1] and out[e][2] == base[2]
+        assert np.array_equal(out[e][0], base[0])
