@@ -550,8 +550,8 @@ __global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
   // FP32 directions in the FP64 buffers' memory, as the standalone solve
   const float *pin = reinterpret_cast<const float *>(d.p[st.cur]);
   float *pout = reinterpret_cast<float *>(d.p[st.cur ^ 1]);
-  // on the cycle's FP32 tiling, as the standalone solve
-  const Tiling &t = d.mg.tl;
+  // on the FP32 tiling, as the standalone solve
+  const Tiling &t = d.mg.tld;
   if (st.mode == kDirUpdate)
     pcg_dir_body<double, CL, kDirUpdate, MgStore, float>(t, lb, t.blocks, d.g, d.mg.buf0(3),
                                                          d.code, pin, pout, d.partials, d.sc,
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(256) void k_bg_finish(const BSolve *solves, BState 
       nblocks = d.mg.tl.blocks;
       nslots = 3;
     } else if (op == FIN_PQ) {
-      nblocks = d.mg.tl.blocks;  // the direction runs on the FP32 tiling
+      nblocks = d.mg.tld.blocks;  // the direction runs on the FP32 tiling
     }
   } else {
     if (!st.need_tr) return;
@@ -1000,6 +1000,12 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
                              nullptr, nullptr, nullptr, nullptr};
       int st = mg_plan(mg, g64, g32, nullptr, tl32[s], B.cl32[s], nullptr);
       if (st) return st;
+      if (mg.cl != B.cl32[s] || mg.tl.zchunk != tl32[s].zchunk) {
+        // the batched sweeps run on the FP32 launch grids
+        set_error(ERR_ARGUMENT,
+                  "batched multigrid solves need the default FFPM_MG_CL and FFPM_MG_ZCHUNK");
+        return ERR_ARGUMENT;
+      }
       total += align256(mg_state_bytes(mg));
       if (B.owner[s] == s) total += align256(mg_operator_bytes(mg));
       const MgLevel &c1 = mg.lv[1];

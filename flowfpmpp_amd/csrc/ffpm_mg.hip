@@ -356,6 +356,15 @@ int mg_check_interval(int check_interval) {
   return c < 1 ? 1 : c;
 }
 
+int mg_cells_per_lane(int nx) {
+  const int want = (int)env_float("FFPM_MG_CL", 4.f);
+  return cells_per_lane(nx, want == 1 || want == 2 || want == 4 || want == 8 ? want : 4);
+}
+
+Tiling mg_cycle_tiling(int nx, int ny, int nz, int rows, int cl) {
+  return make_tiling(nx, ny, nz, (int)env_float("FFPM_MG_ZCHUNK", 0.f), rows, kTileX * cl);
+}
+
 int mg_rows() {
   const char *e = std::getenv("FFPM_MG_ROWS");
   const int r = e ? std::atoi(e) : 4;
@@ -367,14 +376,19 @@ int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
   mg.g64 = g64;
   mg.g32 = g32;
   mg.code = code;
-  mg.tl = tl;
-  mg.cl = cl;
+  mg.tld = tl;
+  mg.cld = cl;
+  mg.cl = mg_cells_per_lane(g64.nx);
+  // the caller's tiling unless another shape is asked for
+  const bool own = mg.cl != cl || env_float("FFPM_MG_ZCHUNK", 0.f) > 0.f;
+  mg.tl = own ? mg_cycle_tiling(g64.nx, g64.ny, g64.nz, tl.ty, mg.cl) : tl;
   mg.partials = partials;
   {
-    const int rows = tl.ty < 2 ? 2 : tl.ty > 8 ? 8 : tl.ty;
-    int zc = tl.zchunk + (tl.zchunk & 1);
+    const Tiling &tc = mg.tl;
+    const int rows = tc.ty < 2 ? 2 : tc.ty > 8 ? 8 : tc.ty;
+    int zc = tc.zchunk + (tc.zchunk & 1);
     if (zc > kMgRestrictPlanes) zc = kMgRestrictPlanes;
-    mg.tlr = make_tiling(g64.nx, g64.ny, g64.nz, zc, rows, kTileX * cl);
+    mg.tlr = make_tiling(g64.nx, g64.ny, g64.nz, zc, rows, kTileX * mg.cl);
     const char *e = std::getenv("FFPM_MG_RESTRICT");
     mg.stream_restrict = !(e && std::strcmp(e, "column") == 0);
   }
@@ -575,7 +589,8 @@ void launch_step0(const Multigrid &mg, const MgStep &st, const float *q) {
     kern<<<mg.tl.blocks, blk, 0, stream()>>>(mg.tl, mg.g32, a, mg.buf0(0), mg.code, o,
                                              mg.partials, st.c2, q, p, st.c1, mg.sscale);
   };
-  if (mg.cl == 4) go(k_mg_sweep_t<4, LAST, NT, CHEB>);
+  if (mg.cl == 8) go(k_mg_sweep_t<8, LAST, NT, CHEB>);
+  else if (mg.cl == 4) go(k_mg_sweep_t<4, LAST, NT, CHEB>);
   else if (mg.cl == 2) go(k_mg_sweep_t<2, LAST, NT, CHEB>);
   else go(k_mg_sweep_t<1, LAST, NT, CHEB>);
 }
@@ -664,11 +679,13 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
                                                 coarse_out(mg, 1), mg.sscale);
     };
     if (mg.nontemporal) {
-      if (mg.cl == 4) go(k_mg_restrict_t<4, true>);
+      if (mg.cl == 8) go(k_mg_restrict_t<8, true>);
+      else if (mg.cl == 4) go(k_mg_restrict_t<4, true>);
       else if (mg.cl == 2) go(k_mg_restrict_t<2, true>);
       else go(k_mg_restrict_t<1, true>);
     } else {
-      if (mg.cl == 4) go(k_mg_restrict_t<4, false>);
+      if (mg.cl == 8) go(k_mg_restrict_t<8, false>);
+      else if (mg.cl == 4) go(k_mg_restrict_t<4, false>);
       else if (mg.cl == 2) go(k_mg_restrict_t<2, false>);
       else go(k_mg_restrict_t<1, false>);
     }

@@ -109,12 +109,14 @@ struct Multigrid {
   GeomT<float> g32{};
   GeomT<double> g64{};
   const uint8_t *code = nullptr;
-  Tiling tl{};              // level-0 streaming launch shape (CL cells per lane)
+  Tiling tl{};              // the cycle's level-0 streaming launch shape (cl cells per lane)
   Tiling tlr{};             // the restriction's: tl with an even number of rows (2-8)
                             // and z chunks of an even number of planes
   bool stream_restrict = true;  // streaming restriction (FFPM_MG_RESTRICT=column: the
                                 // column kernel)
   int cl = 1;
+  Tiling tld{};             // the direction step's (FP32 p: the caller's tiling, cld cells
+  int cld = 1;              // per lane)
   float omega = 0.9f;        // damped-Jacobi weight on level 0
   float omega_coarse = 0.9f; // and on the coarse levels
   float scale = 0.5f;
@@ -166,6 +168,12 @@ int mg_build(const Multigrid &mg);
 // (augment); allocates the level's work buffers (workspace slot S_MG_LRN) and
 // computes its cell volumes.
 int mg_learned_setup(Multigrid &mg, UNet *net, const float *features, int level, int mode);
+
+// Cells per lane of the cycle's level-0 kernels (FFPM_MG_CL: 1, 2, 4 or 8,
+// default 4; fewer where nx is not a multiple) and their launch shape for
+// `rows` rows (z chunk FFPM_MG_ZCHUNK, default chosen by make_tiling).
+int mg_cells_per_lane(int nx);
+Tiling mg_cycle_tiling(int nx, int ny, int nz, int rows, int cl);
 
 // Rows per workgroup of the level-0 FP32 kernels (the cycle's sweeps):
 // FFPM_MG_ROWS (1, 2, 4, 8, 16), default 4, measured faster than the 8 rows

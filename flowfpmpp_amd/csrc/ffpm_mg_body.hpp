@@ -108,6 +108,30 @@ struct CoarseOut {
 };
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
+// The level-0 diagonal of a lane's CL cells in one plane: the wave's cells
+// almost all have six fluid neighbours (code kCodeAll), whose diagonal is
+// dxy + (cb + ct) with dxy = ((cw + ce) + cs) + cn per lane (one add); a wave
+// with any other cell sums the coefficients of its codes.
+constexpr unsigned kCodeAll = kCodeW | kCodeE | kCodeS | kCodeN | kCodeB | kCodeT | kCodeFluid;
+
+template <int CL>
+__device__ __forceinline__ void level0_diag(float (&d)[CL], const Codes<CL> &code,
+                                            const float (&cw)[CL], const float (&ce)[CL],
+                                            const float (&dxy)[CL], float cs, float cn, float cb,
+                                            float ct) {
+  bool full = true;
+#pragma unroll
+  for (int m = 0; m < CL; ++m) full = full && code.b[m] == kCodeAll;
+  if (__all(full)) {
+    const float czk = cb + ct;
+#pragma unroll
+    for (int m = 0; m < CL; ++m) d[m] = dxy[m] + czk;
+  } else {
+#pragma unroll
+    for (int m = 0; m < CL; ++m) d[m] = diag_code<float>(code.b[m], cw[m], ce[m], cs, cn, cb, ct);
+  }
+}
+
 // The first sweep from zero, x = sig omega D^-1 r = sig omega s / d (D = V
 // diag, s = r / V rounded to FP32, 0 where d = 0; sig = sc->sig, the cycle
 // scale), stored in FP16, which the rest of the cycle also reads as its
@@ -283,15 +307,20 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
   // nothing, so every wave of the workgroup reaches every barrier.
   const int ic = i < nx ? i : nx - CL, jc = j < ny ? j : ny - 1;
   const size_t nxy = (size_t)nx * ny;
-  float cw[CL], ce[CL], dxv[CL];
+  float cw[CL], ce[CL], dxv[CL], dxy[CL], cwn[CL], cen[CL];
+  const float cs = g.cs[jc], cn = g.cn[jc], dyj = g.dy[jc];
+  const bool hw = ic > 0, he = ic + CL < nx, hs = jc > 0, hn = jc < ny - 1;
 #pragma unroll
   for (int m = 0; m < CL; ++m) {
     cw[m] = g.cw[ic + m];
     ce[m] = g.ce[ic + m];
     dxv[m] = g.dx[ic + m];
+    dxy[m] = ((cw[m] + ce[m]) + cs) + cn;
+    // neighbour terms past the domain walls drop out with their coefficient
+    cwn[m] = m == 0 && !hw ? 0.f : cw[m];
+    cen[m] = m == CL - 1 && !he ? 0.f : ce[m];
   }
-  const float cs = g.cs[jc], cn = g.cn[jc], dyj = g.dy[jc];
-  const bool hw = ic > 0, he = ic + CL < nx, hs = jc > 0, hn = jc < ny - 1;
+  const float csn = hs ? cs : 0.f, cnn = hn ? cn : 0.f;
   const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
   // only the edge lanes' west / east loads leave their own cells (a clamped
   // lane's row start could lie before the array)
@@ -322,21 +351,20 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
     const float vw = lane == 0 ? (float)u.pl.w.a : from_w;
     const float ve = lane == kTileX - 1 ? (float)u.pl.e.a : from_e;
     const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
-    float con[CL];
+    const float cbk = k > 0 ? cb : 0.f, ctk = k < nz - 1 ? ct : 0.f;
+    float d[CL], con[CL];
+    level0_diag<CL>(d, u.pl.code, cw, ce, dxy, cs, cn, cb, ct);
 #pragma unroll
     for (int m = 0; m < CL; ++m) {
-      const unsigned cd = act ? u.pl.code.b[m] : 0u;
+      // the iterate is 0 outside the unknowns, so every neighbour term enters
       const float west = m == 0 ? vw : a_cur.v[m - 1];
       const float east = m == CL - 1 ? ve : a_cur.v[m + 1];
-      float nb = (cd & kCodeW) ? cw[m] * west : 0.f;
-      nb += (cd & kCodeE) ? ce[m] * east : 0.f;
-      nb += (cd & kCodeS) ? cs * (float)u.pl.s.a.v[m] : 0.f;
-      nb += (cd & kCodeN) ? cn * (float)u.pl.n.a.v[m] : 0.f;
-      nb += (cd & kCodeB) ? cb * a_prev.v[m] : 0.f;
-      nb += (cd & kCodeT) ? ct * (float)u.a_next.v[m] : 0.f;
-      const float d = diag_code<float>(cd, cw[m], ce[m], cs, cn, cb, ct);
-      const float src = (float)u.r.v[m] * d * sscale;
-      con[m] = (cd & kCodeFluid) ? dxv[m] * dyj * dzk * (src - (d * a_cur.v[m] - nb)) : 0.f;
+      const float nb = cwn[m] * west + cen[m] * east + csn * (float)u.pl.s.a.v[m] +
+                       cnn * (float)u.pl.n.a.v[m] + cbk * a_prev.v[m] +
+                       ctk * (float)u.a_next.v[m];
+      const float src = (float)u.r.v[m] * d[m] * sscale;
+      const bool fl = act && (u.pl.code.b[m] & kCodeFluid);
+      con[m] = fl ? dxv[m] * dyj * dzk * (src - (d[m] * a_cur.v[m] - nb)) : 0.f;
     }
     float px[RC];
     if constexpr (CL >= 2) {
@@ -417,34 +445,44 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
         m[b][a] = ok ? mc[cc] : 0.f;
       }
   };
-  // fine plane k = 2K + c from the patches of plane K (x0) and of its z
-  // neighbour on the side of c (xz); wk: that neighbour's weight
-  auto fine_plane = [&](int k, float wk, const float (&x0)[3][3], const float (&m0)[3][3],
-                        const float (&xz)[3][3], const float (&mz)[3][3]) {
-    if (k >= nz) return;
-    float xv[2][2];
-    unsigned cd[2][2];
-    bool ok[2];
-    size_t c0[2];
+  // The fine rows of plane k (x and code of the two fine rows of the
+  // column), loaded one fine plane ahead of their use so the loads overlap
+  // the computation of the plane before.
+  struct Fine {
+    Vec<MgStore, 2> x[2];
+    uchar2 cd[2];
+  };
+  auto fload = [&](int k, Fine &f) {
+    const bool kin = k < nz;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int j = 2 * J + b;
-      ok[b] = j < ny;
-      c0[b] = (size_t)(2 * I) + fsx * ((size_t)(ok[b] ? j : 0) + (size_t)ny * k);
+      const bool ok = kin && j < ny;
+      const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)(ok ? j : 0) + (size_t)ny * (kin ? k : 0));
       if constexpr (VEC) {
-        const Vec<MgStore, 2> t = ok[b] ? ldv<MgStore, 2>(x, c0[b]) : Vec<MgStore, 2>{};
-        const uchar2 u = ok[b] ? *reinterpret_cast<const uchar2 *>(code + c0[b])
-                               : make_uchar2(0, 0);
-        xv[b][0] = (float)t.v[0];
-        xv[b][1] = (float)t.v[1];
-        cd[b][0] = u.x;
-        cd[b][1] = u.y;
+        f.x[b] = ok ? ldv<MgStore, 2>(x, c0) : Vec<MgStore, 2>{};
+        f.cd[b] = ok ? *reinterpret_cast<const uchar2 *>(code + c0) : make_uchar2(0, 0);
       } else {
-        xv[b][0] = ok[b] ? (float)x[c0[b]] : 0.f;
-        cd[b][0] = ok[b] ? code[c0[b]] : 0u;
-        xv[b][1] = ok[b] && has_i1 ? (float)x[c0[b] + 1] : 0.f;
-        cd[b][1] = ok[b] && has_i1 ? code[c0[b] + 1] : 0u;
+        f.x[b].v[0] = ok ? x[c0] : (MgStore)0.f;
+        f.cd[b].x = ok ? code[c0] : 0;
+        f.x[b].v[1] = ok && has_i1 ? x[c0 + 1] : (MgStore)0.f;
+        f.cd[b].y = ok && has_i1 ? code[c0 + 1] : 0;
       }
+    }
+  };
+  // fine plane k = 2K + c (its rows in f) from the patches of plane K (x0)
+  // and of its z neighbour on the side of c (xz); wk: that neighbour's weight
+  auto fine_plane = [&](int k, float wk, const float (&x0)[3][3], const float (&m0)[3][3],
+                        const float (&xz)[3][3], const float (&mz)[3][3], const Fine &f) {
+    if (k >= nz) return;
+    float xv[2][2];
+    unsigned cd[2][2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      xv[b][0] = (float)f.x[b].v[0];
+      xv[b][1] = (float)f.x[b].v[1];
+      cd[b][0] = f.cd[b].x;
+      cd[b][1] = f.cd[b].y;
     }
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
@@ -461,38 +499,47 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
         const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
         const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
         const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
-        xv[b][a] += den > 0.f ? num / den : 0.f;
+        xv[b][a] += den > 0.f ? num * __builtin_amdgcn_rcpf(den) : 0.f;
       }
     }
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      if (!ok[b]) continue;
+      const int j = 2 * J + b;
+      if (j >= ny) continue;
+      const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)j + (size_t)ny * k);
       if constexpr (VEC) {
         Vec<MgStore, 2> t;
         t.v[0] = (MgStore)xv[b][0];
         t.v[1] = (MgStore)xv[b][1];
-        stv<MgStore, 2>(x, c0[b], t);
+        stv<MgStore, 2>(x, c0, t);
       } else {
-        x[c0[b]] = (MgStore)xv[b][0];
-        if (has_i1) x[c0[b] + 1] = (MgStore)xv[b][1];
+        x[c0] = (MgStore)xv[b][0];
+        if (has_i1) x[c0 + 1] = (MgStore)xv[b][1];
       }
     }
   };
   // pa: the patch of plane K, pb: of plane K - 1 (before fine plane 2K) or
-  // K + 1 (before fine plane 2K + 1); the two swap roles each step
+  // K + 1 (before fine plane 2K + 1); the two swap roles each step. fa holds
+  // the rows of the even fine plane, fb of the odd one.
   float xa[3][3], ma[3][3], xb[3][3], mb[3][3];
+  Fine fa, fb;
   cload(K0 - 1, xb, mb);
   cload(K0, xa, ma);
+  fload(2 * K0, fa);
   int K = K0;
   while (K < K1) {
-    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xa, ma, xb, mb);
+    fload(2 * K + 1, fb);
+    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xa, ma, xb, mb, fa);
     cload(K + 1, xb, mb);
-    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xa, ma, xb, mb);
+    if (K + 1 < K1) fload(2 * K + 2, fa);
+    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xa, ma, xb, mb, fb);
     if (++K >= K1) break;
     // plane K (old K + 1) is in xb: the roles swap
-    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xb, mb, xa, ma);
+    fload(2 * K + 1, fb);
+    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xb, mb, xa, ma, fa);
     cload(K + 1, xa, ma);
-    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xb, mb, xa, ma);
+    if (K + 1 < K1) fload(2 * K + 2, fa);
+    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xb, mb, xa, ma, fb);
     ++K;
   }
 }
@@ -777,16 +824,26 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
     using H = Vec<MgStore, CL>;
     const int i = l.i, j = l.j;
     const size_t nxy = (size_t)nx * ny;
-    float cw[CL], ce[CL], dxdy[CL];
+    float cw[CL], ce[CL], dxdy[CL], dxy[CL];
+    const float cs = g.cs[j], cn = g.cn[j];
 #pragma unroll
     for (int m = 0; m < CL; ++m) {
       cw[m] = g.cw[i + m];
       ce[m] = g.ce[i + m];
       dxdy[m] = g.dx[i + m] * g.dy[j];
+      dxy[m] = ((cw[m] + ce[m]) + cs) + cn;
     }
-    const float cs = g.cs[j], cn = g.cn[j];
     const float keep = 1.f - omega;
     const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
+    // neighbour terms past the domain walls drop out with their coefficient
+    // (the loads there are clamped to cells of the grid)
+    float cwn[CL], cen[CL];
+#pragma unroll
+    for (int m = 0; m < CL; ++m) {
+      cwn[m] = m == 0 && !hw ? 0.f : cw[m];
+      cen[m] = m == CL - 1 && !he ? 0.f : ce[m];
+    }
+    const float csn = hs ? cs : 0.f, cnn = hn ? cn : 0.f;
     const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
     const int lane = (int)threadIdx.x;
     const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
@@ -822,33 +879,39 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
       const float vw = lane == 0 ? (float)u.pl.w.a : from_w;
       const float ve = lane == kTileX - 1 ? (float)u.pl.e.a : from_e;
       const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
+      const float cbk = k > 0 ? cb : 0.f, ctk = k < nz - 1 ? ct : 0.f;
       H bv;
       V qz;
       if constexpr (LAST) {
         if (q) qz = ldv_s<NT, float, CL>(q, at(k));
       }
+      float d[CL];
+      level0_diag<CL>(d, u.pl.code, cw, ce, dxy, cs, cn, cb, ct);
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
-        const float west = m == 0 ? (hw ? vw : 0.f) : a_cur.v[m - 1];
-        const float east = m == CL - 1 ? (he ? ve : 0.f) : a_cur.v[m + 1];
-        const float south = hs ? (float)u.pl.s.a.v[m] : 0.f,
-                    north = hn ? (float)u.pl.n.a.v[m] : 0.f;
-        const float d = diag_code<float>(u.pl.code.b[m], cw[m], ce[m], cs, cn, cb, ct);
-        const float src = (same ? a_cur.v[m] : (float)u.r.v[m]) * d * sscale;
-        const float nb = cw[m] * west + ce[m] * east + cs * south + cn * north +
-                         cb * a_prev.v[m] + ct * (float)u.a_next.v[m];
+        const float west = m == 0 ? vw : a_cur.v[m - 1];
+        const float east = m == CL - 1 ? ve : a_cur.v[m + 1];
+        // the source s = r d sscale enters divided by d: rs = r sscale
+        const float rs = (same ? a_cur.v[m] : (float)u.r.v[m]) * sscale;
+        const float nb = cwn[m] * west + cen[m] * east + csn * (float)u.pl.s.a.v[m] +
+                         cnn * (float)u.pl.n.a.v[m] + cbk * a_prev.v[m] +
+                         ctk * (float)u.a_next.v[m];
+        const float rd = __builtin_amdgcn_rcpf(d[m]);
+        const float t = nb * rd + rs;  // (s + sum c a_nb) / d
         float o;
         if constexpr (CHEB) {
           const float av = a_cur.v[m];
-          o = d > 0.f ? av + omega * (src + nb - d * av) / d : 0.f;
-          if (c1 != 0.f && d > 0.f) o += c1 * (av - (prev ? (float)u.p.v[m] : 0.f));
+          o = av + omega * (t - av);
+          if (c1 != 0.f) o += c1 * (av - (prev ? (float)u.p.v[m] : 0.f));
         } else {
-          o = d > 0.f ? keep * a_cur.v[m] + omega / d * (src + nb) : 0.f;
+          o = keep * a_cur.v[m] + omega * t;
         }
+        o = d[m] > 0.f ? o : 0.f;
         bv.v[m] = (MgStore)o;
         if constexpr (LAST) {
           // the sums take z as stored
           const double zd = (double)(float)bv.v[m];
+          const float src = rs * d[m];
           s += zd * ((double)src * (double)(dxdy[m] * dzk));
           if (q) sq += zd * (double)qz.v[m];
           zmx = fmax(zmx, fabs(zd));

@@ -72,6 +72,8 @@ template <> struct NtVec<double, 4> { typedef double type __attribute__((ext_vec
 template <> struct NtVec<_Float16, 1> { typedef _Float16 type; };
 template <> struct NtVec<_Float16, 2> { typedef _Float16 type __attribute__((ext_vector_type(2))); };
 template <> struct NtVec<_Float16, 4> { typedef _Float16 type __attribute__((ext_vector_type(4))); };
+template <> struct NtVec<_Float16, 8> { typedef _Float16 type __attribute__((ext_vector_type(8))); };
+template <> struct NtVec<float, 8> { typedef float type __attribute__((ext_vector_type(8))); };
 
 template <bool NT, typename T, int CL>
 __device__ __forceinline__ Vec<T, CL> ldv_s(const T *__restrict__ a, size_t c) {

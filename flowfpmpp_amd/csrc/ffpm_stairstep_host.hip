@@ -85,9 +85,14 @@ int frame_setup(Frame &f, int nx, int ny, int nz, const double *xc, const double
   for (int cl : {1, 2, 4}) {
     f.tl[cl] = make_tiling(nx, ny, nz, 0, pcg_rows, kTileX * cl);
     if ((size_t)f.tl[cl].blocks > nb) nb = (size_t)f.tl[cl].blocks;
-    // the multigrid cycle's FP32 level-0 tiling (its sweeps' partials)
+    // the multigrid direction's FP32 level-0 tiling (its partials)
     const Tiling tm = make_tiling(nx, ny, nz, 0, mg_rows(), kTileX * cl);
     if ((size_t)tm.blocks > nb) nb = (size_t)tm.blocks;
+  }
+  {
+    // the cycle's level-0 tiling (its last sweep's partials)
+    const Tiling tc = mg_cycle_tiling(nx, ny, nz, mg_rows(), mg_cells_per_lane(nx));
+    if ((size_t)tc.blocks > nb) nb = (size_t)tc.blocks;
   }
   double *coef = (double *)ws.get(S_COEF, sizeof(double) * 2 * (nx + ny + nz), &st);
   if (!coef) return st;
@@ -893,9 +898,9 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
       k_finish<<<1, 256, 0, stream()>>>(FIN_FCG1, 3, f.partials, mg.tl.blocks, f.sc);
       FFPM_LAUNCHED("k_finish");
       const int dst = cur ^ 1;
-      // the direction's streams are FP32: it runs on the cycle's FP32 tiling
-      if ((st = mg_direction(mg, mg.tl, mg.cl, pbuf[cur], pbuf[dst], mode, f.sc))) return st;
-      if ((st = finish(f, FIN_PQ, 1, mg.tl.blocks))) return st;
+      // the direction's streams are FP32: it runs on the FP32 tiling
+      if ((st = mg_direction(mg, mg.tld, mg.cld, pbuf[cur], pbuf[dst], mode, f.sc))) return st;
+      if ((st = finish(f, FIN_PQ, 1, mg.tld.blocks))) return st;
       cur = dst;
       // the chunk's last update also estimates the max-norm residual from
       // the recursively updated r
