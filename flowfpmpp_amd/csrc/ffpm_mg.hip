@@ -47,60 +47,60 @@ template <bool VEC>
 __global__ __launch_bounds__(256) void k_mgf_prolong_z(int nx, int ny, int nz,
                                                        const uint8_t *__restrict__ code,
                                                        MgStore *__restrict__ x, int cnx, int cny,
-                                                       int cnz, const float *__restrict__ xc,
-                                                       const float *__restrict__ mc,
+                                                       int cnz, const MgStore *__restrict__ xc,
+                                                       const MgStore *__restrict__ mc,
                                                        int kchunk) {
   mgf_prolong_z_body<VEC>(blk3(), nx, ny, nz, code, x, cnx, cny, cnz, xc, mc, kchunk);
 }
 
 __global__ __launch_bounds__(256) void k_mgc_sweep(int nx, int ny, int nz,
-                                                   const float *__restrict__ gE,
-                                                   const float *__restrict__ gN,
-                                                   const float *__restrict__ gT,
-                                                   const float *__restrict__ b,
-                                                   const float *__restrict__ in,
-                                                   const float *__restrict__ prev,
-                                                   float *__restrict__ out, float c1, float c2) {
+                                                   const MgStore *__restrict__ gE,
+                                                   const MgStore *__restrict__ gN,
+                                                   const MgStore *__restrict__ gT,
+                                                   const MgStore *__restrict__ b,
+                                                   const MgStore *__restrict__ in,
+                                                   const MgStore *__restrict__ prev,
+                                                   MgStore *__restrict__ out, float c1, float c2) {
   mgc_sweep_body(blk3(), nx, ny, nz, gE, gN, gT, b, in, prev, out, c1, c2);
 }
 
 template <int CL>
 __global__ __launch_bounds__(256) void k_mgc_sweep_t(Tiling t, int nx, int ny, int nz,
-                                                     const float *__restrict__ gE,
-                                                     const float *__restrict__ gN,
-                                                     const float *__restrict__ gT,
-                                                     const float *__restrict__ b,
-                                                     const float *__restrict__ in,
-                                                     const float *__restrict__ prev,
-                                                     float *__restrict__ out, float c1,
+                                                     const MgStore *__restrict__ gE,
+                                                     const MgStore *__restrict__ gN,
+                                                     const MgStore *__restrict__ gT,
+                                                     const MgStore *__restrict__ b,
+                                                     const MgStore *__restrict__ in,
+                                                     const MgStore *__restrict__ prev,
+                                                     MgStore *__restrict__ out, float c1,
                                                      float c2) {
   mgc_sweep_t_body<CL>(blockIdx.x, t, nx, ny, nz, gE, gN, gT, b, in, prev, out, c1, c2);
 }
 
 __global__ __launch_bounds__(256) void k_mgc_restrict(int nx, int ny, int nz,
-                                                      const float *__restrict__ gE,
-                                                      const float *__restrict__ gN,
-                                                      const float *__restrict__ gT,
-                                                      const float *__restrict__ b,
-                                                      const float *__restrict__ x, int cnx,
+                                                      const MgStore *__restrict__ gE,
+                                                      const MgStore *__restrict__ gN,
+                                                      const MgStore *__restrict__ gT,
+                                                      const MgStore *__restrict__ b,
+                                                      const MgStore *__restrict__ x, int cnx,
                                                       int cny, CoarseOut out) {
   mgc_restrict_body(blk3(), nx, ny, nz, gE, gN, gT, b, x, cnx, cny, out);
 }
 
-__global__ __launch_bounds__(256) void k_mgc_prolong(int nx, int ny, const float *__restrict__ m,
-                                                     float *__restrict__ x, int cnx, int cny,
-                                                     int cnz, const float *__restrict__ xc,
-                                                     const float *__restrict__ mc) {
+__global__ __launch_bounds__(256) void k_mgc_prolong(int nx, int ny, const MgStore *__restrict__ m,
+                                                     MgStore *__restrict__ x, int cnx, int cny,
+                                                     int cnz, const MgStore *__restrict__ xc,
+                                                     const MgStore *__restrict__ mc) {
   mgc_prolong_body(blk3(), nx, ny, m, x, cnx, cny, cnz, xc, mc);
 }
 
 __global__ __launch_bounds__(1024) void k_mgc_coarsest(int nx, int ny, int nz,
-                                                       const float *__restrict__ gE,
-                                                       const float *__restrict__ gN,
-                                                       const float *__restrict__ gT,
-                                                       const float *__restrict__ b,
-                                                       const float *__restrict__ rd,
-                                                       float *__restrict__ x, float omega,
+                                                       const MgStore *__restrict__ gE,
+                                                       const MgStore *__restrict__ gN,
+                                                       const MgStore *__restrict__ gT,
+                                                       const MgStore *__restrict__ b,
+                                                       const MgStore *__restrict__ rd,
+                                                       MgStore *__restrict__ x, float omega,
                                                        int sweeps) {
   __shared__ float xs[kMgCoarsestCells];
   mgc_coarsest_body(xs, nx, ny, nz, gE, gN, gT, b, rd, x, omega, sweeps);
@@ -170,11 +170,11 @@ __global__ __launch_bounds__(256) void k_lrn_volume(int L, int nx, int ny, int n
 // res = b - A x where the level carries an unknown (else 0), and one partial
 // max |res| / V per workgroup.
 __global__ __launch_bounds__(256) void k_lrn_residual(int nx, int ny, int nz,
-                                                      const float *__restrict__ gE,
-                                                      const float *__restrict__ gN,
-                                                      const float *__restrict__ gT,
-                                                      const float *__restrict__ b,
-                                                      const float *__restrict__ x,
+                                                      const MgStore *__restrict__ gE,
+                                                      const MgStore *__restrict__ gN,
+                                                      const MgStore *__restrict__ gT,
+                                                      const MgStore *__restrict__ b,
+                                                      const MgStore *__restrict__ x,
                                                       const float *__restrict__ vol,
                                                       float *__restrict__ res,
                                                       double *__restrict__ part) {
@@ -220,9 +220,9 @@ __global__ __launch_bounds__(256) void k_lrn_source(size_t n, const float *__res
 
 // q = A c where the level carries an unknown; partials of c.res and c.q.
 __global__ __launch_bounds__(256) void k_lrn_dots(int nx, int ny, int nz,
-                                                  const float *__restrict__ gE,
-                                                  const float *__restrict__ gN,
-                                                  const float *__restrict__ gT,
+                                                  const MgStore *__restrict__ gE,
+                                                  const MgStore *__restrict__ gN,
+                                                  const MgStore *__restrict__ gT,
                                                   const float *__restrict__ c,
                                                   const float *__restrict__ res,
                                                   double *__restrict__ part) {
@@ -246,10 +246,10 @@ __global__ __launch_bounds__(256) void k_lrn_dots(int nx, int ny, int nz,
 }
 
 // x += alpha c where the level carries an unknown.
-__global__ __launch_bounds__(256) void k_lrn_add(size_t n, const float *__restrict__ m,
+__global__ __launch_bounds__(256) void k_lrn_add(size_t n, const MgStore *__restrict__ m,
                                                  const float *__restrict__ c,
                                                  const double *__restrict__ sc,
-                                                 float *__restrict__ x) {
+                                                 MgStore *__restrict__ x) {
   const float a = (float)sc[1];
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -260,27 +260,27 @@ __global__ __launch_bounds__(256) void k_lrn_add(size_t n, const float *__restri
 __global__ __launch_bounds__(256) void k_mg_coarsen0(GeomT<double> g,
                                                      const uint8_t *__restrict__ code, int cnx,
                                                      int cny, float scale,
-                                                     float *__restrict__ gE,
-                                                     float *__restrict__ gN,
-                                                     float *__restrict__ gT) {
+                                                     MgStore *__restrict__ gE,
+                                                     MgStore *__restrict__ gN,
+                                                     MgStore *__restrict__ gT) {
   mg_coarsen0_body(blk3(), g, code, cnx, cny, scale, gE, gN, gT);
 }
 
 __global__ __launch_bounds__(256) void k_mg_coarsen(int nx, int ny, int nz,
-                                                    const float *__restrict__ fE,
-                                                    const float *__restrict__ fN,
-                                                    const float *__restrict__ fT, int cnx,
+                                                    const MgStore *__restrict__ fE,
+                                                    const MgStore *__restrict__ fN,
+                                                    const MgStore *__restrict__ fT, int cnx,
                                                     int cny, float scale,
-                                                    float *__restrict__ gE,
-                                                    float *__restrict__ gN,
-                                                    float *__restrict__ gT) {
+                                                    MgStore *__restrict__ gE,
+                                                    MgStore *__restrict__ gN,
+                                                    MgStore *__restrict__ gT) {
   mg_coarsen_body(blk3(), nx, ny, nz, fE, fN, fT, cnx, cny, scale, gE, gN, gT);
 }
 
-__global__ __launch_bounds__(256) void k_mg_mask(int nx, int ny, const float *__restrict__ gE,
-                                                 const float *__restrict__ gN,
-                                                 const float *__restrict__ gT, float first,
-                                                 float *__restrict__ m, float *__restrict__ rd) {
+__global__ __launch_bounds__(256) void k_mg_mask(int nx, int ny, const MgStore *__restrict__ gE,
+                                                 const MgStore *__restrict__ gN,
+                                                 const MgStore *__restrict__ gT, float first,
+                                                 MgStore *__restrict__ m, MgStore *__restrict__ rd) {
   mg_mask_body(blk3(), nx, ny, gE, gN, gT, first, m, rd);
 }
 
@@ -453,9 +453,10 @@ namespace {
 
 size_t up256(size_t b) { return (b + 255) / 256 * 256; }
 
-float *carve(char *&base, size_t n) {
-  float *p = reinterpret_cast<float *>(base);
-  base += up256(sizeof(float) * n);
+template <typename T>
+T *carve(char *&base, size_t n) {
+  T *p = reinterpret_cast<T *>(base);
+  base += up256(sizeof(T) * n);
   return p;
 }
 
@@ -463,39 +464,36 @@ float *carve(char *&base, size_t n) {
 
 size_t mg_operator_bytes(const Multigrid &mg) {
   size_t bytes = 0;
-  for (int l = 1; l < mg.levels; ++l) bytes += 5 * up256(sizeof(float) * mg.lv[l].n);
+  for (int l = 1; l < mg.levels; ++l) bytes += 5 * up256(sizeof(MgStore) * mg.lv[l].n);
   return bytes;
 }
 
 size_t mg_state_bytes(const Multigrid &mg) {
   size_t bytes = 5 * up256(sizeof(MgStore) * mg.lv[0].n) + up256(sizeof(float) * mg.lv[0].n);
-  for (int l = 1; l < mg.levels; ++l) bytes += 4 * up256(sizeof(float) * mg.lv[l].n);
+  for (int l = 1; l < mg.levels; ++l) bytes += 4 * up256(sizeof(MgStore) * mg.lv[l].n);
   return bytes;
 }
 
 void mg_bind_operator(Multigrid &mg, char *&base) {
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
-    v.gE = carve(base, v.n);
-    v.gN = carve(base, v.n);
-    v.gT = carve(base, v.n);
-    v.m = carve(base, v.n);
-    v.rd = carve(base, v.n);
+    v.gE = carve<MgStore>(base, v.n);
+    v.gN = carve<MgStore>(base, v.n);
+    v.gT = carve<MgStore>(base, v.n);
+    v.m = carve<MgStore>(base, v.n);
+    v.rd = carve<MgStore>(base, v.n);
   }
 }
 
 void mg_bind_state(Multigrid &mg, char *&base) {
-  for (int i = 0; i < 5; ++i) {
-    mg.f0[i] = reinterpret_cast<MgStore *>(base);
-    base += up256(sizeof(MgStore) * mg.lv[0].n);
-  }
-  mg.q32 = carve(base, mg.lv[0].n);
+  for (int i = 0; i < 5; ++i) mg.f0[i] = carve<MgStore>(base, mg.lv[0].n);
+  mg.q32 = carve<float>(base, mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
-    v.x = carve(base, v.n);
-    v.y = carve(base, v.n);
-    v.w = carve(base, v.n);
-    v.b = carve(base, v.n);
+    v.x = carve<MgStore>(base, v.n);
+    v.y = carve<MgStore>(base, v.n);
+    v.w = carve<MgStore>(base, v.n);
+    v.b = carve<MgStore>(base, v.n);
   }
 }
 
@@ -702,8 +700,9 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
   }
   FFPM_LAUNCHED("k_mgf_restrict_z");
   auto csweep = [&](const MgLevel &v, const MgStep &st) {
-    const float *a = Multigrid::bufc(v, st.a), *p = st.p >= 0 ? Multigrid::bufc(v, st.p) : nullptr;
-    float *o = Multigrid::bufc(v, st.o);
+    const MgStore *a = Multigrid::bufc(v, st.a),
+                  *p = st.p >= 0 ? Multigrid::bufc(v, st.p) : nullptr;
+    MgStore *o = Multigrid::bufc(v, st.o);
     if (v.cl) {
       auto go = [&](auto kern) {
         kern<<<v.tl.blocks, dim3(kTileX, v.tl.ty), 0, stream()>>>(
@@ -724,7 +723,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
   const bool replace = mg.lrn.net && mg.lrn.mode == 0;
   const bool augment = mg.lrn.net && mg.lrn.mode == 1;
   const int Ln = replace ? mg.lrn.level : L;
-  auto learned_correct = [&](const MgLevel &v, float *xm) -> int {
+  auto learned_correct = [&](const MgLevel &v, MgStore *xm) -> int {
     const MgLearned &N = mg.lrn;
     const dim3 grid = cell_grid(v.nx, v.ny, v.nz);
     k_lrn_residual<<<grid, blk, 0, stream()>>>(v.nx, v.ny, v.nz, v.gE, v.gN, v.gT, v.b, xm,
@@ -758,7 +757,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     const MgLevel &v = mg.lv[Ln];
     TimedLaunch tl("mg_learned");
     for (int k = 0; k < sc.npre; ++k) csweep(v, sc.pre[k]);
-    float *xm = Multigrid::bufc(v, sc.mid);
+    MgStore *xm = Multigrid::bufc(v, sc.mid);
     int st = learned_correct(v, xm);
     if (st) return st;
     for (int k = 0; k < sc.npost; ++k) csweep(v, sc.post[k]);
@@ -791,7 +790,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
     TimedLaunch tl("mg_fine_prolong");
     const int kc = column_chunk(c.nx, c.ny, c.nz);
     MgStore *x0 = mg.buf0(s0.mid);
-    const float *xc = Multigrid::bufc(c, sc.out);
+    const MgStore *xc = Multigrid::bufc(c, sc.out);
     if (f0.nx % 2 == 0)
       k_mgf_prolong_z<true><<<column_grid(c.nx, c.ny, c.nz, kc), blk, 0, stream()>>>(
           f0.nx, f0.ny, f0.nz, mg.code, x0, c.nx, c.ny, c.nz, xc, c.m, kc);

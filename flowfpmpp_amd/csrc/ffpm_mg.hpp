@@ -24,16 +24,17 @@
 // damped-Jacobi sweeps in one workgroup. Every step is a fixed linear map, so the
 // CG around it sees the same preconditioner at every iteration.
 //
-// Storage: the level-0 vectors of the cycle (its first step, which also
-// carries the source, the iterates and the output z) are FP16, scaled by a
+// Storage: every vector of the cycle (on level 0 its first step, which also
+// carries the source, the iterates and the output z) is FP16, scaled by a
 // power of two per cycle (Scalars::sig, from the previous cycle's largest
 // output) so that their values sit near 2^10: the cycle is linear, so the
 // scale divides out exactly where z leaves it (the direction step, the z.r
 // and z.q sums). That halves the level-0 bytes of the cycle; with the
 // rounding (11 bits) the 512x384x256 frame takes the same 20 iterations per
 // direction as with FP32 vectors (a CPU model of the cycle: one more of 18
-// in one direction of a 256x192x128 frame). The coarse levels, whose values
-// carry the same scale, stay FP32.
+// in one direction of a 256x192x128 frame). The coarse levels' vectors
+// carry the same scale; their conductances are FP16 too. Arithmetic is FP32
+// throughout.
 #pragma once
 
 #include "ffpm_pcg.hpp"
@@ -49,11 +50,14 @@ constexpr int kMgCoarsestCells = 1024;
 struct MgLevel {
   int nx = 0, ny = 0, nz = 0;
   size_t n = 0;
-  float *gE = nullptr, *gN = nullptr, *gT = nullptr;  // coarse levels only
-  float *m = nullptr;                                 // coarse levels: 1 where diag > 0
-  float *rd = nullptr;                                // coarse levels: first-step scale / diag
-  float *x = nullptr, *y = nullptr, *w = nullptr;     // coarse levels: iterate buffers
-  float *b = nullptr;                                 // coarse levels only
+  // coarse levels only, all FP16: the conductances, the unknown mask (1
+  // where diag > 0), the first-step scale over diag, iterates and right-hand
+  // side
+  MgStore *gE = nullptr, *gN = nullptr, *gT = nullptr;
+  MgStore *m = nullptr;
+  MgStore *rd = nullptr;
+  MgStore *x = nullptr, *y = nullptr, *w = nullptr;
+  MgStore *b = nullptr;
   // coarse levels of at least kMgStreamCells cells: the streaming sweep's
   // launch shape and cells per lane (cl = 0: the one-cell sweep)
   Tiling tl{};
@@ -137,7 +141,7 @@ struct Multigrid {
   // the cycle's output z = B r)
   MgStore *f0[5] = {};
   __host__ __device__ MgStore *buf0(int i) const { return f0[i]; }
-  __host__ __device__ static float *bufc(const MgLevel &v, int i) {
+  __host__ __device__ static MgStore *bufc(const MgLevel &v, int i) {
     return i == 0 ? v.x : i == 1 ? v.y : v.w;
   }
 };

@@ -86,9 +86,9 @@ __device__ __forceinline__ Idx cell_idx(int nx, int ny, Blk bk) {
 }
 
 // Diagonal of a coarse level: the sum of the six face conductances.
-__device__ __forceinline__ float coarse_diag(int nx, int ny, const float *__restrict__ gE,
-                                             const float *__restrict__ gN,
-                                             const float *__restrict__ gT, size_t c, int i,
+__device__ __forceinline__ float coarse_diag(int nx, int ny, const MgStore *__restrict__ gE,
+                                             const MgStore *__restrict__ gN,
+                                             const MgStore *__restrict__ gT, size_t c, int i,
                                              int j, int k) {
   const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
   const float gw = i > 0 ? gE[c - 1] : 0.f, gs = j > 0 ? gN[c - sx] : 0.f,
@@ -99,11 +99,12 @@ __device__ __forceinline__ float coarse_diag(int nx, int ny, const float *__rest
 // The coarse level's first sweep from zero, x = omega b / diag (0 where the
 // diagonal is 0), formed by the restriction that produces b.
 struct CoarseOut {
-  float *b, *x;
-  const float *rd;  // omega / diag, 0 where the diagonal is 0
+  MgStore *b, *x;
+  const MgStore *rd;  // omega / diag, 0 where the diagonal is 0
   __device__ __forceinline__ void store(size_t c, float bv) const {
-    b[c] = bv;
-    x[c] = rd[c] * bv;
+    const MgStore bh = (MgStore)bv;
+    b[c] = bh;
+    x[c] = (MgStore)((float)rd[c] * (float)bh);
   }
 };
 
@@ -423,8 +424,8 @@ template <bool VEC>
 __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int nz,
                                                    const uint8_t *__restrict__ code,
                                                    MgStore *__restrict__ x, int cnx, int cny, int cnz,
-                                                   const float *__restrict__ xc,
-                                                   const float *__restrict__ mc, int kchunk) {
+                                                   const MgStore *__restrict__ xc,
+                                                   const MgStore *__restrict__ mc, int kchunk) {
   const int pc = bk.x * kMt + (int)threadIdx.x;  // coarse column in the plane
   const int J = pc / cnx, I = pc - J * cnx;
   if (J >= cny) return;
@@ -432,17 +433,23 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
   const bool vi[3] = {I > 0, true, I + 1 < cnx}, vj[3] = {J > 0, true, J + 1 < cny};
   const int ii[3] = {vi[0] ? I - 1 : I, I, vi[2] ? I + 1 : I};
   const int jj[3] = {vj[0] ? J - 1 : J, J, vj[2] ? J + 1 : J};
-  const size_t csx = (size_t)cnx, fsx = (size_t)nx;
+  const size_t fsx = (size_t)nx, cplane = (size_t)cnx * cny;
   const bool has_i1 = 2 * I + 1 < nx;
+  const int jo[3] = {jj[0] * cnx, jj[1] * cnx, jj[2] * cnx};
+  // every load in bounds (indices clamped into the level), the values of
+  // absent neighbours replaced by 0 afterwards: no branch around a load
   auto cload = [&](int K, float (&v)[3][3], float (&m)[3][3]) {
+    const bool kok = K >= 0 && K < cnz;
+    const size_t kp = (size_t)(K < 0 ? 0 : K < cnz ? K : cnz - 1) * cplane;
+    const MgStore *xp = xc + kp, *mp = mc + kp;
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const bool ok = K >= 0 && K < cnz && vi[a] && vj[b];
-        const size_t cc = (size_t)ii[a] + csx * ((size_t)jj[b] + (size_t)cny * K);
-        v[b][a] = ok ? xc[cc] : 0.f;
-        m[b][a] = ok ? mc[cc] : 0.f;
+        const bool ok = kok && vi[a] && vj[b];
+        const float xv = xp[jo[b] + ii[a]], mv = mp[jo[b] + ii[a]];
+        v[b][a] = ok ? xv : 0.f;
+        m[b][a] = ok ? mv : 0.f;
       }
   };
   // The fine rows of plane k (x and code of the two fine rows of the
@@ -452,21 +459,22 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
     Vec<MgStore, 2> x[2];
     uchar2 cd[2];
   };
+  // (rows past the grid are loaded from its last row and plane and never
+  // stored)
   auto fload = [&](int k, Fine &f) {
-    const bool kin = k < nz;
+    const int kc = k < nz ? k : nz - 1;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int j = 2 * J + b;
-      const bool ok = kin && j < ny;
-      const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)(ok ? j : 0) + (size_t)ny * (kin ? k : 0));
+      const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)(j < ny ? j : ny - 1) + (size_t)ny * kc);
       if constexpr (VEC) {
-        f.x[b] = ok ? ldv<MgStore, 2>(x, c0) : Vec<MgStore, 2>{};
-        f.cd[b] = ok ? *reinterpret_cast<const uchar2 *>(code + c0) : make_uchar2(0, 0);
+        f.x[b] = ldv<MgStore, 2>(x, c0);
+        f.cd[b] = *reinterpret_cast<const uchar2 *>(code + c0);
       } else {
-        f.x[b].v[0] = ok ? x[c0] : (MgStore)0.f;
-        f.cd[b].x = ok ? code[c0] : 0;
-        f.x[b].v[1] = ok && has_i1 ? x[c0 + 1] : (MgStore)0.f;
-        f.cd[b].y = ok && has_i1 ? code[c0 + 1] : 0;
+        f.x[b].v[0] = x[c0];
+        f.cd[b].x = code[c0];
+        f.x[b].v[1] = x[has_i1 ? c0 + 1 : c0];
+        f.cd[b].y = has_i1 ? code[c0 + 1] : 0;
       }
     }
   };
@@ -549,10 +557,13 @@ struct Coarse {
   float d, nb;
 };
 
-__device__ __forceinline__ Coarse coarse_row(int nx, int ny, int nz, const float *__restrict__ gE,
-                                             const float *__restrict__ gN,
-                                             const float *__restrict__ gT,
-                                             const float *__restrict__ x, size_t c, int i,
+// x: the level's FP16 iterate, or an FP32 vector on the level (the learned
+// correction's).
+template <typename TX>
+__device__ __forceinline__ Coarse coarse_row(int nx, int ny, int nz, const MgStore *__restrict__ gE,
+                                             const MgStore *__restrict__ gN,
+                                             const MgStore *__restrict__ gT,
+                                             const TX *__restrict__ x, size_t c, int i,
                                              int j, int k) {
   const size_t sx = (size_t)nx, sxy = (size_t)nx * ny;
   const size_t cw = i > 0 ? c - 1 : c, cs = j > 0 ? c - sx : c, cb = k > 0 ? c - sxy : c;
@@ -562,24 +573,22 @@ __device__ __forceinline__ Coarse coarse_row(int nx, int ny, int nz, const float
   const float gw = i > 0 ? gE[cw] : 0.f, gs = j > 0 ? gN[cs] : 0.f, gb = k > 0 ? gT[cb] : 0.f;
   Coarse o;
   o.d = gw + ge + gs + gn + gb + gt;
-  o.nb = gw * x[cw] + ge * x[ce] + gs * x[cs] + gn * x[cn] + gb * x[cb] + gt * x[ct];
+  o.nb = gw * (float)x[cw] + ge * (float)x[ce] + gs * (float)x[cs] + gn * (float)x[cn] +
+         gb * (float)x[cb] + gt * (float)x[ct];
   return o;
 }
-
-
-
 
 // One smoothing step of a coarse level: out = x + c2 (b - A x) / d, plus
 // c1 (x - prev) for a Chebyshev step after the first (prev null: a zero
 // previous iterate); damped Jacobi is c2 = omega, c1 = 0.
 __device__ __forceinline__ void mgc_sweep_body(Blk bk, int nx, int ny, int nz,
-                                               const float *__restrict__ gE,
-                                               const float *__restrict__ gN,
-                                               const float *__restrict__ gT,
-                                               const float *__restrict__ b,
-                                               const float *__restrict__ in,
-                                               const float *__restrict__ prev,
-                                               float *__restrict__ out, float c1, float c2) {
+                                               const MgStore *__restrict__ gE,
+                                               const MgStore *__restrict__ gN,
+                                               const MgStore *__restrict__ gT,
+                                               const MgStore *__restrict__ b,
+                                               const MgStore *__restrict__ in,
+                                               const MgStore *__restrict__ prev,
+                                               MgStore *__restrict__ out, float c1, float c2) {
   const Idx t = cell_idx(nx, ny, bk);
   if (!t.in) return;
   const Coarse o = coarse_row(nx, ny, nz, gE, gN, gT, in, t.c, t.i, t.j, t.k);
@@ -599,16 +608,17 @@ __device__ __forceinline__ void mgc_sweep_body(Blk bk, int nx, int ny, int nz,
 // those of coarse_row and mgc_sweep_body, so both give the same bits.
 template <int CL>
 __device__ __forceinline__ void mgc_sweep_t_body(int block, const Tiling &t, int nx, int ny,
-                                                 int nz, const float *__restrict__ gE,
-                                                 const float *__restrict__ gN,
-                                                 const float *__restrict__ gT,
-                                                 const float *__restrict__ b,
-                                                 const float *__restrict__ in,
-                                                 const float *__restrict__ prev,
-                                                 float *__restrict__ out, float c1, float c2) {
+                                                 int nz, const MgStore *__restrict__ gE,
+                                                 const MgStore *__restrict__ gN,
+                                                 const MgStore *__restrict__ gT,
+                                                 const MgStore *__restrict__ b,
+                                                 const MgStore *__restrict__ in,
+                                                 const MgStore *__restrict__ prev,
+                                                 MgStore *__restrict__ out, float c1, float c2) {
   const Lane l = lane_of<CL>(t, nx, ny, nz, block);
   if (!l.active) return;
   using V = Vec<float, CL>;
+  using H = Vec<MgStore, CL>;  // loads stay FP16 until used (in flight meanwhile)
   const int i = l.i, j = l.j, lane = (int)threadIdx.x;
   const size_t nxy = (size_t)nx * ny;
   const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
@@ -619,56 +629,57 @@ __device__ __forceinline__ void mgc_sweep_t_body(int block, const Tiling &t, int
   const int k0 = l.k0, k1 = l.k1, last = min(k1, nz - 1);
   auto at = [&](int k) { return base + nxy * (size_t)k; };
   struct Set {
-    V x_next, xs, xn, bb, pp, ge, gn, gt, gs;
-    float xw, xe, gw;
+    H x_next, xs, xn, bb, pp, ge, gn, gt, gs;
+    MgStore xw, xe, gw;
   };
   auto load = [&](Set &s, int k, int knext) {
     const size_t c = at(k);
-    s.x_next = ldv<float, CL>(in, at(knext));
-    s.xs = ldv<float, CL>(in, c + os);
-    s.xn = ldv<float, CL>(in, c + on);
+    s.x_next = ldv<MgStore, CL>(in, at(knext));
+    s.xs = ldv<MgStore, CL>(in, c + os);
+    s.xn = ldv<MgStore, CL>(in, c + on);
     s.xw = in[c + ow];
     s.xe = in[c + oe];
-    s.bb = ldv<float, CL>(b, c);
-    if (prev) s.pp = ldv<float, CL>(prev, c);
-    s.ge = ldv<float, CL>(gE, c);
-    s.gn = ldv<float, CL>(gN, c);
-    s.gt = ldv<float, CL>(gT, c);
-    s.gs = ldv<float, CL>(gN, c + os);
+    s.bb = ldv<MgStore, CL>(b, c);
+    if (prev) s.pp = ldv<MgStore, CL>(prev, c);
+    s.ge = ldv<MgStore, CL>(gE, c);
+    s.gn = ldv<MgStore, CL>(gN, c);
+    s.gt = ldv<MgStore, CL>(gT, c);
+    s.gs = ldv<MgStore, CL>(gN, c + os);
     s.gw = gE[c + ow];
   };
   Set sa, sb;
-  V x_cur = ldv<float, CL>(in, at(k0));
-  V x_prev = k0 > 0 ? ldv<float, CL>(in, at(k0 - 1)) : x_cur;
-  V gt_prev = k0 > 0 ? ldv<float, CL>(gT, at(k0 - 1)) : x_cur;
+  V x_cur = convert<float>(ldv<MgStore, CL>(in, at(k0)));
+  V x_prev = k0 > 0 ? convert<float>(ldv<MgStore, CL>(in, at(k0 - 1))) : x_cur;
+  V gt_prev = k0 > 0 ? convert<float>(ldv<MgStore, CL>(gT, at(k0 - 1))) : x_cur;
   load(sa, k0, min(k0 + 1, last));
   auto stage = [&](int k, const Set &u, Set &ld) {
     load(ld, min(k + 1, k1 - 1), min(k + 2, last));
     const float from_w = __shfl_up(x_cur.v[CL - 1], 1, kTileX);
     const float from_e = __shfl_down(x_cur.v[0], 1, kTileX);
-    const float gfrom_w = __shfl_up(u.ge.v[CL - 1], 1, kTileX);
-    V o;
+    const float gfrom_w = __shfl_up((float)u.ge.v[CL - 1], 1, kTileX);
+    H o;
 #pragma unroll
     for (int m = 0; m < CL; ++m) {
       const float xv = x_cur.v[m];
-      const float xw = m > 0 ? x_cur.v[m - 1] : hw ? (lane == 0 ? u.xw : from_w) : xv;
-      const float xe = m < CL - 1 ? x_cur.v[m + 1] : he ? (lane == kTileX - 1 ? u.xe : from_e)
-                                                        : xv;
-      const float xs = hs ? u.xs.v[m] : xv, xn = hn ? u.xn.v[m] : xv;
-      const float xb = k > 0 ? x_prev.v[m] : xv, xt = k < nz - 1 ? u.x_next.v[m] : xv;
+      const float xw = m > 0 ? x_cur.v[m - 1] : hw ? (lane == 0 ? (float)u.xw : from_w) : xv;
+      const float xe = m < CL - 1 ? x_cur.v[m + 1]
+                                  : he ? (lane == kTileX - 1 ? (float)u.xe : from_e) : xv;
+      const float xs = hs ? (float)u.xs.v[m] : xv, xn = hn ? (float)u.xn.v[m] : xv;
+      const float xb = k > 0 ? x_prev.v[m] : xv, xt = k < nz - 1 ? (float)u.x_next.v[m] : xv;
       const float ge = u.ge.v[m], gn = u.gn.v[m], gt = u.gt.v[m];
-      const float gw = m > 0 ? u.ge.v[m - 1] : hw ? (lane == 0 ? u.gw : gfrom_w) : 0.f;
-      const float gs = hs ? u.gs.v[m] : 0.f, gb = k > 0 ? gt_prev.v[m] : 0.f;
+      const float gw = m > 0 ? (float)u.ge.v[m - 1] : hw ? (lane == 0 ? (float)u.gw : gfrom_w)
+                                                         : 0.f;
+      const float gs = hs ? (float)u.gs.v[m] : 0.f, gb = k > 0 ? gt_prev.v[m] : 0.f;
       const float d = gw + ge + gs + gn + gb + gt;
       const float nb = gw * xw + ge * xe + gs * xs + gn * xn + gb * xb + gt * xt;
-      float v = d > 0.f ? xv + c2 * (u.bb.v[m] + nb - d * xv) / d : 0.f;
-      if (c1 != 0.f && d > 0.f) v += c1 * (xv - (prev ? u.pp.v[m] : 0.f));
-      o.v[m] = v;
+      float v = d > 0.f ? xv + c2 * ((float)u.bb.v[m] + nb - d * xv) / d : 0.f;
+      if (c1 != 0.f && d > 0.f) v += c1 * (xv - (prev ? (float)u.pp.v[m] : 0.f));
+      o.v[m] = (MgStore)v;
     }
-    stv<float, CL>(out, at(k), o);
+    stv<MgStore, CL>(out, at(k), o);
     x_prev = x_cur;
-    x_cur = u.x_next;
-    gt_prev = u.gt;
+    x_cur = convert<float>(u.x_next);
+    gt_prev = convert<float>(u.gt);
   };
   for (int k = k0; k < k1; k += 2) {
     stage(k, sa, sb);
@@ -677,11 +688,11 @@ __device__ __forceinline__ void mgc_sweep_t_body(int block, const Tiling &t, int
 }
 
 __device__ __forceinline__ void mgc_restrict_body(Blk bk, int nx, int ny, int nz,
-                                                  const float *__restrict__ gE,
-                                                  const float *__restrict__ gN,
-                                                  const float *__restrict__ gT,
-                                                  const float *__restrict__ b,
-                                                  const float *__restrict__ x, int cnx, int cny,
+                                                  const MgStore *__restrict__ gE,
+                                                  const MgStore *__restrict__ gN,
+                                                  const MgStore *__restrict__ gT,
+                                                  const MgStore *__restrict__ b,
+                                                  const MgStore *__restrict__ x, int cnx, int cny,
                                                   CoarseOut out) {
   const Idx t = cell_idx(cnx, cny, bk);
   if (!t.in) return;
@@ -709,8 +720,8 @@ __device__ __forceinline__ void mgc_restrict_body(Blk bk, int nx, int ny, int nz
 // 1/4 of its neighbour on the side of a, per axis; weights are renormalised
 // over coarse cells with mc = 1. Only fine cells that carry unknowns change.
 __device__ __forceinline__ float prolong_value(int i, int j, int k, int cnx, int cny, int cnz,
-                                               const float *__restrict__ xc,
-                                               const float *__restrict__ mc) {
+                                               const MgStore *__restrict__ xc,
+                                               const MgStore *__restrict__ mc) {
   const int I = i >> 1, J = j >> 1, K = k >> 1;
   const int I2 = (i & 1) ? I + 1 : I - 1, J2 = (j & 1) ? J + 1 : J - 1,
             K2 = (k & 1) ? K + 1 : K - 1;
@@ -734,10 +745,10 @@ __device__ __forceinline__ float prolong_value(int i, int j, int k, int cnx, int
 }
 
 __device__ __forceinline__ void mgc_prolong_body(Blk bk, int nx, int ny,
-                                                 const float *__restrict__ m, float *__restrict__ x,
+                                                 const MgStore *__restrict__ m, MgStore *__restrict__ x,
                                                  int cnx, int cny, int cnz,
-                                                 const float *__restrict__ xc,
-                                                 const float *__restrict__ mc) {
+                                                 const MgStore *__restrict__ xc,
+                                                 const MgStore *__restrict__ mc) {
   const Idx t = cell_idx(nx, ny, bk);
   if (!t.in) return;
   if (!(m[t.c] > 0.f)) return;
@@ -748,12 +759,12 @@ __device__ __forceinline__ void mgc_prolong_body(Blk bk, int nx, int ny,
 // sweeps from zero with the iterate in LDS; the first one is rd b, as the
 // restriction into a level forms it.
 __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int nz,
-                                                  const float *__restrict__ gE,
-                                                  const float *__restrict__ gN,
-                                                  const float *__restrict__ gT,
-                                                  const float *__restrict__ b,
-                                                  const float *__restrict__ rd,
-                                                  float *__restrict__ x, float omega, int sweeps) {
+                                                  const MgStore *__restrict__ gE,
+                                                  const MgStore *__restrict__ gN,
+                                                  const MgStore *__restrict__ gT,
+                                                  const MgStore *__restrict__ b,
+                                                  const MgStore *__restrict__ rd,
+                                                  MgStore *__restrict__ x, float omega, int sweeps) {
   const int t = threadIdx.x;
   const int n = nx * ny * nz;
   const bool on = t < n;
@@ -969,9 +980,9 @@ __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds,
 // V c over the fluid-fluid level-0 faces on each aggregate face.
 __device__ __forceinline__ void mg_coarsen0_body(Blk bk, const GeomT<double> &g,
                                                  const uint8_t *__restrict__ code, int cnx,
-                                                 int cny, float scale, float *__restrict__ gE,
-                                                 float *__restrict__ gN,
-                                                 float *__restrict__ gT) {
+                                                 int cny, float scale, MgStore *__restrict__ gE,
+                                                 MgStore *__restrict__ gN,
+                                                 MgStore *__restrict__ gT) {
   const Idx t = cell_idx(cnx, cny, bk);
   if (!t.in) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -1005,12 +1016,12 @@ __device__ __forceinline__ void mg_coarsen0_body(Blk bk, const GeomT<double> &g,
 
 // Level l + 1 conductances from level l: the sum over the aggregate face.
 __device__ __forceinline__ void mg_coarsen_body(Blk bk, int nx, int ny, int nz,
-                                                const float *__restrict__ fE,
-                                                const float *__restrict__ fN,
-                                                const float *__restrict__ fT, int cnx, int cny,
-                                                float scale, float *__restrict__ gE,
-                                                float *__restrict__ gN,
-                                                float *__restrict__ gT) {
+                                                const MgStore *__restrict__ fE,
+                                                const MgStore *__restrict__ fN,
+                                                const MgStore *__restrict__ fT, int cnx, int cny,
+                                                float scale, MgStore *__restrict__ gE,
+                                                MgStore *__restrict__ gN,
+                                                MgStore *__restrict__ gT) {
   const Idx t = cell_idx(cnx, cny, bk);
   if (!t.in) return;
   auto at = [&](int i, int j, int k) {
@@ -1040,10 +1051,10 @@ __device__ __forceinline__ void mg_coarsen_body(Blk bk, int nx, int ny, int nz,
 
 // Unknown mask (diag > 0) and the first-step scale over diag of a level.
 __device__ __forceinline__ void mg_mask_body(Blk bk, int nx, int ny,
-                                             const float *__restrict__ gE,
-                                             const float *__restrict__ gN,
-                                             const float *__restrict__ gT, float first,
-                                             float *__restrict__ m, float *__restrict__ rd) {
+                                             const MgStore *__restrict__ gE,
+                                             const MgStore *__restrict__ gN,
+                                             const MgStore *__restrict__ gT, float first,
+                                             MgStore *__restrict__ m, MgStore *__restrict__ rd) {
   const Idx t = cell_idx(nx, ny, bk);
   if (!t.in) return;
   const float d = coarse_diag(nx, ny, gE, gN, gT, t.c, t.i, t.j, t.k);
