@@ -553,13 +553,13 @@ __global__ __launch_bounds__(512) void k_bg_dir(const BSolve *solves, Grid grid,
   // on the FP32 tiling, as the standalone solve
   const Tiling &t = d.mg.tld;
   if (st.mode == kDirUpdate)
-    pcg_dir_body<double, CL, kDirUpdate, MgStore, float>(t, lb, t.blocks, d.g, d.mg.buf0(3),
-                                                         d.code, pin, pout, d.partials, d.sc,
-                                                         lds);
+    pcg_dir_body<float, CL, kDirUpdate, MgStore, float>(t, lb, t.blocks, d.mg.g32,
+                                                        d.mg.buf0(3), d.code, pin, pout,
+                                                        d.partials, d.sc, lds);
   else
-    pcg_dir_body<double, CL, kDirReset, MgStore, float>(t, lb, t.blocks, d.g, d.mg.buf0(3),
-                                                        d.code, pin, pout, d.partials, d.sc,
-                                                        lds);
+    pcg_dir_body<float, CL, kDirReset, MgStore, float>(t, lb, t.blocks, d.mg.g32,
+                                                       d.mg.buf0(3), d.code, pin, pout,
+                                                       d.partials, d.sc, lds);
 }
 
 // x += alpha p, r -= alpha A p and the next cycle's first step; the slot that

@@ -135,17 +135,19 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
 }
 
 // The flexible CG's direction step on the cycle's output (the PCG direction
-// body with the FP16 z widened and unscaled on load and p stored in FP32).
+// body with the FP16 z widened and unscaled on load, p formed and stored in
+// FP32, q = A p in FP32 and p.q summed in FP64: the step length alpha only
+// needs p.q to FP32 accuracy, while the update recomputes A p in FP64 for r).
 template <int CL, int MODE>
-__global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<double> g,
+__global__ __launch_bounds__(1024) void k_mg_dir_t(Tiling t, GeomT<float> g,
                                                    const MgStore *__restrict__ z,
                                                    const uint8_t *__restrict__ code,
                                                    const float *__restrict__ pin,
                                                    float *__restrict__ pout, double *partials,
                                                    Scalars *sc) {
   __shared__ double lds[16 * kReduceSlots];
-  pcg_dir_body<double, CL, MODE, MgStore, float>(t, blockIdx.x, gridDim.x, g, z, code, pin,
-                                                 pout, partials, sc, lds);
+  pcg_dir_body<float, CL, MODE, MgStore, float>(t, blockIdx.x, gridDim.x, g, z, code, pin,
+                                                pout, partials, sc, lds);
 }
 
 // ---- the learned coarse correction -----------------------------------------------
@@ -821,7 +823,7 @@ int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin,
   const dim3 blk(kTileX, t.ty);
   TimedLaunch tl("mg_direction");
   auto go = [&](auto kern) {
-    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, mg.buf0(3), mg.code, pin, pout, mg.partials,
+    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g32, mg.buf0(3), mg.code, pin, pout, mg.partials,
                                          sc);
   };
   if (cl == 4) {
