@@ -109,28 +109,45 @@ struct CoarseOut {
 };
 
 // ---- level 0: operator from the neighbour code and FP32 1-D coefficients ----
-// The level-0 diagonal of a lane's CL cells in one plane: the wave's cells
-// almost all have six fluid neighbours (code kCodeAll), whose diagonal is
-// dxy + (cb + ct) with dxy = ((cw + ce) + cs) + cn per lane (one add); a wave
-// with any other cell sums the coefficients of its codes.
-constexpr unsigned kCodeAll = kCodeW | kCodeE | kCodeS | kCodeN | kCodeB | kCodeT | kCodeFluid;
-
+// The level-0 diagonal of a lane's CL cells in one plane. Almost every cell
+// is fluid with all its neighbours inside the domain fluid: its code is the
+// one the domain walls alone leave (exy[m] | ez: a lane constant and a plane
+// constant), and its diagonal is dxy[m] + cz, the lane's in-plane
+// coefficients towards the neighbours inside the domain plus the plane's
+// (one add). A wave with any other cell (next to a body) sums the
+// coefficients of its codes.
 template <int CL>
 __device__ __forceinline__ void level0_diag(float (&d)[CL], const Codes<CL> &code,
+                                            const unsigned (&exy)[CL], unsigned ez,
                                             const float (&cw)[CL], const float (&ce)[CL],
                                             const float (&dxy)[CL], float cs, float cn, float cb,
-                                            float ct) {
+                                            float ct, float cz) {
   bool full = true;
 #pragma unroll
-  for (int m = 0; m < CL; ++m) full = full && code.b[m] == kCodeAll;
+  for (int m = 0; m < CL; ++m) full = full && code.b[m] == (exy[m] | ez);
   if (__all(full)) {
-    const float czk = cb + ct;
 #pragma unroll
-    for (int m = 0; m < CL; ++m) d[m] = dxy[m] + czk;
+    for (int m = 0; m < CL; ++m) d[m] = dxy[m] + cz;
   } else {
 #pragma unroll
     for (int m = 0; m < CL; ++m) d[m] = diag_code<float>(code.b[m], cw[m], ce[m], cs, cn, cb, ct);
   }
+}
+
+// The code of a fluid cell whose neighbours inside the domain are all fluid:
+// the in-plane part for the lane's cells (walls at i = 0, i = nx - 1 on the
+// lane's edge cells, j = 0, j = ny - 1) and the plane's part.
+template <int CL>
+__device__ __forceinline__ void wall_codes(unsigned (&exy)[CL], bool hw, bool he, bool hs,
+                                           bool hn) {
+#pragma unroll
+  for (int m = 0; m < CL; ++m)
+    exy[m] = kCodeFluid | (m > 0 || hw ? kCodeW : 0u) | (m < CL - 1 || he ? kCodeE : 0u) |
+             (hs ? kCodeS : 0u) | (hn ? kCodeN : 0u);
+}
+
+__device__ __forceinline__ unsigned wall_code_z(int k, int nz) {
+  return (k > 0 ? kCodeB : 0u) | (k < nz - 1 ? kCodeT : 0u);
 }
 
 // The first sweep from zero, x = sig omega D^-1 r = sig omega s / d (D = V
@@ -311,17 +328,19 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
   float cw[CL], ce[CL], dxv[CL], dxy[CL], cwn[CL], cen[CL];
   const float cs = g.cs[jc], cn = g.cn[jc], dyj = g.dy[jc];
   const bool hw = ic > 0, he = ic + CL < nx, hs = jc > 0, hn = jc < ny - 1;
+  const float csn = hs ? cs : 0.f, cnn = hn ? cn : 0.f;
+  unsigned exy[CL];
+  wall_codes<CL>(exy, hw, he, hs, hn);
 #pragma unroll
   for (int m = 0; m < CL; ++m) {
     cw[m] = g.cw[ic + m];
     ce[m] = g.ce[ic + m];
     dxv[m] = g.dx[ic + m];
-    dxy[m] = ((cw[m] + ce[m]) + cs) + cn;
     // neighbour terms past the domain walls drop out with their coefficient
     cwn[m] = m == 0 && !hw ? 0.f : cw[m];
     cen[m] = m == CL - 1 && !he ? 0.f : ce[m];
+    dxy[m] = ((cwn[m] + cen[m]) + csn) + cnn;
   }
-  const float csn = hs ? cs : 0.f, cnn = hn ? cn : 0.f;
   const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
   // only the edge lanes' west / east loads leave their own cells (a clamped
   // lane's row start could lie before the array)
@@ -354,7 +373,8 @@ __device__ __forceinline__ void mg_restrict_body(int block, float *lds, const Ti
     const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
     const float cbk = k > 0 ? cb : 0.f, ctk = k < nz - 1 ? ct : 0.f;
     float d[CL], con[CL];
-    level0_diag<CL>(d, u.pl.code, cw, ce, dxy, cs, cn, cb, ct);
+    level0_diag<CL>(d, u.pl.code, exy, wall_code_z(k, nz), cw, ce, dxy, cs, cn, cb, ct,
+                    cbk + ctk);
 #pragma unroll
     for (int m = 0; m < CL; ++m) {
       // the iterate is 0 outside the unknowns, so every neighbour term enters
@@ -837,24 +857,23 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
     const size_t nxy = (size_t)nx * ny;
     float cw[CL], ce[CL], dxdy[CL], dxy[CL];
     const float cs = g.cs[j], cn = g.cn[j];
-#pragma unroll
-    for (int m = 0; m < CL; ++m) {
-      cw[m] = g.cw[i + m];
-      ce[m] = g.ce[i + m];
-      dxdy[m] = g.dx[i + m] * g.dy[j];
-      dxy[m] = ((cw[m] + ce[m]) + cs) + cn;
-    }
     const float keep = 1.f - omega;
     const bool hw = i > 0, he = i + CL < nx, hs = j > 0, hn = j < ny - 1;
     // neighbour terms past the domain walls drop out with their coefficient
     // (the loads there are clamped to cells of the grid)
     float cwn[CL], cen[CL];
+    const float csn = hs ? cs : 0.f, cnn = hn ? cn : 0.f;
+    unsigned exy[CL];
+    wall_codes<CL>(exy, hw, he, hs, hn);
 #pragma unroll
     for (int m = 0; m < CL; ++m) {
+      cw[m] = g.cw[i + m];
+      ce[m] = g.ce[i + m];
+      dxdy[m] = g.dx[i + m] * g.dy[j];
       cwn[m] = m == 0 && !hw ? 0.f : cw[m];
       cen[m] = m == CL - 1 && !he ? 0.f : ce[m];
+      dxy[m] = ((cwn[m] + cen[m]) + csn) + cnn;
     }
-    const float csn = hs ? cs : 0.f, cnn = hn ? cn : 0.f;
     const ptrdiff_t os = hs ? -(ptrdiff_t)nx : 0, on = hn ? (ptrdiff_t)nx : 0;
     const int lane = (int)threadIdx.x;
     const ptrdiff_t row0 = -(ptrdiff_t)(lane * CL);
@@ -897,7 +916,8 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
         if (q) qz = ldv_s<NT, float, CL>(q, at(k));
       }
       float d[CL];
-      level0_diag<CL>(d, u.pl.code, cw, ce, dxy, cs, cn, cb, ct);
+      level0_diag<CL>(d, u.pl.code, exy, wall_code_z(k, nz), cw, ce, dxy, cs, cn, cb, ct,
+                      cbk + ctk);
 #pragma unroll
       for (int m = 0; m < CL; ++m) {
         const float west = m == 0 ? vw : a_cur.v[m - 1];
