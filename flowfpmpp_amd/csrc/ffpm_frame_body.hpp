@@ -72,9 +72,10 @@ __device__ __forceinline__ void axis_coefficients_body(int t, int nx, int ny, in
 
 // build_operator (src/fpm_stairstep_mpi_3d.f90:336-379): folds the outer and
 // body face gradients into src, sums fluid-neighbour coefficients into diag,
-// and sets rdiag = 1/(V diag) (PCG) or 0.8/diag (Jacobi); 0 in solids. With
-// write_operator false only src is stored (another right-hand side of an
-// operator that is already built).
+// and sets rdiag = 1/(V diag) (PCG) or 0.8/diag (Jacobi); 0 in solids
+// (rdiag null: not formed, for solvers that rebuild what they need from the
+// neighbour code). With write_operator false only src is stored (another
+// right-hand side of an operator that is already built).
 __device__ __forceinline__ void build_operator_body(
     int block, int nblocks, const Geom &g, const int32_t *__restrict__ solid,
     const double *__restrict__ rhs, const double *__restrict__ xlo,
@@ -130,11 +131,13 @@ __device__ __forceinline__ void build_operator_body(
     if (write_operator) {
       diag[c] = d;
       code[c] = (uint8_t)cb;
-      double rd;
-      if (d <= 0) rd = 0;
-      else if (for_pcg) rd = 1 / (dxi * dyj * dzk * d);
-      else rd = 0.8 / d;
-      rdiag[c] = rd;
+      if (rdiag) {
+        double rd;
+        if (d <= 0) rd = 0;
+        else if (for_pcg) rd = 1 / (dxi * dyj * dzk * d);
+        else rd = 0.8 / d;
+        rdiag[c] = rd;
+      }
     }
   }
   if (local_isolated) atomicAdd(isolated, local_isolated);

@@ -1002,9 +1002,12 @@ int solve_stairstep_dev(double *phi_ref, const double *rhs, const int32_t *solid
   *residual = DBL_MAX;
   const int eb = elementwise_blocks(f.n);
   FFPM_HIP(hipMemsetAsync(f.isolated, 0, sizeof(int), stream()));
+  // the multigrid CG reads no rdiag (unless its learned variant may hand the
+  // solve to diagonal PCG)
+  double *rdiag = method == FFPM_METHOD_FCG_MG && !lrn ? nullptr : f.rdiag;
   k_build_operator<<<eb, kThreads, 0, stream()>>>(f.g, solid, rhs, xlo, xhi, ylo, yhi, zlo, zhi,
                                                   bg[0], bg[1], bg[2], pcg ? 1 : 0, f.src, f.diag,
-                                                  f.rdiag, f.code, f.isolated);
+                                                  rdiag, f.code, f.isolated);
   FFPM_LAUNCHED("k_build_operator");
   int isolated = 0;
   FFPM_HIP(hipMemcpyAsync(&isolated, f.isolated, sizeof(int), hipMemcpyDeviceToHost, stream()));
