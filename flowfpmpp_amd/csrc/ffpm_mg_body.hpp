@@ -433,13 +433,17 @@ __host__ __device__ inline size_t mg_restrict_lds(int ty, int cl) {
 }
 
 // Prolongation onto level 0 with one thread per level-1 (I, J) column walking
-// a range of K: fine plane 2K + c takes coarse planes K and K - 1 (c = 0) or
-// K + 1 (c = 1), so two 3x3 coarse patches (values and unknown masks) are
-// held at a time, the next plane's patch loaded over the one just retired;
-// each
-// fine cell takes the trilinear weights of prolong_value (only fine cells that
-// carry unknowns change; the sum is formed in FP32 and stored in FP16). VEC
-// (even nx): each fine row pair moves as one 4-byte x and 2-byte code access.
+// a range of K: fine plane 2K + c takes 3/4 of coarse plane K and 1/4 of
+// plane K - 1 (c = 0) or K + 1 (c = 1). Each fine cell takes the trilinear
+// weights of prolong_value (only fine cells that carry unknowns change; the
+// sum is formed in FP32 and stored in FP16). The in-plane part of the
+// interpolation (for the four fine columns of the coarse column: the
+// weighted values and unknown masks of the 3x3 coarse patch) depends on the
+// coarse plane alone, and each coarse plane serves four fine planes (two as
+// the 3/4 plane, two as the 1/4 plane): it is formed once per coarse plane,
+// when the patch is first used, so the patch loads of the next plane are in
+// flight while a fine plane is computed. VEC (even nx): each fine row pair
+// moves as one 4-byte x and 2-byte code access.
 template <bool VEC>
 __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int nz,
                                                    const uint8_t *__restrict__ code,
@@ -456,31 +460,61 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
   const size_t fsx = (size_t)nx, cplane = (size_t)cnx * cny;
   const bool has_i1 = 2 * I + 1 < nx;
   const int jo[3] = {jj[0] * cnx, jj[1] * cnx, jj[2] * cnx};
-  // every load in bounds (indices clamped into the level), the values of
-  // absent neighbours replaced by 0 afterwards: no branch around a load
-  auto cload = [&](int K, float (&v)[3][3], float (&m)[3][3]) {
-    const bool kok = K >= 0 && K < cnz;
+  // a coarse plane's raw 3x3 patch; every load in bounds (indices clamped
+  // into the level), the values of absent neighbours replaced by 0 when the
+  // patch is used: no branch around a load
+  struct Patch {
+    MgStore v[3][3], m[3][3];
+    bool kok;
+  };
+  auto cload = [&](int K, Patch &pt) {
+    pt.kok = K >= 0 && K < cnz;
     const size_t kp = (size_t)(K < 0 ? 0 : K < cnz ? K : cnz - 1) * cplane;
-    const MgStore *xp = xc + kp, *mp = mc + kp;
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const bool ok = kok && vi[a] && vj[b];
-        const float xv = xp[jo[b] + ii[a]], mv = mp[jo[b] + ii[a]];
-        v[b][a] = ok ? xv : 0.f;
-        m[b][a] = ok ? mv : 0.f;
+        pt.v[b][a] = xc[kp + jo[b] + ii[a]];
+        pt.m[b][a] = mc[kp + jo[b] + ii[a]];
       }
   };
+  // the in-plane interpolation of a patch for fine column (a, b): the sums
+  // w00 x[1][1] + w10 x[1][ai] + w01 x[bj][1] + w11 x[bj][ai] of values and
+  // of masks, ai = 0 or 2, bj = 0 or 2 by the side of a and b
+  struct Interp {
+    float num[2][2], den[2][2];
+  };
+  auto interp = [&](const Patch &pt, Interp &ip) {
+    float v[3][3], m[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const bool ok = pt.kok && vi[a] && vj[b];
+        v[b][a] = ok ? (float)pt.v[b][a] : 0.f;
+        m[b][a] = ok ? (float)pt.m[b][a] : 0.f;
+      }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int bj = b ? 2 : 0;
+      const float wj = vj[bj] ? 0.25f : 0.f;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int ai = a ? 2 : 0;
+        const float wi = vi[ai] ? 0.25f : 0.f;
+        const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
+        ip.num[b][a] = w00 * v[1][1] + w10 * v[1][ai] + w01 * v[bj][1] + w11 * v[bj][ai];
+        ip.den[b][a] = w00 * m[1][1] + w10 * m[1][ai] + w01 * m[bj][1] + w11 * m[bj][ai];
+      }
+    }
+  };
   // The fine rows of plane k (x and code of the two fine rows of the
-  // column), loaded one fine plane ahead of their use so the loads overlap
-  // the computation of the plane before.
+  // column), loaded one fine plane ahead of their use (rows past the grid
+  // are loaded from its last row and plane and never stored).
   struct Fine {
     Vec<MgStore, 2> x[2];
     uchar2 cd[2];
   };
-  // (rows past the grid are loaded from its last row and plane and never
-  // stored)
   auto fload = [&](int k, Fine &f) {
     const int kc = k < nz ? k : nz - 1;
 #pragma unroll
@@ -498,77 +532,54 @@ __device__ __forceinline__ void mgf_prolong_z_body(Blk bk, int nx, int ny, int n
       }
     }
   };
-  // fine plane k = 2K + c (its rows in f) from the patches of plane K (x0)
-  // and of its z neighbour on the side of c (xz); wk: that neighbour's weight
-  auto fine_plane = [&](int k, float wk, const float (&x0)[3][3], const float (&m0)[3][3],
-                        const float (&xz)[3][3], const float (&mz)[3][3], const Fine &f) {
+  // fine plane k: 3/4 of ic (its own coarse plane) and wk of iz (the z
+  // neighbour's, 0 past the level)
+  auto fine_plane = [&](int k, float wk, const Interp &ic, const Interp &iz, const Fine &f) {
     if (k >= nz) return;
-    float xv[2][2];
-    unsigned cd[2][2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      xv[b][0] = (float)f.x[b].v[0];
-      xv[b][1] = (float)f.x[b].v[1];
-      cd[b][0] = f.cd[b].x;
-      cd[b][1] = f.cd[b].y;
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int bj = b ? 2 : 0;
-      const float wj = vj[bj] ? 0.25f : 0.f;
+      float xv[2] = {(float)f.x[b].v[0], (float)f.x[b].v[1]};
+      const unsigned cd[2] = {f.cd[b].x, f.cd[b].y};
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        if (!(cd[b][a] & kCodeFluid)) continue;
-        const int ai = a ? 2 : 0;
-        const float wi = vi[ai] ? 0.25f : 0.f;
-        const float w00 = 0.75f * 0.75f, w10 = wi * 0.75f, w01 = 0.75f * wj, w11 = wi * wj;
-        const float num0 = w00 * x0[1][1] + w10 * x0[1][ai] + w01 * x0[bj][1] + w11 * x0[bj][ai];
-        const float den0 = w00 * m0[1][1] + w10 * m0[1][ai] + w01 * m0[bj][1] + w11 * m0[bj][ai];
-        const float num1 = w00 * xz[1][1] + w10 * xz[1][ai] + w01 * xz[bj][1] + w11 * xz[bj][ai];
-        const float den1 = w00 * mz[1][1] + w10 * mz[1][ai] + w01 * mz[bj][1] + w11 * mz[bj][ai];
-        const float num = 0.75f * num0 + wk * num1, den = 0.75f * den0 + wk * den1;
-        xv[b][a] += den > 0.f ? num * __builtin_amdgcn_rcpf(den) : 0.f;
+        const float num = 0.75f * ic.num[b][a] + wk * iz.num[b][a];
+        const float den = 0.75f * ic.den[b][a] + wk * iz.den[b][a];
+        const float add = den > 0.f ? num * __builtin_amdgcn_rcpf(den) : 0.f;
+        xv[a] = (cd[a] & kCodeFluid) ? xv[a] + add : xv[a];
       }
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
       const int j = 2 * J + b;
       if (j >= ny) continue;
       const size_t c0 = (size_t)(2 * I) + fsx * ((size_t)j + (size_t)ny * k);
       if constexpr (VEC) {
         Vec<MgStore, 2> t;
-        t.v[0] = (MgStore)xv[b][0];
-        t.v[1] = (MgStore)xv[b][1];
+        t.v[0] = (MgStore)xv[0];
+        t.v[1] = (MgStore)xv[1];
         stv<MgStore, 2>(x, c0, t);
       } else {
-        x[c0] = (MgStore)xv[b][0];
-        if (has_i1) x[c0 + 1] = (MgStore)xv[b][1];
+        x[c0] = (MgStore)xv[0];
+        if (has_i1) x[c0 + 1] = (MgStore)xv[1];
       }
     }
   };
-  // pa: the patch of plane K, pb: of plane K - 1 (before fine plane 2K) or
-  // K + 1 (before fine plane 2K + 1); the two swap roles each step. fa holds
-  // the rows of the even fine plane, fb of the odd one.
-  float xa[3][3], ma[3][3], xb[3][3], mb[3][3];
+  // ibelow, icur: the interpolations of planes K - 1 and K; the patch of
+  // plane K + 1 loads while fine plane 2K is computed
+  Patch pt;
+  Interp ibelow, icur, iabove;
+  cload(K0 - 1, pt);
+  interp(pt, ibelow);
+  cload(K0, pt);
+  interp(pt, icur);
   Fine fa, fb;
-  cload(K0 - 1, xb, mb);
-  cload(K0, xa, ma);
   fload(2 * K0, fa);
-  int K = K0;
-  while (K < K1) {
+  for (int K = K0; K < K1; ++K) {
+    cload(K + 1, pt);
     fload(2 * K + 1, fb);
-    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xa, ma, xb, mb, fa);
-    cload(K + 1, xb, mb);
+    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, icur, ibelow, fa);
+    interp(pt, iabove);
     if (K + 1 < K1) fload(2 * K + 2, fa);
-    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xa, ma, xb, mb, fb);
-    if (++K >= K1) break;
-    // plane K (old K + 1) is in xb: the roles swap
-    fload(2 * K + 1, fb);
-    fine_plane(2 * K, K > 0 ? 0.25f : 0.f, xb, mb, xa, ma, fa);
-    cload(K + 1, xa, ma);
-    if (K + 1 < K1) fload(2 * K + 2, fa);
-    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, xb, mb, xa, ma, fb);
-    ++K;
+    fine_plane(2 * K + 1, K + 1 < cnz ? 0.25f : 0.f, icur, iabove, fb);
+    ibelow = icur;
+    icur = iabove;
   }
 }
 
