@@ -1489,7 +1489,7 @@ int batch_run(Batch &B, double tolerance, int max_sweeps, int check_interval, in
   const int count = B.count;
   int st;
   // All solves must share the PCG workgroup height (one launch per grid);
-  // the multigrid cycle's FP32 sweeps have their own (sblk), the mixed inner
+  // the multigrid cycle's sweeps have their own (sblk), the mixed inner
   // PCG's FP32 kernels launch with the FP64 one.
   const int ty = B.h[0].tl.ty, ty32 = B.mgm ? B.h[0].tl32.ty : ty;
   for (int s = 0; s < count; ++s)

@@ -179,9 +179,9 @@ int mg_learned_setup(Multigrid &mg, UNet *net, const float *features, int level,
 int mg_cells_per_lane(int nx);
 Tiling mg_cycle_tiling(int nx, int ny, int nz, int rows, int cl);
 
-// Rows per workgroup of the level-0 FP32 kernels (the cycle's sweeps):
-// FFPM_MG_ROWS (1, 2, 4, 8, 16), default 4, measured faster than the 8 rows
-// of the FP64 kernels at 512x384x256 (141 against 145 us per sweep).
+// Rows per workgroup of the cycle's level-0 kernels and the direction step:
+// FFPM_MG_ROWS (1, 2, 4, 8, 16), default 4, measured faster than 2 or 8 rows
+// at 512x384x256 (FP16 sweeps 98 us against 99 and 110).
 int mg_rows();
 
 // Convergence-check granularity of the multigrid CG: FFPM_MG_CHECK
@@ -199,7 +199,7 @@ int mg_first(const Multigrid &mg, double *r, const Scalars *sc);
 // x += alpha p, r -= alpha A p, f0[0] = sig w D^-1 r (sig = sc->sig_next,
 // which becomes sc->sig) and q32 = A p (alpha in sc; no-op after a
 // breakdown); counts the iteration. t, cl: FP64 launch
-// shape. The direction p is stored in FP32 (mg_direction rounds it) and
+// shape. The direction p is stored in FP32 (mg_direction forms it) and
 // widened exactly; x, r and the arithmetic stay FP64. With rmax non-null, one
 // partial max |r / V| per workgroup (t.blocks) goes there. xmode XM_DEFER
 // leaves x alone and keeps alpha in sc->alpha_pend; the next update, XM_TWO
@@ -217,9 +217,9 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
 int mg_vcycle(const Multigrid &mg, bool with_q);
 
 // The flexible CG's direction step with the cycle's output: p = z (MODE
-// kDirReset) or z + beta p_old (kDirUpdate), z = f0[3] / sig, formed in FP64 and stored
-// rounded to FP32, q = A p of the rounded p and the p.q partials (t.blocks,
-// launch shape t with cl cells per lane: the cycle's FP32 shape tl, cl).
+// kDirReset) or z + beta p_old (kDirUpdate), z = f0[3] / sig, formed and
+// stored in FP32, q = A p in FP32 and the p.q partials summed in FP64
+// (t.blocks; launch shape t with cl cells per lane: the FP32 shape tld, cld).
 int mg_direction(const Multigrid &mg, const Tiling &t, int cl, const float *pin, float *pout,
                  int mode, Scalars *sc);
 

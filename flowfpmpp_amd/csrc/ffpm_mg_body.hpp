@@ -1007,7 +1007,7 @@ __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds,
 }
 
 // ---- setup --------------------------------------------------------------------
-// Level-1 conductances from level 0 (FP64 sums, stored FP32): half the sum of
+// Level-1 conductances from level 0 (FP64 sums, stored FP16): half the sum of
 // V c over the fluid-fluid level-0 faces on each aggregate face.
 __device__ __forceinline__ void mg_coarsen0_body(Blk bk, const GeomT<double> &g,
                                                  const uint8_t *__restrict__ code, int cnx,
