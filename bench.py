@@ -64,15 +64,15 @@ BYTES_PER_CELL = {
     # CG with the multigrid cycle (level-0 kernels; FP32 arithmetic, the
     # cycle's vectors stored in FP16; the coarse levels add 1/8, 1/64, ... of
     # their own small traffic)
-    "mg_update": 37,         # read p (FP32), code, r; write r, x1 (FP16), q (FP32); every
+    "mg_update": 35,         # read p (FP32), code, r; write r, x1, q (FP16); every
                              # second update also reads/writes phi and reads the
-                             # deferred direction: 27 or 47 B, 37 on average
+                             # deferred direction: 25 or 45 B, 35 on average
     "mg_fine_sweep": 7.5,    # x, the first step x1 (the source), code; write x (FP16);
                              # the step after the first reads x1 once (5 B), the next
                              # ones 7 B, or 9 B with the previous iterate
     "mg_fine_restrict": 5,   # x, x1 (FP16), code per fine cell (coarse writes 1/8 of 4 B)
     "mg_fine_prolong": 5,    # x (FP16) read/write, code (coarse reads 1/8)
-    "mg_fine_last": 13,      # x, previous iterate, x1 (FP16), q (FP32), code; write z (FP16)
+    "mg_fine_last": 11,      # x, previous iterate, x1, q (FP16), code; write z (FP16)
     "mg_direction": 11,      # read z (FP16), p_old (FP32), code; write p (FP32)
 }
 

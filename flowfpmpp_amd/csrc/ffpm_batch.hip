@@ -427,7 +427,7 @@ __global__ __launch_bounds__(1024) void k_bg_sweep(const BSolve *solves, Grid gr
   if (!iterating(st, slot)) return;
   const BSolve &d = solves[s];
   const Multigrid &mg = d.mg;
-  const float *q = LAST && st.mode == kDirUpdate ? mg.q32 : nullptr;
+  const MgStore *q = LAST && st.mode == kDirUpdate ? mg.q16 : nullptr;
   mgk::mg_sweep_body<CL, LAST, NT, CHEB>(grid.local[blockIdx.x], mg.tl.blocks, lds, mg.tl,
                                          mg.g32, mg.buf0(step.a), mg.buf0(0), mg.code,
                                          mg.buf0(step.o), mg.partials, step.c2, q,
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(1024) void k_bg_upd(const BSolve *solves, Grid grid
   double *rmax = slot == st.n_chunk - 1 ? d.partials : nullptr;
   mgk::mg_upd_body<CL>(grid.local[blockIdx.x], d.tl.blocks, lds, d.tl, d.g,
                        reinterpret_cast<const float *>(d.p[st.cur]), d.code, d.x, d.r,
-                       mg.buf0(0), d.sc, (double)mg.first, rmax, mg.q32);
+                       mg.buf0(0), d.sc, (double)mg.first, rmax, mg.q16);
 }
 
 // One workgroup per solve: the finish of the preceding reduction for the
@@ -775,6 +775,13 @@ __global__ __launch_bounds__(256) void k_bs_pack(const BSolve *solves,
 __global__ void k_bs_zero_scalars(const BSolve *solves, int count) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < count) *solves[s].sc = Scalars{};
+}
+
+// Each multigrid solve's operator-norm bound (one workgroup per solve, as the
+// standalone k_mg_anorm).
+__global__ __launch_bounds__(256) void k_bs_anorm(const BSolve *solves) {
+  __shared__ double lds[16];
+  mgk::mg_anorm_body(solves[blockIdx.x].mg.g64, solves[blockIdx.x].sc, lds);
 }
 
 // Multigrid level operators of every frame, level l (grid of level l).
@@ -1216,6 +1223,8 @@ int batch_setup(Batch &B, ffpm_stairstep_problem *P, int count, int rows, int zc
     for (int l = 1; l <= B.mg_top; ++l)
       k_bs_mask<<<B.g_lvl[l].blocks, mgk::kMt, 0, stream()>>>(B.d_solves, B.g_lvl[l], l);
     FFPM_LAUNCHED("k_bs_mask");
+    k_bs_anorm<<<count, 256, 0, stream()>>>(B.d_solves);
+    FFPM_LAUNCHED("k_bs_anorm");
   }
   int isolated = 0;
   FFPM_HIP(hipMemcpyAsync(&isolated, B.d_isolated, sizeof(int), hipMemcpyDeviceToHost,

@@ -21,6 +21,9 @@ namespace ffpm {
 // D^-1 A (20-45 measured on stretched frames, at most a few times 10^4 at
 // 512 cells per axis): kSigFirst = 2^-4 keeps it below 2^12 times that.
 constexpr double kSigTarget = 1024.0, kSigFirst = 0.0625;
+// The stored q = A p: tau puts the bound anorm max |p| (above every |q|) at
+// 2^13, so q never leaves the FP16 range.
+constexpr double kTauTarget = 8192.0;
 
 __host__ __device__ inline double cycle_scale(double cur, double target, double mx) {
   if (!(mx > 0.0) || !(mx <= 1.7976931348623157e308)) return cur;
@@ -66,9 +69,15 @@ __device__ inline void finish_body(int op, int nslots, const double *partials, i
     grid_finish_sum<2>(partials, nblocks, v, lds + 16);
     if (lead) {
       const double sg = sc->sig, inv = 1.0 / sg;
-      sc->beta = sc->pq > 0 ? -(v[1] * inv) / sc->pq : 0.0;
+      const double tq = sc->tau > 0.0 ? sc->tau : 1.0;  // the scale the stored q has
+      sc->beta = sc->pq > 0 ? -(v[1] * inv / tq) / sc->pq : 0.0;
       sc->rz = v[0] * inv * inv;
       sc->sig_next = cycle_scale(sg, kSigTarget, zmax);
+      // the direction about to be formed, p = z + beta p_old: max |p| <=
+      // max |z| + |beta| max |p_old| (1% for the FP32 rounding of p)
+      const double pm = 1.01 * (zmax * inv + fabs(sc->beta) * sc->pmax);
+      sc->pmax = pm;
+      sc->tau = cycle_scale(1.0, kTauTarget, sc->anorm * pm);
     }
   } else if (op == FIN_SUMS) {
     double v[kReduceSlots];

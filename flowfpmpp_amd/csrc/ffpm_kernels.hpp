@@ -34,6 +34,10 @@ struct Scalars {
   // output z carry that factor), sig_next for the next one (from the max of
   // the last output). 0: not set yet (the next FIN_TR1 sets it).
   double sig, sig_next;
+  // q = A p of the last update is stored in FP16 times tau (a power of two):
+  // |q| <= anorm max |p|, anorm a bound on the infinity norm of A (set with
+  // the cycle), pmax a bound on max |p| carried from direction to direction
+  double tau, pmax, anorm;
 };
 
 // Finish operations of k_finish (what the partials of the preceding
@@ -46,8 +50,9 @@ enum FinishOp : int {
   FIN_PQ = 3,     // pq = sum(slot 0); breakdown test; alpha = rz / pq
   FIN_RZNEW = 4,  // rz_new = sum(slot 0); rz_new < 0 test; beta; rz; iters += 1
   FIN_SUMS = 5,   // sums[s] = sum(slot s), s < nslots
-  FIN_FCG1 = 6,   // rz = sum(slot 0) / sig^2 = z.r; beta = -sum(slot 1) / (sig pq) =
-                  // -z.(A p_old) / pq (slots of the scaled z); sig_next from max(slot 2)
+  FIN_FCG1 = 6,   // rz = sum(slot 0) / sig^2 = z.r; beta = -sum(slot 1) / (sig tau pq) =
+                  // -z.(A p_old) / pq (slots of the scaled z and q); sig_next from
+                  // max(slot 2); pmax and tau for the next stored q
 };
 
 struct Bounds {

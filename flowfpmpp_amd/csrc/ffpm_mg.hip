@@ -112,7 +112,7 @@ __global__ __launch_bounds__(1024) void k_mg_sweep_t(Tiling t, GeomT<float> g,
                                                      const MgStore *__restrict__ r,
                                                      const uint8_t *__restrict__ code,
                                                      MgStore *__restrict__ b, double *partials,
-                                                     float c2, const float *__restrict__ q,
+                                                     float c2, const MgStore *__restrict__ q,
                                                      const MgStore *__restrict__ prev, float c1,
                                                      float sscale) {
   __shared__ double lds[16 * kReduceSlots];
@@ -127,10 +127,10 @@ __global__ __launch_bounds__(1024) void k_mg_upd_t(Tiling t, GeomT<double> g,
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    MgStore *__restrict__ x1, Scalars *sc,
                                                    double omega, double *rmax,
-                                                   float *__restrict__ q32,
+                                                   MgStore *__restrict__ q16,
                                                    const float *__restrict__ ppend) {
   __shared__ double lds[16 * kReduceSlots];
-  mg_upd_body<CL, XM>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q32,
+  mg_upd_body<CL, XM>(blockIdx.x, gridDim.x, lds, t, g, p, code, x, r, x1, sc, omega, rmax, q16,
                       ppend);
 }
 
@@ -259,6 +259,12 @@ __global__ __launch_bounds__(256) void k_lrn_add(size_t n, const MgStore *__rest
 }
 
 // ---- setup ----------------------------------------------------------------------
+// sc->anorm (mg_anorm_body), one workgroup of 256 threads.
+__global__ __launch_bounds__(256) void k_mg_anorm(GeomT<double> g, Scalars *sc) {
+  __shared__ double lds[16];
+  mg_anorm_body(g, sc, lds);
+}
+
 __global__ __launch_bounds__(256) void k_mg_coarsen0(GeomT<double> g,
                                                      const uint8_t *__restrict__ code, int cnx,
                                                      int cny, float scale,
@@ -471,7 +477,7 @@ size_t mg_operator_bytes(const Multigrid &mg) {
 }
 
 size_t mg_state_bytes(const Multigrid &mg) {
-  size_t bytes = 5 * up256(sizeof(MgStore) * mg.lv[0].n) + up256(sizeof(float) * mg.lv[0].n);
+  size_t bytes = 6 * up256(sizeof(MgStore) * mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) bytes += 4 * up256(sizeof(MgStore) * mg.lv[l].n);
   return bytes;
 }
@@ -489,7 +495,7 @@ void mg_bind_operator(Multigrid &mg, char *&base) {
 
 void mg_bind_state(Multigrid &mg, char *&base) {
   for (int i = 0; i < 5; ++i) mg.f0[i] = carve<MgStore>(base, mg.lv[0].n);
-  mg.q32 = carve<float>(base, mg.lv[0].n);
+  mg.q16 = carve<MgStore>(base, mg.lv[0].n);
   for (int l = 1; l < mg.levels; ++l) {
     MgLevel &v = mg.lv[l];
     v.x = carve<MgStore>(base, v.n);
@@ -581,7 +587,7 @@ namespace {
 // Level-0 streaming sweep launch (one step of a schedule) for the
 // multigrid's cells-per-lane choice.
 template <bool LAST, bool NT, bool CHEB>
-void launch_step0(const Multigrid &mg, const MgStep &st, const float *q) {
+void launch_step0(const Multigrid &mg, const MgStep &st, const MgStore *q) {
   const dim3 blk(kTileX, mg.tl.ty);
   const MgStore *a = mg.buf0(st.a), *p = st.p >= 0 ? mg.buf0(st.p) : nullptr;
   MgStore *o = mg.buf0(st.o);
@@ -596,7 +602,7 @@ void launch_step0(const Multigrid &mg, const MgStep &st, const float *q) {
 }
 
 template <bool LAST>
-void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr) {
+void launch_step(const Multigrid &mg, const MgStep &st, const MgStore *q = nullptr) {
   if (mg.cheb) {
     if (mg.nontemporal) launch_step0<LAST, true, true>(mg, st, q);
     else launch_step0<LAST, false, true>(mg, st, q);
@@ -607,6 +613,12 @@ void launch_step(const Multigrid &mg, const MgStep &st, const float *q = nullptr
 }
 
 }  // namespace
+
+int mg_anorm(const Multigrid &mg, Scalars *sc) {
+  k_mg_anorm<<<1, 256, 0, stream()>>>(mg.g64, sc);
+  FFPM_LAUNCHED("k_mg_anorm");
+  return OK;
+}
 
 int mg_first(const Multigrid &mg, double *r, const Scalars *sc) {
   const MgLevel &f0 = mg.lv[0];
@@ -627,7 +639,7 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
   }
   TimedLaunch tl("mg_update");
   auto go = [&](auto kern) {
-    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax, mg.q32,
+    kern<<<t.blocks, blk, 0, stream()>>>(t, mg.g64, p, mg.code, x, r, x1, sc, w, rmax, mg.q16,
                                          ppend);
   };
   if (cl == 2) {
@@ -808,7 +820,7 @@ int mg_vcycle(const Multigrid &mg, bool with_q) {
   FFPM_LAUNCHED("k_mg_sweep_t");
   {
     TimedLaunch tl("mg_fine_last");
-    launch_step<true>(mg, s0.post[s0.npost - 1], with_q ? mg.q32 : nullptr);
+    launch_step<true>(mg, s0.post[s0.npost - 1], with_q ? mg.q16 : nullptr);
   }
   FFPM_LAUNCHED("k_mg_sweep_t");
   return OK;

@@ -134,7 +134,7 @@ struct Multigrid {
   MgSchedule s0, sc;         // level 0 (its last step writes z), coarse levels
   MgLearned lrn;               // the learned coarse correction, when net is set
   double *partials = nullptr;  // z.r and z.q partials of the last sweep (tl.blocks)
-  float *q32 = nullptr;        // q = A p of the last update (level-0 cells)
+  MgStore *q16 = nullptr;      // q = A p of the last update, FP16 times Scalars::tau
   float sscale = 1.f;          // 1 / first: the cycle's (scaled) source r / V is
                                // read as f0[0] d sscale (f0[0] = sig first D^-1 r)
   // the level-0 vectors, FP16 times Scalars::sig (index as in MgStep; 3 is
@@ -156,7 +156,7 @@ int mg_setup(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
 // batched solves): parameters and level extents (no allocation); the bytes of
 // the level operators (gE gN gT m rd of the coarse levels, shared by solves
 // of one operator) and of one solve's cycle state (the five FP16 level-0
-// vectors and q32, coarse x y w b), every array 256-byte aligned; carving
+// vectors and q16, coarse x y w b), every array 256-byte aligned; carving
 // them from base (advanced past them); and the launches that fill the level
 // operators.
 int mg_plan(Multigrid &mg, const GeomT<double> &g64, const GeomT<float> &g32,
@@ -196,8 +196,12 @@ int mg_check_interval(int check_interval);
 // scalars of that first pass, with the cycle scale sig).
 int mg_first(const Multigrid &mg, double *r, const Scalars *sc);
 
+// sc->anorm, the bound on the level-0 operator's infinity norm that scales
+// the stored q = A p (Scalars::tau); once per solve, before its first update.
+int mg_anorm(const Multigrid &mg, Scalars *sc);
+
 // x += alpha p, r -= alpha A p, f0[0] = sig w D^-1 r (sig = sc->sig_next,
-// which becomes sc->sig) and q32 = A p (alpha in sc; no-op after a
+// which becomes sc->sig) and q16 = tau A p (alpha in sc; no-op after a
 // breakdown); counts the iteration. t, cl: FP64 launch
 // shape. The direction p is stored in FP32 (mg_direction forms it) and
 // widened exactly; x, r and the arithmetic stay FP64. With rmax non-null, one
@@ -212,7 +216,7 @@ int mg_update(const Multigrid &mg, const Tiling &t, int cl, const float *p, doub
 
 // f0[3] = sig B r (the rest of the cycle after the first step, which must be
 // in f0[0]). Leaves three slots of partials (tl.blocks each) of the scaled
-// output: z.r and z.q32 (0 without with_q) times sig^2 and sig, and max |z|
+// output: z.r and z.q16 (0 without with_q) times sig^2 and sig tau, and max |z|
 // times sig (k_finish FIN_FCG1 divides the scale out).
 int mg_vcycle(const Multigrid &mg, bool with_q);
 

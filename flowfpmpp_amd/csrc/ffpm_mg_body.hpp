@@ -841,8 +841,8 @@ __device__ __forceinline__ void mgc_coarsest_body(float *xs, int nx, int ny, int
 // scale sig.
 // LAST: out is the cycle's output z (times sig), and the partials of the
 // stored (rounded) values go to three slots: z.r = z.(s V) and z.q (q = A
-// p_old from the previous update, FP32, or z.q = 0 when q is null), and
-// max |z|.
+// p_old from the previous update, FP16 times sc->tau, or z.q = 0 when q is
+// null), and max |z|.
 //
 // CHEB: a Chebyshev step, out = a + c2 (s - A a) / d + c1 (a - prev) (prev
 // null: a zero previous iterate); otherwise damped Jacobi with omega = c2.
@@ -855,7 +855,7 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
                                               const MgStore *__restrict__ r,
                                               const uint8_t *__restrict__ code,
                                               MgStore *__restrict__ b, double *partials,
-                                              float omega, const float *__restrict__ q,
+                                              float omega, const MgStore *__restrict__ q,
                                               const MgStore *__restrict__ prev, float c1,
                                               float sscale) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -921,10 +921,9 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
       const float ve = lane == kTileX - 1 ? (float)u.pl.e.a : from_e;
       const float cb = g.cb[k], ct = g.ct[k], dzk = g.dz[k];
       const float cbk = k > 0 ? cb : 0.f, ctk = k < nz - 1 ? ct : 0.f;
-      H bv;
-      V qz;
+      H bv, qz;
       if constexpr (LAST) {
-        if (q) qz = ldv_s<NT, float, CL>(q, at(k));
+        if (q) qz = ldv_s<NT, MgStore, CL>(q, at(k));
       }
       float d[CL];
       level0_diag<CL>(d, u.pl.code, exy, wall_code_z(k, nz), cw, ce, dxy, cs, cn, cb, ct,
@@ -955,7 +954,7 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
           const double zd = (double)(float)bv.v[m];
           const float src = rs * d[m];
           s += zd * ((double)src * (double)(dxdy[m] * dzk));
-          if (q) sq += zd * (double)qz.v[m];
+          if (q) sq += zd * (double)(float)qz.v[m];
           zmx = fmax(zmx, fabs(zd));
         }
       }
@@ -980,8 +979,8 @@ __device__ __forceinline__ void mg_sweep_body(int block, int nblocks, double *ld
 // x += alpha p, r -= alpha A p (the fused PCG update body, p in FP32), the
 // first smoothing step of the next cycle, x1 = sig w D^-1 r (FP16 with the
 // next cycle's scale sig = sc->sig_next, which becomes sc->sig; x1 also
-// carries the cycle's source), and q = A p in FP32; with rmax the
-// per-workgroup max |r / V|; counts the iteration.
+// carries the cycle's source), and q = A p in FP16 times sc->tau; with rmax
+// the per-workgroup max |r / V|; counts the iteration.
 // XM (XM_NOW, XM_DEFER, XM_TWO) and ppend: the treatment of x (see
 // pcg_upd_body); a deferring update keeps its alpha in sc->alpha_pend for the
 // next one, which applies it with p_pend.
@@ -991,13 +990,13 @@ __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds,
                                             const uint8_t *__restrict__ code,
                                             double *__restrict__ x, double *__restrict__ r,
                                             MgStore *__restrict__ x1, Scalars *sc, double omega,
-                                            double *rmax, float *__restrict__ q32,
+                                            double *rmax, MgStore *__restrict__ q16,
                                             const float *__restrict__ ppend = nullptr) {
   const bool broke = sc->status != 0;
   const double sig = sc->sig_next;  // no workgroup of this kernel reads sc->sig
-  pcg_upd_body<double, CL, float, false, float, XM, MgStore>(
-      t, block, nblocks, g, p, code, x, r, x1, nullptr, sc, lds, omega * sig, rmax, q32,
-      nullptr, ppend);
+  pcg_upd_body<double, CL, float, false, float, XM, MgStore, MgStore>(
+      t, block, nblocks, g, p, code, x, r, x1, nullptr, sc, lds, omega * sig, rmax, q16,
+      nullptr, ppend, sc->tau);
   if (block == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
     // a deferring iteration that broke down leaves no step behind
     if constexpr (XM == XM_DEFER) sc->alpha_pend = broke ? 0.0 : sc->alpha;
@@ -1007,6 +1006,30 @@ __device__ __forceinline__ void mg_upd_body(int block, int nblocks, double *lds,
 }
 
 // ---- setup --------------------------------------------------------------------
+// sc->anorm: a bound on the infinity norm of the level-0 operator A = V (diag
+// - offdiag), whose rows sum |entries| to at most 2 V (cw + ce + cs + cn + cb
+// + ct) <= 2 (max dx (cw + ce) max dy max dz + max dx max dy (cs + cn) max dz
+// + max dx max dy max dz (cb + ct)). One workgroup; lds: 16 doubles.
+__device__ __forceinline__ void mg_anorm_body(const GeomT<double> &g, Scalars *sc, double *lds) {
+  double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+    m[0] = fmax(m[0], g.dx[i] * (g.cw[i] + g.ce[i]));
+    m[1] = fmax(m[1], g.dx[i]);
+  }
+  for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
+    m[2] = fmax(m[2], g.dy[j] * (g.cs[j] + g.cn[j]));
+    m[3] = fmax(m[3], g.dy[j]);
+  }
+  for (int k = threadIdx.x; k < g.nz; k += blockDim.x) {
+    m[4] = fmax(m[4], g.dz[k] * (g.cb[k] + g.ct[k]));
+    m[5] = fmax(m[5], g.dz[k]);
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) block_max(m[q], lds);
+  if (threadIdx.x == 0)
+    sc->anorm = 2.0 * (m[0] * m[3] * m[5] + m[1] * m[2] * m[5] + m[1] * m[3] * m[4]);
+}
+
 // Level-1 conductances from level 0 (FP64 sums, stored FP16): half the sum of
 // V c over the fluid-fluid level-0 faces on each aggregate face.
 __device__ __forceinline__ void mg_coarsen0_body(Blk bk, const GeomT<double> &g,

@@ -348,9 +348,10 @@ __device__ __forceinline__ void pcg_dir_body(const Tiling &t, int block, int nbl
 // cells there (an estimate of the max-norm residual), with qout non-null
 // q = A p is stored in ZT (for the flexible-CG coefficient of the next step),
 // and with sout non-null r / V (0 outside fluid cells) is stored in ZT. XT:
-// the type z is stored in (ZT unless given).
+// the type z is stored in (ZT unless given); QT: the type q is stored in, as
+// qscale q (ZT and 1 unless given).
 template <typename T, int CL, typename ZT = T, bool DOT = true, typename PT = T,
-          int XM = XM_NOW, typename XT = ZT>
+          int XM = XM_NOW, typename XT = ZT, typename QT = ZT>
 __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nblocks,
                                              const GeomT<T> &g, const PT *__restrict__ p,
                                              const uint8_t *__restrict__ code,
@@ -358,9 +359,10 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
                                              XT *__restrict__ z, double *partials,
                                              const Scalars *sc, double *lds,
                                              T zscale = T(1), double *rmax = nullptr,
-                                             ZT *__restrict__ qout = nullptr,
+                                             QT *__restrict__ qout = nullptr,
                                              ZT *__restrict__ sout = nullptr,
-                                             const PT *__restrict__ ppend = nullptr) {
+                                             const PT *__restrict__ ppend = nullptr,
+                                             T qscale = T(1)) {
   if (sc->status != 0) {
     if constexpr (XM == XM_TWO) {
       // the iteration broke down: only the deferred step is applied (none
@@ -437,7 +439,8 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       const T dzk = g.dz[k], cb = g.cb[k], ct = g.ct[k];
       Vec<T, CL> xv = u.x, rv = u.r;
       Vec<XT, CL> zv;
-      Vec<ZT, CL> qv, sv;
+      Vec<QT, CL> qv;
+      Vec<ZT, CL> sv;
       const Vec<T, CL> p_next = convert<T>(u.p_next);
       const T from_w = __shfl_up(p_cur.v[CL - 1], 1, kTileX);
       const T from_e = __shfl_down(p_cur.v[0], 1, kTileX);
@@ -461,7 +464,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
           s += (double)zv.v[m] * (double)rv.v[m];
         } else {
           zv.v[m] = (XT)(zscale * rdiag_code<T>(d, dxdy[m], dzk) * rv.v[m]);
-          qv.v[m] = (ZT)q;
+          qv.v[m] = (QT)(qscale * q);
           sv.v[m] = d > 0 ? (ZT)(rv.v[m] / (dxdy[m] * dzk)) : ZT(0);
           if (rmax && d > 0) rmx = fmax(rmx, fabs((double)rv.v[m]) / (double)(dxdy[m] * dzk));
         }
@@ -472,7 +475,7 @@ __device__ __forceinline__ void pcg_upd_body(const Tiling &t, int block, int nbl
       stv_s<!DOT, T, CL>(r, at(k), rv);
       stv_s<!DOT, XT, CL>(z, at(k), zv);
       if constexpr (!DOT) {
-        if (qout) stv_s<true, ZT, CL>(qout, at(k), qv);
+        if (qout) stv_s<true, QT, CL>(qout, at(k), qv);
         if (sout) stv_s<true, ZT, CL>(sout, at(k), sv);
       }
       p_prev = p_cur;

@@ -828,8 +828,8 @@ thread_local MgStats g_mg;
 // Fletcher-Reeves beta of plain PCG stagnates with it on strongly stretched
 // grids.) The direction and SpMV run in the fused PCG direction kernel
 // (p = z + beta p_old, q = A p, p.q), the update in the fused PCG update body,
-// which also forms the cycle's first smoothing step and stores q = A p in FP32
-// for the next beta. Every
+// which also forms the cycle's first smoothing step and stores q = A p in FP16
+// (times a power of two set from a bound on |A| max |p|) for the next beta. Every
 // `check` iterations the FP64 projected true residual replaces r
 // (src/fpm_stairstep_mpi_3d.f90:297-329) and the max-norm residual is tested
 // against the tolerance, as in the reference's PCG (formed only when the
@@ -856,6 +856,7 @@ int fcg_mg_iterate(Frame &f, double tolerance, int max_sweeps, int check_interva
   Multigrid mg;
   const Tiling tm = make_tiling(f.nx, f.ny, f.nz, 0, mg_rows(), kTileX * cl32);
   if ((st = mg_setup(mg, g, g32, f.code, tm, cl32, f.partials))) return st;
+  if ((st = mg_anorm(mg, f.sc))) return st;
   if (lrn && (st = mg_learned_setup(mg, lrn->net, lrn->features, lrn->level, lrn->mode)))
     return st;
   if (lrn) *lrn->fallback_at = -1;
